@@ -1,0 +1,236 @@
+/*
+ * icedem.h -- C ABI of the MI355X (gfx950) DPDEM ice-ship frame step.
+ *
+ * The reference has no FFI: its operator API is a set of JAX functions over
+ * NamedTuple pytrees (medea_ice_ship/jax_simulation.py).  Each entry point
+ * below replaces one of them; the Python host (mbd-icenav_amd/icedem) binds
+ * these symbols with ctypes and keeps the reference's Python signatures.
+ *
+ * Conventions
+ *  - Every function returns int: 0 = ok, < 0 = ICEDEM_E_* error code.
+ *    icedem_last_error(ctx) / icedem_status_string(code) describe it.
+ *  - Pointers named d_* are device pointers (HBM) owned by the caller; the
+ *    library never frees or retains them past the call.  Pointers named h_*
+ *    are host pointers read during the call only.
+ *  - All floating point state is float32, indices int32, flags uint8 (0/1).
+ *    Per-element 2-vectors are structure-of-arrays: x at [0, n), y at [n, 2n).
+ *  - `stream` is a hipStream_t (NULL = the legacy default stream).  Work is
+ *    enqueued asynchronously; the caller keeps buffers alive until the stream
+ *    is synchronised.  One context is used by one host thread at a time.
+ */
+#ifndef ICEDEM_H
+#define ICEDEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ICEDEM_OK 0
+#define ICEDEM_E_INVALID (-1)   /* bad argument / shape */
+#define ICEDEM_E_HIP (-2)       /* HIP runtime error */
+#define ICEDEM_E_LIMIT (-3)     /* size beyond a documented limit */
+#define ICEDEM_E_DEVICE (-4)    /* error flagged by a kernel */
+
+#define ICEDEM_MAX_VERTS 64     /* max_V (padded vertices per element) */
+#define ICEDEM_MAX_EDGE_SAMPLES 16
+#define ICEDEM_SHIP_STATE_LEN 15 /* eta[3], nu[3], n_actual[6], e_int[3] */
+
+typedef struct IceDemCtx IceDemCtx;
+
+/* Static ice data: IceStaticData (jax_ice_state.py:59-96) plus the ship hull
+ * (ShipDynamicsData.ship_hull_local, jax_ice_state.py:114).  Host pointers,
+ * float32 values as the reference holds them after its float64 -> float32
+ * cast; copied into device memory by icedem_create. */
+typedef struct {
+    int32_t n;              /* elements */
+    int32_t n_bonds;
+    int32_t n_floes;        /* len(dpdem_floe_eligible) */
+    int32_t max_v;          /* padded vertices per element, <= 64 */
+    int32_t n_hull;         /* hull points S */
+    const float* h_local_polys;   /* (n, max_v, 2) body-frame vertices */
+    const uint8_t* h_poly_mask;   /* (n, max_v) */
+    const float* h_masses;        /* (n) */
+    const float* h_inertia;       /* (n) */
+    const float* h_radii;         /* (n) */
+    const float* h_areas;         /* (n) */
+    const float* h_thicknesses;   /* (n) */
+    const float* h_bound_radius;  /* (n) max |local vertex|, rounded up */
+    const int32_t* h_floe_of;     /* (n) initial_floe_of */
+    const int32_t* h_bond_i;      /* (n_bonds) */
+    const int32_t* h_bond_j;      /* (n_bonds) */
+    const float* h_rest_lengths;  /* (n_bonds) */
+    const float* h_bond_strengths;/* (n_bonds) */
+    const float* h_sigma_bar;     /* (n_bonds) */
+    const float* h_hull_local;    /* (n_hull, 2) */
+} IceDemStatic;
+
+/* Scalar parameters of _make_frame_step (jax_simulation.py:325-361), already
+ * rounded to float32 in the order the reference applies them (products of
+ * Python scalars folded in binary64 first, products involving traced values
+ * rounded per factor).  Built by icedem.simulation._params(). */
+typedef struct {
+    /* hull contact, jax_geometry.py */
+    float half_len, half_beam;
+    int32_t n_edge_samples;
+    float edge_t[ICEDEM_MAX_EDGE_SAMPLES];      /* linspace samples t_k */
+    float edge_omt[ICEDEM_MAX_EDGE_SAMPLES];    /* 1 - t_k */
+    float ship_bound_radius;                    /* cull radius of the hull */
+    /* contact pre-pass, jax_simulation.py:431-509 */
+    float w_contact_factor, w_contact_cap, flexural_base;
+    float pen_crit_ref, pen_crit_thk_ref, side_push_factor;
+    int32_t has_thrust_cap;
+    float max_ship_ice_force;
+    float submergence_pen_cap, submergence_rate, inv_rise_time, buoyancy_coef;
+    float elig_threshold;          /* min(bond_strengths) * eff_strength */
+    float eff_strength_collision;
+    /* bonds and contact, jax_simulation.py:63-164 */
+    float youngs_modulus, bond_stiffness_factor;
+    float hertz_factor, hertz_exponent, ice_friction, friction_cap;
+    float contact_dilation2;       /* 2*delta (extension; 0 = reference) */
+    /* drag and integration, jax_simulation.py:167-268 */
+    float drag_ref_thickness, drag_coef, linear_drag_factor, rot_drag_coef;
+    float dt, dt_ice, velocity_clip, angular_damping, ang_cap_lo, ang_cap_hi, position_clip;
+    int32_t n_ice_steps;
+    /* fracture, jax_simulation.py:534-653 + jax_bond_fracture.py */
+    float max_fracture_thickness, k_n, k_s;
+    float bond_cohesion, bond_friction, G_c_I, G_c_II, eta_bk;
+    /* ship, jax_vessel_dynamics.py:246-348 */
+    int32_t use_auto_icenav;
+    float ice_brake_threshold;     /* 0 = disabled */
+    float M3inv[9], D3[9], Kp[9], Kd[9], Ki[9], D3pKd[9];
+    float B_pinv[18], B[18], n_max[6];
+    float T_n, vc_surge, vc_sway, vc_yaw;
+    /* uniform broad-phase grid (build choice; any cell >= 2 r_max + 2 delta
+     * gives the reference's exact contact set) */
+    float grid_x0, grid_y0, grid_cell;
+    int32_t grid_nx, grid_ny;
+} IceDemParams;
+
+/* Mutable state (IceState + ShipState, jax_ice_state.py:31-52), device
+ * pointers owned by the caller, updated in place by icedem_frames. */
+typedef struct {
+    float* d_pos;            /* (2, n) */
+    float* d_vel;            /* (2, n) */
+    float* d_rot;            /* (n) */
+    float* d_omega;          /* (n) */
+    float* d_sub;            /* (n) */
+    uint8_t* d_active;       /* (n) elem_active */
+    uint8_t* d_bond_intact;  /* (n_bonds) */
+    float* d_bond_damage;    /* (n_bonds) */
+    float* d_bond_dm0;       /* (n_bonds) bond_delta_m_0 */
+    float* d_bond_dmmax;     /* (n_bonds) bond_delta_m_max */
+    float* d_bond_ds;        /* (n_bonds) bond_delta_s */
+    uint8_t* d_floe_elig;    /* (n_floes) dpdem_floe_eligible */
+    float* d_ship;           /* (15) eta, nu, n_actual, e_int */
+} IceDemState;
+
+/* Per-frame inputs of scan_segment (jax_simulation.py:1328): frame k reads
+ * setpoints[k*6 .. k*6+5], v_water[k*sk + c*sc + e*se] (component c of
+ * element e; sk = 0 broadcasts one field over all frames) and
+ * v_water_ship[k*2 .. k*2+1] (NULL = zero current at the ship). */
+typedef struct {
+    const float* d_setpoints;     /* (K, 6) */
+    const float* d_v_water;       /* NULL = zero current */
+    int64_t vw_stride_k, vw_stride_c, vw_stride_e;
+    const float* d_v_water_ship;  /* (K, 2) or NULL */
+} IceDemInputs;
+
+/* scan_segment histories (jax_simulation.py:1317-1325); any pointer may be
+ * NULL.  positions are (K, 2, n). */
+typedef struct {
+    float* d_positions;     /* (K, 2, n) */
+    float* d_rotations;     /* (K, n) */
+    uint8_t* d_bond_intact; /* (K, n_bonds) */
+    float* d_eta;           /* (K, 3) */
+    float* d_submergence;   /* (K, n) */
+    float* d_J;             /* (K) J_cumul_incr */
+    float* d_F;             /* (K, 2) F_ice_on_ship */
+} IceDemHistory;
+
+/* frame_step diagnostics of the LAST frame of the call
+ * (jax_simulation.py:692-697); any pointer may be NULL. */
+typedef struct {
+    float* d_pen_max;       /* (n) */
+    float* d_force_mag;     /* (n) */
+    float* d_F_ice_on_ship; /* (2) */
+    float* d_J;             /* (1) */
+} IceDemDiag;
+
+const char* icedem_status_string(int code);
+const char* icedem_last_error(const IceDemCtx* ctx);
+int icedem_version(void);
+
+/* Replaces build_static_data + _make_frame_step (jax_ice_state.py:150-243,
+ * jax_simulation.py:302-361): uploads the static data and scalar parameters
+ * (the reference's XLA closure constants) and sizes per-frame scratch. */
+int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, IceDemCtx** out);
+int icedem_destroy(IceDemCtx* ctx);
+
+/* Replaces JaxSimContext.scan_segment / frame_step
+ * (jax_simulation.py:1328-1335, 397-699): runs K frames in place on `state`;
+ * K = 1 is frame_step.  use_graph != 0 replays a cached hipGraph of the
+ * K-frame launch sequence. */
+int icedem_frames(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* in, int32_t K,
+                  const IceDemHistory* hist, const IceDemDiag* diag, int32_t use_graph, void* stream);
+
+/* Waits for `stream` and reports errors flagged by kernels since the last call. */
+int icedem_sync(IceDemCtx* ctx, void* stream);
+
+/* Forces and moments of one sub-step (jax_simulation.py:372-387) for the
+ * given state, without integrating: forces = f_bond + f_ice - f_drag +
+ * ship_forces, moments = m_bond + m_ice + m_rdrag.  d_forces is (2, n). */
+int icedem_substep_forces(IceDemCtx* ctx, const IceDemState* state, const float* d_ship_forces,
+                          const float* d_v_water, int64_t vw_stride_c, int64_t vw_stride_e,
+                          float* d_forces, float* d_moments, void* stream);
+
+/* Contact pairs (i < j, ascending) with dist > 1e-6 and r_i + r_j (+2 delta)
+ * - dist > 0 found by the uniform-grid broad phase: the pairs that
+ * _compute_ice_ice_forces_jax (jax_simulation.py:127-138) gives a non-zero
+ * normal force.  Writes at most max_pairs; *h_n_pairs receives the total
+ * (synchronises the stream). */
+int icedem_contact_pairs(IceDemCtx* ctx, const float* d_pos, int32_t* d_pair_i, int32_t* d_pair_j,
+                         int64_t max_pairs, int64_t* h_n_pairs, void* stream);
+
+/* Replaces compute_all_collisions_combined (jax_geometry.py:515-594).
+ * d_local_polys (n, max_v, 2), d_poly_mask (n, max_v), d_pos (2, n),
+ * d_rot (n), d_hull_local (n_hull, 2); outputs (n) each. */
+int icedem_collisions_combined(const float* d_local_polys, const uint8_t* d_poly_mask,
+                               const float* d_pos, const float* d_rot, int32_t n, int32_t max_v,
+                               float ship_x, float ship_y, float ship_heading,
+                               float half_len, float half_beam,
+                               const float* d_hull_local, int32_t n_hull, int32_t n_edge_samples,
+                               const float* h_edge_t, const float* h_edge_omt,
+                               float* d_pen, float* d_nx, float* d_ny, float* d_cx, float* d_cy,
+                               void* stream);
+
+/* Replaces update_bond_damage_state (jax_bond_fracture.py:133-208);
+ * elementwise over n_bonds; outputs may alias nothing. */
+int icedem_bond_damage(int32_t n_bonds, const float* d_delta_n, const float* d_delta_s,
+                       const float* d_sigma_n, const float* d_sigma_t, const float* d_sigma_bar,
+                       float C, float mu_b, float G_c_I, float G_c_II, float eta_bk,
+                       const float* d_damage, const float* d_dm_max, const float* d_dm_0,
+                       const uint8_t* d_intact, float* d_new_damage, float* d_new_dm_max,
+                       float* d_new_dm_0, uint8_t* d_new_intact, void* stream);
+
+/* Replaces find_connected_components_jax (jax_bonds.py:37-89). */
+int icedem_connected_components(int32_t n, int32_t n_bonds, const int32_t* d_bond_i,
+                                const int32_t* d_bond_j, const uint8_t* d_intact, int32_t n_iter,
+                                int32_t* d_labels, void* stream);
+
+/* Replaces component_com_velocities_jax (jax_bonds.py:96-141); d_vel and
+ * d_out are (2, n). */
+int icedem_component_com_velocities(int32_t n, const int32_t* d_labels, const float* d_masses,
+                                    const float* d_vel, float* d_out, void* stream);
+
+/* Replaces ship_step_jax (jax_vessel_dynamics.py:246-348) for one frame,
+ * with the parameters of `ctx`: d_ship (15) in/out, d_setpoint (6),
+ * d_tau_ice (3), d_v_water_ship (2). */
+int icedem_ship_step(IceDemCtx* ctx, float* d_ship, const float* d_setpoint, const float* d_tau_ice,
+                     const float* d_v_water_ship, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ICEDEM_H */

@@ -1,0 +1,52 @@
+// icedem_internal.h -- device-side views shared by the DPDEM kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/icedem.h"
+#include "detmath.h"
+
+namespace icedem {
+
+constexpr int BLOCK = 256;          // threads per block of the per-element kernels
+constexpr int MAXC = 24;            // contacts kept per element in LDS before the slow path
+constexpr int MAX_PARTIALS = 4096;  // block partials reduced in one workgroup (n <= 1M)
+
+// Static data, per-frame scratch and parameters of one context (device memory).
+struct DevView {
+    IceDemParams p;
+    int32_t n, nb, nf, maxv, S, G, nblk, nbpad;
+    // static
+    const float2* polys;      // (n*maxv) body-frame vertices
+    const uint8_t* mask;      // (n*maxv)
+    const float* mass; const float* inertia; const float* radius; const float* area;
+    const float* thick; const float* bound;
+    const int32_t* floe;
+    const int32_t* bi; const int32_t* bj;
+    const float* rest; const float* bstr; const float* sbar;
+    const float2* hull;       // (S)
+    const int32_t* csr_start; // (n+1) bonds of each element: i-role then j-role
+    const int32_t* csr_ent;   // (2 nb) entry = bond << 1 | role (0 = i, 1 = j)
+    // per-frame scratch
+    float* pen; float* nx; float* ny; float* fm_raw; float* soi_x; float* soi_y; float* fm;
+    float* shipf;             // (2n) ship forces on ice, kept for sub-steps > 0
+    uint8_t* crack;           // (n) element touches a broken bond
+    int32_t* key;             // (n) cell of each element
+    int32_t* cell_count; int32_t* cell_fill; int32_t* cell_start;   // (G), (G), (G+1)
+    int32_t* cell_list;       // (n)
+    uint32_t* floe_coll; uint32_t* floe_maxfm;                      // (nf)
+    float* part1;             // (3 * nbpad) block partials: sum soi_x, soi_y, moment
+    float* part2;             // (2 * nbpad) block partials: sum of scaled soi
+    float* scal;              // [0] thrust scale, [1] M_ice_on_ship
+    float* nxt_pos; float* nxt_vel; float* nxt_rot; float* nxt_omega;
+    int32_t* err;
+};
+
+// Caller buffers of one icedem_frames call (device memory, rewritten per call).
+struct CallDesc {
+    IceDemState s;
+    IceDemInputs in;
+    IceDemHistory h;
+    IceDemDiag dg;
+};
+
+}  // namespace icedem

@@ -1,0 +1,311 @@
+"""Seeded synthetic ice scenes (host side, one-time input construction).
+
+The reference builds scenes with shapely and an O(n^2) Python bond loop
+(``medea_ice_ship/ice_field_generator.py:126-254``, ``tessellation.py:7-123``,
+``bonds.py:6-16``).  This module is a shapely-free, seeded equivalent:
+
+* convex floes: vertices on the circumcircle at jittered, sorted angles;
+* non-overlap by the separating-axis test against already placed floes
+  (spatial hash on circumcircles);
+* tessellation: Voronoi cells of seeded interior points (scipy), clipped to
+  the convex floe by Sutherland-Hodgman; cell centroid = vertex mean
+  (``geometry.py:17-18``);
+* proximity bonds: pairs (i<j) with ``|c_i - c_j| < 1.5 (r_i + r_j)``,
+  r = sqrt(A/pi), the criterion of ``bonds.py:6-16``, found with a KD-tree and
+  returned in the reference's lexicographic order.
+
+The returned dict carries the reference driver's inputs:
+``polygons, centroids, bonds, thicknesses, parent_ids``.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+
+def polygon_area(verts) -> float:
+    """Shoelace with the reference's sequential accumulation (geometry.py:6-14)."""
+    n = len(verts)
+    area = 0
+    for i in range(n):
+        j = (i + 1) % n
+        area += verts[i, 0] * verts[j, 1]
+        area -= verts[j, 0] * verts[i, 1]
+    return abs(area) / 2
+
+
+def convex_polygon(rng, center, radius, n_verts, jitter=0.15) -> np.ndarray:
+    """Convex polygon with vertices on a circle at jittered sorted angles (CCW)."""
+    base = 2.0 * np.pi * np.arange(n_verts) / n_verts
+    ang = base + rng.uniform(-jitter, jitter, size=n_verts)
+    ang = np.sort(ang)
+    pts = np.stack([center[0] + radius * np.cos(ang), center[1] + radius * np.sin(ang)], axis=1)
+    return pts
+
+
+def _sat_overlap(a: np.ndarray, b: np.ndarray, gap: float) -> bool:
+    """Separating-axis test for two convex polygons, with a minimum gap."""
+    for poly in (a, b):
+        e = np.roll(poly, -1, axis=0) - poly
+        normals = np.stack([e[:, 1], -e[:, 0]], axis=1)
+        normals /= np.maximum(np.linalg.norm(normals, axis=1, keepdims=True), 1e-12)
+        pa = a @ normals.T
+        pb = b @ normals.T
+        sep = (pa.min(axis=0) > pb.max(axis=0) + gap) | (pb.min(axis=0) > pa.max(axis=0) + gap)
+        if np.any(sep):
+            return False
+    return True
+
+
+def clip_convex(subject: np.ndarray, clip: np.ndarray) -> Optional[np.ndarray]:
+    """Sutherland-Hodgman: subject polygon clipped by a convex CCW clip polygon."""
+    out = subject
+    m = len(clip)
+    for k in range(m):
+        if len(out) == 0:
+            return None
+        a = clip[k]
+        b = clip[(k + 1) % m]
+        ex, ey = b[0] - a[0], b[1] - a[1]
+        side = ex * (out[:, 1] - a[1]) - ey * (out[:, 0] - a[0])   # >= 0 inside (left of edge)
+        res = []
+        n = len(out)
+        for i in range(n):
+            p, q = out[i], out[(i + 1) % n]
+            sp, sq = side[i], side[(i + 1) % n]
+            if sp >= 0:
+                res.append(p)
+            if (sp >= 0) != (sq >= 0):
+                t = sp / (sp - sq)
+                res.append(p + t * (q - p))
+        out = np.array(res) if res else np.zeros((0, 2))
+    if len(out) < 3:
+        return None
+    # drop near-duplicate consecutive vertices
+    keep = np.linalg.norm(out - np.roll(out, -1, axis=0), axis=1) > 1e-7
+    out = out[keep]
+    return out if len(out) >= 3 else None
+
+
+def tessellate_floe(rng, floe: np.ndarray, n_cells: int) -> List[np.ndarray]:
+    """Voronoi cells of ~n_cells seeded interior points, clipped to the floe."""
+    from scipy.spatial import Voronoi
+
+    area = polygon_area(floe)
+    lo, hi = floe.min(axis=0), floe.max(axis=0)
+    spacing = 0.5 * math.sqrt(area / max(n_cells, 1))
+    pts: List[np.ndarray] = []
+    e = np.roll(floe, -1, axis=0) - floe
+    attempts = 0
+    while len(pts) < n_cells and attempts < 60 * n_cells:
+        attempts += 1
+        p = rng.uniform(lo, hi)
+        side = e[:, 0] * (p[1] - floe[:, 1]) - e[:, 1] * (p[0] - floe[:, 0])
+        if np.any(side < 0):
+            continue
+        if pts and np.min(np.sum((np.array(pts) - p) ** 2, axis=1)) < spacing * spacing:
+            continue
+        pts.append(p)
+    if len(pts) < 3:
+        return [floe.copy()]
+    P = np.array(pts)
+    c = floe.mean(axis=0)
+    R = float(np.max(np.linalg.norm(floe - c, axis=1)))
+    far = c + 20.0 * R * np.array([[1, 1], [1, -1], [-1, 1], [-1, -1], [1, 0], [-1, 0], [0, 1], [0, -1]], float)
+    vor = Voronoi(np.vstack([P, far]))
+    cells = []
+    for i in range(len(P)):
+        reg = vor.regions[vor.point_region[i]]
+        if -1 in reg or len(reg) < 3:
+            continue
+        cell = vor.vertices[reg]
+        cc = cell.mean(axis=0)
+        ang = np.arctan2(cell[:, 1] - cc[1], cell[:, 0] - cc[0])
+        cell = cell[np.argsort(ang)]
+        clipped = clip_convex(cell, floe)
+        if clipped is not None and polygon_area(clipped) > 1e-6 * area:
+            cells.append(clipped)
+    return cells if cells else [floe.copy()]
+
+
+def proximity_bonds(centroids: np.ndarray, radii: np.ndarray, factor: float = 1.5) -> List[Tuple[int, int]]:
+    """bonds.py:6-16 criterion (dist < (r_i + r_j) * factor), KD-tree accelerated."""
+    from scipy.spatial import cKDTree
+
+    n = len(centroids)
+    if n < 2:
+        return []
+    tree = cKDTree(centroids)
+    cand = tree.query_pairs(r=2.0 * float(np.max(radii)) * factor * (1 + 1e-9), output_type="ndarray")
+    if len(cand) == 0:
+        return []
+    cand = np.sort(cand, axis=1)
+    cand = cand[np.lexsort((cand[:, 1], cand[:, 0]))]
+    out = []
+    for i, j in cand:
+        dist = np.linalg.norm(centroids[i] - centroids[j])
+        if dist < (radii[i] + radii[j]) * factor:
+            out.append((int(i), int(j)))
+    return out
+
+
+@dataclass
+class Scene:
+    polygons: List[np.ndarray]
+    centroids: np.ndarray
+    bonds: List[Tuple[int, int]]
+    thicknesses: np.ndarray
+    parent_ids: np.ndarray
+    n_floes: int
+    concentration: float
+    field: Tuple[float, float, float, float]
+
+    def as_dict(self):
+        return dict(polygons=self.polygons, centroids=self.centroids, bonds=self.bonds,
+                    thicknesses=self.thicknesses, parent_ids=self.parent_ids)
+
+
+def place_floes(rng, x_lo, x_hi, y_lo, y_hi, concentration, max_floes, radius_fn, nverts_fn,
+                jitter=0.15, gap=0.05, max_attempts=200000):
+    """Random sequential placement of non-overlapping convex floes."""
+    total = (x_hi - x_lo) * (y_hi - y_lo)
+    target = concentration * total
+    placed: List[np.ndarray] = []
+    radii: List[float] = []
+    centers: List[np.ndarray] = []
+    grid = {}
+    cell = None
+    area_sum = 0.0
+    attempts = 0
+    while area_sum < target and len(placed) < max_floes and attempts < max_attempts:
+        attempts += 1
+        R = float(radius_fn(rng))
+        if 2 * R >= min(x_hi - x_lo, y_hi - y_lo):
+            continue
+        c = np.array([rng.uniform(x_lo + R, x_hi - R), rng.uniform(y_lo + R, y_hi - R)])
+        poly = convex_polygon(rng, c, R, int(nverts_fn(rng)), jitter)
+        if cell is None:
+            cell = 4.0 * R
+        gx, gy = int(math.floor(c[0] / cell)), int(math.floor(c[1] / cell))
+        reach = int(math.ceil((R + 3.0 * max(radii + [R])) / cell))
+        ok = True
+        for ix in range(gx - reach, gx + reach + 1):
+            if not ok:
+                break
+            for iy in range(gy - reach, gy + reach + 1):
+                for k in grid.get((ix, iy), ()):
+                    if np.hypot(*(centers[k] - c)) > radii[k] + R + gap:
+                        continue
+                    if _sat_overlap(placed[k], poly, gap):
+                        ok = False
+                        break
+                if not ok:
+                    break
+        if not ok:
+            continue
+        k = len(placed)
+        placed.append(poly); radii.append(R); centers.append(c)
+        grid.setdefault((gx, gy), []).append(k)
+        area_sum += polygon_area(poly)
+    return placed, area_sum / total
+
+
+def tessellate_all(rng, floes, cells_per_floe, floe_thickness):
+    polys, cents, parents, thick = [], [], [], []
+    for f, floe in enumerate(floes):
+        cells = tessellate_floe(rng, floe, cells_per_floe)
+        for cpoly in cells:
+            polys.append(cpoly)
+            cents.append(np.mean(cpoly, axis=0))
+            parents.append(f)
+            thick.append(floe_thickness[f])
+    cents = np.array(cents)
+    areas = np.array([polygon_area(p) for p in polys])
+    radii = np.sqrt(areas / np.pi)
+    bonds = proximity_bonds(cents, radii, 1.5)
+    return polys, cents, np.array(parents, np.int32), np.array(thick, np.float64), bonds
+
+
+def scene_c0(seed: int = 0) -> Scene:
+    """3 convex 8-gons, circumradius U[10,25] m, in a 150 x 300 m field, ~20 cells each."""
+    rng = np.random.default_rng(seed)
+    x_lo, x_hi, y_lo, y_hi = -75.0, 75.0, 0.0, 300.0
+    floes, conc = place_floes(rng, x_lo, x_hi, y_lo, y_hi, 1.0, 3,
+                              lambda r: r.uniform(10.0, 25.0), lambda r: 8, jitter=0.15, gap=1.0)
+    # keep the three floes in a column in front of the ship: shift so their
+    # x-centres are near 0 is not required; the ship is aimed at the first floe
+    polys, cents, parents, thick, bonds = tessellate_all(rng, floes, 20, np.ones(len(floes)))
+    return Scene(polys, cents, bonds, thick, parents, len(floes), conc, (x_lo, x_hi, y_lo, y_hi))
+
+
+def scene_field(seed: int, width: float, length: float, concentration: float, max_floes: int,
+                median_radius: float = 8.0, sigma: float = 0.5, nv=(6, 12), cells_per_floe: int = 20,
+                thickness_values=(1.0,), thickness_probs=None) -> Scene:
+    rng = np.random.default_rng(seed)
+    x_lo, x_hi, y_lo, y_hi = -width / 2, width / 2, 0.0, length
+    floes, conc = place_floes(
+        rng, x_lo, x_hi, y_lo, y_hi, concentration, max_floes,
+        lambda r: min(max(r.lognormal(math.log(median_radius), sigma), 2.0), 6.0 * median_radius),
+        lambda r: r.integers(nv[0], nv[1] + 1), jitter=0.3)
+    th = rng.choice(np.asarray(thickness_values, float), size=len(floes), p=thickness_probs)
+    polys, cents, parents, thick, bonds = tessellate_all(rng, floes, cells_per_floe, th)
+    return Scene(polys, cents, bonds, thick, parents, len(floes), conc, (x_lo, x_hi, y_lo, y_hi))
+
+
+def scene_c1(seed: int = 0) -> Scene:
+    """300 x 750 m at 30 % (or 400 floes), lognormal radius (median 8 m, sigma 0.5)."""
+    return scene_field(seed, 300.0, 750.0, 0.30, 400)
+
+
+def scene_c2(seed: int = 0) -> Scene:
+    """1.5 x 3 km at 50 % (or 5000 floes), bimodal thickness 0.5 / 2.0 m."""
+    return scene_field(seed, 1500.0, 3000.0, 0.50, 5000, median_radius=11.0,
+                       thickness_values=(0.5, 2.0), thickness_probs=(0.5, 0.5))
+
+
+def scene_patch(seed: int, n_target: int, concentration: float) -> Scene:
+    """Rollout patch with ~n_target elements at the given concentration."""
+    rng = np.random.default_rng(seed)
+    cells_per_floe = 12
+    n_floes = max(1, int(round(n_target / cells_per_floe)))
+    mean_area = 2.4 * 64.0 * math.exp(2 * 0.25)
+    side = math.sqrt(n_floes * mean_area / concentration)
+    width = max(side, 70.0)
+    length = n_floes * mean_area / concentration / width
+    return scene_field(seed, width, max(length, 40.0), concentration, n_floes,
+                       cells_per_floe=cells_per_floe)
+
+
+def hexagon_scene():
+    """The reference's shared 3-floe test scene (test_jax_full_scan.py:187-230)."""
+    angles = np.linspace(0, 2 * np.pi, 7)[:-1]
+    r = 3.0
+    hex_pts = np.column_stack([r * np.cos(angles), r * np.sin(angles)])
+    centroids = np.array([[-10.0, 0.0], [0.0, 0.0], [10.0, 0.0]])
+    polygons = [hex_pts + c for c in centroids]
+    return Scene(polygons, centroids, [(0, 1), (1, 2)], np.array([1.0, 1.0, 1.0]),
+                 np.arange(3, dtype=np.int32), 3, 0.0, (-20.0, 20.0, -10.0, 10.0))
+
+
+def straight_trajectory(x0, y0, heading, speed, dt, n_frames):
+    """(n_frames + 2, 3) setpoints [x, y, psi] moving along ``heading``."""
+    t = np.arange(n_frames + 2) * dt
+    return np.stack([x0 + speed * t * math.cos(heading), y0 + speed * t * math.sin(heading),
+                     np.full_like(t, heading)], axis=1)
+
+
+def ship_start_below(scene: Scene, ship_length: float, gap: float = 1.0, target_floe: Optional[int] = None):
+    """Ship centre so that its bow (+x body, heading +y) is ``gap`` m short of the
+    lowest ice vertex of ``target_floe`` (default: the lowest floe)."""
+    if target_floe is None:
+        lows = [min(p[:, 1].min() for p, f in zip(scene.polygons, scene.parent_ids) if f == k)
+                for k in range(scene.n_floes)]
+        target_floe = int(np.argmin(lows))
+    sel = [p for p, f in zip(scene.polygons, scene.parent_ids) if f == target_floe]
+    allv = np.vstack(sel)
+    x0 = float(allv[:, 0].mean())
+    y0 = float(allv[:, 1].min()) - gap - ship_length / 2.0
+    return x0, y0

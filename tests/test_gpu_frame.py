@@ -1,0 +1,89 @@
+"""GPU parity of the full frame step against the float32 oracle.
+
+Gate (per frame, from identical input state): bond_intact, elem_active,
+dpdem_floe_eligible and has_collision flags bit-exact; every float state
+within 1e-4 relative (floor 1e-3 of the field's max magnitude).  The kernels
+mirror the oracle's operation order, so the floats are expected to be
+bit-identical as well; the report records the bit-identical fraction.
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import Case, Report, compare_frame
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(case, frames, check_every=1, start_check=0):
+    st, ship = case.gpu_initial()
+    worst = Report()
+    for k in range(frames):
+        if k >= start_check and (k - start_check) % check_every == 0:
+            rep = compare_frame(case, st, ship, k)
+            print(f"[{case.name}] frame {k}: {rep.summary()}", flush=True)
+            rep.assert_ok()
+        st, ship = case.gpu_frame(st, ship, k)
+    return st, ship
+
+
+def test_frame_parity_hexagon():
+    case = Case.make("hexagon", n_frames=60)
+    _run(case, 60)
+
+
+def test_frame_parity_c0_contact_window():
+    case = Case.make("c0", n_frames=1100)
+    _run(case, 1100, check_every=5, start_check=700)
+
+
+def test_frame_parity_c0_current_dilation():
+    from icedem.config import benchmark_config
+    cfg = benchmark_config(contact_dilation=0.2)
+    case = Case.make("c0", n_frames=900, cfg=cfg, current=(0.3, np.pi / 6))
+    _run(case, 900, check_every=10, start_check=700)
+
+
+def test_frame_parity_field():
+    case = Case.make("c1small", n_frames=300)
+    _run(case, 300, check_every=3)
+
+
+def test_scan_bit_identical_to_oracle_trajectory():
+    """200 frames in one graph-replayed segment vs an independent oracle run."""
+    case = Case.make("c0", n_frames=1000)
+    eng = case.engine()
+    o = case.oracle()
+    ost, oship = case.oracle_initial()
+    st, ship = case.gpu_initial()
+    K0 = 700
+    sp = torch.as_tensor(np.stack([case.setpoint_row(k) for k in range(K0)]), device=eng.device)
+    eng.run_frames(st, ship, sp, use_graph=True)
+    vw = np.zeros((case.n, 2), np.float32)
+    for k in range(K0):
+        r = case.setpoint_row(k)
+        ost, oship, _ = o.frame_step(ost, oship, r[:3], r[3:], vw, ost.dpdem_floe_eligible)
+    g, gs = case.to_oracle(st, ship)
+    rep = Report()
+    rep.add_float("positions", g.positions, ost.positions)
+    rep.add_float("eta", gs.eta, oship.eta)
+    rep.add_exact("bond_intact", g.bond_intact, ost.bond_intact)
+    print("after", K0, "frames:", rep.summary(), flush=True)
+    rep.assert_ok()
+    K = 200
+    sp = torch.as_tensor(np.stack([case.setpoint_row(k) for k in range(K0, K0 + K)]), device=eng.device)
+    hist = eng.alloc_history(K)
+    eng.run_frames(st, ship, sp, hist=hist, use_graph=True)
+    for k in range(K0, K0 + K):
+        r = case.setpoint_row(k)
+        ost, oship, od = o.frame_step(ost, oship, r[:3], r[3:], vw, ost.dpdem_floe_eligible)
+    g, gs = case.to_oracle(st, ship)
+    rep = Report()
+    rep.add_float("positions", g.positions, ost.positions)
+    rep.add_float("velocities", g.velocities, ost.velocities)
+    rep.add_float("eta", gs.eta, oship.eta)
+    rep.add_exact("bond_intact", g.bond_intact, ost.bond_intact)
+    rep.add_exact("hist_bond_last", hist["bond_intact"][-1].cpu().numpy(), ost.bond_intact)
+    rep.add_float("hist_pos_last", hist["positions"][-1].cpu().numpy().T, ost.positions)
+    print("after", K0 + K, "frames:", rep.summary(), flush=True)
+    rep.assert_ok()
