@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""Benchmark: DPDEM frame step on one MI355X.
+
+Workload (default): the plain-DEM single-GPU configuration -- a seeded
+300 x 750 m field at 30 % concentration (up to 400 lognormal convex floes,
+median circumradius 8 m, sigma 0.5, 6-12 vertices) tessellated to ~20 cells
+per floe (~8k elements, ~25k bonds), h = 1 m, hull 60 x 12 m (12-point
+polygon) moving +y at 2 m/s, contact stiffness 1e8 N/m, friction 0.3,
+dt 5e-3 s.  One step = one frame (hull contact, pre-pass, bonds, ice-ice
+contact, drag, integration, fracture, ship) plus the scan histories.
+
+Prints one JSON line.  ``value`` = element-steps/s with the state resident in
+HBM.  ``roofline`` is for the dominant kernel, its duration measured with HIP
+events on the launch stream; ``cpu_baseline`` times the float32 NumPy oracle
+(the reference's all-pairs algorithm) on a bounded sample of frames on this
+host, single thread.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "mbd-icenav_amd"))
+sys.path.insert(0, str(ROOT))
+
+METRIC = "DEM element-steps/s; planner rollouts scored/s, surrogate vs plain DEM, ≤5% cost err"
+HBM_PEAK_GBS = 8000.0
+KERNELS = ["k_hull", "k_mid", "k_scatter", "k_force", "k_bond", "k_ship"]
+
+
+def build_case(config: str, seed: int):
+    import numpy as np
+    from icedem import scenario
+    from icedem.config import benchmark_config
+    from icedem.properties import ShipProperties
+    from icedem.ice_state import ship_hull_polygon
+    cfg = benchmark_config()
+    if config == "c0":
+        sc = scenario.scene_c0(seed)
+        speed = 2.0
+    elif config == "c1":
+        sc = scenario.scene_c1(seed)
+        speed = 2.0
+    elif config == "c2":
+        sc = scenario.scene_c2(seed)
+        speed = 3.0
+    else:
+        raise SystemExit(f"unknown config {config}")
+    L, B = 60.0, 12.0
+    mass = ShipProperties(L, B, 5.0).mass
+    x0, y0 = scenario.ship_start_below(sc, L, gap=1.0)
+    hull = ship_hull_polygon(L, B, 5)
+    return sc, cfg, L, B, mass, hull, x0, y0, speed
+
+
+def algorithmic_bytes(kernel: str, n: int, nb: int, n_near: int, max_v: int, n_hull: int, G: int) -> float:
+    """Bytes each kernel must move per launch (each array counted once)."""
+    if kernel == "k_force":
+        # per element: state r 33 (pos 8, vel 8, rot 4, omega 4, sub 4, active 1, floe-coll 4),
+        # static r 28 (floe 4, thick, area, radius, mass, inertia 20, csr 4), pre-pass r 24,
+        # v_water 8, cells 8; writes 34 (next pos/vel/rot/omega 24, sub 4, fm 4, active 1, crack 1)
+        # per bond: i, j, rest, damage 16 + intact 1 + two CSR entries 8
+        return 138.0 * n + 25.0 * nb
+    if kernel == "k_hull":
+        return 64.0 * n + n_near * (9.0 * max_v) + 8.0 * n_hull
+    if kernel == "k_bond":
+        return 55.0 * nb + 82.0 * n + 8.0 * G
+    if kernel == "k_scatter":
+        return 16.0 * n
+    if kernel == "k_mid":
+        return 12.0 * G
+    return 64.0
+
+
+def cpu_baseline(case_args, frames_budget_s: float = 20.0, max_frames: int = 8):
+    """Float32 oracle (all-pairs, the reference's algorithm) on this host."""
+    import numpy as np
+    from oracle import model as om, step as ostep
+    sc, cfg, L, B, mass, hull, x0, y0, speed = case_args
+    from tests.helpers import oracle_config
+    ocfg = oracle_config(cfg)
+    st64 = om.build_static_data(sc.polygons, sc.centroids, sc.bonds, sc.thicknesses, sc.parent_ids, ocfg)
+    sd = om.build_ship_dynamics_data(mass, L, B, hull)
+    fos = om.Fossen(mass, L)
+    o = ostep.Oracle(st64, ocfg, sd, L, B, 1, cfg.dt, cfg.dt, True, fos.B, fos.T_n, dtype=np.float32)
+    st = ostep.initial_ice_state(sc.centroids, len(sc.bonds), sc.parent_ids, np.float32)
+    ship = ostep.initial_ship_state((x0, y0), math.pi / 2, np.float32)
+    n = len(sc.polygons)
+    vw = np.zeros((n, 2), np.float32)
+    frames = 0
+    t0 = time.perf_counter()
+    while frames < max_frames:
+        sp = np.array([x0, y0 + speed * cfg.dt * frames, math.pi / 2, x0, y0 + speed * cfg.dt * (frames + 1),
+                       math.pi / 2], np.float32)
+        st, ship, _ = o.frame_step(st, ship, sp[:3], sp[3:], vw, st.dpdem_floe_eligible)
+        frames += 1
+        if time.perf_counter() - t0 > frames_budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": n * frames / dt, "unit": "element-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{frames} frames of the same scene from its initial state, float32 NumPy oracle, "
+                      f"all-pairs contact ({n * (n - 1) // 2} pairs/frame), {dt:.1f} s, "
+                      f"host os.cpu_count()={os.cpu_count()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2"])
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=500)
+    ap.add_argument("--profile-frames", type=int, default=200)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
+              f"(multi-GPU sharding is a design note, not an implemented path).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from icedem import _lib
+    from icedem.engine import DemEngine, _ptr
+    from icedem.ice_state import build_static_data, build_ship_dynamics_data, build_initial_ice_state
+    from icedem.properties import ShipProperties
+    from icedem.simulation import _time_stepping, _thruster_config
+
+    case_args = build_case(args.config, args.seed)
+    sc, cfg, L, B, mass, hull, x0, y0, speed = case_args
+    n, nb = len(sc.polygons), len(sc.bonds)
+    static = build_static_data(sc.polygons, sc.centroids, sc.bonds, sc.thicknesses, sc.parent_ids, cfg)
+    sp_ = ShipProperties(L, B, 1.0, displacement=mass)
+    sd = build_ship_dynamics_data(sp_, cfg, L, B, ship_hull_local=hull)
+    dt_ice, nsteps = _time_stepping(static, cfg, cfg.dt)
+    Bm, T_n = _thruster_config(sp_, L)
+    eng = DemEngine(static, cfg, sd, L, B, nsteps, cfg.dt, dt_ice, True, Bm, T_n, centroids=sc.centroids)
+    dev = eng.device
+    st = build_initial_ice_state(n, sc.centroids, nb, sc.parent_ids, dev)
+    ship = torch.zeros(15, dtype=torch.float32, device=dev)
+    ship[0], ship[1], ship[2] = x0, y0, math.pi / 2
+    total = args.warmup + args.steps
+    t_all = np.arange(total + 1) * cfg.dt
+    traj = np.stack([np.full_like(t_all, x0), y0 + speed * t_all, np.full_like(t_all, math.pi / 2)], axis=1)
+    setp = torch.as_tensor(np.concatenate([traj[:-1], traj[1:]], axis=1).astype(np.float32), device=dev)
+    chunk = max(1, min(args.chunk, args.steps))
+    hist = eng.alloc_history(chunk)
+    use_graph = not args.no_graph
+
+    def run(frame0, frames):
+        k = 0
+        while k < frames:
+            m = min(chunk, frames - k)
+            h = hist if m == chunk else eng.alloc_history(m)
+            eng.run_frames(st, ship, setp[frame0 + k: frame0 + k + m], hist=h, use_graph=use_graph, sync=False)
+            k += m
+
+    run(0, args.warmup)
+    torch.cuda.synchronize()
+    _lib.check(eng.lib.icedem_sync(eng.ctx, torch.cuda.current_stream().cuda_stream), eng.ctx, "sync")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(args.warmup, args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    _lib.check(eng.lib.icedem_sync(eng.ctx, torch.cuda.current_stream().cuda_stream), eng.ctx, "sync")
+
+    # per-kernel durations with HIP events on the launch stream
+    P = max(1, args.profile_frames)
+    ms = (C.c_float * 6)()
+    s = eng.state_struct(st, ship)
+    inp = _lib.IceDemInputs()
+    sp_slice = setp[total - P: total].contiguous() if total >= P else setp[:P].contiguous()
+    inp.d_setpoints = _ptr(sp_slice)
+    _lib.check(eng.lib.icedem_frames_profiled(eng.ctx, C.byref(s), C.byref(inp), P, None, ms,
+                                              torch.cuda.current_stream().cuda_stream), eng.ctx, "profiled")
+    torch.cuda.synchronize()
+    avg_us = {k: ms[i] * 1000.0 / P for i, k in enumerate(KERNELS)}
+    dom = max(avg_us, key=avg_us.get)
+    pen = eng.alloc_diag()
+    n_near = n  # upper bound for the hull kernel's vertex traffic
+    G = eng.grid[3] * eng.grid[4]
+    abytes = algorithmic_bytes(dom, n, nb, n_near, eng.max_v, hull.shape[0], G)
+    achieved = abytes / (avg_us[dom] * 1e-6) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_{args.config}.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get(dom)
+        except Exception:
+            traffic = None
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(case_args)
+    value = n * args.steps / elapsed
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "element-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene generator)",
+        "config": {"workload": f"plain DPDEM step, {args.config}: {n} elements, {nb} bonds, "
+                               f"{sc.n_floes} floes, conc {sc.concentration:.2f}, dt {cfg.dt}",
+                   "elements": n, "bonds": nb, "floes": sc.n_floes, "frames_per_graph": chunk,
+                   "graph": use_graph, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                     "kernel_avg_us": avg_us[dom], "algorithmic_bytes": abytes,
+                     "kernel_avg_us_all": avg_us},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

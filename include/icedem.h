@@ -175,6 +175,13 @@ int icedem_destroy(IceDemCtx* ctx);
 int icedem_frames(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* in, int32_t K,
                   const IceDemHistory* hist, const IceDemDiag* diag, int32_t use_graph, void* stream);
 
+/* icedem_frames without graph replay, timing each kernel of the frame with
+ * HIP events on `stream`: h_kernel_ms[0..5] receive the summed durations of
+ * k_hull, k_mid, k_scatter, k_force (all sub-steps), k_bond and k_ship over
+ * the K frames.  Instrumentation for benchmarks; no reference counterpart. */
+int icedem_frames_profiled(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* in, int32_t K,
+                           const IceDemHistory* hist, float* h_kernel_ms, void* stream);
+
 /* Waits for `stream` and reports errors flagged by kernels since the last call. */
 int icedem_sync(IceDemCtx* ctx, void* stream);
 

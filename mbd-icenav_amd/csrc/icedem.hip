@@ -578,7 +578,7 @@ __global__ __launch_bounds__(BLOCK) void k_collisions_leaf(
     float px = 0.0f, py = 0.0f, ro = 0.0f;
     if (e < n) { px = pos[e]; py = pos[n + e]; ro = rot[e]; }
     float a, b, c, d, f;
-    hull_wave(prm, polys, mask, bound, hull, S, maxv, wave_base, n, px, py, ro, sx, sy, sh, a, b, c, d, f);
+    hull_wave(prm, polys, mask, bound, hull, S, maxv, wave_base, n, px, py, ro, sx, sy, sh, a, b, c, d, f, true);
     if (e < n) { pen[e] = a; nx[e] = b; ny[e] = c; cx[e] = d; cy[e] = f; }
 }
 
@@ -977,7 +977,9 @@ static int put_desc(IceDemCtx* ctx, const CallDesc& cd, hipStream_t stream) {
     return ICEDEM_OK;
 }
 
-static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream) {
+// ev (optional, profiling): 7 event pairs per frame bracket k_hull, k_mid,
+// k_scatter, k_force (all sub-steps with their cell rebuilds), k_bond, k_ship.
+static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, hipEvent_t* ev = nullptr, float* acc = nullptr) {
     const DevView& H = ctx->hv;
     const DevView* V = ctx->dv;
     const CallDesc* D = ctx->dd;
@@ -985,11 +987,16 @@ static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream) {
     const int steps = H.p.n_ice_steps;
     const int post = std::max(std::max(H.nb, H.n), std::max(H.G, H.nf));
     const int nbpost = (post + BLOCK - 1) / BLOCK;
+    auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], stream); };
     for (int k = 0; k < K; ++k) {
         const int last = (k == K - 1) ? 1 : 0;
+        mark(0);
         hipLaunchKernelGGL(k_hull, dim3(nblk), dim3(BLOCK), 0, stream, V, D);
+        mark(1);
         hipLaunchKernelGGL(k_mid, dim3(1), dim3(1024), 0, stream, V, D);
+        mark(2);
         hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(BLOCK), 0, stream, V);
+        mark(3);
         for (int s = 0; s < steps; ++s) {
             if (s > 0) {
                 // rebuild the cell lists for the sub-step's positions
@@ -1002,8 +1009,19 @@ static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream) {
             hipLaunchKernelGGL(k_force, dim3(nblk), dim3(BLOCK), 0, stream, V, D, k, s, (s % 2 == 0) ? 1 : 0, 0,
                                (const float*)nullptr, (float*)nullptr, (float*)nullptr);
         }
+        mark(4);
         hipLaunchKernelGGL(k_bond, dim3(nbpost), dim3(BLOCK), 0, stream, V, D, k, (steps % 2 == 1) ? 1 : 0, last);
+        mark(5);
         hipLaunchKernelGGL(k_ship, dim3(1), dim3(64), 0, stream, V, D, k, last);
+        mark(6);
+        if (ev) {
+            HIP_TRY(hipEventSynchronize(ev[6]));
+            for (int i = 0; i < 6; ++i) {
+                float ms = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+                acc[i] += ms;
+            }
+        }
     }
     HIP_TRY(hipGetLastError());
     return ICEDEM_OK;
@@ -1046,6 +1064,30 @@ int icedem_frames(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* 
     }
     HIP_TRY(hipGraphLaunch(it->second, stream));
     return ICEDEM_OK;
+}
+
+int icedem_frames_profiled(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* in, int32_t K,
+                           const IceDemHistory* hist, float* h_kernel_ms, void* stream_) {
+    if (!ctx || !in || !h_kernel_ms || K < 0) { set_err(ctx, "null argument"); return ICEDEM_E_INVALID; }
+    int rc = validate_state(ctx, state);
+    if (rc) return rc;
+    if (!in->d_setpoints) { set_err(ctx, "setpoints required"); return ICEDEM_E_INVALID; }
+    hipStream_t stream = (hipStream_t)stream_;
+    HIP_TRY(hipSetDevice(ctx->device));
+    CallDesc cd;
+    memset(&cd, 0, sizeof(cd));
+    cd.s = *state;
+    cd.in = *in;
+    if (hist) cd.h = *hist;
+    rc = put_desc(ctx, cd, stream);
+    if (rc) return rc;
+    hipEvent_t ev[7];
+    for (int i = 0; i < 7; ++i) HIP_TRY(hipEventCreate(&ev[i]));
+    float acc[6] = {0, 0, 0, 0, 0, 0};
+    rc = launch_frames(ctx, K, stream, ev, acc);
+    for (int i = 0; i < 7; ++i) (void)hipEventDestroy(ev[i]);
+    for (int i = 0; i < 6; ++i) h_kernel_ms[i] = acc[i];
+    return rc;
 }
 
 int icedem_sync(IceDemCtx* ctx, void* stream) {

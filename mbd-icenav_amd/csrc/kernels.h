@@ -265,7 +265,8 @@ __device__ __forceinline__ void hull_wave(const IceDemParams& prm, const float2*
                                           const float2* __restrict__ hull, int S, int maxv,
                                           int wave_base, int n, float px, float py, float rot,
                                           float sx, float sy, float sh,
-                                          float& pen, float& nxo, float& nyo, float& cxo, float& cyo) {
+                                          float& pen, float& nxo, float& nyo, float& cxo, float& cyo,
+                                          bool want_cp = false) {
     const int lane = threadIdx.x & 63;
     const int e = wave_base + lane;
     pen = 0.0f; nxo = 0.0f; nyo = 0.0f; cxo = 0.0f; cyo = 0.0f;
@@ -274,6 +275,15 @@ __device__ __forceinline__ void hull_wave(const IceDemParams& prm, const float2*
         float dx = px - sx, dy = py - sy;
         float R = bound[e] + prm.ship_bound_radius + 1.0f;
         near = (dx * dx + dy * dy) <= R * R;
+        if (want_cp && !near) {
+            // no contact: the reference's argmax picks point 0, i.e. the
+            // contact point reported is world vertex 0 (jax_geometry.py:276-282)
+            float s0, c0;
+            dm_sincosf(rot, &s0, &c0);
+            float2 v0 = polys[(size_t)e * maxv];
+            cxo = c0 * v0.x - s0 * v0.y + px;
+            cyo = s0 * v0.x + c0 * v0.y + py;
+        }
     }
     uint64_t todo = __ballot(near);
     if (todo == 0) return;

@@ -93,6 +93,8 @@ SIGNATURES = {
     "icedem_destroy": (C.c_int, [_p]),
     "icedem_frames": (C.c_int, [_p, C.POINTER(IceDemState), C.POINTER(IceDemInputs), _i,
                                 C.POINTER(IceDemHistory), C.POINTER(IceDemDiag), _i, _p]),
+    "icedem_frames_profiled": (C.c_int, [_p, C.POINTER(IceDemState), C.POINTER(IceDemInputs), _i,
+                                         C.POINTER(IceDemHistory), C.POINTER(C.c_float), _p]),
     "icedem_sync": (C.c_int, [_p, _p]),
     "icedem_substep_forces": (C.c_int, [_p, C.POINTER(IceDemState), _p, _p, C.c_int64, C.c_int64, _p, _p, _p]),
     "icedem_contact_pairs": (C.c_int, [_p, _p, _p, _p, C.c_int64, C.POINTER(C.c_int64), _p]),

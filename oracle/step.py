@@ -517,57 +517,91 @@ class Oracle:
         return tuple(np.where(use2, b, a).astype(ft) for a, b in zip(p1, p2))
 
     # ── a13: update_bond_damage_state, jax_bond_fracture.py:29-208 ───────────
-    def update_bond_damage_state(self, delta_n, delta_s, sigma_n, sigma_t, sigma_bar,
-                                 C, mu_b, G_c_I, G_c_II, eta_bk,
-                                 damage, dm_max, dm_0, intact):
+    def strength_criterion(self, sigma, tau, sigma_bar, tau_bar):
+        """compute_strength_criterion, jax_bond_fracture.py:35-52."""
         ft = self.ft
         SC = ft(1e12)
-        delta_n = vclip(delta_n, ft(-1e3), ft(1e3))
-        delta_s = vclip(delta_s, ft(-1e3), ft(1e3))
-        sigma_bar = vclip(sigma_bar, ft(1e-10), SC)
-        # compute_equivalent_deformation (:65-74)
+        sp = vmax(vclip(sigma, -SC, SC), ft(0.0))
+        tau = vclip(tau, -SC, SC)
+        sb = vmax(vclip(sigma_bar, ft(1e-10), SC), ft(1e-10))
+        tb = vmax(vclip(tau_bar, ft(1e-10), SC), ft(1e-10))
+        q1 = sp / sb
+        q2 = tau / tb
+        return vclip(q1 * q1 + q2 * q2, ft(0.0), ft(1e10))
+
+    def mohr_coulomb(self, C, mu_b, sigma):
+        """compute_mohr_coulomb_strength, jax_bond_fracture.py:55-62."""
+        ft = self.ft
+        return vmax(ft(C) - ft(mu_b) * sigma, ft(1e-6))
+
+    def equivalent_deformation(self, delta_n, delta_s):
+        """compute_equivalent_deformation, jax_bond_fracture.py:65-74."""
+        ft = self.ft
         dn2 = vclip(delta_n, ft(-1e3), ft(1e3))
         ds2 = vclip(np.abs(delta_s), ft(0.0), ft(1e3))
         dnp = vmax(dn2, ft(0.0))
-        delta_m = np.sqrt(vclip(dnp * dnp + ds2 * ds2 + ft(1e-20), ft(0.0), ft(1e12)))
-        new_dm_max = vmax(dm_max, delta_m)
-        # compute_mohr_coulomb_strength (:55-62)
-        tau_bar = vmax(ft(C) - ft(mu_b) * sigma_n, ft(1e-6))
-        # compute_strength_criterion (:35-52)
-        sp = vmax(vclip(sigma_n, -SC, SC), ft(0.0))
-        tau = vclip(sigma_t, -SC, SC)
-        sb = vmax(vclip(sigma_bar, ft(1e-10), SC), ft(1e-10))
-        tb = vmax(vclip(tau_bar, ft(1e-10), SC), ft(1e-10))
-        q1 = sp / sb; q2 = tau / tb
-        crit = vclip(q1 * q1 + q2 * q2, ft(0.0), ft(1e10))
-        init = (crit >= ft(1.0)) & (dm_0 <= ft(0.0)) & (delta_m > ft(1e-5))
-        new_dm0 = np.where(init, delta_m, dm_0)
-        # compute_fracture_energy_ratio (:98-105)
-        dnp2 = vmax(delta_n, ft(0.0))
-        G_ratio = (delta_s * delta_s) / (dnp2 * dnp2 + ft(1e-10))
-        G_I = ft(1.0) / ((ft(1.0) + G_ratio) + ft(1e-10))
-        G_II = G_ratio / ((ft(1.0) + G_ratio) + ft(1e-10))
-        # compute_benzeggagh_kenane_energy (:108-119)
+        return np.sqrt(vclip(dnp * dnp + ds2 * ds2 + ft(1e-20), ft(0.0), ft(1e12)))
+
+    def damage_variable(self, delta_m, delta_m_0, delta_m_f, delta_m_max):
+        """compute_damage_variable, jax_bond_fracture.py:77-95."""
+        ft = self.ft
+        d0 = vclip(delta_m_0, ft(0.0), ft(1e6))
+        df = vmax(vclip(delta_m_f, ft(1e-10), ft(1e6)), ft(1e-10))
+        dmax = vmax(vclip(delta_m_max, ft(0.0), ft(1e6)), d0 + ft(1e-10))
+        denom = dmax * ((df - d0) + ft(1e-10))
+        denom = vmax(denom, ft(1e-20))
+        D = (df * (dmax - d0)) / denom
+        return vclip(D, ft(0.0), ft(1.0))
+
+    def fracture_energy_ratio(self, delta_n, delta_s):
+        """compute_fracture_energy_ratio, jax_bond_fracture.py:98-105."""
+        ft = self.ft
+        dnp = vmax(delta_n, ft(0.0))
+        return (delta_s * delta_s) / (dnp * dnp + ft(1e-10))
+
+    def bk_energy(self, G_I, G_II, G_c_I, G_c_II, eta):
+        """compute_benzeggagh_kenane_energy, jax_bond_fracture.py:108-119."""
+        ft = self.ft
         G_total = (G_I + G_II) + ft(1e-10)
         ratio = G_II / G_total
-        G_c = ft(G_c_I) + (ft(G_c_II) - ft(G_c_I)) * self.pow(ratio, eta_bk)
+        return ft(G_c_I) + (ft(G_c_II) - ft(G_c_I)) * self.pow(ratio, eta)
+
+    def critical_deformation(self, G_c, sigma_m_0):
+        """compute_critical_deformation, jax_bond_fracture.py:122-128."""
+        ft = self.ft
+        return (ft(2.0) * G_c) / (np.abs(sigma_m_0) + ft(1e-10))
+
+    def update_bond_damage_state(self, delta_n, delta_s, sigma_n, sigma_t, sigma_bar,
+                                 C, mu_b, G_c_I, G_c_II, eta_bk,
+                                 damage, dm_max, dm_0, intact):
+        """update_bond_damage_state, jax_bond_fracture.py:133-208."""
+        ft = self.ft
+        SC = ft(1e12)
+        delta_n = vclip(np.asarray(delta_n, ft), ft(-1e3), ft(1e3))
+        delta_s = vclip(np.asarray(delta_s, ft), ft(-1e3), ft(1e3))
+        sigma_bar = vclip(np.asarray(sigma_bar, ft), ft(1e-10), SC)
+        sigma_n = np.asarray(sigma_n, ft); sigma_t = np.asarray(sigma_t, ft)
+        damage = np.asarray(damage, ft); dm_max = np.asarray(dm_max, ft); dm_0 = np.asarray(dm_0, ft)
+        intact = np.asarray(intact, bool)
+        delta_m = self.equivalent_deformation(delta_n, delta_s)
+        new_dm_max = vmax(dm_max, delta_m)
+        tau_bar = self.mohr_coulomb(C, mu_b, sigma_n)
+        crit = self.strength_criterion(sigma_n, sigma_t, sigma_bar, tau_bar)
+        init = (crit >= ft(1.0)) & (dm_0 <= ft(0.0)) & (delta_m > ft(1e-5))
+        new_dm0 = np.where(init, delta_m, dm_0)
+        G_ratio = self.fracture_energy_ratio(delta_n, delta_s)
+        G_I = ft(1.0) / ((ft(1.0) + G_ratio) + ft(1e-10))
+        G_II = G_ratio / ((ft(1.0) + G_ratio) + ft(1e-10))
+        G_c = self.bk_energy(G_I, G_II, G_c_I, G_c_II, eta_bk)
         snc = vclip(sigma_n, -SC, SC)
         stc = vclip(sigma_t, -SC, SC)
         mp = vmax(snc, ft(0.0))
         sm0_raw = np.sqrt(vclip(mp * mp + stc * stc + ft(1e-20), ft(0.0), ft(1e24)))
         sm0 = vmax(sm0_raw, ft(1e-10))
-        # compute_critical_deformation (:122-128)
-        dmf = (ft(2.0) * G_c) / (np.abs(sm0) + ft(1e-10))
+        dmf = self.critical_deformation(G_c, sm0)
         meaningful = sm0_raw >= ft(1e4)
         has_damage = (new_dm0 > ft(0.0)) & meaningful
-        # compute_damage_variable (:77-95)
-        d0 = vclip(new_dm0, ft(0.0), ft(1e6))
-        df = vmax(vclip(dmf, ft(1e-10), ft(1e6)), ft(1e-10))
-        dmax = vmax(vclip(new_dm_max, ft(0.0), ft(1e6)), d0 + ft(1e-10))
-        denom = dmax * ((df - d0) + ft(1e-10))
-        denom = vmax(denom, ft(1e-20))
-        D = (df * (dmax - d0)) / denom
-        D = vclip(D, ft(0.0), ft(1.0))
+        D = self.damage_variable(delta_m, new_dm0, dmf, new_dm_max)
         computed = np.where(has_damage, D, damage)
         nd = np.where(meaningful, vmax(computed, damage), damage)
         nd = vmin(nd, damage + ft(0.15))
