@@ -32,7 +32,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "DEM element-steps/s; planner rollouts scored/s, surrogate vs plain DEM, ≤5% cost err"
 HBM_PEAK_GBS = 8000.0
-KERNELS = ["k_hull", "k_mid", "k_scatter", "k_force", "k_bond", "k_ship"]
+KERNELS = ["k_cull", "k_near", "k_mid", "k_scatter", "k_force", "k_bond", "k_ship"]
 
 
 def build_case(config: str, seed: int):
@@ -68,8 +68,10 @@ def algorithmic_bytes(kernel: str, n: int, nb: int, n_near: int, max_v: int, n_h
         # v_water 8, cells 8; writes 34 (next pos/vel/rot/omega 24, sub 4, fm 4, active 1, crack 1)
         # per bond: i, j, rest, damage 16 + intact 1 + two CSR entries 8
         return 138.0 * n + 25.0 * nb
-    if kernel == "k_hull":
-        return 64.0 * n + n_near * (9.0 * max_v) + 8.0 * n_hull
+    if kernel == "k_cull":
+        return 25.0 * n
+    if kernel == "k_near":
+        return n_near * (9.0 * max_v + 60.0) + 8.0 * n_hull
     if kernel == "k_bond":
         return 55.0 * nb + 82.0 * n + 8.0 * G
     if kernel == "k_scatter":
@@ -176,7 +178,7 @@ def main():
 
     # per-kernel durations with HIP events on the launch stream
     P = max(1, args.profile_frames)
-    ms = (C.c_float * 6)()
+    ms = (C.c_float * len(KERNELS))()
     s = eng.state_struct(st, ship)
     inp = _lib.IceDemInputs()
     sp_slice = setp[total - P: total].contiguous() if total >= P else setp[:P].contiguous()
@@ -188,7 +190,7 @@ def main():
     dom = max(avg_us, key=avg_us.get)
     pen = eng.alloc_diag()
     n_near = n  # upper bound for the hull kernel's vertex traffic
-    G = eng.grid[3] * eng.grid[4]
+    G = eng.grid[4] * eng.grid[5]
     abytes = algorithmic_bytes(dom, n, nb, n_near, eng.max_v, hull.shape[0], G)
     achieved = abytes / (avg_us[dom] * 1e-6) / 1e9
     traffic = None

@@ -102,10 +102,13 @@ typedef struct {
     float M3inv[9], D3[9], Kp[9], Kd[9], Ki[9], D3pKd[9];
     float B_pinv[18], B[18], n_max[6];
     float T_n, vc_surge, vc_sway, vc_yaw;
-    /* uniform broad-phase grid (build choice; any cell >= 2 r_max + 2 delta
-     * gives the reference's exact contact set) */
-    float grid_x0, grid_y0, grid_cell;
-    int32_t grid_nx, grid_ny;
+    /* uniform broad-phase grid (build choice; the contact set is the
+     * reference's exact one for any cell >= 2 (grid_rsmall + delta)):
+     * elements with radius <= grid_rsmall sit in one cell and scan 3x3;
+     * larger ones are inserted into, and scan, every cell within
+     * r + grid_rsmall + 2 delta.  grid_cell_cap bounds the cell list. */
+    float grid_x0, grid_y0, grid_cell, grid_rsmall;
+    int32_t grid_nx, grid_ny, grid_cell_cap;
 } IceDemParams;
 
 /* Mutable state (IceState + ShipState, jax_ice_state.py:31-52), device
@@ -176,9 +179,10 @@ int icedem_frames(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* 
                   const IceDemHistory* hist, const IceDemDiag* diag, int32_t use_graph, void* stream);
 
 /* icedem_frames without graph replay, timing each kernel of the frame with
- * HIP events on `stream`: h_kernel_ms[0..5] receive the summed durations of
- * k_hull, k_mid, k_scatter, k_force (all sub-steps), k_bond and k_ship over
- * the K frames.  Instrumentation for benchmarks; no reference counterpart. */
+ * HIP events on `stream`: h_kernel_ms[0..6] receive the summed durations of
+ * k_cull, k_near, k_mid, k_scatter, k_force (all sub-steps), k_bond and
+ * k_ship over the K frames.  Instrumentation for benchmarks; no reference
+ * counterpart. */
 int icedem_frames_profiled(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* in, int32_t K,
                            const IceDemHistory* hist, float* h_kernel_ms, void* stream);
 

@@ -14,7 +14,7 @@ constexpr int MAX_PARTIALS = 4096;  // block partials reduced in one workgroup (
 // Static data, per-frame scratch and parameters of one context (device memory).
 struct DevView {
     IceDemParams p;
-    int32_t n, nb, nf, maxv, S, G, nblk, nbpad;
+    int32_t n, nb, nf, maxv, S, G, nblk, nbpad, cell_cap;
     // static
     const float2* polys;      // (n*maxv) body-frame vertices
     const uint8_t* mask;      // (n*maxv)
@@ -27,16 +27,20 @@ struct DevView {
     const int32_t* csr_start; // (n+1) bonds of each element: i-role then j-role
     const int32_t* csr_ent;   // (2 nb) entry = bond << 1 | role (0 = i, 1 = j)
     // per-frame scratch
-    float* pen; float* nx; float* ny; float* fm_raw; float* soi_x; float* soi_y; float* fm;
+    float* pen; float* nx; float* ny; float* fm_raw; float* soi_x; float* soi_y; float* mterm; float* fm;
+    uint8_t* nearf;           // (n) element passed the hull cull this frame
+    int32_t* near_list;       // (n) queue of near elements
+    int32_t* near_count;      // (1)
+    uint8_t* nearblk;         // (nblk) block of 256 elements holds a near element
     float* shipf;             // (2n) ship forces on ice, kept for sub-steps > 0
     uint8_t* crack;           // (n) element touches a broken bond
     int32_t* key;             // (n) cell of each element
     int32_t* cell_count; int32_t* cell_fill; int32_t* cell_start;   // (G), (G), (G+1)
-    int32_t* cell_list;       // (n)
+    int32_t* cell_list;       // (cell_cap): small elements once, big ones per covered cell
     uint32_t* floe_coll; uint32_t* floe_maxfm;                      // (nf)
     float* part1;             // (3 * nbpad) block partials: sum soi_x, soi_y, moment
     float* part2;             // (2 * nbpad) block partials: sum of scaled soi
-    float* scal;              // [0] thrust scale, [1] M_ice_on_ship
+    float* scal;              // [0] thrust scale, [1] M_ice_on_ship, [2..3] sum of scaled ship forces
     float* nxt_pos; float* nxt_vel; float* nxt_rot; float* nxt_omega;
     int32_t* err;
 };

@@ -58,8 +58,8 @@ class IceDemParams(C.Structure):
         ("M3inv", _f * 9), ("D3", _f * 9), ("Kp", _f * 9), ("Kd", _f * 9), ("Ki", _f * 9), ("D3pKd", _f * 9),
         ("B_pinv", _f * 18), ("B", _f * 18), ("n_max", _f * 6),
         ("T_n", _f), ("vc_surge", _f), ("vc_sway", _f), ("vc_yaw", _f),
-        ("grid_x0", _f), ("grid_y0", _f), ("grid_cell", _f),
-        ("grid_nx", _i), ("grid_ny", _i),
+        ("grid_x0", _f), ("grid_y0", _f), ("grid_cell", _f), ("grid_rsmall", _f),
+        ("grid_nx", _i), ("grid_ny", _i), ("grid_cell_cap", _i),
     ]
 
 

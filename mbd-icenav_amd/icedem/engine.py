@@ -41,23 +41,38 @@ def bound_radius(local_polys_padded: np.ndarray) -> np.ndarray:
 
 
 def choose_grid(centroids: np.ndarray, radii: np.ndarray, dilation: float = 0.0, cell: Optional[float] = None,
-                margin: float = 60.0, max_cells: int = 1 << 22):
-    """Uniform grid for the broad phase.  Any cell >= 2 (r_max + delta) finds
-    every contacting pair in the 3x3 neighbourhood; positions outside the grid
-    are clamped to border cells (still exact, only slower)."""
-    rmax = float(np.max(radii)) if len(radii) else 1.0
-    min_cell = 2.0 * (rmax + dilation) * 1.01 + 1e-3
+                margin: float = 60.0, max_cells: int = 1 << 22, small_quantile: float = 0.98):
+    """Uniform grid for the broad phase.
+
+    Elements with radius <= r_s (the ``small_quantile`` radius) live in one
+    cell of side >= 2 (r_s + delta) and scan the 3x3 cells around it; the few
+    larger ones are inserted into, and scan, every cell within
+    r + r_s + 2 delta.  Either way every contacting pair is found (exactly the
+    reference's contact set); positions outside the grid are clamped to the
+    border cells.  Returns (x0, y0, cell, r_s, nx, ny, cell_list_capacity).
+    """
+    r = np.asarray(radii, dtype=np.float64)
+    r32 = r.astype(F32)
+    if len(r):
+        r_s = float(F32(np.quantile(r, small_quantile)))
+    else:
+        r_s = 1.0
+    min_cell = 2.0 * (r_s + dilation) * 1.01 + 1e-3
     cell = max(min_cell, float(cell) if cell is not None else min_cell)
     c = np.asarray(centroids, dtype=np.float64)
-    lo = c.min(axis=0) - margin
-    hi = c.max(axis=0) + margin
+    lo = c.min(axis=0) - margin if len(c) else np.zeros(2)
+    hi = c.max(axis=0) + margin if len(c) else np.ones(2)
     while True:
         nx = max(1, int(math.ceil((hi[0] - lo[0]) / cell)))
         ny = max(1, int(math.ceil((hi[1] - lo[1]) / cell)))
         if nx * ny <= max_cells:
             break
         cell *= 1.5
-    return float(lo[0]), float(lo[1]), float(cell), nx, ny
+    big = r32 > F32(r_s)
+    reach = (r32[big].astype(np.float64) + r_s + 2.0 * dilation) * 1.001 + 1e-3
+    per_big = (np.floor(2.0 * reach / cell) + 3) ** 2
+    cap = int(np.sum(~big) + np.sum(np.minimum(per_big, nx * ny))) + 64
+    return float(lo[0]), float(lo[1]), float(cell), r_s, nx, ny, cap
 
 
 def make_params(static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_length, ship_beam, n_ice_steps,
@@ -139,8 +154,9 @@ def make_params(static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_len
     vc = tuple(float(v) for v in cfg.ship_velocity_clip)
     p.vc_surge, p.vc_sway, p.vc_yaw = F32(vc[0]), F32(vc[1]), F32(vc[2])
     if grid is None:
-        raise ValueError("grid (x0, y0, cell, nx, ny) is required; see choose_grid")
-    p.grid_x0, p.grid_y0, p.grid_cell, p.grid_nx, p.grid_ny = F32(grid[0]), F32(grid[1]), F32(grid[2]), grid[3], grid[4]
+        raise ValueError("grid (x0, y0, cell, r_s, nx, ny, cap) is required; see choose_grid")
+    p.grid_x0, p.grid_y0, p.grid_cell, p.grid_rsmall = F32(grid[0]), F32(grid[1]), F32(grid[2]), F32(grid[3])
+    p.grid_nx, p.grid_ny, p.grid_cell_cap = int(grid[4]), int(grid[5]), int(grid[6])
     return p
 
 

@@ -134,7 +134,7 @@ def build_jax_sim_context(polygons, centroids, bonds, ship_pos, ship_length, shi
 
     if verbose:
         print(f"  [icedem-ctx] Ready in {time.perf_counter() - t0:.2f} s "
-              f"({eng.n} elements, {eng.n_bonds} bonds, grid {eng.grid[3]}x{eng.grid[4]})", flush=True)
+              f"({eng.n} elements, {eng.n_bonds} bonds, grid {eng.grid[4]}x{eng.grid[5]})", flush=True)
     return JaxSimContext(fs, scan_segment, static, ship_dyn, eng.n, eng.n_bonds, dt, chunk_size)
 
 

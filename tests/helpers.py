@@ -82,6 +82,24 @@ class Case:
             traj = scenario.straight_trajectory(x0, y0, math.pi / 2, 2.0, dt, n_frames)
             hull = omodel.hull_polygon(L, B, n_pts=hull_points // 2 - 1)
             warm = 0
+        elif name == "ring":
+            # one big square ringed by 60 small squares: >48 contacts on one element
+            polys = [np.array([[-10.0, -10.0], [10.0, -10.0], [10.0, 10.0], [-10.0, 10.0]])]
+            for k in range(60):
+                a = 2 * np.pi * k / 60
+                c = 11.6 * np.array([np.cos(a), np.sin(a)])
+                polys.append(c + np.array([[-1.0, -1.0], [1.0, -1.0], [1.0, 1.0], [-1.0, 1.0]]))
+            cents = np.array([p.mean(axis=0) for p in polys])
+            n = len(polys)
+            sc = scenario.Scene(polys, cents, [(0, 1), (1, 2), (2, 3)], np.ones(n), np.arange(n, dtype=np.int32),
+                                n, 0.0, (-30.0, 30.0, -30.0, 30.0))
+            cfg = cfg or benchmark_config()
+            L, B = 60.0, 12.0
+            mass = ShipProperties(L, B, 5.0).mass
+            dt = cfg.dt
+            traj = scenario.straight_trajectory(0.0, -60.0, math.pi / 2, 2.0, dt, n_frames)
+            hull = omodel.hull_polygon(L, B, n_pts=hull_points // 2 - 1)
+            warm = 0
         else:
             raise ValueError(name)
         case = Case(name, sc, cfg, L, B, mass, hull, traj, dt, use_auto, warm)

@@ -9,8 +9,9 @@ from oracle import bonds as obonds
 pytestmark = pytest.mark.gpu
 
 
-def test_contact_pairs_exact():
-    case = Case.make("c1small")
+@pytest.mark.parametrize("name", ["c1small", "ring"])
+def test_contact_pairs_exact(name):
+    case = Case.make(name)
     eng = case.engine()
     o = case.oracle()
     rng = np.random.default_rng(1)
@@ -23,8 +24,9 @@ def test_contact_pairs_exact():
     np.testing.assert_array_equal(gj.cpu().numpy(), oj)
 
 
-def test_substep_forces():
-    case = Case.make("c1small")
+@pytest.mark.parametrize("name", ["c1small", "ring"])
+def test_substep_forces(name):
+    case = Case.make(name)
     eng = case.engine()
     o = case.oracle()
     st, ship = case.gpu_initial()
