@@ -32,7 +32,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "DEM element-steps/s; planner rollouts scored/s, surrogate vs plain DEM, ≤5% cost err"
 HBM_PEAK_GBS = 8000.0
-KERNELS = ["k_cull", "k_near", "k_mid", "k_scatter", "k_force", "k_bond", "k_ship"]
+KERNELS = ["k_cull", "k_near", "k_force", "k_bond"]
 
 
 def build_case(config: str, seed: int):
@@ -71,7 +71,7 @@ def algorithmic_bytes(kernel: str, n: int, nb: int, n_near: int, max_v: int, n_h
     if kernel == "k_cull":
         return 25.0 * n
     if kernel == "k_near":
-        return n_near * (9.0 * max_v + 60.0) + 8.0 * n_hull
+        return n_near * (9.0 * max_v + 60.0) + 8.0 * n_hull + 12.0 * G / 256.0
     if kernel == "k_bond":
         return 55.0 * nb + 82.0 * n + 8.0 * G
     if kernel == "k_scatter":

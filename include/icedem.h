@@ -98,6 +98,13 @@ typedef struct {
     float bond_cohesion, bond_friction, G_c_I, G_c_II, eta_bk;
     /* ship, jax_vessel_dynamics.py:246-348 */
     int32_t use_auto_icenav;
+    /* ship_mode 0: the reference's DP tracking of setpoints.  ship_mode 1
+     * (rollouts; no reference counterpart): per-frame controls (u, r_cmd) in
+     * [-1, 1] give tau = [u ctrl_thrust_max, 0, Kd[2,2] (r_cmd
+     * ctrl_yaw_rate_max - r)]; ship_mass/ship_Izz feed the rollout's
+     * kinetic-energy loss. */
+    int32_t ship_mode;
+    float ctrl_thrust_max, ctrl_yaw_rate_max, ship_mass, ship_Izz;
     float ice_brake_threshold;     /* 0 = disabled */
     float M3inv[9], D3[9], Kp[9], Kd[9], Ki[9], D3pKd[9];
     float B_pinv[18], B[18], n_max[6];
@@ -106,9 +113,10 @@ typedef struct {
      * reference's exact one for any cell >= 2 (grid_rsmall + delta)):
      * elements with radius <= grid_rsmall sit in one cell and scan 3x3;
      * larger ones are inserted into, and scan, every cell within
-     * r + grid_rsmall + 2 delta.  grid_cell_cap bounds the cell list. */
+     * r + grid_rsmall + 2 delta.  Each cell holds up to grid_bucket
+     * elements; overflow goes to a list that every element also scans. */
     float grid_x0, grid_y0, grid_cell, grid_rsmall;
-    int32_t grid_nx, grid_ny, grid_cell_cap;
+    int32_t grid_nx, grid_ny, grid_bucket;
 } IceDemParams;
 
 /* Mutable state (IceState + ShipState, jax_ice_state.py:31-52), device
@@ -134,10 +142,12 @@ typedef struct {
  * element e; sk = 0 broadcasts one field over all frames) and
  * v_water_ship[k*2 .. k*2+1] (NULL = zero current at the ship). */
 typedef struct {
-    const float* d_setpoints;     /* (K, 6) */
+    const float* d_setpoints;     /* (K, 6); unused when ship_mode == 1 */
     const float* d_v_water;       /* NULL = zero current */
     int64_t vw_stride_k, vw_stride_c, vw_stride_e;
     const float* d_v_water_ship;  /* (K, 2) or NULL */
+    const float* d_controls;      /* ship_mode 1: (n_knots, 2) (u, r_cmd) */
+    int32_t n_knots, hold;        /* frame k uses knot min(k / hold, n_knots - 1) */
 } IceDemInputs;
 
 /* scan_segment histories (jax_simulation.py:1317-1325); any pointer may be
@@ -159,6 +169,9 @@ typedef struct {
     float* d_force_mag;     /* (n) */
     float* d_F_ice_on_ship; /* (2) */
     float* d_J;             /* (1) */
+    float* d_rollout;       /* (4) over the call's K frames: impulse sum |F| dt,
+                               kinetic-energy loss, peak |F|, initial kinetic
+                               energy (no reference counterpart; rollouts) */
 } IceDemDiag;
 
 const char* icedem_status_string(int code);
@@ -185,6 +198,21 @@ int icedem_frames(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* 
  * counterpart. */
 int icedem_frames_profiled(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* in, int32_t K,
                            const IceDemHistory* hist, float* h_kernel_ms, void* stream);
+
+/* Batched rollouts (no reference counterpart: the reference's planners never
+ * roll out the DEM).  A batch groups n_inst existing contexts (one scene
+ * each, any sizes up to 65536 elements, n_ice_steps == 1; several contexts
+ * may hold the same scene); icedem_batch_rollouts advances every instance by
+ * `frames` frames in ONE persistent launch, one workgroup per instance, with
+ * the same device code as icedem_frames (results are identical to K =
+ * `frames` calls of icedem_frames on each context).  states/inputs/diags are
+ * arrays of n_inst; diags[i].d_rollout receives the rollout cost terms.
+ * use_graph != 0 replays a cached hipGraph of the launch. */
+typedef struct IceDemBatch IceDemBatch;
+int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** out);
+int icedem_batch_rollouts(IceDemBatch* batch, const IceDemState* states, const IceDemInputs* inputs, int32_t frames,
+                          const IceDemDiag* diags, int32_t use_graph, void* stream);
+int icedem_batch_destroy(IceDemBatch* batch);
 
 /* Waits for `stream` and reports errors flagged by kernels since the last call. */
 int icedem_sync(IceDemCtx* ctx, void* stream);

@@ -1,17 +1,17 @@
 // frame_kernels.h -- the per-frame kernels of the DPDEM step.
 //
-//   k_cull     per element: near-ship test (bounding circles), near work
-//              queue, zero pre-pass for far elements, cell histogram
-//   k_near     one wavefront per queued element: hull pass 1 + 2, pre-pass,
-//              floe or/max (integer atomics)
-//   k_mid      one workgroup: pairwise-tree totals over the blocks that hold
-//              contacts, thrust scale, ship reaction, floe eligibility, scan
-//   k_scatter  per element: counting-sort scatter into cells
-//   k_force    16 lanes per element: activation, submergence, bond forces,
-//              contacts (lane-parallel candidate scan, LDS sort by partner),
-//              drag, integration
-//   k_bond     per bond: fracture; per element: commit + histories; clears
-//   k_ship     one wave: ship step and ship histories
+//   k_cull    per element: near-ship test (bounding circles) into a work
+//             queue; insertion into the fixed-capacity cell buckets
+//   k_near    one wavefront per queued element: hull pass 1 + 2, pre-pass,
+//             floe or/max (integer atomics).  The last workgroup to finish
+//             (ticket counter) computes the pairwise-tree totals, the thrust
+//             scale, floe eligibility and the ship step.
+//   k_force   16 lanes per element: activation, submergence, bond forces
+//             (lane-parallel terms, sequential CSR-order sum), contacts
+//             (lane-parallel scan of the 3x3 buckets, LDS sort by partner),
+//             drag, integration
+//   k_bond    per bond: fracture; per element: commit + histories; clears
+// The same bodies run inside one workgroup per scene for batched rollouts.
 #pragma once
 #include "kernels.h"
 
@@ -21,11 +21,15 @@ constexpr int GRP = 16;                 // lanes per element in k_force
 constexpr int EPB = BLOCK / GRP;        // elements per k_force block
 constexpr int MAXC2 = 48;               // contact candidates kept per element
 
-// ── broad phase: uniform grid, big elements inserted into every cell of their reach ──
+// ── broad phase: uniform grid of fixed-capacity buckets ─────────────────────
+// Elements of radius <= grid_rsmall sit in the bucket of their cell and scan
+// the 3x3 cells around it (cell >= 2 (grid_rsmall + delta)); larger elements
+// are inserted into, and scan, every cell within r + grid_rsmall + 2 delta.
+// A full bucket sends the element to an overflow list that every element
+// also scans.  Any of these gives exactly the reference's contact set.
 
 __device__ __forceinline__ bool is_big(const IceDemParams& P, float r) { return r > P.grid_rsmall; }
 
-// cell range of the box of half-width R around (x, y), clamped to the grid
 __device__ __forceinline__ void cell_box(const IceDemParams& P, float x, float y, float R,
                                          int& x0, int& x1, int& y0, int& y1) {
     float fx0 = floorf((x - R - P.grid_x0) / P.grid_cell), fx1 = floorf((x + R - P.grid_x0) / P.grid_cell);
@@ -35,41 +39,28 @@ __device__ __forceinline__ void cell_box(const IceDemParams& P, float x, float y
     y0 = (int)fminf(fmaxf(fy0, 0.0f), my); y1 = (int)fminf(fmaxf(fy1, 0.0f), my);
 }
 
-// reach of a big element: every partner j satisfies |x_j - x_e| < reach
 __device__ __forceinline__ float big_reach(const IceDemParams& P, float r) {
     return (r + P.grid_rsmall + P.contact_dilation2) * 1.001f + 1e-3f;
 }
 
-__device__ __forceinline__ void cells_count(const DevView* __restrict__ V, int e, float x, float y) {
-    const IceDemParams& P = V->p;
-    const float r = V->radius[e];
-    if (!is_big(P, r)) {
-        int cx, cy;
-        int key = cell_of(P, x, y, cx, cy);
-        atomicAdd(&V->cell_count[key], 1);
-    } else {
-        int x0, x1, y0, y1;
-        cell_box(P, x, y, big_reach(P, r), x0, x1, y0, y1);
-        for (int gy = y0; gy <= y1; ++gy)
-            for (int gx = x0; gx <= x1; ++gx) atomicAdd(&V->cell_count[gy * P.grid_nx + gx], 1);
-    }
+__device__ __forceinline__ void bucket_put(const DevView* __restrict__ V, int c, int e) {
+    const int cap = V->p.grid_bucket;
+    const int idx = atomicAdd(&V->cell_count[c], 1);
+    if (idx < cap) V->cell_slots[(size_t)c * cap + idx] = e;
+    else V->ovf_list[atomicAdd(V->ovf_count, 1)] = e;
 }
 
-__device__ __forceinline__ void cells_scatter(const DevView* __restrict__ V, int e, float x, float y) {
+__device__ __forceinline__ void cells_insert(const DevView* __restrict__ V, int e, float x, float y) {
     const IceDemParams& P = V->p;
     const float r = V->radius[e];
     if (!is_big(P, r)) {
         int cx, cy;
-        int key = cell_of(P, x, y, cx, cy);
-        V->cell_list[V->cell_start[key] + atomicAdd(&V->cell_fill[key], 1)] = e;
+        bucket_put(V, cell_of(P, x, y, cx, cy), e);
     } else {
         int x0, x1, y0, y1;
         cell_box(P, x, y, big_reach(P, r), x0, x1, y0, y1);
         for (int gy = y0; gy <= y1; ++gy)
-            for (int gx = x0; gx <= x1; ++gx) {
-                const int c = gy * P.grid_nx + gx;
-                V->cell_list[V->cell_start[c] + atomicAdd(&V->cell_fill[c], 1)] = e;
-            }
+            for (int gx = x0; gx <= x1; ++gx) bucket_put(V, gy * P.grid_nx + gx, e);
     }
 }
 
@@ -163,12 +154,14 @@ __device__ __forceinline__ PrePass prepass(const IceDemParams& P, float pen, flo
     return o;
 }
 
+
 // ── k_cull ───────────────────────────────────────────────────────────────────
 
-__global__ __launch_bounds__(BLOCK) void k_cull(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
+// elements [chunk*256, chunk*256+256) by a 256-thread block (all threads call)
+__device__ __forceinline__ void cull_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int chunk) {
     const IceDemParams& P = V->p;
     const int n = V->n;
-    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    const int e = chunk * BLOCK + threadIdx.x;
     const float* pos = D->s.d_pos;
     const float* ship = D->s.d_ship;
     bool near = false;
@@ -179,20 +172,40 @@ __global__ __launch_bounds__(BLOCK) void k_cull(const DevView* __restrict__ V, c
         near = (dx * dx + dy * dy) <= R * R;
         V->nearf[e] = near ? 1 : 0;
         if (near) V->near_list[atomicAdd(V->near_count, 1)] = e;
-        cells_count(V, e, px, py);
+        cells_insert(V, e, px, py);
     }
     const int any = __syncthreads_or(near ? 1 : 0);
-    if (threadIdx.x == 0) V->nearblk[blockIdx.x] = any ? 1 : 0;
+    if (threadIdx.x == 0) V->nearblk[chunk] = any ? 1 : 0;
 }
 
-// ── k_near: one wavefront per queued element ─────────────────────────────────
+__global__ __launch_bounds__(BLOCK) void k_cull(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
+    cull_body(V, D, blockIdx.x);
+}
 
-__global__ __launch_bounds__(BLOCK) void k_near(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
+// cell insertion only (sub-steps > 0 and the leaf entry points)
+__global__ __launch_bounds__(BLOCK) void k_cellkey(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                                   const float* __restrict__ pos, int from_state) {
+    const int n = V->n;
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= n) return;
+    if (from_state) pos = D->s.d_pos;
+    cells_insert(V, e, pos[e], pos[n + e]);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_clear_cells(const DevView* __restrict__ V) {
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t < V->G) V->cell_count[t] = 0;
+    if (t == 0) *V->ovf_count = 0;
+}
+
+// ── k_near ───────────────────────────────────────────────────────────────────
+
+// waves [wave, wave + nwaves, ...] of the near queue; every lane returns normally
+__device__ __forceinline__ void near_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                          int wave, int nwaves) {
     const IceDemParams& P = V->p;
     const int n = V->n, maxv = V->maxv;
     const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
-    const int nwaves = gridDim.x * (BLOCK / 64);
     const int count = *V->near_count;
     if (wave >= count) return;
     const float* pos = D->s.d_pos;
@@ -227,34 +240,9 @@ __global__ __launch_bounds__(BLOCK) void k_near(const DevView* __restrict__ V, c
     }
 }
 
-// ── k_mid ────────────────────────────────────────────────────────────────────
-
-// Exclusive scan of V->cell_count into V->cell_start (any block size <= 1024).
-__device__ void scan_cells(const DevView* __restrict__ V, int* lds) {
-    const int G = V->G;
-    const int T = blockDim.x;
-    const int tid = threadIdx.x;
-    const int chunk = (G + T - 1) / T;
-    const int lo = min(G, tid * chunk), hi = min(G, lo + chunk);
-    int s = 0;
-    for (int i = lo; i < hi; ++i) s += V->cell_count[i];
-    lds[tid] = s;
-    __syncthreads();
-    for (int off = 1; off < T; off <<= 1) {
-        int v = (tid >= off) ? lds[tid - off] : 0;
-        __syncthreads();
-        lds[tid] += v;
-        __syncthreads();
-    }
-    int run = lds[tid] - s;
-    for (int i = lo; i < hi; ++i) { V->cell_start[i] = run; run += V->cell_count[i]; }
-    if (tid == T - 1) V->cell_start[G] = lds[T - 1];
-    __syncthreads();
-}
-
 // Tree sum of one 256-element block of a near-masked per-element array
-// (far elements contribute +0), done by one wave: lane l sums its 4
-// consecutive elements pairwise, then the butterfly combines aligned groups.
+// (far elements contribute +0), by one wave: lane l sums its 4 consecutive
+// elements pairwise, then the butterfly combines aligned groups.
 __device__ __forceinline__ float block256_tree(const DevView* __restrict__ V, const float* __restrict__ a,
                                                int blk, float scale, bool scaled) {
     const int lane = threadIdx.x & 63;
@@ -270,12 +258,71 @@ __device__ __forceinline__ float block256_tree(const DevView* __restrict__ V, co
     return wave_tree_sum(x);
 }
 
-__global__ __launch_bounds__(1024) void k_mid(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
-    __shared__ float la[MAX_PARTIALS], lb[MAX_PARTIALS];
+// ship reaction and ship step (jax_simulation.py:471-479, 658-697); thread 0
+__device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                           int k, int last) {
+    const IceDemParams& P = V->p;
+    float Fx = -V->scal[2], Fy = -V->scal[3];
+    float tau[3] = {Fx, Fy, V->scal[1]};
+    float* ship = D->s.d_ship;
+    float spc[3] = {0.0f, 0.0f, 0.0f}, spn[3] = {0.0f, 0.0f, 0.0f};
+    float cu = 0.0f, cr = 0.0f;
+    if (P.ship_mode == 1) {
+        if (D->in.d_controls && D->in.n_knots > 0) {
+            const int knot = min(k / max(D->in.hold, 1), D->in.n_knots - 1);
+            cu = D->in.d_controls[2 * knot];
+            cr = D->in.d_controls[2 * knot + 1];
+        }
+    } else {
+        const float* sp = D->in.d_setpoints + (size_t)k * 6;
+        spc[0] = sp[0]; spc[1] = sp[1]; spc[2] = sp[2];
+        spn[0] = sp[3]; spn[1] = sp[4]; spn[2] = sp[5];
+        if (P.ice_brake_threshold > 0.0f) {
+            float thr = P.ice_brake_threshold;
+            float surge = fabsf(tau[0]);
+            float bf = vclip(thr / vmax(surge, thr), 0.1f, 1.0f);
+            spc[0] = ship[0] + bf * (sp[0] - ship[0]);
+            spc[1] = ship[1] + bf * (sp[1] - ship[1]);
+        }
+    }
+    float vwx = 0.0f, vwy = 0.0f;
+    if (D->in.d_v_water_ship) { vwx = D->in.d_v_water_ship[2 * k]; vwy = D->in.d_v_water_ship[2 * k + 1]; }
+    float st[ICEDEM_SHIP_STATE_LEN];
+    for (int i = 0; i < ICEDEM_SHIP_STATE_LEN; ++i) st[i] = ship[i];
+    const float Fm = sqrtf(Fx * Fx + Fy * Fy);
+    const float J = Fm * P.dt;
+    float* ro = D->dg.d_rollout;
+    if (ro) {
+        if (k == 0) {
+            ro[0] = 0.0f; ro[2] = 0.0f;
+            ro[3] = 0.5f * P.ship_mass * (st[3] * st[3] + st[4] * st[4]) + 0.5f * P.ship_Izz * (st[5] * st[5]);
+        }
+        ro[0] = ro[0] + J;
+        ro[2] = vmax(ro[2], Fm);
+    }
+    ship_step(P, st, spc, spn, tau, vwx, vwy, cu, cr);
+    for (int i = 0; i < ICEDEM_SHIP_STATE_LEN; ++i) ship[i] = st[i];
+    if (ro && last) {
+        const float ke = 0.5f * P.ship_mass * (st[3] * st[3] + st[4] * st[4]) + 0.5f * P.ship_Izz * (st[5] * st[5]);
+        ro[1] = ro[3] - ke;
+    }
+    if (D->h.d_eta) { D->h.d_eta[3 * k] = st[0]; D->h.d_eta[3 * k + 1] = st[1]; D->h.d_eta[3 * k + 2] = st[2]; }
+    if (D->h.d_J) D->h.d_J[k] = J;
+    if (D->h.d_F) { D->h.d_F[2 * k] = Fx; D->h.d_F[2 * k + 1] = Fy; }
+    if (last) {
+        if (D->dg.d_F_ice_on_ship) { D->dg.d_F_ice_on_ship[0] = Fx; D->dg.d_F_ice_on_ship[1] = Fy; }
+        if (D->dg.d_J) D->dg.d_J[0] = J;
+    }
+}
+
+// Totals, thrust cap (jax_simulation.py:456-479), floe eligibility
+// (:511-524) and the ship step, by one workgroup of any size; la, lb hold
+// nbpad floats each.
+__device__ void mid_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D, float* la, float* lb,
+                         int k, int last) {
     const IceDemParams& P = V->p;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int nb = V->nblk, nbpad = V->nbpad;
-    // raw totals: block partials for blocks holding near elements, 0 elsewhere
     for (int b = w; b < nb; b += nw) {
         float a = 0.0f, c = 0.0f, d = 0.0f;
         if (V->nearblk[b]) {
@@ -294,7 +341,6 @@ __global__ __launch_bounds__(1024) void k_mid(const DevView* __restrict__ V, con
         float F_tot = sqrtf(tx * tx + ty * ty);
         scale = (F_tot > P.max_ship_ice_force) ? (P.max_ship_ice_force / vmax(F_tot, 1e-6f)) : 1.0f;
     }
-    // reaction on the ship: sum of the scaled contact forces (jax_simulation.py:472)
     for (int b = w; b < nb; b += nw) {
         float a = 0.0f, c = 0.0f;
         if (V->nearblk[b]) {
@@ -317,72 +363,110 @@ __global__ __launch_bounds__(1024) void k_mid(const DevView* __restrict__ V, con
             elig[f] = el ? 1 : 0;
         }
     }
-    scan_cells(V, reinterpret_cast<int*>(la));
+    __syncthreads();
+    if (threadIdx.x == 0) ship_frame(V, D, k, last);
+    __syncthreads();
 }
 
-__global__ __launch_bounds__(1024) void k_scan_cells(const DevView* __restrict__ V) {
-    __shared__ int lds[1024];
-    scan_cells(V, lds);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_cellkey(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                                   const float* __restrict__ pos, int from_state) {
-    const int n = V->n;
-    const int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= n) return;
-    if (from_state) pos = D->s.d_pos;
-    cells_count(V, e, pos[e], pos[n + e]);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_scatter(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                                   const float* __restrict__ pos, int from_state) {
-    const int n = V->n;
-    const int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= n) return;
-    if (from_state) pos = D->s.d_pos;
-    cells_scatter(V, e, pos[e], pos[n + e]);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_clear_cells(const DevView* __restrict__ V) {
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
-    if (t < V->G) { V->cell_count[t] = 0; V->cell_fill[t] = 0; }
+__global__ __launch_bounds__(BLOCK) void k_near(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                                int k, int last) {
+    __shared__ float la[MAX_PARTIALS], lb[MAX_PARTIALS];
+    __shared__ int am_last;
+    near_body(V, D, blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6), gridDim.x * (BLOCK / 64));
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) am_last = (atomicAdd(V->ticket, 1) == (int)gridDim.x - 1) ? 1 : 0;
+    __syncthreads();
+    if (!am_last) return;
+    __threadfence();
+    mid_body(V, D, la, lb, k, last);
+    if (threadIdx.x == 0) *V->ticket = 0;
 }
 
 // ── k_force ──────────────────────────────────────────────────────────────────
 
-// Ice-ice contact force on element e (valid group-uniform), summed in the
-// reference's scatter order.  Result valid in the group's lane 0.
+struct ForceLds {
+    int cand[EPB][MAXC2];
+    int sorted[EPB][MAXC2];
+    int keys[EPB][MAXC2];
+    float cfx[EPB][MAXC2], cfy[EPB][MAXC2], cfm[EPB][MAXC2];
+    int gcell[EPB][GRP];
+    int goff[EPB][GRP + 1];
+    float bf[EPB][3][GRP];
+};
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// group-wide (16 lanes) inclusive prefix sum
+__device__ __forceinline__ int group_incl_scan(int v) {
+    const int l = threadIdx.x & (GRP - 1);
+    #pragma unroll
+    for (int off = 1; off < GRP; off <<= 1) {
+        int o = __shfl_up(v, off, GRP);
+        if (l >= off) v += o;
+    }
+    return v;
+}
+
+// Ice-ice contact force on element e (group-uniform), summed in the
+// reference's scatter order; result valid in the group's lane 0.
 __device__ void contact_force_group(const DevView* __restrict__ V, const float* __restrict__ pos,
                                     const float* __restrict__ vel, int e, float px, float py,
-                                    int* __restrict__ cand, int* __restrict__ sorted, int* __restrict__ keys,
-                                    float* __restrict__ cfx, float* __restrict__ cfy, float* __restrict__ cfm,
-                                    float& fx, float& fy, float& fm) {
+                                    ForceLds& L, int g, float& fx, float& fy, float& fm) {
     const IceDemParams& P = V->p;
     const int n = V->n;
+    const int cap = P.grid_bucket;
     const int l = threadIdx.x & (GRP - 1);
     const int gshift = (threadIdx.x & 63) & ~(GRP - 1);
+    int* cand = L.cand[g];
     int x0, x1, y0, y1;
     scan_range(V, e, px, py, x0, x1, y0, y1);
+    const int W = x1 - x0 + 1;
+    const int ncell = W * (y1 - y0 + 1);
     int cnt = 0;
-    for (int gy = y0; gy <= y1; ++gy) {
-        for (int gx = x0; gx <= x1; ++gx) {
-            const int c = gy * P.grid_nx + gx;
-            const int s0 = V->cell_start[c], s1 = V->cell_start[c + 1];
-            for (int base = s0; base < s1; base += GRP) {
-                const int s = base + l;
-                bool ov = false;
-                int j = -1;
-                if (s < s1) {
-                    j = V->cell_list[s];
-                    if (j != e) ov = pair_overlap(P, pos, V->radius, n, min(e, j), max(e, j));
-                }
-                const uint64_t bal = __ballot(ov);
-                const uint32_t m = (uint32_t)(bal >> gshift) & ((1u << GRP) - 1u);
-                const int before = __builtin_popcount(m & ((1u << l) - 1u));
-                if (ov && cnt + before < MAXC2) cand[cnt + before] = j;
-                cnt += __builtin_popcount(m);
-            }
+    auto test = [&](bool have, int j) {
+        bool ov = false;
+        if (have && j != e) ov = pair_overlap(P, pos, V->radius, n, min(e, j), max(e, j));
+        const uint64_t bal = __ballot(ov);
+        const uint32_t m = (uint32_t)(bal >> gshift) & ((1u << GRP) - 1u);
+        const int before = __builtin_popcount(m & ((1u << l) - 1u));
+        if (ov && cnt + before < MAXC2) cand[cnt + before] = j;
+        cnt += __builtin_popcount(m);
+    };
+    for (int c0 = 0; c0 < ncell; c0 += GRP) {
+        const int q = c0 + l;
+        int cell = 0, cc = 0;
+        if (q < ncell) {
+            cell = (y0 + q / W) * P.grid_nx + (x0 + q % W);
+            cc = min(V->cell_count[cell], cap);
         }
+        const int incl = group_incl_scan(cc);
+        const int total = __shfl(incl, GRP - 1, GRP);
+        L.gcell[g][l] = cell;
+        L.goff[g][l] = incl - cc;
+        wave_sync_lds();
+        for (int base = 0; base < total; base += GRP) {
+            const int f = base + l;
+            int j = -1;
+            if (f < total) {
+                int qq = 0;
+                #pragma unroll
+                for (int t = 1; t < GRP; ++t) qq += (L.goff[g][t] <= f && t < ncell - c0) ? 1 : 0;
+                j = V->cell_slots[(size_t)L.gcell[g][qq] * cap + (f - L.goff[g][qq])];
+            }
+            test(f < total, j);
+        }
+        wave_sync_lds();
+    }
+    const int novf = *V->ovf_count;
+    for (int base = 0; base < novf; base += GRP) {
+        const int f = base + l;
+        const int j = (f < novf) ? V->ovf_list[f] : -1;
+        test(f < novf, j);
     }
     fx = 0.0f; fy = 0.0f; fm = 0.0f;
     if (cnt <= MAXC2) {
@@ -392,46 +476,41 @@ __device__ void contact_force_group(const DevView* __restrict__ V, const float* 
             const int ci = cand[i];
             bool dup = false;
             for (int q = 0; q < i; ++q) dup = dup || (cand[q] == ci);
-            keys[i] = dup ? 0x7fffffff : rot_key(ci, e, n);
+            L.keys[g][i] = dup ? 0x7fffffff : rot_key(ci, e, n);
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync_lds();
         int uniq = 0;
-        for (int q = 0; q < cnt; ++q) uniq += (keys[q] != 0x7fffffff) ? 1 : 0;
+        for (int q = 0; q < cnt; ++q) uniq += (L.keys[g][q] != 0x7fffffff) ? 1 : 0;
         for (int i = l; i < cnt; i += GRP) {
-            const int ki = keys[i];
+            const int ki = L.keys[g][i];
             if (ki == 0x7fffffff) continue;
             int rank = 0;
-            for (int q = 0; q < cnt; ++q) rank += (keys[q] < ki) ? 1 : 0;
-            sorted[rank] = cand[i];
+            for (int q = 0; q < cnt; ++q) rank += (L.keys[g][q] < ki) ? 1 : 0;
+            L.sorted[g][rank] = cand[i];
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync_lds();
         for (int i = l; i < uniq; i += GRP) {
-            const int j = sorted[i];
+            const int j = L.sorted[g][i];
             float a, b, c;
             pair_contrib(P, pos, vel, V->radius, n, min(e, j), max(e, j), e < j, a, b, c);
-            cfx[i] = a; cfy[i] = b; cfm[i] = c;
+            L.cfx[g][i] = a; L.cfy[g][i] = b; L.cfm[g][i] = c;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync_lds();
         if (l == 0)
-            for (int i = 0; i < uniq; ++i) { fx = fx + cfx[i]; fy = fy + cfy[i]; fm = fm + cfm[i]; }
+            for (int i = 0; i < uniq; ++i) { fx = fx + L.cfx[g][i]; fy = fy + L.cfy[g][i]; fm = fm + L.cfm[g][i]; }
+        wave_sync_lds();
         return;
     }
-    // more than MAXC2 candidate hits: exact selection of the next partner
-    if (l != 0) return;
-    int last = 0;
-    for (;;) {
-        int best = 0x7fffffff, bj = -1;
-        for (int gy = y0; gy <= y1; ++gy)
-            for (int gx = x0; gx <= x1; ++gx) {
-                const int c = gy * P.grid_nx + gx;
-                for (int s = V->cell_start[c]; s < V->cell_start[c + 1]; ++s) {
-                    const int j = V->cell_list[s];
+    // more than MAXC2 candidate hits: exact selection of the next partner by lane 0
+    if (l == 0) {
+        int last = 0;
+        for (;;) {
+            int best = 0x7fffffff, bj = -1;
+            for (int q = 0; q < ncell; ++q) {
+                const int cell = (y0 + q / W) * P.grid_nx + (x0 + q % W);
+                const int cc = min(V->cell_count[cell], cap);
+                for (int s = 0; s < cc; ++s) {
+                    const int j = V->cell_slots[(size_t)cell * cap + s];
                     if (j == e) continue;
                     const int kj = rot_key(j, e, n);
                     if (kj <= last || kj >= best) continue;
@@ -439,29 +518,93 @@ __device__ void contact_force_group(const DevView* __restrict__ V, const float* 
                     best = kj; bj = j;
                 }
             }
-        if (bj < 0) break;
-        last = best;
-        float a, b, c;
-        pair_contrib(P, pos, vel, V->radius, n, min(e, bj), max(e, bj), e < bj, a, b, c);
-        fx = fx + a; fy = fy + b; fm = fm + c;
+            for (int s = 0; s < novf; ++s) {
+                const int j = V->ovf_list[s];
+                if (j == e) continue;
+                const int kj = rot_key(j, e, n);
+                if (kj <= last || kj >= best) continue;
+                if (!pair_overlap(P, pos, V->radius, n, min(e, j), max(e, j))) continue;
+                best = kj; bj = j;
+            }
+            if (bj < 0) break;
+            last = best;
+            float a, b, c;
+            pair_contrib(P, pos, vel, V->radius, n, min(e, bj), max(e, bj), e < bj, a, b, c);
+            fx = fx + a; fy = fy + b; fm = fm + c;
+        }
     }
 }
 
-// Sub-step: forces + integration (jax_simulation.py:363-395, 63-268) and, on
-// sub-step 0, the per-element pre-pass tail (:481-509).
-// mode 0: integrate into the other buffer; mode 1: write forces/moments.
-__global__ __launch_bounds__(BLOCK) void k_force(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                                 int k, int sub, int src_is_state, int mode,
-                                                 const float* __restrict__ fship_in,
-                                                 float* __restrict__ fout, float* __restrict__ mout) {
-    __shared__ int cand[EPB][MAXC2];
-    __shared__ int sorted[EPB][MAXC2];
-    __shared__ int keys[EPB][MAXC2];
-    __shared__ float cfx[EPB][MAXC2], cfy[EPB][MAXC2], cfm[EPB][MAXC2];
+// Bond forces on element e (jax_simulation.py:63-108): lanes compute the terms
+// of the element's CSR entries, lane 0 adds them in CSR order (the order of
+// the reference's two scatter-add passes).  Result valid in lane 0.
+__device__ void bond_force_group(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                 const float* __restrict__ pos, int e, int mode, ForceLds& L, int g,
+                                 float& fbx, float& fby, float& mb, bool& crack) {
+    const IceDemParams& P = V->p;
+    const int n = V->n;
+    const int l = threadIdx.x & (GRP - 1);
+    const int gshift = (threadIdx.x & 63) & ~(GRP - 1);
+    const uint8_t* active = D->s.d_active;
+    fbx = 0.0f; fby = 0.0f; mb = 0.0f;
+    crack = false;
+    const int c0 = V->csr_start[e], c1 = V->csr_start[e + 1];
+    for (int base = c0; base < c1; base += GRP) {
+        const int q = base + l;
+        bool broken = false;
+        if (q < c1) {
+            const int ent = V->csr_ent[q];
+            const int b = ent >> 1;
+            const bool role_j = (ent & 1) != 0;
+            const int i = V->bi[b], j = V->bj[b];
+            const bool intact = D->s.d_bond_intact[b] != 0;
+            broken = !intact;
+            float dx = pos[j] - pos[i];
+            float dy = pos[n + j] - pos[n + i];
+            float dist = sqrtf(dx * dx + dy * dy);
+            float dsafe = vmax(dist, 1e-10f);
+            float dirx = dx / dsafe, diry = dy / dsafe;
+            float avg_h = (V->thick[i] + V->thick[j]) * 0.5f;
+            float stiff = P.youngs_modulus * avg_h * P.bond_stiffness_factor;
+            float stretch = dist - V->rest[b];
+            float Fm = stiff * stretch * (1.0f - D->s.d_bond_damage[b]);
+            float Fx = dirx * Fm, Fy = diry * Fm;
+            if (!intact) { Fx = 0.0f; Fy = 0.0f; }
+            bool ai, aj;
+            if (mode == 1) { ai = active[i] != 0; aj = active[j] != 0; }
+            else {
+                ai = (active[i] != 0) || (V->floe_coll[V->floe[i]] != 0u);
+                aj = (active[j] != 0) || (V->floe_coll[V->floe[j]] != 0u);
+            }
+            float ok = ((ai && aj) || (!ai && !aj)) ? 1.0f : 0.0f;
+            Fx = Fx * ok; Fy = Fy * ok;
+            float cross = 0.5f * (dx * Fy - dy * Fx);
+            L.bf[g][0][l] = role_j ? -Fx : Fx;
+            L.bf[g][1][l] = role_j ? -Fy : Fy;
+            L.bf[g][2][l] = role_j ? -cross : cross;
+        }
+        const uint64_t bal = __ballot(broken);
+        crack = crack || (((uint32_t)(bal >> gshift) & ((1u << GRP) - 1u)) != 0u);
+        wave_sync_lds();
+        if (l == 0) {
+            const int m = min(GRP, c1 - base);
+            for (int t = 0; t < m; ++t) {
+                fbx = fbx + L.bf[g][0][t]; fby = fby + L.bf[g][1][t]; mb = mb + L.bf[g][2][t];
+            }
+        }
+        wave_sync_lds();
+    }
+}
+
+// elements [chunk*16, chunk*16+16), 16 lanes each (no block barrier inside)
+__device__ void force_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                           int k, int sub, int src_is_state, int mode,
+                           const float* __restrict__ fship_in, float* __restrict__ fout, float* __restrict__ mout,
+                           ForceLds& L, int chunk) {
     const IceDemParams& P = V->p;
     const int n = V->n;
     const int g = threadIdx.x / GRP, l = threadIdx.x & (GRP - 1);
-    const int e = blockIdx.x * EPB + g;
+    const int e = chunk * EPB + g;
     if (e >= n) return;                       // group-uniform
     const float* pos = src_is_state ? D->s.d_pos : V->nxt_pos;
     const float* vel = src_is_state ? D->s.d_vel : V->nxt_vel;
@@ -474,7 +617,10 @@ __global__ __launch_bounds__(BLOCK) void k_force(const DevView* __restrict__ V, 
     const uint8_t* active = D->s.d_active;
     const float px = pos[e], py = pos[n + e];
     float fix, fiy, mi;
-    contact_force_group(V, pos, vel, e, px, py, cand[g], sorted[g], keys[g], cfx[g], cfy[g], cfm[g], fix, fiy, mi);
+    contact_force_group(V, pos, vel, e, px, py, L, g, fix, fiy, mi);
+    float fbx, fby, mb;
+    bool crack;
+    bond_force_group(V, D, pos, e, mode, L, g, fbx, fby, mb, crack);
     if (l != 0) return;
     const float h = V->thick[e];
     float shx, shy, subv;
@@ -507,40 +653,6 @@ __global__ __launch_bounds__(BLOCK) void k_force(const DevView* __restrict__ V, 
         act = active[e] != 0;
         shx = V->shipf[e]; shy = V->shipf[n + e];
         subv = D->s.d_sub[e];
-    }
-    // bond forces (jax_simulation.py:63-108) in the reference scatter order
-    float fbx = 0.0f, fby = 0.0f, mb = 0.0f;
-    bool crack = false;
-    const int c0 = V->csr_start[e], c1 = V->csr_start[e + 1];
-    for (int q = c0; q < c1; ++q) {
-        const int ent = V->csr_ent[q];
-        const int b = ent >> 1;
-        const bool role_j = (ent & 1) != 0;
-        const int i = V->bi[b], j = V->bj[b];
-        const bool intact = D->s.d_bond_intact[b] != 0;
-        crack = crack || !intact;
-        float dx = pos[j] - pos[i];
-        float dy = pos[n + j] - pos[n + i];
-        float dist = sqrtf(dx * dx + dy * dy);
-        float dsafe = vmax(dist, 1e-10f);
-        float dirx = dx / dsafe, diry = dy / dsafe;
-        float avg_h = (V->thick[i] + V->thick[j]) * 0.5f;
-        float stiff = P.youngs_modulus * avg_h * P.bond_stiffness_factor;
-        float stretch = dist - V->rest[b];
-        float Fm = stiff * stretch * (1.0f - D->s.d_bond_damage[b]);
-        float Fx = dirx * Fm, Fy = diry * Fm;
-        if (!intact) { Fx = 0.0f; Fy = 0.0f; }
-        bool ai, aj;
-        if (mode == 1) { ai = active[i] != 0; aj = active[j] != 0; }
-        else {
-            ai = (active[i] != 0) || (V->floe_coll[V->floe[i]] != 0u);
-            aj = (active[j] != 0) || (V->floe_coll[V->floe[j]] != 0u);
-        }
-        float ok = ((ai && aj) || (!ai && !aj)) ? 1.0f : 0.0f;
-        Fx = Fx * ok; Fy = Fy * ok;
-        float cross = 0.5f * (dx * Fy - dy * Fx);
-        if (!role_j) { fbx = fbx + Fx; fby = fby + Fy; mb = mb + cross; }
-        else { fbx = fbx + (-Fx); fby = fby + (-Fy); mb = mb + (-cross); }
     }
     if (sub == 0 && mode == 0) V->crack[e] = crack ? 1 : 0;
     // drag (:167-192)
@@ -603,13 +715,21 @@ __global__ __launch_bounds__(BLOCK) void k_force(const DevView* __restrict__ V, 
     drot[e] = nr; domg[e] = nw;
 }
 
+
+__global__ __launch_bounds__(BLOCK) void k_force(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                                 int k, int sub, int src_is_state, int mode,
+                                                 const float* __restrict__ fship_in,
+                                                 float* __restrict__ fout, float* __restrict__ mout) {
+    __shared__ ForceLds L;
+    force_body(V, D, k, sub, src_is_state, mode, fship_in, fout, mout, L, blockIdx.x);
+}
+
 // ── k_bond: fracture (jax_simulation.py:534-656), commit, histories, clears ─
 
-__global__ __launch_bounds__(BLOCK) void k_bond(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                                int k, int final_in_nxt, int last) {
+__device__ __forceinline__ void bond_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                          int k, int final_in_nxt, int last, int t) {
     const IceDemParams& P = V->p;
     const int n = V->n, nb = V->nb;
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
     const float* pos = final_in_nxt ? V->nxt_pos : D->s.d_pos;
     const float* vel = final_in_nxt ? V->nxt_vel : D->s.d_vel;
     if (t < nb) {
@@ -685,48 +805,41 @@ __global__ __launch_bounds__(BLOCK) void k_bond(const DevView* __restrict__ V, c
             if (D->dg.d_force_mag) D->dg.d_force_mag[t] = V->fm[t];
         }
     }
-    if (t < V->G) { V->cell_count[t] = 0; V->cell_fill[t] = 0; }
+    if (t < V->G) V->cell_count[t] = 0;
     if (t < V->nf) { V->floe_coll[t] = 0u; V->floe_maxfm[t] = 0u; }
-    if (t == 0) *V->near_count = 0;
+    if (t == 0) { *V->near_count = 0; *V->ovf_count = 0; }
 }
 
-// ── k_ship: ship step (jax_simulation.py:658-697) ────────────────────────────
-
-__device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                           int k, int last) {
-    const IceDemParams& P = V->p;
-    float Fx = -V->scal[2], Fy = -V->scal[3];
-    float tau[3] = {Fx, Fy, V->scal[1]};
-    float* ship = D->s.d_ship;
-    const float* sp = D->in.d_setpoints + (size_t)k * 6;
-    float spc[3] = {sp[0], sp[1], sp[2]};
-    float spn[3] = {sp[3], sp[4], sp[5]};
-    if (P.ice_brake_threshold > 0.0f) {
-        float thr = P.ice_brake_threshold;
-        float surge = fabsf(tau[0]);
-        float bf = vclip(thr / vmax(surge, thr), 0.1f, 1.0f);
-        spc[0] = ship[0] + bf * (sp[0] - ship[0]);
-        spc[1] = ship[1] + bf * (sp[1] - ship[1]);
-    }
-    float vwx = 0.0f, vwy = 0.0f;
-    if (D->in.d_v_water_ship) { vwx = D->in.d_v_water_ship[2 * k]; vwy = D->in.d_v_water_ship[2 * k + 1]; }
-    float st[ICEDEM_SHIP_STATE_LEN];
-    for (int i = 0; i < ICEDEM_SHIP_STATE_LEN; ++i) st[i] = ship[i];
-    ship_step(P, st, spc, spn, tau, vwx, vwy);
-    for (int i = 0; i < ICEDEM_SHIP_STATE_LEN; ++i) ship[i] = st[i];
-    float J = sqrtf(Fx * Fx + Fy * Fy) * P.dt;
-    if (D->h.d_eta) { D->h.d_eta[3 * k] = st[0]; D->h.d_eta[3 * k + 1] = st[1]; D->h.d_eta[3 * k + 2] = st[2]; }
-    if (D->h.d_J) D->h.d_J[k] = J;
-    if (D->h.d_F) { D->h.d_F[2 * k] = Fx; D->h.d_F[2 * k + 1] = Fy; }
-    if (last) {
-        if (D->dg.d_F_ice_on_ship) { D->dg.d_F_ice_on_ship[0] = Fx; D->dg.d_F_ice_on_ship[1] = Fy; }
-        if (D->dg.d_J) D->dg.d_J[0] = J;
-    }
+__global__ __launch_bounds__(BLOCK) void k_bond(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                                int k, int final_in_nxt, int last) {
+    bond_body(V, D, k, final_in_nxt, last, blockIdx.x * BLOCK + threadIdx.x);
 }
 
-__global__ __launch_bounds__(64) void k_ship(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                             int k, int last) {
-    if (threadIdx.x == 0) ship_frame(V, D, k, last);
+
+// ── batched rollouts: one workgroup simulates one scene for `frames` frames ──
+
+__global__ __launch_bounds__(BLOCK) void k_scene_rollouts(const DevView* __restrict__ views,
+                                                          const CallDesc* __restrict__ descs, int frames) {
+    __shared__ ForceLds L;
+    __shared__ float la[256], lb[256];
+    const DevView* V = views + blockIdx.x;
+    const CallDesc* D = descs + blockIdx.x;
+    const int nblk = V->nblk;
+    const int fch = (V->n + EPB - 1) / EPB;
+    const int post = max(max(V->nb, V->n), max(V->G, V->nf));
+    for (int k = 0; k < frames; ++k) {
+        const int last = (k == frames - 1) ? 1 : 0;
+        for (int c = 0; c < nblk; ++c) cull_body(V, D, c);
+        __syncthreads();
+        near_body(V, D, threadIdx.x >> 6, BLOCK / 64);
+        __syncthreads();
+        mid_body(V, D, la, lb, k, last);
+        for (int c = 0; c < fch; ++c)
+            force_body(V, D, k, 0, 1, 0, nullptr, nullptr, nullptr, L, c);
+        __syncthreads();
+        for (int t = threadIdx.x; t < post; t += BLOCK) bond_body(V, D, k, 1, last, t);
+        __syncthreads();
+    }
 }
 
 }  // namespace icedem

@@ -167,20 +167,26 @@ __global__ __launch_bounds__(1024) void k_excl_scan(int m, const int32_t* in, in
 __device__ int pairs_of(const DevView* __restrict__ V, const float* __restrict__ pos, int e,
                         int32_t* pi, int32_t* pj, int64_t o, int64_t max_pairs) {
     const IceDemParams& P = V->p;
-    const int n = V->n;
+    const int n = V->n, cap = P.grid_bucket;
     int x0, x1, y0, y1;
     scan_range(V, e, pos[e], pos[n + e], x0, x1, y0, y1);
+    const int novf = *V->ovf_count;
     int last = e, c = 0;
     for (;;) {
         int best = 0x7fffffff;
         for (int gy = y0; gy <= y1; ++gy)
             for (int gx = x0; gx <= x1; ++gx) {
                 const int cell = gy * P.grid_nx + gx;
-                for (int s = V->cell_start[cell]; s < V->cell_start[cell + 1]; ++s) {
-                    const int j = V->cell_list[s];
+                const int cc = min(V->cell_count[cell], cap);
+                for (int s = 0; s < cc; ++s) {
+                    const int j = V->cell_slots[(size_t)cell * cap + s];
                     if (j > last && j < best && pair_overlap(P, pos, V->radius, n, e, j)) best = j;
                 }
             }
+        for (int s = 0; s < novf; ++s) {
+            const int j = V->ovf_list[s];
+            if (j > last && j < best && pair_overlap(P, pos, V->radius, n, e, j)) best = j;
+        }
         if (best == 0x7fffffff) break;
         if (pi && o + c < max_pairs) { pi[o + c] = e; pj[o + c] = best; }
         last = best;
@@ -333,8 +339,21 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     H.n = n; H.nb = st->n_bonds; H.nf = st->n_floes; H.maxv = st->max_v; H.S = st->n_hull;
     H.G = prm->grid_nx * prm->grid_ny;
     H.nblk = nblk; H.nbpad = nbpad;
-    H.cell_cap = prm->grid_cell_cap;
-    if (H.cell_cap < n) { set_err(nullptr, "grid_cell_cap < n"); delete ctx; return ICEDEM_E_INVALID; }
+    if (prm->grid_bucket < 1 || (int64_t)H.G * prm->grid_bucket > (1LL << 28)) {
+        set_err(nullptr, "bad grid_bucket"); delete ctx; return ICEDEM_E_INVALID;
+    }
+    {   // bound on cell insertions: small elements once, big ones per covered cell
+        int64_t cap = 0;
+        for (int e = 0; e < n; ++e) {
+            const float r = st->h_radii[e];
+            if (!(r > prm->grid_rsmall)) { ++cap; continue; }
+            const double reach = ((double)r + prm->grid_rsmall + prm->contact_dilation2) * 1.001 + 1e-3;
+            const int64_t w = (int64_t)std::floor(2.0 * reach / prm->grid_cell) + 3;
+            cap += std::min<int64_t>(w * w, (int64_t)H.G);
+        }
+        if (cap > (1LL << 30)) { set_err(nullptr, "cell insertions beyond 2^30"); delete ctx; return ICEDEM_E_LIMIT; }
+        H.ovf_cap = (int32_t)cap + 64;
+    }
     const size_t nv = (size_t)n * st->max_v;
     int rc = 0;
 #define UP(field, type, src, cnt) do { type* _p; rc = dupload(ctx, &_p, (const type*)(src), (cnt)); if (rc) { icedem_destroy(ctx); return rc; } H.field = _p; } while (0)
@@ -369,9 +388,9 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     AL(soi_x, float, n); AL(soi_y, float, n); AL(mterm, float, n); AL(fm, float, n);
     AL(shipf, float, 2 * (size_t)n);
     AL(nearf, uint8_t, n); AL(near_list, int32_t, n); AL(near_count, int32_t, 1); AL(nearblk, uint8_t, nblk);
-    AL(crack, uint8_t, n); AL(key, int32_t, n);
-    AL(cell_count, int32_t, H.G); AL(cell_fill, int32_t, H.G); AL(cell_start, int32_t, H.G + 1);
-    AL(cell_list, int32_t, H.cell_cap);
+    AL(crack, uint8_t, n);
+    AL(cell_count, int32_t, H.G); AL(cell_slots, int32_t, (size_t)H.G * prm->grid_bucket);
+    AL(ovf_list, int32_t, H.ovf_cap); AL(ovf_count, int32_t, 1); AL(ticket, int32_t, 1);
     AL(floe_coll, uint32_t, H.nf); AL(floe_maxfm, uint32_t, H.nf);
     AL(part1, float, 3 * (size_t)nbpad); AL(part2, float, 2 * (size_t)nbpad); AL(scal, float, 4);
     AL(nxt_pos, float, 2 * (size_t)n); AL(nxt_vel, float, 2 * (size_t)n);
@@ -380,7 +399,8 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
 #undef UP
 #undef AL
     HIP_TRY(hipMemset(H.cell_count, 0, sizeof(int32_t) * H.G));
-    HIP_TRY(hipMemset(H.cell_fill, 0, sizeof(int32_t) * H.G));
+    HIP_TRY(hipMemset(H.ovf_count, 0, sizeof(int32_t)));
+    HIP_TRY(hipMemset(H.ticket, 0, sizeof(int32_t)));
     HIP_TRY(hipMemset(H.floe_coll, 0, sizeof(uint32_t) * std::max(H.nf, 1)));
     HIP_TRY(hipMemset(H.floe_maxfm, 0, sizeof(uint32_t) * std::max(H.nf, 1)));
     HIP_TRY(hipMemset(H.part1, 0, sizeof(float) * 3 * nbpad));
@@ -428,9 +448,9 @@ static int put_desc(IceDemCtx* ctx, const CallDesc& cd, hipStream_t stream) {
     return ICEDEM_OK;
 }
 
-// ev (optional, profiling): events bracket k_cull, k_near, k_mid, k_scatter,
-// k_force (all sub-steps with their cell rebuilds), k_bond, k_ship.
-constexpr int NPROF = 7;
+// ev (optional, profiling): events bracket k_cull, k_near (with the totals
+// and the ship step), k_force (all sub-steps with their cell rebuilds), k_bond.
+constexpr int NPROF = 4;
 static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, hipEvent_t* ev = nullptr, float* acc = nullptr) {
     const DevView& H = ctx->hv;
     const DevView* V = ctx->dv;
@@ -448,30 +468,21 @@ static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, hipEvent_t* 
         mark(0);
         hipLaunchKernelGGL(k_cull, dim3(nblk), dim3(BLOCK), 0, stream, V, D);
         mark(1);
-        hipLaunchKernelGGL(k_near, dim3(near_blocks), dim3(BLOCK), 0, stream, V, D);
+        hipLaunchKernelGGL(k_near, dim3(near_blocks), dim3(BLOCK), 0, stream, V, D, k, last);
         mark(2);
-        hipLaunchKernelGGL(k_mid, dim3(1), dim3(1024), 0, stream, V, D);
-        mark(3);
-        hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(BLOCK), 0, stream, V, D, (const float*)nullptr, 1);
-        mark(4);
         for (int s = 0; s < steps; ++s) {
             if (s > 0) {
-                // rebuild the cell lists for the sub-step's positions
+                // rebuild the cell buckets for the sub-step's positions
                 const float* sp = (s % 2 == 1) ? (const float*)H.nxt_pos : (const float*)nullptr;
-                const int fs = (s % 2 == 0) ? 1 : 0;
                 hipLaunchKernelGGL(k_clear_cells, dim3(gcl), dim3(BLOCK), 0, stream, V);
-                hipLaunchKernelGGL(k_cellkey, dim3(nblk), dim3(BLOCK), 0, stream, V, D, sp, fs);
-                hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, stream, V);
-                hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(BLOCK), 0, stream, V, D, sp, fs);
+                hipLaunchKernelGGL(k_cellkey, dim3(nblk), dim3(BLOCK), 0, stream, V, D, sp, (s % 2 == 0) ? 1 : 0);
             }
             hipLaunchKernelGGL(k_force, dim3(fblk), dim3(BLOCK), 0, stream, V, D, k, s, (s % 2 == 0) ? 1 : 0, 0,
                                (const float*)nullptr, (float*)nullptr, (float*)nullptr);
         }
-        mark(5);
+        mark(3);
         hipLaunchKernelGGL(k_bond, dim3(nbpost), dim3(BLOCK), 0, stream, V, D, k, (steps % 2 == 1) ? 1 : 0, last);
-        mark(6);
-        hipLaunchKernelGGL(k_ship, dim3(1), dim3(64), 0, stream, V, D, k, last);
-        mark(7);
+        mark(4);
         if (ev) {
             HIP_TRY(hipEventSynchronize(ev[NPROF]));
             for (int i = 0; i < NPROF; ++i) {
@@ -541,11 +552,118 @@ int icedem_frames_profiled(IceDemCtx* ctx, const IceDemState* state, const IceDe
     if (rc) return rc;
     hipEvent_t ev[NPROF + 1];
     for (int i = 0; i <= NPROF; ++i) HIP_TRY(hipEventCreate(&ev[i]));
-    float acc[NPROF] = {0, 0, 0, 0, 0, 0, 0};
+    float acc[NPROF] = {0, 0, 0, 0};
     rc = launch_frames(ctx, K, stream, ev, acc);
     for (int i = 0; i <= NPROF; ++i) (void)hipEventDestroy(ev[i]);
     for (int i = 0; i < NPROF; ++i) h_kernel_ms[i] = acc[i];
     return rc;
+}
+
+struct IceDemBatch {
+    int device = 0;
+    int n = 0;
+    int max_frames_graph = 0;
+    DevView* dviews = nullptr;   // (n) copies of the contexts' device views
+    CallDesc* ddesc = nullptr;   // (n)
+    CallDesc* hdesc = nullptr;   // pinned staging
+    hipEvent_t desc_done = nullptr;
+    std::map<int, hipGraphExec_t> graphs;
+    std::map<int, hipGraph_t> graph_defs;
+    std::string err;
+};
+
+int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** out) {
+    IceDemCtx* ctx = nullptr;
+    if (!ctxs || n_inst < 1 || !out) { set_err(nullptr, "bad argument"); return ICEDEM_E_INVALID; }
+    *out = nullptr;
+    for (int i = 0; i < n_inst; ++i) {
+        const DevView& H = ctxs[i]->hv;
+        if (H.nbpad > 256 || H.p.n_ice_steps != 1 || ctxs[i]->device != ctxs[0]->device) {
+            set_err(nullptr, "batched scenes need n <= 65536, one ice sub-step and one device");
+            return ICEDEM_E_LIMIT;
+        }
+    }
+    IceDemBatch* b = new IceDemBatch();
+    b->device = ctxs[0]->device;
+    b->n = n_inst;
+    hipError_t e = hipSetDevice(b->device);
+    std::vector<DevView> hv(n_inst);
+    for (int i = 0; i < n_inst; ++i) hv[i] = ctxs[i]->hv;
+    if (e == hipSuccess) e = hipMalloc((void**)&b->dviews, sizeof(DevView) * n_inst);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->ddesc, sizeof(CallDesc) * n_inst);
+    if (e == hipSuccess) e = hipMemcpy(b->dviews, hv.data(), sizeof(DevView) * n_inst, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&b->hdesc, sizeof(CallDesc) * n_inst, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->desc_done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        set_err(nullptr, std::string("icedem_batch_create: ") + hipGetErrorString(e));
+        if (b->dviews) (void)hipFree(b->dviews);
+        if (b->ddesc) (void)hipFree(b->ddesc);
+        if (b->hdesc) (void)hipHostFree(b->hdesc);
+        delete b;
+        return ICEDEM_E_HIP;
+    }
+    memset(b->hdesc, 0, sizeof(CallDesc) * n_inst);
+    *out = b;
+    return ICEDEM_OK;
+}
+
+int icedem_batch_destroy(IceDemBatch* b) {
+    if (!b) return ICEDEM_OK;
+    (void)hipSetDevice(b->device);
+    for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto& g : b->graph_defs) (void)hipGraphDestroy(g.second);
+    if (b->dviews) (void)hipFree(b->dviews);
+    if (b->ddesc) (void)hipFree(b->ddesc);
+    if (b->hdesc) (void)hipHostFree(b->hdesc);
+    if (b->desc_done) (void)hipEventDestroy(b->desc_done);
+    delete b;
+    return ICEDEM_OK;
+}
+
+int icedem_batch_rollouts(IceDemBatch* b, const IceDemState* states, const IceDemInputs* inputs, int32_t frames,
+                          const IceDemDiag* diags, int32_t use_graph, void* stream_) {
+    IceDemCtx* ctx = nullptr;
+    if (!b || !states || !inputs || frames < 0) { set_err(nullptr, "bad argument"); return ICEDEM_E_INVALID; }
+    if (frames == 0) return ICEDEM_OK;
+    hipStream_t stream = (hipStream_t)stream_;
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipEventSynchronize(b->desc_done));
+    for (int i = 0; i < b->n; ++i) {
+        const IceDemState& s = states[i];
+        if (!s.d_pos || !s.d_vel || !s.d_rot || !s.d_omega || !s.d_sub || !s.d_active || !s.d_ship) {
+            set_err(nullptr, "batch state has a null pointer"); return ICEDEM_E_INVALID;
+        }
+        CallDesc& cd = b->hdesc[i];
+        memset(&cd, 0, sizeof(cd));
+        cd.s = s;
+        cd.in = inputs[i];
+        if (diags) cd.dg = diags[i];
+    }
+    HIP_TRY(hipMemcpyAsync(b->ddesc, b->hdesc, sizeof(CallDesc) * b->n, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipEventRecord(b->desc_done, stream));
+    if (!use_graph) {
+        hipLaunchKernelGGL(k_scene_rollouts, dim3(b->n), dim3(BLOCK), 0, stream, b->dviews, b->ddesc, frames);
+        HIP_TRY(hipGetLastError());
+        return ICEDEM_OK;
+    }
+    auto it = b->graphs.find(frames);
+    if (it == b->graphs.end()) {
+        hipStream_t cap;
+        HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        hipLaunchKernelGGL(k_scene_rollouts, dim3(b->n), dim3(BLOCK), 0, cap, b->dviews, b->ddesc, frames);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(cap, &g);
+        (void)hipStreamDestroy(cap);
+        if (e2 != hipSuccess) { set_err(nullptr, std::string("capture: ") + hipGetErrorString(e2)); return ICEDEM_E_HIP; }
+        hipGraphExec_t ge = nullptr;
+        HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        b->graph_defs[frames] = g;
+        b->graphs[frames] = ge;
+        it = b->graphs.find(frames);
+    }
+    HIP_TRY(hipGraphLaunch(it->second, stream));
+    return ICEDEM_OK;
 }
 
 int icedem_sync(IceDemCtx* ctx, void* stream) {
@@ -577,8 +695,6 @@ int icedem_substep_forces(IceDemCtx* ctx, const IceDemState* state, const float*
     const DevView& H = ctx->hv;
     hipLaunchKernelGGL(k_clear_cells, dim3((H.G + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, ctx->dv);
     hipLaunchKernelGGL(k_cellkey, dim3(H.nblk), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, (const float*)state->d_pos, 0);
-    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, stream, ctx->dv);
-    hipLaunchKernelGGL(k_scatter, dim3(H.nblk), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, (const float*)state->d_pos, 0);
     hipLaunchKernelGGL(k_force, dim3((H.n + EPB - 1) / EPB), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, 0, 0, 1, 1,
                        d_ship_forces, d_forces, d_moments);
     hipLaunchKernelGGL(k_clear_cells, dim3((H.G + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, ctx->dv);
@@ -600,8 +716,6 @@ int icedem_contact_pairs(IceDemCtx* ctx, const float* d_pos, int32_t* d_pair_i, 
     HIP_TRY(hipMallocAsync((void**)&off, sizeof(int32_t) * (H.n + 1), stream));
     hipLaunchKernelGGL(k_clear_cells, dim3((H.G + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, ctx->dv);
     hipLaunchKernelGGL(k_cellkey, dim3(H.nblk), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, d_pos, 0);
-    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, stream, ctx->dv);
-    hipLaunchKernelGGL(k_scatter, dim3(H.nblk), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, d_pos, 0);
     hipLaunchKernelGGL(k_pair_count, dim3(H.nblk), dim3(BLOCK), 0, stream, ctx->dv, d_pos, cnt);
     hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, stream, H.n, cnt, off);
     hipLaunchKernelGGL(k_pair_emit, dim3(H.nblk), dim3(BLOCK), 0, stream, ctx->dv, d_pos, off, max_pairs,

@@ -8,13 +8,12 @@
 namespace icedem {
 
 constexpr int BLOCK = 256;          // threads per block of the per-element kernels
-constexpr int MAXC = 24;            // contacts kept per element in LDS before the slow path
 constexpr int MAX_PARTIALS = 4096;  // block partials reduced in one workgroup (n <= 1M)
 
 // Static data, per-frame scratch and parameters of one context (device memory).
 struct DevView {
     IceDemParams p;
-    int32_t n, nb, nf, maxv, S, G, nblk, nbpad, cell_cap;
+    int32_t n, nb, nf, maxv, S, G, nblk, nbpad, ovf_cap;
     // static
     const float2* polys;      // (n*maxv) body-frame vertices
     const uint8_t* mask;      // (n*maxv)
@@ -34,9 +33,11 @@ struct DevView {
     uint8_t* nearblk;         // (nblk) block of 256 elements holds a near element
     float* shipf;             // (2n) ship forces on ice, kept for sub-steps > 0
     uint8_t* crack;           // (n) element touches a broken bond
-    int32_t* key;             // (n) cell of each element
-    int32_t* cell_count; int32_t* cell_fill; int32_t* cell_start;   // (G), (G), (G+1)
-    int32_t* cell_list;       // (cell_cap): small elements once, big ones per covered cell
+    int32_t* cell_count;      // (G) elements inserted per cell this frame
+    int32_t* cell_slots;      // (G * grid_bucket) bucket contents
+    int32_t* ovf_list;        // (ovf_cap) insertions that found their bucket full
+    int32_t* ovf_count;       // (1)
+    int32_t* ticket;          // (1) workgroups of k_near that have finished
     uint32_t* floe_coll; uint32_t* floe_maxfm;                      // (nf)
     float* part1;             // (3 * nbpad) block partials: sum soi_x, soi_y, moment
     float* part2;             // (2 * nbpad) block partials: sum of scaled soi

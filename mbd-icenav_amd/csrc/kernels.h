@@ -388,9 +388,13 @@ __device__ __forceinline__ float jremf(float x, float y) {
     return r;
 }
 
-// ship_step_jax (jax_vessel_dynamics.py:246-348); ship = eta[3] nu[3] n[6] e_int[3]
+// ship_step_jax (jax_vessel_dynamics.py:246-348); ship = eta[3] nu[3] n[6] e_int[3].
+// ship_mode 1 (rollouts; no reference counterpart): the thrust is given
+// directly, tau = [u X_max, 0, Kd[2,2] (r_cmd r_max - r)] with u, r_cmd in
+// [-1, 1]; the Euler step and the velocity clip are the reference's.
 __device__ __forceinline__ void ship_step(const IceDemParams& P, float* ship, const float* sp, const float* spn,
-                                          const float* tau_ice, float vwx, float vwy) {
+                                          const float* tau_ice, float vwx, float vwy,
+                                          float ctrl_u = 0.0f, float ctrl_r = 0.0f) {
     const float PI_F = (float)3.141592653589793;        // float32(pi)
     const float TWO_PI_F = (float)6.283185307179586;    // float32(2 pi)
     const float dt = P.dt;
@@ -407,44 +411,54 @@ __device__ __forceinline__ void ship_step(const IceDemParams& P, float* ship, co
     float nu_c[3], tib[3];
     mv3(RT, vw3, nu_c);
     mv3(RT, tau_ice, tib);
-    float dpsi = jremf(spn[2] - sp[2] + PI_F, TWO_PI_F) - PI_F;
-    float vd[3] = {(spn[0] - sp[0]) / dt, (spn[1] - sp[1]) / dt, dpsi / dt};
-    float nud[3], tff[3];
-    mv3(RT, vd, nud);
-    mv3(P.D3pKd, nud, tff);
-    // dp_control_jax (:127-184)
-    float e_psi = jremf(eta[2] - sp[2] + PI_F, TWO_PI_F) - PI_F;
-    float e[3] = {eta[0] - sp[0], eta[1] - sp[1], e_psi};
-    float RKp[9], RKi[9], a[3], b[3], cc[3];
-    mm3(RT, P.Kp, RKp);
-    mv3(RKp, e, a);
-    mv3(P.Kd, nu, b);
-    mm3(RT, P.Ki, RKi);
-    mv3(RKi, e_int, cc);
-    float tau_c[3], e_new[3];
-    for (int i = 0; i < 3; ++i) {
-        tau_c[i] = ((-a[i]) - b[i]) - cc[i];
-        e_new[i] = e_int[i] + dt * e[i];
-    }
     float tau_act[3];
     float nnew[6];
-    if (P.use_auto_icenav) {
-        float t3[3] = {tau_c[0] + tff[0], tau_c[1] + tff[1], tau_c[2] + tff[2]};
-        for (int i = 0; i < 6; ++i) {
-            float u = 0.0f;
-            for (int k = 0; k < 3; ++k) u = u + P.B_pinv[3 * i + k] * t3[k];
-            float nc = vsign(u) * sqrtf(fabsf(u) + 1e-12f);
-            nc = vclip(nc, -P.n_max[i], P.n_max[i]);
-            nnew[i] = nact[i] + dt * (nc - nact[i]) / P.T_n;
-        }
-        for (int i = 0; i < 3; ++i) {
-            float acc = 0.0f;
-            for (int k = 0; k < 6; ++k) acc = acc + P.B[6 * i + k] * (fabsf(nnew[k]) * nnew[k]);
-            tau_act[i] = acc;
-        }
-    } else {
-        for (int i = 0; i < 3; ++i) tau_act[i] = tau_c[i] + tff[i];
+    float e_new[3];
+    if (P.ship_mode == 1) {
+        const float u = vclip(ctrl_u, -1.0f, 1.0f), rc = vclip(ctrl_r, -1.0f, 1.0f);
+        tau_act[0] = u * P.ctrl_thrust_max;
+        tau_act[1] = 0.0f;
+        tau_act[2] = P.Kd[8] * (rc * P.ctrl_yaw_rate_max - nu[2]);
         for (int i = 0; i < 6; ++i) nnew[i] = nact[i];
+        for (int i = 0; i < 3; ++i) e_new[i] = e_int[i];
+    } else {
+        float dpsi = jremf(spn[2] - sp[2] + PI_F, TWO_PI_F) - PI_F;
+        float vd[3] = {(spn[0] - sp[0]) / dt, (spn[1] - sp[1]) / dt, dpsi / dt};
+        float nud[3], tff[3];
+        mv3(RT, vd, nud);
+        mv3(P.D3pKd, nud, tff);
+        // dp_control_jax (:127-184)
+        float e_psi = jremf(eta[2] - sp[2] + PI_F, TWO_PI_F) - PI_F;
+        float e[3] = {eta[0] - sp[0], eta[1] - sp[1], e_psi};
+        float RKp[9], RKi[9], a[3], b[3], cc[3];
+        mm3(RT, P.Kp, RKp);
+        mv3(RKp, e, a);
+        mv3(P.Kd, nu, b);
+        mm3(RT, P.Ki, RKi);
+        mv3(RKi, e_int, cc);
+        float tau_c[3];
+        for (int i = 0; i < 3; ++i) {
+            tau_c[i] = ((-a[i]) - b[i]) - cc[i];
+            e_new[i] = e_int[i] + dt * e[i];
+        }
+        if (P.use_auto_icenav) {
+            float t3[3] = {tau_c[0] + tff[0], tau_c[1] + tff[1], tau_c[2] + tff[2]};
+            for (int i = 0; i < 6; ++i) {
+                float u = 0.0f;
+                for (int k = 0; k < 3; ++k) u = u + P.B_pinv[3 * i + k] * t3[k];
+                float nc = vsign(u) * sqrtf(fabsf(u) + 1e-12f);
+                nc = vclip(nc, -P.n_max[i], P.n_max[i]);
+                nnew[i] = nact[i] + dt * (nc - nact[i]) / P.T_n;
+            }
+            for (int i = 0; i < 3; ++i) {
+                float acc = 0.0f;
+                for (int k = 0; k < 6; ++k) acc = acc + P.B[6 * i + k] * (fabsf(nnew[k]) * nnew[k]);
+                tau_act[i] = acc;
+            }
+        } else {
+            for (int i = 0; i < 3; ++i) tau_act[i] = tau_c[i] + tff[i];
+            for (int i = 0; i < 6; ++i) nnew[i] = nact[i];
+        }
     }
     float tot[3] = {tau_act[0] + tib[0], tau_act[1] + tib[1], tau_act[2] + tib[2]};
     // ship_euler_step_jax (:54-84)

@@ -54,12 +54,13 @@ class IceDemParams(C.Structure):
         ("max_fracture_thickness", _f), ("k_n", _f), ("k_s", _f),
         ("bond_cohesion", _f), ("bond_friction", _f), ("G_c_I", _f), ("G_c_II", _f), ("eta_bk", _f),
         ("use_auto_icenav", _i),
+        ("ship_mode", _i), ("ctrl_thrust_max", _f), ("ctrl_yaw_rate_max", _f), ("ship_mass", _f), ("ship_Izz", _f),
         ("ice_brake_threshold", _f),
         ("M3inv", _f * 9), ("D3", _f * 9), ("Kp", _f * 9), ("Kd", _f * 9), ("Ki", _f * 9), ("D3pKd", _f * 9),
         ("B_pinv", _f * 18), ("B", _f * 18), ("n_max", _f * 6),
         ("T_n", _f), ("vc_surge", _f), ("vc_sway", _f), ("vc_yaw", _f),
         ("grid_x0", _f), ("grid_y0", _f), ("grid_cell", _f), ("grid_rsmall", _f),
-        ("grid_nx", _i), ("grid_ny", _i), ("grid_cell_cap", _i),
+        ("grid_nx", _i), ("grid_ny", _i), ("grid_bucket", _i),
     ]
 
 
@@ -72,7 +73,7 @@ class IceDemState(C.Structure):
 class IceDemInputs(C.Structure):
     _fields_ = [("d_setpoints", _p), ("d_v_water", _p),
                 ("vw_stride_k", C.c_int64), ("vw_stride_c", C.c_int64), ("vw_stride_e", C.c_int64),
-                ("d_v_water_ship", _p)]
+                ("d_v_water_ship", _p), ("d_controls", _p), ("n_knots", _i), ("hold", _i)]
 
 
 class IceDemHistory(C.Structure):
@@ -81,7 +82,7 @@ class IceDemHistory(C.Structure):
 
 
 class IceDemDiag(C.Structure):
-    _fields_ = [(name, _p) for name in ("d_pen_max", "d_force_mag", "d_F_ice_on_ship", "d_J")]
+    _fields_ = [(name, _p) for name in ("d_pen_max", "d_force_mag", "d_F_ice_on_ship", "d_J", "d_rollout")]
 
 
 # name -> (restype, argtypes)
@@ -95,6 +96,10 @@ SIGNATURES = {
                                 C.POINTER(IceDemHistory), C.POINTER(IceDemDiag), _i, _p]),
     "icedem_frames_profiled": (C.c_int, [_p, C.POINTER(IceDemState), C.POINTER(IceDemInputs), _i,
                                          C.POINTER(IceDemHistory), C.POINTER(C.c_float), _p]),
+    "icedem_batch_create": (C.c_int, [C.POINTER(_p), _i, C.POINTER(_p)]),
+    "icedem_batch_rollouts": (C.c_int, [_p, C.POINTER(IceDemState), C.POINTER(IceDemInputs), _i,
+                                        C.POINTER(IceDemDiag), _i, _p]),
+    "icedem_batch_destroy": (C.c_int, [_p]),
     "icedem_sync": (C.c_int, [_p, _p]),
     "icedem_substep_forces": (C.c_int, [_p, C.POINTER(IceDemState), _p, _p, C.c_int64, C.c_int64, _p, _p, _p]),
     "icedem_contact_pairs": (C.c_int, [_p, _p, _p, _p, C.c_int64, C.POINTER(C.c_int64), _p]),

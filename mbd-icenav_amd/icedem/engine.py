@@ -49,7 +49,7 @@ def choose_grid(centroids: np.ndarray, radii: np.ndarray, dilation: float = 0.0,
     larger ones are inserted into, and scan, every cell within
     r + r_s + 2 delta.  Either way every contacting pair is found (exactly the
     reference's contact set); positions outside the grid are clamped to the
-    border cells.  Returns (x0, y0, cell, r_s, nx, ny, cell_list_capacity).
+    border cells.  Returns (x0, y0, cell, r_s, nx, ny, bucket_capacity).
     """
     r = np.asarray(radii, dtype=np.float64)
     r32 = r.astype(F32)
@@ -68,15 +68,30 @@ def choose_grid(centroids: np.ndarray, radii: np.ndarray, dilation: float = 0.0,
         if nx * ny <= max_cells:
             break
         cell *= 1.5
+    # bucket capacity from the initial occupancy (2x headroom); a full bucket
+    # spills into the overflow list, so this is a speed choice only
     big = r32 > F32(r_s)
-    reach = (r32[big].astype(np.float64) + r_s + 2.0 * dilation) * 1.001 + 1e-3
-    per_big = (np.floor(2.0 * reach / cell) + 3) ** 2
-    cap = int(np.sum(~big) + np.sum(np.minimum(per_big, nx * ny))) + 64
-    return float(lo[0]), float(lo[1]), float(cell), r_s, nx, ny, cap
+    counts = np.zeros(nx * ny, dtype=np.int64)
+    if len(c):
+        cx = np.clip(np.floor((c[:, 0] - lo[0]) / cell), 0, nx - 1).astype(np.int64)
+        cy = np.clip(np.floor((c[:, 1] - lo[1]) / cell), 0, ny - 1).astype(np.int64)
+        np.add.at(counts, (cy * nx + cx)[~big], 1)
+        for e in np.nonzero(big)[0]:
+            R = (float(r32[e]) + r_s + 2.0 * dilation) * 1.001 + 1e-3
+            x0 = int(np.clip(np.floor((c[e, 0] - R - lo[0]) / cell), 0, nx - 1))
+            x1 = int(np.clip(np.floor((c[e, 0] + R - lo[0]) / cell), 0, nx - 1))
+            y0 = int(np.clip(np.floor((c[e, 1] - R - lo[1]) / cell), 0, ny - 1))
+            y1 = int(np.clip(np.floor((c[e, 1] + R - lo[1]) / cell), 0, ny - 1))
+            for gy in range(y0, y1 + 1):
+                counts[gy * nx + x0: gy * nx + x1 + 1] += 1
+    bucket = int(max(16, 2 * int(counts.max(initial=0)) + 8))
+    bucket = (bucket + 3) // 4 * 4
+    return float(lo[0]), float(lo[1]), float(cell), r_s, nx, ny, bucket
 
 
 def make_params(static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_length, ship_beam, n_ice_steps,
-                dt, dt_ice, use_auto_icenav, B_mat, T_n, grid=None) -> _lib.IceDemParams:
+                dt, dt_ice, use_auto_icenav, B_mat, T_n, grid=None, ship_mode: int = 0,
+                ctrl_thrust_max: float = 2e6, ctrl_yaw_rate_max: float = 0.15) -> _lib.IceDemParams:
     """Scalar parameters rounded to float32 in the reference's order of use."""
     p = _lib.IceDemParams()
     hl = ship_length / 2.0
@@ -141,6 +156,11 @@ def make_params(static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_len
     p.G_c_II = F32(cfg.bond_G_c_II)
     p.eta_bk = F32(cfg.bond_eta_bk)
     p.use_auto_icenav = 1 if use_auto_icenav else 0
+    p.ship_mode = int(ship_mode)
+    p.ctrl_thrust_max = F32(ctrl_thrust_max)
+    p.ctrl_yaw_rate_max = F32(ctrl_yaw_rate_max)
+    p.ship_mass = F32(ship_dyn.mass)
+    p.ship_Izz = F32(ship_dyn.Izz)
     p.ice_brake_threshold = F32(float(getattr(cfg, "ice_brake_threshold", 0.0)))
     m32 = lambda a: np.asarray(a, dtype=F32)
     for name, arr in (("M3inv", ship_dyn.M3inv), ("D3", ship_dyn.D3), ("Kp", ship_dyn.Kp), ("Kd", ship_dyn.Kd),
@@ -156,7 +176,7 @@ def make_params(static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_len
     if grid is None:
         raise ValueError("grid (x0, y0, cell, r_s, nx, ny, cap) is required; see choose_grid")
     p.grid_x0, p.grid_y0, p.grid_cell, p.grid_rsmall = F32(grid[0]), F32(grid[1]), F32(grid[2]), F32(grid[3])
-    p.grid_nx, p.grid_ny, p.grid_cell_cap = int(grid[4]), int(grid[5]), int(grid[6])
+    p.grid_nx, p.grid_ny, p.grid_bucket = int(grid[4]), int(grid[5]), int(grid[6])
     return p
 
 
@@ -174,7 +194,8 @@ class DemEngine:
 
     def __init__(self, static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_length, ship_beam,
                  n_ice_steps, dt, dt_ice, use_auto_icenav, B_mat, T_n, centroids=None, device=None, grid=None,
-                 cell_size=None):
+                 cell_size=None, ship_mode: int = 0, ctrl_thrust_max: float = 2e6, ctrl_yaw_rate_max: float = 0.15,
+                 grid_margin: float = 60.0, small_quantile: float = 0.98):
         if not torch.cuda.is_available():
             raise _lib.IceDemError("icedem needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.lib = _lib.load()
@@ -188,10 +209,12 @@ class DemEngine:
         self.dt = float(dt)
         if grid is None:
             cen = np.asarray(centroids, dtype=np.float64) if centroids is not None else np.zeros((self.n, 2))
-            grid = choose_grid(cen, static.radii, float(getattr(cfg, "contact_dilation", 0.0)), cell=cell_size)
+            grid = choose_grid(cen, static.radii, float(getattr(cfg, "contact_dilation", 0.0)), cell=cell_size,
+                               margin=grid_margin, small_quantile=small_quantile)
         self.grid = grid
         self.params = make_params(static, cfg, ship_dyn, ship_length, ship_beam, n_ice_steps, dt, dt_ice,
-                                  use_auto_icenav, B_mat, T_n, grid=grid)
+                                  use_auto_icenav, B_mat, T_n, grid=grid, ship_mode=ship_mode,
+                                  ctrl_thrust_max=ctrl_thrust_max, ctrl_yaw_rate_max=ctrl_yaw_rate_max)
         lp = np.ascontiguousarray(static.local_polys_padded, dtype=F32)
         self._host = dict(
             lp=lp,
@@ -297,7 +320,7 @@ class DemEngine:
     def run_frames(self, st: IceState, ship_buf: torch.Tensor, setpoints: torch.Tensor,
                    v_water: Optional[torch.Tensor] = None, v_water_ship: Optional[torch.Tensor] = None,
                    hist: Optional[dict] = None, diag: Optional[dict] = None, use_graph: bool = True,
-                   stream=None, sync: bool = True):
+                   stream=None, sync: bool = True, controls: Optional[torch.Tensor] = None, hold: int = 1):
         """Run K = len(setpoints) frames in place.  v_water: (K, n, 2), (n, 2)
         (broadcast over frames) or None; any strides."""
         K = int(setpoints.shape[0])
@@ -322,6 +345,12 @@ class DemEngine:
                 inp.vw_stride_k, inp.vw_stride_e, inp.vw_stride_c = vw.stride()
             inp.d_v_water = _ptr(vw)
             keep.append(vw)
+        if controls is not None:
+            ctl = torch.as_tensor(controls).to(device=self.device, dtype=torch.float32).contiguous()
+            inp.d_controls = _ptr(ctl)
+            inp.n_knots = int(ctl.shape[0])
+            inp.hold = int(hold)
+            keep.append(ctl)
         if v_water_ship is not None:
             vws = torch.as_tensor(v_water_ship).to(device=self.device, dtype=torch.float32).contiguous()
             if tuple(vws.shape) != (K, 2):
@@ -338,7 +367,8 @@ class DemEngine:
         d = _lib.IceDemDiag()
         if diag:
             for key, field in (("pen_max", "d_pen_max"), ("force_mag", "d_force_mag"),
-                               ("F_ice_on_ship", "d_F_ice_on_ship"), ("J_cumul_incr", "d_J")):
+                               ("F_ice_on_ship", "d_F_ice_on_ship"), ("J_cumul_incr", "d_J"),
+                               ("rollout", "d_rollout")):
                 if key in diag and diag[key] is not None:
                     setattr(d, field, _ptr(diag[key]))
         s = self.state_struct(st, ship_buf)

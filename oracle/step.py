@@ -120,7 +120,8 @@ class Oracle:
 
     def __init__(self, static: StaticF64, cfg: OracleConfig, ship_dyn: ShipDynF64,
                  ship_length, ship_beam, n_ice_steps, dt, dt_ice, use_auto_icenav,
-                 B_mat, T_n, dtype=np.float32, pair_mode="all", pair_chunk=1 << 22):
+                 B_mat, T_n, dtype=np.float32, pair_mode="all", pair_chunk=1 << 22,
+                 ship_mode=0, ctrl_thrust_max=2e6, ctrl_yaw_rate_max=0.15):
         self.ft = ft = np.dtype(dtype).type
         self.f32 = ft is np.float32
         self.cfg = cfg
@@ -150,6 +151,12 @@ class Oracle:
         self.pair_mode = pair_mode
         self.pair_chunk = pair_chunk
         self.t_edge = edge_sample_t(cfg.n_edge_samples, ft) if cfg.n_edge_samples > 0 else None
+        # rollout extension (no reference counterpart): direct thrust control
+        self.ship_mode = int(ship_mode)
+        self.ctrl_thrust_max = ctrl_thrust_max
+        self.ctrl_yaw_rate_max = ctrl_yaw_rate_max
+        self.ship_mass = ship_dyn.mass
+        self.ship_Izz = ship_dyn.Izz
 
     # ── transcendentals ──────────────────────────────────────────────────────
     def cos(self, x):
@@ -641,14 +648,24 @@ class Oracle:
         e_int_new = e_int + ft(self.dt) * e
         return tau.astype(ft), e_int_new.astype(ft)
 
-    def ship_step(self, eta, nu, n_actual, e_int, sp, spn, tau_ice, v_water_ship):
-        """ship_step_jax, jax_vessel_dynamics.py:246-348."""
+    def ship_step(self, eta, nu, n_actual, e_int, sp, spn, tau_ice, v_water_ship, control=None):
+        """ship_step_jax, jax_vessel_dynamics.py:246-348.  With ship_mode 1
+        (rollout extension, no reference counterpart) the thrust is
+        tau = [u X_max, 0, Kd[2,2] (r_cmd r_max - r)], u and r_cmd clipped to
+        [-1, 1]; the Euler step and velocity clip are the reference's."""
         ft = self.ft
         dt = ft(self.dt)
         R, c, s = self._R(eta[2])
         RT = R.T.copy()
         nu_c_body = self._mv(RT, np.array([v_water_ship[0], v_water_ship[1], 0.0], dtype=ft))
         tau_ice_body = self._mv(RT, tau_ice)
+        if self.ship_mode == 1:
+            u = vclip(ft(control[0]), ft(-1.0), ft(1.0))
+            rc = vclip(ft(control[1]), ft(-1.0), ft(1.0))
+            tau_act = np.array([u * ft(self.ctrl_thrust_max), 0.0,
+                                self.Kd[2, 2] * (rc * ft(self.ctrl_yaw_rate_max) - nu[2])], dtype=ft)
+            return self._euler_tail(eta, nu, tau_act, tau_ice_body, nu_c_body, c, s,
+                                    np.asarray(n_actual, ft), np.asarray(e_int, ft))
         dpsi = self.remainder(spn[2] - sp[2] + ft(np.pi), ft(2.0 * np.pi)) - ft(np.pi)
         v_d = np.array([(spn[0] - sp[0]) / dt, (spn[1] - sp[1]) / dt, dpsi / dt], dtype=ft)
         nu_d = self._mv(RT, v_d)
@@ -664,6 +681,11 @@ class Oracle:
         else:
             tau_act = tau_c + tau_ff
             n_new = n_actual
+        return self._euler_tail(eta, nu, tau_act, tau_ice_body, nu_c_body, c, s, np.asarray(n_new, ft), e_int_new)
+
+    def _euler_tail(self, eta, nu, tau_act, tau_ice_body, nu_c_body, c, s, n_new, e_int_new):
+        ft = self.ft
+        dt = ft(self.dt)
         tau_total = tau_act + tau_ice_body
         # ship_euler_step_jax (:54-84)
         nu_r = nu - nu_c_body
@@ -681,7 +703,7 @@ class Oracle:
 
     # ── a3/a6/a7/a8/a13: frame_step, jax_simulation.py:397-699 ───────────────
     def frame_step(self, st: IceState, ship: ShipState, setpoint, setpoint_next, v_water,
-                   dpdem_eligible, v_water_ship=None):
+                   dpdem_eligible, v_water_ship=None, control=None):
         ft = self.ft
         cfg = self.cfg
         n = st.positions.shape[0]
@@ -760,7 +782,7 @@ class Oracle:
             st = self._fracture(st, fm, new_active, new_elig)
         st = st._replace(dpdem_floe_eligible=new_elig)
         # ship (:658-689)
-        if cfg.ice_brake_threshold > 0.0:
+        if cfg.ice_brake_threshold > 0.0 and self.ship_mode == 0:
             thr_b = ft(cfg.ice_brake_threshold)
             surge = np.abs(tau_ice[0])
             bf = vclip(thr_b / vmax(surge, thr_b), ft(0.1), ft(1.0))
@@ -769,7 +791,7 @@ class Oracle:
         else:
             eff = setpoint
         eta, nu, n_act, e_int = self.ship_step(ship.eta, ship.nu, ship.n_actual, ship.e_int,
-                                               eff, setpoint_next, tau_ice, v_water_ship)
+                                               eff, setpoint_next, tau_ice, v_water_ship, control)
         ship_new = ShipState(eta, nu, n_act, e_int)
         diag = {
             "pen_max": pen, "force_mag": fm.astype(ft),
@@ -861,3 +883,32 @@ def initial_ice_state(centroids, n_bonds, floe_of, ft=np.float32) -> IceState:
 def initial_ship_state(pos, heading, ft=np.float32) -> ShipState:
     """build_initial_ship_state, jax_ice_state.py:344-355."""
     return ShipState(np.array([pos[0], pos[1], heading], ft), np.zeros(3, ft), np.zeros(6, ft), np.zeros(3, ft))
+
+
+def kinetic_energy(o: Oracle, nu):
+    """Ship kinetic energy 0.5 m (u^2 + v^2) + 0.5 Izz r^2 (rollout extension)."""
+    ft = o.ft
+    return (ft(0.5) * ft(o.ship_mass) * (nu[0] * nu[0] + nu[1] * nu[1])
+            + ft(0.5) * ft(o.ship_Izz) * (nu[2] * nu[2]))
+
+
+def rollout(o: Oracle, st: IceState, ship: ShipState, controls, hold, frames, v_water=None):
+    """Direct-control rollout (extension): returns (state, ship, [impulse,
+    KE loss, peak |F|, initial KE]) with the device's accumulation order."""
+    ft = o.ft
+    n = st.positions.shape[0]
+    vw = np.zeros((n, 2), ft) if v_water is None else np.asarray(v_water, ft)
+    controls = np.asarray(controls, ft)
+    ke0 = kinetic_energy(o, ship.nu)
+    imp = ft(0.0)
+    peak = ft(0.0)
+    zero = np.zeros(3, ft)
+    for k in range(frames):
+        knot = min(k // max(hold, 1), len(controls) - 1)
+        st, ship, d = o.frame_step(st, ship, zero, zero, vw, st.dpdem_floe_eligible, None, controls[knot])
+        F = d["F_ice_on_ship"]
+        Fm = ft(np.sqrt(F[0] * F[0] + F[1] * F[1]))
+        imp = ft(imp + Fm * ft(o.dt))
+        peak = vmax(peak, Fm)
+    ke1 = kinetic_energy(o, ship.nu)
+    return st, ship, np.array([imp, ke0 - ke1, peak, ke0], dtype=ft)
