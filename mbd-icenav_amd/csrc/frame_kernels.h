@@ -1,16 +1,23 @@
 // frame_kernels.h -- the per-frame kernels of the DPDEM step.
 //
-//   k_cull    per element: near-ship test (bounding circles) into a work
-//             queue; insertion into the fixed-capacity cell buckets
+// A call of K frames (one ice sub-step each) launches
+//     k_cull | k_near k_force k_post | k_near k_force k_post | ...
+//   k_cull    once per call: near-ship test of every element into a work
+//             queue, insertion into the cell buckets of set 0
 //   k_near    one wavefront per queued element: hull pass 1 + 2, pre-pass,
-//             floe or/max (integer atomics).  The last workgroup to finish
-//             (ticket counter) computes the pairwise-tree totals, the thrust
-//             scale, floe eligibility and the ship step.
-//   k_force   16 lanes per element: activation, submergence, bond forces
-//             (lane-parallel terms, sequential CSR-order sum), contacts
-//             (lane-parallel scan of the 3x3 buckets, LDS sort by partner),
-//             drag, integration
-//   k_bond    per bond: fracture; per element: commit + histories; clears
+//             floe or/max (integer atomics).  The wave that finishes the
+//             last near element of a 256-element block writes that block's
+//             pairwise-tree partials; the wave that finishes the last such
+//             block forms the totals and the thrust scale.
+//   k_force   16 lanes per element: activation, submergence, bond forces,
+//             contacts, drag, integration.  nblk extra workgroups form the
+//             scaled ship-force partials; the last of them computes the ship
+//             reaction, floe eligibility and the ship step.
+//   k_post    per bond: fracture; per element: commit, histories, and the
+//             next frame's near test and cell insertion (the bucket set of
+//             the next frame's parity); clears the set this frame used.
+// Frame k uses bucket set k & 1.  Between calls every counter, both bucket
+// sets and the floe flags are zero.
 // The same bodies run inside one workgroup per scene for batched rollouts.
 #pragma once
 #include "kernels.h"
@@ -21,12 +28,23 @@ constexpr int GRP = 16;                 // lanes per element in k_force
 constexpr int EPB = BLOCK / GRP;        // elements per k_force block
 constexpr int MAXC2 = 48;               // contact candidates kept per element
 
-// ── broad phase: uniform grid of fixed-capacity buckets ─────────────────────
-// Elements of radius <= grid_rsmall sit in the bucket of their cell and scan
-// the 3x3 cells around it (cell >= 2 (grid_rsmall + delta)); larger elements
-// are inserted into, and scan, every cell within r + grid_rsmall + 2 delta.
-// A full bucket sends the element to an overflow list that every element
-// also scans.  Any of these gives exactly the reference's contact set.
+__device__ __forceinline__ int32_t* near_count(const DevView* V) { return V->counters; }
+__device__ __forceinline__ int32_t* nearblk_count(const DevView* V) { return V->counters + 1; }
+__device__ __forceinline__ int32_t* near_ticket(const DevView* V) { return V->counters + 2; }
+__device__ __forceinline__ int32_t* force_ticket(const DevView* V) { return V->counters + 3; }
+
+// ── broad phase: two uniform grids of fixed-capacity buckets ────────────────
+// Elements of radius <= grid_rsmall sit in the bucket of their fine cell
+// (cell >= 2 (grid_rsmall + delta)); larger ones sit in the bucket of their
+// coarse cell (cell2 >= 2 r_max + delta).  Every element scans the 3x3 coarse
+// cells around it for large partners; small partners are in the 3x3 fine
+// cells around a small element, or in the fine cells within
+// r + grid_rsmall + delta of a large one.  Each element is in exactly one
+// bucket, so each partner is seen once.  A full bucket sends the element to
+// an overflow list that every element also scans.  Any of these gives
+// exactly the reference's contact set.  A slot holds a copy of the element's
+// position, velocity and radius, so the contact pass reads its partners from
+// the buckets alone.
 
 __device__ __forceinline__ bool is_big(const IceDemParams& P, float r) { return r > P.grid_rsmall; }
 
@@ -43,32 +61,46 @@ __device__ __forceinline__ float big_reach(const IceDemParams& P, float r) {
     return (r + P.grid_rsmall + P.contact_dilation2) * 1.001f + 1e-3f;
 }
 
-__device__ __forceinline__ void bucket_put(const DevView* __restrict__ V, int c, int e) {
-    const int cap = V->p.grid_bucket;
-    const int idx = atomicAdd(&V->cell_count[c], 1);
-    if (idx < cap) V->cell_slots[(size_t)c * cap + idx] = e;
-    else V->ovf_list[atomicAdd(V->ovf_count, 1)] = e;
+__device__ __forceinline__ void coarse_of(const DevView* __restrict__ V, float x, float y, int& cx, int& cy) {
+    const IceDemParams& P = V->p;
+    float fx = floorf((x - P.grid_x0) / V->cell2);
+    float fy = floorf((y - P.grid_y0) / V->cell2);
+    cx = (int)fminf(fmaxf(fx, 0.0f), (float)(V->nx2 - 1));
+    cy = (int)fminf(fmaxf(fy, 0.0f), (float)(V->ny2 - 1));
 }
 
-__device__ __forceinline__ void cells_insert(const DevView* __restrict__ V, int e, float x, float y) {
+// Element e (state x, y, vx, vy, radius r) into its bucket of set `set`.
+__device__ __forceinline__ void cell_insert(const DevView* __restrict__ V, int set, int e,
+                                            float x, float y, float vx, float vy, float r) {
     const IceDemParams& P = V->p;
-    const float r = V->radius[e];
+    int32_t* cnt;
+    float4* kin;
+    int2* id;
+    int cap;
+    size_t cs;
     if (!is_big(P, r)) {
         int cx, cy;
-        bucket_put(V, cell_of(P, x, y, cx, cy), e);
+        cs = (size_t)set * V->G + cell_of(P, x, y, cx, cy);
+        cnt = V->cell_count; kin = V->cell_kin; id = V->cell_id; cap = P.grid_bucket;
     } else {
-        int x0, x1, y0, y1;
-        cell_box(P, x, y, big_reach(P, r), x0, x1, y0, y1);
-        for (int gy = y0; gy <= y1; ++gy)
-            for (int gx = x0; gx <= x1; ++gx) bucket_put(V, gy * P.grid_nx + gx, e);
+        int cx, cy;
+        coarse_of(V, x, y, cx, cy);
+        cs = (size_t)set * V->G2 + cy * V->nx2 + cx;
+        cnt = V->cell2_count; kin = V->cell2_kin; id = V->cell2_id; cap = V->cap2;
+    }
+    const int idx = atomicAdd(&cnt[cs], 1);
+    if (idx < cap) {
+        kin[cs * cap + idx] = make_float4(x, y, vx, vy);
+        id[cs * cap + idx] = make_int2(e, __float_as_int(r));
+    } else {
+        V->ovf_list[(size_t)set * V->ovf_cap + atomicAdd(&V->ovf_count[set], 1)] = e;
     }
 }
 
-// cells to scan for element e: 3x3 around its cell (small) or its box (big)
-__device__ __forceinline__ void scan_range(const DevView* __restrict__ V, int e, float x, float y,
+// fine cells to scan for an element of radius r at (x, y): 3x3 around its
+// cell (small) or its reach box (big)
+__device__ __forceinline__ void scan_range(const IceDemParams& P, float r, float x, float y,
                                            int& x0, int& x1, int& y0, int& y1) {
-    const IceDemParams& P = V->p;
-    const float r = V->radius[e];
     if (!is_big(P, r)) {
         int cx, cy;
         cell_of(P, x, y, cx, cy);
@@ -79,28 +111,44 @@ __device__ __forceinline__ void scan_range(const DevView* __restrict__ V, int e,
     }
 }
 
-// One ice-ice pair (lo < hi) of _compute_ice_ice_forces_jax (jax_simulation.py:111-164).
-__device__ __forceinline__ bool pair_overlap(const IceDemParams& P, const float* __restrict__ pos,
-                                             const float* __restrict__ radius, int n, int lo, int hi) {
-    float dx = pos[hi] - pos[lo];
-    float dy = pos[n + hi] - pos[n + lo];
+// coarse cells to scan: 3x3 around the element's coarse cell
+__device__ __forceinline__ void scan_range2(const DevView* __restrict__ V, float x, float y,
+                                            int& x0, int& x1, int& y0, int& y1) {
+    int cx, cy;
+    coarse_of(V, x, y, cx, cy);
+    x0 = max(cx - 1, 0); x1 = min(cx + 1, V->nx2 - 1);
+    y0 = max(cy - 1, 0); y1 = min(cy + 1, V->ny2 - 1);
+}
+
+// Overlap test of one ice-ice pair (lo < hi) of _compute_ice_ice_forces_jax
+// (jax_simulation.py:111-164), on the pair's values.
+__device__ __forceinline__ bool overlap_v(const IceDemParams& P, float xl, float yl, float rlo,
+                                          float xh, float yh, float rhi) {
+    float dx = xh - xl;
+    float dy = yh - yl;
     float dist = sqrtf(dx * dx + dy * dy);
-    float md = radius[lo] + radius[hi];
+    float md = rlo + rhi;
     if (P.contact_dilation2 != 0.0f) md = md + P.contact_dilation2;
     float ov = vmax(md - dist, 0.0f);
     ov = (dist > 1e-6f) ? ov : 0.0f;
     return ov > 0.0f;
 }
 
-// Contribution of pair (lo, hi) to element `me` (me == lo: i-role, else j-role).
-__device__ __forceinline__ void pair_contrib(const IceDemParams& P, const float* __restrict__ pos,
-                                             const float* __restrict__ vel, const float* __restrict__ radius,
-                                             int n, int lo, int hi, bool me_is_lo,
-                                             float& fx, float& fy, float& m) {
-    float dx = pos[hi] - pos[lo];
-    float dy = pos[n + hi] - pos[n + lo];
+__device__ __forceinline__ bool pair_overlap(const IceDemParams& P, const float* __restrict__ pos,
+                                             const float* __restrict__ radius, int n, int lo, int hi) {
+    return overlap_v(P, pos[lo], pos[n + lo], radius[lo], pos[hi], pos[n + hi], radius[hi]);
+}
+
+// Contribution of pair (lo, hi) to one of its elements (me_is_lo: the
+// i-role, else the j-role), jax_simulation.py:111-164.
+__device__ __forceinline__ void contrib_v(const IceDemParams& P, bool me_is_lo,
+                                          float xl, float yl, float vxl, float vyl, float rlo,
+                                          float xh, float yh, float vxh, float vyh, float rhi,
+                                          float& fx, float& fy, float& m) {
+    float dx = xh - xl;
+    float dy = yh - yl;
     float dist = sqrtf(dx * dx + dy * dy);
-    float md = radius[lo] + radius[hi];
+    float md = rlo + rhi;
     if (P.contact_dilation2 != 0.0f) md = md + P.contact_dilation2;
     float ov = vmax(md - dist, 0.0f);
     ov = (dist > 1e-6f) ? ov : 0.0f;
@@ -108,8 +156,8 @@ __device__ __forceinline__ void pair_contrib(const IceDemParams& P, const float*
     float dirx = dx / den, diry = dy / den;
     float f_n = P.hertz_factor * dm_powf(ov, P.hertz_exponent);
     float fnx = dirx * f_n, fny = diry * f_n;
-    float vrx = vel[hi] - vel[lo];
-    float vry = vel[n + hi] - vel[n + lo];
+    float vrx = vxh - vxl;
+    float vry = vyh - vyl;
     float vn = vrx * dirx + vry * diry;
     float vtx = vrx - vn * dirx, vty = vry - vn * diry;
     float vt = sqrtf(vtx * vtx + vty * vty);
@@ -122,6 +170,13 @@ __device__ __forceinline__ void pair_contrib(const IceDemParams& P, const float*
     float cross = -(dx * fony - dy * fonx);
     if (me_is_lo) { fx = -fnx + ftx; fy = -fny + fty; m = 0.5f * cross; }
     else { fx = fnx - ftx; fy = fny - fty; m = -0.5f * cross; }
+}
+
+__device__ __forceinline__ void pair_contrib(const IceDemParams& P, const float* __restrict__ pos,
+                                             const float* __restrict__ vel, const float* __restrict__ radius,
+                                             int n, int lo, int hi, bool me_is_lo, float& fx, float& fy, float& m) {
+    contrib_v(P, me_is_lo, pos[lo], pos[n + lo], vel[lo], vel[n + lo], radius[lo],
+              pos[hi], pos[n + hi], vel[hi], vel[n + hi], radius[hi], fx, fy, m);
 }
 
 // partners after e first (pair_i role, ascending), then partners before e
@@ -154,91 +209,7 @@ __device__ __forceinline__ PrePass prepass(const IceDemParams& P, float pen, flo
     return o;
 }
 
-
-// ── k_cull ───────────────────────────────────────────────────────────────────
-
-// elements [chunk*256, chunk*256+256) by a 256-thread block (all threads call)
-__device__ __forceinline__ void cull_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int chunk) {
-    const IceDemParams& P = V->p;
-    const int n = V->n;
-    const int e = chunk * BLOCK + threadIdx.x;
-    const float* pos = D->s.d_pos;
-    const float* ship = D->s.d_ship;
-    bool near = false;
-    if (e < n) {
-        const float px = pos[e], py = pos[n + e];
-        const float dx = px - ship[0], dy = py - ship[1];
-        const float R = V->bound[e] + P.ship_bound_radius + 1.0f;
-        near = (dx * dx + dy * dy) <= R * R;
-        V->nearf[e] = near ? 1 : 0;
-        if (near) V->near_list[atomicAdd(V->near_count, 1)] = e;
-        cells_insert(V, e, px, py);
-    }
-    const int any = __syncthreads_or(near ? 1 : 0);
-    if (threadIdx.x == 0) V->nearblk[chunk] = any ? 1 : 0;
-}
-
-__global__ __launch_bounds__(BLOCK) void k_cull(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
-    cull_body(V, D, blockIdx.x);
-}
-
-// cell insertion only (sub-steps > 0 and the leaf entry points)
-__global__ __launch_bounds__(BLOCK) void k_cellkey(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                                   const float* __restrict__ pos, int from_state) {
-    const int n = V->n;
-    const int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= n) return;
-    if (from_state) pos = D->s.d_pos;
-    cells_insert(V, e, pos[e], pos[n + e]);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_clear_cells(const DevView* __restrict__ V) {
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
-    if (t < V->G) V->cell_count[t] = 0;
-    if (t == 0) *V->ovf_count = 0;
-}
-
-// ── k_near ───────────────────────────────────────────────────────────────────
-
-// waves [wave, wave + nwaves, ...] of the near queue; every lane returns normally
-__device__ __forceinline__ void near_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                          int wave, int nwaves) {
-    const IceDemParams& P = V->p;
-    const int n = V->n, maxv = V->maxv;
-    const int lane = threadIdx.x & 63;
-    const int count = *V->near_count;
-    if (wave >= count) return;
-    const float* pos = D->s.d_pos;
-    const float* ship = D->s.d_ship;
-    const float sx = ship[0], sy = ship[1];
-    ShipPose SP = make_pose(sx, sy, ship[2], P.half_len, P.half_beam);
-    for (int q = wave; q < count; q += nwaves) {
-        const int e = V->near_list[q];
-        const float px = pos[e], py = pos[n + e];
-        float se, ce;
-        dm_sincosf(D->s.d_rot[e], &se, &ce);
-        float wx = 0.0f, wy = 0.0f;
-        bool mv = false;
-        if (lane < maxv) {
-            float2 lv = V->polys[(size_t)e * maxv + lane];
-            wx = ce * lv.x - se * lv.y + px;
-            wy = se * lv.x + ce * lv.y + py;
-            mv = V->mask[(size_t)e * maxv + lane] != 0;
-        }
-        float pen, nxo, nyo, cxo, cyo;
-        hull_element(SP, P, V->hull, V->S, maxv, wx, wy, mv, pen, nxo, nyo, cxo, cyo);
-        if (lane == 0) {
-            PrePass o = prepass(P, pen, nxo, nyo, V->radius[e], V->thick[e], px, py, sx, sy);
-            V->pen[e] = pen; V->nx[e] = nxo; V->ny[e] = nyo;
-            V->fm_raw[e] = o.fm; V->soi_x[e] = o.sx; V->soi_y[e] = o.sy; V->mterm[e] = o.m;
-            if (pen > 0.0f) {
-                const int f = V->floe[e];
-                atomicOr(&V->floe_coll[f], 1u);
-                atomicMax(&V->floe_maxfm[f], __float_as_uint(o.fm));
-            }
-        }
-    }
-}
+// ── pairwise trees with the order of the oracle's tree_sum ───────────────────
 
 // Tree sum of one 256-element block of a near-masked per-element array
 // (far elements contribute +0), by one wave: lane l sums its 4 consecutive
@@ -258,12 +229,209 @@ __device__ __forceinline__ float block256_tree(const DevView* __restrict__ V, co
     return wave_tree_sum(x);
 }
 
-// ship reaction and ship step (jax_simulation.py:471-479, 658-697); thread 0
-__device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                           int k, int last) {
+// Pairwise tree over a[0..npad) (npad a power of two, at most 4096) by one
+// wave; result in every lane.  Up to 64 entries: one per lane and a butterfly
+// of log2(npad) steps; beyond: each lane reduces its contiguous chunk
+// pairwise (a stack of partial sums per level), then the full butterfly.
+__device__ __forceinline__ float wave_tree_global(const float* __restrict__ a, int npad) {
+    const int lane = threadIdx.x & 63;
+    if (npad <= 64) {
+        float x = lane < npad ? a[lane] : 0.0f;
+        for (int off = 1; off < npad; off <<= 1) x = x + __shfl_xor(x, off, 64);
+        return __shfl(x, 0, 64);
+    }
+    const int c = npad >> 6;
+    const float* p = a + (size_t)lane * c;
+    float stk[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    #pragma unroll
+    for (int i = 0; i < 64; ++i) {
+        if (i < c) {
+            float x = p[i];
+            int lvl = 0;
+            #pragma unroll
+            for (int t = 0; t < 6; ++t) {
+                if (((i >> t) & 1) == 0) break;
+                x = stk[t] + x;
+                lvl = t + 1;
+            }
+            stk[lvl] = x;
+        }
+    }
+    float r = stk[0];
+    #pragma unroll
+    for (int t = 1; t < 7; ++t) r = (c == (1 << t)) ? stk[t] : r;
+    return wave_tree_sum(r);
+}
+
+// ── near test and cell insertion ─────────────────────────────────────────────
+
+// Near-ship test and cell insertion of the elements chunk*256 + threadIdx.x
+// for the frame that uses bucket set `set`; (x, y, vx, vy) is the element's
+// state at the start of that frame, (sx, sy) the ship's position.  Every
+// thread of the 256-thread block calls.
+__device__ __forceinline__ void cull_chunk(const DevView* __restrict__ V, int chunk, bool valid, int e,
+                                           float x, float y, float vx, float vy, float sx, float sy, int set) {
     const IceDemParams& P = V->p;
-    float Fx = -V->scal[2], Fy = -V->scal[3];
-    float tau[3] = {Fx, Fy, V->scal[1]};
+    bool near = false;
+    float r = 0.0f;
+    if (valid) {
+        r = V->radius[e];
+        const float dx = x - sx, dy = y - sy;
+        const float R = V->bound[e] + P.ship_bound_radius + 1.0f;
+        near = (dx * dx + dy * dy) <= R * R;
+        V->nearf[e] = near ? 1 : 0;
+        if (near) V->near_list[atomicAdd(near_count(V), 1)] = e;
+        cell_insert(V, set, e, x, y, vx, vy, r);
+    }
+    const int cnt = __syncthreads_count(near ? 1 : 0);
+    if (threadIdx.x == 0) {
+        V->nearcnt[chunk] = cnt;
+        V->blkdone[chunk] = 0;
+        if (cnt > 0) {
+            atomicAdd(nearblk_count(V), 1);
+        } else {
+            V->part1[chunk] = 0.0f;
+            V->part1[V->nbpad + chunk] = 0.0f;
+            V->part1[2 * V->nbpad + chunk] = 0.0f;
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_cull(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
+    const int n = V->n;
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    const bool valid = e < n;
+    const float* pos = D->s.d_pos;
+    const float* vel = D->s.d_vel;
+    float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f;
+    if (valid) { x = pos[e]; y = pos[n + e]; vx = vel[e]; vy = vel[n + e]; }
+    cull_chunk(V, blockIdx.x, valid, e, x, y, vx, vy, D->s.d_ship[0], D->s.d_ship[1], 0);
+}
+
+// cell insertion only (sub-steps > 0 and the leaf entry points)
+__global__ __launch_bounds__(BLOCK) void k_cellkey(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                                   const float* __restrict__ pos, const float* __restrict__ vel,
+                                                   int from_state, int set) {
+    const int n = V->n;
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    const bool valid = e < n;
+    if (from_state) { pos = D->s.d_pos; vel = D->s.d_vel; }
+    if (valid) cell_insert(V, set, e, pos[e], pos[n + e], vel[e], vel[n + e], V->radius[e]);
+}
+
+__device__ __forceinline__ void clear_cells(const DevView* __restrict__ V, int set, int t) {
+    if (t < V->G) V->cell_count[(size_t)set * V->G + t] = 0;
+    if (t < V->G2) V->cell2_count[(size_t)set * V->G2 + t] = 0;
+    if (t == 0) V->ovf_count[set] = 0;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_clear_cells(const DevView* __restrict__ V, int set) {
+    clear_cells(V, set, blockIdx.x * BLOCK + threadIdx.x);
+}
+
+// ── k_near ───────────────────────────────────────────────────────────────────
+
+// Hull narrow phase and pre-pass of near element e by the calling wave;
+// lane 0 stores the per-element results and the floe or/max.
+__device__ __forceinline__ void near_elem(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                          const ShipPose& SP, int e) {
+    const IceDemParams& P = V->p;
+    const int n = V->n, maxv = V->maxv;
+    const int lane = threadIdx.x & 63;
+    const float* pos = D->s.d_pos;
+    const float px = pos[e], py = pos[n + e];
+    const float rad = V->radius[e], thk = V->thick[e];
+    float wx = 0.0f, wy = 0.0f;
+    bool mv = false;
+    float2 lv = make_float2(0.0f, 0.0f);
+    if (lane < maxv) {
+        lv = V->polys[(size_t)e * maxv + lane];
+        mv = V->mask[(size_t)e * maxv + lane] != 0;
+    }
+    float se, ce;
+    dm_sincosf(D->s.d_rot[e], &se, &ce);
+    if (lane < maxv) {
+        wx = ce * lv.x - se * lv.y + px;
+        wy = se * lv.x + ce * lv.y + py;
+    }
+    float pen, nxo, nyo, cxo, cyo;
+    hull_element(SP, P, V->hull, V->S, maxv, wx, wy, mv, pen, nxo, nyo, cxo, cyo);
+    if (lane == 0) {
+        PrePass o = prepass(P, pen, nxo, nyo, rad, thk, px, py, SP.sx, SP.sy);
+        V->pen[e] = pen; V->nx[e] = nxo; V->ny[e] = nyo;
+        V->fm_raw[e] = o.fm; V->soi_x[e] = o.sx; V->soi_y[e] = o.sy; V->mterm[e] = o.m;
+        if (pen > 0.0f) {
+            const int f = V->floe[e];
+            atomicOr(&V->floe_coll[f], 1u);
+            atomicMax(&V->floe_maxfm[f], __float_as_uint(o.fm));
+        }
+    }
+}
+
+// The near elements of block b are all done: its partials, and for the last
+// such block the totals and the thrust scale (jax_simulation.py:456-470).
+// One wave.
+__device__ void near_block_done(const DevView* __restrict__ V, int b) {
+    const IceDemParams& P = V->p;
+    const int lane = threadIdx.x & 63, nbpad = V->nbpad;
+    __threadfence();
+    const float a = block256_tree(V, V->soi_x, b, 1.0f, false);
+    const float c = block256_tree(V, V->soi_y, b, 1.0f, false);
+    const float d = block256_tree(V, V->mterm, b, 1.0f, false);
+    int lastb = 0;
+    if (lane == 0) {
+        V->part1[b] = a; V->part1[nbpad + b] = c; V->part1[2 * nbpad + b] = d;
+        __threadfence();
+        lastb = (atomicAdd(near_ticket(V), 1) == *nearblk_count(V) - 1) ? 1 : 0;
+    }
+    if (!__shfl(lastb, 0, 64)) return;
+    __threadfence();
+    const float tx = wave_tree_global(V->part1, nbpad);
+    const float ty = wave_tree_global(V->part1 + nbpad, nbpad);
+    const float tm = wave_tree_global(V->part1 + 2 * nbpad, nbpad);
+    float scale = 1.0f;
+    if (P.has_thrust_cap) {
+        const float F_tot = sqrtf(tx * tx + ty * ty);
+        scale = (F_tot > P.max_ship_ice_force) ? (P.max_ship_ice_force / vmax(F_tot, 1e-6f)) : 1.0f;
+    }
+    if (lane == 0) { V->scal[0] = scale; V->scal[1] = tm; V->scal[4] = tx; V->scal[5] = ty; }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_near(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * (BLOCK / 64);
+    const int count = *near_count(V);
+    const int e0 = (wave < V->n) ? V->near_list[wave] : 0;   // read beside the count
+    if (count == 0) {
+        if (wave == 0 && lane == 0) { V->scal[0] = 1.0f; V->scal[1] = 0.0f; V->scal[4] = 0.0f; V->scal[5] = 0.0f; }
+        return;
+    }
+    if (wave >= count) return;
+    const float* ship = D->s.d_ship;
+    const ShipPose SP = make_pose(ship[0], ship[1], ship[2], V->p.half_len, V->p.half_beam);
+    for (int q = wave; q < count; q += nwaves) {
+        const int e = (q == wave) ? e0 : V->near_list[q];
+        near_elem(V, D, SP, e);
+        int fin = 0;
+        if (lane == 0) {
+            const int b = e / BLOCK;
+            __threadfence();
+            fin = (atomicAdd(&V->blkdone[b], 1) == V->nearcnt[b] - 1) ? 1 : 0;
+        }
+        if (__shfl(fin, 0, 64)) near_block_done(V, e / BLOCK);
+    }
+}
+
+// ── ship reaction, floe eligibility, ship step ───────────────────────────────
+
+// ship reaction and ship step (jax_simulation.py:471-479, 658-697); one thread.
+// (fsx, fsy) is the sum of the (scaled) ship forces on ice, tm their moment.
+__device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                           int k, int last, float fsx, float fsy, float tm) {
+    const IceDemParams& P = V->p;
+    float Fx = -fsx, Fy = -fsy;
+    float tau[3] = {Fx, Fy, tm};
     float* ship = D->s.d_ship;
     float spc[3] = {0.0f, 0.0f, 0.0f}, spn[3] = {0.0f, 0.0f, 0.0f};
     float cu = 0.0f, cr = 0.0f;
@@ -315,83 +483,75 @@ __device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const 
     }
 }
 
-// Totals, thrust cap (jax_simulation.py:456-479), floe eligibility
-// (:511-524) and the ship step, by one workgroup of any size; la, lb hold
-// nbpad floats each.
-__device__ void mid_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D, float* la, float* lb,
-                         int k, int last) {
+// floe eligibility (jax_simulation.py:511-524); all threads of a workgroup
+__device__ __forceinline__ void floe_eligibility(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                                 float scale) {
     const IceDemParams& P = V->p;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int nb = V->nblk, nbpad = V->nbpad;
-    for (int b = w; b < nb; b += nw) {
-        float a = 0.0f, c = 0.0f, d = 0.0f;
-        if (V->nearblk[b]) {
-            a = block256_tree(V, V->soi_x, b, 1.0f, false);
-            c = block256_tree(V, V->soi_y, b, 1.0f, false);
-            d = block256_tree(V, V->mterm, b, 1.0f, false);
-        }
-        if (lane == 0) { V->part1[b] = a; V->part1[nbpad + b] = c; V->part1[2 * nbpad + b] = d; }
+    if (V->nb <= 0) return;
+    uint8_t* elig = D->s.d_floe_elig;
+    for (int f = threadIdx.x; f < V->nf; f += blockDim.x) {
+        bool coll = V->floe_coll[f] != 0u;
+        float mx = __uint_as_float(V->floe_maxfm[f]);
+        if (P.has_thrust_cap) mx = mx * scale;
+        bool el = (elig[f] != 0) || (coll && (mx > P.elig_threshold));
+        elig[f] = el ? 1 : 0;
     }
-    __syncthreads();
-    float tx = lds_tree_sum(V->part1, nb, nbpad, la, lb);
-    float ty = lds_tree_sum(V->part1 + nbpad, nb, nbpad, la, lb);
-    float tm = lds_tree_sum(V->part1 + 2 * nbpad, nb, nbpad, la, lb);
-    float scale = 1.0f;
-    if (P.has_thrust_cap) {
-        float F_tot = sqrtf(tx * tx + ty * ty);
-        scale = (F_tot > P.max_ship_ice_force) ? (P.max_ship_ice_force / vmax(F_tot, 1e-6f)) : 1.0f;
-    }
-    for (int b = w; b < nb; b += nw) {
-        float a = 0.0f, c = 0.0f;
-        if (V->nearblk[b]) {
-            a = block256_tree(V, V->soi_x, b, scale, P.has_thrust_cap != 0);
-            c = block256_tree(V, V->soi_y, b, scale, P.has_thrust_cap != 0);
-        }
-        if (lane == 0) { V->part2[b] = a; V->part2[nbpad + b] = c; }
-    }
-    __syncthreads();
-    float fx = lds_tree_sum(V->part2, nb, nbpad, la, lb);
-    float fy = lds_tree_sum(V->part2 + nbpad, nb, nbpad, la, lb);
-    if (threadIdx.x == 0) { V->scal[0] = scale; V->scal[1] = tm; V->scal[2] = fx; V->scal[3] = fy; }
-    if (V->nb > 0) {
-        uint8_t* elig = D->s.d_floe_elig;
-        for (int f = threadIdx.x; f < V->nf; f += blockDim.x) {
-            bool coll = V->floe_coll[f] != 0u;
-            float mx = __uint_as_float(V->floe_maxfm[f]);
-            if (P.has_thrust_cap) mx = mx * scale;
-            bool el = (elig[f] != 0) || (coll && (mx > P.elig_threshold));
-            elig[f] = el ? 1 : 0;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) ship_frame(V, D, k, last);
-    __syncthreads();
 }
 
-__global__ __launch_bounds__(BLOCK) void k_near(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                                int k, int last) {
-    __shared__ float la[MAX_PARTIALS], lb[MAX_PARTIALS];
-    __shared__ int am_last;
-    near_body(V, D, blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6), gridDim.x * (BLOCK / 64));
-    __threadfence();
+// Extra workgroup b of k_force: the scaled partials of element block b; the
+// last of the nextra workgroups forms the ship reaction, the floe
+// eligibility and the ship step, and resets the near counters.
+__device__ void force_tail(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int k, int last,
+                           int b, int nextra, int* am_last) {
+    const IceDemParams& P = V->p;
+    const int nbpad = V->nbpad;
+    const bool capped = P.has_thrust_cap != 0;
+    if (capped && threadIdx.x < 64) {
+        float a = 0.0f, c = 0.0f;
+        if (V->nearcnt[b] > 0) {
+            const float s = V->scal[0];
+            a = block256_tree(V, V->soi_x, b, s, true);
+            c = block256_tree(V, V->soi_y, b, s, true);
+        }
+        if (threadIdx.x == 0) { V->part2[b] = a; V->part2[nbpad + b] = c; }
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        *am_last = (atomicAdd(force_ticket(V), 1) == nextra - 1) ? 1 : 0;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) am_last = (atomicAdd(V->ticket, 1) == (int)gridDim.x - 1) ? 1 : 0;
-    __syncthreads();
-    if (!am_last) return;
+    if (!*am_last) return;
     __threadfence();
-    mid_body(V, D, la, lb, k, last);
-    if (threadIdx.x == 0) *V->ticket = 0;
+    const float scale = V->scal[0];
+    float fx = 0.0f, fy = 0.0f;
+    if (threadIdx.x < 64) {
+        if (capped) {
+            fx = wave_tree_global(V->part2, nbpad);
+            fy = wave_tree_global(V->part2 + nbpad, nbpad);
+        } else {
+            fx = V->scal[4];
+            fy = V->scal[5];
+        }
+    }
+    floe_eligibility(V, D, scale);
+    if (threadIdx.x == 0) {
+        ship_frame(V, D, k, last, fx, fy, V->scal[1]);
+        *near_count(V) = 0;
+        *nearblk_count(V) = 0;
+        *near_ticket(V) = 0;
+    }
 }
 
 // ── k_force ──────────────────────────────────────────────────────────────────
 
 struct ForceLds {
-    int cand[EPB][MAXC2];
-    int sorted[EPB][MAXC2];
+    int cj[EPB][MAXC2];
+    float cx[EPB][MAXC2], cy[EPB][MAXC2], cvx[EPB][MAXC2], cvy[EPB][MAXC2], cr[EPB][MAXC2];
     int keys[EPB][MAXC2];
+    int ord[EPB][MAXC2];
     float cfx[EPB][MAXC2], cfy[EPB][MAXC2], cfm[EPB][MAXC2];
     int gcell[EPB][GRP];
-    int goff[EPB][GRP + 1];
+    int goff[EPB][GRP];
     float bf[EPB][3][GRP];
 };
 
@@ -412,120 +572,157 @@ __device__ __forceinline__ int group_incl_scan(int v) {
     return v;
 }
 
-// Ice-ice contact force on element e (group-uniform), summed in the
-// reference's scatter order; result valid in the group's lane 0.
-__device__ void contact_force_group(const DevView* __restrict__ V, const float* __restrict__ pos,
-                                    const float* __restrict__ vel, int e, float px, float py,
-                                    ForceLds& L, int g, float& fx, float& fy, float& fm) {
-    const IceDemParams& P = V->p;
-    const int n = V->n;
-    const int cap = P.grid_bucket;
+// The occupied slots of cells [x0..x1] x [y0..y1] of one grid (row length
+// nxg, buckets at cnt / kin / id with capacity cap), 16 cells at a time:
+// the lanes read the cell counts, a group prefix sum lays the occupied
+// slots out, and each lane takes one slot per round.  visit(have, j, kin, r)
+// is called by every lane of the group, the same number of times.
+template <class F>
+__device__ __forceinline__ void scan_grid(const int32_t* __restrict__ cnt, const float4* __restrict__ kin,
+                                          const int2* __restrict__ id, int cap, int nxg,
+                                          int x0, int x1, int y0, int y1, ForceLds& L, int g, F&& visit) {
     const int l = threadIdx.x & (GRP - 1);
-    const int gshift = (threadIdx.x & 63) & ~(GRP - 1);
-    int* cand = L.cand[g];
-    int x0, x1, y0, y1;
-    scan_range(V, e, px, py, x0, x1, y0, y1);
     const int W = x1 - x0 + 1;
     const int ncell = W * (y1 - y0 + 1);
-    int cnt = 0;
-    auto test = [&](bool have, int j) {
-        bool ov = false;
-        if (have && j != e) ov = pair_overlap(P, pos, V->radius, n, min(e, j), max(e, j));
-        const uint64_t bal = __ballot(ov);
-        const uint32_t m = (uint32_t)(bal >> gshift) & ((1u << GRP) - 1u);
-        const int before = __builtin_popcount(m & ((1u << l) - 1u));
-        if (ov && cnt + before < MAXC2) cand[cnt + before] = j;
-        cnt += __builtin_popcount(m);
-    };
     for (int c0 = 0; c0 < ncell; c0 += GRP) {
         const int q = c0 + l;
         int cell = 0, cc = 0;
         if (q < ncell) {
-            cell = (y0 + q / W) * P.grid_nx + (x0 + q % W);
-            cc = min(V->cell_count[cell], cap);
+            cell = (y0 + q / W) * nxg + (x0 + q % W);
+            cc = min(cnt[cell], cap);
         }
         const int incl = group_incl_scan(cc);
         const int total = __shfl(incl, GRP - 1, GRP);
         L.gcell[g][l] = cell;
         L.goff[g][l] = incl - cc;
         wave_sync_lds();
+        const int nq = min(GRP, ncell - c0);
         for (int base = 0; base < total; base += GRP) {
             const int f = base + l;
-            int j = -1;
-            if (f < total) {
+            const bool have = f < total;
+            float4 k4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            int2 i2 = make_int2(-1, 0);
+            if (have) {
                 int qq = 0;
                 #pragma unroll
-                for (int t = 1; t < GRP; ++t) qq += (L.goff[g][t] <= f && t < ncell - c0) ? 1 : 0;
-                j = V->cell_slots[(size_t)L.gcell[g][qq] * cap + (f - L.goff[g][qq])];
+                for (int t = 1; t < GRP; ++t) qq += (t < nq && L.goff[g][t] <= f) ? 1 : 0;
+                const size_t si = (size_t)L.gcell[g][qq] * cap + (f - L.goff[g][qq]);
+                k4 = kin[si];
+                i2 = id[si];
             }
-            test(f < total, j);
+            visit(have, i2.x, k4, __int_as_float(i2.y));
         }
         wave_sync_lds();
     }
-    const int novf = *V->ovf_count;
+}
+
+// Ice-ice contact force on element e (group-uniform), summed in the
+// reference's scatter order; result valid in the group's lane 0.
+// (px, py, pvx, pvy, pr) is e's own state; partners come from bucket set
+// `set` (and, when buckets overflowed, from pos/vel/radius).
+__device__ void contact_force_group(const DevView* __restrict__ V, int set, const float* __restrict__ pos,
+                                    const float* __restrict__ vel, int e, float px, float py, float pvx,
+                                    float pvy, float pr, int novf, ForceLds& L, int g,
+                                    float& fx, float& fy, float& fm) {
+    const IceDemParams& P = V->p;
+    const int n = V->n;
+    const int l = threadIdx.x & (GRP - 1);
+    const int gshift = (threadIdx.x & 63) & ~(GRP - 1);
+    int count = 0;
+    auto push = [&](bool ov, int j, float x, float y, float vx, float vy, float r) {
+        const uint64_t bal = __ballot(ov);
+        const uint32_t m = (uint32_t)(bal >> gshift) & ((1u << GRP) - 1u);
+        const int before = __builtin_popcount(m & ((1u << l) - 1u));
+        if (ov && count + before < MAXC2) {
+            const int s = count + before;
+            L.cj[g][s] = j; L.cx[g][s] = x; L.cy[g][s] = y; L.cvx[g][s] = vx; L.cvy[g][s] = vy; L.cr[g][s] = r;
+        }
+        count += __builtin_popcount(m);
+    };
+    auto visit = [&](bool have, int j, float4 k4, float r) {
+        bool ov = false;
+        if (have && j != e)
+            ov = (e < j) ? overlap_v(P, px, py, pr, k4.x, k4.y, r) : overlap_v(P, k4.x, k4.y, r, px, py, pr);
+        push(ov, j, k4.x, k4.y, k4.z, k4.w, r);
+    };
+    int x0, x1, y0, y1;
+    scan_range(P, pr, px, py, x0, x1, y0, y1);
+    scan_grid(V->cell_count + (size_t)set * V->G, V->cell_kin + (size_t)set * V->G * P.grid_bucket,
+              V->cell_id + (size_t)set * V->G * P.grid_bucket, P.grid_bucket, P.grid_nx, x0, x1, y0, y1, L, g, visit);
+    int u0 = 0, u1 = -1, v0 = 0, v1 = -1;
+    if (V->nbig > 0) {
+        scan_range2(V, px, py, u0, u1, v0, v1);
+        scan_grid(V->cell2_count + (size_t)set * V->G2, V->cell2_kin + (size_t)set * V->G2 * V->cap2,
+                  V->cell2_id + (size_t)set * V->G2 * V->cap2, V->cap2, V->nx2, u0, u1, v0, v1, L, g, visit);
+    }
     for (int base = 0; base < novf; base += GRP) {
         const int f = base + l;
-        const int j = (f < novf) ? V->ovf_list[f] : -1;
-        test(f < novf, j);
+        bool ov = false;
+        int j = -1;
+        float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f, r = 0.0f;
+        if (f < novf) {
+            j = V->ovf_list[(size_t)set * V->ovf_cap + f];
+            x = pos[j]; y = pos[n + j]; vx = vel[j]; vy = vel[n + j]; r = V->radius[j];
+            if (j != e) ov = (e < j) ? overlap_v(P, px, py, pr, x, y, r) : overlap_v(P, x, y, r, px, py, pr);
+        }
+        push(ov, j, x, y, vx, vy, r);
     }
     fx = 0.0f; fy = 0.0f; fm = 0.0f;
-    if (cnt <= MAXC2) {
-        // sort by rotated partner index; a big element can sit in several of
-        // the scanned cells, so duplicates keep their first copy only
-        for (int i = l; i < cnt; i += GRP) {
-            const int ci = cand[i];
-            bool dup = false;
-            for (int q = 0; q < i; ++q) dup = dup || (cand[q] == ci);
-            L.keys[g][i] = dup ? 0x7fffffff : rot_key(ci, e, n);
-        }
+    if (count <= MAXC2) {
+        // every partner was seen once: sort by rotated partner index
+        for (int i = l; i < count; i += GRP) L.keys[g][i] = rot_key(L.cj[g][i], e, n);
         wave_sync_lds();
-        int uniq = 0;
-        for (int q = 0; q < cnt; ++q) uniq += (L.keys[g][q] != 0x7fffffff) ? 1 : 0;
-        for (int i = l; i < cnt; i += GRP) {
+        for (int i = l; i < count; i += GRP) {
             const int ki = L.keys[g][i];
-            if (ki == 0x7fffffff) continue;
             int rank = 0;
-            for (int q = 0; q < cnt; ++q) rank += (L.keys[g][q] < ki) ? 1 : 0;
-            L.sorted[g][rank] = cand[i];
+            for (int q = 0; q < count; ++q) rank += (L.keys[g][q] < ki) ? 1 : 0;
+            L.ord[g][rank] = i;
         }
         wave_sync_lds();
-        for (int i = l; i < uniq; i += GRP) {
-            const int j = L.sorted[g][i];
+        for (int i = l; i < count; i += GRP) {
+            const int s = L.ord[g][i];
+            const int j = L.cj[g][s];
+            const float x = L.cx[g][s], y = L.cy[g][s], vx = L.cvx[g][s], vy = L.cvy[g][s], r = L.cr[g][s];
             float a, b, c;
-            pair_contrib(P, pos, vel, V->radius, n, min(e, j), max(e, j), e < j, a, b, c);
+            if (e < j) contrib_v(P, true, px, py, pvx, pvy, pr, x, y, vx, vy, r, a, b, c);
+            else contrib_v(P, false, x, y, vx, vy, r, px, py, pvx, pvy, pr, a, b, c);
             L.cfx[g][i] = a; L.cfy[g][i] = b; L.cfm[g][i] = c;
         }
         wave_sync_lds();
         if (l == 0)
-            for (int i = 0; i < uniq; ++i) { fx = fx + L.cfx[g][i]; fy = fy + L.cfy[g][i]; fm = fm + L.cfm[g][i]; }
+            for (int i = 0; i < count; ++i) { fx = fx + L.cfx[g][i]; fy = fy + L.cfy[g][i]; fm = fm + L.cfm[g][i]; }
         wave_sync_lds();
         return;
     }
-    // more than MAXC2 candidate hits: exact selection of the next partner by lane 0
+    // more than MAXC2 contacts: exact selection of the next partner by lane 0
     if (l == 0) {
+        const int32_t* cnt1 = V->cell_count + (size_t)set * V->G;
+        const int2* id1 = V->cell_id + (size_t)set * V->G * P.grid_bucket;
+        const int32_t* cnt2 = V->cell2_count + (size_t)set * V->G2;
+        const int2* id2 = V->cell2_id + (size_t)set * V->G2 * V->cap2;
         int last = 0;
         for (;;) {
             int best = 0x7fffffff, bj = -1;
-            for (int q = 0; q < ncell; ++q) {
-                const int cell = (y0 + q / W) * P.grid_nx + (x0 + q % W);
-                const int cc = min(V->cell_count[cell], cap);
-                for (int s = 0; s < cc; ++s) {
-                    const int j = V->cell_slots[(size_t)cell * cap + s];
-                    if (j == e) continue;
-                    const int kj = rot_key(j, e, n);
-                    if (kj <= last || kj >= best) continue;
-                    if (!pair_overlap(P, pos, V->radius, n, min(e, j), max(e, j))) continue;
-                    best = kj; bj = j;
-                }
-            }
-            for (int s = 0; s < novf; ++s) {
-                const int j = V->ovf_list[s];
-                if (j == e) continue;
+            auto consider = [&](int j) {
+                if (j == e) return;
                 const int kj = rot_key(j, e, n);
-                if (kj <= last || kj >= best) continue;
-                if (!pair_overlap(P, pos, V->radius, n, min(e, j), max(e, j))) continue;
+                if (kj <= last || kj >= best) return;
+                if (!pair_overlap(P, pos, V->radius, n, min(e, j), max(e, j))) return;
                 best = kj; bj = j;
-            }
+            };
+            for (int gy = y0; gy <= y1; ++gy)
+                for (int gx = x0; gx <= x1; ++gx) {
+                    const int cell = gy * P.grid_nx + gx;
+                    const int cc = min(cnt1[cell], P.grid_bucket);
+                    for (int s = 0; s < cc; ++s) consider(id1[(size_t)cell * P.grid_bucket + s].x);
+                }
+            for (int gy = v0; gy <= v1; ++gy)
+                for (int gx = u0; gx <= u1; ++gx) {
+                    const int cell = gy * V->nx2 + gx;
+                    const int cc = min(cnt2[cell], V->cap2);
+                    for (int s = 0; s < cc; ++s) consider(id2[(size_t)cell * V->cap2 + s].x);
+                }
+            for (int s = 0; s < novf; ++s) consider(V->ovf_list[(size_t)set * V->ovf_cap + s]);
             if (bj < 0) break;
             last = best;
             float a, b, c;
@@ -536,46 +733,47 @@ __device__ void contact_force_group(const DevView* __restrict__ V, const float* 
 }
 
 // Bond forces on element e (jax_simulation.py:63-108): lanes compute the terms
-// of the element's CSR entries, lane 0 adds them in CSR order (the order of
-// the reference's two scatter-add passes).  Result valid in lane 0.
+// of the element's bond-table entries, lane 0 adds them in table order (the
+// order of the reference's two scatter-add passes).  act_e is e's activity
+// as the bond pass sees it; (bt0, bp0) is the lane's first table entry,
+// read ahead by the caller.  Result valid in lane 0.
 __device__ void bond_force_group(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                 const float* __restrict__ pos, int e, int mode, ForceLds& L, int g,
+                                 const float* __restrict__ pos, int e, float px, float py, bool act_e, int deg,
+                                 int4 bt0, float2 bp0, int mode, ForceLds& L, int g,
                                  float& fbx, float& fby, float& mb, bool& crack) {
-    const IceDemParams& P = V->p;
     const int n = V->n;
     const int l = threadIdx.x & (GRP - 1);
     const int gshift = (threadIdx.x & 63) & ~(GRP - 1);
     const uint8_t* active = D->s.d_active;
     fbx = 0.0f; fby = 0.0f; mb = 0.0f;
     crack = false;
-    const int c0 = V->csr_start[e], c1 = V->csr_start[e + 1];
-    for (int base = c0; base < c1; base += GRP) {
+    for (int base = 0; base < deg; base += GRP) {
         const int q = base + l;
         bool broken = false;
-        if (q < c1) {
-            const int ent = V->csr_ent[q];
-            const int b = ent >> 1;
-            const bool role_j = (ent & 1) != 0;
-            const int i = V->bi[b], j = V->bj[b];
+        if (q < deg) {
+            int4 t = bt0;
+            float2 pr = bp0;
+            if (base > 0) { t = V->btab[(size_t)e * V->W + q]; pr = V->bprm[(size_t)e * V->W + q]; }
+            const int b = t.x, o = t.y;
+            const bool role_j = t.w != 0;
+            const float ox = pos[o], oy = pos[n + o];
             const bool intact = D->s.d_bond_intact[b] != 0;
+            const float dmg = D->s.d_bond_damage[b];
+            bool ao = active[o] != 0;
+            if (mode != 1) ao = ao || (V->floe_coll[t.z] != 0u);
             broken = !intact;
-            float dx = pos[j] - pos[i];
-            float dy = pos[n + j] - pos[n + i];
+            const float xi = role_j ? ox : px, yi = role_j ? oy : py;
+            const float xj = role_j ? px : ox, yj = role_j ? py : oy;
+            const bool ai = role_j ? ao : act_e, aj = role_j ? act_e : ao;
+            float dx = xj - xi;
+            float dy = yj - yi;
             float dist = sqrtf(dx * dx + dy * dy);
             float dsafe = vmax(dist, 1e-10f);
             float dirx = dx / dsafe, diry = dy / dsafe;
-            float avg_h = (V->thick[i] + V->thick[j]) * 0.5f;
-            float stiff = P.youngs_modulus * avg_h * P.bond_stiffness_factor;
-            float stretch = dist - V->rest[b];
-            float Fm = stiff * stretch * (1.0f - D->s.d_bond_damage[b]);
+            float stretch = dist - pr.x;
+            float Fm = pr.y * stretch * (1.0f - dmg);
             float Fx = dirx * Fm, Fy = diry * Fm;
             if (!intact) { Fx = 0.0f; Fy = 0.0f; }
-            bool ai, aj;
-            if (mode == 1) { ai = active[i] != 0; aj = active[j] != 0; }
-            else {
-                ai = (active[i] != 0) || (V->floe_coll[V->floe[i]] != 0u);
-                aj = (active[j] != 0) || (V->floe_coll[V->floe[j]] != 0u);
-            }
             float ok = ((ai && aj) || (!ai && !aj)) ? 1.0f : 0.0f;
             Fx = Fx * ok; Fy = Fy * ok;
             float cross = 0.5f * (dx * Fy - dy * Fx);
@@ -587,7 +785,7 @@ __device__ void bond_force_group(const DevView* __restrict__ V, const CallDesc* 
         crack = crack || (((uint32_t)(bal >> gshift) & ((1u << GRP) - 1u)) != 0u);
         wave_sync_lds();
         if (l == 0) {
-            const int m = min(GRP, c1 - base);
+            const int m = min(GRP, deg - base);
             for (int t = 0; t < m; ++t) {
                 fbx = fbx + L.bf[g][0][t]; fby = fby + L.bf[g][1][t]; mb = mb + L.bf[g][2][t];
             }
@@ -596,9 +794,10 @@ __device__ void bond_force_group(const DevView* __restrict__ V, const CallDesc* 
     }
 }
 
-// elements [chunk*16, chunk*16+16), 16 lanes each (no block barrier inside)
+// elements [chunk*16, chunk*16+16), 16 lanes each (no block barrier inside).
+// mode 0: a frame sub-step (integrates); mode 1: forces out (fout, mout).
 __device__ void force_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                           int k, int sub, int src_is_state, int mode,
+                           int k, int sub, int src_is_state, int mode, int set,
                            const float* __restrict__ fship_in, float* __restrict__ fout, float* __restrict__ mout,
                            ForceLds& L, int chunk) {
     const IceDemParams& P = V->p;
@@ -615,22 +814,35 @@ __device__ void force_body(const DevView* __restrict__ V, const CallDesc* __rest
     float* drot = src_is_state ? V->nxt_rot : D->s.d_rot;
     float* domg = src_is_state ? V->nxt_omega : D->s.d_omega;
     const uint8_t* active = D->s.d_active;
-    const float px = pos[e], py = pos[n + e];
+    // everything that depends only on e, read in one round trip
+    const float px = pos[e], py = pos[n + e], vx = vel[e], vy = vel[n + e];
+    const float ro = rotp[e], om = omp[e];
+    const float r = V->radius[e], h = V->thick[e], area = V->area[e];
+    const float mass = V->mass[e], inertia = V->inertia[e];
+    const int fl = V->floe[e];
+    const bool act_raw = active[e] != 0;
+    const int deg = V->W > 0 ? V->bdeg[e] : 0;
+    const int novf = V->ovf_count[set];
+    int4 bt0 = make_int4(0, 0, 0, 0);
+    float2 bp0 = make_float2(0.0f, 0.0f);
+    if (l < V->W) { bt0 = V->btab[(size_t)e * V->W + l]; bp0 = V->bprm[(size_t)e * V->W + l]; }
+    const bool fcoll = (mode != 1) && (V->floe_coll[fl] != 0u);
+    const bool act_bond = act_raw || fcoll;   // activity seen by the bond pass (mode 0)
     float fix, fiy, mi;
-    contact_force_group(V, pos, vel, e, px, py, L, g, fix, fiy, mi);
+    contact_force_group(V, set, pos, vel, e, px, py, vx, vy, r, novf, L, g, fix, fiy, mi);
     float fbx, fby, mb;
     bool crack;
-    bond_force_group(V, D, pos, e, mode, L, g, fbx, fby, mb, crack);
+    bond_force_group(V, D, pos, e, px, py, mode == 1 ? act_raw : act_bond, deg, bt0, bp0, mode, L, g,
+                     fbx, fby, mb, crack);
     if (l != 0) return;
-    const float h = V->thick[e];
     float shx, shy, subv;
     bool act;
     if (mode == 1) {
-        act = active[e] != 0;
+        act = act_raw;
         shx = fship_in[e]; shy = fship_in[n + e];
         subv = D->s.d_sub[e];
     } else if (sub == 0) {
-        act = (active[e] != 0) || (V->floe_coll[V->floe[e]] != 0u);
+        act = act_bond;
         D->s.d_active[e] = act ? 1 : 0;
         const bool nr = V->nearf[e] != 0;
         const float pen = nr ? V->pen[e] : 0.0f;
@@ -645,18 +857,17 @@ __device__ void force_body(const DevView* __restrict__ V, const CallDesc* __rest
         D->s.d_sub[e] = ns;
         subv = ns;
         float z = vmin(ns, h);
-        float Fb = P.buoyancy_coef * V->area[e] * z;
+        float Fb = P.buoyancy_coef * area * z;
         shx = sx_ + (hc ? Fb * V->nx[e] : 0.0f);
         shy = sy_ + (hc ? Fb * V->ny[e] : 0.0f);
         if (P.n_ice_steps > 1) { V->shipf[e] = shx; V->shipf[n + e] = shy; }
     } else {
-        act = active[e] != 0;
+        act = act_raw;
         shx = V->shipf[e]; shy = V->shipf[n + e];
         subv = D->s.d_sub[e];
     }
     if (sub == 0 && mode == 0) V->crack[e] = crack ? 1 : 0;
     // drag (:167-192)
-    const float vx = vel[e], vy = vel[n + e];
     float vwx = 0.0f, vwy = 0.0f;
     const float* vwp = D->in.d_v_water;
     if (vwp) {
@@ -664,7 +875,6 @@ __device__ void force_body(const DevView* __restrict__ V, const CallDesc* __rest
         vwx = vwp[base];
         vwy = vwp[base + D->in.vw_stride_c];
     }
-    const float area = V->area[e];
     float A_drag = area * (h / P.drag_ref_thickness);
     float vrx = vx - vwx, vry = vy - vwy;
     float vm = sqrtf(vrx * vrx + vry * vry);
@@ -678,8 +888,6 @@ __device__ void force_body(const DevView* __restrict__ V, const CallDesc* __rest
         fdx = lf * vrx; fdy = lf * vry;
     }
     // rotational drag (:195-212)
-    const float r = V->radius[e];
-    const float om = omp[e];
     float A_sub = area * vmin(subv / vmax(h, 1e-10f), 1.0f);
     float r3 = r * (r * r);
     float Md = P.rot_drag_coef * r3 * om * fabsf(om) * A_sub;
@@ -693,13 +901,12 @@ __device__ void force_body(const DevView* __restrict__ V, const CallDesc* __rest
     // integration (:215-268)
     const float dt = P.dt_ice;
     const float vc = P.velocity_clip, pc = P.position_clip;
-    const float ro = rotp[e];
-    float ax = Fx_t / V->mass[e], ay = Fy_t / V->mass[e];
+    float ax = Fx_t / mass, ay = Fy_t / mass;
     float nvx = vclip(vx + ax * dt, -vc, vc);
     float nvy = vclip(vy + ay * dt, -vc, vc);
     float npx = px + nvx * dt;
     float npy = py + nvy * dt;
-    float alpha = M_t / V->inertia[e];
+    float alpha = M_t / inertia;
     float nw = om * P.angular_damping + alpha * dt;
     nw = vclip(nw, P.ang_cap_lo, P.ang_cap_hi);
     float nr = ro + nw * dt;
@@ -715,19 +922,30 @@ __device__ void force_body(const DevView* __restrict__ V, const CallDesc* __rest
     drot[e] = nr; domg[e] = nw;
 }
 
-
+// Blocks [0, fblk) integrate their elements; with nextra > 0 (a frame's
+// first sub-step) blocks [fblk, fblk + nextra) run force_tail.
 __global__ __launch_bounds__(BLOCK) void k_force(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                                 int k, int sub, int src_is_state, int mode,
+                                                 int k, int sub, int src_is_state, int mode, int set, int last,
+                                                 int fblk, int nextra,
                                                  const float* __restrict__ fship_in,
                                                  float* __restrict__ fout, float* __restrict__ mout) {
     __shared__ ForceLds L;
-    force_body(V, D, k, sub, src_is_state, mode, fship_in, fout, mout, L, blockIdx.x);
+    __shared__ int am_last;
+    if ((int)blockIdx.x >= fblk) {
+        force_tail(V, D, k, last, (int)blockIdx.x - fblk, nextra, &am_last);
+        return;
+    }
+    force_body(V, D, k, sub, src_is_state, mode, set, fship_in, fout, mout, L, blockIdx.x);
 }
 
-// ── k_bond: fracture (jax_simulation.py:534-656), commit, histories, clears ─
+// ── k_post: fracture (jax_simulation.py:534-656), commit, histories, next cull ─
 
-__device__ __forceinline__ void bond_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                          int k, int final_in_nxt, int last, int t) {
+// Thread t of chunk t / 256 (every thread of the 256-thread block calls):
+// bond t, element t, cell t of the set just used, floe t.  With `cull`, the
+// element's committed state also goes through the next frame's near test
+// and into bucket set set ^ 1.
+__device__ __forceinline__ void post_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                          int k, int final_in_nxt, int last, int t, int chunk, int set, int cull) {
     const IceDemParams& P = V->p;
     const int n = V->n, nb = V->nb;
     const float* pos = final_in_nxt ? V->nxt_pos : D->s.d_pos;
@@ -787,11 +1005,13 @@ __device__ __forceinline__ void bond_body(const DevView* __restrict__ V, const C
         D->s.d_bond_intact[t] = fin ? 1 : 0;
         if (D->h.d_bond_intact) D->h.d_bond_intact[(size_t)k * nb + t] = fin ? 1 : 0;
     }
+    float px = 0.0f, py = 0.0f, pvx = 0.0f, pvy = 0.0f;
     if (t < n) {
-        float px = pos[t], py = pos[n + t];
+        px = pos[t]; py = pos[n + t];
+        pvx = vel[t]; pvy = vel[n + t];
         if (final_in_nxt) {
             D->s.d_pos[t] = px; D->s.d_pos[n + t] = py;
-            D->s.d_vel[t] = V->nxt_vel[t]; D->s.d_vel[n + t] = V->nxt_vel[n + t];
+            D->s.d_vel[t] = pvx; D->s.d_vel[n + t] = pvy;
             D->s.d_rot[t] = V->nxt_rot[t]; D->s.d_omega[t] = V->nxt_omega[t];
         }
         if (D->h.d_positions) {
@@ -805,18 +1025,70 @@ __device__ __forceinline__ void bond_body(const DevView* __restrict__ V, const C
             if (D->dg.d_force_mag) D->dg.d_force_mag[t] = V->fm[t];
         }
     }
-    if (t < V->G) V->cell_count[t] = 0;
+    clear_cells(V, set, t);
     if (t < V->nf) { V->floe_coll[t] = 0u; V->floe_maxfm[t] = 0u; }
-    if (t == 0) { *V->near_count = 0; *V->ovf_count = 0; }
+    if (t == 0) *force_ticket(V) = 0;
+    if (cull && chunk < V->nblk)
+        cull_chunk(V, chunk, t < n, t, px, py, pvx, pvy, D->s.d_ship[0], D->s.d_ship[1], set ^ 1);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_bond(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                                int k, int final_in_nxt, int last) {
-    bond_body(V, D, k, final_in_nxt, last, blockIdx.x * BLOCK + threadIdx.x);
+__global__ __launch_bounds__(BLOCK) void k_post(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                                int k, int final_in_nxt, int last, int set, int cull) {
+    post_body(V, D, k, final_in_nxt, last, blockIdx.x * BLOCK + threadIdx.x, blockIdx.x, set, cull);
 }
 
 
 // ── batched rollouts: one workgroup simulates one scene for `frames` frames ──
+
+// Totals, thrust cap (jax_simulation.py:456-479), floe eligibility
+// (:511-524) and the ship step, by one workgroup; la, lb hold nbpad floats.
+__device__ void mid_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D, float* la, float* lb,
+                         int k, int last) {
+    const IceDemParams& P = V->p;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int nb = V->nblk, nbpad = V->nbpad;
+    for (int b = w; b < nb; b += nw) {
+        float a = 0.0f, c = 0.0f, d = 0.0f;
+        if (V->nearcnt[b] > 0) {
+            a = block256_tree(V, V->soi_x, b, 1.0f, false);
+            c = block256_tree(V, V->soi_y, b, 1.0f, false);
+            d = block256_tree(V, V->mterm, b, 1.0f, false);
+        }
+        if (lane == 0) { V->part1[b] = a; V->part1[nbpad + b] = c; V->part1[2 * nbpad + b] = d; }
+    }
+    __syncthreads();
+    float tx = lds_tree_sum(V->part1, nb, nbpad, la, lb);
+    float ty = lds_tree_sum(V->part1 + nbpad, nb, nbpad, la, lb);
+    float tm = lds_tree_sum(V->part1 + 2 * nbpad, nb, nbpad, la, lb);
+    float scale = 1.0f;
+    if (P.has_thrust_cap) {
+        float F_tot = sqrtf(tx * tx + ty * ty);
+        scale = (F_tot > P.max_ship_ice_force) ? (P.max_ship_ice_force / vmax(F_tot, 1e-6f)) : 1.0f;
+    }
+    float fx = tx, fy = ty;
+    if (P.has_thrust_cap) {
+        for (int b = w; b < nb; b += nw) {
+            float a = 0.0f, c = 0.0f;
+            if (V->nearcnt[b] > 0) {
+                a = block256_tree(V, V->soi_x, b, scale, true);
+                c = block256_tree(V, V->soi_y, b, scale, true);
+            }
+            if (lane == 0) { V->part2[b] = a; V->part2[nbpad + b] = c; }
+        }
+        __syncthreads();
+        fx = lds_tree_sum(V->part2, nb, nbpad, la, lb);
+        fy = lds_tree_sum(V->part2 + nbpad, nb, nbpad, la, lb);
+    }
+    if (threadIdx.x == 0) { V->scal[0] = scale; V->scal[1] = tm; }
+    floe_eligibility(V, D, scale);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ship_frame(V, D, k, last, fx, fy, tm);
+        *near_count(V) = 0;
+        *nearblk_count(V) = 0;
+    }
+    __syncthreads();
+}
 
 __global__ __launch_bounds__(BLOCK) void k_scene_rollouts(const DevView* __restrict__ views,
                                                           const CallDesc* __restrict__ descs, int frames) {
@@ -824,20 +1096,35 @@ __global__ __launch_bounds__(BLOCK) void k_scene_rollouts(const DevView* __restr
     __shared__ float la[256], lb[256];
     const DevView* V = views + blockIdx.x;
     const CallDesc* D = descs + blockIdx.x;
-    const int nblk = V->nblk;
-    const int fch = (V->n + EPB - 1) / EPB;
-    const int post = max(max(V->nb, V->n), max(V->G, V->nf));
+    const int n = V->n, nblk = V->nblk;
+    const int fch = (n + EPB - 1) / EPB;
+    const int post = max(max(V->nb, n), max(V->G, V->nf));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int c = 0; c < nblk; ++c) {
+        const int e = c * BLOCK + threadIdx.x;
+        const bool valid = e < n;
+        float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f;
+        if (valid) { x = D->s.d_pos[e]; y = D->s.d_pos[n + e]; vx = D->s.d_vel[e]; vy = D->s.d_vel[n + e]; }
+        cull_chunk(V, c, valid, e, x, y, vx, vy, D->s.d_ship[0], D->s.d_ship[1], 0);
+    }
+    __syncthreads();
     for (int k = 0; k < frames; ++k) {
         const int last = (k == frames - 1) ? 1 : 0;
-        for (int c = 0; c < nblk; ++c) cull_body(V, D, c);
-        __syncthreads();
-        near_body(V, D, threadIdx.x >> 6, BLOCK / 64);
+        const int set = k & 1;
+        const int count = __hip_atomic_load(near_count(V), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (count > 0) {
+            const float* ship = D->s.d_ship;
+            const ShipPose SP = make_pose(ship[0], ship[1], ship[2], V->p.half_len, V->p.half_beam);
+            for (int q = w; q < count; q += BLOCK / 64) near_elem(V, D, SP, V->near_list[q]);
+        }
+        (void)lane;
         __syncthreads();
         mid_body(V, D, la, lb, k, last);
         for (int c = 0; c < fch; ++c)
-            force_body(V, D, k, 0, 1, 0, nullptr, nullptr, nullptr, L, c);
+            force_body(V, D, k, 0, 1, 0, set, nullptr, nullptr, nullptr, L, c);
         __syncthreads();
-        for (int t = threadIdx.x; t < post; t += BLOCK) bond_body(V, D, k, 1, last, t);
+        for (int base = 0; base < post; base += BLOCK)
+            post_body(V, D, k, 1, last, base + threadIdx.x, base / BLOCK, set, last ? 0 : 1);
         __syncthreads();
     }
 }

@@ -162,31 +162,54 @@ __global__ __launch_bounds__(1024) void k_excl_scan(int m, const int32_t* in, in
     if (tid == T - 1) out[m] = lds[T - 1];
 }
 
+// Static per-entry bond parameters of the bond table: rest length and the
+// stiffness E * (h_i + h_j) / 2 * factor of jax_simulation.py:63-108, with
+// the reference's operation order.
+__global__ void k_bond_prm(const DevView* __restrict__ V) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int W = V->W;
+    if (W <= 0 || idx >= (size_t)V->n * W) return;
+    const int e = (int)(idx / W), q = (int)(idx % W);
+    float2 out = make_float2(0.0f, 0.0f);
+    if (q < V->bdeg[e]) {
+        const int b = V->btab[idx].x;
+        const int i = V->bi[b], j = V->bj[b];
+        const float avg_h = (V->thick[i] + V->thick[j]) * 0.5f;
+        const float stiff = V->p.youngs_modulus * avg_h * V->p.bond_stiffness_factor;
+        out = make_float2(V->rest[b], stiff);
+    }
+    const_cast<float2*>(V->bprm)[idx] = out;
+}
+
 // Contact pairs (e < j) of element e in ascending j, by repeated selection
-// over the scanned cells (duplicates of big elements fall out).
+// over the scanned cells of both grids and the overflow list.
 __device__ int pairs_of(const DevView* __restrict__ V, const float* __restrict__ pos, int e,
                         int32_t* pi, int32_t* pj, int64_t o, int64_t max_pairs) {
     const IceDemParams& P = V->p;
     const int n = V->n, cap = P.grid_bucket;
-    int x0, x1, y0, y1;
-    scan_range(V, e, pos[e], pos[n + e], x0, x1, y0, y1);
-    const int novf = *V->ovf_count;
+    int x0, x1, y0, y1, u0 = 0, u1 = -1, v0 = 0, v1 = -1;
+    scan_range(P, V->radius[e], pos[e], pos[n + e], x0, x1, y0, y1);
+    if (V->nbig > 0) scan_range2(V, pos[e], pos[n + e], u0, u1, v0, v1);
+    const int novf = V->ovf_count[0];
     int last = e, c = 0;
     for (;;) {
         int best = 0x7fffffff;
+        auto consider = [&](int j) {
+            if (j > last && j < best && pair_overlap(P, pos, V->radius, n, e, j)) best = j;
+        };
         for (int gy = y0; gy <= y1; ++gy)
             for (int gx = x0; gx <= x1; ++gx) {
                 const int cell = gy * P.grid_nx + gx;
                 const int cc = min(V->cell_count[cell], cap);
-                for (int s = 0; s < cc; ++s) {
-                    const int j = V->cell_slots[(size_t)cell * cap + s];
-                    if (j > last && j < best && pair_overlap(P, pos, V->radius, n, e, j)) best = j;
-                }
+                for (int s = 0; s < cc; ++s) consider(V->cell_id[(size_t)cell * cap + s].x);
             }
-        for (int s = 0; s < novf; ++s) {
-            const int j = V->ovf_list[s];
-            if (j > last && j < best && pair_overlap(P, pos, V->radius, n, e, j)) best = j;
-        }
+        for (int gy = v0; gy <= v1; ++gy)
+            for (int gx = u0; gx <= u1; ++gx) {
+                const int cell = gy * V->nx2 + gx;
+                const int cc = min(V->cell2_count[cell], V->cap2);
+                for (int s = 0; s < cc; ++s) consider(V->cell2_id[(size_t)cell * V->cap2 + s].x);
+            }
+        for (int s = 0; s < novf; ++s) consider(V->ovf_list[s]);
         if (best == 0x7fffffff) break;
         if (pi && o + c < max_pairs) { pi[o + c] = e; pj[o + c] = best; }
         last = best;
@@ -342,17 +365,24 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     if (prm->grid_bucket < 1 || (int64_t)H.G * prm->grid_bucket > (1LL << 28)) {
         set_err(nullptr, "bad grid_bucket"); delete ctx; return ICEDEM_E_INVALID;
     }
-    {   // bound on cell insertions: small elements once, big ones per covered cell
-        int64_t cap = 0;
+    {   // coarse grid for the elements above grid_rsmall: cell2 >= 2 r_max + delta
+        double rmax = 0.0;
+        int nbig = 0;
         for (int e = 0; e < n; ++e) {
             const float r = st->h_radii[e];
-            if (!(r > prm->grid_rsmall)) { ++cap; continue; }
-            const double reach = ((double)r + prm->grid_rsmall + prm->contact_dilation2) * 1.001 + 1e-3;
-            const int64_t w = (int64_t)std::floor(2.0 * reach / prm->grid_cell) + 3;
-            cap += std::min<int64_t>(w * w, (int64_t)H.G);
+            if (!(r >= 0.0f) || !std::isfinite(r)) { set_err(nullptr, "bad radius"); delete ctx; return ICEDEM_E_INVALID; }
+            rmax = std::max(rmax, (double)r);
+            if (r > prm->grid_rsmall) ++nbig;
         }
-        if (cap > (1LL << 30)) { set_err(nullptr, "cell insertions beyond 2^30"); delete ctx; return ICEDEM_E_LIMIT; }
-        H.ovf_cap = (int32_t)cap + 64;
+        const double c2 = std::max((double)prm->grid_cell, (2.0 * rmax + prm->contact_dilation2) * 1.001 + 1e-3);
+        H.cell2 = (float)c2;
+        if ((double)H.cell2 < c2) H.cell2 = std::nextafter(H.cell2, INFINITY);
+        H.nx2 = std::max(1, (int)std::ceil((double)prm->grid_nx * prm->grid_cell / H.cell2));
+        H.ny2 = std::max(1, (int)std::ceil((double)prm->grid_ny * prm->grid_cell / H.cell2));
+        H.G2 = H.nx2 * H.ny2;
+        H.nbig = nbig;
+        H.cap2 = std::max(1, std::min(nbig, 8));
+        H.ovf_cap = n + 64;   // every element is inserted once
     }
     const size_t nv = (size_t)n * st->max_v;
     int rc = 0;
@@ -373,42 +403,65 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     UP(bstr, float, st->h_bond_strengths, st->n_bonds);
     UP(sbar, float, st->h_sigma_bar, st->n_bonds);
     UP(hull, float2, st->h_hull_local, st->n_hull);
-    // element -> bond CSR in the reference's scatter order
+    // element-major bond table in the reference's scatter order: bonds where
+    // the element is i (ascending), then bonds where it is j (ascending)
     {
-        std::vector<int32_t> cnt(n + 1, 0), start(n + 1, 0), ent(2 * (size_t)st->n_bonds);
-        for (int b = 0; b < st->n_bonds; ++b) { cnt[st->h_bond_i[b]]++; cnt[st->h_bond_j[b]]++; }
-        for (int e = 0; e < n; ++e) start[e + 1] = start[e] + cnt[e];
-        std::vector<int32_t> fill(start.begin(), start.end() - 1);
-        for (int b = 0; b < st->n_bonds; ++b) ent[fill[st->h_bond_i[b]]++] = (b << 1);
-        for (int b = 0; b < st->n_bonds; ++b) ent[fill[st->h_bond_j[b]]++] = (b << 1) | 1;
-        UP(csr_start, int32_t, start.data(), n + 1);
-        UP(csr_ent, int32_t, ent.data(), ent.size());
+        std::vector<int32_t> deg(n, 0);
+        for (int b = 0; b < st->n_bonds; ++b) { deg[st->h_bond_i[b]]++; deg[st->h_bond_j[b]]++; }
+        int W = 0;
+        for (int e = 0; e < n; ++e) W = std::max(W, deg[e]);
+        if ((int64_t)n * W > (1LL << 30)) {
+            set_err(nullptr, "bond table beyond 2^30 entries"); icedem_destroy(ctx); return ICEDEM_E_LIMIT;
+        }
+        H.W = W;
+        const size_t tw = std::max<size_t>(1, (size_t)n * W);
+        std::vector<int4> tab(tw, make_int4(0, 0, 0, 0));
+        std::vector<int32_t> fill(n, 0);
+        for (int b = 0; b < st->n_bonds; ++b) {
+            const int i = st->h_bond_i[b], j = st->h_bond_j[b];
+            tab[(size_t)i * W + fill[i]++] = make_int4(b, j, st->h_floe_of[j], 0);
+        }
+        for (int b = 0; b < st->n_bonds; ++b) {
+            const int i = st->h_bond_i[b], j = st->h_bond_j[b];
+            tab[(size_t)j * W + fill[j]++] = make_int4(b, i, st->h_floe_of[i], 1);
+        }
+        UP(bdeg, int32_t, deg.data(), n);
+        UP(btab, int4, tab.data(), tw);
+        AL(bprm, float2, tw);
     }
     AL(pen, float, n); AL(nx, float, n); AL(ny, float, n); AL(fm_raw, float, n);
     AL(soi_x, float, n); AL(soi_y, float, n); AL(mterm, float, n); AL(fm, float, n);
     AL(shipf, float, 2 * (size_t)n);
-    AL(nearf, uint8_t, n); AL(near_list, int32_t, n); AL(near_count, int32_t, 1); AL(nearblk, uint8_t, nblk);
+    AL(nearf, uint8_t, n); AL(near_list, int32_t, n);
+    AL(nearcnt, int32_t, nblk); AL(blkdone, int32_t, nblk); AL(counters, int32_t, 4);
     AL(crack, uint8_t, n);
-    AL(cell_count, int32_t, H.G); AL(cell_slots, int32_t, (size_t)H.G * prm->grid_bucket);
-    AL(ovf_list, int32_t, H.ovf_cap); AL(ovf_count, int32_t, 1); AL(ticket, int32_t, 1);
+    {
+        const size_t slots = 2 * (size_t)H.G * prm->grid_bucket;
+        AL(cell_count, int32_t, 2 * (size_t)H.G); AL(cell_kin, float4, slots); AL(cell_id, int2, slots);
+        const size_t slots2 = 2 * (size_t)H.G2 * H.cap2;
+        AL(cell2_count, int32_t, 2 * (size_t)H.G2); AL(cell2_kin, float4, slots2); AL(cell2_id, int2, slots2);
+    }
+    AL(ovf_list, int32_t, 2 * (size_t)H.ovf_cap); AL(ovf_count, int32_t, 2);
     AL(floe_coll, uint32_t, H.nf); AL(floe_maxfm, uint32_t, H.nf);
-    AL(part1, float, 3 * (size_t)nbpad); AL(part2, float, 2 * (size_t)nbpad); AL(scal, float, 4);
+    AL(part1, float, 3 * (size_t)nbpad); AL(part2, float, 2 * (size_t)nbpad); AL(scal, float, 8);
     AL(nxt_pos, float, 2 * (size_t)n); AL(nxt_vel, float, 2 * (size_t)n);
     AL(nxt_rot, float, n); AL(nxt_omega, float, n);
     AL(err, int32_t, 4);
 #undef UP
 #undef AL
-    HIP_TRY(hipMemset(H.cell_count, 0, sizeof(int32_t) * H.G));
-    HIP_TRY(hipMemset(H.ovf_count, 0, sizeof(int32_t)));
-    HIP_TRY(hipMemset(H.ticket, 0, sizeof(int32_t)));
+    HIP_TRY(hipMemset(H.cell_count, 0, sizeof(int32_t) * 2 * (size_t)H.G));
+    HIP_TRY(hipMemset(H.cell2_count, 0, sizeof(int32_t) * 2 * (size_t)H.G2));
+    HIP_TRY(hipMemset(H.ovf_count, 0, sizeof(int32_t) * 2));
+    HIP_TRY(hipMemset(H.counters, 0, sizeof(int32_t) * 4));
     HIP_TRY(hipMemset(H.floe_coll, 0, sizeof(uint32_t) * std::max(H.nf, 1)));
     HIP_TRY(hipMemset(H.floe_maxfm, 0, sizeof(uint32_t) * std::max(H.nf, 1)));
     HIP_TRY(hipMemset(H.part1, 0, sizeof(float) * 3 * nbpad));
     HIP_TRY(hipMemset(H.part2, 0, sizeof(float) * 2 * nbpad));
+    HIP_TRY(hipMemset(H.scal, 0, sizeof(float) * 8));
     HIP_TRY(hipMemset(H.err, 0, sizeof(int32_t) * 4));
-    HIP_TRY(hipMemset(H.near_count, 0, sizeof(int32_t)));
     HIP_TRY(hipMemset(H.nearf, 0, n));
-    HIP_TRY(hipMemset(H.nearblk, 0, nblk));
+    HIP_TRY(hipMemset(H.nearcnt, 0, sizeof(int32_t) * nblk));
+    HIP_TRY(hipMemset(H.blkdone, 0, sizeof(int32_t) * nblk));
     {
         DevView* d;
         rc = dalloc(ctx, &d, 1);
@@ -420,6 +473,11 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
         ctx->dd = cd;
     }
     HIP_TRY(hipMemcpy(ctx->dv, &H, sizeof(DevView), hipMemcpyHostToDevice));
+    if ((size_t)n * H.W > 0) {
+        const size_t tw = (size_t)n * H.W;
+        hipLaunchKernelGGL(k_bond_prm, dim3((unsigned)((tw + 255) / 256)), dim3(256), 0, 0, (const DevView*)ctx->dv);
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipHostMalloc((void**)&ctx->hd, sizeof(CallDesc), hipHostMallocDefault));
     memset(ctx->hd, 0, sizeof(CallDesc));
     HIP_TRY(hipEventCreateWithFlags(&ctx->desc_done, hipEventDisableTiming));
@@ -448,8 +506,9 @@ static int put_desc(IceDemCtx* ctx, const CallDesc& cd, hipStream_t stream) {
     return ICEDEM_OK;
 }
 
-// ev (optional, profiling): events bracket k_cull, k_near (with the totals
-// and the ship step), k_force (all sub-steps with their cell rebuilds), k_bond.
+// ev (optional, profiling): per frame, events bracket k_near, k_force (all
+// sub-steps with their cell rebuilds) and k_post; acc[3] takes the call's
+// k_cull.
 constexpr int NPROF = 4;
 static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, hipEvent_t* ev = nullptr, float* acc = nullptr) {
     const DevView& H = ctx->hv;
@@ -457,40 +516,52 @@ static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, hipEvent_t* 
     const CallDesc* D = ctx->dd;
     const int nblk = H.nblk;
     const int fblk = (H.n + EPB - 1) / EPB;
+    const int nextra = H.p.has_thrust_cap ? nblk : 1;
     const int near_blocks = std::max(1, std::min(256, (H.n + 3) / 4));
     const int steps = H.p.n_ice_steps;
-    const int post = std::max(std::max(H.nb, H.n), std::max(H.G, H.nf));
+    const int post = std::max(std::max(H.nb, H.n), std::max(std::max(H.G, H.G2), H.nf));
     const int nbpost = (post + BLOCK - 1) / BLOCK;
-    const int gcl = (H.G + BLOCK - 1) / BLOCK;
+    const int gcl = (std::max(H.G, H.G2) + BLOCK - 1) / BLOCK;
     auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], stream); };
+    auto collect = [&](int first, int count, int slot0) -> int {
+        if (!ev) return ICEDEM_OK;
+        HIP_TRY(hipEventSynchronize(ev[first + count]));
+        for (int i = 0; i < count; ++i) {
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, ev[first + i], ev[first + i + 1]));
+            acc[slot0 + i] += ms;
+        }
+        return ICEDEM_OK;
+    };
+    mark(0);
+    hipLaunchKernelGGL(k_cull, dim3(nblk), dim3(BLOCK), 0, stream, V, D);
+    mark(1);
+    if (collect(0, 1, 3)) return ICEDEM_E_HIP;
     for (int k = 0; k < K; ++k) {
         const int last = (k == K - 1) ? 1 : 0;
+        const int set = k & 1;
         mark(0);
-        hipLaunchKernelGGL(k_cull, dim3(nblk), dim3(BLOCK), 0, stream, V, D);
+        hipLaunchKernelGGL(k_near, dim3(near_blocks), dim3(BLOCK), 0, stream, V, D);
         mark(1);
-        hipLaunchKernelGGL(k_near, dim3(near_blocks), dim3(BLOCK), 0, stream, V, D, k, last);
-        mark(2);
         for (int s = 0; s < steps; ++s) {
             if (s > 0) {
                 // rebuild the cell buckets for the sub-step's positions
-                const float* sp = (s % 2 == 1) ? (const float*)H.nxt_pos : (const float*)nullptr;
-                hipLaunchKernelGGL(k_clear_cells, dim3(gcl), dim3(BLOCK), 0, stream, V);
-                hipLaunchKernelGGL(k_cellkey, dim3(nblk), dim3(BLOCK), 0, stream, V, D, sp, (s % 2 == 0) ? 1 : 0);
+                const bool odd = (s % 2 == 1);
+                const float* sp = odd ? (const float*)H.nxt_pos : (const float*)nullptr;
+                const float* sv = odd ? (const float*)H.nxt_vel : (const float*)nullptr;
+                hipLaunchKernelGGL(k_clear_cells, dim3(gcl), dim3(BLOCK), 0, stream, V, set);
+                hipLaunchKernelGGL(k_cellkey, dim3(nblk), dim3(BLOCK), 0, stream, V, D, sp, sv, odd ? 0 : 1, set);
             }
-            hipLaunchKernelGGL(k_force, dim3(fblk), dim3(BLOCK), 0, stream, V, D, k, s, (s % 2 == 0) ? 1 : 0, 0,
+            const int extra = (s == 0) ? nextra : 0;
+            hipLaunchKernelGGL(k_force, dim3(fblk + extra), dim3(BLOCK), 0, stream, V, D, k, s,
+                               (s % 2 == 0) ? 1 : 0, 0, set, last, fblk, extra,
                                (const float*)nullptr, (float*)nullptr, (float*)nullptr);
         }
+        mark(2);
+        hipLaunchKernelGGL(k_post, dim3(nbpost), dim3(BLOCK), 0, stream, V, D, k, (steps % 2 == 1) ? 1 : 0, last,
+                           set, last ? 0 : 1);
         mark(3);
-        hipLaunchKernelGGL(k_bond, dim3(nbpost), dim3(BLOCK), 0, stream, V, D, k, (steps % 2 == 1) ? 1 : 0, last);
-        mark(4);
-        if (ev) {
-            HIP_TRY(hipEventSynchronize(ev[NPROF]));
-            for (int i = 0; i < NPROF; ++i) {
-                float ms = 0.0f;
-                HIP_TRY(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
-                acc[i] += ms;
-            }
-        }
+        if (collect(0, 3, 0)) return ICEDEM_E_HIP;
     }
     HIP_TRY(hipGetLastError());
     return ICEDEM_OK;
@@ -573,7 +644,6 @@ struct IceDemBatch {
 };
 
 int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** out) {
-    IceDemCtx* ctx = nullptr;
     if (!ctxs || n_inst < 1 || !out) { set_err(nullptr, "bad argument"); return ICEDEM_E_INVALID; }
     *out = nullptr;
     for (int i = 0; i < n_inst; ++i) {
@@ -693,11 +763,12 @@ int icedem_substep_forces(IceDemCtx* ctx, const IceDemState* state, const float*
     rc = put_desc(ctx, cd, stream);
     if (rc) return rc;
     const DevView& H = ctx->hv;
-    hipLaunchKernelGGL(k_clear_cells, dim3((H.G + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, ctx->dv);
-    hipLaunchKernelGGL(k_cellkey, dim3(H.nblk), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, (const float*)state->d_pos, 0);
-    hipLaunchKernelGGL(k_force, dim3((H.n + EPB - 1) / EPB), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, 0, 0, 1, 1,
+    const int fblk = (H.n + EPB - 1) / EPB;
+    hipLaunchKernelGGL(k_cellkey, dim3(H.nblk), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, (const float*)nullptr,
+                       (const float*)nullptr, 1, 0);
+    hipLaunchKernelGGL(k_force, dim3(fblk), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, 0, 0, 1, 1, 0, 0, fblk, 0,
                        d_ship_forces, d_forces, d_moments);
-    hipLaunchKernelGGL(k_clear_cells, dim3((H.G + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, ctx->dv);
+    hipLaunchKernelGGL(k_clear_cells, dim3((std::max(H.G, H.G2) + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, ctx->dv, 0);
     HIP_TRY(hipGetLastError());
     return ICEDEM_OK;
 }
@@ -714,13 +785,12 @@ int icedem_contact_pairs(IceDemCtx* ctx, const float* d_pos, int32_t* d_pair_i, 
     int32_t* off = nullptr;
     HIP_TRY(hipMallocAsync((void**)&cnt, sizeof(int32_t) * (H.n + 1), stream));
     HIP_TRY(hipMallocAsync((void**)&off, sizeof(int32_t) * (H.n + 1), stream));
-    hipLaunchKernelGGL(k_clear_cells, dim3((H.G + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, ctx->dv);
-    hipLaunchKernelGGL(k_cellkey, dim3(H.nblk), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, d_pos, 0);
+    hipLaunchKernelGGL(k_cellkey, dim3(H.nblk), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, d_pos, d_pos, 0, 0);
     hipLaunchKernelGGL(k_pair_count, dim3(H.nblk), dim3(BLOCK), 0, stream, ctx->dv, d_pos, cnt);
     hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, stream, H.n, cnt, off);
     hipLaunchKernelGGL(k_pair_emit, dim3(H.nblk), dim3(BLOCK), 0, stream, ctx->dv, d_pos, off, max_pairs,
                        d_pair_i, d_pair_j);
-    hipLaunchKernelGGL(k_clear_cells, dim3((H.G + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, ctx->dv);
+    hipLaunchKernelGGL(k_clear_cells, dim3((std::max(H.G, H.G2) + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, ctx->dv, 0);
     HIP_TRY(hipGetLastError());
     int32_t total = 0;
     HIP_TRY(hipMemcpyAsync(&total, off + H.n, sizeof(int32_t), hipMemcpyDeviceToHost, stream));

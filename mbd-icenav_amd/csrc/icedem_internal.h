@@ -23,25 +23,40 @@ struct DevView {
     const int32_t* bi; const int32_t* bj;
     const float* rest; const float* bstr; const float* sbar;
     const float2* hull;       // (S)
-    const int32_t* csr_start; // (n+1) bonds of each element: i-role then j-role
-    const int32_t* csr_ent;   // (2 nb) entry = bond << 1 | role (0 = i, 1 = j)
+    // bonds of each element, element-major with fixed width W, in the
+    // reference's scatter order (bonds where the element is i, ascending,
+    // then bonds where it is j, ascending)
+    int32_t W;
+    const int32_t* bdeg;      // (n) entries of each element
+    const int4* btab;         // (n W) bond, other element, floe of other, role (0 = i, 1 = j)
+    const float2* bprm;       // (n W) rest length, stiffness E * h_avg * factor
     // per-frame scratch
     float* pen; float* nx; float* ny; float* fm_raw; float* soi_x; float* soi_y; float* mterm; float* fm;
     uint8_t* nearf;           // (n) element passed the hull cull this frame
     int32_t* near_list;       // (n) queue of near elements
-    int32_t* near_count;      // (1)
-    uint8_t* nearblk;         // (nblk) block of 256 elements holds a near element
+    int32_t* nearcnt;         // (nblk) near elements of each 256-element block
+    int32_t* blkdone;         // (nblk) near elements of the block that k_near has finished
+    int32_t* counters;        // [0] near elements, [1] blocks holding one, [2] k_near ticket, [3] k_force ticket
     float* shipf;             // (2n) ship forces on ice, kept for sub-steps > 0
     uint8_t* crack;           // (n) element touches a broken bond
-    int32_t* cell_count;      // (G) elements inserted per cell this frame
-    int32_t* cell_slots;      // (G * grid_bucket) bucket contents
-    int32_t* ovf_list;        // (ovf_cap) insertions that found their bucket full
-    int32_t* ovf_count;       // (1)
-    int32_t* ticket;          // (1) workgroups of k_near that have finished
+    // Two grids, each with two bucket sets (frame parity): set s, cell c,
+    // slot q at (s G + c) cap + q.  The fine grid (params grid_*) holds the
+    // elements of radius <= grid_rsmall; the coarse grid (cell2 >= 2 r_max +
+    // delta, same origin) holds the larger ones.
+    int32_t* cell_count;      // (2 G) insertions per fine cell
+    float4* cell_kin;         // (2 G cap) x, y, vx, vy of the inserted element
+    int2* cell_id;            // (2 G cap) element index, radius bits
+    int32_t nbig, nx2, ny2, G2, cap2;
+    float cell2;
+    int32_t* cell2_count;     // (2 G2)
+    float4* cell2_kin;        // (2 G2 cap2)
+    int2* cell2_id;           // (2 G2 cap2)
+    int32_t* ovf_list;        // (2 ovf_cap) insertions that found their bucket full
+    int32_t* ovf_count;       // (2)
     uint32_t* floe_coll; uint32_t* floe_maxfm;                      // (nf)
     float* part1;             // (3 * nbpad) block partials: sum soi_x, soi_y, moment
     float* part2;             // (2 * nbpad) block partials: sum of scaled soi
-    float* scal;              // [0] thrust scale, [1] M_ice_on_ship, [2..3] sum of scaled ship forces
+    float* scal;              // [0] thrust scale, [1] M_ice_on_ship, [4..5] unscaled sum of soi
     float* nxt_pos; float* nxt_vel; float* nxt_rot; float* nxt_omega;
     int32_t* err;
 };
