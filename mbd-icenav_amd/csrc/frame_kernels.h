@@ -26,7 +26,7 @@ namespace icedem {
 
 constexpr int GRP = 16;                 // lanes per element in k_force
 constexpr int EPB = BLOCK / GRP;        // elements per k_force block
-constexpr int MAXC2 = 48;               // contact candidates kept per element
+constexpr int MAXC2 = 32;               // contact partners kept per element (more: exact slow path)
 
 __device__ __forceinline__ int32_t* near_count(const DevView* V) { return V->counters; }
 __device__ __forceinline__ int32_t* nearblk_count(const DevView* V) { return V->counters + 1; }
@@ -371,7 +371,7 @@ __device__ __forceinline__ void near_elem(const DevView* __restrict__ V, const C
 // The near elements of block b are all done: its partials, and for the last
 // such block the totals and the thrust scale (jax_simulation.py:456-470).
 // One wave.
-__device__ void near_block_done(const DevView* __restrict__ V, int b) {
+__device__ __forceinline__ void near_block_done(const DevView* __restrict__ V, int b) {
     const IceDemParams& P = V->p;
     const int lane = threadIdx.x & 63, nbpad = V->nbpad;
     __threadfence();
@@ -501,7 +501,7 @@ __device__ __forceinline__ void floe_eligibility(const DevView* __restrict__ V, 
 // Extra workgroup b of k_force: the scaled partials of element block b; the
 // last of the nextra workgroups forms the ship reaction, the floe
 // eligibility and the ship step, and resets the near counters.
-__device__ void force_tail(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int k, int last,
+__device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int k, int last,
                            int b, int nextra, int* am_last) {
     const IceDemParams& P = V->p;
     const int nbpad = V->nbpad;
@@ -547,9 +547,7 @@ __device__ void force_tail(const DevView* __restrict__ V, const CallDesc* __rest
 struct ForceLds {
     int cj[EPB][MAXC2];
     float cx[EPB][MAXC2], cy[EPB][MAXC2], cvx[EPB][MAXC2], cvy[EPB][MAXC2], cr[EPB][MAXC2];
-    int keys[EPB][MAXC2];
     int ord[EPB][MAXC2];
-    float cfx[EPB][MAXC2], cfy[EPB][MAXC2], cfm[EPB][MAXC2];
     int gcell[EPB][GRP];
     int goff[EPB][GRP];
     float bf[EPB][3][GRP];
@@ -620,7 +618,7 @@ __device__ __forceinline__ void scan_grid(const int32_t* __restrict__ cnt, const
 // reference's scatter order; result valid in the group's lane 0.
 // (px, py, pvx, pvy, pr) is e's own state; partners come from bucket set
 // `set` (and, when buckets overflowed, from pos/vel/radius).
-__device__ void contact_force_group(const DevView* __restrict__ V, int set, const float* __restrict__ pos,
+__device__ __forceinline__ void contact_force_group(const DevView* __restrict__ V, int set, const float* __restrict__ pos,
                                     const float* __restrict__ vel, int e, float px, float py, float pvx,
                                     float pvy, float pr, int novf, ForceLds& L, int g,
                                     float& fx, float& fy, float& fm) {
@@ -669,28 +667,39 @@ __device__ void contact_force_group(const DevView* __restrict__ V, int set, cons
     }
     fx = 0.0f; fy = 0.0f; fm = 0.0f;
     if (count <= MAXC2) {
-        // every partner was seen once: sort by rotated partner index
-        for (int i = l; i < count; i += GRP) L.keys[g][i] = rot_key(L.cj[g][i], e, n);
-        wave_sync_lds();
+        // every partner was seen once: rank by rotated partner index
         for (int i = l; i < count; i += GRP) {
-            const int ki = L.keys[g][i];
+            const int ki = rot_key(L.cj[g][i], e, n);
             int rank = 0;
-            for (int q = 0; q < count; ++q) rank += (L.keys[g][q] < ki) ? 1 : 0;
+            for (int q = 0; q < count; ++q) rank += (rot_key(L.cj[g][q], e, n) < ki) ? 1 : 0;
             L.ord[g][rank] = i;
         }
         wave_sync_lds();
-        for (int i = l; i < count; i += GRP) {
-            const int s = L.ord[g][i];
-            const int j = L.cj[g][s];
-            const float x = L.cx[g][s], y = L.cy[g][s], vx = L.cvx[g][s], vy = L.cvy[g][s], r = L.cr[g][s];
-            float a, b, c;
-            if (e < j) contrib_v(P, true, px, py, pvx, pvy, pr, x, y, vx, vy, r, a, b, c);
-            else contrib_v(P, false, x, y, vx, vy, r, px, py, pvx, pvy, pr, a, b, c);
-            L.cfx[g][i] = a; L.cfy[g][i] = b; L.cfm[g][i] = c;
+        // contributions in rank order, written over the partner arrays once
+        // every lane has read its inputs; lane 0 adds them in order
+        constexpr int PER = MAXC2 / GRP;
+        float ca[PER], cb[PER], cc[PER];
+        #pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = l + u * GRP;
+            ca[u] = 0.0f; cb[u] = 0.0f; cc[u] = 0.0f;
+            if (i < count) {
+                const int s = L.ord[g][i];
+                const int j = L.cj[g][s];
+                const float x = L.cx[g][s], y = L.cy[g][s], vx = L.cvx[g][s], vy = L.cvy[g][s], r = L.cr[g][s];
+                if (e < j) contrib_v(P, true, px, py, pvx, pvy, pr, x, y, vx, vy, r, ca[u], cb[u], cc[u]);
+                else contrib_v(P, false, x, y, vx, vy, r, px, py, pvx, pvy, pr, ca[u], cb[u], cc[u]);
+            }
+        }
+        wave_sync_lds();
+        #pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = l + u * GRP;
+            if (i < count) { L.cx[g][i] = ca[u]; L.cy[g][i] = cb[u]; L.cvx[g][i] = cc[u]; }
         }
         wave_sync_lds();
         if (l == 0)
-            for (int i = 0; i < count; ++i) { fx = fx + L.cfx[g][i]; fy = fy + L.cfy[g][i]; fm = fm + L.cfm[g][i]; }
+            for (int i = 0; i < count; ++i) { fx = fx + L.cx[g][i]; fy = fy + L.cy[g][i]; fm = fm + L.cvx[g][i]; }
         wave_sync_lds();
         return;
     }
@@ -737,7 +746,7 @@ __device__ void contact_force_group(const DevView* __restrict__ V, int set, cons
 // order of the reference's two scatter-add passes).  act_e is e's activity
 // as the bond pass sees it; (bt0, bp0) is the lane's first table entry,
 // read ahead by the caller.  Result valid in lane 0.
-__device__ void bond_force_group(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+__device__ __forceinline__ void bond_force_group(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                  const float* __restrict__ pos, int e, float px, float py, bool act_e, int deg,
                                  int4 bt0, float2 bp0, int mode, ForceLds& L, int g,
                                  float& fbx, float& fby, float& mb, bool& crack) {
@@ -796,7 +805,7 @@ __device__ void bond_force_group(const DevView* __restrict__ V, const CallDesc* 
 
 // elements [chunk*16, chunk*16+16), 16 lanes each (no block barrier inside).
 // mode 0: a frame sub-step (integrates); mode 1: forces out (fout, mout).
-__device__ void force_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+__device__ __forceinline__ void force_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                            int k, int sub, int src_is_state, int mode, int set,
                            const float* __restrict__ fship_in, float* __restrict__ fout, float* __restrict__ mout,
                            ForceLds& L, int chunk) {
@@ -826,6 +835,19 @@ __device__ void force_body(const DevView* __restrict__ V, const CallDesc* __rest
     int4 bt0 = make_int4(0, 0, 0, 0);
     float2 bp0 = make_float2(0.0f, 0.0f);
     if (l < V->W) { bt0 = V->btab[(size_t)e * V->W + l]; bp0 = V->bprm[(size_t)e * V->W + l]; }
+    const float sub0 = D->s.d_sub[e];
+    const bool first = (mode == 0 && sub == 0);
+    const bool is_near = first && V->nearf[e] != 0;
+    float pen = 0.0f, fmr = 0.0f, sx_ = 0.0f, sy_ = 0.0f, nxe = 0.0f, nye = 0.0f;
+    if (is_near) { pen = V->pen[e]; fmr = V->fm_raw[e]; sx_ = V->soi_x[e]; sy_ = V->soi_y[e]; nxe = V->nx[e]; nye = V->ny[e]; }
+    const float tscale = first ? V->scal[0] : 1.0f;
+    float vwx = 0.0f, vwy = 0.0f;
+    const float* vwp = D->in.d_v_water;
+    if (vwp) {
+        const int64_t base = (int64_t)k * D->in.vw_stride_k + (int64_t)e * D->in.vw_stride_e;
+        vwx = vwp[base];
+        vwy = vwp[base + D->in.vw_stride_c];
+    }
     const bool fcoll = (mode != 1) && (V->floe_coll[fl] != 0u);
     const bool act_bond = act_raw || fcoll;   // activity seen by the bond pass (mode 0)
     float fix, fiy, mi;
@@ -840,17 +862,12 @@ __device__ void force_body(const DevView* __restrict__ V, const CallDesc* __rest
     if (mode == 1) {
         act = act_raw;
         shx = fship_in[e]; shy = fship_in[n + e];
-        subv = D->s.d_sub[e];
+        subv = sub0;
     } else if (sub == 0) {
         act = act_bond;
         D->s.d_active[e] = act ? 1 : 0;
-        const bool nr = V->nearf[e] != 0;
-        const float pen = nr ? V->pen[e] : 0.0f;
-        const float fmr = nr ? V->fm_raw[e] : 0.0f;
-        const float sx_ = nr ? V->soi_x[e] : 0.0f, sy_ = nr ? V->soi_y[e] : 0.0f;
-        V->fm[e] = P.has_thrust_cap ? fmr * V->scal[0] : fmr;
+        V->fm[e] = P.has_thrust_cap ? fmr * tscale : fmr;
         const bool hc = pen > 0.0f;
-        const float sub0 = D->s.d_sub[e];
         const float pen_c = vmin(pen, P.submergence_pen_cap);
         float ns = hc ? vmin(sub0 + P.submergence_rate * pen_c * P.dt, h)
                       : vmax(0.0f, sub0 - P.inv_rise_time * sub0 * P.dt);
@@ -858,23 +875,16 @@ __device__ void force_body(const DevView* __restrict__ V, const CallDesc* __rest
         subv = ns;
         float z = vmin(ns, h);
         float Fb = P.buoyancy_coef * area * z;
-        shx = sx_ + (hc ? Fb * V->nx[e] : 0.0f);
-        shy = sy_ + (hc ? Fb * V->ny[e] : 0.0f);
+        shx = sx_ + (hc ? Fb * nxe : 0.0f);
+        shy = sy_ + (hc ? Fb * nye : 0.0f);
         if (P.n_ice_steps > 1) { V->shipf[e] = shx; V->shipf[n + e] = shy; }
     } else {
         act = act_raw;
         shx = V->shipf[e]; shy = V->shipf[n + e];
-        subv = D->s.d_sub[e];
+        subv = sub0;
     }
     if (sub == 0 && mode == 0) V->crack[e] = crack ? 1 : 0;
     // drag (:167-192)
-    float vwx = 0.0f, vwy = 0.0f;
-    const float* vwp = D->in.d_v_water;
-    if (vwp) {
-        const int64_t base = (int64_t)k * D->in.vw_stride_k + (int64_t)e * D->in.vw_stride_e;
-        vwx = vwp[base];
-        vwy = vwp[base + D->in.vw_stride_c];
-    }
     float A_drag = area * (h / P.drag_ref_thickness);
     float vrx = vx - vwx, vry = vy - vwy;
     float vm = sqrtf(vrx * vrx + vry * vry);
@@ -922,20 +932,21 @@ __device__ void force_body(const DevView* __restrict__ V, const CallDesc* __rest
     drot[e] = nr; domg[e] = nw;
 }
 
-// Blocks [0, fblk) integrate their elements; with nextra > 0 (a frame's
-// first sub-step) blocks [fblk, fblk + nextra) run force_tail.
+// With nextra > 0 (a frame's first sub-step) blocks [0, nextra) run
+// force_tail (dispatched first, so the ship step overlaps the element
+// work); blocks [nextra, nextra + fblk) integrate their elements.
 __global__ __launch_bounds__(BLOCK) void k_force(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                                  int k, int sub, int src_is_state, int mode, int set, int last,
-                                                 int fblk, int nextra,
+                                                 int nextra,
                                                  const float* __restrict__ fship_in,
                                                  float* __restrict__ fout, float* __restrict__ mout) {
     __shared__ ForceLds L;
     __shared__ int am_last;
-    if ((int)blockIdx.x >= fblk) {
-        force_tail(V, D, k, last, (int)blockIdx.x - fblk, nextra, &am_last);
+    if ((int)blockIdx.x < nextra) {
+        force_tail(V, D, k, last, (int)blockIdx.x, nextra, &am_last);
         return;
     }
-    force_body(V, D, k, sub, src_is_state, mode, set, fship_in, fout, mout, L, blockIdx.x);
+    force_body(V, D, k, sub, src_is_state, mode, set, fship_in, fout, mout, L, (int)blockIdx.x - nextra);
 }
 
 // ── k_post: fracture (jax_simulation.py:534-656), commit, histories, next cull ─
@@ -1042,7 +1053,7 @@ __global__ __launch_bounds__(BLOCK) void k_post(const DevView* __restrict__ V, c
 
 // Totals, thrust cap (jax_simulation.py:456-479), floe eligibility
 // (:511-524) and the ship step, by one workgroup; la, lb hold nbpad floats.
-__device__ void mid_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D, float* la, float* lb,
+__device__ __forceinline__ void mid_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D, float* la, float* lb,
                          int k, int last) {
     const IceDemParams& P = V->p;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;

@@ -554,7 +554,7 @@ static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, hipEvent_t* 
             }
             const int extra = (s == 0) ? nextra : 0;
             hipLaunchKernelGGL(k_force, dim3(fblk + extra), dim3(BLOCK), 0, stream, V, D, k, s,
-                               (s % 2 == 0) ? 1 : 0, 0, set, last, fblk, extra,
+                               (s % 2 == 0) ? 1 : 0, 0, set, last, extra,
                                (const float*)nullptr, (float*)nullptr, (float*)nullptr);
         }
         mark(2);
@@ -766,7 +766,7 @@ int icedem_substep_forces(IceDemCtx* ctx, const IceDemState* state, const float*
     const int fblk = (H.n + EPB - 1) / EPB;
     hipLaunchKernelGGL(k_cellkey, dim3(H.nblk), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, (const float*)nullptr,
                        (const float*)nullptr, 1, 0);
-    hipLaunchKernelGGL(k_force, dim3(fblk), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, 0, 0, 1, 1, 0, 0, fblk, 0,
+    hipLaunchKernelGGL(k_force, dim3(fblk), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, 0, 0, 1, 1, 0, 0, 0,
                        d_ship_forces, d_forces, d_moments);
     hipLaunchKernelGGL(k_clear_cells, dim3((std::max(H.G, H.G2) + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, ctx->dv, 0);
     HIP_TRY(hipGetLastError());
