@@ -111,13 +111,18 @@ __device__ __forceinline__ void scan_range(const IceDemParams& P, float r, float
     }
 }
 
-// coarse cells to scan: 3x3 around the element's coarse cell
+// coarse cells to scan: contacts reach at most cell2 / 2, so the 2x2 cells
+// on the element's side of its coarse cell hold every large partner
 __device__ __forceinline__ void scan_range2(const DevView* __restrict__ V, float x, float y,
                                             int& x0, int& x1, int& y0, int& y1) {
-    int cx, cy;
-    coarse_of(V, x, y, cx, cy);
-    x0 = max(cx - 1, 0); x1 = min(cx + 1, V->nx2 - 1);
-    y0 = max(cy - 1, 0); y1 = min(cy + 1, V->ny2 - 1);
+    const IceDemParams& P = V->p;
+    const float ux = (x - P.grid_x0) / V->cell2, uy = (y - P.grid_y0) / V->cell2;
+    const float fx = floorf(ux), fy = floorf(uy);
+    const int cx = (int)fminf(fmaxf(fx, 0.0f), (float)(V->nx2 - 1));
+    const int cy = (int)fminf(fmaxf(fy, 0.0f), (float)(V->ny2 - 1));
+    const bool lox = (ux - (float)cx) < 0.5f, loy = (uy - (float)cy) < 0.5f;
+    x0 = lox ? max(cx - 1, 0) : cx; x1 = lox ? cx : min(cx + 1, V->nx2 - 1);
+    y0 = loy ? max(cy - 1, 0) : cy; y1 = loy ? cy : min(cy + 1, V->ny2 - 1);
 }
 
 // Overlap test of one ice-ice pair (lo < hi) of _compute_ice_ice_forces_jax
@@ -548,8 +553,9 @@ struct ForceLds {
     int cj[EPB][MAXC2];
     float cx[EPB][MAXC2], cy[EPB][MAXC2], cvx[EPB][MAXC2], cvy[EPB][MAXC2], cr[EPB][MAXC2];
     int ord[EPB][MAXC2];
-    int gcell[EPB][GRP];
+    int gbase[EPB][GRP];
     int goff[EPB][GRP];
+    int gcnt[EPB][GRP];
     float bf[EPB][3][GRP];
 };
 
@@ -559,54 +565,82 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// group-wide (16 lanes) inclusive prefix sum
+// group-wide (16 lanes = one DPP row) inclusive prefix sum
 __device__ __forceinline__ int group_incl_scan(int v) {
-    const int l = threadIdx.x & (GRP - 1);
-    #pragma unroll
-    for (int off = 1; off < GRP; off <<= 1) {
-        int o = __shfl_up(v, off, GRP);
-        if (l >= off) v += o;
-    }
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);   // row_shr:8
     return v;
 }
 
-// The occupied slots of cells [x0..x1] x [y0..y1] of one grid (row length
-// nxg, buckets at cnt / kin / id with capacity cap), 16 cells at a time:
-// the lanes read the cell counts, a group prefix sum lays the occupied
-// slots out, and each lane takes one slot per round.  visit(have, j, kin, r)
-// is called by every lane of the group, the same number of times.
+// value of the group's last lane (lane 15 of each 16-lane row)
+__device__ __forceinline__ int group_last(int v) {
+    return __builtin_amdgcn_ds_swizzle(v, 0x1F0);   // lane (l & 0x10) | 0xF of each 32-lane half
+}
+
+// The element's scan list: the fine cells [x0..x1] x [y0..y1] first, then
+// the coarse cells [u0..u1] x [v0..v1].  Cell q of the list has its count
+// at cnt[cidx] and its slots at kin / id [sbase, sbase + cap).
+struct ScanList {
+    int x0, y0, W, nfine;
+    int u0, v0, W2, ntot;
+};
+
+__device__ __forceinline__ void list_cell(const DevView* __restrict__ V, int set, const ScanList& S, int q,
+                                          int& cidx, int& sbase, int& cap) {
+    if (q < S.nfine) {
+        const int cell = (S.y0 + q / S.W) * V->p.grid_nx + (S.x0 + q % S.W);
+        cap = V->p.grid_bucket;
+        cidx = set * V->G + cell;
+        sbase = cidx * cap;
+    } else {
+        const int r = q - S.nfine;
+        const int cell = (S.v0 + r / S.W2) * V->nx2 + (S.u0 + r % S.W2);
+        cap = V->cap2;
+        cidx = 2 * V->G + set * V->G2 + cell;
+        sbase = 2 * V->G * V->p.grid_bucket + (set * V->G2 + cell) * cap;
+    }
+}
+
+// The occupied slots of the scan list, 16 cells at a time: the lanes read
+// the cell counts, a group prefix sum lays the occupied slots out, and each
+// lane takes one slot per round.  visit(have, j, kin, r) is called by every
+// lane of the group, the same number of times.
 template <class F>
-__device__ __forceinline__ void scan_grid(const int32_t* __restrict__ cnt, const float4* __restrict__ kin,
-                                          const int2* __restrict__ id, int cap, int nxg,
-                                          int x0, int x1, int y0, int y1, ForceLds& L, int g, F&& visit) {
+__device__ __forceinline__ void scan_list(const DevView* __restrict__ V, int set, const ScanList& S,
+                                          ForceLds& L, int g, F&& visit) {
     const int l = threadIdx.x & (GRP - 1);
-    const int W = x1 - x0 + 1;
-    const int ncell = W * (y1 - y0 + 1);
-    for (int c0 = 0; c0 < ncell; c0 += GRP) {
+    for (int c0 = 0; c0 < S.ntot; c0 += GRP) {
         const int q = c0 + l;
-        int cell = 0, cc = 0;
-        if (q < ncell) {
-            cell = (y0 + q / W) * nxg + (x0 + q % W);
-            cc = min(cnt[cell], cap);
+        int cidx = 0, sbase = 0, cap = 0, cc = 0;
+        if (q < S.ntot) {
+            list_cell(V, set, S, q, cidx, sbase, cap);
+            cc = min(V->cell_count[cidx], cap);
         }
         const int incl = group_incl_scan(cc);
-        const int total = __shfl(incl, GRP - 1, GRP);
-        L.gcell[g][l] = cell;
+        const int total = group_last(incl);
+        L.gbase[g][l] = sbase;
         L.goff[g][l] = incl - cc;
+        L.gcnt[g][l] = cc;
         wave_sync_lds();
-        const int nq = min(GRP, ncell - c0);
+        const int nq = min(GRP, S.ntot - c0);
         for (int base = 0; base < total; base += GRP) {
             const int f = base + l;
-            const bool have = f < total;
+            bool have = f < total;
             float4 k4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             int2 i2 = make_int2(-1, 0);
             if (have) {
                 int qq = 0;
                 #pragma unroll
                 for (int t = 1; t < GRP; ++t) qq += (t < nq && L.goff[g][t] <= f) ? 1 : 0;
-                const size_t si = (size_t)L.gcell[g][qq] * cap + (f - L.goff[g][qq]);
-                k4 = kin[si];
-                i2 = id[si];
+                const int sq = f - L.goff[g][qq];
+                have = sq >= 0 && sq < L.gcnt[g][qq];
+                if (have) {
+                    const int si = L.gbase[g][qq] + sq;
+                    k4 = V->cell_kin[si];
+                    i2 = V->cell_id[si];
+                }
             }
             visit(have, i2.x, k4, __int_as_float(i2.y));
         }
@@ -643,16 +677,13 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
             ov = (e < j) ? overlap_v(P, px, py, pr, k4.x, k4.y, r) : overlap_v(P, k4.x, k4.y, r, px, py, pr);
         push(ov, j, k4.x, k4.y, k4.z, k4.w, r);
     };
-    int x0, x1, y0, y1;
+    int x0, x1, y0, y1, u0 = 0, u1 = -1, v0 = 0, v1 = -1;
     scan_range(P, pr, px, py, x0, x1, y0, y1);
-    scan_grid(V->cell_count + (size_t)set * V->G, V->cell_kin + (size_t)set * V->G * P.grid_bucket,
-              V->cell_id + (size_t)set * V->G * P.grid_bucket, P.grid_bucket, P.grid_nx, x0, x1, y0, y1, L, g, visit);
-    int u0 = 0, u1 = -1, v0 = 0, v1 = -1;
-    if (V->nbig > 0) {
-        scan_range2(V, px, py, u0, u1, v0, v1);
-        scan_grid(V->cell2_count + (size_t)set * V->G2, V->cell2_kin + (size_t)set * V->G2 * V->cap2,
-                  V->cell2_id + (size_t)set * V->G2 * V->cap2, V->cap2, V->nx2, u0, u1, v0, v1, L, g, visit);
-    }
+    if (V->nbig > 0) scan_range2(V, px, py, u0, u1, v0, v1);
+    ScanList S;
+    S.x0 = x0; S.y0 = y0; S.W = x1 - x0 + 1; S.nfine = S.W * (y1 - y0 + 1);
+    S.u0 = u0; S.v0 = v0; S.W2 = u1 - u0 + 1; S.ntot = S.nfine + S.W2 * (v1 - v0 + 1);
+    scan_list(V, set, S, L, g, visit);
     for (int base = 0; base < novf; base += GRP) {
         const int f = base + l;
         bool ov = false;
@@ -741,48 +772,64 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
     }
 }
 
+// Partner and bond state of one bond-table entry (the dynamic reads of the
+// bond pass).
+struct BondIn {
+    float ox, oy, dmg;
+    bool intact, act;
+};
+
+__device__ __forceinline__ BondIn bond_in(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                          const float* __restrict__ pos, int4 t4, int mode) {
+    BondIn o;
+    o.ox = pos[t4.y]; o.oy = pos[V->n + t4.y];
+    o.intact = D->s.d_bond_intact[t4.x] != 0;
+    o.dmg = D->s.d_bond_damage[t4.x];
+    o.act = D->s.d_active[t4.y] != 0;
+    if (mode != 1) o.act = o.act || (V->floe_coll[t4.z] != 0u);
+    return o;
+}
+
 // Bond forces on element e (jax_simulation.py:63-108): lanes compute the terms
 // of the element's bond-table entries, lane 0 adds them in table order (the
 // order of the reference's two scatter-add passes).  act_e is e's activity
-// as the bond pass sees it; (bt0, bp0) is the lane's first table entry,
-// read ahead by the caller.  Result valid in lane 0.
+// as the bond pass sees it; (t0, p0, in0) is the lane's entry l with its
+// state, read ahead by the caller.  Result valid in lane 0.
 __device__ __forceinline__ void bond_force_group(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                 const float* __restrict__ pos, int e, float px, float py, bool act_e, int deg,
-                                 int4 bt0, float2 bp0, int mode, ForceLds& L, int g,
-                                 float& fbx, float& fby, float& mb, bool& crack) {
-    const int n = V->n;
+                                                 const float* __restrict__ pos, int e, float px, float py,
+                                                 bool act_e, int deg, int4 t0, float2 p0, BondIn in0, int mode,
+                                                 ForceLds& L, int g, float& fbx, float& fby, float& mb,
+                                                 bool& crack) {
     const int l = threadIdx.x & (GRP - 1);
     const int gshift = (threadIdx.x & 63) & ~(GRP - 1);
-    const uint8_t* active = D->s.d_active;
     fbx = 0.0f; fby = 0.0f; mb = 0.0f;
     crack = false;
     for (int base = 0; base < deg; base += GRP) {
         const int q = base + l;
         bool broken = false;
         if (q < deg) {
-            int4 t = bt0;
-            float2 pr = bp0;
-            if (base > 0) { t = V->btab[(size_t)e * V->W + q]; pr = V->bprm[(size_t)e * V->W + q]; }
-            const int b = t.x, o = t.y;
-            const bool role_j = t.w != 0;
-            const float ox = pos[o], oy = pos[n + o];
-            const bool intact = D->s.d_bond_intact[b] != 0;
-            const float dmg = D->s.d_bond_damage[b];
-            bool ao = active[o] != 0;
-            if (mode != 1) ao = ao || (V->floe_coll[t.z] != 0u);
-            broken = !intact;
-            const float xi = role_j ? ox : px, yi = role_j ? oy : py;
-            const float xj = role_j ? px : ox, yj = role_j ? py : oy;
-            const bool ai = role_j ? ao : act_e, aj = role_j ? act_e : ao;
+            int4 t4 = t0;
+            float2 pr = p0;
+            BondIn bi = in0;
+            if (base > 0) {
+                t4 = V->btab[(size_t)e * V->W + q];
+                pr = V->bprm[(size_t)e * V->W + q];
+                bi = bond_in(V, D, pos, t4, mode);
+            }
+            const bool role_j = t4.w != 0;
+            broken = !bi.intact;
+            const float xi = role_j ? bi.ox : px, yi = role_j ? bi.oy : py;
+            const float xj = role_j ? px : bi.ox, yj = role_j ? py : bi.oy;
+            const bool ai = role_j ? bi.act : act_e, aj = role_j ? act_e : bi.act;
             float dx = xj - xi;
             float dy = yj - yi;
             float dist = sqrtf(dx * dx + dy * dy);
             float dsafe = vmax(dist, 1e-10f);
             float dirx = dx / dsafe, diry = dy / dsafe;
             float stretch = dist - pr.x;
-            float Fm = pr.y * stretch * (1.0f - dmg);
+            float Fm = pr.y * stretch * (1.0f - bi.dmg);
             float Fx = dirx * Fm, Fy = diry * Fm;
-            if (!intact) { Fx = 0.0f; Fy = 0.0f; }
+            if (!bi.intact) { Fx = 0.0f; Fy = 0.0f; }
             float ok = ((ai && aj) || (!ai && !aj)) ? 1.0f : 0.0f;
             Fx = Fx * ok; Fy = Fy * ok;
             float cross = 0.5f * (dx * Fy - dy * Fx);
@@ -850,11 +897,13 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
     }
     const bool fcoll = (mode != 1) && (V->floe_coll[fl] != 0u);
     const bool act_bond = act_raw || fcoll;   // activity seen by the bond pass (mode 0)
+    BondIn bin0 = {0.0f, 0.0f, 0.0f, false, false};
+    if (l < deg) bin0 = bond_in(V, D, pos, bt0, mode);   // read ahead of the contact pass
     float fix, fiy, mi;
     contact_force_group(V, set, pos, vel, e, px, py, vx, vy, r, novf, L, g, fix, fiy, mi);
     float fbx, fby, mb;
     bool crack;
-    bond_force_group(V, D, pos, e, px, py, mode == 1 ? act_raw : act_bond, deg, bt0, bp0, mode, L, g,
+    bond_force_group(V, D, pos, e, px, py, mode == 1 ? act_raw : act_bond, deg, bt0, bp0, bin0, mode, L, g,
                      fbx, fby, mb, crack);
     if (l != 0) return;
     float shx, shy, subv;

@@ -374,7 +374,9 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
             rmax = std::max(rmax, (double)r);
             if (r > prm->grid_rsmall) ++nbig;
         }
-        const double c2 = std::max((double)prm->grid_cell, (2.0 * rmax + prm->contact_dilation2) * 1.001 + 1e-3);
+        // contacts reach at most 2 r_max + delta; with cell2 >= 2 (2 r_max + delta)
+        // the 2x2 coarse cells on an element's side hold all its large partners
+        const double c2 = std::max((double)prm->grid_cell, 2.0 * (2.0 * rmax + prm->contact_dilation2) * 1.002 + 1e-3);
         H.cell2 = (float)c2;
         if ((double)H.cell2 < c2) H.cell2 = std::nextafter(H.cell2, INFINITY);
         H.nx2 = std::max(1, (int)std::ceil((double)prm->grid_nx * prm->grid_cell / H.cell2));
@@ -435,11 +437,15 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     AL(nearf, uint8_t, n); AL(near_list, int32_t, n);
     AL(nearcnt, int32_t, nblk); AL(blkdone, int32_t, nblk); AL(counters, int32_t, 4);
     AL(crack, uint8_t, n);
-    {
-        const size_t slots = 2 * (size_t)H.G * prm->grid_bucket;
-        AL(cell_count, int32_t, 2 * (size_t)H.G); AL(cell_kin, float4, slots); AL(cell_id, int2, slots);
-        const size_t slots2 = 2 * (size_t)H.G2 * H.cap2;
-        AL(cell2_count, int32_t, 2 * (size_t)H.G2); AL(cell2_kin, float4, slots2); AL(cell2_id, int2, slots2);
+    {   // one allocation per array for both grids: fine sets, then coarse sets
+        const size_t slots = 2 * (size_t)H.G * prm->grid_bucket, slots2 = 2 * (size_t)H.G2 * H.cap2;
+        if (slots + slots2 > (size_t)INT32_MAX) { set_err(nullptr, "bucket slots beyond 2^31"); icedem_destroy(ctx); return ICEDEM_E_LIMIT; }
+        AL(cell_count, int32_t, 2 * (size_t)H.G + 2 * (size_t)H.G2);
+        AL(cell_kin, float4, slots + slots2);
+        AL(cell_id, int2, slots + slots2);
+        H.cell2_count = H.cell_count + 2 * (size_t)H.G;
+        H.cell2_kin = H.cell_kin + slots;
+        H.cell2_id = H.cell_id + slots;
     }
     AL(ovf_list, int32_t, 2 * (size_t)H.ovf_cap); AL(ovf_count, int32_t, 2);
     AL(floe_coll, uint32_t, H.nf); AL(floe_maxfm, uint32_t, H.nf);
@@ -449,8 +455,7 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     AL(err, int32_t, 4);
 #undef UP
 #undef AL
-    HIP_TRY(hipMemset(H.cell_count, 0, sizeof(int32_t) * 2 * (size_t)H.G));
-    HIP_TRY(hipMemset(H.cell2_count, 0, sizeof(int32_t) * 2 * (size_t)H.G2));
+    HIP_TRY(hipMemset(H.cell_count, 0, sizeof(int32_t) * (2 * (size_t)H.G + 2 * (size_t)H.G2)));
     HIP_TRY(hipMemset(H.ovf_count, 0, sizeof(int32_t) * 2));
     HIP_TRY(hipMemset(H.counters, 0, sizeof(int32_t) * 4));
     HIP_TRY(hipMemset(H.floe_coll, 0, sizeof(uint32_t) * std::max(H.nf, 1)));
