@@ -234,17 +234,56 @@ __device__ __forceinline__ float block256_tree(const DevView* __restrict__ V, co
     return wave_tree_sum(x);
 }
 
+// Tree sums of one 256-element block of three near-masked per-element arrays
+// (the order of block256_tree), every load issued before the first add.
+__device__ __forceinline__ void block256_tree3(const DevView* __restrict__ V, const float* __restrict__ a0,
+                                               const float* __restrict__ a1, const float* __restrict__ a2,
+                                               int blk, float& s0, float& s1, float& s2) {
+    const int lane = threadIdx.x & 63;
+    const int e0 = blk * BLOCK + lane * 4;
+    uint32_t nf = 0;
+    float4 x0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), x1 = x0, x2 = x0;
+    if (e0 + 3 < V->n) {
+        nf = *reinterpret_cast<const uint32_t*>(V->nearf + e0);
+        x0 = *reinterpret_cast<const float4*>(a0 + e0);
+        x1 = *reinterpret_cast<const float4*>(a1 + e0);
+        x2 = *reinterpret_cast<const float4*>(a2 + e0);
+    } else {
+        float v0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, v1[4] = {0.0f, 0.0f, 0.0f, 0.0f}, v2[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        #pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (e0 + i < V->n) {
+                nf |= (uint32_t)V->nearf[e0 + i] << (8 * i);
+                v0[i] = a0[e0 + i]; v1[i] = a1[e0 + i]; v2[i] = a2[e0 + i];
+            }
+        }
+        x0 = make_float4(v0[0], v0[1], v0[2], v0[3]);
+        x1 = make_float4(v1[0], v1[1], v1[2], v1[3]);
+        x2 = make_float4(v2[0], v2[1], v2[2], v2[3]);
+    }
+    const bool m0 = (nf & 0xffu) != 0u, m1 = (nf & 0xff00u) != 0u, m2 = (nf & 0xff0000u) != 0u, m3 = (nf >> 24) != 0u;
+    auto tree4 = [&](float4 x) {
+        return ((m0 ? x.x : 0.0f) + (m1 ? x.y : 0.0f)) + ((m2 ? x.z : 0.0f) + (m3 ? x.w : 0.0f));
+    };
+    s0 = wave_tree_sum(tree4(x0));
+    s1 = wave_tree_sum(tree4(x1));
+    s2 = wave_tree_sum(tree4(x2));
+}
+
 // Pairwise tree over a[0..npad) (npad a power of two, at most 4096) by one
 // wave; result in every lane.  Up to 64 entries: one per lane and a butterfly
 // of log2(npad) steps; beyond: each lane reduces its contiguous chunk
 // pairwise (a stack of partial sums per level), then the full butterfly.
+// pairwise tree of one value per lane over lanes [0, npad), npad <= 64 a
+// power of two; result in every lane
+__device__ __forceinline__ float wave_tree_small(float x, int npad) {
+    for (int off = 1; off < npad; off <<= 1) x = x + __shfl_xor(x, off, 64);
+    return __shfl(x, 0, 64);
+}
+
 __device__ __forceinline__ float wave_tree_global(const float* __restrict__ a, int npad) {
     const int lane = threadIdx.x & 63;
-    if (npad <= 64) {
-        float x = lane < npad ? a[lane] : 0.0f;
-        for (int off = 1; off < npad; off <<= 1) x = x + __shfl_xor(x, off, 64);
-        return __shfl(x, 0, 64);
-    }
+    if (npad <= 64) return wave_tree_small(lane < npad ? a[lane] : 0.0f, npad);
     const int c = npad >> 6;
     const float* p = a + (size_t)lane * c;
     float stk[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
@@ -302,6 +341,12 @@ __device__ __forceinline__ void cull_chunk(const DevView* __restrict__ V, int ch
     }
 }
 
+// sin/cos of the ship heading for the next k_near (scal[8..11])
+__device__ __forceinline__ void store_pose(const DevView* __restrict__ V, const float* __restrict__ ship) {
+    const ShipPose SP = make_pose(ship[0], ship[1], ship[2], V->p.half_len, V->p.half_beam);
+    V->scal[8] = SP.s_neg; V->scal[9] = SP.c_neg; V->scal[10] = SP.s_h; V->scal[11] = SP.c_h;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_cull(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
     const int n = V->n;
     const int e = blockIdx.x * BLOCK + threadIdx.x;
@@ -311,6 +356,7 @@ __global__ __launch_bounds__(BLOCK) void k_cull(const DevView* __restrict__ V, c
     float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f;
     if (valid) { x = pos[e]; y = pos[n + e]; vx = vel[e]; vy = vel[n + e]; }
     cull_chunk(V, blockIdx.x, valid, e, x, y, vx, vy, D->s.d_ship[0], D->s.d_ship[1], 0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) store_pose(V, D->s.d_ship);
 }
 
 // cell insertion only (sub-steps > 0 and the leaf entry points)
@@ -380,9 +426,8 @@ __device__ __forceinline__ void near_block_done(const DevView* __restrict__ V, i
     const IceDemParams& P = V->p;
     const int lane = threadIdx.x & 63, nbpad = V->nbpad;
     __threadfence();
-    const float a = block256_tree(V, V->soi_x, b, 1.0f, false);
-    const float c = block256_tree(V, V->soi_y, b, 1.0f, false);
-    const float d = block256_tree(V, V->mterm, b, 1.0f, false);
+    float a, c, d;
+    block256_tree3(V, V->soi_x, V->soi_y, V->mterm, b, a, c, d);
     int lastb = 0;
     if (lane == 0) {
         V->part1[b] = a; V->part1[nbpad + b] = c; V->part1[2 * nbpad + b] = d;
@@ -391,9 +436,17 @@ __device__ __forceinline__ void near_block_done(const DevView* __restrict__ V, i
     }
     if (!__shfl(lastb, 0, 64)) return;
     __threadfence();
-    const float tx = wave_tree_global(V->part1, nbpad);
-    const float ty = wave_tree_global(V->part1 + nbpad, nbpad);
-    const float tm = wave_tree_global(V->part1 + 2 * nbpad, nbpad);
+    float tx, ty, tm;
+    if (nbpad <= 64) {   // the three loads in one round trip
+        const float l0 = lane < nbpad ? V->part1[lane] : 0.0f;
+        const float l1 = lane < nbpad ? V->part1[nbpad + lane] : 0.0f;
+        const float l2 = lane < nbpad ? V->part1[2 * nbpad + lane] : 0.0f;
+        tx = wave_tree_small(l0, nbpad); ty = wave_tree_small(l1, nbpad); tm = wave_tree_small(l2, nbpad);
+    } else {
+        tx = wave_tree_global(V->part1, nbpad);
+        ty = wave_tree_global(V->part1 + nbpad, nbpad);
+        tm = wave_tree_global(V->part1 + 2 * nbpad, nbpad);
+    }
     float scale = 1.0f;
     if (P.has_thrust_cap) {
         const float F_tot = sqrtf(tx * tx + ty * ty);
@@ -414,7 +467,10 @@ __global__ __launch_bounds__(BLOCK) void k_near(const DevView* __restrict__ V, c
     }
     if (wave >= count) return;
     const float* ship = D->s.d_ship;
-    const ShipPose SP = make_pose(ship[0], ship[1], ship[2], V->p.half_len, V->p.half_beam);
+    ShipPose SP;   // heading sin/cos from the frame's ship step
+    SP.sx = ship[0]; SP.sy = ship[1];
+    SP.s_neg = V->scal[8]; SP.c_neg = V->scal[9]; SP.s_h = V->scal[10]; SP.c_h = V->scal[11];
+    SP.hl = V->p.half_len; SP.hb = V->p.half_beam;
     for (int q = wave; q < count; q += nwaves) {
         const int e = (q == wave) ? e0 : V->near_list[q];
         near_elem(V, D, SP, e);
@@ -541,6 +597,7 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
     floe_eligibility(V, D, scale);
     if (threadIdx.x == 0) {
         ship_frame(V, D, k, last, fx, fy, V->scal[1]);
+        store_pose(V, D->s.d_ship);
         *near_count(V) = 0;
         *nearblk_count(V) = 0;
         *near_ticket(V) = 0;

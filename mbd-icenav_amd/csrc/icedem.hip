@@ -449,7 +449,7 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     }
     AL(ovf_list, int32_t, 2 * (size_t)H.ovf_cap); AL(ovf_count, int32_t, 2);
     AL(floe_coll, uint32_t, H.nf); AL(floe_maxfm, uint32_t, H.nf);
-    AL(part1, float, 3 * (size_t)nbpad); AL(part2, float, 2 * (size_t)nbpad); AL(scal, float, 8);
+    AL(part1, float, 3 * (size_t)nbpad); AL(part2, float, 2 * (size_t)nbpad); AL(scal, float, 16);
     AL(nxt_pos, float, 2 * (size_t)n); AL(nxt_vel, float, 2 * (size_t)n);
     AL(nxt_rot, float, n); AL(nxt_omega, float, n);
     AL(err, int32_t, 4);
@@ -462,7 +462,7 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     HIP_TRY(hipMemset(H.floe_maxfm, 0, sizeof(uint32_t) * std::max(H.nf, 1)));
     HIP_TRY(hipMemset(H.part1, 0, sizeof(float) * 3 * nbpad));
     HIP_TRY(hipMemset(H.part2, 0, sizeof(float) * 2 * nbpad));
-    HIP_TRY(hipMemset(H.scal, 0, sizeof(float) * 8));
+    HIP_TRY(hipMemset(H.scal, 0, sizeof(float) * 16));
     HIP_TRY(hipMemset(H.err, 0, sizeof(int32_t) * 4));
     HIP_TRY(hipMemset(H.nearf, 0, n));
     HIP_TRY(hipMemset(H.nearcnt, 0, sizeof(int32_t) * nblk));

@@ -56,7 +56,8 @@ struct DevView {
     uint32_t* floe_coll; uint32_t* floe_maxfm;                      // (nf)
     float* part1;             // (3 * nbpad) block partials: sum soi_x, soi_y, moment
     float* part2;             // (2 * nbpad) block partials: sum of scaled soi
-    float* scal;              // [0] thrust scale, [1] M_ice_on_ship, [4..5] unscaled sum of soi
+    float* scal;              // [0] thrust scale, [1] M_ice_on_ship, [4..5] unscaled sum of soi,
+                              // [8..11] sin/cos of -heading and heading
     float* nxt_pos; float* nxt_vel; float* nxt_rot; float* nxt_omega;
     int32_t* err;
 };

@@ -197,21 +197,31 @@ __device__ __forceinline__ void hull_element(const ShipPose& P, const IceDemPara
     float ccx = sxv / nreal, ccy = syv / nreal;
     float b2v = -1.0f, b2nx = 0.0f, b2ny = 0.0f, b2hx = 0.0f, b2hy = 0.0f;
     int b2i = 0x7fffffff;
-    for (int base = 0; base < S; base += 64) {
-        int s = base + lane;
-        int sc = s < S ? s : 0;
+    // GP lanes per hull point share its polygon edges (kk = sub, sub + GP,
+    // ...); crossings add up and the nearest edge is the first minimum, so
+    // combining the lanes gives the sequential scan's result exactly
+    const int GP = S < 64 ? 64 / S : 1;
+    const int per_round = 64 / GP;
+    for (int base = 0; base < S; base += per_round) {
+        const int s = base + lane / GP, sub = lane % GP;
+        const bool live = (lane / GP) < per_round && s < S;
+        const int sc = live ? s : 0;
         float2 hl = hull[sc];
         float hx = (hl.x * P.c_h + hl.y * (-P.s_h)) + P.sx;
         float hy = (hl.x * P.s_h + hl.y * P.c_h) + P.sy;
         int cross = 0;
         float bd = 0.0f;
-        int bk = 0;
-        for (int kk = 0; kk < maxv; ++kk) {
-            int kj = (kk + 1 == maxv) ? 0 : kk + 1;
-            float xi = rl(wx, kk), yi = rl(wy, kk), xj = rl(wx, kj), yj = rl(wy, kj);
+        int bk = 0x7fffffff;
+        for (int k0 = 0; k0 < maxv; k0 += GP) {
+            const int kk = k0 + sub;
+            const bool ok = kk < maxv;
+            const int kkc = ok ? kk : 0;
+            int kj = (kkc + 1 == maxv) ? 0 : kkc + 1;
+            float xi = __shfl(wx, kkc, 64), yi = __shfl(wy, kkc, 64);
+            float xj = __shfl(wx, kj, 64), yj = __shfl(wy, kj, 64);
             bool cy = (yi > hy) != (yj > hy);
             float x_int = (xj - xi) * (hy - yi) / ((yj - yi) + 1e-10f) + xi;
-            if (cy && (hx < x_int)) cross++;
+            if (ok && cy && (hx < x_int)) cross++;
             float ex = xj - xi, ey = yj - yi;
             float wxp = hx - xi, wyp = hy - yi;
             float c1 = wxp * ex + wyp * ey;
@@ -220,12 +230,27 @@ __device__ __forceinline__ void hull_element(const ShipPose& P, const IceDemPara
             float qx = xi + t * ex, qy = yi + t * ey;
             float ddx = hx - qx, ddy = hy - qy;
             float d = sqrtf((ddx * ddx + ddy * ddy) + 1e-20f);
-            if (kk == 0 || d < bd) { bd = d; bk = kk; }
+            if (ok && (bk == 0x7fffffff || d < bd)) { bd = d; bk = kk; }
         }
+        // combine the GP lanes of each hull point into its first lane (reading
+        // every lane's own partial result, not a combined one)
+        const int cross0 = cross, bk0 = bk;
+        const float bd0 = bd;
+        for (int t = 1; t < GP; ++t) {
+            const int src = min(lane + t, 63);
+            const int oc = __shfl(cross0, src, 64);
+            const float od = __shfl(bd0, src, 64);
+            const int ok_ = __shfl(bk0, src, 64);
+            if (sub + t < GP) {
+                cross += oc;
+                if (ok_ != 0x7fffffff && (bk == 0x7fffffff || od < bd || (od == bd && ok_ < bk))) { bd = od; bk = ok_; }
+            }
+        }
+        if (bk == 0x7fffffff) bk = 0;
         int bkj = (bk + 1 == maxv) ? 0 : bk + 1;
         float xi = __shfl(wx, bk, 64), yi = __shfl(wy, bk, 64);
         float xj = __shfl(wx, bkj, 64), yj = __shfl(wy, bkj, 64);
-        if (s < S) {
+        if (live && sub == 0) {
             float ex = xj - xi, ey = yj - yi;
             float nxe = ey, nye = -ex;
             float nlen = sqrtf((nxe * nxe + nye * nye) + 1e-20f);
