@@ -60,24 +60,33 @@ def build_case(config: str, seed: int):
     return sc, cfg, L, B, mass, hull, x0, y0, speed
 
 
-def algorithmic_bytes(kernel: str, n: int, nb: int, n_near: int, max_v: int, n_hull: int, G: int) -> float:
-    """Bytes each kernel must move per launch (each array counted once)."""
+def algorithmic_bytes(kernel: str, n: int, nb: int, nf: int, G: int, n_near: int, max_v: int, n_hull: int,
+                      near_blocks: int, nbpad: int) -> float:
+    """Bytes each kernel must move per launch, every array counted once.
+
+    k_force, per element: state reads pos 8, vel 8, rot 4, omega 4, active 1,
+    sub 4; static reads radius, thickness, area, mass, inertia, floe, bond
+    count 28; the element's bucket slot as a contact partner 24; writes next
+    pos/vel/rot/omega 24, active 1, sub 4, force_mag 4, crack flag 1 -> 115.
+    Per bond (two table entries of 24 B, intact 1 + damage 4) -> 53; cell
+    counts 4 per cell; floe flags 4 per floe.
+    k_near, per near element: pos 8, rot 4, polygon 9 per vertex slot,
+    radius/thickness/floe 12, pre-pass writes 28, floe atomics 8; hull points
+    8 each; per near block the three masked block trees (256 x 13 B) and the
+    partials 12 per block slot.
+    k_post, per bond: indices 8, rest/strength/sigma 12, damage state 17 read
+    and written -> 54; per element: next state 24 read, state 24 written,
+    position/rotation/submergence history 16 (+ submergence read 4), force_mag
+    4, crack 1, thickness 4, floe 4, active 1, elig 1, cull reads
+    radius/bound 8, near flag 1, bucket slot 24 and count 4 -> 120; cell
+    clears 4 per cell, floe clears 8 per floe.
+    """
     if kernel == "k_force":
-        # per element: state r 33 (pos 8, vel 8, rot 4, omega 4, sub 4, active 1, floe-coll 4),
-        # static r 28 (floe 4, thick, area, radius, mass, inertia 20, csr 4), pre-pass r 24,
-        # v_water 8, cells 8; writes 34 (next pos/vel/rot/omega 24, sub 4, fm 4, active 1, crack 1)
-        # per bond: i, j, rest, damage 16 + intact 1 + two CSR entries 8
-        return 138.0 * n + 25.0 * nb
-    if kernel == "k_cull":
-        return 25.0 * n
+        return 115.0 * n + 53.0 * nb + 4.0 * G + 4.0 * nf
     if kernel == "k_near":
-        return n_near * (9.0 * max_v + 60.0) + 8.0 * n_hull + 12.0 * G / 256.0
-    if kernel in ("k_bond", "k_post"):
-        return 55.0 * nb + 82.0 * n + 8.0 * G + 33.0 * n
-    if kernel == "k_scatter":
-        return 16.0 * n
-    if kernel == "k_mid":
-        return 12.0 * G
+        return n_near * (9.0 * max_v + 60.0) + 8.0 * n_hull + near_blocks * 256 * 13.0 + 12.0 * nbpad
+    if kernel == "k_post":
+        return 54.0 * nb + 120.0 * n + 4.0 * G + 8.0 * nf
     return 64.0
 
 
@@ -123,7 +132,18 @@ def main():
     ap.add_argument("--profile-frames", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB five times (reference kernel for the PMC byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": x.numel() * 4, "copies": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
               f"(multi-GPU sharding is a design note, not an implemented path).", file=sys.stderr)
@@ -176,8 +196,14 @@ def main():
     elapsed = time.perf_counter() - t0
     _lib.check(eng.lib.icedem_sync(eng.ctx, torch.cuda.current_stream().cuda_stream), eng.ctx, "sync")
 
-    # per-kernel durations with HIP events on the launch stream
-    P = max(1, args.profile_frames)
+    # per-kernel durations with HIP events on the launch stream (0 frames: skipped)
+    P = args.profile_frames
+    if P <= 0:
+        print(json.dumps({"metric": METRIC, "value": n * args.steps / elapsed, "unit": "element-steps/s",
+                          "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": 1000.0 * elapsed / args.steps, "roofline": None,
+                          "cpu_baseline": None, "note": "per-kernel timing skipped"}), flush=True)
+        return
     ms = (C.c_float * len(KERNELS))()
     s = eng.state_struct(st, ship)
     inp = _lib.IceDemInputs()
@@ -189,10 +215,12 @@ def main():
     avg_us = {k: ms[i] * 1000.0 / P for i, k in enumerate(KERNELS[:3])}
     avg_us["k_cull (once per call)"] = ms[3] * 1000.0
     dom = max(KERNELS[:3], key=avg_us.get)
-    pen = eng.alloc_diag()
-    n_near = n  # upper bound for the hull kernel's vertex traffic
     G = eng.grid[4] * eng.grid[5]
-    abytes = algorithmic_bytes(dom, n, nb, n_near, eng.max_v, hull.shape[0], G)
+    n_near, near_blk = eng.near_stats(st, ship)
+    nbpad = 1
+    while nbpad < (n + 255) // 256:
+        nbpad <<= 1
+    abytes = algorithmic_bytes(dom, n, nb, sc.n_floes, G, n_near, eng.max_v, hull.shape[0], near_blk, nbpad)
     achieved = abytes / (avg_us[dom] * 1e-6) / 1e9
     traffic = None
     tfile = ROOT / "profiles" / f"traffic_{args.config}.json"

@@ -191,11 +191,11 @@ int icedem_destroy(IceDemCtx* ctx);
 int icedem_frames(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* in, int32_t K,
                   const IceDemHistory* hist, const IceDemDiag* diag, int32_t use_graph, void* stream);
 
-/* icedem_frames without graph replay, timing each kernel of the frame with
- * HIP events on `stream`: h_kernel_ms[0..6] receive the summed durations of
- * k_cull, k_near, k_mid, k_scatter, k_force (all sub-steps), k_bond and
- * k_ship over the K frames.  Instrumentation for benchmarks; no reference
- * counterpart. */
+/* icedem_frames launched on `stream` without graph replay, with a HIP event
+ * recorded after every kernel; synchronizes `stream`.  h_kernel_ms[0..3] receive the summed
+ * durations of k_near, k_force (all sub-steps with their cell rebuilds) and
+ * k_post over the K frames, and the call's one k_cull.  Instrumentation for
+ * benchmarks; no reference counterpart. */
 int icedem_frames_profiled(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* in, int32_t K,
                            const IceDemHistory* hist, float* h_kernel_ms, void* stream);
 

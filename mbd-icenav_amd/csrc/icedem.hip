@@ -1,6 +1,7 @@
 // icedem.hip -- kernels and C ABI of the MI355X DPDEM ice-ship frame step.
 // Build: see Makefile (hipcc --offload-arch=gfx950 -O3 -ffp-contract=off).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -511,11 +512,38 @@ static int put_desc(IceDemCtx* ctx, const CallDesc& cd, hipStream_t stream) {
     return ICEDEM_OK;
 }
 
-// ev (optional, profiling): per frame, events bracket k_near, k_force (all
-// sub-steps with their cell rebuilds) and k_post; acc[3] takes the call's
-// k_cull.
+}  // extern "C"
+
+// Kernel timing for icedem_frames_profiled: every launch gets a start/stop
+// event pair recorded by its own dispatch (hipExtLaunchKernelGGL), so a pair
+// spans exactly that kernel's execution.  Category: 0 k_near, 1 k_force
+// (with sub-step cell rebuilds), 2 k_post, 3 k_cull.
 constexpr int NPROF = 4;
-static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, hipEvent_t* ev = nullptr, float* acc = nullptr) {
+struct LaunchTimer {
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pairs;
+    int slot(int cat, hipEvent_t& a, hipEvent_t& b) {
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1;
+        pairs.push_back({cat, {a, b}});
+        return 0;
+    }
+    ~LaunchTimer() {
+        for (auto& p : pairs) { (void)hipEventDestroy(p.second.first); (void)hipEventDestroy(p.second.second); }
+    }
+};
+
+template <typename F, typename... Args>
+static int launch(LaunchTimer* lt, int cat, F kernel, dim3 grid, dim3 block, hipStream_t stream, Args... args) {
+    if (!lt) {
+        hipLaunchKernelGGL(kernel, grid, block, 0, stream, args...);
+        return ICEDEM_OK;
+    }
+    hipEvent_t a = nullptr, b = nullptr;
+    if (lt->slot(cat, a, b)) return ICEDEM_E_HIP;
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, stream, a, b, 0, args...);
+    return ICEDEM_OK;
+}
+
+static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, LaunchTimer* lt = nullptr) {
     const DevView& H = ctx->hv;
     const DevView* V = ctx->dv;
     const CallDesc* D = ctx->dd;
@@ -527,50 +555,34 @@ static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, hipEvent_t* 
     const int post = std::max(std::max(H.nb, H.n), std::max(std::max(H.G, H.G2), H.nf));
     const int nbpost = (post + BLOCK - 1) / BLOCK;
     const int gcl = (std::max(H.G, H.G2) + BLOCK - 1) / BLOCK;
-    auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], stream); };
-    auto collect = [&](int first, int count, int slot0) -> int {
-        if (!ev) return ICEDEM_OK;
-        HIP_TRY(hipEventSynchronize(ev[first + count]));
-        for (int i = 0; i < count; ++i) {
-            float ms = 0.0f;
-            HIP_TRY(hipEventElapsedTime(&ms, ev[first + i], ev[first + i + 1]));
-            acc[slot0 + i] += ms;
-        }
-        return ICEDEM_OK;
-    };
-    mark(0);
-    hipLaunchKernelGGL(k_cull, dim3(nblk), dim3(BLOCK), 0, stream, V, D);
-    mark(1);
-    if (collect(0, 1, 3)) return ICEDEM_E_HIP;
-    for (int k = 0; k < K; ++k) {
+    int rc = launch(lt, 3, k_cull, dim3(nblk), dim3(BLOCK), stream, V, D);
+    for (int k = 0; k < K && rc == ICEDEM_OK; ++k) {
         const int last = (k == K - 1) ? 1 : 0;
         const int set = k & 1;
-        mark(0);
-        hipLaunchKernelGGL(k_near, dim3(near_blocks), dim3(BLOCK), 0, stream, V, D);
-        mark(1);
+        rc |= launch(lt, 0, k_near, dim3(near_blocks), dim3(BLOCK), stream, V, D);
         for (int s = 0; s < steps; ++s) {
             if (s > 0) {
                 // rebuild the cell buckets for the sub-step's positions
                 const bool odd = (s % 2 == 1);
                 const float* sp = odd ? (const float*)H.nxt_pos : (const float*)nullptr;
                 const float* sv = odd ? (const float*)H.nxt_vel : (const float*)nullptr;
-                hipLaunchKernelGGL(k_clear_cells, dim3(gcl), dim3(BLOCK), 0, stream, V, set);
-                hipLaunchKernelGGL(k_cellkey, dim3(nblk), dim3(BLOCK), 0, stream, V, D, sp, sv, odd ? 0 : 1, set);
+                rc |= launch(lt, 1, k_clear_cells, dim3(gcl), dim3(BLOCK), stream, V, set);
+                rc |= launch(lt, 1, k_cellkey, dim3(nblk), dim3(BLOCK), stream, V, D, sp, sv, odd ? 0 : 1, set);
             }
             const int extra = (s == 0) ? nextra : 0;
-            hipLaunchKernelGGL(k_force, dim3(fblk + extra), dim3(BLOCK), 0, stream, V, D, k, s,
-                               (s % 2 == 0) ? 1 : 0, 0, set, last, extra,
-                               (const float*)nullptr, (float*)nullptr, (float*)nullptr);
+            rc |= launch(lt, 1, k_force, dim3(fblk + extra), dim3(BLOCK), stream, V, D, k, s,
+                         (s % 2 == 0) ? 1 : 0, 0, set, last, extra,
+                         (const float*)nullptr, (float*)nullptr, (float*)nullptr);
         }
-        mark(2);
-        hipLaunchKernelGGL(k_post, dim3(nbpost), dim3(BLOCK), 0, stream, V, D, k, (steps % 2 == 1) ? 1 : 0, last,
-                           set, last ? 0 : 1);
-        mark(3);
-        if (collect(0, 3, 0)) return ICEDEM_E_HIP;
+        rc |= launch(lt, 2, k_post, dim3(nbpost), dim3(BLOCK), stream, V, D, k, (steps % 2 == 1) ? 1 : 0, last,
+                     set, last ? 0 : 1);
     }
+    if (rc) { set_err(ctx, "event creation failed"); return ICEDEM_E_HIP; }
     HIP_TRY(hipGetLastError());
     return ICEDEM_OK;
 }
+
+extern "C" {
 
 int icedem_frames(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* in, int32_t K,
                   const IceDemHistory* hist, const IceDemDiag* diag, int32_t use_graph, void* stream_) {
@@ -614,6 +626,8 @@ int icedem_frames(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* 
 int icedem_frames_profiled(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* in, int32_t K,
                            const IceDemHistory* hist, float* h_kernel_ms, void* stream_) {
     if (!ctx || !in || !h_kernel_ms || K < 0) { set_err(ctx, "null argument"); return ICEDEM_E_INVALID; }
+    for (int i = 0; i < NPROF; ++i) h_kernel_ms[i] = 0.0f;
+    if (K == 0) return ICEDEM_OK;
     int rc = validate_state(ctx, state);
     if (rc) return rc;
     if (!in->d_setpoints) { set_err(ctx, "setpoints required"); return ICEDEM_E_INVALID; }
@@ -626,13 +640,18 @@ int icedem_frames_profiled(IceDemCtx* ctx, const IceDemState* state, const IceDe
     if (hist) cd.h = *hist;
     rc = put_desc(ctx, cd, stream);
     if (rc) return rc;
-    hipEvent_t ev[NPROF + 1];
-    for (int i = 0; i <= NPROF; ++i) HIP_TRY(hipEventCreate(&ev[i]));
+    LaunchTimer lt;
+    rc = launch_frames(ctx, K, stream, &lt);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(stream));
     float acc[NPROF] = {0, 0, 0, 0};
-    rc = launch_frames(ctx, K, stream, ev, acc);
-    for (int i = 0; i <= NPROF; ++i) (void)hipEventDestroy(ev[i]);
+    for (auto& p : lt.pairs) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+        acc[p.first] += ms;
+    }
     for (int i = 0; i < NPROF; ++i) h_kernel_ms[i] = acc[i];
-    return rc;
+    return ICEDEM_OK;
 }
 
 struct IceDemBatch {

@@ -251,6 +251,18 @@ class DemEngine:
         self.ctx = ctx
         self.max_v = st.max_v
 
+    def near_stats(self, st: IceState, ship_buf: torch.Tensor):
+        """Elements passing the near-ship bounding-circle test for the current
+        state, and the 256-element blocks holding one (host-side count, for
+        traffic accounting)."""
+        pos = st.positions.detach().float().cpu().numpy()
+        sx, sy = (float(v) for v in ship_buf[:2].detach().cpu().numpy())
+        R = self._host["bound"].astype(np.float64) + float(self.params.ship_bound_radius) + 1.0
+        d2 = (pos[:, 0] - sx) ** 2 + (pos[:, 1] - sy) ** 2
+        near = d2 <= R * R
+        blocks = np.unique(np.nonzero(near)[0] // 256)
+        return int(near.sum()), int(len(blocks))
+
     def close(self):
         if getattr(self, "ctx", None):
             self.lib.icedem_destroy(self.ctx)
