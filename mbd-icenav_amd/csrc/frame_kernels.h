@@ -69,29 +69,61 @@ __device__ __forceinline__ void coarse_of(const DevView* __restrict__ V, float x
     cy = (int)fminf(fmaxf(fy, 0.0f), (float)(V->ny2 - 1));
 }
 
+// bucket record r (fine: r < 2 G) and its capacity
+__device__ __forceinline__ uint8_t* rec_ptr(const DevView* __restrict__ V, int r) {
+    return r < 2 * V->G ? V->cells + (size_t)r * V->rec_f
+                        : V->cells + V->coarse_off + (size_t)(r - 2 * V->G) * V->rec_c;
+}
+__device__ __forceinline__ int rec_cap(const DevView* __restrict__ V, int r) {
+    return r < 2 * V->G ? V->p.grid_bucket : V->cap2;
+}
+__device__ __forceinline__ int fine_rec(const DevView* __restrict__ V, int set, int cell) { return set * V->G + cell; }
+__device__ __forceinline__ int coarse_rec(const DevView* __restrict__ V, int set, int cell) {
+    return 2 * V->G + set * V->G2 + cell;
+}
+
+struct Slot {
+    float x, y, vx, vy, r;
+    int e;
+};
+
+// a slot is 24 bytes at 8-byte alignment: one 16-byte and one 8-byte access
+// (global memory accesses on gfx950 need only 4-byte alignment)
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
+
+__device__ __forceinline__ Slot slot_load(const uint8_t* __restrict__ rec, int s) {
+    const uint8_t* q = rec + 16 + 24 * s;
+    const f4u a = *reinterpret_cast<const f4u*>(q);
+    const f2u c = *reinterpret_cast<const f2u*>(q + 16);
+    Slot o;
+    o.x = a.x; o.y = a.y; o.vx = a.z; o.vy = a.w; o.r = c.x; o.e = __float_as_int(c.y);
+    return o;
+}
+
 // Element e (state x, y, vx, vy, radius r) into its bucket of set `set`.
 __device__ __forceinline__ void cell_insert(const DevView* __restrict__ V, int set, int e,
                                             float x, float y, float vx, float vy, float r) {
     const IceDemParams& P = V->p;
-    int32_t* cnt;
-    float4* kin;
-    int2* id;
-    int cap;
-    size_t cs;
+    int rix;
     if (!is_big(P, r)) {
         int cx, cy;
-        cs = (size_t)set * V->G + cell_of(P, x, y, cx, cy);
-        cnt = V->cell_count; kin = V->cell_kin; id = V->cell_id; cap = P.grid_bucket;
+        rix = fine_rec(V, set, cell_of(P, x, y, cx, cy));
     } else {
         int cx, cy;
         coarse_of(V, x, y, cx, cy);
-        cs = (size_t)set * V->G2 + cy * V->nx2 + cx;
-        cnt = V->cell2_count; kin = V->cell2_kin; id = V->cell2_id; cap = V->cap2;
+        rix = coarse_rec(V, set, cy * V->nx2 + cx);
     }
-    const int idx = atomicAdd(&cnt[cs], 1);
-    if (idx < cap) {
-        kin[cs * cap + idx] = make_float4(x, y, vx, vy);
-        id[cs * cap + idx] = make_int2(e, __float_as_int(r));
+    uint8_t* rec = rec_ptr(V, rix);
+    const int idx = atomicAdd(reinterpret_cast<int*>(rec), 1);
+    if (idx < rec_cap(V, rix)) {
+        uint8_t* q = rec + 16 + 24 * idx;
+        f4u a;
+        a.x = x; a.y = y; a.z = vx; a.w = vy;
+        f2u c;
+        c.x = r; c.y = __int_as_float(e);
+        *reinterpret_cast<f4u*>(q) = a;
+        *reinterpret_cast<f2u*>(q + 16) = c;
     } else {
         V->ovf_list[(size_t)set * V->ovf_cap + atomicAdd(&V->ovf_count[set], 1)] = e;
     }
@@ -371,8 +403,8 @@ __global__ __launch_bounds__(BLOCK) void k_cellkey(const DevView* __restrict__ V
 }
 
 __device__ __forceinline__ void clear_cells(const DevView* __restrict__ V, int set, int t) {
-    if (t < V->G) V->cell_count[(size_t)set * V->G + t] = 0;
-    if (t < V->G2) V->cell2_count[(size_t)set * V->G2 + t] = 0;
+    if (t < V->G) *reinterpret_cast<int*>(rec_ptr(V, fine_rec(V, set, t))) = 0;
+    if (t < V->G2) *reinterpret_cast<int*>(rec_ptr(V, coarse_rec(V, set, t))) = 0;
     if (t == 0) V->ovf_count[set] = 0;
 }
 
@@ -610,7 +642,7 @@ struct ForceLds {
     int cj[EPB][MAXC2];
     float cx[EPB][MAXC2], cy[EPB][MAXC2], cvx[EPB][MAXC2], cvy[EPB][MAXC2], cr[EPB][MAXC2];
     int ord[EPB][MAXC2];
-    int gbase[EPB][GRP];
+    int grec[EPB][GRP];
     int goff[EPB][GRP];
     int gcnt[EPB][GRP];
     float bf[EPB][3][GRP];
@@ -637,27 +669,18 @@ __device__ __forceinline__ int group_last(int v) {
 }
 
 // The element's scan list: the fine cells [x0..x1] x [y0..y1] first, then
-// the coarse cells [u0..u1] x [v0..v1].  Cell q of the list has its count
-// at cnt[cidx] and its slots at kin / id [sbase, sbase + cap).
+// the coarse cells [u0..u1] x [v0..v1]; cell q of the list is bucket
+// record list_rec(q).
 struct ScanList {
     int x0, y0, W, nfine;
     int u0, v0, W2, ntot;
 };
 
-__device__ __forceinline__ void list_cell(const DevView* __restrict__ V, int set, const ScanList& S, int q,
-                                          int& cidx, int& sbase, int& cap) {
-    if (q < S.nfine) {
-        const int cell = (S.y0 + q / S.W) * V->p.grid_nx + (S.x0 + q % S.W);
-        cap = V->p.grid_bucket;
-        cidx = set * V->G + cell;
-        sbase = cidx * cap;
-    } else {
-        const int r = q - S.nfine;
-        const int cell = (S.v0 + r / S.W2) * V->nx2 + (S.u0 + r % S.W2);
-        cap = V->cap2;
-        cidx = 2 * V->G + set * V->G2 + cell;
-        sbase = 2 * V->G * V->p.grid_bucket + (set * V->G2 + cell) * cap;
-    }
+
+__device__ __forceinline__ int list_rec(const DevView* __restrict__ V, int set, const ScanList& S, int q) {
+    if (q < S.nfine) return fine_rec(V, set, (S.y0 + q / S.W) * V->p.grid_nx + (S.x0 + q % S.W));
+    const int r = q - S.nfine;
+    return coarse_rec(V, set, (S.v0 + r / S.W2) * V->nx2 + (S.u0 + r % S.W2));
 }
 
 // The occupied slots of the scan list, 16 cells at a time: the lanes read
@@ -670,14 +693,14 @@ __device__ __forceinline__ void scan_list(const DevView* __restrict__ V, int set
     const int l = threadIdx.x & (GRP - 1);
     for (int c0 = 0; c0 < S.ntot; c0 += GRP) {
         const int q = c0 + l;
-        int cidx = 0, sbase = 0, cap = 0, cc = 0;
+        int rix = 0, cc = 0;
         if (q < S.ntot) {
-            list_cell(V, set, S, q, cidx, sbase, cap);
-            cc = min(V->cell_count[cidx], cap);
+            rix = list_rec(V, set, S, q);
+            cc = min(*reinterpret_cast<const int*>(rec_ptr(V, rix)), rec_cap(V, rix));
         }
         const int incl = group_incl_scan(cc);
         const int total = group_last(incl);
-        L.gbase[g][l] = sbase;
+        L.grec[g][l] = rix;
         L.goff[g][l] = incl - cc;
         L.gcnt[g][l] = cc;
         wave_sync_lds();
@@ -685,21 +708,16 @@ __device__ __forceinline__ void scan_list(const DevView* __restrict__ V, int set
         for (int base = 0; base < total; base += GRP) {
             const int f = base + l;
             bool have = f < total;
-            float4 k4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            int2 i2 = make_int2(-1, 0);
+            Slot sl = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, -1};
             if (have) {
                 int qq = 0;
                 #pragma unroll
                 for (int t = 1; t < GRP; ++t) qq += (t < nq && L.goff[g][t] <= f) ? 1 : 0;
                 const int sq = f - L.goff[g][qq];
                 have = sq >= 0 && sq < L.gcnt[g][qq];
-                if (have) {
-                    const int si = L.gbase[g][qq] + sq;
-                    k4 = V->cell_kin[si];
-                    i2 = V->cell_id[si];
-                }
+                if (have) sl = slot_load(rec_ptr(V, L.grec[g][qq]), sq);
             }
-            visit(have, i2.x, k4, __int_as_float(i2.y));
+            visit(have, sl.e, make_float4(sl.x, sl.y, sl.vx, sl.vy), sl.r);
         }
         wave_sync_lds();
     }
@@ -793,10 +811,6 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
     }
     // more than MAXC2 contacts: exact selection of the next partner by lane 0
     if (l == 0) {
-        const int32_t* cnt1 = V->cell_count + (size_t)set * V->G;
-        const int2* id1 = V->cell_id + (size_t)set * V->G * P.grid_bucket;
-        const int32_t* cnt2 = V->cell2_count + (size_t)set * V->G2;
-        const int2* id2 = V->cell2_id + (size_t)set * V->G2 * V->cap2;
         int last = 0;
         for (;;) {
             int best = 0x7fffffff, bj = -1;
@@ -809,15 +823,15 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
             };
             for (int gy = y0; gy <= y1; ++gy)
                 for (int gx = x0; gx <= x1; ++gx) {
-                    const int cell = gy * P.grid_nx + gx;
-                    const int cc = min(cnt1[cell], P.grid_bucket);
-                    for (int s = 0; s < cc; ++s) consider(id1[(size_t)cell * P.grid_bucket + s].x);
+                    const uint8_t* rec = rec_ptr(V, fine_rec(V, set, gy * P.grid_nx + gx));
+                    const int cc = min(*reinterpret_cast<const int*>(rec), P.grid_bucket);
+                    for (int s = 0; s < cc; ++s) consider(slot_load(rec, s).e);
                 }
             for (int gy = v0; gy <= v1; ++gy)
                 for (int gx = u0; gx <= u1; ++gx) {
-                    const int cell = gy * V->nx2 + gx;
-                    const int cc = min(cnt2[cell], V->cap2);
-                    for (int s = 0; s < cc; ++s) consider(id2[(size_t)cell * V->cap2 + s].x);
+                    const uint8_t* rec = rec_ptr(V, coarse_rec(V, set, gy * V->nx2 + gx));
+                    const int cc = min(*reinterpret_cast<const int*>(rec), V->cap2);
+                    for (int s = 0; s < cc; ++s) consider(slot_load(rec, s).e);
                 }
             for (int s = 0; s < novf; ++s) consider(V->ovf_list[(size_t)set * V->ovf_cap + s]);
             if (bj < 0) break;
@@ -1041,6 +1055,17 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
 // With nextra > 0 (a frame's first sub-step) blocks [0, nextra) run
 // force_tail (dispatched first, so the ship step overlaps the element
 // work); blocks [nextra, nextra + fblk) integrate their elements.
+// Workgroup ids are dealt to the 8 XCDs round-robin; chunk_of maps them so
+// that each XCD takes one contiguous run of element chunks (its L2 then
+// serves the bucket and partner reads of spatially close elements when the
+// element order is spatial).  A bijection on [0, total).
+__device__ __forceinline__ int chunk_of(int w, int total) {
+    constexpr int NX = 8;
+    const int q = total / NX, r = total % NX;
+    const int x = w % NX, slot = w / NX;
+    return (x < r) ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_force(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                                  int k, int sub, int src_is_state, int mode, int set, int last,
                                                  int nextra,
@@ -1052,7 +1077,9 @@ __global__ __launch_bounds__(BLOCK) void k_force(const DevView* __restrict__ V, 
         force_tail(V, D, k, last, (int)blockIdx.x, nextra, &am_last);
         return;
     }
-    force_body(V, D, k, sub, src_is_state, mode, set, fship_in, fout, mout, L, (int)blockIdx.x - nextra);
+    const int nf = (int)gridDim.x - nextra;
+    force_body(V, D, k, sub, src_is_state, mode, set, fship_in, fout, mout, L,
+               chunk_of((int)blockIdx.x - nextra, nf));
 }
 
 // ── k_post: fracture (jax_simulation.py:534-656), commit, histories, next cull ─

@@ -201,14 +201,16 @@ __device__ int pairs_of(const DevView* __restrict__ V, const float* __restrict__
         for (int gy = y0; gy <= y1; ++gy)
             for (int gx = x0; gx <= x1; ++gx) {
                 const int cell = gy * P.grid_nx + gx;
-                const int cc = min(V->cell_count[cell], cap);
-                for (int s = 0; s < cc; ++s) consider(V->cell_id[(size_t)cell * cap + s].x);
+                const uint8_t* rec = rec_ptr(V, fine_rec(V, 0, cell));
+                const int cc = min(*reinterpret_cast<const int*>(rec), cap);
+                for (int s = 0; s < cc; ++s) consider(slot_load(rec, s).e);
             }
         for (int gy = v0; gy <= v1; ++gy)
             for (int gx = u0; gx <= u1; ++gx) {
                 const int cell = gy * V->nx2 + gx;
-                const int cc = min(V->cell2_count[cell], V->cap2);
-                for (int s = 0; s < cc; ++s) consider(V->cell2_id[(size_t)cell * V->cap2 + s].x);
+                const uint8_t* rec = rec_ptr(V, coarse_rec(V, 0, cell));
+                const int cc = min(*reinterpret_cast<const int*>(rec), V->cap2);
+                for (int s = 0; s < cc; ++s) consider(slot_load(rec, s).e);
             }
         for (int s = 0; s < novf; ++s) consider(V->ovf_list[s]);
         if (best == 0x7fffffff) break;
@@ -438,15 +440,16 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     AL(nearf, uint8_t, n); AL(near_list, int32_t, n);
     AL(nearcnt, int32_t, nblk); AL(blkdone, int32_t, nblk); AL(counters, int32_t, 4);
     AL(crack, uint8_t, n);
-    {   // one allocation per array for both grids: fine sets, then coarse sets
-        const size_t slots = 2 * (size_t)H.G * prm->grid_bucket, slots2 = 2 * (size_t)H.G2 * H.cap2;
-        if (slots + slots2 > (size_t)INT32_MAX) { set_err(nullptr, "bucket slots beyond 2^31"); icedem_destroy(ctx); return ICEDEM_E_LIMIT; }
-        AL(cell_count, int32_t, 2 * (size_t)H.G + 2 * (size_t)H.G2);
-        AL(cell_kin, float4, slots + slots2);
-        AL(cell_id, int2, slots + slots2);
-        H.cell2_count = H.cell_count + 2 * (size_t)H.G;
-        H.cell2_kin = H.cell_kin + slots;
-        H.cell2_id = H.cell_id + slots;
+    {   // bucket records of both grids in one allocation: fine sets, then coarse
+        auto rec_bytes = [](int64_t cap) { return ((16 + 24 * cap + 127) / 128) * 128; };
+        H.rec_f = rec_bytes(prm->grid_bucket);
+        H.rec_c = rec_bytes(H.cap2);
+        H.coarse_off = 2 * (int64_t)H.G * H.rec_f;
+        const int64_t total = H.coarse_off + 2 * (int64_t)H.G2 * H.rec_c;
+        if (2 * (int64_t)H.G + 2 * (int64_t)H.G2 > (int64_t)INT32_MAX || total > (1LL << 36)) {
+            set_err(nullptr, "bucket records beyond the addressable range"); icedem_destroy(ctx); return ICEDEM_E_LIMIT;
+        }
+        AL(cells, uint8_t, (size_t)total);
     }
     AL(ovf_list, int32_t, 2 * (size_t)H.ovf_cap); AL(ovf_count, int32_t, 2);
     AL(floe_coll, uint32_t, H.nf); AL(floe_maxfm, uint32_t, H.nf);
@@ -456,7 +459,7 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     AL(err, int32_t, 4);
 #undef UP
 #undef AL
-    HIP_TRY(hipMemset(H.cell_count, 0, sizeof(int32_t) * (2 * (size_t)H.G + 2 * (size_t)H.G2)));
+    HIP_TRY(hipMemset(H.cells, 0, (size_t)(H.coarse_off + 2 * (int64_t)H.G2 * H.rec_c)));
     HIP_TRY(hipMemset(H.ovf_count, 0, sizeof(int32_t) * 2));
     HIP_TRY(hipMemset(H.counters, 0, sizeof(int32_t) * 4));
     HIP_TRY(hipMemset(H.floe_coll, 0, sizeof(uint32_t) * std::max(H.nf, 1)));

@@ -39,18 +39,19 @@ struct DevView {
     int32_t* counters;        // [0] near elements, [1] blocks holding one, [2] k_near ticket, [3] k_force ticket
     float* shipf;             // (2n) ship forces on ice, kept for sub-steps > 0
     uint8_t* crack;           // (n) element touches a broken bond
-    // Two grids, each with two bucket sets (frame parity): set s, cell c,
-    // slot q at (s G + c) cap + q.  The fine grid (params grid_*) holds the
-    // elements of radius <= grid_rsmall; the coarse grid (cell2 >= 2 r_max +
-    // delta, same origin) holds the larger ones.
-    int32_t* cell_count;      // (2 G) insertions per fine cell
-    float4* cell_kin;         // (2 G cap) x, y, vx, vy of the inserted element
-    int2* cell_id;            // (2 G cap) element index, radius bits
+    // Two grids, each with two bucket sets (frame parity).  A bucket is one
+    // record: a 16-byte header whose first int is the insertion count, then
+    // up to cap slots of 24 bytes (x, y, vx, vy, radius, element index), the
+    // record padded to a multiple of 128 bytes so a lightly filled bucket is
+    // one cache line.  Record r: fine set s cell c is r = s G + c; coarse set
+    // s cell c is r = 2 G + s G2 + c.  The fine grid (params grid_*) holds
+    // the elements of radius <= grid_rsmall; the coarse grid (cell2 >=
+    // 2 (2 r_max + delta), same origin) the larger ones.
+    uint8_t* cells;
+    int64_t rec_f, rec_c;     // record bytes, fine and coarse
+    int64_t coarse_off;       // byte offset of the first coarse record
     int32_t nbig, nx2, ny2, G2, cap2;
     float cell2;
-    int32_t* cell2_count;     // (2 G2)
-    float4* cell2_kin;        // (2 G2 cap2)
-    int2* cell2_id;           // (2 G2 cap2)
     int32_t* ovf_list;        // (2 ovf_cap) insertions that found their bucket full
     int32_t* ovf_count;       // (2)
     uint32_t* floe_coll; uint32_t* floe_maxfm;                      // (nf)

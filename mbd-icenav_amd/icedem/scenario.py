@@ -210,7 +210,10 @@ def place_floes(rng, x_lo, x_hi, y_lo, y_hi, concentration, max_floes, radius_fn
         placed.append(poly); radii.append(R); centers.append(c)
         grid.setdefault((gx, gy), []).append(k)
         area_sum += polygon_area(poly)
-    return placed, area_sum / total
+    # spatial order (by centre y, then x): consecutive elements are neighbours
+    # in space, which keeps the GPU's per-XCD caches on one band of the field
+    order = sorted(range(len(placed)), key=lambda i: (float(centers[i][1]), float(centers[i][0])))
+    return [placed[i] for i in order], area_sum / total
 
 
 def tessellate_all(rng, floes, cells_per_floe, floe_thickness):
