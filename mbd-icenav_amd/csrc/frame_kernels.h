@@ -70,9 +70,9 @@ __device__ __forceinline__ void coarse_of(const DevView* __restrict__ V, float x
 }
 
 // bucket record r (fine: r < 2 G) and its capacity
-__device__ __forceinline__ uint8_t* rec_ptr(const DevView* __restrict__ V, int r) {
-    return r < 2 * V->G ? V->cells + (size_t)r * V->rec_f
-                        : V->cells + V->coarse_off + (size_t)(r - 2 * V->G) * V->rec_c;
+__device__ __forceinline__ GAS uint8_t* rec_ptr(const DevView* __restrict__ V, int r) {
+    return r < 2 * V->G ? G(V->cells) + (size_t)r * V->rec_f
+                        : G(V->cells) + V->coarse_off + (size_t)(r - 2 * V->G) * V->rec_c;
 }
 __device__ __forceinline__ int rec_cap(const DevView* __restrict__ V, int r) {
     return r < 2 * V->G ? V->p.grid_bucket : V->cap2;
@@ -92,10 +92,10 @@ struct Slot {
 typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
 typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
 
-__device__ __forceinline__ Slot slot_load(const uint8_t* __restrict__ rec, int s) {
-    const uint8_t* q = rec + 16 + 24 * s;
-    const f4u a = *reinterpret_cast<const f4u*>(q);
-    const f2u c = *reinterpret_cast<const f2u*>(q + 16);
+__device__ __forceinline__ Slot slot_load(GAS const uint8_t* __restrict__ rec, int s) {
+    GAS const uint8_t* q = rec + 16 + 24 * s;
+    const f4u a = *(GAS const f4u*)q;
+    const f2u c = *(GAS const f2u*)(q + 16);
     Slot o;
     o.x = a.x; o.y = a.y; o.vx = a.z; o.vy = a.w; o.r = c.x; o.e = __float_as_int(c.y);
     return o;
@@ -114,18 +114,18 @@ __device__ __forceinline__ void cell_insert(const DevView* __restrict__ V, int s
         coarse_of(V, x, y, cx, cy);
         rix = coarse_rec(V, set, cy * V->nx2 + cx);
     }
-    uint8_t* rec = rec_ptr(V, rix);
-    const int idx = atomicAdd(reinterpret_cast<int*>(rec), 1);
+    GAS uint8_t* rec = rec_ptr(V, rix);
+    const int idx = atomicAdd((int*)rec, 1);
     if (idx < rec_cap(V, rix)) {
-        uint8_t* q = rec + 16 + 24 * idx;
+        GAS uint8_t* q = rec + 16 + 24 * idx;
         f4u a;
         a.x = x; a.y = y; a.z = vx; a.w = vy;
         f2u c;
         c.x = r; c.y = __int_as_float(e);
-        *reinterpret_cast<f4u*>(q) = a;
-        *reinterpret_cast<f2u*>(q + 16) = c;
+        *(GAS f4u*)q = a;
+        *(GAS f2u*)(q + 16) = c;
     } else {
-        V->ovf_list[(size_t)set * V->ovf_cap + atomicAdd(&V->ovf_count[set], 1)] = e;
+        G(V->ovf_list)[(size_t)set * V->ovf_cap + atomicAdd(&V->ovf_count[set], 1)] = e;
     }
 }
 
@@ -171,8 +171,8 @@ __device__ __forceinline__ bool overlap_v(const IceDemParams& P, float xl, float
     return ov > 0.0f;
 }
 
-__device__ __forceinline__ bool pair_overlap(const IceDemParams& P, const float* __restrict__ pos,
-                                             const float* __restrict__ radius, int n, int lo, int hi) {
+__device__ __forceinline__ bool pair_overlap(const IceDemParams& P, GAS const float* __restrict__ pos,
+                                             GAS const float* __restrict__ radius, int n, int lo, int hi) {
     return overlap_v(P, pos[lo], pos[n + lo], radius[lo], pos[hi], pos[n + hi], radius[hi]);
 }
 
@@ -209,8 +209,8 @@ __device__ __forceinline__ void contrib_v(const IceDemParams& P, bool me_is_lo,
     else { fx = fnx - ftx; fy = fny - fty; m = -0.5f * cross; }
 }
 
-__device__ __forceinline__ void pair_contrib(const IceDemParams& P, const float* __restrict__ pos,
-                                             const float* __restrict__ vel, const float* __restrict__ radius,
+__device__ __forceinline__ void pair_contrib(const IceDemParams& P, GAS const float* __restrict__ pos,
+                                             GAS const float* __restrict__ vel, GAS const float* __restrict__ radius,
                                              int n, int lo, int hi, bool me_is_lo, float& fx, float& fy, float& m) {
     contrib_v(P, me_is_lo, pos[lo], pos[n + lo], vel[lo], vel[n + lo], radius[lo],
               pos[hi], pos[n + hi], vel[hi], vel[n + hi], radius[hi], fx, fy, m);
@@ -251,7 +251,7 @@ __device__ __forceinline__ PrePass prepass(const IceDemParams& P, float pen, flo
 // Tree sum of one 256-element block of a near-masked per-element array
 // (far elements contribute +0), by one wave: lane l sums its 4 consecutive
 // elements pairwise, then the butterfly combines aligned groups.
-__device__ __forceinline__ float block256_tree(const DevView* __restrict__ V, const float* __restrict__ a,
+__device__ __forceinline__ float block256_tree(const DevView* __restrict__ V, GAS const float* __restrict__ a,
                                                int blk, float scale, bool scaled) {
     const int lane = threadIdx.x & 63;
     float v[4];
@@ -259,7 +259,7 @@ __device__ __forceinline__ float block256_tree(const DevView* __restrict__ V, co
     for (int i = 0; i < 4; ++i) {
         const int e = blk * BLOCK + lane * 4 + i;
         float x = 0.0f;
-        if (e < V->n && V->nearf[e]) x = scaled ? a[e] * scale : a[e];
+        if (e < V->n && G(V->nearf)[e]) x = scaled ? a[e] * scale : a[e];
         v[i] = x;
     }
     float x = (v[0] + v[1]) + (v[2] + v[3]);
@@ -268,24 +268,25 @@ __device__ __forceinline__ float block256_tree(const DevView* __restrict__ V, co
 
 // Tree sums of one 256-element block of three near-masked per-element arrays
 // (the order of block256_tree), every load issued before the first add.
-__device__ __forceinline__ void block256_tree3(const DevView* __restrict__ V, const float* __restrict__ a0,
-                                               const float* __restrict__ a1, const float* __restrict__ a2,
+__device__ __forceinline__ void block256_tree3(const DevView* __restrict__ V, GAS const float* __restrict__ a0,
+                                               GAS const float* __restrict__ a1, GAS const float* __restrict__ a2,
                                                int blk, float& s0, float& s1, float& s2) {
     const int lane = threadIdx.x & 63;
     const int e0 = blk * BLOCK + lane * 4;
     uint32_t nf = 0;
     float4 x0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), x1 = x0, x2 = x0;
     if (e0 + 3 < V->n) {
-        nf = *reinterpret_cast<const uint32_t*>(V->nearf + e0);
-        x0 = *reinterpret_cast<const float4*>(a0 + e0);
-        x1 = *reinterpret_cast<const float4*>(a1 + e0);
-        x2 = *reinterpret_cast<const float4*>(a2 + e0);
+        nf = *(GAS const uint32_t*)(G(V->nearf) + e0);
+        const f4u y0 = *(GAS const f4u*)(a0 + e0), y1 = *(GAS const f4u*)(a1 + e0), y2 = *(GAS const f4u*)(a2 + e0);
+        x0 = make_float4(y0.x, y0.y, y0.z, y0.w);
+        x1 = make_float4(y1.x, y1.y, y1.z, y1.w);
+        x2 = make_float4(y2.x, y2.y, y2.z, y2.w);
     } else {
         float v0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, v1[4] = {0.0f, 0.0f, 0.0f, 0.0f}, v2[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         #pragma unroll
         for (int i = 0; i < 4; ++i) {
             if (e0 + i < V->n) {
-                nf |= (uint32_t)V->nearf[e0 + i] << (8 * i);
+                nf |= (uint32_t)G(V->nearf)[e0 + i] << (8 * i);
                 v0[i] = a0[e0 + i]; v1[i] = a1[e0 + i]; v2[i] = a2[e0 + i];
             }
         }
@@ -313,11 +314,11 @@ __device__ __forceinline__ float wave_tree_small(float x, int npad) {
     return __shfl(x, 0, 64);
 }
 
-__device__ __forceinline__ float wave_tree_global(const float* __restrict__ a, int npad) {
+__device__ __forceinline__ float wave_tree_global(GAS const float* __restrict__ a, int npad) {
     const int lane = threadIdx.x & 63;
     if (npad <= 64) return wave_tree_small(lane < npad ? a[lane] : 0.0f, npad);
     const int c = npad >> 6;
-    const float* p = a + (size_t)lane * c;
+    GAS const float* p = a + (size_t)lane * c;
     float stk[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     #pragma unroll
     for (int i = 0; i < 64; ++i) {
@@ -351,24 +352,24 @@ __device__ __forceinline__ void cull_chunk(const DevView* __restrict__ V, int ch
     bool near = false;
     float r = 0.0f;
     if (valid) {
-        r = V->radius[e];
+        r = G(V->radius)[e];
         const float dx = x - sx, dy = y - sy;
-        const float R = V->bound[e] + P.ship_bound_radius + 1.0f;
+        const float R = G(V->bound)[e] + P.ship_bound_radius + 1.0f;
         near = (dx * dx + dy * dy) <= R * R;
-        V->nearf[e] = near ? 1 : 0;
-        if (near) V->near_list[atomicAdd(near_count(V), 1)] = e;
+        G(V->nearf)[e] = near ? 1 : 0;
+        if (near) G(V->near_list)[atomicAdd(near_count(V), 1)] = e;
         cell_insert(V, set, e, x, y, vx, vy, r);
     }
     const int cnt = __syncthreads_count(near ? 1 : 0);
     if (threadIdx.x == 0) {
-        V->nearcnt[chunk] = cnt;
-        V->blkdone[chunk] = 0;
+        G(V->nearcnt)[chunk] = cnt;
+        G(V->blkdone)[chunk] = 0;
         if (cnt > 0) {
             atomicAdd(nearblk_count(V), 1);
         } else {
-            V->part1[chunk] = 0.0f;
-            V->part1[V->nbpad + chunk] = 0.0f;
-            V->part1[2 * V->nbpad + chunk] = 0.0f;
+            G(V->part1)[chunk] = 0.0f;
+            G(V->part1)[V->nbpad + chunk] = 0.0f;
+            G(V->part1)[2 * V->nbpad + chunk] = 0.0f;
         }
     }
 }
@@ -376,18 +377,18 @@ __device__ __forceinline__ void cull_chunk(const DevView* __restrict__ V, int ch
 // sin/cos of the ship heading for the next k_near (scal[8..11])
 __device__ __forceinline__ void store_pose(const DevView* __restrict__ V, const float* __restrict__ ship) {
     const ShipPose SP = make_pose(ship[0], ship[1], ship[2], V->p.half_len, V->p.half_beam);
-    V->scal[8] = SP.s_neg; V->scal[9] = SP.c_neg; V->scal[10] = SP.s_h; V->scal[11] = SP.c_h;
+    G(V->scal)[8] = SP.s_neg; G(V->scal)[9] = SP.c_neg; G(V->scal)[10] = SP.s_h; G(V->scal)[11] = SP.c_h;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_cull(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
     const int n = V->n;
     const int e = blockIdx.x * BLOCK + threadIdx.x;
     const bool valid = e < n;
-    const float* pos = D->s.d_pos;
-    const float* vel = D->s.d_vel;
+    GAS const float* pos = G(D->s.d_pos);
+    GAS const float* vel = G(D->s.d_vel);
     float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f;
     if (valid) { x = pos[e]; y = pos[n + e]; vx = vel[e]; vy = vel[n + e]; }
-    cull_chunk(V, blockIdx.x, valid, e, x, y, vx, vy, D->s.d_ship[0], D->s.d_ship[1], 0);
+    cull_chunk(V, blockIdx.x, valid, e, x, y, vx, vy, G(D->s.d_ship)[0], G(D->s.d_ship)[1], 0);
     if (blockIdx.x == 0 && threadIdx.x == 0) store_pose(V, D->s.d_ship);
 }
 
@@ -399,13 +400,13 @@ __global__ __launch_bounds__(BLOCK) void k_cellkey(const DevView* __restrict__ V
     const int e = blockIdx.x * BLOCK + threadIdx.x;
     const bool valid = e < n;
     if (from_state) { pos = D->s.d_pos; vel = D->s.d_vel; }
-    if (valid) cell_insert(V, set, e, pos[e], pos[n + e], vel[e], vel[n + e], V->radius[e]);
+    if (valid) cell_insert(V, set, e, pos[e], pos[n + e], vel[e], vel[n + e], G(V->radius)[e]);
 }
 
 __device__ __forceinline__ void clear_cells(const DevView* __restrict__ V, int set, int t) {
-    if (t < V->G) *reinterpret_cast<int*>(rec_ptr(V, fine_rec(V, set, t))) = 0;
-    if (t < V->G2) *reinterpret_cast<int*>(rec_ptr(V, coarse_rec(V, set, t))) = 0;
-    if (t == 0) V->ovf_count[set] = 0;
+    if (t < V->G) *(GAS int*)rec_ptr(V, fine_rec(V, set, t)) = 0;
+    if (t < V->G2) *(GAS int*)rec_ptr(V, coarse_rec(V, set, t)) = 0;
+    if (t == 0) G(V->ovf_count)[set] = 0;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_clear_cells(const DevView* __restrict__ V, int set) {
@@ -421,30 +422,30 @@ __device__ __forceinline__ void near_elem(const DevView* __restrict__ V, const C
     const IceDemParams& P = V->p;
     const int n = V->n, maxv = V->maxv;
     const int lane = threadIdx.x & 63;
-    const float* pos = D->s.d_pos;
+    GAS const float* pos = G(D->s.d_pos);
     const float px = pos[e], py = pos[n + e];
-    const float rad = V->radius[e], thk = V->thick[e];
+    const float rad = G(V->radius)[e], thk = G(V->thick)[e];
     float wx = 0.0f, wy = 0.0f;
     bool mv = false;
     float2 lv = make_float2(0.0f, 0.0f);
     if (lane < maxv) {
-        lv = V->polys[(size_t)e * maxv + lane];
-        mv = V->mask[(size_t)e * maxv + lane] != 0;
+        lv = ldg(G(V->polys) + (size_t)e * maxv + lane);
+        mv = G(V->mask)[(size_t)e * maxv + lane] != 0;
     }
     float se, ce;
-    dm_sincosf(D->s.d_rot[e], &se, &ce);
+    dm_sincosf(G(D->s.d_rot)[e], &se, &ce);
     if (lane < maxv) {
         wx = ce * lv.x - se * lv.y + px;
         wy = se * lv.x + ce * lv.y + py;
     }
     float pen, nxo, nyo, cxo, cyo;
-    hull_element(SP, P, V->hull, V->S, maxv, wx, wy, mv, pen, nxo, nyo, cxo, cyo);
+    hull_element(SP, P, G(V->hull), V->S, maxv, wx, wy, mv, pen, nxo, nyo, cxo, cyo);
     if (lane == 0) {
         PrePass o = prepass(P, pen, nxo, nyo, rad, thk, px, py, SP.sx, SP.sy);
-        V->pen[e] = pen; V->nx[e] = nxo; V->ny[e] = nyo;
-        V->fm_raw[e] = o.fm; V->soi_x[e] = o.sx; V->soi_y[e] = o.sy; V->mterm[e] = o.m;
+        G(V->pen)[e] = pen; G(V->nx)[e] = nxo; G(V->ny)[e] = nyo;
+        G(V->fm_raw)[e] = o.fm; G(V->soi_x)[e] = o.sx; G(V->soi_y)[e] = o.sy; G(V->mterm)[e] = o.m;
         if (pen > 0.0f) {
-            const int f = V->floe[e];
+            const int f = G(V->floe)[e];
             atomicOr(&V->floe_coll[f], 1u);
             atomicMax(&V->floe_maxfm[f], __float_as_uint(o.fm));
         }
@@ -459,10 +460,10 @@ __device__ __forceinline__ void near_block_done(const DevView* __restrict__ V, i
     const int lane = threadIdx.x & 63, nbpad = V->nbpad;
     __threadfence();
     float a, c, d;
-    block256_tree3(V, V->soi_x, V->soi_y, V->mterm, b, a, c, d);
+    block256_tree3(V, G(V->soi_x), G(V->soi_y), G(V->mterm), b, a, c, d);
     int lastb = 0;
     if (lane == 0) {
-        V->part1[b] = a; V->part1[nbpad + b] = c; V->part1[2 * nbpad + b] = d;
+        G(V->part1)[b] = a; G(V->part1)[nbpad + b] = c; G(V->part1)[2 * nbpad + b] = d;
         __threadfence();
         lastb = (atomicAdd(near_ticket(V), 1) == *nearblk_count(V) - 1) ? 1 : 0;
     }
@@ -470,21 +471,21 @@ __device__ __forceinline__ void near_block_done(const DevView* __restrict__ V, i
     __threadfence();
     float tx, ty, tm;
     if (nbpad <= 64) {   // the three loads in one round trip
-        const float l0 = lane < nbpad ? V->part1[lane] : 0.0f;
-        const float l1 = lane < nbpad ? V->part1[nbpad + lane] : 0.0f;
-        const float l2 = lane < nbpad ? V->part1[2 * nbpad + lane] : 0.0f;
+        const float l0 = lane < nbpad ? G(V->part1)[lane] : 0.0f;
+        const float l1 = lane < nbpad ? G(V->part1)[nbpad + lane] : 0.0f;
+        const float l2 = lane < nbpad ? G(V->part1)[2 * nbpad + lane] : 0.0f;
         tx = wave_tree_small(l0, nbpad); ty = wave_tree_small(l1, nbpad); tm = wave_tree_small(l2, nbpad);
     } else {
-        tx = wave_tree_global(V->part1, nbpad);
-        ty = wave_tree_global(V->part1 + nbpad, nbpad);
-        tm = wave_tree_global(V->part1 + 2 * nbpad, nbpad);
+        tx = wave_tree_global(G(V->part1), nbpad);
+        ty = wave_tree_global(G(V->part1) + nbpad, nbpad);
+        tm = wave_tree_global(G(V->part1) + 2 * nbpad, nbpad);
     }
     float scale = 1.0f;
     if (P.has_thrust_cap) {
         const float F_tot = sqrtf(tx * tx + ty * ty);
         scale = (F_tot > P.max_ship_ice_force) ? (P.max_ship_ice_force / vmax(F_tot, 1e-6f)) : 1.0f;
     }
-    if (lane == 0) { V->scal[0] = scale; V->scal[1] = tm; V->scal[4] = tx; V->scal[5] = ty; }
+    if (lane == 0) { G(V->scal)[0] = scale; G(V->scal)[1] = tm; G(V->scal)[4] = tx; G(V->scal)[5] = ty; }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_near(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
@@ -492,25 +493,25 @@ __global__ __launch_bounds__(BLOCK) void k_near(const DevView* __restrict__ V, c
     const int wave = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
     const int nwaves = gridDim.x * (BLOCK / 64);
     const int count = *near_count(V);
-    const int e0 = (wave < V->n) ? V->near_list[wave] : 0;   // read beside the count
+    const int e0 = (wave < V->n) ? G(V->near_list)[wave] : 0;   // read beside the count
     if (count == 0) {
-        if (wave == 0 && lane == 0) { V->scal[0] = 1.0f; V->scal[1] = 0.0f; V->scal[4] = 0.0f; V->scal[5] = 0.0f; }
+        if (wave == 0 && lane == 0) { G(V->scal)[0] = 1.0f; G(V->scal)[1] = 0.0f; G(V->scal)[4] = 0.0f; G(V->scal)[5] = 0.0f; }
         return;
     }
     if (wave >= count) return;
     const float* ship = D->s.d_ship;
     ShipPose SP;   // heading sin/cos from the frame's ship step
     SP.sx = ship[0]; SP.sy = ship[1];
-    SP.s_neg = V->scal[8]; SP.c_neg = V->scal[9]; SP.s_h = V->scal[10]; SP.c_h = V->scal[11];
+    SP.s_neg = G(V->scal)[8]; SP.c_neg = G(V->scal)[9]; SP.s_h = G(V->scal)[10]; SP.c_h = G(V->scal)[11];
     SP.hl = V->p.half_len; SP.hb = V->p.half_beam;
     for (int q = wave; q < count; q += nwaves) {
-        const int e = (q == wave) ? e0 : V->near_list[q];
+        const int e = (q == wave) ? e0 : G(V->near_list)[q];
         near_elem(V, D, SP, e);
         int fin = 0;
         if (lane == 0) {
             const int b = e / BLOCK;
             __threadfence();
-            fin = (atomicAdd(&V->blkdone[b], 1) == V->nearcnt[b] - 1) ? 1 : 0;
+            fin = (atomicAdd(&V->blkdone[b], 1) == G(V->nearcnt)[b] - 1) ? 1 : 0;
         }
         if (__shfl(fin, 0, 64)) near_block_done(V, e / BLOCK);
     }
@@ -531,8 +532,8 @@ __device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const 
     if (P.ship_mode == 1) {
         if (D->in.d_controls && D->in.n_knots > 0) {
             const int knot = min(k / max(D->in.hold, 1), D->in.n_knots - 1);
-            cu = D->in.d_controls[2 * knot];
-            cr = D->in.d_controls[2 * knot + 1];
+            cu = G(D->in.d_controls)[2 * knot];
+            cr = G(D->in.d_controls)[2 * knot + 1];
         }
     } else {
         const float* sp = D->in.d_setpoints + (size_t)k * 6;
@@ -547,7 +548,7 @@ __device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const 
         }
     }
     float vwx = 0.0f, vwy = 0.0f;
-    if (D->in.d_v_water_ship) { vwx = D->in.d_v_water_ship[2 * k]; vwy = D->in.d_v_water_ship[2 * k + 1]; }
+    if (D->in.d_v_water_ship) { vwx = G(D->in.d_v_water_ship)[2 * k]; vwy = G(D->in.d_v_water_ship)[2 * k + 1]; }
     float st[ICEDEM_SHIP_STATE_LEN];
     for (int i = 0; i < ICEDEM_SHIP_STATE_LEN; ++i) st[i] = ship[i];
     const float Fm = sqrtf(Fx * Fx + Fy * Fy);
@@ -567,12 +568,12 @@ __device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const 
         const float ke = 0.5f * P.ship_mass * (st[3] * st[3] + st[4] * st[4]) + 0.5f * P.ship_Izz * (st[5] * st[5]);
         ro[1] = ro[3] - ke;
     }
-    if (D->h.d_eta) { D->h.d_eta[3 * k] = st[0]; D->h.d_eta[3 * k + 1] = st[1]; D->h.d_eta[3 * k + 2] = st[2]; }
-    if (D->h.d_J) D->h.d_J[k] = J;
-    if (D->h.d_F) { D->h.d_F[2 * k] = Fx; D->h.d_F[2 * k + 1] = Fy; }
+    if (D->h.d_eta) { G(D->h.d_eta)[3 * k] = st[0]; G(D->h.d_eta)[3 * k + 1] = st[1]; G(D->h.d_eta)[3 * k + 2] = st[2]; }
+    if (D->h.d_J) G(D->h.d_J)[k] = J;
+    if (D->h.d_F) { G(D->h.d_F)[2 * k] = Fx; G(D->h.d_F)[2 * k + 1] = Fy; }
     if (last) {
-        if (D->dg.d_F_ice_on_ship) { D->dg.d_F_ice_on_ship[0] = Fx; D->dg.d_F_ice_on_ship[1] = Fy; }
-        if (D->dg.d_J) D->dg.d_J[0] = J;
+        if (D->dg.d_F_ice_on_ship) { G(D->dg.d_F_ice_on_ship)[0] = Fx; G(D->dg.d_F_ice_on_ship)[1] = Fy; }
+        if (D->dg.d_J) G(D->dg.d_J)[0] = J;
     }
 }
 
@@ -583,8 +584,8 @@ __device__ __forceinline__ void floe_eligibility(const DevView* __restrict__ V, 
     if (V->nb <= 0) return;
     uint8_t* elig = D->s.d_floe_elig;
     for (int f = threadIdx.x; f < V->nf; f += blockDim.x) {
-        bool coll = V->floe_coll[f] != 0u;
-        float mx = __uint_as_float(V->floe_maxfm[f]);
+        bool coll = G(V->floe_coll)[f] != 0u;
+        float mx = __uint_as_float(G(V->floe_maxfm)[f]);
         if (P.has_thrust_cap) mx = mx * scale;
         bool el = (elig[f] != 0) || (coll && (mx > P.elig_threshold));
         elig[f] = el ? 1 : 0;
@@ -601,12 +602,12 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
     const bool capped = P.has_thrust_cap != 0;
     if (capped && threadIdx.x < 64) {
         float a = 0.0f, c = 0.0f;
-        if (V->nearcnt[b] > 0) {
-            const float s = V->scal[0];
-            a = block256_tree(V, V->soi_x, b, s, true);
-            c = block256_tree(V, V->soi_y, b, s, true);
+        if (G(V->nearcnt)[b] > 0) {
+            const float s = G(V->scal)[0];
+            a = block256_tree(V, G(V->soi_x), b, s, true);
+            c = block256_tree(V, G(V->soi_y), b, s, true);
         }
-        if (threadIdx.x == 0) { V->part2[b] = a; V->part2[nbpad + b] = c; }
+        if (threadIdx.x == 0) { G(V->part2)[b] = a; G(V->part2)[nbpad + b] = c; }
     }
     if (threadIdx.x == 0) {
         __threadfence();
@@ -615,20 +616,20 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
     __syncthreads();
     if (!*am_last) return;
     __threadfence();
-    const float scale = V->scal[0];
+    const float scale = G(V->scal)[0];
     float fx = 0.0f, fy = 0.0f;
     if (threadIdx.x < 64) {
         if (capped) {
-            fx = wave_tree_global(V->part2, nbpad);
-            fy = wave_tree_global(V->part2 + nbpad, nbpad);
+            fx = wave_tree_global(G(V->part2), nbpad);
+            fy = wave_tree_global(G(V->part2) + nbpad, nbpad);
         } else {
-            fx = V->scal[4];
-            fy = V->scal[5];
+            fx = G(V->scal)[4];
+            fy = G(V->scal)[5];
         }
     }
     floe_eligibility(V, D, scale);
     if (threadIdx.x == 0) {
-        ship_frame(V, D, k, last, fx, fy, V->scal[1]);
+        ship_frame(V, D, k, last, fx, fy, G(V->scal)[1]);
         store_pose(V, D->s.d_ship);
         *near_count(V) = 0;
         *nearblk_count(V) = 0;
@@ -696,7 +697,7 @@ __device__ __forceinline__ void scan_list(const DevView* __restrict__ V, int set
         int rix = 0, cc = 0;
         if (q < S.ntot) {
             rix = list_rec(V, set, S, q);
-            cc = min(*reinterpret_cast<const int*>(rec_ptr(V, rix)), rec_cap(V, rix));
+            cc = min(*(GAS const int*)rec_ptr(V, rix), rec_cap(V, rix));
         }
         const int incl = group_incl_scan(cc);
         const int total = group_last(incl);
@@ -727,8 +728,8 @@ __device__ __forceinline__ void scan_list(const DevView* __restrict__ V, int set
 // reference's scatter order; result valid in the group's lane 0.
 // (px, py, pvx, pvy, pr) is e's own state; partners come from bucket set
 // `set` (and, when buckets overflowed, from pos/vel/radius).
-__device__ __forceinline__ void contact_force_group(const DevView* __restrict__ V, int set, const float* __restrict__ pos,
-                                    const float* __restrict__ vel, int e, float px, float py, float pvx,
+__device__ __forceinline__ void contact_force_group(const DevView* __restrict__ V, int set, GAS const float* __restrict__ pos,
+                                    GAS const float* __restrict__ vel, int e, float px, float py, float pvx,
                                     float pvy, float pr, int novf, ForceLds& L, int g,
                                     float& fx, float& fy, float& fm) {
     const IceDemParams& P = V->p;
@@ -765,8 +766,8 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
         int j = -1;
         float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f, r = 0.0f;
         if (f < novf) {
-            j = V->ovf_list[(size_t)set * V->ovf_cap + f];
-            x = pos[j]; y = pos[n + j]; vx = vel[j]; vy = vel[n + j]; r = V->radius[j];
+            j = G(V->ovf_list)[(size_t)set * V->ovf_cap + f];
+            x = pos[j]; y = pos[n + j]; vx = vel[j]; vy = vel[n + j]; r = G(V->radius)[j];
             if (j != e) ov = (e < j) ? overlap_v(P, px, py, pr, x, y, r) : overlap_v(P, x, y, r, px, py, pr);
         }
         push(ov, j, x, y, vx, vy, r);
@@ -818,26 +819,26 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
                 if (j == e) return;
                 const int kj = rot_key(j, e, n);
                 if (kj <= last || kj >= best) return;
-                if (!pair_overlap(P, pos, V->radius, n, min(e, j), max(e, j))) return;
+                if (!pair_overlap(P, pos, G(V->radius), n, min(e, j), max(e, j))) return;
                 best = kj; bj = j;
             };
             for (int gy = y0; gy <= y1; ++gy)
                 for (int gx = x0; gx <= x1; ++gx) {
-                    const uint8_t* rec = rec_ptr(V, fine_rec(V, set, gy * P.grid_nx + gx));
-                    const int cc = min(*reinterpret_cast<const int*>(rec), P.grid_bucket);
+                    GAS const uint8_t* rec = rec_ptr(V, fine_rec(V, set, gy * P.grid_nx + gx));
+                    const int cc = min(*(GAS const int*)rec, P.grid_bucket);
                     for (int s = 0; s < cc; ++s) consider(slot_load(rec, s).e);
                 }
             for (int gy = v0; gy <= v1; ++gy)
                 for (int gx = u0; gx <= u1; ++gx) {
-                    const uint8_t* rec = rec_ptr(V, coarse_rec(V, set, gy * V->nx2 + gx));
-                    const int cc = min(*reinterpret_cast<const int*>(rec), V->cap2);
+                    GAS const uint8_t* rec = rec_ptr(V, coarse_rec(V, set, gy * V->nx2 + gx));
+                    const int cc = min(*(GAS const int*)rec, V->cap2);
                     for (int s = 0; s < cc; ++s) consider(slot_load(rec, s).e);
                 }
-            for (int s = 0; s < novf; ++s) consider(V->ovf_list[(size_t)set * V->ovf_cap + s]);
+            for (int s = 0; s < novf; ++s) consider(G(V->ovf_list)[(size_t)set * V->ovf_cap + s]);
             if (bj < 0) break;
             last = best;
             float a, b, c;
-            pair_contrib(P, pos, vel, V->radius, n, min(e, bj), max(e, bj), e < bj, a, b, c);
+            pair_contrib(P, pos, vel, G(V->radius), n, min(e, bj), max(e, bj), e < bj, a, b, c);
             fx = fx + a; fy = fy + b; fm = fm + c;
         }
     }
@@ -851,13 +852,13 @@ struct BondIn {
 };
 
 __device__ __forceinline__ BondIn bond_in(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                          const float* __restrict__ pos, int4 t4, int mode) {
+                                          GAS const float* __restrict__ pos, int4 t4, int mode) {
     BondIn o;
     o.ox = pos[t4.y]; o.oy = pos[V->n + t4.y];
-    o.intact = D->s.d_bond_intact[t4.x] != 0;
-    o.dmg = D->s.d_bond_damage[t4.x];
-    o.act = D->s.d_active[t4.y] != 0;
-    if (mode != 1) o.act = o.act || (V->floe_coll[t4.z] != 0u);
+    o.intact = G(D->s.d_bond_intact)[t4.x] != 0;
+    o.dmg = G(D->s.d_bond_damage)[t4.x];
+    o.act = G(D->s.d_active)[t4.y] != 0;
+    if (mode != 1) o.act = o.act || (G(V->floe_coll)[t4.z] != 0u);
     return o;
 }
 
@@ -867,7 +868,7 @@ __device__ __forceinline__ BondIn bond_in(const DevView* __restrict__ V, const C
 // as the bond pass sees it; (t0, p0, in0) is the lane's entry l with its
 // state, read ahead by the caller.  Result valid in lane 0.
 __device__ __forceinline__ void bond_force_group(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                                 const float* __restrict__ pos, int e, float px, float py,
+                                                 GAS const float* __restrict__ pos, int e, float px, float py,
                                                  bool act_e, int deg, int4 t0, float2 p0, BondIn in0, int mode,
                                                  ForceLds& L, int g, float& fbx, float& fby, float& mb,
                                                  bool& crack) {
@@ -883,8 +884,8 @@ __device__ __forceinline__ void bond_force_group(const DevView* __restrict__ V, 
             float2 pr = p0;
             BondIn bi = in0;
             if (base > 0) {
-                t4 = V->btab[(size_t)e * V->W + q];
-                pr = V->bprm[(size_t)e * V->W + q];
+                t4 = ldg(G(V->btab) + (size_t)e * V->W + q);
+                pr = ldg(G(V->bprm) + (size_t)e * V->W + q);
                 bi = bond_in(V, D, pos, t4, mode);
             }
             const bool role_j = t4.w != 0;
@@ -932,41 +933,41 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
     const int g = threadIdx.x / GRP, l = threadIdx.x & (GRP - 1);
     const int e = chunk * EPB + g;
     if (e >= n) return;                       // group-uniform
-    const float* pos = src_is_state ? D->s.d_pos : V->nxt_pos;
-    const float* vel = src_is_state ? D->s.d_vel : V->nxt_vel;
-    const float* rotp = src_is_state ? D->s.d_rot : V->nxt_rot;
-    const float* omp = src_is_state ? D->s.d_omega : V->nxt_omega;
-    float* dpos = src_is_state ? V->nxt_pos : D->s.d_pos;
-    float* dvel = src_is_state ? V->nxt_vel : D->s.d_vel;
-    float* drot = src_is_state ? V->nxt_rot : D->s.d_rot;
-    float* domg = src_is_state ? V->nxt_omega : D->s.d_omega;
-    const uint8_t* active = D->s.d_active;
+    GAS const float* pos = G(src_is_state ? D->s.d_pos : V->nxt_pos);
+    GAS const float* vel = G(src_is_state ? D->s.d_vel : V->nxt_vel);
+    GAS const float* rotp = G(src_is_state ? D->s.d_rot : V->nxt_rot);
+    GAS const float* omp = G(src_is_state ? D->s.d_omega : V->nxt_omega);
+    GAS float* dpos = G(src_is_state ? V->nxt_pos : D->s.d_pos);
+    GAS float* dvel = G(src_is_state ? V->nxt_vel : D->s.d_vel);
+    GAS float* drot = G(src_is_state ? V->nxt_rot : D->s.d_rot);
+    GAS float* domg = G(src_is_state ? V->nxt_omega : D->s.d_omega);
+    GAS const uint8_t* active = G(D->s.d_active);
     // everything that depends only on e, read in one round trip
     const float px = pos[e], py = pos[n + e], vx = vel[e], vy = vel[n + e];
     const float ro = rotp[e], om = omp[e];
-    const float r = V->radius[e], h = V->thick[e], area = V->area[e];
-    const float mass = V->mass[e], inertia = V->inertia[e];
-    const int fl = V->floe[e];
+    const float r = G(V->radius)[e], h = G(V->thick)[e], area = G(V->area)[e];
+    const float mass = G(V->mass)[e], inertia = G(V->inertia)[e];
+    const int fl = G(V->floe)[e];
     const bool act_raw = active[e] != 0;
-    const int deg = V->W > 0 ? V->bdeg[e] : 0;
-    const int novf = V->ovf_count[set];
+    const int deg = V->W > 0 ? G(V->bdeg)[e] : 0;
+    const int novf = G(V->ovf_count)[set];
     int4 bt0 = make_int4(0, 0, 0, 0);
     float2 bp0 = make_float2(0.0f, 0.0f);
-    if (l < V->W) { bt0 = V->btab[(size_t)e * V->W + l]; bp0 = V->bprm[(size_t)e * V->W + l]; }
-    const float sub0 = D->s.d_sub[e];
+    if (l < V->W) { bt0 = ldg(G(V->btab) + (size_t)e * V->W + l); bp0 = ldg(G(V->bprm) + (size_t)e * V->W + l); }
+    const float sub0 = G(D->s.d_sub)[e];
     const bool first = (mode == 0 && sub == 0);
-    const bool is_near = first && V->nearf[e] != 0;
+    const bool is_near = first && G(V->nearf)[e] != 0;
     float pen = 0.0f, fmr = 0.0f, sx_ = 0.0f, sy_ = 0.0f, nxe = 0.0f, nye = 0.0f;
-    if (is_near) { pen = V->pen[e]; fmr = V->fm_raw[e]; sx_ = V->soi_x[e]; sy_ = V->soi_y[e]; nxe = V->nx[e]; nye = V->ny[e]; }
-    const float tscale = first ? V->scal[0] : 1.0f;
+    if (is_near) { pen = G(V->pen)[e]; fmr = G(V->fm_raw)[e]; sx_ = G(V->soi_x)[e]; sy_ = G(V->soi_y)[e]; nxe = G(V->nx)[e]; nye = G(V->ny)[e]; }
+    const float tscale = first ? G(V->scal)[0] : 1.0f;
     float vwx = 0.0f, vwy = 0.0f;
-    const float* vwp = D->in.d_v_water;
+    GAS const float* vwp = G(D->in.d_v_water);
     if (vwp) {
         const int64_t base = (int64_t)k * D->in.vw_stride_k + (int64_t)e * D->in.vw_stride_e;
         vwx = vwp[base];
         vwy = vwp[base + D->in.vw_stride_c];
     }
-    const bool fcoll = (mode != 1) && (V->floe_coll[fl] != 0u);
+    const bool fcoll = (mode != 1) && (G(V->floe_coll)[fl] != 0u);
     const bool act_bond = act_raw || fcoll;   // activity seen by the bond pass (mode 0)
     BondIn bin0 = {0.0f, 0.0f, 0.0f, false, false};
     if (l < deg) bin0 = bond_in(V, D, pos, bt0, mode);   // read ahead of the contact pass
@@ -985,25 +986,25 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
         subv = sub0;
     } else if (sub == 0) {
         act = act_bond;
-        D->s.d_active[e] = act ? 1 : 0;
-        V->fm[e] = P.has_thrust_cap ? fmr * tscale : fmr;
+        G(D->s.d_active)[e] = act ? 1 : 0;
+        G(V->fm)[e] = P.has_thrust_cap ? fmr * tscale : fmr;
         const bool hc = pen > 0.0f;
         const float pen_c = vmin(pen, P.submergence_pen_cap);
         float ns = hc ? vmin(sub0 + P.submergence_rate * pen_c * P.dt, h)
                       : vmax(0.0f, sub0 - P.inv_rise_time * sub0 * P.dt);
-        D->s.d_sub[e] = ns;
+        G(D->s.d_sub)[e] = ns;
         subv = ns;
         float z = vmin(ns, h);
         float Fb = P.buoyancy_coef * area * z;
         shx = sx_ + (hc ? Fb * nxe : 0.0f);
         shy = sy_ + (hc ? Fb * nye : 0.0f);
-        if (P.n_ice_steps > 1) { V->shipf[e] = shx; V->shipf[n + e] = shy; }
+        if (P.n_ice_steps > 1) { G(V->shipf)[e] = shx; G(V->shipf)[n + e] = shy; }
     } else {
         act = act_raw;
-        shx = V->shipf[e]; shy = V->shipf[n + e];
+        shx = G(V->shipf)[e]; shy = G(V->shipf)[n + e];
         subv = sub0;
     }
-    if (sub == 0 && mode == 0) V->crack[e] = crack ? 1 : 0;
+    if (sub == 0 && mode == 0) G(V->crack)[e] = crack ? 1 : 0;
     // drag (:167-192)
     float A_drag = area * (h / P.drag_ref_thickness);
     float vrx = vx - vwx, vry = vy - vwy;
@@ -1092,24 +1093,24 @@ __device__ __forceinline__ void post_body(const DevView* __restrict__ V, const C
                                           int k, int final_in_nxt, int last, int t, int chunk, int set, int cull) {
     const IceDemParams& P = V->p;
     const int n = V->n, nb = V->nb;
-    const float* pos = final_in_nxt ? V->nxt_pos : D->s.d_pos;
-    const float* vel = final_in_nxt ? V->nxt_vel : D->s.d_vel;
+    GAS const float* pos = G(final_in_nxt ? V->nxt_pos : D->s.d_pos);
+    GAS const float* vel = G(final_in_nxt ? V->nxt_vel : D->s.d_vel);
     if (t < nb) {
-        const int i = V->bi[t], j = V->bj[t];
+        const int i = G(V->bi)[t], j = G(V->bj)[t];
         float dx = pos[j] - pos[i];
         float dy = pos[n + j] - pos[n + i];
         float dist = sqrtf(dx * dx + dy * dy);
         float ds_ = vmax(dist, 1e-10f);
         float dirx = dx / ds_, diry = dy / ds_;
-        const uint8_t* act = D->s.d_active;
-        const uint8_t* elig = D->s.d_floe_elig;
+        GAS const uint8_t* act = G(D->s.d_active);
+        GAS const uint8_t* elig = G(D->s.d_floe_elig);
         bool both = act[i] && act[j];
-        bool floe_ok = elig[V->floe[i]] && elig[V->floe[j]];
-        const float hi_ = V->thick[i], hj = V->thick[j];
+        bool floe_ok = elig[G(V->floe)[i]] && elig[G(V->floe)[j]];
+        const float hi_ = G(V->thick)[i], hj = G(V->thick)[j];
         bool breakable = (hi_ <= P.max_fracture_thickness) && (hj <= P.max_fracture_thickness);
-        const float fmi = V->fm[i], fmj = V->fm[j];
+        const float fmi = G(V->fm)[i], fmj = G(V->fm)[j];
         bool near = (fmi > 0.0f) || (fmj > 0.0f);
-        bool at_crack = V->crack[i] || V->crack[j];
+        bool at_crack = G(V->crack)[i] || G(V->crack)[j];
         bool dpdem_ok = both && floe_ok && (near || at_crack) && breakable;
         float vrx = vel[j] - vel[i];
         float vry = vel[n + j] - vel[n + i];
@@ -1117,63 +1118,63 @@ __device__ __forceinline__ void post_body(const DevView* __restrict__ V, const C
         float vtx = vrx - vn * dirx, vty = vry - vn * diry;
         float ds_incr = sqrtf(vtx * vtx + vty * vty) * P.dt_ice;
         bool gate = near && both && floe_ok && breakable;
-        const float ds_old = D->s.d_bond_ds[t];
+        const float ds_old = G(D->s.d_bond_ds)[t];
         float new_ds = ds_old + (gate ? ds_incr : 0.0f);
-        const float rest = V->rest[t];
+        const float rest = G(V->rest)[t];
         float L0 = vmax(rest, 1e-10f);
         float delta_n = dist - rest;
         float dn_d = near ? delta_n : 0.0f;
         float sn_el = P.k_n * (dn_d / L0);
         float st_el = P.k_s * (new_ds / L0);
-        const float dmg = D->s.d_bond_damage[t];
+        const float dmg = G(D->s.d_bond_damage)[t];
         float sigma_n = sn_el * (1.0f - dmg);
         float sigma_t = st_el * (1.0f - dmg);
-        const float dmx = D->s.d_bond_dmmax[t], dm0 = D->s.d_bond_dm0[t];
-        const bool intact = D->s.d_bond_intact[t] != 0;
+        const float dmx = G(D->s.d_bond_dmmax)[t], dm0 = G(D->s.d_bond_dm0)[t];
+        const bool intact = G(D->s.d_bond_intact)[t] != 0;
         float nd, ndmx, ndm0;
         bool nint;
-        bond_damage_update(delta_n, new_ds, sigma_n, sigma_t, V->sbar[t], P.bond_cohesion, P.bond_friction,
+        bond_damage_update(delta_n, new_ds, sigma_n, sigma_t, G(V->sbar)[t], P.bond_cohesion, P.bond_friction,
                            P.G_c_I, P.G_c_II, P.eta_bk, dmg, dmx, dm0, intact, nd, ndmx, ndm0, nint);
         float out_d = dpdem_ok ? nd : dmg;
         float out_mx = dpdem_ok ? ndmx : dmx;
         float out_m0 = dpdem_ok ? ndm0 : dm0;
         bool int_d = (dpdem_ok && breakable) ? nint : intact;
         float cf = vmax(fmi, fmj);
-        float eff = V->bstr[t] * P.eff_strength_collision;
+        float eff = G(V->bstr)[t] * P.eff_strength_collision;
         bool brk = intact && near && (cf > eff) && breakable;
         bool fin = int_d && !brk;
-        D->s.d_bond_damage[t] = out_d;
-        D->s.d_bond_dmmax[t] = out_mx;
-        D->s.d_bond_dm0[t] = out_m0;
-        D->s.d_bond_ds[t] = new_ds;
-        D->s.d_bond_intact[t] = fin ? 1 : 0;
-        if (D->h.d_bond_intact) D->h.d_bond_intact[(size_t)k * nb + t] = fin ? 1 : 0;
+        G(D->s.d_bond_damage)[t] = out_d;
+        G(D->s.d_bond_dmmax)[t] = out_mx;
+        G(D->s.d_bond_dm0)[t] = out_m0;
+        G(D->s.d_bond_ds)[t] = new_ds;
+        G(D->s.d_bond_intact)[t] = fin ? 1 : 0;
+        if (D->h.d_bond_intact) G(D->h.d_bond_intact)[(size_t)k * nb + t] = fin ? 1 : 0;
     }
     float px = 0.0f, py = 0.0f, pvx = 0.0f, pvy = 0.0f;
     if (t < n) {
         px = pos[t]; py = pos[n + t];
         pvx = vel[t]; pvy = vel[n + t];
         if (final_in_nxt) {
-            D->s.d_pos[t] = px; D->s.d_pos[n + t] = py;
-            D->s.d_vel[t] = pvx; D->s.d_vel[n + t] = pvy;
-            D->s.d_rot[t] = V->nxt_rot[t]; D->s.d_omega[t] = V->nxt_omega[t];
+            G(D->s.d_pos)[t] = px; G(D->s.d_pos)[n + t] = py;
+            G(D->s.d_vel)[t] = pvx; G(D->s.d_vel)[n + t] = pvy;
+            G(D->s.d_rot)[t] = G(V->nxt_rot)[t]; G(D->s.d_omega)[t] = G(V->nxt_omega)[t];
         }
         if (D->h.d_positions) {
-            D->h.d_positions[(size_t)k * 2 * n + t] = px;
-            D->h.d_positions[(size_t)k * 2 * n + n + t] = py;
+            G(D->h.d_positions)[(size_t)k * 2 * n + t] = px;
+            G(D->h.d_positions)[(size_t)k * 2 * n + n + t] = py;
         }
-        if (D->h.d_rotations) D->h.d_rotations[(size_t)k * n + t] = final_in_nxt ? V->nxt_rot[t] : D->s.d_rot[t];
-        if (D->h.d_submergence) D->h.d_submergence[(size_t)k * n + t] = D->s.d_sub[t];
+        if (D->h.d_rotations) G(D->h.d_rotations)[(size_t)k * n + t] = final_in_nxt ? G(V->nxt_rot)[t] : G(D->s.d_rot)[t];
+        if (D->h.d_submergence) G(D->h.d_submergence)[(size_t)k * n + t] = G(D->s.d_sub)[t];
         if (last) {
-            if (D->dg.d_pen_max) D->dg.d_pen_max[t] = V->nearf[t] ? V->pen[t] : 0.0f;
-            if (D->dg.d_force_mag) D->dg.d_force_mag[t] = V->fm[t];
+            if (D->dg.d_pen_max) G(D->dg.d_pen_max)[t] = G(V->nearf)[t] ? G(V->pen)[t] : 0.0f;
+            if (D->dg.d_force_mag) G(D->dg.d_force_mag)[t] = G(V->fm)[t];
         }
     }
     clear_cells(V, set, t);
-    if (t < V->nf) { V->floe_coll[t] = 0u; V->floe_maxfm[t] = 0u; }
+    if (t < V->nf) { G(V->floe_coll)[t] = 0u; G(V->floe_maxfm)[t] = 0u; }
     if (t == 0) *force_ticket(V) = 0;
     if (cull && chunk < V->nblk)
-        cull_chunk(V, chunk, t < n, t, px, py, pvx, pvy, D->s.d_ship[0], D->s.d_ship[1], set ^ 1);
+        cull_chunk(V, chunk, t < n, t, px, py, pvx, pvy, G(D->s.d_ship)[0], G(D->s.d_ship)[1], set ^ 1);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_post(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
@@ -1193,12 +1194,12 @@ __device__ __forceinline__ void mid_body(const DevView* __restrict__ V, const Ca
     const int nb = V->nblk, nbpad = V->nbpad;
     for (int b = w; b < nb; b += nw) {
         float a = 0.0f, c = 0.0f, d = 0.0f;
-        if (V->nearcnt[b] > 0) {
-            a = block256_tree(V, V->soi_x, b, 1.0f, false);
-            c = block256_tree(V, V->soi_y, b, 1.0f, false);
-            d = block256_tree(V, V->mterm, b, 1.0f, false);
+        if (G(V->nearcnt)[b] > 0) {
+            a = block256_tree(V, G(V->soi_x), b, 1.0f, false);
+            c = block256_tree(V, G(V->soi_y), b, 1.0f, false);
+            d = block256_tree(V, G(V->mterm), b, 1.0f, false);
         }
-        if (lane == 0) { V->part1[b] = a; V->part1[nbpad + b] = c; V->part1[2 * nbpad + b] = d; }
+        if (lane == 0) { G(V->part1)[b] = a; G(V->part1)[nbpad + b] = c; G(V->part1)[2 * nbpad + b] = d; }
     }
     __syncthreads();
     float tx = lds_tree_sum(V->part1, nb, nbpad, la, lb);
@@ -1213,17 +1214,17 @@ __device__ __forceinline__ void mid_body(const DevView* __restrict__ V, const Ca
     if (P.has_thrust_cap) {
         for (int b = w; b < nb; b += nw) {
             float a = 0.0f, c = 0.0f;
-            if (V->nearcnt[b] > 0) {
-                a = block256_tree(V, V->soi_x, b, scale, true);
-                c = block256_tree(V, V->soi_y, b, scale, true);
+            if (G(V->nearcnt)[b] > 0) {
+                a = block256_tree(V, G(V->soi_x), b, scale, true);
+                c = block256_tree(V, G(V->soi_y), b, scale, true);
             }
-            if (lane == 0) { V->part2[b] = a; V->part2[nbpad + b] = c; }
+            if (lane == 0) { G(V->part2)[b] = a; G(V->part2)[nbpad + b] = c; }
         }
         __syncthreads();
         fx = lds_tree_sum(V->part2, nb, nbpad, la, lb);
         fy = lds_tree_sum(V->part2 + nbpad, nb, nbpad, la, lb);
     }
-    if (threadIdx.x == 0) { V->scal[0] = scale; V->scal[1] = tm; }
+    if (threadIdx.x == 0) { G(V->scal)[0] = scale; G(V->scal)[1] = tm; }
     floe_eligibility(V, D, scale);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1248,8 +1249,8 @@ __global__ __launch_bounds__(BLOCK) void k_scene_rollouts(const DevView* __restr
         const int e = c * BLOCK + threadIdx.x;
         const bool valid = e < n;
         float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f;
-        if (valid) { x = D->s.d_pos[e]; y = D->s.d_pos[n + e]; vx = D->s.d_vel[e]; vy = D->s.d_vel[n + e]; }
-        cull_chunk(V, c, valid, e, x, y, vx, vy, D->s.d_ship[0], D->s.d_ship[1], 0);
+        if (valid) { x = G(D->s.d_pos)[e]; y = G(D->s.d_pos)[n + e]; vx = G(D->s.d_vel)[e]; vy = G(D->s.d_vel)[n + e]; }
+        cull_chunk(V, c, valid, e, x, y, vx, vy, G(D->s.d_ship)[0], G(D->s.d_ship)[1], 0);
     }
     __syncthreads();
     for (int k = 0; k < frames; ++k) {
@@ -1259,7 +1260,7 @@ __global__ __launch_bounds__(BLOCK) void k_scene_rollouts(const DevView* __restr
         if (count > 0) {
             const float* ship = D->s.d_ship;
             const ShipPose SP = make_pose(ship[0], ship[1], ship[2], V->p.half_len, V->p.half_beam);
-            for (int q = w; q < count; q += BLOCK / 64) near_elem(V, D, SP, V->near_list[q]);
+            for (int q = w; q < count; q += BLOCK / 64) near_elem(V, D, SP, G(V->near_list)[q]);
         }
         (void)lane;
         __syncthreads();

@@ -196,20 +196,20 @@ __device__ int pairs_of(const DevView* __restrict__ V, const float* __restrict__
     for (;;) {
         int best = 0x7fffffff;
         auto consider = [&](int j) {
-            if (j > last && j < best && pair_overlap(P, pos, V->radius, n, e, j)) best = j;
+            if (j > last && j < best && pair_overlap(P, G(pos), G(V->radius), n, e, j)) best = j;
         };
         for (int gy = y0; gy <= y1; ++gy)
             for (int gx = x0; gx <= x1; ++gx) {
                 const int cell = gy * P.grid_nx + gx;
-                const uint8_t* rec = rec_ptr(V, fine_rec(V, 0, cell));
-                const int cc = min(*reinterpret_cast<const int*>(rec), cap);
+                GAS const uint8_t* rec = rec_ptr(V, fine_rec(V, 0, cell));
+                const int cc = min(*(GAS const int*)rec, cap);
                 for (int s = 0; s < cc; ++s) consider(slot_load(rec, s).e);
             }
         for (int gy = v0; gy <= v1; ++gy)
             for (int gx = u0; gx <= u1; ++gx) {
                 const int cell = gy * V->nx2 + gx;
-                const uint8_t* rec = rec_ptr(V, coarse_rec(V, 0, cell));
-                const int cc = min(*reinterpret_cast<const int*>(rec), V->cap2);
+                GAS const uint8_t* rec = rec_ptr(V, coarse_rec(V, 0, cell));
+                const int cc = min(*(GAS const int*)rec, V->cap2);
                 for (int s = 0; s < cc; ++s) consider(slot_load(rec, s).e);
             }
         for (int s = 0; s < novf; ++s) consider(V->ovf_list[s]);

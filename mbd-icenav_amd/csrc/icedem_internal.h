@@ -7,6 +7,25 @@
 
 namespace icedem {
 
+// Pointers held in DevView / CallDesc are generic; reading them through the
+// global address space lets the compiler emit global (not flat) accesses,
+// whose completion is tracked apart from LDS and scalar loads.
+#define GAS __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ GAS T* G(T* p) { return (GAS T*)p; }
+
+// loads of the HIP vector structs through a global pointer
+typedef float gas_f2 __attribute__((ext_vector_type(2)));
+typedef int gas_i4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float2 ldg(GAS const float2* p) {
+    const gas_f2 v = *(GAS const gas_f2*)p;
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ int4 ldg(GAS const int4* p) {
+    const gas_i4 v = *(GAS const gas_i4*)p;
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+
 constexpr int BLOCK = 256;          // threads per block of the per-element kernels
 constexpr int MAX_PARTIALS = 4096;  // block partials reduced in one workgroup (n <= 1M)
 

@@ -142,7 +142,7 @@ __device__ __forceinline__ void point_to_ship(const ShipPose& P, float px, float
 // Lane v holds world vertex v in (wx, wy) and its mask bit in mv.
 // Returns pass-1/pass-2 combined (pen, nx, ny, cx, cy) in every lane.
 __device__ __forceinline__ void hull_element(const ShipPose& P, const IceDemParams& prm,
-                                             const float2* __restrict__ hull, int S, int maxv,
+                                             GAS const float2* __restrict__ hull, int S, int maxv,
                                              float wx, float wy, bool mv,
                                              float& pen_o, float& nx_o, float& ny_o,
                                              float& cx_o, float& cy_o) {
@@ -206,7 +206,7 @@ __device__ __forceinline__ void hull_element(const ShipPose& P, const IceDemPara
         const int s = base + lane / GP, sub = lane % GP;
         const bool live = (lane / GP) < per_round && s < S;
         const int sc = live ? s : 0;
-        float2 hl = hull[sc];
+        float2 hl = ldg(hull + sc);
         float hx = (hl.x * P.c_h + hl.y * (-P.s_h)) + P.sx;
         float hy = (hl.x * P.s_h + hl.y * P.c_h) + P.sy;
         int cross = 0;
@@ -329,7 +329,7 @@ __device__ __forceinline__ void hull_wave(const IceDemParams& prm, const float2*
             mv = mask[(size_t)qe * maxv + lane] != 0;
         }
         float a, b, c, d, f;
-        hull_element(P, prm, hull, S, maxv, wx, wy, mv, a, b, c, d, f);
+        hull_element(P, prm, G(hull), S, maxv, wx, wy, mv, a, b, c, d, f);
         if (lane == q) { pen = a; nxo = b; nyo = c; cxo = d; cyo = f; }
     }
 }
