@@ -200,16 +200,24 @@ int icedem_frames_profiled(IceDemCtx* ctx, const IceDemState* state, const IceDe
                            const IceDemHistory* hist, float* h_kernel_ms, void* stream);
 
 /* Batched rollouts (no reference counterpart: the reference's planners never
- * roll out the DEM).  A batch groups n_inst existing contexts (one scene
- * each, any sizes up to 65536 elements, n_ice_steps == 1; several contexts
- * may hold the same scene); icedem_batch_rollouts advances every instance by
- * `frames` frames in ONE persistent launch, one workgroup per instance, with
- * the same device code as icedem_frames (results are identical to K =
- * `frames` calls of icedem_frames on each context).  states/inputs/diags are
- * arrays of n_inst; diags[i].d_rollout receives the rollout cost terms.
- * use_graph != 0 replays a cached hipGraph of the launch. */
+ * roll out the DEM).  A batch groups n_inst distinct existing contexts (one
+ * scene each, any sizes up to 65536 elements, n_ice_steps == 1; several
+ * contexts may hold the same scene); icedem_batch_rollouts advances every
+ * instance by `frames` frames with the same device code as icedem_frames
+ * (results are identical to K = `frames` calls of icedem_frames on each
+ * context).  states/inputs/diags are arrays of n_inst; diags[i].d_rollout
+ * receives the rollout cost terms.  use_graph != 0 replays a cached hipGraph
+ * of the whole launch sequence (one graph launch for all frames).
+ * icedem_batch_mode picks how the instances share the GPU:
+ *   ICEDEM_BATCH_FRAMES  every frame is three launches that cover all
+ *                        instances' workgroups (default);
+ *   ICEDEM_BATCH_SCENES  one persistent launch, one workgroup per instance
+ *                        running all frames. */
+#define ICEDEM_BATCH_FRAMES 0
+#define ICEDEM_BATCH_SCENES 1
 typedef struct IceDemBatch IceDemBatch;
 int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** out);
+int icedem_batch_mode(IceDemBatch* batch, int32_t mode);
 int icedem_batch_rollouts(IceDemBatch* batch, const IceDemState* states, const IceDemInputs* inputs, int32_t frames,
                           const IceDemDiag* diags, int32_t use_graph, void* stream);
 int icedem_batch_destroy(IceDemBatch* batch);

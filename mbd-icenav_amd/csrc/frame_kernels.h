@@ -380,16 +380,22 @@ __device__ __forceinline__ void store_pose(const DevView* __restrict__ V, const 
     G(V->scal)[8] = SP.s_neg; G(V->scal)[9] = SP.c_neg; G(V->scal)[10] = SP.s_h; G(V->scal)[11] = SP.c_h;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_cull(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
+// workgroup lb of a call's first near test (grid: nblk workgroups)
+__device__ __forceinline__ void cull_kernel_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                                 int lb) {
     const int n = V->n;
-    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    const int e = lb * BLOCK + threadIdx.x;
     const bool valid = e < n;
     GAS const float* pos = G(D->s.d_pos);
     GAS const float* vel = G(D->s.d_vel);
     float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f;
     if (valid) { x = pos[e]; y = pos[n + e]; vx = vel[e]; vy = vel[n + e]; }
-    cull_chunk(V, blockIdx.x, valid, e, x, y, vx, vy, G(D->s.d_ship)[0], G(D->s.d_ship)[1], 0);
-    if (blockIdx.x == 0 && threadIdx.x == 0) store_pose(V, D->s.d_ship);
+    cull_chunk(V, lb, valid, e, x, y, vx, vy, G(D->s.d_ship)[0], G(D->s.d_ship)[1], 0);
+    if (lb == 0 && threadIdx.x == 0) store_pose(V, D->s.d_ship);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_cull(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
+    cull_kernel_body(V, D, blockIdx.x);
 }
 
 // cell insertion only (sub-steps > 0 and the leaf entry points)
@@ -488,10 +494,12 @@ __device__ __forceinline__ void near_block_done(const DevView* __restrict__ V, i
     if (lane == 0) { G(V->scal)[0] = scale; G(V->scal)[1] = tm; G(V->scal)[4] = tx; G(V->scal)[5] = ty; }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_near(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
+// workgroup lb of lg working on one context's near queue
+__device__ __forceinline__ void near_kernel_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                                 int lb, int lg) {
     const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
-    const int nwaves = gridDim.x * (BLOCK / 64);
+    const int wave = lb * (BLOCK / 64) + (threadIdx.x >> 6);
+    const int nwaves = lg * (BLOCK / 64);
     const int count = *near_count(V);
     const int e0 = (wave < V->n) ? G(V->near_list)[wave] : 0;   // read beside the count
     if (count == 0) {
@@ -515,6 +523,10 @@ __global__ __launch_bounds__(BLOCK) void k_near(const DevView* __restrict__ V, c
         }
         if (__shfl(fin, 0, 64)) near_block_done(V, e / BLOCK);
     }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_near(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
+    near_kernel_body(V, D, blockIdx.x, gridDim.x);
 }
 
 // ── ship reaction, floe eligibility, ship step ───────────────────────────────
@@ -1067,6 +1079,22 @@ __device__ __forceinline__ int chunk_of(int w, int total) {
     return (x < r) ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
 }
 
+// workgroup lb of lg = nextra + element chunks of one context's force pass;
+// xcd: lb is the hardware workgroup id (chunks spread XCD by XCD)
+__device__ __forceinline__ void force_kernel_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                                  int k, int sub, int src_is_state, int mode, int set, int last,
+                                                  int nextra, int lb, int lg, bool xcd,
+                                                  const float* __restrict__ fship_in, float* __restrict__ fout,
+                                                  float* __restrict__ mout, ForceLds& L, int* am_last) {
+    if (lb < nextra) {
+        force_tail(V, D, k, last, lb, nextra, am_last);
+        return;
+    }
+    const int nf = lg - nextra;
+    force_body(V, D, k, sub, src_is_state, mode, set, fship_in, fout, mout, L,
+               xcd ? chunk_of(lb - nextra, nf) : lb - nextra);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_force(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                                  int k, int sub, int src_is_state, int mode, int set, int last,
                                                  int nextra,
@@ -1074,13 +1102,8 @@ __global__ __launch_bounds__(BLOCK) void k_force(const DevView* __restrict__ V, 
                                                  float* __restrict__ fout, float* __restrict__ mout) {
     __shared__ ForceLds L;
     __shared__ int am_last;
-    if ((int)blockIdx.x < nextra) {
-        force_tail(V, D, k, last, (int)blockIdx.x, nextra, &am_last);
-        return;
-    }
-    const int nf = (int)gridDim.x - nextra;
-    force_body(V, D, k, sub, src_is_state, mode, set, fship_in, fout, mout, L,
-               chunk_of((int)blockIdx.x - nextra, nf));
+    force_kernel_body(V, D, k, sub, src_is_state, mode, set, last, nextra, (int)blockIdx.x, (int)gridDim.x, true,
+                      fship_in, fout, mout, L, &am_last);
 }
 
 // ── k_post: fracture (jax_simulation.py:534-656), commit, histories, next cull ─
@@ -1180,6 +1203,38 @@ __device__ __forceinline__ void post_body(const DevView* __restrict__ V, const C
 __global__ __launch_bounds__(BLOCK) void k_post(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                                 int k, int final_in_nxt, int last, int set, int cull) {
     post_body(V, D, k, final_in_nxt, last, blockIdx.x * BLOCK + threadIdx.x, blockIdx.x, set, cull);
+}
+
+// ── batched frames: the same kernels over many contexts in one launch ───────
+// map[blockIdx.x] = (instance, workgroup of that instance, its workgroup
+// count, instance's nextra); the instances' contexts and call descriptors
+// are views[i], descs[i].  Every instance runs the same frame k.
+
+__global__ __launch_bounds__(BLOCK) void kb_cull(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
+                                                 const int4* __restrict__ map) {
+    const int4 m = map[blockIdx.x];
+    cull_kernel_body(views + m.x, descs + m.x, m.y);
+}
+
+__global__ __launch_bounds__(BLOCK) void kb_near(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
+                                                 const int4* __restrict__ map) {
+    const int4 m = map[blockIdx.x];
+    near_kernel_body(views + m.x, descs + m.x, m.y, m.z);
+}
+
+__global__ __launch_bounds__(BLOCK) void kb_force(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
+                                                  const int4* __restrict__ map, int k, int last, int set) {
+    __shared__ ForceLds L;
+    __shared__ int am_last;
+    const int4 m = map[blockIdx.x];
+    force_kernel_body(views + m.x, descs + m.x, k, 0, 1, 0, set, last, m.w, m.y, m.z, false, nullptr, nullptr,
+                      nullptr, L, &am_last);
+}
+
+__global__ __launch_bounds__(BLOCK) void kb_post(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
+                                                 const int4* __restrict__ map, int k, int last, int set) {
+    const int4 m = map[blockIdx.x];
+    post_body(views + m.x, descs + m.x, k, 1, last, m.y * BLOCK + threadIdx.x, m.y, set, last ? 0 : 1);
 }
 
 

@@ -660,43 +660,83 @@ int icedem_frames_profiled(IceDemCtx* ctx, const IceDemState* state, const IceDe
 struct IceDemBatch {
     int device = 0;
     int n = 0;
-    int max_frames_graph = 0;
+    int mode = ICEDEM_BATCH_FRAMES;
     DevView* dviews = nullptr;   // (n) copies of the contexts' device views
     CallDesc* ddesc = nullptr;   // (n)
     CallDesc* hdesc = nullptr;   // pinned staging
     hipEvent_t desc_done = nullptr;
-    std::map<int, hipGraphExec_t> graphs;
-    std::map<int, hipGraph_t> graph_defs;
+    // workgroup maps of the batched frame kernels: (instance, workgroup of
+    // the instance, the instance's workgroup count, the instance's nextra)
+    int4* map_cull = nullptr; int4* map_near = nullptr; int4* map_force = nullptr; int4* map_post = nullptr;
+    int n_cull = 0, n_near = 0, n_force = 0, n_post = 0;
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;   // (frames, mode)
+    std::map<std::pair<int, int>, hipGraph_t> graph_defs;
     std::string err;
 };
+
+static void batch_free(IceDemBatch* b) {
+    for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto& g : b->graph_defs) (void)hipGraphDestroy(g.second);
+    if (b->dviews) (void)hipFree(b->dviews);
+    if (b->ddesc) (void)hipFree(b->ddesc);
+    if (b->hdesc) (void)hipHostFree(b->hdesc);
+    if (b->desc_done) (void)hipEventDestroy(b->desc_done);
+    for (int4* m : {b->map_cull, b->map_near, b->map_force, b->map_post})
+        if (m) (void)hipFree(m);
+    delete b;
+}
 
 int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** out) {
     if (!ctxs || n_inst < 1 || !out) { set_err(nullptr, "bad argument"); return ICEDEM_E_INVALID; }
     *out = nullptr;
     for (int i = 0; i < n_inst; ++i) {
+        if (!ctxs[i]) { set_err(nullptr, "null context"); return ICEDEM_E_INVALID; }
         const DevView& H = ctxs[i]->hv;
         if (H.nbpad > 256 || H.p.n_ice_steps != 1 || ctxs[i]->device != ctxs[0]->device) {
             set_err(nullptr, "batched scenes need n <= 65536, one ice sub-step and one device");
             return ICEDEM_E_LIMIT;
         }
     }
+    {   // each instance works in its own context's scratch
+        std::vector<const IceDemCtx*> v(ctxs, ctxs + n_inst);
+        std::sort(v.begin(), v.end());
+        if (std::adjacent_find(v.begin(), v.end()) != v.end()) {
+            set_err(nullptr, "a context appears twice in the batch"); return ICEDEM_E_INVALID;
+        }
+    }
     IceDemBatch* b = new IceDemBatch();
     b->device = ctxs[0]->device;
     b->n = n_inst;
+    std::vector<int4> mc, mn, mf, mp;
+    for (int i = 0; i < n_inst; ++i) {
+        const DevView& H = ctxs[i]->hv;
+        const int fblk = (H.n + EPB - 1) / EPB;
+        const int nextra = H.p.has_thrust_cap ? H.nblk : 1;
+        const int nnear = std::max(1, std::min(16, (H.n + 255) / 256));
+        const int post = std::max(std::max(H.nb, H.n), std::max(std::max(H.G, H.G2), H.nf));
+        const int npost = (post + BLOCK - 1) / BLOCK;
+        for (int l = 0; l < H.nblk; ++l) mc.push_back(make_int4(i, l, H.nblk, 0));
+        for (int l = 0; l < nnear; ++l) mn.push_back(make_int4(i, l, nnear, 0));
+        for (int l = 0; l < nextra + fblk; ++l) mf.push_back(make_int4(i, l, nextra + fblk, nextra));
+        for (int l = 0; l < npost; ++l) mp.push_back(make_int4(i, l, npost, 0));
+    }
+    b->n_cull = (int)mc.size(); b->n_near = (int)mn.size(); b->n_force = (int)mf.size(); b->n_post = (int)mp.size();
     hipError_t e = hipSetDevice(b->device);
     std::vector<DevView> hv(n_inst);
     for (int i = 0; i < n_inst; ++i) hv[i] = ctxs[i]->hv;
+    auto upmap = [&](int4** d, const std::vector<int4>& h) {
+        if (e == hipSuccess) e = hipMalloc((void**)d, sizeof(int4) * h.size());
+        if (e == hipSuccess) e = hipMemcpy(*d, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice);
+    };
     if (e == hipSuccess) e = hipMalloc((void**)&b->dviews, sizeof(DevView) * n_inst);
     if (e == hipSuccess) e = hipMalloc((void**)&b->ddesc, sizeof(CallDesc) * n_inst);
     if (e == hipSuccess) e = hipMemcpy(b->dviews, hv.data(), sizeof(DevView) * n_inst, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipHostMalloc((void**)&b->hdesc, sizeof(CallDesc) * n_inst, hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->desc_done, hipEventDisableTiming);
+    upmap(&b->map_cull, mc); upmap(&b->map_near, mn); upmap(&b->map_force, mf); upmap(&b->map_post, mp);
     if (e != hipSuccess) {
         set_err(nullptr, std::string("icedem_batch_create: ") + hipGetErrorString(e));
-        if (b->dviews) (void)hipFree(b->dviews);
-        if (b->ddesc) (void)hipFree(b->ddesc);
-        if (b->hdesc) (void)hipHostFree(b->hdesc);
-        delete b;
+        batch_free(b);
         return ICEDEM_E_HIP;
     }
     memset(b->hdesc, 0, sizeof(CallDesc) * n_inst);
@@ -707,14 +747,36 @@ int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** ou
 int icedem_batch_destroy(IceDemBatch* b) {
     if (!b) return ICEDEM_OK;
     (void)hipSetDevice(b->device);
-    for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.second);
-    for (auto& g : b->graph_defs) (void)hipGraphDestroy(g.second);
-    if (b->dviews) (void)hipFree(b->dviews);
-    if (b->ddesc) (void)hipFree(b->ddesc);
-    if (b->hdesc) (void)hipHostFree(b->hdesc);
-    if (b->desc_done) (void)hipEventDestroy(b->desc_done);
-    delete b;
+    batch_free(b);
     return ICEDEM_OK;
+}
+
+int icedem_batch_mode(IceDemBatch* b, int32_t mode) {
+    if (!b || (mode != ICEDEM_BATCH_FRAMES && mode != ICEDEM_BATCH_SCENES)) {
+        set_err(nullptr, "bad batch mode"); return ICEDEM_E_INVALID;
+    }
+    b->mode = mode;
+    return ICEDEM_OK;
+}
+
+// the launch sequence of `frames` frames over every instance
+static void batch_launch(IceDemBatch* b, int frames, hipStream_t stream) {
+    if (b->mode == ICEDEM_BATCH_SCENES) {
+        hipLaunchKernelGGL(k_scene_rollouts, dim3(b->n), dim3(BLOCK), 0, stream, b->dviews, b->ddesc, frames);
+        return;
+    }
+    hipLaunchKernelGGL(kb_cull, dim3(b->n_cull), dim3(BLOCK), 0, stream, b->dviews, b->ddesc,
+                       (const int4*)b->map_cull);
+    for (int k = 0; k < frames; ++k) {
+        const int last = (k == frames - 1) ? 1 : 0;
+        const int set = k & 1;
+        hipLaunchKernelGGL(kb_near, dim3(b->n_near), dim3(BLOCK), 0, stream, b->dviews, b->ddesc,
+                           (const int4*)b->map_near);
+        hipLaunchKernelGGL(kb_force, dim3(b->n_force), dim3(BLOCK), 0, stream, b->dviews, b->ddesc,
+                           (const int4*)b->map_force, k, last, set);
+        hipLaunchKernelGGL(kb_post, dim3(b->n_post), dim3(BLOCK), 0, stream, b->dviews, b->ddesc,
+                           (const int4*)b->map_post, k, last, set);
+    }
 }
 
 int icedem_batch_rollouts(IceDemBatch* b, const IceDemState* states, const IceDemInputs* inputs, int32_t frames,
@@ -739,25 +801,26 @@ int icedem_batch_rollouts(IceDemBatch* b, const IceDemState* states, const IceDe
     HIP_TRY(hipMemcpyAsync(b->ddesc, b->hdesc, sizeof(CallDesc) * b->n, hipMemcpyHostToDevice, stream));
     HIP_TRY(hipEventRecord(b->desc_done, stream));
     if (!use_graph) {
-        hipLaunchKernelGGL(k_scene_rollouts, dim3(b->n), dim3(BLOCK), 0, stream, b->dviews, b->ddesc, frames);
+        batch_launch(b, frames, stream);
         HIP_TRY(hipGetLastError());
         return ICEDEM_OK;
     }
-    auto it = b->graphs.find(frames);
+    const std::pair<int, int> key(frames, b->mode);
+    auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
         hipStream_t cap;
         HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
         HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-        hipLaunchKernelGGL(k_scene_rollouts, dim3(b->n), dim3(BLOCK), 0, cap, b->dviews, b->ddesc, frames);
+        batch_launch(b, frames, cap);
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(cap, &g);
         (void)hipStreamDestroy(cap);
         if (e2 != hipSuccess) { set_err(nullptr, std::string("capture: ") + hipGetErrorString(e2)); return ICEDEM_E_HIP; }
         hipGraphExec_t ge = nullptr;
         HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-        b->graph_defs[frames] = g;
-        b->graphs[frames] = ge;
-        it = b->graphs.find(frames);
+        b->graph_defs[key] = g;
+        b->graphs[key] = ge;
+        it = b->graphs.find(key);
     }
     HIP_TRY(hipGraphLaunch(it->second, stream));
     return ICEDEM_OK;

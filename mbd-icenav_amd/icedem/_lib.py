@@ -86,6 +86,9 @@ class IceDemDiag(C.Structure):
 
 
 # name -> (restype, argtypes)
+BATCH_FRAMES = 0   # ICEDEM_BATCH_FRAMES
+BATCH_SCENES = 1   # ICEDEM_BATCH_SCENES
+
 SIGNATURES = {
     "icedem_status_string": (C.c_char_p, [C.c_int]),
     "icedem_last_error": (C.c_char_p, [_p]),
@@ -100,6 +103,7 @@ SIGNATURES = {
     "icedem_batch_rollouts": (C.c_int, [_p, C.POINTER(IceDemState), C.POINTER(IceDemInputs), _i,
                                         C.POINTER(IceDemDiag), _i, _p]),
     "icedem_batch_destroy": (C.c_int, [_p]),
+    "icedem_batch_mode": (C.c_int, [_p, _i]),
     "icedem_sync": (C.c_int, [_p, _p]),
     "icedem_substep_forces": (C.c_int, [_p, C.POINTER(IceDemState), _p, _p, C.c_int64, C.c_int64, _p, _p, _p]),
     "icedem_contact_pairs": (C.c_int, [_p, _p, _p, _p, C.c_int64, C.POINTER(C.c_int64), _p]),

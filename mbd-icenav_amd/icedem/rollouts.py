@@ -5,7 +5,10 @@ The reference's MPPI/MBD planners (``jax_mppi_planner.py:34-115``,
 model against a costmap; they never roll out the DEM.  This module scores
 candidates with the DEM itself: every candidate is an instance (a scene
 context + its own state) and all instances advance ``frames`` frames in one
-persistent launch (``icedem_batch_rollouts``), one workgroup per instance.
+graph launch (``icedem_batch_rollouts``): either every frame as three
+kernels spanning all instances (mode "frames") or one workgroup per instance
+running all frames (mode "scenes").  ``autotune`` picks the mode and the grid
+cell size by timing.
 
 Ship control for rollouts is direct (ship_mode 1): a control sequence of
 ``n_knots`` knots (u, r_cmd) in [-1, 1], each held ``hold`` frames, gives the
@@ -106,6 +109,7 @@ class RolloutBatch:
         _lib.check(lib.icedem_batch_create(ctxs, self.R, C.byref(h)), None, "icedem_batch_create")
         self.handle = h
         self.lib = lib
+        self.mode = "frames"
         self.out = torch.zeros((self.R, 4), dtype=torch.float32, device=self.device)
 
     def _alloc_states(self):
@@ -155,6 +159,11 @@ class RolloutBatch:
         self.f.copy_(self.f_init)
         self.u.copy_(self.u_init)
         self.ships.copy_(self.ship_init)
+
+    def set_mode(self, mode: str):
+        m = {"frames": _lib.BATCH_FRAMES, "scenes": _lib.BATCH_SCENES}[mode]
+        _lib.check(self.lib.icedem_batch_mode(self.handle, m), None, "icedem_batch_mode")
+        self.mode = mode
 
     def run(self, controls: torch.Tensor, hold: int, frames: int, use_graph: bool = True, sync: bool = True):
         """controls (R, n_knots, 2) -> (R, 4) [impulse, KE loss, peak |F|, KE0]."""

@@ -1,4 +1,4 @@
-"""Batched rollouts: one persistent launch == per-rollout frame calls == oracle.
+"""Batched rollouts (both batch modes) == per-rollout frame calls == oracle.
 
 The rollout ship control (direct thrust) and the cost terms are this build's
 extension; the oracle restates them (oracle.step.rollout) with the same
@@ -35,9 +35,11 @@ def _oracle_for(rb, scene):
                             ctrl_yaw_rate_max=rb.ship.yaw_rate_max)
 
 
-def test_batched_equals_sequential_and_oracle():
+@pytest.mark.parametrize("mode", ["frames", "scenes"])
+def test_batched_equals_sequential_and_oracle(mode):
     from icedem.rollouts import nominal_controls
     rb, scenes = _batch([(1000, 60, 0.3), (1001, 150, 0.4), (1002, 90, 0.5)], [0, 1, 2, 0])
+    rb.set_mode(mode)
     frames, hold, nk = 120, 5, 24
     ctl = nominal_controls(rb.R, nk, seed=3)
     rb.reset()
