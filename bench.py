@@ -121,6 +121,105 @@ def cpu_baseline(case_args, frames_budget_s: float = 20.0, max_frames: int = 8):
                       f"host os.cpu_count()={os.cpu_count()}"}
 
 
+def _cpu_rollout(args):
+    """One oracle rollout (float32 NumPy, all-pairs contact) on a host core."""
+    r, frames, hold, ctl = args
+    import numpy as np
+    sys.path.insert(0, str(ROOT / "mbd-icenav_amd"))
+    sys.path.insert(0, str(ROOT))
+    from icedem import scenario
+    from icedem.config import benchmark_config
+    from icedem.rollouts import ShipSpec, patch_params, ship_start
+    from icedem.properties import ShipProperties
+    from icedem.ice_state import ship_hull_polygon
+    from oracle import model as om, step as ostep
+    from tests.helpers import oracle_config
+    sc = scenario.scene_patch(*patch_params(r))
+    cfg = benchmark_config()
+    ship = ShipSpec()
+    mass = ShipProperties(ship.length, ship.beam, ship.draft).mass
+    hull = ship_hull_polygon(ship.length, ship.beam, ship.hull_points // 2 - 1)
+    st64 = om.build_static_data(sc.polygons, sc.centroids, sc.bonds, sc.thicknesses, sc.parent_ids, oracle_config(cfg))
+    sd = om.build_ship_dynamics_data(mass, ship.length, ship.beam, hull)
+    fos = om.Fossen(mass, ship.length)
+    o = ostep.Oracle(st64, oracle_config(cfg), sd, ship.length, ship.beam, 1, cfg.dt, cfg.dt, False, fos.B, fos.T_n,
+                     dtype=np.float32, ship_mode=1, ctrl_thrust_max=ship.thrust_max,
+                     ctrl_yaw_rate_max=ship.yaw_rate_max)
+    st = ostep.initial_ice_state(sc.centroids, len(sc.bonds), sc.parent_ids, np.float32)
+    x0, y0 = ship_start(sc, ship)
+    s0 = np.zeros(15, np.float32)
+    s0[0], s0[1], s0[2], s0[3] = x0, y0, math.pi / 2, ship.speed0
+    sh = ostep.ShipState(s0[0:3].copy(), s0[3:6].copy(), s0[6:12].copy(), s0[12:15].copy())
+    t0 = time.perf_counter()
+    _, _, out = ostep.rollout(o, st, sh, ctl, hold, frames)
+    return r, time.perf_counter() - t0, len(sc.polygons), out.tolist()
+
+
+def rollout_bench(R: int = 256, frames: int = 400, hold: int = 10, n_knots: int = 40, cpu_workers: int = 16):
+    """Batched rollouts: R patches (seed 1000 + r, 50-800 elements, 20-60 %
+    concentration), controls = nominal (u 0.7, r 0) + N(0, 0.2^2) clipped,
+    `frames` frames each; the batch as one graph launch vs the plain
+    per-rollout GPU path vs oracle rollouts on the host cores."""
+    import numpy as np
+    import torch
+    from icedem import rollouts as ro
+    workers = max(1, min(cpu_workers, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    scenes = ro.rollout_patches(R, workers=workers)
+    t_gen = time.perf_counter() - t0
+    n_el = [len(sc.polygons) for sc in scenes]
+    mode, cf, tune = ro.autotune(scenes, sample=32, frames=40)
+    rb = ro.RolloutBatch(scenes, list(range(R)), cell_size_factor=cf)
+    rb.set_mode(mode)
+    ctl_np = ro.nominal_controls(R, n_knots, seed=0)
+    ctl = torch.as_tensor(ctl_np)
+    rb.reset()
+    rb.run(ctl, hold, frames)                  # graph capture and warm-up
+    best = float("inf")
+    for _ in range(3):
+        rb.reset()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        out_b = rb.run(ctl, hold, frames)
+        best = min(best, time.perf_counter() - t1)
+    out_b = out_b.cpu().numpy().copy()
+    rb.reset()
+    rb.run_sequential(ctl, hold, frames)       # per-context graph capture
+    rb.reset()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    out_s = rb.run_sequential(ctl, hold, frames)
+    t_plain = time.perf_counter() - t1
+    out_s = out_s.cpu().numpy().copy()
+    rb.close()
+    # oracle rollouts of the first `workers` patches, one per host core
+    import multiprocessing as mp
+    sample = list(range(min(workers, R)))
+    t1 = time.perf_counter()
+    with mp.get_context("spawn").Pool(len(sample)) as pool:
+        cpu = pool.map(_cpu_rollout, [(r, frames, hold, ctl_np[r]) for r in sample])
+    t_cpu = time.perf_counter() - t1
+    cpu_equal = all(np.array_equal(np.asarray(c[3], np.float32), out_b[c[0]]) for c in cpu)
+    cost = ro.rollout_cost(out_b)
+    return {
+        "workload": f"{R} rollouts x {frames} frames ({n_knots} knots x hold {hold}); patches seed 1000+r, "
+                    f"{min(n_el)}-{max(n_el)} elements (sum {sum(n_el)}), concentration 0.2-0.6",
+        "batched_rollouts_per_s": R / best, "batched_s": best, "mode": mode, "cell_size_factor": cf,
+        "autotune_s": {f"{m}/{c}": t for (m, c), t in tune.items()},
+        "plain_gpu_rollouts_per_s": R / t_plain, "plain_gpu_s": t_plain,
+        "batched_vs_plain": t_plain / best,
+        "batched_equals_plain": bool(np.array_equal(out_b, out_s)),
+        "element_steps_per_s": sum(n_el) * frames / best,
+        "cost": {"definition": "impulse / 1e5 N s + KE loss / 1e6 J", "mean": float(cost.mean()),
+                 "min": float(cost.min()), "max": float(cost.max())},
+        "cpu_baseline": {"rollouts_per_s": len(sample) / t_cpu, "cores": len(sample), "kind": "port",
+                         "sample": f"rollouts 0..{len(sample) - 1} ({sum(c[2] for c in cpu)} elements), float32 "
+                                   f"NumPy oracle with all-pairs contact, one process per core, {t_cpu:.1f} s",
+                         "equals_gpu": bool(cpu_equal)},
+        "patch_generation_s": t_gen,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +231,8 @@ def main():
     ap.add_argument("--profile-frames", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-rollouts", action="store_true", help="skip the batched-rollout measurement")
+    ap.add_argument("--rollouts", type=int, default=256)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB five times (reference kernel for the PMC byte counters)")
     args = ap.parse_args()
@@ -231,6 +332,8 @@ def main():
             traffic = None
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(case_args)
+    eng.close()
+    rollouts = None if args.no_rollouts else rollout_bench(R=args.rollouts)
     value = n * args.steps / elapsed
     out = {
         "metric": METRIC,
@@ -254,6 +357,7 @@ def main():
                      "kernel_avg_us": avg_us[dom], "algorithmic_bytes": abytes,
                      "kernel_avg_us_all": avg_us},
         "cpu_baseline": cpu,
+        "rollouts": rollouts,
     }
     if cpu:
         out["speedup_vs_cpu"] = value / cpu["value"]

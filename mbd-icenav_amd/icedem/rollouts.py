@@ -213,6 +213,54 @@ class RolloutBatch:
             pass
 
 
+def patch_params(r: int, n_lo: int = 50, n_hi: int = 800, c_lo: float = 0.2, c_hi: float = 0.6):
+    """Patch of rollout r: seed 1000 + r, ~U{n_lo..n_hi} elements, concentration U[c_lo, c_hi]."""
+    rng = np.random.default_rng(1000 + r)
+    return 1000 + r, int(rng.integers(n_lo, n_hi + 1)), float(rng.uniform(c_lo, c_hi))
+
+
+def _patch(args):
+    from .scenario import scene_patch
+    return scene_patch(*args)
+
+
+def rollout_patches(R: int, workers: int = 1) -> List[Scene]:
+    """The R rollout patches (deterministic per seed), built in a process pool."""
+    params = [patch_params(r) for r in range(R)]
+    if workers <= 1:
+        return [_patch(a) for a in params]
+    import multiprocessing as mp
+    with mp.get_context("spawn").Pool(workers) as pool:
+        return pool.map(_patch, params, chunksize=max(1, R // (4 * workers)))
+
+
+def autotune(scenes: Sequence[Scene], cfg=None, ship: Optional[ShipSpec] = None, sample: int = 32,
+             frames: int = 40, cell_factors=(1.0, 1.25, 1.6), modes=("frames", "scenes"), seed: int = 0):
+    """Pick the batch mode and grid cell size factor by timing `frames` frames
+    of a `sample`-instance batch for every combination; returns (mode,
+    cell_factor, {(mode, factor): seconds})."""
+    import time
+    sub = list(range(min(sample, len(scenes))))
+    ctl = torch.as_tensor(nominal_controls(len(sub), 4, seed=seed))
+    times = {}
+    for cf in cell_factors:
+        rb = RolloutBatch([scenes[i] for i in sub], list(range(len(sub))), cfg, ship, cell_size_factor=cf)
+        for m in modes:
+            rb.set_mode(m)
+            rb.reset(); rb.run(ctl, max(1, frames // 4), frames)          # capture + warm
+            best = float("inf")
+            for _ in range(3):
+                rb.reset()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                rb.run(ctl, max(1, frames // 4), frames)
+                best = min(best, time.perf_counter() - t0)
+            times[(m, cf)] = best
+        rb.close()
+    (m, cf) = min(times, key=times.get)
+    return m, cf, times
+
+
 def nominal_controls(R: int, n_knots: int = 40, u: float = 0.7, r: float = 0.0, sigma: float = 0.2,
                      seed: int = 0) -> np.ndarray:
     """MPPI-style candidates: nominal plus N(0, sigma^2) noise, clipped to [-1, 1]."""
