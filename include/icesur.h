@@ -1,0 +1,104 @@
+/*
+ * icesur.h -- C ABI of the rollout-cost surrogate kernels (gfx950, BF16 MFMA).
+ *
+ * No reference counterpart: the reference has no learned model of the DEM.
+ * The surrogate (a CNN over a 64x64x2 ice raster, concatenated with the ship
+ * state and the control sequence, then an MLP) is assembled by the Python
+ * host (mbd-icenav_amd/icedem/surrogate.py) from these operations; every
+ * floating-point operation of its forward and backward pass runs here.
+ *
+ * Conventions: bf16 tensors are uint16_t bit patterns (round to nearest
+ * even), matrices are row-major with a leading dimension in elements, device
+ * pointers d_*, `stream` a hipStream_t.  Every function returns 0 or an
+ * ICEDEM_E_* code (icedem.h); icedem_last_error(NULL) describes it.
+ * Reductions (split-K, column sums, col2im) run in a fixed order, so results
+ * are identical from run to run.
+ */
+#ifndef ICESUR_H
+#define ICESUR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ICESUR_RELU 1      /* epilogue: max(x, 0) */
+#define ICESUR_ACCUM 2     /* epilogue: C += result (fp32 C only) */
+#define ICESUR_BIAS 4      /* epilogue: + bias[col] */
+
+/* C (M x N) = op(A) (M x K) . op(B) (K x N) with bf16 operands and fp32
+ * accumulation on MFMA; op(A) = A (M x K, lda) or, with trans_a, A^T where
+ * A is K x M; op(B) = B (K x N, ldb) or, with trans_b, B^T where B is N x K.
+ * The epilogue adds bias, applies ReLU and accumulates per `flags`, then
+ * writes fp32 d_c (ldc) and/or bf16 d_cb (ldcb); either may be NULL.
+ * split_k > 1 partitions K over split_k workgroup sets whose fp32 partial
+ * products go to d_work (split_k * M * N floats) and are added in split
+ * order before the epilogue. */
+int icesur_gemm(int32_t trans_a, int32_t trans_b, int32_t M, int32_t N, int32_t K,
+                const uint16_t* d_a, int32_t lda, const uint16_t* d_b, int32_t ldb,
+                float* d_c, int32_t ldc, uint16_t* d_cb, int32_t ldcb, const float* d_bias, int32_t flags,
+                int32_t split_k, float* d_work, void* stream);
+
+/* 3x3, stride 2, padding 1 patches of an NHWC bf16 tensor (B, H, W, C):
+ * d_col row b*Ho*Wo + ho*Wo + wo holds (kh*3 + kw)*C + c, Ho = (H+1)/2,
+ * Wo = (W+1)/2, zeros outside the input. */
+int icesur_im2col(const uint16_t* d_x, int32_t B, int32_t H, int32_t W, int32_t C, uint16_t* d_col,
+                  void* stream);
+
+/* Adjoint of icesur_im2col: d_dx (B, H, W, C) fp32 = the sum over the
+ * patches that read each input position (kh, kw order). */
+int icesur_col2im(const float* d_dcol, int32_t B, int32_t H, int32_t W, int32_t C, float* d_dx, void* stream);
+
+/* ReLU backward: g[i] = d_y[i] > 0 ? g[i] : 0 in place (fp32; d_y is the
+ * layer's bf16 output), and d_gb = bf16(g) when not NULL.  Rows of `cols`
+ * values; g has row stride ldg, d_y ldy, d_gb ldgb. */
+int icesur_relu_back(float* d_g, int32_t ldg, const uint16_t* d_y, int32_t ldy, int32_t rows, int32_t cols,
+                     uint16_t* d_gb, int32_t ldgb, void* stream);
+
+/* d_out[c] (+)= sum over rows of d_x[r * ldx + c], rows in blocks of 256,
+ * block sums added in block order; d_work holds ceil(rows/256) * cols floats.
+ * accumulate != 0 adds to d_out. */
+int icesur_colsum(const float* d_x, int32_t ldx, int32_t rows, int32_t cols, float* d_out, int32_t accumulate,
+                  float* d_work, void* stream);
+
+/* bf16 copy of n floats (round to nearest even). */
+int icesur_to_bf16(const float* d_x, int64_t n, uint16_t* d_out, void* stream);
+
+/* Adam step t (>= 1) on n fp32 parameters d_w with gradients d_g and moments
+ * d_m, d_v; d_wb (optional) receives bf16(w). */
+int icesur_adam(float* d_w, const float* d_g, float* d_m, float* d_v, int64_t n, float lr, float beta1,
+                float beta2, float eps, int32_t t, uint16_t* d_wb, void* stream);
+
+/* Weighted mean squared error over B x D predictions (ld = D): d_loss[0] =
+ * sum_{b,o} w_o (y - t)^2 / (B D) (rows summed in order), d_dy = its gradient
+ * 2 w_o (y - t) / (B D) in fp32 and, if d_dyb, bf16. */
+int icesur_mse(const float* d_y, const float* d_t, const float* d_w, int32_t B, int32_t D, float* d_dy,
+               uint16_t* d_dyb, float* d_loss, void* stream);
+
+/* MLP input rows: d_z[b * ldz + j] = bf16 of feat[b, j] for j < F (the
+ * flattened CNN output, bf16, ldf), then the E extra features of sample b
+ * (fp32, lde), then zeros up to ldz. */
+int icesur_concat(const uint16_t* d_feat, int32_t ldf, int32_t F, const float* d_extra, int32_t lde, int32_t E,
+                  int32_t B, uint16_t* d_z, int32_t ldz, void* stream);
+
+/* Gradient of the CNN part from the MLP input gradient: d_dfeat[b * F + j] =
+ * d_dz[b * lddz + j] for j < F (fp32). */
+int icesur_split(const float* d_dz, int32_t lddz, int32_t F, int32_t B, float* d_dfeat, void* stream);
+
+/* Ice raster of a scene seen from a ship pose: a G x G grid of `cell` metres
+ * in the ship frame (x forward, y to port), covering x in [x_lo, x_lo + G
+ * cell) and y in [-G cell / 2, G cell / 2).  For each of the B poses
+ * (d_pose[b] = x, y, heading), element e of the scene (centroid d_pos (2,n)
+ * SoA, area d_area, thickness d_thick) is taken as the disc of its area
+ * around its centroid and adds the area of every cell whose centre the disc
+ * covers (fixed point, 2^-12 m^2); d_out (B, G, G, 2) bf16 gets the area
+ * fraction (capped at 1) and the area-weighted mean thickness.  d_acc
+ * (B G G 2 ints) must be zero on entry and is left zero. */
+int icesur_raster(const float* d_pos, const float* d_area, const float* d_thick, int32_t n, const float* d_pose,
+                  int32_t B, int32_t G, float cell, float x_lo, int32_t* d_acc, uint16_t* d_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

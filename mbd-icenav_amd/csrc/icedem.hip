@@ -279,6 +279,9 @@ static void set_err(IceDemCtx* ctx, const std::string& s) {
     g_err = s;
 }
 
+// error text for the surrogate entry points (surrogate.hip)
+void icedem_set_error(const std::string& msg) { set_err(nullptr, msg); }
+
 template <class T>
 static int dalloc(IceDemCtx* ctx, T** p, size_t count) {
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);

@@ -1,0 +1,481 @@
+// surrogate.hip -- kernels of the rollout-cost surrogate (gfx950).
+//
+// GEMMs run on the matrix cores: v_mfma_f32_16x16x32_bf16, bf16 operands,
+// fp32 accumulation.  A workgroup of 4 wavefronts computes a 64 x 64 tile of
+// C; each wave owns a 32 x 32 quarter as 2 x 2 MFMA tiles.  K advances 32 at
+// a time through LDS: A as (m, k) rows and B as (n, k) rows, so every MFMA
+// operand fragment (8 consecutive k of one row or column) is one 16-byte LDS
+// read.  Split-K writes fp32 partials that a second kernel adds in split
+// order.  No floating-point atomics anywhere.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include "../../include/icedem.h"
+#include "../../include/icesur.h"
+
+void icedem_set_error(const std::string& msg);   // icedem.hip
+
+namespace icesur {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float x) {
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x0040u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+constexpr int TM = 64, TN = 64, TK = 32, KP = TK + 8;   // LDS rows padded to 80 bytes
+
+// op(A)[m][k] for the tile loader
+template <bool TA>
+__device__ __forceinline__ void load_a(const uint16_t* __restrict__ A, int lda, int M, int K, int m0, int k0,
+                                       int kend, uint16_t (*As)[KP]) {
+    const int t = threadIdx.x;
+    if (!TA) {   // A is M x K: thread loads 8 consecutive k of one row
+        const int m = t >> 2, kq = (t & 3) * 8;
+        const int gm = m0 + m, gk = k0 + kq;
+        uint16_t v[8];
+        const uint16_t* p = A + (size_t)gm * lda + gk;
+        if (gm < M && gk + 7 < kend && ((uintptr_t)p & 15) == 0) {
+            const uint4 q = *reinterpret_cast<const uint4*>(p);
+            v[0] = q.x & 0xffff; v[1] = q.x >> 16; v[2] = q.y & 0xffff; v[3] = q.y >> 16;
+            v[4] = q.z & 0xffff; v[5] = q.z >> 16; v[6] = q.w & 0xffff; v[7] = q.w >> 16;
+        } else {
+            #pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = (gm < M && gk + j < kend) ? p[j] : (uint16_t)0;
+        }
+        #pragma unroll
+        for (int j = 0; j < 8; ++j) As[m][kq + j] = v[j];
+    } else {     // A is K x M: thread loads 8 consecutive m of one k
+        const int k = t >> 3, mq = (t & 7) * 8;
+        const int gk = k0 + k, gm = m0 + mq;
+        const uint16_t* p = A + (size_t)gk * lda + gm;
+        #pragma unroll
+        for (int j = 0; j < 8; ++j) As[mq + j][k] = (gk < kend && gm + j < M) ? p[j] : (uint16_t)0;
+    }
+}
+
+// op(B)[k][n] stored as Bs[n][k]
+template <bool TB>
+__device__ __forceinline__ void load_b(const uint16_t* __restrict__ B, int ldb, int N, int K, int n0, int k0,
+                                       int kend, uint16_t (*Bs)[KP]) {
+    const int t = threadIdx.x;
+    if (!TB) {   // B is K x N: thread loads 8 consecutive n of one k
+        const int k = t >> 3, nq = (t & 7) * 8;
+        const int gk = k0 + k, gn = n0 + nq;
+        const uint16_t* p = B + (size_t)gk * ldb + gn;
+        #pragma unroll
+        for (int j = 0; j < 8; ++j) Bs[nq + j][k] = (gk < kend && gn + j < N) ? p[j] : (uint16_t)0;
+    } else {     // B is N x K: thread loads 8 consecutive k of one row
+        const int n = t >> 2, kq = (t & 3) * 8;
+        const int gn = n0 + n, gk = k0 + kq;
+        uint16_t v[8];
+        const uint16_t* p = B + (size_t)gn * ldb + gk;
+        if (gn < N && gk + 7 < kend && ((uintptr_t)p & 15) == 0) {
+            const uint4 q = *reinterpret_cast<const uint4*>(p);
+            v[0] = q.x & 0xffff; v[1] = q.x >> 16; v[2] = q.y & 0xffff; v[3] = q.y >> 16;
+            v[4] = q.z & 0xffff; v[5] = q.z >> 16; v[6] = q.w & 0xffff; v[7] = q.w >> 16;
+        } else {
+            #pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = (gn < N && gk + j < kend) ? p[j] : (uint16_t)0;
+        }
+        #pragma unroll
+        for (int j = 0; j < 8; ++j) Bs[n][kq + j] = v[j];
+    }
+}
+
+struct Epi {
+    float* C; int ldc;
+    uint16_t* Cb; int ldcb;
+    const float* bias;
+    int flags;
+};
+
+__device__ __forceinline__ void epilogue_store(const Epi& E, int row, int col, float x) {
+    if (E.flags & ICESUR_BIAS) x = x + E.bias[col];
+    if (E.flags & ICESUR_ACCUM) x = E.C[(size_t)row * E.ldc + col] + x;
+    if (E.flags & ICESUR_RELU) x = fmaxf(x, 0.0f);
+    if (E.C) E.C[(size_t)row * E.ldc + col] = x;
+    if (E.Cb) E.Cb[(size_t)row * E.ldcb + col] = f2bf(x);
+}
+
+// grid (ceil(M/64), ceil(N/64), splits); split z covers k in [z ks, (z+1) ks)
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_t* __restrict__ A, int lda,
+                                              const uint16_t* __restrict__ B, int ldb, Epi E, int ks,
+                                              float* __restrict__ part) {
+    __shared__ __align__(16) uint16_t As[TM][KP];
+    __shared__ __align__(16) uint16_t Bs[TN][KP];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    const int kbeg = blockIdx.z * ks, kend = min(K, kbeg + ks);
+    f32x4 acc[2][2];
+    #pragma unroll
+    for (int i = 0; i < 2; ++i)
+        #pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
+        load_a<TA>(A, lda, M, K, m0, k0, kend, As);
+        load_b<TB>(B, ldb, N, K, n0, k0, kend, Bs);
+        __syncthreads();
+        bf16x8 a[2], b[2];
+        #pragma unroll
+        for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&As[wm + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+        #pragma unroll
+        for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn + j * 16 + (lane & 15)][8 * (lane >> 4)]);
+        #pragma unroll
+        for (int i = 0; i < 2; ++i)
+            #pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+    // D fragment: lane holds column lane % 16, rows 4 (lane / 16) + v
+    #pragma unroll
+    for (int i = 0; i < 2; ++i)
+        #pragma unroll
+        for (int j = 0; j < 2; ++j)
+            #pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = m0 + wm + i * 16 + 4 * (lane >> 4) + v;
+                const int col = n0 + wn + j * 16 + (lane & 15);
+                if (row < M && col < N) {
+                    if (part) part[((size_t)blockIdx.z * M + row) * N + col] = acc[i][j][v];
+                    else epilogue_store(E, row, col, acc[i][j][v]);
+                }
+            }
+}
+
+// sum of the split partials in split order, then the epilogue
+__global__ void k_splitk_reduce(int M, int N, int splits, const float* __restrict__ part, Epi E) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)M * N) return;
+    float x = part[i];
+    for (int z = 1; z < splits; ++z) x = x + part[(size_t)z * M * N + i];
+    epilogue_store(E, (int)(i / N), (int)(i % N), x);
+}
+
+__global__ void k_im2col(const uint16_t* __restrict__ X, int B, int H, int W, int C, int Ho, int Wo,
+                         uint16_t* __restrict__ col) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t total = (size_t)B * Ho * Wo * 9 * C;
+    if (idx >= total) return;
+    const int c = (int)(idx % C);
+    const size_t r1 = idx / C;
+    const int tap = (int)(r1 % 9);
+    const size_t row = r1 / 9;
+    const int wo = (int)(row % Wo), ho = (int)((row / Wo) % Ho), b = (int)(row / ((size_t)Wo * Ho));
+    const int h = 2 * ho + tap / 3 - 1, w = 2 * wo + tap % 3 - 1;
+    col[idx] = (h >= 0 && h < H && w >= 0 && w < W) ? X[(((size_t)b * H + h) * W + w) * C + c] : (uint16_t)0;
+}
+
+__global__ void k_col2im(const float* __restrict__ dcol, int B, int H, int W, int C, int Ho, int Wo,
+                         float* __restrict__ dx) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)B * H * W * C) return;
+    const int c = (int)(idx % C);
+    const size_t p = idx / C;
+    const int w = (int)(p % W), h = (int)((p / W) % H), b = (int)(p / ((size_t)W * H));
+    float s = 0.0f;
+    for (int kh = 0; kh < 3; ++kh) {
+        const int hh = h + 1 - kh;
+        if (hh < 0 || (hh & 1) || (hh >> 1) >= Ho) continue;
+        for (int kw = 0; kw < 3; ++kw) {
+            const int ww = w + 1 - kw;
+            if (ww < 0 || (ww & 1) || (ww >> 1) >= Wo) continue;
+            const size_t row = ((size_t)b * Ho + (hh >> 1)) * Wo + (ww >> 1);
+            s = s + dcol[row * 9 * C + (size_t)(kh * 3 + kw) * C + c];
+        }
+    }
+    dx[idx] = s;
+}
+
+__global__ void k_relu_back(float* __restrict__ g, int ldg, const uint16_t* __restrict__ y, int ldy, int rows,
+                            int cols, uint16_t* __restrict__ gb, int ldgb) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)rows * cols) return;
+    const int r = (int)(idx / cols), c = (int)(idx % cols);
+    float v = g[(size_t)r * ldg + c];
+    v = (bf2f(y[(size_t)r * ldy + c]) > 0.0f) ? v : 0.0f;
+    g[(size_t)r * ldg + c] = v;
+    if (gb) gb[(size_t)r * ldgb + c] = f2bf(v);
+}
+
+// block b of 256 rows: work[b * cols + c] = sum of its rows in order
+__global__ void k_colsum_part(const float* __restrict__ x, int ldx, int rows, int cols, float* __restrict__ work) {
+    const int c = blockIdx.y * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    const int r0 = blockIdx.x * 256, r1 = min(rows, r0 + 256);
+    float s = 0.0f;
+    for (int r = r0; r < r1; ++r) s = s + x[(size_t)r * ldx + c];
+    work[(size_t)blockIdx.x * cols + c] = s;
+}
+
+__global__ void k_colsum_final(const float* __restrict__ work, int nblk, int cols, float* __restrict__ out,
+                               int accumulate) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    float s = 0.0f;
+    for (int b = 0; b < nblk; ++b) s = s + work[(size_t)b * cols + c];
+    out[c] = accumulate ? out[c] + s : s;
+}
+
+__global__ void k_to_bf16(const float* __restrict__ x, int64_t n, uint16_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = f2bf(x[i]);
+}
+
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps, float c1, float c2,
+                       uint16_t* __restrict__ wb) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * (gi * gi);
+    m[i] = mi; v[i] = vi;
+    const float mh = mi / c1, vh = vi / c2;
+    const float wi = w[i] - lr * mh / (sqrtf(vh) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = f2bf(wi);
+}
+
+// one workgroup: per-row sums in parallel, rows added in order by thread 0
+__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const float* __restrict__ t,
+                                             const float* __restrict__ wt, int B, int D, float* __restrict__ dy,
+                                             uint16_t* __restrict__ dyb, float* __restrict__ loss) {
+    __shared__ float rs[256];
+    const float inv = 1.0f / (float)((int64_t)B * D);
+    float total = 0.0f;
+    for (int r0 = 0; r0 < B; r0 += 256) {
+        const int r = r0 + threadIdx.x;
+        float s = 0.0f;
+        if (r < B) {
+            for (int o = 0; o < D; ++o) {
+                const size_t i = (size_t)r * D + o;
+                const float d = y[i] - t[i];
+                s = s + wt[o] * (d * d);
+                const float gi = 2.0f * wt[o] * d * inv;
+                dy[i] = gi;
+                if (dyb) dyb[i] = f2bf(gi);
+            }
+        }
+        rs[threadIdx.x] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int m = min(256, B - r0);
+            for (int q = 0; q < m; ++q) total = total + rs[q];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss[0] = total * inv;
+}
+
+__global__ void k_concat(const uint16_t* __restrict__ feat, int ldf, int F, const float* __restrict__ extra, int lde,
+                         int E, int B, uint16_t* __restrict__ z, int ldz) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)B * ldz) return;
+    const int b = (int)(idx / ldz), j = (int)(idx % ldz);
+    uint16_t v = 0;
+    if (j < F) v = feat[(size_t)b * ldf + j];
+    else if (j < F + E) v = f2bf(extra[(size_t)b * lde + (j - F)]);
+    z[idx] = v;
+}
+
+__global__ void k_split(const float* __restrict__ dz, int lddz, int F, int B, float* __restrict__ dfeat) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)B * F) return;
+    dfeat[idx] = dz[(idx / F) * lddz + idx % F];
+}
+
+// thread per (pose, element): splat the element's equivalent disc onto the
+// cells whose centres it covers (integer atomics: the sum is order-free)
+__global__ void k_raster_splat(const float* __restrict__ pos, const float* __restrict__ area,
+                               const float* __restrict__ thick, int n, const float* __restrict__ pose, int B, int G,
+                               float cell, float x_lo, int32_t* __restrict__ acc) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)B * n) return;
+    const int b = (int)(idx / n), e = (int)(idx % n);
+    const float px = pose[3 * b], py = pose[3 * b + 1], ph = pose[3 * b + 2];
+    const float c = cosf(ph), s = sinf(ph);
+    const float dx = pos[e] - px, dy = pos[n + e] - py;
+    const float u = c * dx + s * dy, v = -s * dx + c * dy;      // ship frame
+    const float r = sqrtf(area[e] / 3.14159265f);
+    const float y_lo = -0.5f * (float)G * cell;
+    const int i0 = max(0, (int)floorf((u - r - x_lo) / cell)), i1 = min(G - 1, (int)floorf((u + r - x_lo) / cell));
+    const int j0 = max(0, (int)floorf((v - r - y_lo) / cell)), j1 = min(G - 1, (int)floorf((v + r - y_lo) / cell));
+    const int ca = (int)(cell * cell * 4096.0f);
+    const int ch = (int)(cell * cell * thick[e] * 4096.0f);
+    for (int i = i0; i <= i1; ++i) {
+        const float cu = x_lo + ((float)i + 0.5f) * cell;
+        for (int j = j0; j <= j1; ++j) {
+            const float cv = y_lo + ((float)j + 0.5f) * cell;
+            if ((cu - u) * (cu - u) + (cv - v) * (cv - v) > r * r) continue;
+            int32_t* a = acc + (((size_t)b * G + i) * G + j) * 2;
+            atomicAdd(a, ca);
+            atomicAdd(a + 1, ch);
+        }
+    }
+}
+
+__global__ void k_raster_out(int32_t* __restrict__ acc, int B, int G, float cell, uint16_t* __restrict__ out) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)B * G * G) return;
+    const float a = (float)acc[2 * idx], ah = (float)acc[2 * idx + 1];
+    const float ca = cell * cell * 4096.0f;
+    out[2 * idx] = f2bf(fminf(a / ca, 1.0f));
+    out[2 * idx + 1] = f2bf(a > 0.0f ? ah / a : 0.0f);
+    acc[2 * idx] = 0;
+    acc[2 * idx + 1] = 0;
+}
+
+}  // namespace icesur
+
+using namespace icesur;
+
+#define SUR_CHECK(cond, msg) do { if (!(cond)) { icedem_set_error(msg); return ICEDEM_E_INVALID; } } while (0)
+#define SUR_LAUNCHED() do { hipError_t _e = hipGetLastError(); if (_e != hipSuccess) { icedem_set_error(std::string("launch: ") + hipGetErrorString(_e)); return ICEDEM_E_HIP; } } while (0)
+
+static unsigned grid1(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
+
+extern "C" {
+
+int icesur_gemm(int32_t trans_a, int32_t trans_b, int32_t M, int32_t N, int32_t K, const uint16_t* d_a, int32_t lda,
+                const uint16_t* d_b, int32_t ldb, float* d_c, int32_t ldc, uint16_t* d_cb, int32_t ldcb,
+                const float* d_bias, int32_t flags, int32_t split_k, float* d_work, void* stream) {
+    SUR_CHECK(M >= 0 && N >= 0 && K >= 0 && d_a && d_b && (d_c || d_cb), "icesur_gemm: bad arguments");
+    SUR_CHECK(!(flags & ICESUR_ACCUM) || d_c, "icesur_gemm: accumulate needs fp32 C");
+    SUR_CHECK(!(flags & ICESUR_BIAS) || d_bias, "icesur_gemm: bias missing");
+    SUR_CHECK(split_k >= 1 && (split_k == 1 || d_work), "icesur_gemm: split_k needs a workspace");
+    SUR_CHECK((trans_a ? lda >= M : lda >= K) && (trans_b ? ldb >= K : ldb >= N), "icesur_gemm: leading dimension");
+    if (M == 0 || N == 0) return ICEDEM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    Epi E{d_c, ldc, d_cb, ldcb, d_bias, flags};
+    int ks = K;
+    int splits = 1;
+    if (split_k > 1) {
+        ks = ((K + split_k - 1) / split_k + TK - 1) / TK * TK;
+        splits = (K + ks - 1) / ks;
+        if (splits < 1) splits = 1;
+    }
+    dim3 grid((M + TM - 1) / TM, (N + TN - 1) / TN, splits);
+    float* part = splits > 1 ? d_work : nullptr;
+    if (K == 0) part = nullptr;
+#define GEMM_LAUNCH(TA_, TB_) hipLaunchKernelGGL((k_gemm<TA_, TB_>), grid, dim3(256), 0, s, M, N, K, d_a, lda, d_b, ldb, E, ks > 0 ? ks : 1, part)
+    if (!trans_a && !trans_b) GEMM_LAUNCH(false, false);
+    else if (trans_a && !trans_b) GEMM_LAUNCH(true, false);
+    else if (!trans_a && trans_b) GEMM_LAUNCH(false, true);
+    else GEMM_LAUNCH(true, true);
+#undef GEMM_LAUNCH
+    if (part) hipLaunchKernelGGL(k_splitk_reduce, dim3(grid1((size_t)M * N)), dim3(256), 0, s, M, N, splits, d_work, E);
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+int icesur_im2col(const uint16_t* d_x, int32_t B, int32_t H, int32_t W, int32_t C, uint16_t* d_col, void* stream) {
+    SUR_CHECK(B >= 0 && H > 0 && W > 0 && C > 0 && d_x && d_col, "icesur_im2col: bad arguments");
+    const int Ho = (H + 1) / 2, Wo = (W + 1) / 2;
+    const size_t total = (size_t)B * Ho * Wo * 9 * C;
+    if (total == 0) return ICEDEM_OK;
+    hipLaunchKernelGGL(k_im2col, dim3(grid1(total)), dim3(256), 0, (hipStream_t)stream, d_x, B, H, W, C, Ho, Wo, d_col);
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+int icesur_col2im(const float* d_dcol, int32_t B, int32_t H, int32_t W, int32_t C, float* d_dx, void* stream) {
+    SUR_CHECK(B >= 0 && H > 0 && W > 0 && C > 0 && d_dcol && d_dx, "icesur_col2im: bad arguments");
+    const int Ho = (H + 1) / 2, Wo = (W + 1) / 2;
+    const size_t total = (size_t)B * H * W * C;
+    if (total == 0) return ICEDEM_OK;
+    hipLaunchKernelGGL(k_col2im, dim3(grid1(total)), dim3(256), 0, (hipStream_t)stream, d_dcol, B, H, W, C, Ho, Wo, d_dx);
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+int icesur_relu_back(float* d_g, int32_t ldg, const uint16_t* d_y, int32_t ldy, int32_t rows, int32_t cols,
+                     uint16_t* d_gb, int32_t ldgb, void* stream) {
+    SUR_CHECK(rows >= 0 && cols >= 0 && d_g && d_y, "icesur_relu_back: bad arguments");
+    if ((size_t)rows * cols == 0) return ICEDEM_OK;
+    hipLaunchKernelGGL(k_relu_back, dim3(grid1((size_t)rows * cols)), dim3(256), 0, (hipStream_t)stream, d_g, ldg, d_y,
+                       ldy, rows, cols, d_gb, ldgb);
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+int icesur_colsum(const float* d_x, int32_t ldx, int32_t rows, int32_t cols, float* d_out, int32_t accumulate,
+                  float* d_work, void* stream) {
+    SUR_CHECK(rows >= 0 && cols > 0 && d_x && d_out && d_work, "icesur_colsum: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const int nblk = (rows + 255) / 256;
+    if (nblk > 0)
+        hipLaunchKernelGGL(k_colsum_part, dim3(nblk, (cols + 63) / 64), dim3(64), 0, s, d_x, ldx, rows, cols, d_work);
+    hipLaunchKernelGGL(k_colsum_final, dim3(grid1(cols, 64)), dim3(64), 0, s, d_work, nblk, cols, d_out, accumulate);
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+int icesur_to_bf16(const float* d_x, int64_t n, uint16_t* d_out, void* stream) {
+    SUR_CHECK(n >= 0 && d_x && d_out, "icesur_to_bf16: bad arguments");
+    if (n == 0) return ICEDEM_OK;
+    hipLaunchKernelGGL(k_to_bf16, dim3(grid1((size_t)n)), dim3(256), 0, (hipStream_t)stream, d_x, n, d_out);
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+int icesur_adam(float* d_w, const float* d_g, float* d_m, float* d_v, int64_t n, float lr, float beta1, float beta2,
+                float eps, int32_t t, uint16_t* d_wb, void* stream) {
+    SUR_CHECK(n >= 0 && t >= 1 && d_w && d_g && d_m && d_v, "icesur_adam: bad arguments");
+    if (n == 0) return ICEDEM_OK;
+    const float c1 = 1.0f - powf(beta1, (float)t), c2 = 1.0f - powf(beta2, (float)t);
+    hipLaunchKernelGGL(k_adam, dim3(grid1((size_t)n)), dim3(256), 0, (hipStream_t)stream, d_w, d_g, d_m, d_v, n, lr,
+                       beta1, beta2, eps, c1, c2, d_wb);
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+int icesur_mse(const float* d_y, const float* d_t, const float* d_w, int32_t B, int32_t D, float* d_dy,
+               uint16_t* d_dyb, float* d_loss, void* stream) {
+    SUR_CHECK(B > 0 && D > 0 && d_y && d_t && d_w && d_dy && d_loss, "icesur_mse: bad arguments");
+    hipLaunchKernelGGL(k_mse, dim3(1), dim3(256), 0, (hipStream_t)stream, d_y, d_t, d_w, B, D, d_dy, d_dyb, d_loss);
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+int icesur_concat(const uint16_t* d_feat, int32_t ldf, int32_t F, const float* d_extra, int32_t lde, int32_t E,
+                  int32_t B, uint16_t* d_z, int32_t ldz, void* stream) {
+    SUR_CHECK(B >= 0 && F >= 0 && E >= 0 && ldz >= F + E && d_feat && d_z && (E == 0 || d_extra),
+              "icesur_concat: bad arguments");
+    if ((size_t)B * ldz == 0) return ICEDEM_OK;
+    hipLaunchKernelGGL(k_concat, dim3(grid1((size_t)B * ldz)), dim3(256), 0, (hipStream_t)stream, d_feat, ldf, F,
+                       d_extra, lde, E, B, d_z, ldz);
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+int icesur_split(const float* d_dz, int32_t lddz, int32_t F, int32_t B, float* d_dfeat, void* stream) {
+    SUR_CHECK(B >= 0 && F >= 0 && lddz >= F && d_dz && d_dfeat, "icesur_split: bad arguments");
+    if ((size_t)B * F == 0) return ICEDEM_OK;
+    hipLaunchKernelGGL(k_split, dim3(grid1((size_t)B * F)), dim3(256), 0, (hipStream_t)stream, d_dz, lddz, F, B, d_dfeat);
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+int icesur_raster(const float* d_pos, const float* d_area, const float* d_thick, int32_t n, const float* d_pose,
+                  int32_t B, int32_t G, float cell, float x_lo, int32_t* d_acc, uint16_t* d_out, void* stream) {
+    SUR_CHECK(n >= 0 && B >= 0 && G > 0 && G <= 4096 && cell > 0.0f && d_pose && d_acc && d_out &&
+              (n == 0 || (d_pos && d_area && d_thick)), "icesur_raster: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    if ((size_t)B * n > 0)
+        hipLaunchKernelGGL(k_raster_splat, dim3(grid1((size_t)B * n)), dim3(256), 0, s, d_pos, d_area, d_thick, n,
+                           d_pose, B, G, cell, x_lo, d_acc);
+    if ((size_t)B * G * G > 0)
+        hipLaunchKernelGGL(k_raster_out, dim3(grid1((size_t)B * G * G)), dim3(256), 0, s, d_acc, B, G, cell, d_out);
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+}  // extern "C"

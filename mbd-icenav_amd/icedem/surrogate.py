@@ -1,0 +1,315 @@
+"""Rollout-cost surrogate: CNN over an ice raster + MLP (no reference counterpart).
+
+Predicts a DEM rollout's (impulse, KE loss, peak |F|) from
+  - a G x G x 2 raster of the ice ahead of the ship (area fraction, mean
+    thickness; bf16), built on the GPU by ``icesur_raster``,
+  - the ship's initial state (6 values) and the rollout's control knots
+    (n_knots x 2), fp32.
+Network: 3x3 stride-2 convolutions (32/64/128/128 channels, ReLU) -> 4x4x128
+features, concatenated with the 86 extra inputs -> MLP 512 -> 512 -> 3.
+
+Every floating-point operation of the forward pass, the backward pass and
+the Adam update runs in this package's HIP kernels (icesur.h): GEMMs in
+BF16 on the matrix cores with fp32 accumulation, fp32 master weights.
+Torch tensors are only device buffers here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+
+RELU, ACCUM, BIAS = 1, 2, 4
+
+
+def _p(t: Optional[torch.Tensor]):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _rup(x: int, m: int) -> int:
+    return (x + m - 1) // m * m
+
+
+@dataclass
+class Arch:
+    G: int = 64
+    cin: int = 2
+    chans: Tuple[int, ...] = (32, 64, 128, 128)
+    extra: int = 86
+    hidden: Tuple[int, ...] = (512, 512)
+    out: int = 3
+
+    def conv_shapes(self):
+        s, c, shapes = self.G, self.cin, []
+        for co in self.chans:
+            shapes.append((s, c, co))
+            s, c = (s + 1) // 2, co
+        return shapes, s, c
+
+    def feat(self) -> int:
+        _, s, c = self.conv_shapes()
+        return s * s * c
+
+    def param_shapes(self) -> Dict[str, tuple]:
+        p = {}
+        shapes, _, _ = self.conv_shapes()
+        for i, (_, c, co) in enumerate(shapes):
+            p[f"cw{i}"] = (9 * c, co)
+            p[f"cb{i}"] = (co,)
+        dims = [self.feat() + self.extra] + list(self.hidden) + [self.out]
+        for i in range(len(dims) - 1):
+            p[f"fw{i}"] = (dims[i], dims[i + 1])
+            p[f"fb{i}"] = (dims[i + 1],)
+        return p
+
+    def flops_per_sample(self) -> float:
+        """Multiply-adds x 2 of one forward pass."""
+        shapes, _, _ = self.conv_shapes()
+        mac = sum(((s + 1) // 2) ** 2 * 9 * c * co for s, c, co in shapes)
+        dims = [self.feat() + self.extra] + list(self.hidden) + [self.out]
+        mac += sum(dims[i] * dims[i + 1] for i in range(len(dims) - 1))
+        return 2.0 * mac
+
+
+class Surrogate:
+    """Parameters, Adam state and work buffers of the surrogate on one device."""
+
+    def __init__(self, arch: Optional[Arch] = None, device=None, seed: int = 0,
+                 params: Optional[Dict[str, np.ndarray]] = None):
+        if not torch.cuda.is_available():
+            raise _lib.IceDemError("the surrogate needs a ROCm GPU")
+        self.lib = _lib.load()
+        self.arch = arch or Arch()
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        shapes = self.arch.param_shapes()
+        self.names = list(shapes)
+        self.offsets = {}
+        off = 0
+        for k, shp in shapes.items():
+            n = int(np.prod(shp))
+            self.offsets[k] = (off, shp)
+            off = _rup(off + n, 8)          # every parameter 16-byte aligned in bf16
+        self.nparam = off
+        dev = self.device
+        self.w = torch.zeros(off, dtype=torch.float32, device=dev)
+        self.g = torch.zeros(off, dtype=torch.float32, device=dev)
+        self.m = torch.zeros(off, dtype=torch.float32, device=dev)
+        self.v = torch.zeros(off, dtype=torch.float32, device=dev)
+        self.wb = torch.zeros(off, dtype=torch.bfloat16, device=dev)
+        self.t = 0
+        self.load(params if params is not None else self.init_params(seed))
+        self._bufs: Dict[Tuple[str, int], torch.Tensor] = {}
+        self.out_w = torch.ones(self.arch.out, dtype=torch.float32, device=dev)
+        self.loss = torch.zeros(1, dtype=torch.float32, device=dev)
+
+    # ── parameters ──────────────────────────────────────────────────────────
+    def init_params(self, seed: int) -> Dict[str, np.ndarray]:
+        """He-uniform weights, zero biases."""
+        rng = np.random.default_rng(seed)
+        out = {}
+        for k, (_, shp) in self.offsets.items():
+            if len(shp) == 2:
+                lim = math.sqrt(6.0 / shp[0])
+                out[k] = rng.uniform(-lim, lim, shp).astype(np.float32)
+            else:
+                out[k] = np.zeros(shp, np.float32)
+        return out
+
+    def load(self, params: Dict[str, np.ndarray]):
+        host = np.zeros(self.nparam, np.float32)
+        for k, (o, shp) in self.offsets.items():
+            host[o:o + int(np.prod(shp))] = np.asarray(params[k], np.float32).reshape(-1)
+        self.w.copy_(torch.as_tensor(host))
+        self._chk(self.lib.icesur_to_bf16(_p(self.w), self.nparam, _p(self.wb), self._s()), "to_bf16")
+
+    def params(self) -> Dict[str, np.ndarray]:
+        host = self.w.cpu().numpy()
+        return {k: host[o:o + int(np.prod(shp))].reshape(shp).copy() for k, (o, shp) in self.offsets.items()}
+
+    def grads(self) -> Dict[str, np.ndarray]:
+        host = self.g.cpu().numpy()
+        return {k: host[o:o + int(np.prod(shp))].reshape(shp).copy() for k, (o, shp) in self.offsets.items()}
+
+    def _view(self, t: torch.Tensor, k: str) -> torch.Tensor:
+        o, shp = self.offsets[k]
+        return t[o:o + int(np.prod(shp))]
+
+    # ── plumbing ────────────────────────────────────────────────────────────
+    def _s(self):
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _chk(self, rc, what):
+        _lib.check(rc, None, what)
+
+    def _buf(self, name: str, n: int, dtype) -> torch.Tensor:
+        key = (name, 0)
+        b = self._bufs.get(key)
+        if b is None or b.numel() < n or b.dtype != dtype:
+            b = torch.empty(max(n, 1), dtype=dtype, device=self.device)
+            self._bufs[key] = b
+        return b[:n]
+
+    def _gemm(self, ta, tb, M, N, K, A, lda, B, ldb, Cf=None, ldc=0, Cb=None, ldcb=0, bias=None, flags=0,
+              split=1):
+        work = None
+        if split > 1:
+            work = self._buf("splitk", split * M * N, torch.float32)
+        self._chk(self.lib.icesur_gemm(int(ta), int(tb), M, N, K, _p(A), lda, _p(B), ldb, _p(Cf), ldc, _p(Cb), ldcb,
+                                       _p(bias), flags, split, _p(work), self._s()), "icesur_gemm")
+
+    @staticmethod
+    def _split_for(K: int) -> int:
+        """Split-K factor for a long reduction with few output tiles."""
+        return int(min(64, max(1, K // 4096)))
+
+    # ── forward ─────────────────────────────────────────────────────────────
+    def forward(self, raster: torch.Tensor, extra: torch.Tensor, keep: bool = False) -> torch.Tensor:
+        """raster (B, G, G, cin) bf16, extra (B, arch.extra) fp32 -> (B, out) fp32."""
+        a = self.arch
+        B = int(raster.shape[0])
+        assert raster.dtype == torch.bfloat16 and tuple(raster.shape[1:]) == (a.G, a.G, a.cin)
+        assert extra.dtype == torch.float32 and tuple(extra.shape) == (B, a.extra)
+        raster = raster.contiguous()
+        extra = extra.contiguous()
+        x = raster
+        shapes, _, _ = a.conv_shapes()
+        self._acts: List[torch.Tensor] = []
+        self._cols: List[torch.Tensor] = []
+        for i, (s, c, co) in enumerate(shapes):
+            so = (s + 1) // 2
+            rows = B * so * so
+            col = self._buf(f"col{i}", rows * 9 * c, torch.bfloat16)
+            self._chk(self.lib.icesur_im2col(_p(x), B, s, s, c, _p(col), self._s()), "im2col")
+            act = self._buf(f"act{i}", rows * co, torch.bfloat16)
+            self._gemm(0, 0, rows, co, 9 * c, col, 9 * c, self._view(self.wb, f"cw{i}"), co, Cb=act, ldcb=co,
+                       bias=self._view(self.w, f"cb{i}"), flags=BIAS | RELU)
+            self._cols.append(col)
+            self._acts.append(act)
+            x = act
+        F = a.feat()
+        ldz = _rup(F + a.extra, 8)
+        z = self._buf("z0", B * ldz, torch.bfloat16)
+        self._chk(self.lib.icesur_concat(_p(x), F, F, _p(extra), a.extra, a.extra, B, _p(z), ldz, self._s()),
+                  "concat")
+        dims = [F + a.extra] + list(a.hidden) + [a.out]
+        self._zs = [(z, ldz)]
+        nl = len(dims) - 1
+        y = None
+        for i in range(nl):
+            zin, ldin = self._zs[-1]
+            N = dims[i + 1]
+            if i < nl - 1:
+                h = self._buf(f"z{i + 1}", B * N, torch.bfloat16)
+                self._gemm(0, 0, B, N, dims[i], zin, ldin, self._view(self.wb, f"fw{i}"), N, Cb=h, ldcb=N,
+                           bias=self._view(self.w, f"fb{i}"), flags=BIAS | RELU)
+                self._zs.append((h, N))
+            else:
+                y = self._buf("y", B * N, torch.float32)
+                self._gemm(0, 0, B, N, dims[i], zin, ldin, self._view(self.wb, f"fw{i}"), N, Cf=y, ldc=N,
+                           bias=self._view(self.w, f"fb{i}"), flags=BIAS)
+        self._B = B
+        return y.view(B, a.out)
+
+    # ── backward ────────────────────────────────────────────────────────────
+    def backward(self, dy: torch.Tensor, dyb: torch.Tensor):
+        """Gradients (into self.g) of sum(dy * y) for the last forward;
+        dyb = bf16(dy)."""
+        a = self.arch
+        B = self._B
+        F = a.feat()
+        dims = [F + a.extra] + list(a.hidden) + [a.out]
+        nl = len(dims) - 1
+        d, db = dy.reshape(B, -1), dyb.reshape(B, -1)
+        dz = None
+        for i in reversed(range(nl)):
+            zin, ldin = self._zs[i]
+            K, N = dims[i], dims[i + 1]
+            self._gemm(1, 0, K, N, B, zin, ldin, db, N, Cf=self._view(self.g, f"fw{i}"), ldc=N,
+                       split=self._split_for(B))
+            self._chk(self.lib.icesur_colsum(_p(d), N, B, N, _p(self._view(self.g, f"fb{i}")), 0,
+                                             _p(self._buf("csum", ((B + 255) // 256) * N, torch.float32)),
+                                             self._s()), "colsum")
+            ldk = ldin
+            dz = self._buf(f"dz{i}", B * ldk, torch.float32)
+            self._gemm(0, 1, B, K, N, db, N, self._view(self.wb, f"fw{i}"), N, Cf=dz, ldc=ldk)
+            if i > 0:
+                dzb = self._buf(f"dzb{i}", B * ldk, torch.bfloat16)
+                self._chk(self.lib.icesur_relu_back(_p(dz), ldk, _p(zin), ldin, B, K, _p(dzb), ldk, self._s()),
+                          "relu_back")
+                d, db = dz, dzb
+        # into the CNN: the first F columns of dz, masked by the last activation
+        shapes, _, _ = a.conv_shapes()
+        dfeat = self._buf("dfeat", B * F, torch.float32)
+        self._chk(self.lib.icesur_split(_p(dz), self._zs[0][1], F, B, _p(dfeat), self._s()), "split")
+        co_last = shapes[-1][2]
+        rows_last = B * F // co_last
+        dfb = self._buf("dfeatb", B * F, torch.bfloat16)
+        self._chk(self.lib.icesur_relu_back(_p(dfeat), co_last, _p(self._acts[-1]), co_last, rows_last, co_last,
+                                            _p(dfb), co_last, self._s()), "relu_back")
+        d, db = dfeat, dfb
+        for i in reversed(range(len(shapes))):
+            s, c, co = shapes[i]
+            so = (s + 1) // 2
+            rows = B * so * so
+            self._gemm(1, 0, 9 * c, co, rows, self._cols[i], 9 * c, db, co, Cf=self._view(self.g, f"cw{i}"), ldc=co,
+                       split=self._split_for(rows))
+            self._chk(self.lib.icesur_colsum(_p(d), co, rows, co, _p(self._view(self.g, f"cb{i}")), 0,
+                                             _p(self._buf("csum", ((rows + 255) // 256) * co, torch.float32)),
+                                             self._s()), "colsum")
+            if i > 0:
+                dcol = self._buf("dcol", rows * 9 * c, torch.float32)
+                self._gemm(0, 1, rows, 9 * c, co, db, co, self._view(self.wb, f"cw{i}"), co, Cf=dcol, ldc=9 * c)
+                dx = self._buf(f"dx{i}", B * s * s * c, torch.float32)
+                self._chk(self.lib.icesur_col2im(_p(dcol), B, s, s, c, _p(dx), self._s()), "col2im")
+                dxb = self._buf(f"dxb{i}", B * s * s * c, torch.bfloat16)
+                self._chk(self.lib.icesur_relu_back(_p(dx), c, _p(self._acts[i - 1]), c, B * s * s, c, _p(dxb), c,
+                                                    self._s()), "relu_back")
+                d, db = dx, dxb
+
+    # ── training ────────────────────────────────────────────────────────────
+    def train_step(self, raster, extra, target, lr: float = 1e-3, beta1: float = 0.9, beta2: float = 0.999,
+                   eps: float = 1e-8) -> torch.Tensor:
+        """One Adam step on the weighted MSE; returns the loss (device tensor)."""
+        y = self.forward(raster, extra)
+        B = y.shape[0]
+        dy = self._buf("dy", B * self.arch.out, torch.float32)
+        dyb = self._buf("dyb", B * self.arch.out, torch.bfloat16)
+        self._chk(self.lib.icesur_mse(_p(y), _p(target.contiguous()), _p(self.out_w), B, self.arch.out, _p(dy),
+                                      _p(dyb), _p(self.loss), self._s()), "mse")
+        self.backward(dy, dyb)
+        self.t += 1
+        self._chk(self.lib.icesur_adam(_p(self.w), _p(self.g), _p(self.m), _p(self.v), self.nparam, lr, beta1, beta2,
+                                       eps, self.t, _p(self.wb), self._s()), "adam")
+        return self.loss
+
+    def predict(self, raster, extra, chunk: int = 4096) -> torch.Tensor:
+        """Forward pass over any number of samples, `chunk` at a time."""
+        B = raster.shape[0]
+        out = torch.empty((B, self.arch.out), dtype=torch.float32, device=self.device)
+        for s in range(0, B, chunk):
+            e = min(B, s + chunk)
+            out[s:e] = self.forward(raster[s:e], extra[s:e])
+        return out
+
+
+def raster(lib, pos: torch.Tensor, area: torch.Tensor, thick: torch.Tensor, poses: torch.Tensor, G: int = 64,
+           cell: float = 1.0, x_lo: float = 20.0, acc: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """(B, G, G, 2) bf16 rasters of one scene (pos (2, n) SoA fp32) seen from
+    B ship poses (B, 3) = x, y, heading; the window starts x_lo ahead of the
+    ship's centre."""
+    B = int(poses.shape[0])
+    n = int(area.shape[0])
+    dev = poses.device
+    if acc is None or acc.numel() < B * G * G * 2:
+        acc = torch.zeros(B * G * G * 2, dtype=torch.int32, device=dev)
+    out = torch.empty((B, G, G, 2), dtype=torch.bfloat16, device=dev)
+    s = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    _lib.check(lib.icesur_raster(_p(pos), _p(area), _p(thick), n, _p(poses.contiguous()), B, G, float(cell),
+                                 float(x_lo), _p(acc), _p(out), s), None, "icesur_raster")
+    return out

@@ -1,0 +1,204 @@
+"""Surrogate kernels (BF16 MFMA GEMM, im2col/col2im, reductions, Adam, raster)
+and the whole network's forward and backward against the NumPy oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import surrogate as osur
+
+pytestmark = pytest.mark.gpu
+
+
+def _lib():
+    from icedem import _lib
+    return _lib, _lib.load()
+
+
+def _bf(a):
+    return torch.as_tensor(osur.bf16(a)).to(device="cuda", dtype=torch.bfloat16)
+
+
+def _p(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _s():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("M,N,K,split", [(16, 16, 32, 1), (70, 45, 100, 1), (128, 64, 2134, 1), (37, 29, 5000, 4)])
+def test_gemm_matches_float64_of_bf16_inputs(ta, tb, M, N, K, split):
+    L, lib = _lib()
+    rng = np.random.default_rng(M * 7 + N * 3 + K + ta * 2 + tb)
+    A = osur.bf16(rng.normal(size=(K, M) if ta else (M, K)))
+    B = osur.bf16(rng.normal(size=(N, K) if tb else (K, N)))
+    bias = rng.normal(size=N).astype(np.float32)
+    opA = A.T.astype(np.float64) if ta else A.astype(np.float64)
+    opB = B.T.astype(np.float64) if tb else B.astype(np.float64)
+    ref = opA @ opB + bias
+    bound = (np.abs(opA) @ np.abs(opB)) * (K * 2.0 ** -23) + 1e-6
+    dA, dB = _bf(A), _bf(B)
+    Cg = torch.full((M, N), 7.0, dtype=torch.float32, device="cuda")
+    Cb = torch.zeros((M, N), dtype=torch.bfloat16, device="cuda")
+    work = torch.zeros(split * M * N, dtype=torch.float32, device="cuda")
+    db = torch.as_tensor(bias, device="cuda")
+    rc = lib.icesur_gemm(ta, tb, M, N, K, _p(dA), M if ta else K, _p(dB), K if tb else N, _p(Cg), N, _p(Cb), N,
+                         _p(db), 4, split, _p(work), _s())
+    assert rc == 0, lib.icedem_last_error(None)
+    got = Cg.cpu().numpy().astype(np.float64)
+    err = np.abs(got - ref)
+    print(f"gemm ta={ta} tb={tb} {M}x{N}x{K} split={split}: max err/bound {np.max(err / bound):.3g}", flush=True)
+    assert np.all(err <= bound), np.max(err / bound)
+    np.testing.assert_array_equal(Cb.float().cpu().numpy(), osur.bf16(Cg.cpu().numpy()))
+    # relu + accumulate
+    C2 = Cg.clone()
+    rc = lib.icesur_gemm(ta, tb, M, N, K, _p(dA), M if ta else K, _p(dB), K if tb else N, _p(C2), N, None, 0,
+                         None, 1 | 2, split, _p(work), _s())
+    assert rc == 0
+    ref2 = np.maximum(got + (opA @ opB), 0.0)
+    assert np.all(np.abs(C2.cpu().numpy() - ref2) <= 2 * bound + 1e-5 * np.abs(ref2))
+
+
+def test_im2col_col2im_relu_colsum():
+    L, lib = _lib()
+    rng = np.random.default_rng(5)
+    B, H, W, Cc = 3, 9, 8, 5
+    x = osur.bf16(rng.normal(size=(B, H, W, Cc)))
+    dx_ = _bf(x)
+    Ho, Wo = (H + 1) // 2, (W + 1) // 2
+    col = torch.zeros((B * Ho * Wo, 9 * Cc), dtype=torch.bfloat16, device="cuda")
+    assert lib.icesur_im2col(_p(dx_), B, H, W, Cc, _p(col), _s()) == 0
+    np.testing.assert_array_equal(col.float().cpu().numpy(), osur.im2col(x))
+    d = rng.normal(size=col.shape).astype(np.float32)
+    dd = torch.as_tensor(d, device="cuda")
+    out = torch.zeros((B, H, W, Cc), dtype=torch.float32, device="cuda")
+    assert lib.icesur_col2im(_p(dd), B, H, W, Cc, _p(out), _s()) == 0
+    ref = osur.col2im(d.astype(np.float64), (B, H, W, Cc))
+    assert np.max(np.abs(out.cpu().numpy() - ref)) <= 1e-5
+    # relu backward
+    g = torch.as_tensor(d.copy(), device="cuda")
+    y = _bf(rng.normal(size=d.shape))
+    gb = torch.zeros(d.shape, dtype=torch.bfloat16, device="cuda")
+    assert lib.icesur_relu_back(_p(g), d.shape[1], _p(y), d.shape[1], d.shape[0], d.shape[1], _p(gb), d.shape[1],
+                                _s()) == 0
+    mref = np.where(y.float().cpu().numpy() > 0, d, 0.0).astype(np.float32)
+    np.testing.assert_array_equal(g.cpu().numpy(), mref)
+    np.testing.assert_array_equal(gb.float().cpu().numpy(), osur.bf16(mref))
+    # column sums: 256-row blocks summed in order, then the blocks in order (float32)
+    R, Cn = 700, 13
+    xs = rng.normal(size=(R, Cn)).astype(np.float32)
+    out = torch.zeros(Cn, dtype=torch.float32, device="cuda")
+    work = torch.zeros(3 * Cn, dtype=torch.float32, device="cuda")
+    assert lib.icesur_colsum(_p(torch.as_tensor(xs, device="cuda")), Cn, R, Cn, _p(out), 0, _p(work), _s()) == 0
+    ref = np.zeros(Cn, np.float32)
+    for b0 in range(0, R, 256):
+        part = np.zeros(Cn, np.float32)
+        for r in range(b0, min(R, b0 + 256)):
+            part = (part + xs[r]).astype(np.float32)
+        ref = (ref + part).astype(np.float32)
+    np.testing.assert_array_equal(out.cpu().numpy(), ref)
+
+
+def test_mse_and_adam():
+    L, lib = _lib()
+    rng = np.random.default_rng(9)
+    Bn, D = 300, 3
+    y = rng.normal(size=(Bn, D)).astype(np.float32)
+    t = rng.normal(size=(Bn, D)).astype(np.float32)
+    w = np.array([1.0, 0.5, 2.0], np.float32)
+    dy = torch.zeros((Bn, D), dtype=torch.float32, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    assert lib.icesur_mse(_p(torch.as_tensor(y, device="cuda")), _p(torch.as_tensor(t, device="cuda")),
+                          _p(torch.as_tensor(w, device="cuda")), Bn, D, _p(dy), None, _p(loss), _s()) == 0
+    ref_loss = float((w * (y - t) ** 2).sum() / (Bn * D))
+    assert abs(loss.item() - ref_loss) <= 1e-5 * ref_loss
+    np.testing.assert_allclose(dy.cpu().numpy(), 2 * w * (y - t) / (Bn * D), rtol=1e-6)
+    n = 1000
+    wt = rng.normal(size=n).astype(np.float32)
+    g = rng.normal(size=n).astype(np.float32)
+    dw = torch.as_tensor(wt.copy(), device="cuda")
+    dm = torch.zeros(n, device="cuda")
+    dv = torch.zeros(n, device="cuda")
+    wb = torch.zeros(n, dtype=torch.bfloat16, device="cuda")
+    for step in (1, 2):
+        assert lib.icesur_adam(_p(dw), _p(torch.as_tensor(g, device="cuda")), _p(dm), _p(dv), n, 1e-2, 0.9, 0.999,
+                               1e-8, step, _p(wb), _s()) == 0
+    m = np.zeros(n); v = np.zeros(n); ww = wt.astype(np.float64)
+    for step in (1, 2):
+        m = 0.9 * m + 0.1 * g; v = 0.999 * v + 0.001 * g * g
+        ww = ww - 1e-2 * (m / (1 - 0.9 ** step)) / (np.sqrt(v / (1 - 0.999 ** step)) + 1e-8)
+    np.testing.assert_allclose(dw.cpu().numpy(), ww, rtol=1e-5, atol=1e-6)
+    np.testing.assert_array_equal(wb.float().cpu().numpy(), osur.bf16(dw.cpu().numpy()))
+
+
+def test_raster_of_known_discs():
+    from icedem import surrogate as sur
+    L, lib = _lib()
+    # two discs of area pi (r = 1) and 4 pi (r = 2), ship at the origin heading +x
+    pos = torch.tensor([[5.0, 10.0], [0.0, 3.0]], dtype=torch.float32, device="cuda").reshape(-1)
+    area = torch.tensor([np.pi, 4 * np.pi], dtype=torch.float32, device="cuda")
+    thick = torch.tensor([1.0, 2.0], dtype=torch.float32, device="cuda")
+    poses = torch.tensor([[0.0, 0.0, 0.0]], dtype=torch.float32, device="cuda")
+    r = sur.raster(lib, pos, area, thick, poses, G=16, cell=1.0, x_lo=0.0).float().cpu().numpy()[0]
+    conc, th = r[..., 0], r[..., 1]
+    # disc 1 centred at (5, 0): covers cell centres within 1 m -> cells (4,7),(4,8),(5,7),(5,8)
+    assert conc[4, 8] == 1.0 and conc[5, 7] == 1.0 and th[4, 8] == 1.0
+    # disc 2 centred at (10, 3): thickness 2 where only it covers
+    assert conc[10, 11] == 1.0 and th[10, 11] == 2.0
+    assert conc[0, 0] == 0.0 and th[0, 0] == 0.0
+    # covered cell count = cell centres inside the discs
+    u = np.arange(16) + 0.5
+    cu, cv = np.meshgrid(u, u - 8.0, indexing="ij")
+    inside = ((cu - 5) ** 2 + cv ** 2 <= 1.0) | ((cu - 10) ** 2 + (cv - 3) ** 2 <= 4.0)
+    np.testing.assert_array_equal(conc > 0, inside)
+
+
+def test_network_forward_backward_matches_oracle():
+    from icedem import surrogate as sur
+    arch = sur.Arch(G=16, cin=2, chans=(8, 16, 16), extra=10, hidden=(32, 24), out=3)
+    net = sur.Surrogate(arch, seed=4)
+    params = net.params()
+    rng = np.random.default_rng(11)
+    Bn = 37
+    raster = osur.bf16(rng.uniform(0, 1, (Bn, 16, 16, 2)))
+    extra = rng.normal(0, 1, (Bn, 10)).astype(np.float32)
+    oarch = osur.Arch(G=16, cin=2, chans=(8, 16, 16), extra=10, hidden=(32, 24), out=3)
+    onet = osur.Net(oarch, params, round_bf16=True)
+    yo = onet.forward(raster, extra)
+    y = net.forward(_bf(raster), torch.as_tensor(extra, device="cuda")).cpu().numpy()
+    rel = np.linalg.norm(y - yo) / np.linalg.norm(yo)
+    print("forward rel err", rel, flush=True)
+    assert rel < 2e-3
+    dy = rng.normal(0, 1, (Bn, 3)).astype(np.float32)
+    ddy = torch.as_tensor(dy, device="cuda")
+    net.backward(ddy, ddy.to(torch.bfloat16))
+    g = net.grads()
+    go = onet.backward(dy.astype(np.float64))
+    for k in g:
+        e = np.linalg.norm(g[k] - go[k]) / max(np.linalg.norm(go[k]), 1e-12)
+        print(f"grad {k}: rel err {e:.2e}", flush=True)
+        assert e < 2e-2, k
+
+
+def test_training_reduces_loss():
+    from icedem import surrogate as sur
+    arch = sur.Arch(G=16, cin=2, chans=(8, 16, 16), extra=10, hidden=(32, 24), out=3)
+    net = sur.Surrogate(arch, seed=1)
+    rng = np.random.default_rng(12)
+    Bn = 256
+    raster = _bf(rng.uniform(0, 1, (Bn, 16, 16, 2)))
+    extra_np = rng.normal(0, 1, (Bn, 10)).astype(np.float32)
+    extra = torch.as_tensor(extra_np, device="cuda")
+    # a smooth target of the inputs
+    r = raster.float().cpu().numpy()
+    tgt = np.stack([r[..., 0].mean((1, 2)) * 3, extra_np[:, 0] * 0.5, (r[..., 1] * r[..., 0]).mean((1, 2))], 1)
+    t = torch.as_tensor(tgt.astype(np.float32), device="cuda")
+    first = net.train_step(raster, extra, t, lr=3e-3).item()
+    for _ in range(200):
+        loss = net.train_step(raster, extra, t, lr=3e-3)
+    last = loss.item()
+    print("loss", first, "->", last, flush=True)
+    assert last < 0.2 * first

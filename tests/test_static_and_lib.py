@@ -84,6 +84,10 @@ def test_header_symbols_exported_and_abi_layout(tmp_path):
     hdr = (ROOT / "include" / "icedem.h").read_text()
     names = re.findall(r"^(?:const char\*|int)\s+(icedem_\w+)\s*\(", hdr, flags=re.M)
     assert len(names) >= 14
+    sur = (ROOT / "include" / "icesur.h").read_text()
+    sur_names = re.findall(r"^int\s+(icesur_\w+)\s*\(", sur, flags=re.M)
+    assert len(sur_names) >= 11
+    names += sur_names
     for nm in names:
         assert hasattr(lib, nm), nm
         assert nm in _lib.SIGNATURES, nm
