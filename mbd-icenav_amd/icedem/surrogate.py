@@ -169,15 +169,12 @@ class Surrogate:
         return int(min(64, max(1, K // 4096)))
 
     # ── forward ─────────────────────────────────────────────────────────────
-    def forward(self, raster: torch.Tensor, extra: torch.Tensor, keep: bool = False) -> torch.Tensor:
-        """raster (B, G, G, cin) bf16, extra (B, arch.extra) fp32 -> (B, out) fp32."""
+    def _cnn(self, raster: torch.Tensor) -> torch.Tensor:
+        """(B, G, G, cin) bf16 -> (B, F) bf16 features (the last activation)."""
         a = self.arch
         B = int(raster.shape[0])
         assert raster.dtype == torch.bfloat16 and tuple(raster.shape[1:]) == (a.G, a.G, a.cin)
-        assert extra.dtype == torch.float32 and tuple(extra.shape) == (B, a.extra)
-        raster = raster.contiguous()
-        extra = extra.contiguous()
-        x = raster
+        x = raster.contiguous()
         shapes, _, _ = a.conv_shapes()
         self._acts: List[torch.Tensor] = []
         self._cols: List[torch.Tensor] = []
@@ -192,10 +189,18 @@ class Surrogate:
             self._cols.append(col)
             self._acts.append(act)
             x = act
+        return x
+
+    def _mlp(self, feat: torch.Tensor, ldf: int, extra: torch.Tensor, B: int) -> torch.Tensor:
+        """Concatenate features (row stride ldf; 0 = one row for all) with the
+        extra inputs, then the MLP -> (B, out) fp32."""
+        a = self.arch
         F = a.feat()
+        assert extra.dtype == torch.float32 and tuple(extra.shape) == (B, a.extra)
+        extra = extra.contiguous()
         ldz = _rup(F + a.extra, 8)
         z = self._buf("z0", B * ldz, torch.bfloat16)
-        self._chk(self.lib.icesur_concat(_p(x), F, F, _p(extra), a.extra, a.extra, B, _p(z), ldz, self._s()),
+        self._chk(self.lib.icesur_concat(_p(feat), ldf, F, _p(extra), a.extra, a.extra, B, _p(z), ldz, self._s()),
                   "concat")
         dims = [F + a.extra] + list(a.hidden) + [a.out]
         self._zs = [(z, ldz)]
@@ -215,6 +220,23 @@ class Surrogate:
                            bias=self._view(self.w, f"fb{i}"), flags=BIAS)
         self._B = B
         return y.view(B, a.out)
+
+    def forward(self, raster: torch.Tensor, extra: torch.Tensor) -> torch.Tensor:
+        """raster (B, G, G, cin) bf16, extra (B, arch.extra) fp32 -> (B, out) fp32."""
+        B = int(raster.shape[0])
+        feat = self._cnn(raster)
+        return self._mlp(feat, self.arch.feat(), extra, B)
+
+    def predict_shared(self, raster1: torch.Tensor, extra: torch.Tensor, chunk: int = 16384) -> torch.Tensor:
+        """Many candidates that share one raster (1, G, G, cin): the CNN runs
+        once, the MLP per candidate; -> (S, out) fp32."""
+        S = int(extra.shape[0])
+        feat = self._cnn(raster1).clone()
+        out = torch.empty((S, self.arch.out), dtype=torch.float32, device=self.device)
+        for s0 in range(0, S, chunk):
+            e = min(S, s0 + chunk)
+            out[s0:e] = self._mlp(feat, 0, extra[s0:e], e - s0)
+        return out
 
     # ── backward ────────────────────────────────────────────────────────────
     def backward(self, dy: torch.Tensor, dyb: torch.Tensor):
