@@ -109,11 +109,14 @@ typedef struct {
     float M3inv[9], D3[9], Kp[9], Kd[9], Ki[9], D3pKd[9];
     float B_pinv[18], B[18], n_max[6];
     float T_n, vc_surge, vc_sway, vc_yaw;
-    /* uniform broad-phase grid (build choice; the contact set is the
-     * reference's exact one for any cell >= 2 (grid_rsmall + delta)):
-     * elements with radius <= grid_rsmall sit in one cell and scan 3x3;
-     * larger ones are inserted into, and scan, every cell within
-     * r + grid_rsmall + 2 delta.  Each cell holds up to grid_bucket
+    /* broad-phase grids (build choice; the contact set is the reference's
+     * exact one for any cell >= 2 (grid_rsmall + delta)): elements with
+     * radius <= grid_rsmall sit in the bucket of their fine cell; larger ones
+     * sit in the bucket of their cell of a coarse grid that the library sizes
+     * (cell >= 2 (2 r_max + delta)).  A small element scans the 3x3 fine
+     * cells around it, a large one the fine cells within
+     * r + grid_rsmall + delta; every element scans the 2x2 coarse cells on
+     * its side of its coarse cell.  A fine bucket holds up to grid_bucket
      * elements; overflow goes to a list that every element also scans. */
     float grid_x0, grid_y0, grid_cell, grid_rsmall;
     int32_t grid_nx, grid_ny, grid_bucket;

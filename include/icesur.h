@@ -70,6 +70,11 @@ int icesur_to_bf16(const float* d_x, int64_t n, uint16_t* d_out, void* stream);
 int icesur_adam(float* d_w, const float* d_g, float* d_m, float* d_v, int64_t n, float lr, float beta1,
                 float beta2, float eps, int32_t t, uint16_t* d_wb, void* stream);
 
+/* Adam with decoupled weight decay: w -= lr (m_hat / (sqrt(v_hat) + eps) +
+ * weight_decay w); otherwise as icesur_adam. */
+int icesur_adamw(float* d_w, const float* d_g, float* d_m, float* d_v, int64_t n, float lr, float beta1,
+                 float beta2, float eps, float weight_decay, int32_t t, uint16_t* d_wb, void* stream);
+
 /* Weighted mean squared error over B x D predictions (ld = D): d_loss[0] =
  * sum_{b,o} w_o (y - t)^2 / (B D) (rows summed in order), d_dy = its gradient
  * 2 w_o (y - t) / (B D) in fp32 and, if d_dyb, bf16. */
@@ -97,6 +102,14 @@ int icesur_split(const float* d_dz, int32_t lddz, int32_t F, int32_t B, float* d
  * (B G G 2 ints) must be zero on entry and is left zero. */
 int icesur_raster(const float* d_pos, const float* d_area, const float* d_thick, int32_t n, const float* d_pose,
                   int32_t B, int32_t G, float cell, float x_lo, int32_t* d_acc, uint16_t* d_out, void* stream);
+
+/* The same window from the elements' convex polygons: d_verts (n, max_v, 2)
+ * world vertices (the first d_nv[e] used, either orientation), each cell
+ * sampled at 4 x 4 points; coverage and thickness in the same fixed point
+ * as icesur_raster.  No reference counterpart. */
+int icesur_raster_poly(const float* d_verts, const int32_t* d_nv, const float* d_thick, int32_t n, int32_t max_v,
+                       const float* d_pose, int32_t B, int32_t G, float cell, float x_lo, int32_t* d_acc,
+                       uint16_t* d_out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -36,10 +36,10 @@ __device__ __forceinline__ int32_t* force_ticket(const DevView* V) { return V->c
 // ── broad phase: two uniform grids of fixed-capacity buckets ────────────────
 // Elements of radius <= grid_rsmall sit in the bucket of their fine cell
 // (cell >= 2 (grid_rsmall + delta)); larger ones sit in the bucket of their
-// coarse cell (cell2 >= 2 r_max + delta).  Every element scans the 3x3 coarse
-// cells around it for large partners; small partners are in the 3x3 fine
-// cells around a small element, or in the fine cells within
-// r + grid_rsmall + delta of a large one.  Each element is in exactly one
+// coarse cell (cell2 >= 2 (2 r_max + delta)).  Every element scans the 2x2
+// coarse cells on its side of its own coarse cell for large partners; small
+// partners are in the 3x3 fine cells around a small element, or in the fine
+// cells within r + grid_rsmall + delta of a large one.  Each element is in exactly one
 // bucket, so each partner is seen once.  A full bucket sends the element to
 // an overflow list that every element also scans.  Any of these gives
 // exactly the reference's contact set.  A slot holds a copy of the element's

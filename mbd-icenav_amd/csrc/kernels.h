@@ -1,18 +1,8 @@
-// kernels.h -- DPDEM device functions and kernels for gfx950 (wave64).
-//
-// Frame pipeline (one HIP stream, optionally replayed as a hipGraph):
-//   k_hull     per element: hull cull, wave-cooperative hull narrow phase for
-//              near elements, contact pre-pass, floe max/or atomics (exact:
-//              integer max/or), cell histogram, block partial sums
-//   k_mid      one workgroup: pairwise-tree totals -> thrust scale, floe
-//              eligibility, exclusive scan of the cell histogram
-//   k_scatter  per element: counting-sort scatter into cells
-//   k_force    per element (per sub-step): activation, submergence, bond
-//              forces (CSR, reference scatter order), contact forces (cells,
-//              partner-index order), drag, integration
-//   k_bond     per bond: fracture; per element: commit + histories;
-//              clears the per-frame histograms
-//   k_ship     one wave: ship reaction totals and the 3-DOF ship step
+// kernels.h -- DPDEM device functions for gfx950 (wave64) shared by the
+// frame kernels (frame_kernels.h) and the leaf entry points: fixed-order
+// wave/block reductions, the hull collision (passes 1 and 2 of
+// compute_all_collisions_combined), the contact pre-pass, bond, contact,
+// drag and integration terms, bond fracture and the ship step.
 // No floating-point atomics anywhere: every float sum has a fixed order.
 #pragma once
 #include "icedem_internal.h"

@@ -231,7 +231,7 @@ __global__ void k_to_bf16(const float* __restrict__ x, int64_t n, uint16_t* __re
 
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps, float c1, float c2,
-                       uint16_t* __restrict__ wb) {
+                       float wd, uint16_t* __restrict__ wb) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float gi = g[i];
@@ -239,7 +239,8 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
     const float vi = b2 * v[i] + (1.0f - b2) * (gi * gi);
     m[i] = mi; v[i] = vi;
     const float mh = mi / c1, vh = vi / c2;
-    const float wi = w[i] - lr * mh / (sqrtf(vh) + eps);
+    const float w0 = w[i];
+    const float wi = w0 - lr * (mh / (sqrtf(vh) + eps) + wd * w0);
     w[i] = wi;
     if (wb) wb[i] = f2bf(wi);
 }
@@ -318,6 +319,68 @@ __global__ void k_raster_splat(const float* __restrict__ pos, const float* __res
             int32_t* a = acc + (((size_t)b * G + i) * G + j) * 2;
             atomicAdd(a, ca);
             atomicAdd(a + 1, ch);
+        }
+    }
+}
+
+// Convex polygons (world vertices (n, maxV) float2, nv[e] used) sampled at
+// SS x SS points per cell of the ship-frame window; one thread per (pose,
+// element); integer atomics, so the raster does not depend on thread order.
+constexpr int RASTER_SS = 4;
+
+__global__ void k_raster_poly(const float2* __restrict__ verts, const int32_t* __restrict__ nv,
+                              const float* __restrict__ thick, int n, int maxV, const float* __restrict__ pose,
+                              int B, int G, float cell, float x_lo, int32_t* __restrict__ acc) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)B * n) return;
+    const int b = (int)(idx / n), e = (int)(idx % n);
+    const int m = nv[e];
+    if (m < 3) return;
+    const float px = pose[3 * b], py = pose[3 * b + 1], ph = pose[3 * b + 2];
+    const float c = cosf(ph), s = sinf(ph);
+    const float2* P = verts + (size_t)e * maxV;
+    float u_lo = INFINITY, u_hi = -INFINITY, v_lo = INFINITY, v_hi = -INFINITY, area2 = 0.0f;
+    float pu = 0.0f, pv = 0.0f;
+    for (int k = 0; k <= m; ++k) {
+        const float2 w = P[k % m];
+        const float dx = w.x - px, dy = w.y - py;
+        const float u = c * dx + s * dy, v = -s * dx + c * dy;
+        if (k > 0) area2 += pu * v - pv * u;
+        u_lo = fminf(u_lo, u); u_hi = fmaxf(u_hi, u); v_lo = fminf(v_lo, v); v_hi = fmaxf(v_hi, v);
+        pu = u; pv = v;
+    }
+    const float orient = area2 >= 0.0f ? 1.0f : -1.0f;
+    const float y_lo = -0.5f * (float)G * cell;
+    const int i0 = max(0, (int)floorf((u_lo - x_lo) / cell)), i1 = min(G - 1, (int)floorf((u_hi - x_lo) / cell));
+    const int j0 = max(0, (int)floorf((v_lo - y_lo) / cell)), j1 = min(G - 1, (int)floorf((v_hi - y_lo) / cell));
+    if (i0 > i1 || j0 > j1) return;
+    const int w1 = (int)(cell * cell * 4096.0f / (float)(RASTER_SS * RASTER_SS));
+    const int wh = (int)(cell * cell * thick[e] * 4096.0f / (float)(RASTER_SS * RASTER_SS));
+    const float h = cell / (float)RASTER_SS;
+    for (int i = i0; i <= i1; ++i) {
+        for (int j = j0; j <= j1; ++j) {
+            int cnt = 0;
+            for (int a = 0; a < RASTER_SS; ++a) {
+                const float qu = x_lo + (float)i * cell + ((float)a + 0.5f) * h;
+                for (int q = 0; q < RASTER_SS; ++q) {
+                    const float qv = y_lo + (float)j * cell + ((float)q + 0.5f) * h;
+                    bool in = true;
+                    float au = 0.0f, av = 0.0f;
+                    for (int k = 0; k <= m && in; ++k) {
+                        const float2 w = P[k % m];
+                        const float dx = w.x - px, dy = w.y - py;
+                        const float u = c * dx + s * dy, v = -s * dx + c * dy;
+                        if (k > 0) in = orient * ((u - au) * (qv - av) - (v - av) * (qu - au)) >= 0.0f;
+                        au = u; av = v;
+                    }
+                    cnt += in ? 1 : 0;
+                }
+            }
+            if (cnt) {
+                int32_t* d = acc + (((size_t)b * G + i) * G + j) * 2;
+                atomicAdd(d, cnt * w1);
+                atomicAdd(d + 1, cnt * wh);
+            }
         }
     }
 }
@@ -432,7 +495,18 @@ int icesur_adam(float* d_w, const float* d_g, float* d_m, float* d_v, int64_t n,
     if (n == 0) return ICEDEM_OK;
     const float c1 = 1.0f - powf(beta1, (float)t), c2 = 1.0f - powf(beta2, (float)t);
     hipLaunchKernelGGL(k_adam, dim3(grid1((size_t)n)), dim3(256), 0, (hipStream_t)stream, d_w, d_g, d_m, d_v, n, lr,
-                       beta1, beta2, eps, c1, c2, d_wb);
+                       beta1, beta2, eps, c1, c2, 0.0f, d_wb);
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+int icesur_adamw(float* d_w, const float* d_g, float* d_m, float* d_v, int64_t n, float lr, float beta1,
+                 float beta2, float eps, float weight_decay, int32_t t, uint16_t* d_wb, void* stream) {
+    SUR_CHECK(n >= 0 && t >= 1 && weight_decay >= 0.0f && d_w && d_g && d_m && d_v, "icesur_adamw: bad arguments");
+    if (n == 0) return ICEDEM_OK;
+    const float c1 = 1.0f - powf(beta1, (float)t), c2 = 1.0f - powf(beta2, (float)t);
+    hipLaunchKernelGGL(k_adam, dim3(grid1((size_t)n)), dim3(256), 0, (hipStream_t)stream, d_w, d_g, d_m, d_v, n, lr,
+                       beta1, beta2, eps, c1, c2, weight_decay, d_wb);
     SUR_LAUNCHED();
     return ICEDEM_OK;
 }
@@ -472,6 +546,21 @@ int icesur_raster(const float* d_pos, const float* d_area, const float* d_thick,
     if ((size_t)B * n > 0)
         hipLaunchKernelGGL(k_raster_splat, dim3(grid1((size_t)B * n)), dim3(256), 0, s, d_pos, d_area, d_thick, n,
                            d_pose, B, G, cell, x_lo, d_acc);
+    if ((size_t)B * G * G > 0)
+        hipLaunchKernelGGL(k_raster_out, dim3(grid1((size_t)B * G * G)), dim3(256), 0, s, d_acc, B, G, cell, d_out);
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+int icesur_raster_poly(const float* d_verts, const int32_t* d_nv, const float* d_thick, int32_t n, int32_t max_v,
+                       const float* d_pose, int32_t B, int32_t G, float cell, float x_lo, int32_t* d_acc,
+                       uint16_t* d_out, void* stream) {
+    SUR_CHECK(n >= 0 && max_v >= 3 && B >= 0 && G > 0 && G <= 4096 && cell > 0.0f && d_pose && d_acc && d_out &&
+              (n == 0 || (d_verts && d_nv && d_thick)), "icesur_raster_poly: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    if ((size_t)B * n > 0)
+        hipLaunchKernelGGL(k_raster_poly, dim3(grid1((size_t)B * n)), dim3(256), 0, s, (const float2*)d_verts, d_nv,
+                           d_thick, n, max_v, d_pose, B, G, cell, x_lo, d_acc);
     if ((size_t)B * G * G > 0)
         hipLaunchKernelGGL(k_raster_out, dim3(grid1((size_t)B * G * G)), dim3(256), 0, s, d_acc, B, G, cell, d_out);
     SUR_LAUNCHED();

@@ -112,10 +112,13 @@ SIGNATURES = {
     "icesur_colsum": (C.c_int, [_p, _i, _i, _i, _p, _i, _p, _p]),
     "icesur_to_bf16": (C.c_int, [_p, C.c_int64, _p, _p]),
     "icesur_adam": (C.c_int, [_p, _p, _p, _p, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, _i, _p, _p]),
+    "icesur_adamw": (C.c_int, [_p, _p, _p, _p, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                               _i, _p, _p]),
     "icesur_mse": (C.c_int, [_p, _p, _p, _i, _i, _p, _p, _p, _p]),
     "icesur_concat": (C.c_int, [_p, _i, _i, _p, _i, _i, _i, _p, _i, _p]),
     "icesur_split": (C.c_int, [_p, _i, _i, _i, _p, _p]),
     "icesur_raster": (C.c_int, [_p, _p, _p, _i, _p, _i, _i, C.c_float, C.c_float, _p, _p, _p]),
+    "icesur_raster_poly": (C.c_int, [_p, _p, _p, _i, _i, _p, _i, _i, C.c_float, C.c_float, _p, _p, _p]),
     "icedem_sync": (C.c_int, [_p, _p]),
     "icedem_substep_forces": (C.c_int, [_p, C.POINTER(IceDemState), _p, _p, C.c_int64, C.c_int64, _p, _p, _p]),
     "icedem_contact_pairs": (C.c_int, [_p, _p, _p, _p, C.c_int64, C.POINTER(C.c_int64), _p]),

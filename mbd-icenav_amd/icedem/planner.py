@@ -30,12 +30,12 @@ import torch
 from . import _lib
 from .rollouts import J_REF, KE_REF, RolloutBatch, ShipSpec, nominal_controls, rollout_cost
 from .scenario import Scene
-from .surrogate import Arch, Surrogate, raster
+from .surrogate import Arch, Surrogate, raster, raster_poly
 
 PEAK_REF = 1.0e6     # N
 # raster: 64 x 64 cells of 0.5 m from 26 m ahead of the ship's centre (4 m
 # behind the bow of the 60 m hull), +-16 m to either side
-RASTER_G, RASTER_CELL, RASTER_X0 = 64, 0.5, 26.0
+RASTER_G, RASTER_CELL, RASTER_X0, RASTER_KIND = 64, 0.5, 26.0, "disc"
 
 
 @dataclass
@@ -61,28 +61,41 @@ def ship_features(ship: torch.Tensor, start_xy: torch.Tensor) -> torch.Tensor:
 
 
 class PatchRasters:
-    """Device copies of each patch's initial element centroids, areas and
-    thicknesses, for ship-frame rasters."""
+    """Device copies of each patch's initial element polygons (or centroids
+    and areas for ``kind="disc"``) and thicknesses, for ship-frame rasters."""
 
-    def __init__(self, rb: RolloutBatch):
+    def __init__(self, rb: RolloutBatch, kind: str = None, cell: float = None, x0: float = None):
         dev = rb.device
         self.lib = _lib.load()
+        self.kind = kind or RASTER_KIND
+        self.cell = RASTER_CELL if cell is None else cell
+        self.x0 = RASTER_X0 if x0 is None else x0
         self.items = []
         for r, eng in enumerate(rb.engines):
             sc = rb.scenes[rb.inst[r]]
-            c = np.asarray(sc.centroids, np.float32)
-            pos = torch.as_tensor(np.concatenate([c[:, 0], c[:, 1]]), device=dev)
-            area = torch.as_tensor(eng._host["areas"], device=dev)
             thick = torch.as_tensor(eng._host["thick"], device=dev)
-            self.items.append((pos, area, thick))
+            if self.kind == "poly":
+                mv = max(len(p) for p in sc.polygons)
+                verts = np.zeros((len(sc.polygons), mv, 2), np.float32)
+                for i, p in enumerate(sc.polygons):
+                    verts[i, :len(p)] = p
+                nv = np.array([len(p) for p in sc.polygons], np.int32)
+                self.items.append((torch.as_tensor(verts, device=dev), torch.as_tensor(nv, device=dev), thick))
+            else:
+                c = np.asarray(sc.centroids, np.float32)
+                pos = torch.as_tensor(np.concatenate([c[:, 0], c[:, 1]]), device=dev)
+                area = torch.as_tensor(eng._host["areas"], device=dev)
+                self.items.append((pos, area, thick))
         self.acc = torch.zeros(RASTER_G * RASTER_G * 2 * 64, dtype=torch.int32, device=dev)
 
     def of(self, r: int, poses: torch.Tensor) -> torch.Tensor:
-        pos, area, thick = self.items[r]
         need = poses.shape[0] * RASTER_G * RASTER_G * 2
         if self.acc.numel() < need:
             self.acc = torch.zeros(need, dtype=torch.int32, device=poses.device)
-        return raster(self.lib, pos, area, thick, poses, RASTER_G, RASTER_CELL, RASTER_X0, self.acc)
+        a, b, thick = self.items[r]
+        if self.kind == "poly":
+            return raster_poly(self.lib, a, b, thick, poses, RASTER_G, self.cell, self.x0, self.acc)
+        return raster(self.lib, a, b, thick, poses, RASTER_G, self.cell, self.x0, self.acc)
 
 
 def random_starts(rb: RolloutBatch, nominal: torch.Tensor, rng: np.random.Generator, spread: StartSpread):
@@ -168,10 +181,23 @@ class Norm:
         return y * s + m
 
 
+def mirror_signs(n_knots: int = 40, device=None) -> torch.Tensor:
+    """Signs of the extra inputs under the mirror image about the ship's
+    centre line: sway, yaw rate, heading offset, lateral offset and the yaw
+    commands change sign; surge, along-track offset and thrust do not."""
+    s = torch.ones(6 + 2 * n_knots, dtype=torch.float32, device=device)
+    s[1:5] = -1.0
+    s[7::2] = -1.0
+    return s
+
+
 def train(net: Surrogate, data: Dict[str, torch.Tensor], epochs: int = 10, batch: int = 512, lr: float = 1e-3,
-          holdout: float = 0.05, seed: int = 0, progress=None):
+          holdout: float = 0.05, seed: int = 0, weight_decay: float = 0.0, mirror: bool = False, progress=None):
     """Adam on standardised targets; returns (norm, history) with the
-    held-out relative cost error after every epoch."""
+    held-out relative cost error after every epoch.  ``mirror`` replaces a
+    random half of every batch by its mirror image (the hull, the contact
+    model and the drag are symmetric about the ship's centre line, so the
+    mirrored input has the same outcome distribution)."""
     N = data["raster"].shape[0]
     rng = np.random.default_rng(seed)
     perm = rng.permutation(N)
@@ -180,14 +206,23 @@ def train(net: Surrogate, data: Dict[str, torch.Tensor], epochs: int = 10, batch
     t_np = data["target"].cpu().numpy()
     norm = Norm(t_np[tr].mean(0), t_np[tr].std(0) + 1e-6)
     tgt = norm.fwd(data["target"])
+    dev = data["raster"].device
+    signs = mirror_signs((data["extra"].shape[1] - 6) // 2, dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
     hist = []
     steps = max(1, len(tr) // batch)
     for ep in range(epochs):
-        order = torch.as_tensor(rng.permutation(tr), device=data["raster"].device)
+        order = torch.as_tensor(rng.permutation(tr), device=dev)
         lr_ep = lr * 0.5 * (1.0 + math.cos(math.pi * ep / max(1, epochs)))
         for s in range(steps):
             idx = order[s * batch:(s + 1) * batch]
-            loss = net.train_step(data["raster"][idx], data["extra"][idx], tgt[idx], lr=lr_ep)
+            ras, ext = data["raster"][idx], data["extra"][idx]
+            if mirror:
+                flip = torch.rand(idx.shape[0], device=dev, generator=gen) < 0.5
+                ras = torch.where(flip[:, None, None, None], ras.flip(2), ras)
+                ext = torch.where(flip[:, None], ext * signs, ext)
+            loss = net.train_step(ras, ext, tgt[idx], lr=lr_ep, weight_decay=weight_decay)
         err = relative_cost_error(net, norm, data, val)
         hist.append({"epoch": ep + 1, "loss": float(loss.item()), "val_rel_cost_err": err})
         if progress:
@@ -215,6 +250,8 @@ class Planner:
         self.net, self.norm = net, norm
         self.k, self.frames, self.hold, self.nk = k, frames, hold, n_knots
         self.thr, self.max_judged = err_threshold, max_judged
+        self.cell_size_factor = cell_size_factor
+        self._ex = None
         self.judge = RolloutBatch([scene], [0] * k, cell_size_factor=cell_size_factor)
         self.judge.set_mode("frames")
         self.nominal_start = self.judge.ship_init[0].clone()
@@ -289,5 +326,27 @@ class Planner:
         b = int(np.argmin(c))
         return {"best_cost": float(c[b]), "best_index": int(pick[b])}
 
+    def exhaustive(self, cand: np.ndarray, batch: int = 256) -> np.ndarray:
+        """DEM costs of every candidate (S, n_knots, 2), ``batch`` instances
+        of the patch per launch: the plain batched-DEM planner's scores."""
+        if self._ex is None or self._ex.R != batch:
+            self._ex = RolloutBatch([self.judge.scenes[0]], [0] * batch, cell_size_factor=self.cell_size_factor)
+            self._ex.set_mode("frames")
+            self._ex.ship_init.copy_(self.start.expand(batch, 15))
+        S = cand.shape[0]
+        out = np.empty(S, np.float64)
+        ctl = torch.as_tensor(cand, device=self.judge.device)
+        for s0 in range(0, S, batch):
+            m = min(batch, S - s0)
+            full = torch.empty((batch, self.nk, 2), dtype=torch.float32, device=ctl.device)
+            full[:m] = ctl[s0:s0 + m]
+            full[m:] = ctl[s0:s0 + 1]
+            self._ex.reset()
+            out[s0:s0 + m] = rollout_cost(self._ex.run(full, self.hold, self.frames).cpu().numpy()[:m])
+        return out
+
     def close(self):
         self.judge.close()
+        if self._ex is not None:
+            self._ex.close()
+            self._ex = None

@@ -296,8 +296,9 @@ class Surrogate:
 
     # ── training ────────────────────────────────────────────────────────────
     def train_step(self, raster, extra, target, lr: float = 1e-3, beta1: float = 0.9, beta2: float = 0.999,
-                   eps: float = 1e-8) -> torch.Tensor:
-        """One Adam step on the weighted MSE; returns the loss (device tensor)."""
+                   eps: float = 1e-8, weight_decay: float = 0.0) -> torch.Tensor:
+        """One Adam step (decoupled weight decay when ``weight_decay`` > 0) on
+        the weighted MSE; returns the loss (device tensor)."""
         y = self.forward(raster, extra)
         B = y.shape[0]
         dy = self._buf("dy", B * self.arch.out, torch.float32)
@@ -306,8 +307,8 @@ class Surrogate:
                                       _p(dyb), _p(self.loss), self._s()), "mse")
         self.backward(dy, dyb)
         self.t += 1
-        self._chk(self.lib.icesur_adam(_p(self.w), _p(self.g), _p(self.m), _p(self.v), self.nparam, lr, beta1, beta2,
-                                       eps, self.t, _p(self.wb), self._s()), "adam")
+        self._chk(self.lib.icesur_adamw(_p(self.w), _p(self.g), _p(self.m), _p(self.v), self.nparam, lr, beta1, beta2,
+                                        eps, weight_decay, self.t, _p(self.wb), self._s()), "adamw")
         return self.loss
 
     def predict(self, raster, extra, chunk: int = 4096) -> torch.Tensor:
@@ -334,4 +335,23 @@ def raster(lib, pos: torch.Tensor, area: torch.Tensor, thick: torch.Tensor, pose
     s = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
     _lib.check(lib.icesur_raster(_p(pos), _p(area), _p(thick), n, _p(poses.contiguous()), B, G, float(cell),
                                  float(x_lo), _p(acc), _p(out), s), None, "icesur_raster")
+    return out
+
+
+def raster_poly(lib, verts: torch.Tensor, nv: torch.Tensor, thick: torch.Tensor, poses: torch.Tensor, G: int = 64,
+                cell: float = 0.25, x_lo: float = 26.0, acc: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """(B, G, G, 2) bf16 rasters of one scene's convex polygons (verts (n,
+    max_v, 2) world vertices fp32, nv (n,) int32 vertex counts), 4 x 4 samples
+    per cell, seen from B ship poses (B, 3); the window starts x_lo ahead of
+    the ship's centre and spans +-G cell / 2 to either side."""
+    B = int(poses.shape[0])
+    n, max_v = int(verts.shape[0]), int(verts.shape[1])
+    dev = poses.device
+    if acc is None or acc.numel() < B * G * G * 2:
+        acc = torch.zeros(B * G * G * 2, dtype=torch.int32, device=dev)
+    out = torch.empty((B, G, G, 2), dtype=torch.bfloat16, device=dev)
+    s = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    _lib.check(lib.icesur_raster_poly(_p(verts.contiguous()), _p(nv.contiguous()), _p(thick), n, max_v,
+                                      _p(poses.contiguous()), B, G, float(cell), float(x_lo), _p(acc), _p(out), s),
+               None, "icesur_raster_poly")
     return out

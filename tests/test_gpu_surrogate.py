@@ -132,6 +132,12 @@ def test_mse_and_adam():
         ww = ww - 1e-2 * (m / (1 - 0.9 ** step)) / (np.sqrt(v / (1 - 0.999 ** step)) + 1e-8)
     np.testing.assert_allclose(dw.cpu().numpy(), ww, rtol=1e-5, atol=1e-6)
     np.testing.assert_array_equal(wb.float().cpu().numpy(), osur.bf16(dw.cpu().numpy()))
+    # decoupled weight decay: one step from zero moments with g = 0 shrinks w by (1 - lr wd)
+    dw2 = torch.as_tensor(wt.copy(), device="cuda")
+    dm.zero_(); dv.zero_()
+    assert lib.icesur_adamw(_p(dw2), _p(torch.zeros(n, device="cuda")), _p(dm), _p(dv), n, 1e-2, 0.9, 0.999, 1e-8,
+                            0.5, 1, _p(wb), _s()) == 0
+    np.testing.assert_allclose(dw2.cpu().numpy(), wt.astype(np.float64) * (1 - 1e-2 * 0.5), rtol=1e-6)
 
 
 def test_raster_of_known_discs():
@@ -154,6 +160,53 @@ def test_raster_of_known_discs():
     cu, cv = np.meshgrid(u, u - 8.0, indexing="ij")
     inside = ((cu - 5) ** 2 + cv ** 2 <= 1.0) | ((cu - 10) ** 2 + (cv - 3) ** 2 <= 4.0)
     np.testing.assert_array_equal(conc > 0, inside)
+
+
+def _poly_counts(poly, G, cell, x_lo, ss=4):
+    """Per cell, the number of the ss x ss sample points strictly inside the
+    convex polygon (lo) and inside or on its boundary (hi), float64."""
+    P = np.asarray(poly, np.float64)
+    a2 = sum(P[k - 1, 0] * P[k, 1] - P[k, 0] * P[k - 1, 1] for k in range(len(P)))
+    o = 1.0 if a2 >= 0 else -1.0
+    h = cell / ss
+    off = (np.arange(ss) + 0.5) * h
+    qu = (x_lo + np.arange(G)[:, None] * cell + off[None, :]).reshape(-1)
+    qv = (-0.5 * G * cell + np.arange(G)[:, None] * cell + off[None, :]).reshape(-1)
+    U, V = np.meshgrid(qu, qv, indexing="ij")
+    cr = np.stack([o * ((P[k, 0] - P[k - 1, 0]) * (V - P[k - 1, 1]) - (P[k, 1] - P[k - 1, 1]) * (U - P[k - 1, 0]))
+                   for k in range(len(P))])
+    lo = np.all(cr > 1e-9, axis=0).reshape(G, ss, G, ss).sum(axis=(1, 3))
+    hi = np.all(cr > -1e-9, axis=0).reshape(G, ss, G, ss).sum(axis=(1, 3))
+    return lo, hi
+
+
+def test_raster_of_known_polygons():
+    from icedem import surrogate as sur
+    L, lib = _lib()
+    # a 2 x 2 m square at (5, 0) (clockwise) and a right triangle of area 2
+    # (counter-clockwise, padded to 4 vertices), ship at the origin heading +x
+    sq = [[4.0, -1.0], [4.0, 1.0], [6.0, 1.0], [6.0, -1.0]]
+    tri = [[10.0, 2.0], [12.0, 2.0], [10.0, 4.0], [0.0, 0.0]]
+    verts = torch.tensor([sq, tri], dtype=torch.float32, device="cuda")
+    nv = torch.tensor([4, 3], dtype=torch.int32, device="cuda")
+    thick = torch.tensor([1.0, 2.0], dtype=torch.float32, device="cuda")
+    poses = torch.tensor([[0.0, 0.0, 0.0]], dtype=torch.float32, device="cuda")
+    r = sur.raster_poly(lib, verts, nv, thick, poses, G=16, cell=1.0, x_lo=0.0).float().cpu().numpy()[0]
+    conc, th = r[..., 0], r[..., 1]
+    # the square covers cells i = 4, 5 and j = 7, 8 exactly (v from -8)
+    assert np.all(conc[4:6, 7:9] == 1.0) and np.all(th[4:6, 7:9] == 1.0)
+    # every cell's coverage = its share of the 16 sample points inside (brute force)
+    lo1, hi1 = _poly_counts(sq, 16, 1.0, 0.0)
+    lo2, hi2 = _poly_counts(tri[:3], 16, 1.0, 0.0)
+    got = np.rint(conc * 16).astype(int)
+    assert np.all(got >= np.minimum(lo1 + lo2, 16)) and np.all(got <= np.minimum(hi1 + hi2, 16))
+    assert np.all(th[(hi2 > 0) & (hi1 == 0) & (got > 0)] == 2.0)
+    assert conc[0, 0] == 0.0 and th[0, 0] == 0.0
+    # rotating the world by +90 degrees and the ship with it reproduces the raster
+    rot = torch.tensor([[0.0, -1.0], [1.0, 0.0]], device="cuda")
+    pose2 = torch.tensor([[0.0, 0.0, np.pi / 2]], dtype=torch.float32, device="cuda")
+    r2 = sur.raster_poly(lib, verts @ rot.T, nv, thick, pose2, G=16, cell=1.0, x_lo=0.0).float().cpu().numpy()[0]
+    assert np.abs(r2[..., 0] - conc).max() <= 4 / 16 and abs(r2[..., 0].sum() - conc.sum()) <= 8 / 16
 
 
 def test_network_forward_backward_matches_oracle():
