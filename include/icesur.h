@@ -33,12 +33,23 @@ extern "C" {
  * The epilogue adds bias, applies ReLU and accumulates per `flags`, then
  * writes fp32 d_c (ldc) and/or bf16 d_cb (ldcb); either may be NULL.
  * split_k > 1 partitions K over split_k workgroup sets whose fp32 partial
- * products go to d_work (split_k * M * N floats) and are added in split
- * order before the epilogue. */
+ * products go to d_work and are added in split order before the epilogue;
+ * with more than 32 splits, in split order within groups of 32 and then the
+ * group sums in order.  d_work holds split_k * M * N floats, plus
+ * ceil(split_k / 32) * M * N when split_k > 32. */
 int icesur_gemm(int32_t trans_a, int32_t trans_b, int32_t M, int32_t N, int32_t K,
                 const uint16_t* d_a, int32_t lda, const uint16_t* d_b, int32_t ldb,
                 float* d_c, int32_t ldc, uint16_t* d_cb, int32_t ldcb, const float* d_bias, int32_t flags,
                 int32_t split_k, float* d_work, void* stream);
+
+/* 3x3 stride-2 padding-1 convolution of NHWC bf16 d_x (B, H, W, C) with
+ * weights d_w (9 C, N) bf16 (row k = tap C + c, tap = 3 kh + kw), as an
+ * implicit GEMM (no im2col matrix): d_out (B, Ho, Wo, N) bf16 =
+ * [relu](conv + bias), Ho = (H + 1) / 2.  flags: ICESUR_BIAS, ICESUR_RELU.
+ * d_wt: caller workspace of 9 C N bf16 (the transposed weights).  Equals
+ * icesur_im2col followed by icesur_gemm with the same flags. */
+int icesur_conv(const uint16_t* d_x, int32_t B, int32_t H, int32_t W, int32_t C, const uint16_t* d_w, int32_t N,
+                const float* d_bias, int32_t flags, uint16_t* d_wt, uint16_t* d_out, void* stream);
 
 /* 3x3, stride 2, padding 1 patches of an NHWC bf16 tensor (B, H, W, C):
  * d_col row b*Ho*Wo + ho*Wo + wo holds (kh*3 + kw)*C + c, Ho = (H+1)/2,
@@ -56,8 +67,10 @@ int icesur_col2im(const float* d_dcol, int32_t B, int32_t H, int32_t W, int32_t 
 int icesur_relu_back(float* d_g, int32_t ldg, const uint16_t* d_y, int32_t ldy, int32_t rows, int32_t cols,
                      uint16_t* d_gb, int32_t ldgb, void* stream);
 
-/* d_out[c] (+)= sum over rows of d_x[r * ldx + c], rows in blocks of 256,
- * block sums added in block order; d_work holds ceil(rows/256) * cols floats.
+/* d_out[c] (+)= sum over rows of d_x[r * ldx + c] in a fixed order: rows in
+ * blocks of 256, each block as 4 interleaved running sums (rows p, p + 4, ...)
+ * added as (s0 + s1) + (s2 + s3); the block sums as 16 interleaved running
+ * sums added as a pairwise tree.  d_work holds ceil(rows/256) * cols floats.
  * accumulate != 0 adds to d_out. */
 int icesur_colsum(const float* d_x, int32_t ldx, int32_t rows, int32_t cols, float* d_out, int32_t accumulate,
                   float* d_work, void* stream);

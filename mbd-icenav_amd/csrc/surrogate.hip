@@ -54,8 +54,17 @@ __device__ __forceinline__ void load_a(const uint16_t* __restrict__ A, int lda, 
         const int k = t >> 3, mq = (t & 7) * 8;
         const int gk = k0 + k, gm = m0 + mq;
         const uint16_t* p = A + (size_t)gk * lda + gm;
+        uint16_t v[8];
+        if (gk < kend && gm + 7 < M && ((uintptr_t)p & 15) == 0) {
+            const uint4 q = *reinterpret_cast<const uint4*>(p);
+            v[0] = q.x & 0xffff; v[1] = q.x >> 16; v[2] = q.y & 0xffff; v[3] = q.y >> 16;
+            v[4] = q.z & 0xffff; v[5] = q.z >> 16; v[6] = q.w & 0xffff; v[7] = q.w >> 16;
+        } else {
+            #pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = (gk < kend && gm + j < M) ? p[j] : (uint16_t)0;
+        }
         #pragma unroll
-        for (int j = 0; j < 8; ++j) As[mq + j][k] = (gk < kend && gm + j < M) ? p[j] : (uint16_t)0;
+        for (int j = 0; j < 8; ++j) As[mq + j][k] = v[j];
     }
 }
 
@@ -68,8 +77,17 @@ __device__ __forceinline__ void load_b(const uint16_t* __restrict__ B, int ldb, 
         const int k = t >> 3, nq = (t & 7) * 8;
         const int gk = k0 + k, gn = n0 + nq;
         const uint16_t* p = B + (size_t)gk * ldb + gn;
+        uint16_t v[8];
+        if (gk < kend && gn + 7 < N && ((uintptr_t)p & 15) == 0) {
+            const uint4 q = *reinterpret_cast<const uint4*>(p);
+            v[0] = q.x & 0xffff; v[1] = q.x >> 16; v[2] = q.y & 0xffff; v[3] = q.y >> 16;
+            v[4] = q.z & 0xffff; v[5] = q.z >> 16; v[6] = q.w & 0xffff; v[7] = q.w >> 16;
+        } else {
+            #pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = (gk < kend && gn + j < N) ? p[j] : (uint16_t)0;
+        }
         #pragma unroll
-        for (int j = 0; j < 8; ++j) Bs[nq + j][k] = (gk < kend && gn + j < N) ? p[j] : (uint16_t)0;
+        for (int j = 0; j < 8; ++j) Bs[nq + j][k] = v[j];
     } else {     // B is N x K: thread loads 8 consecutive k of one row
         const int n = t >> 2, kq = (t & 3) * 8;
         const int gn = n0 + n, gk = k0 + kq;
@@ -150,6 +168,18 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_
             }
 }
 
+// groups of 32 split partials summed in split order (the first level when
+// there are more than 32 splits)
+__global__ void k_splitk_group(int M, int N, int splits, const float* __restrict__ part, float* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t MN = (size_t)M * N;
+    if (i >= MN) return;
+    const int g = blockIdx.y, z0 = 32 * g, z1 = min(splits, z0 + 32);
+    float x = part[(size_t)z0 * MN + i];
+    for (int z = z0 + 1; z < z1; ++z) x = x + part[(size_t)z * MN + i];
+    out[(size_t)g * MN + i] = x;
+}
+
 // sum of the split partials in split order, then the epilogue
 __global__ void k_splitk_reduce(int M, int N, int splits, const float* __restrict__ part, Epi E) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -157,6 +187,199 @@ __global__ void k_splitk_reduce(int M, int N, int splits, const float* __restric
     float x = part[i];
     for (int z = 1; z < splits; ++z) x = x + part[(size_t)z * M * N + i];
     epilogue_store(E, (int)(i / N), (int)(i % N), x);
+}
+
+// ── implicit-GEMM convolution ───────────────────────────────────────────────
+// 3x3, stride 2, padding 1, NHWC: out[m][n] = epi(sum_k A[m][k] Wt[n][k]) with
+// m = (b, ho, wo) and k = tap C + c; the A tile is gathered from X while it is
+// staged into LDS, so no im2col matrix is written.  128 x TN tile per
+// workgroup of 4 waves (each 32 rows x TN as 2 x TN/16 MFMA tiles), K in
+// steps of 32, two LDS buffers: the next step's global loads are in flight
+// while the current step's MFMAs run.  Gather modes: CONV_VEC (C a multiple
+// of 32: a 32-wide k step lies inside one tap, 16-byte chunks), CONV_PAIR
+// (C = 2: one dword per tap, 9 taps = 18 k in one step), CONV_GEN (element
+// by element).  Wt is the weight matrix transposed, (N, K) row-major.  The
+// output tile goes through LDS so that global stores are 16-byte rows.
+constexpr int CM = 128, CK = 32, CKP = CK + 8;
+constexpr int CONV_GEN = 0, CONV_VEC = 1, CONV_PAIR = 2;
+
+template <int TN, int MODE>
+__global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ X, int B, int H, int W, int C, int Ho,
+                                              int Wo, const uint16_t* __restrict__ Wt, int N,
+                                              const float* __restrict__ bias, uint16_t* __restrict__ out, int flags) {
+    constexpr int A_BYTES = 2 * CM * CKP * 2, B_BYTES = 2 * TN * CKP * 2;
+    constexpr int O_LD = TN + 8;
+    constexpr int O_BYTES = CM * O_LD * 2;
+    constexpr int L_BYTES = (A_BYTES + B_BYTES) > O_BYTES ? (A_BYTES + B_BYTES) : O_BYTES;
+    __shared__ __align__(16) uint8_t lds[L_BYTES];
+    uint16_t (*As)[CM][CKP] = reinterpret_cast<uint16_t (*)[CM][CKP]>(lds);
+    uint16_t (*Bs)[TN][CKP] = reinterpret_cast<uint16_t (*)[TN][CKP]>(lds + A_BYTES);
+    uint16_t (*Os)[O_LD] = reinterpret_cast<uint16_t (*)[O_LD]>(lds);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int M = B * Ho * Wo, K = 9 * C;
+    const int m0 = blockIdx.x * CM, n0 = blockIdx.y * TN;
+    // this thread's rows: VEC t/4 and t/4 + 64 (8-channel chunk t % 4); PAIR t/2 (taps 0-4 or 5-8)
+    int rb[2] = {0, 0}, rh[2] = {0, 0}, rw[2] = {0, 0};
+    bool rv[2] = {false, false};
+    const int q = t & 3;
+    #pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int m = MODE == CONV_VEC ? m0 + (t >> 2) + 64 * r : m0 + (t >> 1);
+        if (MODE == CONV_GEN || (MODE == CONV_PAIR && r > 0)) continue;
+        rv[r] = m < M;
+        const int mm = rv[r] ? m : 0;
+        rb[r] = mm / (Ho * Wo);
+        const int rem = mm - rb[r] * Ho * Wo;
+        rh[r] = 2 * (rem / Wo) - 1;
+        rw[r] = 2 * (rem % Wo) - 1;
+    }
+    const int p0 = (t & 1) ? 5 : 0, pn = (t & 1) ? 4 : 5;   // PAIR: first tap, tap count
+    constexpr int BCH = TN * CK / 8;                        // 16-byte chunks of the B tile
+    uint4 pa[2], pb[(BCH + 255) / 256];
+    uint32_t pp[5];
+    uint16_t ga[16];
+    auto fetch = [&](int k0) {
+        if (MODE == CONV_VEC) {
+            const int tap = k0 / C, c0 = k0 - tap * C + 8 * q;
+            const int kh = tap / 3, kw = tap - 3 * (tap / 3);
+            #pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int h = rh[r] + kh, ww = rw[r] + kw;
+                if (rv[r] && h >= 0 && h < H && ww >= 0 && ww < W)
+                    pa[r] = *reinterpret_cast<const uint4*>(X + (((size_t)rb[r] * H + h) * W + ww) * C + c0);
+                else
+                    pa[r] = make_uint4(0u, 0u, 0u, 0u);
+            }
+        } else if (MODE == CONV_PAIR) {
+            #pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                const int tap = p0 + i;
+                const int h = rh[0] + tap / 3, ww = rw[0] + tap % 3;
+                pp[i] = (i < pn && rv[0] && h >= 0 && h < H && ww >= 0 && ww < W)
+                            ? *reinterpret_cast<const uint32_t*>(X + (((size_t)rb[0] * H + h) * W + ww) * 2)
+                            : 0u;
+            }
+        } else {
+            #pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int e = t + 256 * i, row = e >> 5, k = k0 + (e & 31);
+                const int m = m0 + row;
+                uint16_t v = 0;
+                if (m < M && k < K) {
+                    const int b = m / (Ho * Wo), rem = m - b * Ho * Wo;
+                    const int tap = k / C, c = k - tap * C;
+                    const int h = 2 * (rem / Wo) - 1 + tap / 3, ww = 2 * (rem % Wo) - 1 + tap % 3;
+                    if (h >= 0 && h < H && ww >= 0 && ww < W) v = X[(((size_t)b * H + h) * W + ww) * C + c];
+                }
+                ga[i] = v;
+            }
+        }
+        #pragma unroll
+        for (int i = 0; i < (BCH + 255) / 256; ++i) {
+            const int ch = t + 256 * i;
+            if (ch < BCH) {
+                const int n = ch >> 2, kq = k0 + 8 * (ch & 3);
+                if (MODE == CONV_VEC) {
+                    pb[i] = (n0 + n < N) ? *reinterpret_cast<const uint4*>(Wt + (size_t)(n0 + n) * K + kq)
+                                         : make_uint4(0u, 0u, 0u, 0u);
+                } else {
+                    uint16_t v[8];
+                    #pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        v[j] = (n0 + n < N && kq + j < K) ? Wt[(size_t)(n0 + n) * K + kq + j] : (uint16_t)0;
+                    pb[i] = make_uint4(v[0] | ((uint32_t)v[1] << 16), v[2] | ((uint32_t)v[3] << 16),
+                                       v[4] | ((uint32_t)v[5] << 16), v[6] | ((uint32_t)v[7] << 16));
+                }
+            }
+        }
+    };
+    auto stage = [&](int buf) {
+        if (MODE == CONV_VEC) {
+            #pragma unroll
+            for (int r = 0; r < 2; ++r) *reinterpret_cast<uint4*>(&As[buf][(t >> 2) + 64 * r][8 * q]) = pa[r];
+        } else if (MODE == CONV_PAIR) {
+            uint16_t* row = As[buf][t >> 1];
+            #pragma unroll
+            for (int i = 0; i < 5; ++i)
+                if (i < pn) *reinterpret_cast<uint32_t*>(row + 2 * (p0 + i)) = pp[i];
+            if (t & 1) {                                      // k = 18 .. 31 are zero
+                #pragma unroll
+                for (int c = 18; c < 32; c += 2) *reinterpret_cast<uint32_t*>(row + c) = 0u;
+            }
+        } else {
+            #pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int e = t + 256 * i;
+                As[buf][e >> 5][e & 31] = ga[i];
+            }
+        }
+        #pragma unroll
+        for (int i = 0; i < (BCH + 255) / 256; ++i) {
+            const int ch = t + 256 * i;
+            if (ch < BCH) *reinterpret_cast<uint4*>(&Bs[buf][ch >> 2][8 * (ch & 3)]) = pb[i];
+        }
+    };
+    f32x4 acc[2][TN / 16];
+    #pragma unroll
+    for (int i = 0; i < 2; ++i)
+        #pragma unroll
+        for (int j = 0; j < TN / 16; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    fetch(0);
+    stage(0);
+    __syncthreads();
+    int cur = 0;
+    for (int k0 = 0; k0 < K; k0 += CK) {
+        const bool more = k0 + CK < K;
+        if (more) fetch(k0 + CK);
+        bf16x8 a[2];
+        #pragma unroll
+        for (int i = 0; i < 2; ++i)
+            a[i] = *reinterpret_cast<const bf16x8*>(&As[cur][w * 32 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+        #pragma unroll
+        for (int j = 0; j < TN / 16; ++j) {
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[cur][j * 16 + (lane & 15)][8 * (lane >> 4)]);
+            #pragma unroll
+            for (int i = 0; i < 2; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b, acc[i][j], 0, 0, 0);
+        }
+        if (more) stage(cur ^ 1);
+        __syncthreads();
+        cur ^= 1;
+    }
+    // epilogue into LDS (the loop's last barrier retired every read of As / Bs)
+    #pragma unroll
+    for (int j = 0; j < TN / 16; ++j) {
+        const int cl = j * 16 + (lane & 15);
+        const float bc = (n0 + cl < N && (flags & ICESUR_BIAS)) ? bias[n0 + cl] : 0.0f;
+        #pragma unroll
+        for (int i = 0; i < 2; ++i)
+            #pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                float x = acc[i][j][v] + bc;
+                if (flags & ICESUR_RELU) x = fmaxf(x, 0.0f);
+                Os[w * 32 + i * 16 + 4 * (lane >> 4) + v][cl] = f2bf(x);
+            }
+    }
+    __syncthreads();
+    const bool vo = (N & 7) == 0 && (((uintptr_t)out) & 15) == 0;
+    constexpr int RCH = TN / 8;                        // 16-byte chunks per tile row
+    for (int ch = t; ch < CM * RCH; ch += 256) {
+        const int r = ch / RCH, c8 = 8 * (ch % RCH);
+        const int row = m0 + r, col = n0 + c8;
+        if (row >= M || col >= N) continue;
+        if (vo) {
+            *reinterpret_cast<uint4*>(out + (size_t)row * N + col) = *reinterpret_cast<const uint4*>(&Os[r][c8]);
+        } else {
+            for (int j = 0; j < 8 && col + j < N; ++j) out[(size_t)row * N + col + j] = Os[r][c8 + j];
+        }
+    }
+}
+
+// (K, N) -> (N, K) bf16
+__global__ void k_transpose16(const uint16_t* __restrict__ a, int K, int N, uint16_t* __restrict__ at) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)K * N) return;
+    const int n = (int)(i / K), k = (int)(i % K);
+    at[i] = a[(size_t)k * N + n];
 }
 
 __global__ void k_im2col(const uint16_t* __restrict__ X, int B, int H, int W, int C, int Ho, int Wo,
@@ -205,23 +428,44 @@ __global__ void k_relu_back(float* __restrict__ g, int ldg, const uint16_t* __re
     if (gb) gb[(size_t)r * ldgb + c] = f2bf(v);
 }
 
-// block b of 256 rows: work[b * cols + c] = sum of its rows in order
-__global__ void k_colsum_part(const float* __restrict__ x, int ldx, int rows, int cols, float* __restrict__ work) {
-    const int c = blockIdx.y * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
+// block (b, cb) of 256 threads = 64 columns x 4 row phases: phase p sums rows
+// 256 b + p, + 4, + 8, ... in order; work[b * cols + c] = (s0 + s1) + (s2 + s3)
+__global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ x, int ldx, int rows, int cols,
+                                                     float* __restrict__ work) {
+    __shared__ float sp[4][64];
+    const int cl = threadIdx.x & 63, p = threadIdx.x >> 6;
+    const int c = blockIdx.y * 64 + cl;
     const int r0 = blockIdx.x * 256, r1 = min(rows, r0 + 256);
     float s = 0.0f;
-    for (int r = r0; r < r1; ++r) s = s + x[(size_t)r * ldx + c];
-    work[(size_t)blockIdx.x * cols + c] = s;
+    if (c < cols)
+        for (int r = r0 + p; r < r1; r += 4) s = s + x[(size_t)r * ldx + c];
+    sp[p][cl] = s;
+    __syncthreads();
+    if (p == 0 && c < cols) work[(size_t)blockIdx.x * cols + c] = (sp[0][cl] + sp[1][cl]) + (sp[2][cl] + sp[3][cl]);
 }
 
-__global__ void k_colsum_final(const float* __restrict__ work, int nblk, int cols, float* __restrict__ out,
-                               int accumulate) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
+// block cb of 256 threads = 16 columns x 16 phases: phase g sums the block sums
+// g, g + 16, ... in order; the 16 phase sums are added as a pairwise tree
+__global__ __launch_bounds__(256) void k_colsum_final(const float* __restrict__ work, int nblk, int cols,
+                                                      float* __restrict__ out, int accumulate) {
+    __shared__ float sp[16][16];
+    const int cl = threadIdx.x & 15, g = threadIdx.x >> 4;
+    const int c = blockIdx.x * 16 + cl;
     float s = 0.0f;
-    for (int b = 0; b < nblk; ++b) s = s + work[(size_t)b * cols + c];
-    out[c] = accumulate ? out[c] + s : s;
+    if (c < cols)
+        for (int b = g; b < nblk; b += 16) s = s + work[(size_t)b * cols + c];
+    sp[g][cl] = s;
+    __syncthreads();
+    if (g == 0 && c < cols) {
+        float v[16];
+        #pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = sp[i][cl];
+        #pragma unroll
+        for (int w = 1; w < 16; w *= 2)
+            #pragma unroll
+            for (int i = 0; i + w < 16; i += 2 * w) v[i] = v[i] + v[i + w];
+        out[c] = accumulate ? out[c] + v[0] : v[0];
+    }
 }
 
 __global__ void k_to_bf16(const float* __restrict__ x, int64_t n, uint16_t* __restrict__ out) {
@@ -434,7 +678,40 @@ int icesur_gemm(int32_t trans_a, int32_t trans_b, int32_t M, int32_t N, int32_t 
     else if (!trans_a && trans_b) GEMM_LAUNCH(false, true);
     else GEMM_LAUNCH(true, true);
 #undef GEMM_LAUNCH
-    if (part) hipLaunchKernelGGL(k_splitk_reduce, dim3(grid1((size_t)M * N)), dim3(256), 0, s, M, N, splits, d_work, E);
+    if (part && splits > 32) {
+        const int ng = (splits + 31) / 32;
+        float* part2 = d_work + (size_t)splits * M * N;
+        hipLaunchKernelGGL(k_splitk_group, dim3(grid1((size_t)M * N), ng), dim3(256), 0, s, M, N, splits, d_work, part2);
+        hipLaunchKernelGGL(k_splitk_reduce, dim3(grid1((size_t)M * N)), dim3(256), 0, s, M, N, ng, part2, E);
+    } else if (part) {
+        hipLaunchKernelGGL(k_splitk_reduce, dim3(grid1((size_t)M * N)), dim3(256), 0, s, M, N, splits, d_work, E);
+    }
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+int icesur_conv(const uint16_t* d_x, int32_t B, int32_t H, int32_t W, int32_t C, const uint16_t* d_w, int32_t N,
+                const float* d_bias, int32_t flags, uint16_t* d_wt, uint16_t* d_out, void* stream) {
+    SUR_CHECK(B >= 0 && H > 0 && W > 0 && C > 0 && N > 0 && d_x && d_w && d_wt && d_out, "icesur_conv: bad arguments");
+    SUR_CHECK(!(flags & ICESUR_BIAS) || d_bias, "icesur_conv: bias missing");
+    SUR_CHECK(!(flags & ICESUR_ACCUM), "icesur_conv: accumulate is not supported");
+    const int Ho = (H + 1) / 2, Wo = (W + 1) / 2, K = 9 * C;
+    const size_t M = (size_t)B * Ho * Wo;
+    SUR_CHECK(M < (size_t)1 << 31, "icesur_conv: too many output pixels");
+    if (M == 0) return ICEDEM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_transpose16, dim3(grid1((size_t)K * N)), dim3(256), 0, s, d_w, K, N, d_wt);
+    const int mode = ((C % 32 == 0) && ((uintptr_t)d_x % 16 == 0) && ((uintptr_t)d_wt % 16 == 0)) ? CONV_VEC
+                     : (C == 2 && ((uintptr_t)d_x % 4 == 0)) ? CONV_PAIR : CONV_GEN;
+    const int tn = (N % 128 == 0) ? 128 : (N % 64 == 0) ? 64 : 32;
+    dim3 grid((unsigned)((M + CM - 1) / CM), (unsigned)((N + tn - 1) / tn));
+#define CONV_LAUNCH(TN_, M_) hipLaunchKernelGGL((k_conv<TN_, M_>), grid, dim3(256), 0, s, d_x, B, H, W, C, Ho, Wo, d_wt, N, d_bias, d_out, flags)
+#define CONV_MODES(TN_) do { if (mode == CONV_VEC) CONV_LAUNCH(TN_, CONV_VEC); else if (mode == CONV_PAIR) CONV_LAUNCH(TN_, CONV_PAIR); else CONV_LAUNCH(TN_, CONV_GEN); } while (0)
+    if (tn == 128) CONV_MODES(128);
+    else if (tn == 64) CONV_MODES(64);
+    else CONV_MODES(32);
+#undef CONV_MODES
+#undef CONV_LAUNCH
     SUR_LAUNCHED();
     return ICEDEM_OK;
 }
@@ -475,8 +752,8 @@ int icesur_colsum(const float* d_x, int32_t ldx, int32_t rows, int32_t cols, flo
     hipStream_t s = (hipStream_t)stream;
     const int nblk = (rows + 255) / 256;
     if (nblk > 0)
-        hipLaunchKernelGGL(k_colsum_part, dim3(nblk, (cols + 63) / 64), dim3(64), 0, s, d_x, ldx, rows, cols, d_work);
-    hipLaunchKernelGGL(k_colsum_final, dim3(grid1(cols, 64)), dim3(64), 0, s, d_work, nblk, cols, d_out, accumulate);
+        hipLaunchKernelGGL(k_colsum_part, dim3(nblk, (cols + 63) / 64), dim3(256), 0, s, d_x, ldx, rows, cols, d_work);
+    hipLaunchKernelGGL(k_colsum_final, dim3((cols + 15) / 16), dim3(256), 0, s, d_work, nblk, cols, d_out, accumulate);
     SUR_LAUNCHED();
     return ICEDEM_OK;
 }

@@ -106,6 +106,7 @@ class Surrogate:
         self.load(params if params is not None else self.init_params(seed))
         self._bufs: Dict[Tuple[str, int], torch.Tensor] = {}
         self.out_w = torch.ones(self.arch.out, dtype=torch.float32, device=dev)
+        self.implicit_conv = True     # False: im2col + GEMM (the same sums, for tests)
         self.loss = torch.zeros(1, dtype=torch.float32, device=dev)
 
     # ── parameters ──────────────────────────────────────────────────────────
@@ -159,14 +160,17 @@ class Surrogate:
               split=1):
         work = None
         if split > 1:
-            work = self._buf("splitk", split * M * N, torch.float32)
+            extra = ((split + 31) // 32) * M * N if split > 32 else 0
+            work = self._buf("splitk", split * M * N + extra, torch.float32)
         self._chk(self.lib.icesur_gemm(int(ta), int(tb), M, N, K, _p(A), lda, _p(B), ldb, _p(Cf), ldc, _p(Cb), ldcb,
                                        _p(bias), flags, split, _p(work), self._s()), "icesur_gemm")
 
     @staticmethod
-    def _split_for(K: int) -> int:
-        """Split-K factor for a long reduction with few output tiles."""
-        return int(min(64, max(1, K // 4096)))
+    def _split_for(M: int, N: int, K: int) -> int:
+        """Split-K factor for a long reduction with few output tiles: about
+        1024 workgroups, at least 512 of K per split."""
+        tiles = ((M + 63) // 64) * ((N + 63) // 64)
+        return int(max(1, min((1024 + tiles - 1) // tiles, K // 512, 1024)))
 
     # ── forward ─────────────────────────────────────────────────────────────
     def _cnn(self, raster: torch.Tensor) -> torch.Tensor:
@@ -177,16 +181,22 @@ class Surrogate:
         x = raster.contiguous()
         shapes, _, _ = a.conv_shapes()
         self._acts: List[torch.Tensor] = []
-        self._cols: List[torch.Tensor] = []
+        self._ins: List[torch.Tensor] = []
         for i, (s, c, co) in enumerate(shapes):
             so = (s + 1) // 2
             rows = B * so * so
-            col = self._buf(f"col{i}", rows * 9 * c, torch.bfloat16)
-            self._chk(self.lib.icesur_im2col(_p(x), B, s, s, c, _p(col), self._s()), "im2col")
             act = self._buf(f"act{i}", rows * co, torch.bfloat16)
-            self._gemm(0, 0, rows, co, 9 * c, col, 9 * c, self._view(self.wb, f"cw{i}"), co, Cb=act, ldcb=co,
-                       bias=self._view(self.w, f"cb{i}"), flags=BIAS | RELU)
-            self._cols.append(col)
+            if self.implicit_conv:
+                wt = self._buf(f"wt{i}", 9 * c * co, torch.bfloat16)
+                self._chk(self.lib.icesur_conv(_p(x), B, s, s, c, _p(self._view(self.wb, f"cw{i}")), co,
+                                               _p(self._view(self.w, f"cb{i}")), BIAS | RELU, _p(wt), _p(act),
+                                               self._s()), "conv")
+            else:
+                col = self._buf("col", rows * 9 * c, torch.bfloat16)
+                self._chk(self.lib.icesur_im2col(_p(x), B, s, s, c, _p(col), self._s()), "im2col")
+                self._gemm(0, 0, rows, co, 9 * c, col, 9 * c, self._view(self.wb, f"cw{i}"), co, Cb=act, ldcb=co,
+                           bias=self._view(self.w, f"cb{i}"), flags=BIAS | RELU)
+            self._ins.append(x)
             self._acts.append(act)
             x = act
         return x
@@ -253,7 +263,7 @@ class Surrogate:
             zin, ldin = self._zs[i]
             K, N = dims[i], dims[i + 1]
             self._gemm(1, 0, K, N, B, zin, ldin, db, N, Cf=self._view(self.g, f"fw{i}"), ldc=N,
-                       split=self._split_for(B))
+                       split=self._split_for(K, N, B))
             self._chk(self.lib.icesur_colsum(_p(d), N, B, N, _p(self._view(self.g, f"fb{i}")), 0,
                                              _p(self._buf("csum", ((B + 255) // 256) * N, torch.float32)),
                                              self._s()), "colsum")
@@ -279,8 +289,10 @@ class Surrogate:
             s, c, co = shapes[i]
             so = (s + 1) // 2
             rows = B * so * so
-            self._gemm(1, 0, 9 * c, co, rows, self._cols[i], 9 * c, db, co, Cf=self._view(self.g, f"cw{i}"), ldc=co,
-                       split=self._split_for(rows))
+            col = self._buf("col", rows * 9 * c, torch.bfloat16)
+            self._chk(self.lib.icesur_im2col(_p(self._ins[i]), B, s, s, c, _p(col), self._s()), "im2col")
+            self._gemm(1, 0, 9 * c, co, rows, col, 9 * c, db, co, Cf=self._view(self.g, f"cw{i}"), ldc=co,
+                       split=self._split_for(9 * c, co, rows))
             self._chk(self.lib.icesur_colsum(_p(d), co, rows, co, _p(self._view(self.g, f"cb{i}")), 0,
                                              _p(self._buf("csum", ((rows + 255) // 256) * co, torch.float32)),
                                              self._s()), "colsum")

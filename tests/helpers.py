@@ -260,3 +260,32 @@ def compare_frame(case: Case, st, ship, k: int, report: Optional[Report] = None)
     r.add_float("ship_n", gs.n_actual, oship2.n_actual, abs_floor=1e-6)
     r.add_float("F_ice_on_ship", diag["F_ice_on_ship"].cpu().numpy(), od["F_ice_on_ship"], abs_floor=1.0)
     return r
+
+
+def oracle_run(case: "Case", frames: int, dtype, checkpoints=()):
+    """Oracle trajectory of `case` (setpoints in the dtype: float64 rows come
+    straight from the float64 trajectory); returns the final (state, ship)
+    and {frame: (state, ship)} at the checkpoints (frames completed)."""
+    o = case.oracle(dtype)
+    st, ship = case.oracle_initial(dtype)
+    vw = np.zeros((case.n, 2), dtype)
+    snaps = {}
+    for k in range(frames):
+        i, j = min(k, len(case.traj) - 1), min(k + 1, len(case.traj) - 1)
+        sp = np.asarray(case.traj[i], dtype)
+        spn = np.asarray(case.traj[j], dtype)
+        st, ship, _ = o.frame_step(st, ship, sp, spn, vw, st.dpdem_floe_eligible)
+        if k + 1 in checkpoints:
+            snaps[k + 1] = (st, ship)
+    return st, ship, snaps
+
+
+def drift_stats(a_state, a_ship, b_state, b_ship) -> Dict[str, float]:
+    """Differences of a float32 run (a) from a float64 run (b)."""
+    dp = np.linalg.norm(np.asarray(a_state.positions, np.float64) - np.asarray(b_state.positions, np.float64), axis=1)
+    dv = np.linalg.norm(np.asarray(a_state.velocities, np.float64) - np.asarray(b_state.velocities, np.float64), axis=1)
+    de = np.abs(np.asarray(a_ship.eta, np.float64) - np.asarray(b_ship.eta, np.float64))
+    return {"pos_max_m": float(dp.max()), "pos_median_m": float(np.median(dp)),
+            "vel_max_mps": float(dv.max()), "ship_pos_m": float(np.hypot(de[0], de[1])), "ship_heading_rad": float(de[2]),
+            "bond_flags_differing": int(np.sum(np.asarray(a_state.bond_intact) != np.asarray(b_state.bond_intact))),
+            "active_flags_differing": int(np.sum(np.asarray(a_state.elem_active) != np.asarray(b_state.elem_active)))}

@@ -29,7 +29,8 @@ def _s():
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
-@pytest.mark.parametrize("M,N,K,split", [(16, 16, 32, 1), (70, 45, 100, 1), (128, 64, 2134, 1), (37, 29, 5000, 4)])
+@pytest.mark.parametrize("M,N,K,split", [(16, 16, 32, 1), (70, 45, 100, 1), (128, 64, 2134, 1), (37, 29, 5000, 4),
+                                         (18, 32, 40000, 70)])
 def test_gemm_matches_float64_of_bf16_inputs(ta, tb, M, N, K, split):
     L, lib = _lib()
     rng = np.random.default_rng(M * 7 + N * 3 + K + ta * 2 + tb)
@@ -43,7 +44,7 @@ def test_gemm_matches_float64_of_bf16_inputs(ta, tb, M, N, K, split):
     dA, dB = _bf(A), _bf(B)
     Cg = torch.full((M, N), 7.0, dtype=torch.float32, device="cuda")
     Cb = torch.zeros((M, N), dtype=torch.bfloat16, device="cuda")
-    work = torch.zeros(split * M * N, dtype=torch.float32, device="cuda")
+    work = torch.zeros((split + (split + 31) // 32) * M * N, dtype=torch.float32, device="cuda")
     db = torch.as_tensor(bias, device="cuda")
     rc = lib.icesur_gemm(ta, tb, M, N, K, _p(dA), M if ta else K, _p(dB), K if tb else N, _p(Cg), N, _p(Cb), N,
                          _p(db), 4, split, _p(work), _s())
@@ -87,19 +88,67 @@ def test_im2col_col2im_relu_colsum():
     mref = np.where(y.float().cpu().numpy() > 0, d, 0.0).astype(np.float32)
     np.testing.assert_array_equal(g.cpu().numpy(), mref)
     np.testing.assert_array_equal(gb.float().cpu().numpy(), osur.bf16(mref))
-    # column sums: 256-row blocks summed in order, then the blocks in order (float32)
-    R, Cn = 700, 13
-    xs = rng.normal(size=(R, Cn)).astype(np.float32)
-    out = torch.zeros(Cn, dtype=torch.float32, device="cuda")
-    work = torch.zeros(3 * Cn, dtype=torch.float32, device="cuda")
-    assert lib.icesur_colsum(_p(torch.as_tensor(xs, device="cuda")), Cn, R, Cn, _p(out), 0, _p(work), _s()) == 0
-    ref = np.zeros(Cn, np.float32)
-    for b0 in range(0, R, 256):
-        part = np.zeros(Cn, np.float32)
-        for r in range(b0, min(R, b0 + 256)):
-            part = (part + xs[r]).astype(np.float32)
-        ref = (ref + part).astype(np.float32)
-    np.testing.assert_array_equal(out.cpu().numpy(), ref)
+    # column sums in the documented float32 order; and within float32 rounding of the float64 sum
+    for R, Cn in [(700, 13), (5000, 70)]:
+        xs = rng.normal(size=(R, Cn)).astype(np.float32)
+        out = torch.zeros(Cn, dtype=torch.float32, device="cuda")
+        nb = (R + 255) // 256
+        work = torch.zeros(nb * Cn, dtype=torch.float32, device="cuda")
+        assert lib.icesur_colsum(_p(torch.as_tensor(xs, device="cuda")), Cn, R, Cn, _p(out), 0, _p(work), _s()) == 0
+        parts = []
+        for b0 in range(0, R, 256):
+            ph = [np.zeros(Cn, np.float32) for _ in range(4)]
+            for p in range(4):
+                for r in range(b0 + p, min(R, b0 + 256), 4):
+                    ph[p] = (ph[p] + xs[r]).astype(np.float32)
+            parts.append(((ph[0] + ph[1]).astype(np.float32) + (ph[2] + ph[3]).astype(np.float32)).astype(np.float32))
+        v = []
+        for g in range(16):
+            acc = np.zeros(Cn, np.float32)
+            for b in range(g, nb, 16):
+                acc = (acc + parts[b]).astype(np.float32)
+            v.append(acc)
+        w = 1
+        while w < 16:
+            for i in range(0, 16 - w, 2 * w):
+                v[i] = (v[i] + v[i + w]).astype(np.float32)
+            w *= 2
+        np.testing.assert_array_equal(out.cpu().numpy(), v[0])
+        np.testing.assert_allclose(out.cpu().numpy(), xs.astype(np.float64).sum(0), rtol=1e-4, atol=1e-3)
+
+
+@pytest.mark.parametrize("B,H,C,N", [(3, 16, 2, 32), (5, 7, 32, 64), (2, 9, 64, 128), (3, 4, 128, 128),
+                                     (130, 8, 32, 32), (1, 5, 3, 40)])
+def test_implicit_conv_equals_im2col_gemm(B, H, C, N):
+    """The implicit-GEMM convolution adds the same products in the same
+    order as im2col + GEMM: bit-identical, with and without ReLU; and both
+    match the float64 convolution of the bf16 inputs."""
+    L, lib = _lib()
+    rng = np.random.default_rng(B * 1000 + H * 10 + C)
+    x = rng.normal(size=(B, H, H, C)).astype(np.float32)
+    w = rng.normal(size=(9 * C, N)).astype(np.float32) / np.sqrt(9 * C)
+    bias = rng.normal(size=N).astype(np.float32)
+    dx, dw = _bf(x), _bf(w)
+    db = torch.as_tensor(bias, device="cuda")
+    Ho = (H + 1) // 2
+    M = B * Ho * Ho
+    wt = torch.empty(9 * C * N, dtype=torch.bfloat16, device="cuda")
+    col = torch.empty(M * 9 * C, dtype=torch.bfloat16, device="cuda")
+    assert lib.icesur_im2col(_p(dx), B, H, H, C, _p(col), _s()) == 0
+    ref_col = osur.im2col(osur.bf16(x).astype(np.float64))
+    ref = ref_col @ osur.bf16(w).astype(np.float64) + bias
+    for flags in (4, 5):     # BIAS, BIAS | RELU
+        out = torch.empty(M * N, dtype=torch.bfloat16, device="cuda")
+        assert lib.icesur_conv(_p(dx), B, H, H, C, _p(dw), N, _p(db), flags, _p(wt), _p(out), _s()) == 0
+        out2 = torch.empty(M * N, dtype=torch.bfloat16, device="cuda")
+        assert lib.icesur_gemm(0, 0, M, N, 9 * C, _p(col), 9 * C, _p(dw), N, None, 0, _p(out2), N, _p(db), flags, 1,
+                               None, _s()) == 0
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out.view(torch.int16).cpu().numpy(), out2.view(torch.int16).cpu().numpy())
+        r = np.maximum(ref, 0.0) if flags & 1 else ref
+        got = out.float().cpu().numpy().reshape(M, N)
+        assert np.all(np.abs(got - r) <= 2 ** -8 * np.abs(r) + 1e-5 * (1 + np.abs(r).max())), (flags, np.abs(got - r).max())
+
 
 
 def test_mse_and_adam():
@@ -234,6 +283,18 @@ def test_network_forward_backward_matches_oracle():
         e = np.linalg.norm(g[k] - go[k]) / max(np.linalg.norm(go[k]), 1e-12)
         print(f"grad {k}: rel err {e:.2e}", flush=True)
         assert e < 2e-2, k
+
+
+def test_network_implicit_conv_equals_im2col_path():
+    from icedem import surrogate as sur
+    net = sur.Surrogate(sur.Arch(), seed=3)
+    rng = np.random.default_rng(5)
+    raster = _bf(rng.uniform(0, 1, (33, 64, 64, 2)))
+    extra = torch.as_tensor(rng.normal(0, 1, (33, 86)).astype(np.float32), device="cuda")
+    y1 = net.forward(raster, extra).clone()
+    net.implicit_conv = False
+    y2 = net.forward(raster, extra).clone()
+    np.testing.assert_array_equal(y1.cpu().numpy(), y2.cpu().numpy())
 
 
 def test_training_reduces_loss():
