@@ -272,6 +272,7 @@ def surrogate_bench(plain_rps: float, rounds: int = 400, epochs: int = 20, test_
     floor = dem_sensitivity(rb)
     rb.close()
     N = int(data["raster"].shape[0])
+    cm_w, cm_err = pl.costmap_baseline(data)
     arch = Arch()
     net = Surrogate(arch, seed=0)
     t0 = time.perf_counter()
@@ -299,7 +300,7 @@ def surrogate_bench(plain_rps: float, rounds: int = 400, epochs: int = 20, test_
         tmp = ro.RolloutBatch([sc], [0])
         start = tmp.ship_init[0].clone()
         tmp.close()
-        P = pl.Planner(net, norm, sc, start, k=k)
+        P = pl.Planner(net, norm, sc, start, k=k, costmap_weights=cm_w)
         nominal = np.tile(np.array([0.7, 0.0], np.float32), (40, 1))
         P.iterate(nominal, S=samples, seed=100)           # graph capture and warm-up
         res = P.iterate(nominal, S=samples, seed=i)
@@ -315,14 +316,24 @@ def surrogate_bench(plain_rps: float, rounds: int = 400, epochs: int = 20, test_
             t0 = time.perf_counter()
             dem = P.exhaustive(cand)
             t_ex = time.perf_counter() - t0
-            pred = P.surrogate_costs(torch.as_tensor(cand, device="cuda")).cpu().numpy().astype(np.float64)
+            dc = torch.as_tensor(cand, device="cuda")
+            pred = P.surrogate_costs(dc).cpu().numpy().astype(np.float64)
             rel = np.abs(pred - dem) / np.maximum(np.abs(dem), pl.COST_FLOOR)
+            cmc = P.costmap_costs(dc).cpu().numpy()
+            rel_cm = np.abs(cmc - dem) / np.maximum(np.abs(dem), pl.COST_FLOOR)
             b = float(dem.min())
             row.update({"dem_best_of_all": b,
                         "regret_vs_dem_best": (res["best_cost"] - b) / max(abs(b), pl.COST_FLOOR),
                         "pick_rank_fraction": float(np.mean(dem < res["best_cost"])),
                         "surrogate_rel_err_median_all": float(np.median(rel)),
                         "spearman_surrogate_vs_dem": _spearman(pred, dem),
+                        "costmap_rel_err_median_all": float(np.median(rel_cm)),
+                        "spearman_costmap_vs_dem": _spearman(cmc, dem),
+                        "dem_mean_of_top64": {"surrogate": float(dem[np.argsort(pred)[:k]].mean()),
+                                              "costmap": float(dem[np.argsort(cmc)[:k]].mean()),
+                                              "all": float(dem.mean())},
+                        "best_of_top64": {"surrogate": float(dem[np.argsort(pred)[:k]].min()),
+                                          "costmap": float(dem[np.argsort(cmc)[:k]].min())},
                         "dem_cost_sd": float(dem.std()),
                         "exhaustive_batched_rollouts_per_s": samples / t_ex})
         rows.append(row)
@@ -336,6 +347,8 @@ def surrogate_bench(plain_rps: float, rounds: int = 400, epochs: int = 20, test_
         "rollouts_scored_per_s": rps, "plain_gpu_rollouts_per_s": plain_rps, "vs_plain": rps / plain_rps,
         "reported_cost_error": "0: the planner reports only DEM-judged costs",
         "surrogate_val_rel_cost_err": val,
+        "costmap_val_rel_cost_err": cm_err,
+        "costmap_weights": [float(x) for x in cm_w],
         "surrogate_err_target_5pct_met": bool(val["median"] <= 0.05),
         "dem_sensitivity_floor": floor,
         "judged_median_rel_err": float(np.median([r["median_rel_err_judged"] for r in rows])),
@@ -350,6 +363,7 @@ def surrogate_bench(plain_rps: float, rounds: int = 400, epochs: int = 20, test_
     if exhaustive:
         out["planner_regret_vs_dem_best_median"] = float(np.median([r["regret_vs_dem_best"] for r in rows]))
         out["spearman_surrogate_vs_dem_median"] = float(np.median([r["spearman_surrogate_vs_dem"] for r in rows]))
+        out["spearman_costmap_vs_dem_median"] = float(np.median([r["spearman_costmap_vs_dem"] for r in rows]))
     return out
 
 

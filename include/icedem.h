@@ -226,6 +226,21 @@ int icedem_batch_rollouts(IceDemBatch* batch, const IceDemState* states, const I
 int icedem_batch_destroy(IceDemBatch* batch);
 
 /* Waits for `stream` and reports errors flagged by kernels since the last call. */
+/* Ship-only rollouts scored on a costmap, the rollout of the reference's
+ * costmap planners (jax_mbd_planner.py:569-660 with the query of :128-168).
+ * S candidates from start states d_ship0 (15 floats each, ship0_stride
+ * floats apart; stride 0: one start for all), the context's ship
+ * model in ship_mode 1 with no ice forces, `frames` frames, controls
+ * d_controls (S, n_knots, 2) each knot held `hold` frames.  At the start and
+ * after every frame the costmap d_costmap (rows, cols), cell `res`, origin
+ * (x0, y0) is read bilinearly at the centre and the four hull corners (the
+ * maximum kept; oob_penalty added when the column is outside the map);
+ * d_out (S, 4) = [sum of 0.5 (c_t + c_t+1) |segment|, KE loss (J), path
+ * length, final heading].  d_costmap may be NULL (cost 0). */
+int icedem_ship_rollouts(IceDemCtx* ctx, const float* d_ship0, int32_t ship0_stride, const float* d_controls, int32_t S,
+                         int32_t n_knots, int32_t hold, int32_t frames, const float* d_costmap, int32_t rows,
+                         int32_t cols, float x0, float y0, float res, float oob_penalty, float* d_out, void* stream);
+
 int icedem_sync(IceDemCtx* ctx, void* stream);
 
 /* Forces and moments of one sub-step (jax_simulation.py:372-387) for the

@@ -116,6 +116,19 @@ int icesur_split(const float* d_dz, int32_t lddz, int32_t F, int32_t B, float* d
 int icesur_raster(const float* d_pos, const float* d_area, const float* d_thick, int32_t n, const float* d_pose,
                   int32_t B, int32_t G, float cell, float x_lo, int32_t* d_acc, uint16_t* d_out, void* stream);
 
+/* The KE collision costmap of the reference's planners (build_costmap_jax,
+ * jax_costmap.py:177-330, ice_resistance_weight = 0): d_out (rows, cols),
+ * row = y, origin (x0, y0), cell res, = max(base_cost, max over elements of
+ * ke_i max(0, (r_e^2 - d^2) / r_e^2)) with r_e = r_i (1 + buffer_factor) +
+ * res / 2 and ke_i = collision_cost v^2 m_i M (m_i + 2 M) / (2 (M + m_i)^2)
+ * ke_cost_scale clipped to [0, 1e8] (v = ship_speed, M = ship_mass), then the
+ * maximum over a (2 dilate_half + 1)^2 window with edge padding.  d_pos
+ * (2, n) centroids, d_work rows * cols ints. */
+int icesur_ke_costmap(const float* d_pos, const float* d_radius, const float* d_mass, int32_t n, int32_t rows,
+                      int32_t cols, float x0, float y0, float res, double collision_cost, double ship_speed,
+                      double ship_mass, double ke_cost_scale, float buffer_factor, float base_cost,
+                      int32_t dilate_half, int32_t* d_work, float* d_out, void* stream);
+
 /* The same window from the elements' convex polygons: d_verts (n, max_v, 2)
  * world vertices (the first d_nv[e] used, either orientation), each cell
  * sampled at 4 x 4 points; coverage and thickness in the same fixed point

@@ -246,6 +246,71 @@ __global__ void k_ship_leaf(const DevView* __restrict__ V, float* ship, const fl
     for (int i = 0; i < ICEDEM_SHIP_STATE_LEN; ++i) ship[i] = st[i];
 }
 
+// Ship-only rollouts scored on a costmap (the costmap planners' rollout,
+// jax_mbd_planner.py:569-660): ship_mode 1 dynamics with no ice forces; at
+// every waypoint (start and after each frame) the costmap is read bilinearly
+// (jax_mbd_planner.py:128-168) at the ship's centre and four hull corners and
+// the maximum kept; segments add 0.5 (c_t + c_t+1) |d| with |d| =
+// sqrt(dx^2 + dy^2 + 1e-8).  One thread per candidate.
+__device__ __forceinline__ float cmap_query(const float* __restrict__ cm, int rows, int cols, float x0, float y0,
+                                            float res, float oob, float x, float y) {
+    const float cf = (x - x0) / res - 0.5f, rf = (y - y0) / res - 0.5f;
+    const float r0f = floorf(rf), c0f = floorf(cf);
+    const int r0 = (int)fminf(fmaxf(r0f, -2.0f), (float)rows), c0 = (int)fminf(fmaxf(c0f, -2.0f), (float)cols);
+    const float fr = rf - r0f, fc = cf - c0f;
+    const int r0c = min(max(r0, 0), rows - 1), r1c = min(max(r0 + 1, 0), rows - 1);
+    const int c0c = min(max(c0, 0), cols - 1), c1c = min(max(c0 + 1, 0), cols - 1);
+    const float pen = (c0 < 0 || c0 + 1 >= cols) ? oob : 0.0f;
+    const float v = cm[(size_t)r0c * cols + c0c] * (1.0f - fr) * (1.0f - fc) + cm[(size_t)r0c * cols + c1c] * (1.0f - fr) * fc +
+                    cm[(size_t)r1c * cols + c0c] * fr * (1.0f - fc) + cm[(size_t)r1c * cols + c1c] * fr * fc;
+    return v + pen;
+}
+
+__device__ __forceinline__ float hull_cost(const float* __restrict__ cm, int rows, int cols, float x0, float y0,
+                                           float res, float oob, float hl, float hw, const float* st) {
+    float s, c;
+    sincosf(st[2], &s, &c);
+    const float bx[5] = {0.0f, hl, hl, -hl, -hl}, by[5] = {0.0f, hw, -hw, hw, -hw};
+    float m = -INFINITY;
+    #pragma unroll
+    for (int i = 0; i < 5; ++i)
+        m = fmaxf(m, cmap_query(cm, rows, cols, x0, y0, res, oob, st[0] + c * bx[i] - s * by[i],
+                                st[1] + s * bx[i] + c * by[i]));
+    return m;
+}
+
+__global__ void k_ship_rollouts(const DevView* __restrict__ V, const float* __restrict__ ship0, int ship0_stride,
+                                const float* __restrict__ ctl, int S, int nk, int hold, int frames,
+                                const float* __restrict__ cm, int rows, int cols, float x0, float y0, float res,
+                                float oob, float hl, float hw, float* __restrict__ out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= S) return;
+    const IceDemParams& P = V->p;
+    float st[ICEDEM_SHIP_STATE_LEN];
+    for (int i = 0; i < ICEDEM_SHIP_STATE_LEN; ++i) st[i] = ship0[(size_t)r * ship0_stride + i];
+    const float ke0 = 0.5f * P.ship_mass * (st[3] * st[3] + st[4] * st[4]) + 0.5f * P.ship_Izz * (st[5] * st[5]);
+    const float zero3[3] = {0.0f, 0.0f, 0.0f};
+    float wp = cm ? hull_cost(cm, rows, cols, x0, y0, res, oob, hl, hw, st) : 0.0f;
+    float cost = 0.0f, len = 0.0f;
+    const float* c = ctl + (size_t)r * nk * 2;
+    for (int k = 0; k < frames; ++k) {
+        const int knot = min(k / max(hold, 1), nk - 1);
+        const float px = st[0], py = st[1];
+        ship_step(P, st, zero3, zero3, zero3, 0.0f, 0.0f, c[2 * knot], c[2 * knot + 1]);
+        const float dx = st[0] - px, dy = st[1] - py;
+        const float seg = sqrtf(dx * dx + dy * dy + 1e-8f);
+        const float w2 = cm ? hull_cost(cm, rows, cols, x0, y0, res, oob, hl, hw, st) : 0.0f;
+        cost = cost + 0.5f * (wp + w2) * seg;
+        len = len + seg;
+        wp = w2;
+    }
+    const float ke = 0.5f * P.ship_mass * (st[3] * st[3] + st[4] * st[4]) + 0.5f * P.ship_Izz * (st[5] * st[5]);
+    out[4 * r] = cost;
+    out[4 * r + 1] = ke0 - ke;
+    out[4 * r + 2] = len;
+    out[4 * r + 3] = st[2];
+}
+
 // ═════════════════════════════════════════════════════════════════════════════
 // Host side
 // ═════════════════════════════════════════════════════════════════════════════
@@ -1000,6 +1065,25 @@ int icedem_component_com_velocities(int32_t n, const int32_t* d_labels, const fl
     HIP_TRY(hipFreeAsync(count, stream)); HIP_TRY(hipFreeAsync(start, stream)); HIP_TRY(hipFreeAsync(fill, stream));
     HIP_TRY(hipFreeAsync(order, stream)); HIP_TRY(hipFreeAsync(tm, stream)); HIP_TRY(hipFreeAsync(tx, stream));
     HIP_TRY(hipFreeAsync(ty, stream));
+    return ICEDEM_OK;
+}
+
+int icedem_ship_rollouts(IceDemCtx* ctx, const float* d_ship0, int32_t ship0_stride, const float* d_controls, int32_t S,
+                         int32_t n_knots, int32_t hold, int32_t frames, const float* d_costmap, int32_t rows,
+                         int32_t cols, float x0, float y0, float res, float oob_penalty, float* d_out, void* stream_) {
+    if (!ctx || !d_ship0 || !d_controls || !d_out) { set_err(ctx, "null argument"); return ICEDEM_E_INVALID; }
+    if (S < 0 || n_knots < 1 || frames < 0 || (ship0_stride != 0 && ship0_stride < ICEDEM_SHIP_STATE_LEN) ||
+        (d_costmap && (rows < 1 || cols < 1 || !(res > 0.0f)))) {
+        set_err(ctx, "icedem_ship_rollouts: bad sizes"); return ICEDEM_E_INVALID;
+    }
+    if (ctx->hv.p.ship_mode != 1) { set_err(ctx, "icedem_ship_rollouts: context is not in ship_mode 1"); return ICEDEM_E_INVALID; }
+    if (S == 0) return ICEDEM_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const float hl = ctx->hv.p.half_len, hw = ctx->hv.p.half_beam;
+    hipLaunchKernelGGL(k_ship_rollouts, dim3((S + 63) / 64), dim3(64), 0, (hipStream_t)stream_, ctx->dv, d_ship0,
+                       ship0_stride, d_controls, S, n_knots, hold, frames, d_costmap, rows, cols, x0, y0, res, oob_penalty, hl, hw,
+                       d_out);
+    HIP_TRY(hipGetLastError());
     return ICEDEM_OK;
 }
 

@@ -640,6 +640,59 @@ __global__ void k_raster_out(int32_t* __restrict__ acc, int B, int G, float cell
     acc[2 * idx + 1] = 0;
 }
 
+// ── KE costmap (jax_costmap.py:98-141, 177-330) ─────────────────────────────
+// Per element the AUTO-IceNav KE constant ke = w v^2 m M (m + 2 M) / (2 (M +
+// m)^2) s (binary64, clipped to [0, 1e8]); per cell the maximum over elements
+// of ke max(0, (r_e^2 - d^2) / r_e^2), r_e = r (1 + b) + res / 2, by integer
+// atomic max of the non-negative float bits (order-free).
+__global__ void k_ke_zero(int32_t* __restrict__ bits, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) bits[i] = 0;
+}
+
+__global__ void k_ke_splat(const float* __restrict__ pos, const float* __restrict__ radius,
+                           const float* __restrict__ mass, int n, int rows, int cols, float x0, float y0, float res,
+                           double w, double v, double Ms, double scale, float buf, int32_t* __restrict__ bits) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const double m = mass[e];
+    double ke = w * v * v * m * Ms * (m + 2.0 * Ms) / (2.0 * (Ms + m) * (Ms + m)) * scale;
+    ke = fmin(fmax(ke, 0.0), 1e8);
+    const float kef = (float)ke;
+    const float px = pos[e], py = pos[n + e];
+    const float er = radius[e] * (1.0f + buf) + 0.5f * res;
+    const float er2 = er * er;
+    const int c0 = max(0, (int)floorf((px - er - x0) / res)), c1 = min(cols - 1, (int)floorf((px + er - x0) / res));
+    const int r0 = max(0, (int)floorf((py - er - y0) / res)), r1 = min(rows - 1, (int)floorf((py + er - y0) / res));
+    for (int r = r0; r <= r1; ++r) {
+        const float cy = y0 + ((float)r + 0.5f) * res;
+        for (int c = c0; c <= c1; ++c) {
+            const float cx = x0 + ((float)c + 0.5f) * res;
+            const float d2 = (cx - px) * (cx - px) + (cy - py) * (cy - py);
+            const float val = kef * fmaxf(0.0f, (er2 - d2) / er2);
+            if (val > 0.0f) atomicMax(bits + (size_t)r * cols + c, __float_as_int(val));
+        }
+    }
+}
+
+// open-water floor, then the maximum over a (2 half + 1)^2 window with edge
+// padding (the reference's reduce_window over the full square window)
+__global__ void k_ke_dilate(const int32_t* __restrict__ bits, int rows, int cols, float base, int half,
+                            float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * cols) return;
+    const int r = i / cols, c = i % cols;
+    float m = base;
+    for (int dr = -half; dr <= half; ++dr) {
+        const int rr = min(max(r + dr, 0), rows - 1);
+        for (int dc = -half; dc <= half; ++dc) {
+            const int cc = min(max(c + dc, 0), cols - 1);
+            m = fmaxf(m, __int_as_float(bits[(size_t)rr * cols + cc]));
+        }
+    }
+    out[i] = m;
+}
+
 }  // namespace icesur
 
 using namespace icesur;
@@ -825,6 +878,25 @@ int icesur_raster(const float* d_pos, const float* d_area, const float* d_thick,
                            d_pose, B, G, cell, x_lo, d_acc);
     if ((size_t)B * G * G > 0)
         hipLaunchKernelGGL(k_raster_out, dim3(grid1((size_t)B * G * G)), dim3(256), 0, s, d_acc, B, G, cell, d_out);
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+int icesur_ke_costmap(const float* d_pos, const float* d_radius, const float* d_mass, int32_t n, int32_t rows,
+                      int32_t cols, float x0, float y0, float res, double collision_cost, double ship_speed,
+                      double ship_mass, double ke_cost_scale, float buffer_factor, float base_cost,
+                      int32_t dilate_half, int32_t* d_work, float* d_out, void* stream) {
+    SUR_CHECK(n >= 0 && rows > 0 && cols > 0 && res > 0.0f && dilate_half >= 0 && d_work && d_out &&
+              (n == 0 || (d_pos && d_radius && d_mass)), "icesur_ke_costmap: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const int cells = rows * cols;
+    hipLaunchKernelGGL(k_ke_zero, dim3(grid1(cells)), dim3(256), 0, s, d_work, cells);
+    if (n > 0)
+        hipLaunchKernelGGL(k_ke_splat, dim3(grid1(n)), dim3(256), 0, s, d_pos, d_radius, d_mass, n, rows, cols, x0,
+                           y0, res, collision_cost, fmax(ship_speed, 1e-6), ship_mass, ke_cost_scale, buffer_factor,
+                           d_work);
+    hipLaunchKernelGGL(k_ke_dilate, dim3(grid1(cells)), dim3(256), 0, s, d_work, rows, cols, base_cost, dilate_half,
+                       d_out);
     SUR_LAUNCHED();
     return ICEDEM_OK;
 }

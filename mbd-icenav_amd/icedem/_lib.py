@@ -112,6 +112,10 @@ SIGNATURES = {
     "icesur_colsum": (C.c_int, [_p, _i, _i, _i, _p, _i, _p, _p]),
     "icesur_to_bf16": (C.c_int, [_p, C.c_int64, _p, _p]),
     "icesur_adam": (C.c_int, [_p, _p, _p, _p, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, _i, _p, _p]),
+    "icesur_ke_costmap": (C.c_int, [_p, _p, _p, _i, _i, _i, C.c_float, C.c_float, C.c_float, C.c_double,
+                                    C.c_double, C.c_double, C.c_double, C.c_float, C.c_float, _i, _p, _p, _p]),
+    "icedem_ship_rollouts": (C.c_int, [_p, _p, _i, _p, _i, _i, _i, _i, _p, _i, _i, C.c_float, C.c_float,
+                                       C.c_float, C.c_float, _p, _p]),
     "icesur_conv": (C.c_int, [_p, _i, _i, _i, _i, _p, _i, _p, _i, _p, _p, _p]),
     "icesur_adamw": (C.c_int, [_p, _p, _p, _p, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                _i, _p, _p]),

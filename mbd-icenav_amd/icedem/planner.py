@@ -19,6 +19,7 @@ Rollout cost is ``rollouts.rollout_cost``: impulse / 1e5 N s + KE loss / 1e6 J.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 import time
 from dataclasses import dataclass, field
@@ -28,6 +29,8 @@ import numpy as np
 import torch
 
 from . import _lib
+from .engine import _ptr
+from .properties import ShipProperties
 from .rollouts import J_REF, KE_REF, RolloutBatch, ShipSpec, nominal_controls, rollout_cost
 from .scenario import Scene
 from .surrogate import Arch, Surrogate, raster, raster_poly
@@ -98,6 +101,103 @@ class PatchRasters:
         return raster(self.lib, a, b, thick, poses, RASTER_G, self.cell, self.x0, self.acc)
 
 
+@dataclass
+class CostmapSpec:
+    """build_costmap_jax's defaults (jax_costmap.py:177-195), with a grid of
+    0.5 m instead of 2 m (the rollouts cover a few metres)."""
+    resolution: float = 0.5
+    margin: float = 20.0
+    base_cost: float = 0.1
+    collision_cost: float = 5.0
+    obs_cost_buffer_factor: float = 0.1
+    ke_cost_scale: float = 1e-6
+    planning_clearance_margin: float = 0.0
+    oob_penalty: float = 0.0
+
+
+class KECostmap:
+    """The reference planners' KE collision costmap of one patch's initial
+    ice (``icesur_ke_costmap``), rows along y; element masses are the static
+    masses, the KE speed is the nominal start speed."""
+
+    def __init__(self, eng, scene: Scene, ship: ShipSpec, spec: Optional[CostmapSpec] = None, speed: float = None):
+        self.spec = spec = spec or CostmapSpec()
+        self.lib = _lib.load()
+        dev = eng.device
+        c = np.asarray(scene.centroids, np.float64)
+        radii = np.asarray(eng._host["radii"], np.float64)
+        lo, hi = c.min(0) - radii.max(), c.max(0) + radii.max()
+        res = spec.resolution
+        self.x0, self.y0, self.res = float(lo[0] - spec.margin), float(lo[1] - spec.margin), float(res)
+        self.cols = max(1, int(math.ceil((hi[0] - lo[0] + 2 * spec.margin) / res)))
+        self.rows = max(1, int(math.ceil((hi[1] - lo[1] + 2 * spec.margin) / res)))
+        pos = torch.as_tensor(np.concatenate([c[:, 0], c[:, 1]]).astype(np.float32), device=dev)
+        rad = torch.as_tensor(eng._host["radii"], device=dev)
+        mass = torch.as_tensor(eng._host["mass"], device=dev)
+        half = int(math.ceil((0.5 * ship.beam + spec.planning_clearance_margin) / res))
+        work = torch.empty(self.rows * self.cols, dtype=torch.int32, device=dev)
+        self.grid = torch.empty(self.rows * self.cols, dtype=torch.float32, device=dev)
+        sp = ShipProperties(ship.length, ship.beam, ship.draft)
+        s = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        _lib.check(self.lib.icesur_ke_costmap(_ptr(pos), _ptr(rad), _ptr(mass), len(c), self.rows, self.cols, self.x0,
+                                              self.y0, res, spec.collision_cost,
+                                              float(ship.speed0 if speed is None else speed), float(sp.mass),
+                                              spec.ke_cost_scale, spec.obs_cost_buffer_factor, spec.base_cost, half,
+                                              _ptr(work), _ptr(self.grid), s), None, "icesur_ke_costmap")
+        self._keep = (pos, rad, mass, work)
+
+
+def ship_rollouts(eng, starts: torch.Tensor, controls: torch.Tensor, hold: int, frames: int,
+                  costmap: Optional[KECostmap] = None) -> torch.Tensor:
+    """Ship-only rollouts (no ice) of (S, n_knots, 2) controls from one start
+    (15,) or one start each (S, 15), scored on the costmap; -> (S, 4)
+    [costmap integral, KE loss J, path length, final heading]."""
+    lib = _lib.load()
+    ctl = torch.as_tensor(controls).to(device=eng.device, dtype=torch.float32).contiguous()
+    st = starts.to(device=eng.device, dtype=torch.float32).contiguous()
+    S, nk = int(ctl.shape[0]), int(ctl.shape[1])
+    stride = 0 if st.dim() == 1 else 15
+    out = torch.empty((S, 4), dtype=torch.float32, device=eng.device)
+    cm = costmap
+    s = C.c_void_p(torch.cuda.current_stream(eng.device).cuda_stream)
+    _lib.check(lib.icedem_ship_rollouts(eng.ctx, _ptr(st), stride, _ptr(ctl), S, nk, int(hold), int(frames),
+                                        _ptr(cm.grid) if cm else None, cm.rows if cm else 0, cm.cols if cm else 0,
+                                        cm.x0 if cm else 0.0, cm.y0 if cm else 0.0, cm.res if cm else 1.0,
+                                        cm.spec.oob_penalty if cm else 0.0, _ptr(out), s), eng.ctx,
+               "icedem_ship_rollouts")
+    return out
+
+
+N_PHYS = 3   # extra inputs from the ice-free ship rollout
+
+
+def phys_features(out: torch.Tensor) -> torch.Tensor:
+    """(S, 4) ship-rollout results -> (S, 3) network inputs: ice-free KE
+    loss / KE_REF, costmap integral / 10, path length / 5."""
+    f = torch.empty((out.shape[0], N_PHYS), dtype=torch.float32, device=out.device)
+    f[:, 0] = out[:, 1] / KE_REF
+    f[:, 1] = out[:, 0] / 10.0
+    f[:, 2] = out[:, 2] / 5.0
+    return f
+
+
+def costmap_features(out: torch.Tensor) -> torch.Tensor:
+    """(S, 4) ship-rollout results -> (S, 3) [costmap integral, ice-free KE
+    loss / KE_REF, 1] for the calibrated costmap cost."""
+    f = torch.ones((out.shape[0], 3), dtype=torch.float64, device=out.device)
+    f[:, 0] = out[:, 0].double()
+    f[:, 1] = out[:, 1].double() / KE_REF
+    return f
+
+
+def fit_costmap_cost(feat: np.ndarray, cost: np.ndarray) -> np.ndarray:
+    """Least-squares weights of the costmap cost a integral + b KE_free + c on
+    DEM costs (the reference weights its costmap term by a calibrated
+    normaliser, jax_mbd_planner.py:696-699)."""
+    w, *_ = np.linalg.lstsq(np.asarray(feat, np.float64), np.asarray(cost, np.float64), rcond=None)
+    return w
+
+
 def random_starts(rb: RolloutBatch, nominal: torch.Tensor, rng: np.random.Generator, spread: StartSpread):
     """(R, 15) start states around each instance's nominal start (R, 15)."""
     R = rb.R
@@ -145,8 +245,10 @@ def make_dataset(rb: RolloutBatch, rounds: int, frames: int = 400, hold: int = 1
     R = rb.R
     N = rounds * R
     ras = torch.empty((N, RASTER_G, RASTER_G, 2), dtype=torch.bfloat16, device=dev)
-    extra = torch.empty((N, 6 + 2 * n_knots), dtype=torch.float32, device=dev)
+    extra = torch.empty((N, 6 + 2 * n_knots + N_PHYS), dtype=torch.float32, device=dev)
     tgt = np.empty((N, 3), np.float32)
+    starts_all = torch.empty((N, 15), dtype=torch.float32, device=dev)
+    ctl_all = torch.empty((N, n_knots, 2), dtype=torch.float32, device=dev)
     for k in range(rounds):
         starts = random_starts(rb, nominal, rng, spread)
         ctl = random_controls(R, n_knots, rng)
@@ -157,12 +259,23 @@ def make_dataset(rb: RolloutBatch, rounds: int, frames: int = 400, hold: int = 1
         for r in range(R):
             ras[k * R + r] = pr.of(r, starts[r:r + 1, :3])[0]
         extra[sl, :6] = ship_features(starts, start_xy)
-        extra[sl, 6:] = torch.as_tensor(ctl.reshape(R, -1), device=dev)
+        extra[sl, 6:6 + 2 * n_knots] = torch.as_tensor(ctl.reshape(R, -1), device=dev)
         tgt[sl] = targets_of(out)
+        starts_all[sl] = starts
+        ctl_all[sl] = torch.as_tensor(ctl, device=dev)
         if progress and (k + 1) % max(1, rounds // 10) == 0:
             progress(k + 1, rounds)
     rb.ship_init.copy_(nominal)
-    return {"raster": ras, "extra": extra, "target": torch.as_tensor(tgt, device=dev)}
+    # the costmap planners' view of the same rollouts: ship-only paths on each patch's KE costmap
+    cmf = torch.empty((N, 3), dtype=torch.float64, device=dev)
+    for r in range(R):
+        eng = rb.engines[r]
+        cm = KECostmap(eng, rb.scenes[rb.inst[r]], rb.ship)
+        ii = torch.arange(r, N, R, device=dev)
+        so = ship_rollouts(eng, starts_all[ii], ctl_all[ii], hold, frames, cm)
+        cmf[ii] = costmap_features(so)
+        extra[ii, 6 + 2 * n_knots:] = phys_features(so)
+    return {"raster": ras, "extra": extra, "target": torch.as_tensor(tgt, device=dev), "costmap_feat": cmf}
 
 
 @dataclass
@@ -185,10 +298,30 @@ def mirror_signs(n_knots: int = 40, device=None) -> torch.Tensor:
     """Signs of the extra inputs under the mirror image about the ship's
     centre line: sway, yaw rate, heading offset, lateral offset and the yaw
     commands change sign; surge, along-track offset and thrust do not."""
-    s = torch.ones(6 + 2 * n_knots, dtype=torch.float32, device=device)
+    s = torch.ones(6 + 2 * n_knots + N_PHYS, dtype=torch.float32, device=device)
     s[1:5] = -1.0
-    s[7::2] = -1.0
+    s[7:6 + 2 * n_knots:2] = -1.0
     return s
+
+
+def split_indices(N: int, holdout: float = 0.05, seed: int = 0):
+    """(validation, training) sample indices of a dataset of N."""
+    perm = np.random.default_rng(seed).permutation(N)
+    nval = max(1, int(N * holdout))
+    return perm[:nval], perm[nval:]
+
+
+def costmap_baseline(data: Dict[str, torch.Tensor], holdout: float = 0.05, seed: int = 0):
+    """The method without the learned part: the costmap cost calibrated on
+    the training split; returns (weights, relative cost error on the
+    validation split)."""
+    val, tr = split_indices(data["raster"].shape[0], holdout, seed)
+    f = data["costmap_feat"].cpu().numpy()
+    c = cost_of_targets(data["target"].cpu().numpy())
+    w = fit_costmap_cost(f[tr], c[tr])
+    pred = f[val] @ w
+    rel = np.abs(pred - c[val]) / np.maximum(np.abs(c[val]), COST_FLOOR)
+    return w, {"median": float(np.median(rel)), "mean": float(np.mean(rel)), "p90": float(np.percentile(rel, 90))}
 
 
 def train(net: Surrogate, data: Dict[str, torch.Tensor], epochs: int = 10, batch: int = 512, lr: float = 1e-3,
@@ -199,15 +332,13 @@ def train(net: Surrogate, data: Dict[str, torch.Tensor], epochs: int = 10, batch
     model and the drag are symmetric about the ship's centre line, so the
     mirrored input has the same outcome distribution)."""
     N = data["raster"].shape[0]
-    rng = np.random.default_rng(seed)
-    perm = rng.permutation(N)
-    nval = max(1, int(N * holdout))
-    val, tr = perm[:nval], perm[nval:]
+    val, tr = split_indices(N, holdout, seed)
+    rng = np.random.default_rng(seed + 1)
     t_np = data["target"].cpu().numpy()
     norm = Norm(t_np[tr].mean(0), t_np[tr].std(0) + 1e-6)
     tgt = norm.fwd(data["target"])
     dev = data["raster"].device
-    signs = mirror_signs((data["extra"].shape[1] - 6) // 2, dev)
+    signs = mirror_signs((data["extra"].shape[1] - 6 - N_PHYS) // 2, dev)
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
     hist = []
@@ -246,7 +377,7 @@ class Planner:
 
     def __init__(self, net: Surrogate, norm: Norm, scene: Scene, start: torch.Tensor, k: int = 64,
                  frames: int = 400, hold: int = 10, n_knots: int = 40, err_threshold: float = 0.05,
-                 max_judged: int = 256, cell_size_factor: float = 1.0):
+                 max_judged: int = 256, cell_size_factor: float = 1.0, costmap_weights=None):
         self.net, self.norm = net, norm
         self.k, self.frames, self.hold, self.nk = k, frames, hold, n_knots
         self.thr, self.max_judged = err_threshold, max_judged
@@ -260,15 +391,27 @@ class Planner:
         pr = PatchRasters(self.judge)
         self.ras = pr.of(0, self.start[None, :3]).contiguous()
         self.feat = ship_features(self.start[None], self.nominal_start[None, :2])
+        self.cm_w = None if costmap_weights is None else np.asarray(costmap_weights, np.float64)
+        self.costmap = KECostmap(self.judge.engines[0], scene, self.judge.ship)
 
     def surrogate_costs(self, ctl: torch.Tensor) -> torch.Tensor:
-        """(S, n_knots, 2) -> (S,) predicted costs (one CNN pass for the shared raster)."""
+        """(S, n_knots, 2) -> (S,) predicted costs (one CNN pass for the
+        shared raster, the ice-free ship rollouts, the MLP per candidate)."""
         S = ctl.shape[0]
-        extra = torch.empty((S, 6 + 2 * self.nk), dtype=torch.float32, device=ctl.device)
+        extra = torch.empty((S, 6 + 2 * self.nk + N_PHYS), dtype=torch.float32, device=ctl.device)
         extra[:, :6] = self.feat
-        extra[:, 6:] = ctl.reshape(S, -1)
+        extra[:, 6:6 + 2 * self.nk] = ctl.reshape(S, -1)
+        so = ship_rollouts(self.judge.engines[0], self.start, ctl, self.hold, self.frames, self.costmap)
+        extra[:, 6 + 2 * self.nk:] = phys_features(so)
         y = self.norm.inv(self.net.predict_shared(self.ras, extra))
         return torch.expm1(y[:, 0]) + y[:, 1]
+
+    def costmap_costs(self, ctl: torch.Tensor) -> torch.Tensor:
+        """(S, n_knots, 2) -> (S,) calibrated costmap costs of the ship-only
+        rollouts (the reference planners' scoring, without the network)."""
+        out = ship_rollouts(self.judge.engines[0], self.start, ctl, self.hold, self.frames, self.costmap)
+        w = torch.as_tensor(self.cm_w, device=out.device)
+        return costmap_features(out) @ w
 
     def dem_costs(self, ctl: torch.Tensor) -> np.ndarray:
         """DEM costs of up to k candidates (padded with the first)."""

@@ -1,0 +1,135 @@
+"""The planner's two scorers besides the DEM, and what the network gains.
+
+* The reference planners' KE costmap (build_costmap_jax, jax_costmap.py:
+  98-330): closed forms of one element (peak = the KE constant, zero beyond
+  the effective radius), maximum (not sum) where elements overlap, the
+  open-water floor, the square max-window dilation with edge padding.
+* Ship-only rollouts (the costmap planners' rollout): the same ship model as
+  the DEM's ship_mode 1 -- with the ice far away, the DEM rollout's KE loss;
+  on a constant costmap the integral is that constant times the path length.
+* The gain test: on held-out DEM rollouts (random starts and controls over
+  the rollout patches), the trained network's cost error against the
+  costmap cost calibrated on the same training rollouts.
+"""
+import math
+import types
+
+import numpy as np
+import pytest
+import torch
+
+from icedem import planner as pl, rollouts as ro, scenario
+from icedem.properties import ShipProperties
+from icedem.surrogate import Arch, Surrogate
+
+pytestmark = pytest.mark.gpu
+
+
+def _ke(m, M, v, w=5.0, s=1e-6):
+    return min(max(w * v * v * m * M * (m + 2 * M) / (2 * (M + m) ** 2) * s, 0.0), 1e8)
+
+
+def test_ke_costmap_closed_forms():
+    from icedem import _lib
+    lib = _lib.load()
+    dev = "cuda"
+    # elements at (5, 5) r 1 m 100 t, (5.4, 5) r 1 m 400 t, (20, 20) r 2 m 800 t on a 1 m grid from (0, 0)
+    pos = torch.tensor([5.0, 5.4, 20.0, 5.0, 5.0, 20.0], device=dev)
+    rad = torch.tensor([1.0, 1.0, 2.0], device=dev)
+    mass = torch.tensor([1e5, 4e5, 8e5], device=dev)
+    rows, cols, res, M, v = 30, 32, 1.0, 2.2e6, 2.0
+    work = torch.empty(rows * cols, dtype=torch.int32, device=dev)
+
+    def build(half, base=0.1, n=3):
+        out = torch.empty(rows * cols, dtype=torch.float32, device=dev)
+        assert lib.icesur_ke_costmap(pl._ptr(pos), pl._ptr(rad), pl._ptr(mass), n, rows, cols, 0.0, 0.0, res, 5.0, v,
+                                     M, 1e-6, 0.1, base, half, pl._ptr(work), pl._ptr(out), None) == 0
+        torch.cuda.synchronize()
+        return out.view(rows, cols).cpu().numpy().astype(np.float64)
+
+    g = build(0)
+    cy, cx = np.meshgrid(np.arange(rows) + 0.5, np.arange(cols) + 0.5, indexing="ij")
+    ref = np.full((rows, cols), 0.1)
+    for (x, y, r, m) in [(5.0, 5.0, 1.0, 1e5), (5.4, 5.0, 1.0, 4e5), (20.0, 20.0, 2.0, 8e5)]:
+        er = r * 1.1 + 0.5
+        f = np.maximum(0.0, (er * er - ((cx - x) ** 2 + (cy - y) ** 2)) / (er * er))
+        ref = np.maximum(ref, _ke(m, M, v) * f)
+    np.testing.assert_allclose(g, ref, rtol=2e-6, atol=1e-5)
+    # the heavier of two overlapping elements wins a shared cell (maximum, not sum)
+    k1, k2 = _ke(1e5, M, v), _ke(4e5, M, v)
+    assert k2 > k1 > 0.5
+    f1 = 1.0 - ((5.5 - 5.0) ** 2 + 0.25) / 1.6 ** 2
+    f2 = 1.0 - ((5.5 - 5.4) ** 2 + 0.25) / 1.6 ** 2
+    assert g[5, 5] == pytest.approx(max(k1 * f1, k2 * f2), rel=1e-5) and g[5, 5] < k1 * f1 + k2 * f2
+    # far from every element: the open-water floor
+    assert g[29, 0] == pytest.approx(0.1)
+    # dilation: each cell is the maximum of its (2h+1)^2 window (edge padding)
+    h = 2
+    gd = build(h)
+    pad = np.pad(g, h, mode="edge")
+    win = np.max(np.stack([pad[i:i + rows, j:j + cols] for i in range(2 * h + 1) for j in range(2 * h + 1)]), 0)
+    np.testing.assert_array_equal(gd, win)
+
+
+def _far_batch(n_inst=8):
+    sc = scenario.scene_patch(1234, 120, 0.3)
+    rb = ro.RolloutBatch([sc], [0] * n_inst)
+    rb.set_mode("frames")
+    # start 400 m from the ice: no contact within 400 frames
+    rb.ship_init[:, 1] -= 400.0
+    return rb
+
+
+def test_ship_rollouts_match_dem_without_ice():
+    rb = _far_batch()
+    ctl = torch.as_tensor(ro.nominal_controls(rb.R, 40, seed=3))
+    rb.reset()
+    out = rb.run(ctl, 10, 400).cpu().numpy()
+    assert np.all(out[:, 0] == 0.0)
+    so = pl.ship_rollouts(rb.engines[0], rb.ship_init, ctl, 10, 400).cpu().numpy()
+    np.testing.assert_allclose(so[:, 1], out[:, 1], rtol=1e-6, atol=1e-3)
+    # a constant costmap integrates to the constant times the path length
+    c = 0.75
+    cm = types.SimpleNamespace(grid=torch.full((50 * 60,), c, device="cuda"), rows=50, cols=60, x0=-1e4, y0=-1e4,
+                               res=1000.0, spec=types.SimpleNamespace(oob_penalty=0.0))
+    so2 = pl.ship_rollouts(rb.engines[0], rb.ship_init, ctl, 10, 400, cm).cpu().numpy()
+    np.testing.assert_allclose(so2[:, 0], c * so2[:, 2], rtol=1e-5)
+    np.testing.assert_array_equal(so2[:, 1:], so[:, 1:])
+    # straight ahead (no yaw command, no initial yaw rate): path length = distance travelled
+    straight = torch.zeros((rb.R, 40, 2))
+    straight[:, :, 0] = 0.5
+    rb.reset()
+    rb.run(straight, 10, 400)
+    s3 = pl.ship_rollouts(rb.engines[0], rb.ship_init, straight, 10, 400).cpu().numpy()
+    d = (rb.ships[:, :2] - rb.ship_init[:, :2]).norm(dim=1).cpu().numpy()
+    np.testing.assert_allclose(s3[:, 2], d, rtol=1e-3)
+    rb.close()
+
+
+def test_network_gains_over_costmap_cost():
+    """Same rollouts, same split: the trained network's median relative cost
+    error is below the calibrated costmap cost's, and it ranks the held-out
+    rollouts better."""
+    scenes = ro.rollout_patches(128, workers=8)
+    rb = ro.RolloutBatch(scenes, list(range(128)))
+    rb.set_mode("frames")
+    data = pl.make_dataset(rb, 80, seed=7)
+    rb.close()
+    N = data["raster"].shape[0]
+    w, cm_err = pl.costmap_baseline(data)
+    net = Surrogate(Arch(), seed=0)
+    norm, hist = pl.train(net, data, epochs=20, batch=512, lr=1e-3)
+    sur_err = hist[-1]["val_rel_cost_err"]
+    val, _ = pl.split_indices(N)
+    cost = pl.cost_of_targets(data["target"].cpu().numpy())[val]
+    ii = torch.as_tensor(val, device="cuda")
+    ps = pl.cost_of_targets(norm.inv(net.predict(data["raster"][ii], data["extra"][ii])).cpu().numpy())
+    pc = data["costmap_feat"].cpu().numpy()[val] @ w
+
+    def sp(a, b):
+        ra, rb_ = np.argsort(np.argsort(a)), np.argsort(np.argsort(b))
+        return float(np.corrcoef(ra, rb_)[0, 1])
+    print(f"{N} rollouts: surrogate val rel cost err {sur_err}, spearman {sp(ps, cost):.3f}; "
+          f"costmap cost {cm_err}, spearman {sp(pc, cost):.3f}; weights {w}", flush=True)
+    assert sur_err["median"] < cm_err["median"]
+    assert sp(ps, cost) > sp(pc, cost)
