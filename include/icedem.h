@@ -117,9 +117,10 @@ typedef struct {
      * cells around it, a large one the fine cells within
      * r + grid_rsmall + delta; every element scans the 2x2 coarse cells on
      * its side of its coarse cell.  A fine bucket holds up to grid_bucket
-     * elements; overflow goes to a list that every element also scans. */
+     * elements, a coarse one grid_bucket2 (0: min(large elements, 8));
+     * overflow goes to a list that every element also scans. */
     float grid_x0, grid_y0, grid_cell, grid_rsmall;
-    int32_t grid_nx, grid_ny, grid_bucket;
+    int32_t grid_nx, grid_ny, grid_bucket, grid_bucket2;
 } IceDemParams;
 
 /* Mutable state (IceState + ShipState, jax_ice_state.py:31-52), device
@@ -236,10 +237,14 @@ int icedem_batch_destroy(IceDemBatch* batch);
  * (x0, y0) is read bilinearly at the centre and the four hull corners (the
  * maximum kept; oob_penalty added when the column is outside the map);
  * d_out (S, 4) = [sum of 0.5 (c_t + c_t+1) |segment|, KE loss (J), path
- * length, final heading].  d_costmap may be NULL (cost 0). */
+ * length, final heading].  d_costmap may be NULL (cost 0).  d_snap (S,
+ * n_snap, 4), optional: after every snap_every frames (up to n_snap) the
+ * position in the start pose's frame (along, across), the heading change and
+ * the speed. */
 int icedem_ship_rollouts(IceDemCtx* ctx, const float* d_ship0, int32_t ship0_stride, const float* d_controls, int32_t S,
                          int32_t n_knots, int32_t hold, int32_t frames, const float* d_costmap, int32_t rows,
-                         int32_t cols, float x0, float y0, float res, float oob_penalty, float* d_out, void* stream);
+                         int32_t cols, float x0, float y0, float res, float oob_penalty, float* d_out,
+                         int32_t snap_every, int32_t n_snap, float* d_snap, void* stream);
 
 int icedem_sync(IceDemCtx* ctx, void* stream);
 

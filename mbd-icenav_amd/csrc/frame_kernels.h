@@ -26,7 +26,7 @@ namespace icedem {
 
 constexpr int GRP = 16;                 // lanes per element in k_force
 constexpr int EPB = BLOCK / GRP;        // elements per k_force block
-constexpr int MAXC2 = 32;               // contact partners kept per element (more: exact slow path)
+constexpr int MAXC2 = 16;               // contact partners kept per element (more: exact slow path)
 
 __device__ __forceinline__ int32_t* near_count(const DevView* V) { return V->counters; }
 __device__ __forceinline__ int32_t* nearblk_count(const DevView* V) { return V->counters + 1; }
@@ -796,7 +796,7 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
         wave_sync_lds();
         // contributions in rank order, written over the partner arrays once
         // every lane has read its inputs; lane 0 adds them in order
-        constexpr int PER = MAXC2 / GRP;
+        constexpr int PER = (MAXC2 + GRP - 1) / GRP;
         float ca[PER], cb[PER], cc[PER];
         #pragma unroll
         for (int u = 0; u < PER; ++u) {

@@ -282,7 +282,8 @@ __device__ __forceinline__ float hull_cost(const float* __restrict__ cm, int row
 __global__ void k_ship_rollouts(const DevView* __restrict__ V, const float* __restrict__ ship0, int ship0_stride,
                                 const float* __restrict__ ctl, int S, int nk, int hold, int frames,
                                 const float* __restrict__ cm, int rows, int cols, float x0, float y0, float res,
-                                float oob, float hl, float hw, float* __restrict__ out) {
+                                float oob, float hl, float hw, float* __restrict__ out, int snap_every, int n_snap,
+                                float* __restrict__ snap) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= S) return;
     const IceDemParams& P = V->p;
@@ -293,6 +294,9 @@ __global__ void k_ship_rollouts(const DevView* __restrict__ V, const float* __re
     float wp = cm ? hull_cost(cm, rows, cols, x0, y0, res, oob, hl, hw, st) : 0.0f;
     float cost = 0.0f, len = 0.0f;
     const float* c = ctl + (size_t)r * nk * 2;
+    const float sx0 = st[0], sy0 = st[1], psi0 = st[2];
+    float s0, c0;
+    sincosf(psi0, &s0, &c0);
     for (int k = 0; k < frames; ++k) {
         const int knot = min(k / max(hold, 1), nk - 1);
         const float px = st[0], py = st[1];
@@ -303,6 +307,15 @@ __global__ void k_ship_rollouts(const DevView* __restrict__ V, const float* __re
         cost = cost + 0.5f * (wp + w2) * seg;
         len = len + seg;
         wp = w2;
+        if (snap && snap_every > 0 && (k + 1) % snap_every == 0 && (k + 1) / snap_every <= n_snap) {
+            // position in the start's ship frame, heading change, speed
+            float* o = snap + ((size_t)r * n_snap + ((k + 1) / snap_every - 1)) * 4;
+            const float ex = st[0] - sx0, ey = st[1] - sy0;
+            o[0] = c0 * ex + s0 * ey;
+            o[1] = -s0 * ex + c0 * ey;
+            o[2] = st[2] - psi0;
+            o[3] = sqrtf(st[3] * st[3] + st[4] * st[4]);
+        }
     }
     const float ke = 0.5f * P.ship_mass * (st[3] * st[3] + st[4] * st[4]) + 0.5f * P.ship_Izz * (st[5] * st[5]);
     out[4 * r] = cost;
@@ -454,7 +467,8 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
         H.ny2 = std::max(1, (int)std::ceil((double)prm->grid_ny * prm->grid_cell / H.cell2));
         H.G2 = H.nx2 * H.ny2;
         H.nbig = nbig;
-        H.cap2 = std::max(1, std::min(nbig, 8));
+        H.cap2 = prm->grid_bucket2 > 0 ? std::min(prm->grid_bucket2, std::max(nbig, 1)) : std::max(1, std::min(nbig, 8));
+        if ((int64_t)H.G2 * H.cap2 > (1LL << 28)) { set_err(nullptr, "bad grid_bucket2"); delete ctx; return ICEDEM_E_INVALID; }
         H.ovf_cap = n + 64;   // every element is inserted once
     }
     const size_t nv = (size_t)n * st->max_v;
@@ -1070,10 +1084,11 @@ int icedem_component_com_velocities(int32_t n, const int32_t* d_labels, const fl
 
 int icedem_ship_rollouts(IceDemCtx* ctx, const float* d_ship0, int32_t ship0_stride, const float* d_controls, int32_t S,
                          int32_t n_knots, int32_t hold, int32_t frames, const float* d_costmap, int32_t rows,
-                         int32_t cols, float x0, float y0, float res, float oob_penalty, float* d_out, void* stream_) {
+                         int32_t cols, float x0, float y0, float res, float oob_penalty, float* d_out,
+                         int32_t snap_every, int32_t n_snap, float* d_snap, void* stream_) {
     if (!ctx || !d_ship0 || !d_controls || !d_out) { set_err(ctx, "null argument"); return ICEDEM_E_INVALID; }
     if (S < 0 || n_knots < 1 || frames < 0 || (ship0_stride != 0 && ship0_stride < ICEDEM_SHIP_STATE_LEN) ||
-        (d_costmap && (rows < 1 || cols < 1 || !(res > 0.0f)))) {
+        (d_costmap && (rows < 1 || cols < 1 || !(res > 0.0f))) || (d_snap && (snap_every < 1 || n_snap < 1))) {
         set_err(ctx, "icedem_ship_rollouts: bad sizes"); return ICEDEM_E_INVALID;
     }
     if (ctx->hv.p.ship_mode != 1) { set_err(ctx, "icedem_ship_rollouts: context is not in ship_mode 1"); return ICEDEM_E_INVALID; }
@@ -1082,7 +1097,7 @@ int icedem_ship_rollouts(IceDemCtx* ctx, const float* d_ship0, int32_t ship0_str
     const float hl = ctx->hv.p.half_len, hw = ctx->hv.p.half_beam;
     hipLaunchKernelGGL(k_ship_rollouts, dim3((S + 63) / 64), dim3(64), 0, (hipStream_t)stream_, ctx->dv, d_ship0,
                        ship0_stride, d_controls, S, n_knots, hold, frames, d_costmap, rows, cols, x0, y0, res, oob_penalty, hl, hw,
-                       d_out);
+                       d_out, snap_every, n_snap, d_snap);
     HIP_TRY(hipGetLastError());
     return ICEDEM_OK;
 }

@@ -60,7 +60,7 @@ class IceDemParams(C.Structure):
         ("B_pinv", _f * 18), ("B", _f * 18), ("n_max", _f * 6),
         ("T_n", _f), ("vc_surge", _f), ("vc_sway", _f), ("vc_yaw", _f),
         ("grid_x0", _f), ("grid_y0", _f), ("grid_cell", _f), ("grid_rsmall", _f),
-        ("grid_nx", _i), ("grid_ny", _i), ("grid_bucket", _i),
+        ("grid_nx", _i), ("grid_ny", _i), ("grid_bucket", _i), ("grid_bucket2", _i),
     ]
 
 
@@ -115,7 +115,7 @@ SIGNATURES = {
     "icesur_ke_costmap": (C.c_int, [_p, _p, _p, _i, _i, _i, C.c_float, C.c_float, C.c_float, C.c_double,
                                     C.c_double, C.c_double, C.c_double, C.c_float, C.c_float, _i, _p, _p, _p]),
     "icedem_ship_rollouts": (C.c_int, [_p, _p, _i, _p, _i, _i, _i, _i, _p, _i, _i, C.c_float, C.c_float,
-                                       C.c_float, C.c_float, _p, _p]),
+                                       C.c_float, C.c_float, _p, _i, _i, _p, _p]),
     "icesur_conv": (C.c_int, [_p, _i, _i, _i, _i, _p, _i, _p, _i, _p, _p, _p]),
     "icesur_adamw": (C.c_int, [_p, _p, _p, _p, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                _i, _p, _p]),

@@ -42,14 +42,15 @@ def bound_radius(local_polys_padded: np.ndarray) -> np.ndarray:
 
 def choose_grid(centroids: np.ndarray, radii: np.ndarray, dilation: float = 0.0, cell: Optional[float] = None,
                 margin: float = 60.0, max_cells: int = 1 << 22, small_quantile: float = 0.98):
-    """Uniform grid for the broad phase.
+    """Fine uniform grid of the broad phase.
 
     Elements with radius <= r_s (the ``small_quantile`` radius) live in one
     cell of side >= 2 (r_s + delta) and scan the 3x3 cells around it; the few
-    larger ones are inserted into, and scan, every cell within
-    r + r_s + 2 delta.  Either way every contacting pair is found (exactly the
-    reference's contact set); positions outside the grid are clamped to the
-    border cells.  Returns (x0, y0, cell, r_s, nx, ny, bucket_capacity).
+    larger ones live in a coarse grid that the library sizes and scan the
+    fine cells within r + r_s + delta.  Either way every contacting pair is
+    found (exactly the reference's contact set); positions outside the grid
+    are clamped to the border cells.  Returns (x0, y0, cell, r_s, nx, ny,
+    bucket_capacity, coarse_bucket_capacity).
     """
     r = np.asarray(radii, dtype=np.float64)
     r32 = r.astype(F32)
@@ -68,25 +69,29 @@ def choose_grid(centroids: np.ndarray, radii: np.ndarray, dilation: float = 0.0,
         if nx * ny <= max_cells:
             break
         cell *= 1.5
-    # bucket capacity from the initial occupancy (2x headroom); a full bucket
-    # spills into the overflow list, so this is a speed choice only
+    # fine-bucket capacity from the initial occupancy of the small elements
+    # (2x headroom); a full bucket spills into the overflow list, so this is a
+    # speed choice only
     big = r32 > F32(r_s)
     counts = np.zeros(nx * ny, dtype=np.int64)
     if len(c):
         cx = np.clip(np.floor((c[:, 0] - lo[0]) / cell), 0, nx - 1).astype(np.int64)
         cy = np.clip(np.floor((c[:, 1] - lo[1]) / cell), 0, ny - 1).astype(np.int64)
         np.add.at(counts, (cy * nx + cx)[~big], 1)
-        for e in np.nonzero(big)[0]:
-            R = (float(r32[e]) + r_s + 2.0 * dilation) * 1.001 + 1e-3
-            x0 = int(np.clip(np.floor((c[e, 0] - R - lo[0]) / cell), 0, nx - 1))
-            x1 = int(np.clip(np.floor((c[e, 0] + R - lo[0]) / cell), 0, nx - 1))
-            y0 = int(np.clip(np.floor((c[e, 1] - R - lo[1]) / cell), 0, ny - 1))
-            y1 = int(np.clip(np.floor((c[e, 1] + R - lo[1]) / cell), 0, ny - 1))
-            for gy in range(y0, y1 + 1):
-                counts[gy * nx + x0: gy * nx + x1 + 1] += 1
     bucket = int(max(16, 2 * int(counts.max(initial=0)) + 8))
     bucket = (bucket + 3) // 4 * 4
-    return float(lo[0]), float(lo[1]), float(cell), r_s, nx, ny, bucket
+    # coarse-bucket capacity the same way (the library sizes the coarse grid:
+    # cell >= 2 (2 r_max + 2 delta), cells over the fine grid's extent)
+    bucket2 = 0
+    if big.any():
+        c2 = max(cell, 2.0 * (2.0 * float(r32.max()) + 2.0 * dilation) * 1.002 + 1e-3)
+        nx2 = max(1, int(math.ceil(nx * cell / c2)))
+        ny2 = max(1, int(math.ceil(ny * cell / c2)))
+        bx = np.clip(np.floor((c[big, 0] - lo[0]) / c2), 0, nx2 - 1).astype(np.int64)
+        by = np.clip(np.floor((c[big, 1] - lo[1]) / c2), 0, ny2 - 1).astype(np.int64)
+        cnt2 = np.bincount(by * nx2 + bx, minlength=nx2 * ny2)
+        bucket2 = int(max(8, 2 * int(cnt2.max()) + 4))
+    return float(lo[0]), float(lo[1]), float(cell), r_s, nx, ny, bucket, bucket2
 
 
 def make_params(static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_length, ship_beam, n_ice_steps,
@@ -177,6 +182,7 @@ def make_params(static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_len
         raise ValueError("grid (x0, y0, cell, r_s, nx, ny, cap) is required; see choose_grid")
     p.grid_x0, p.grid_y0, p.grid_cell, p.grid_rsmall = F32(grid[0]), F32(grid[1]), F32(grid[2]), F32(grid[3])
     p.grid_nx, p.grid_ny, p.grid_bucket = int(grid[4]), int(grid[5]), int(grid[6])
+    p.grid_bucket2 = int(grid[7]) if len(grid) > 7 else 0
     return p
 
 

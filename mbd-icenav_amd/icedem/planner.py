@@ -148,37 +148,57 @@ class KECostmap:
 
 
 def ship_rollouts(eng, starts: torch.Tensor, controls: torch.Tensor, hold: int, frames: int,
-                  costmap: Optional[KECostmap] = None) -> torch.Tensor:
+                  costmap: Optional[KECostmap] = None, n_snap: int = 0):
     """Ship-only rollouts (no ice) of (S, n_knots, 2) controls from one start
     (15,) or one start each (S, 15), scored on the costmap; -> (S, 4)
-    [costmap integral, KE loss J, path length, final heading]."""
+    [costmap integral, KE loss J, path length, final heading], and with
+    n_snap > 0 also (S, n_snap, 4) snapshots every frames / n_snap frames
+    [along, across, heading change, speed]."""
     lib = _lib.load()
     ctl = torch.as_tensor(controls).to(device=eng.device, dtype=torch.float32).contiguous()
     st = starts.to(device=eng.device, dtype=torch.float32).contiguous()
     S, nk = int(ctl.shape[0]), int(ctl.shape[1])
     stride = 0 if st.dim() == 1 else 15
     out = torch.empty((S, 4), dtype=torch.float32, device=eng.device)
+    snap = torch.empty((S, max(n_snap, 1), 4), dtype=torch.float32, device=eng.device) if n_snap else None
+    every = max(1, int(frames) // max(n_snap, 1))
     cm = costmap
     s = C.c_void_p(torch.cuda.current_stream(eng.device).cuda_stream)
     _lib.check(lib.icedem_ship_rollouts(eng.ctx, _ptr(st), stride, _ptr(ctl), S, nk, int(hold), int(frames),
                                         _ptr(cm.grid) if cm else None, cm.rows if cm else 0, cm.cols if cm else 0,
                                         cm.x0 if cm else 0.0, cm.y0 if cm else 0.0, cm.res if cm else 1.0,
-                                        cm.spec.oob_penalty if cm else 0.0, _ptr(out), s), eng.ctx,
-               "icedem_ship_rollouts")
-    return out
+                                        cm.spec.oob_penalty if cm else 0.0, _ptr(out), every, int(n_snap),
+                                        _ptr(snap) if snap is not None else None, s), eng.ctx, "icedem_ship_rollouts")
+    return (out, snap) if n_snap else out
 
 
-N_PHYS = 3   # extra inputs from the ice-free ship rollout
+# Network inputs computed from the controls by the ice-free ship model: 3
+# summary values, plus 4 per path snapshot.  With the path snapshots the
+# network ranked the candidates of one start worse (more inputs to fit noise
+# to), so none are used.
+N_SNAP = 0
+N_PHYS = 3 + 4 * N_SNAP
 
 
-def phys_features(out: torch.Tensor) -> torch.Tensor:
-    """(S, 4) ship-rollout results -> (S, 3) network inputs: ice-free KE
-    loss / KE_REF, costmap integral / 10, path length / 5."""
+def phys_features(out: torch.Tensor, snap: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Ship-rollout results (S, 4) and snapshots (S, N_SNAP, 4) -> (S,
+    N_PHYS) network inputs: ice-free KE loss / KE_REF, costmap integral / 10,
+    path length / 5, then per snapshot along / 5, across, 10 x heading
+    change, speed / 3."""
     f = torch.empty((out.shape[0], N_PHYS), dtype=torch.float32, device=out.device)
     f[:, 0] = out[:, 1] / KE_REF
     f[:, 1] = out[:, 0] / 10.0
     f[:, 2] = out[:, 2] / 5.0
+    if N_SNAP:
+        sc = torch.tensor([0.2, 1.0, 10.0, 1.0 / 3.0], dtype=torch.float32, device=out.device)
+        f[:, 3:] = (snap * sc).reshape(out.shape[0], -1)
     return f
+
+
+def _phys(eng, starts, ctl, hold, frames, cm):
+    if N_SNAP:
+        return ship_rollouts(eng, starts, ctl, hold, frames, cm, N_SNAP)
+    return ship_rollouts(eng, starts, ctl, hold, frames, cm), None
 
 
 def costmap_features(out: torch.Tensor) -> torch.Tensor:
@@ -234,8 +254,12 @@ COST_FLOOR = 1.0   # cost units; relative errors use max(|cost|, COST_FLOOR)
 
 
 def make_dataset(rb: RolloutBatch, rounds: int, frames: int = 400, hold: int = 10, n_knots: int = 40,
-                 seed: int = 0, spread: Optional[StartSpread] = None, progress=None) -> Dict[str, torch.Tensor]:
-    """`rounds` x R DEM rollouts (one per patch instance per round)."""
+                 seed: int = 0, spread: Optional[StartSpread] = None, raw_controls: bool = False,
+                 progress=None) -> Dict[str, torch.Tensor]:
+    """`rounds` x R DEM rollouts (one per patch instance per round).  The
+    extra inputs are the ship state (6) and the controls seen through the
+    ice-free ship model (N_PHYS); ``raw_controls`` appends the knots
+    themselves (2 n_knots)."""
     spread = spread or StartSpread()
     rng = np.random.default_rng(seed)
     dev = rb.device
@@ -245,7 +269,7 @@ def make_dataset(rb: RolloutBatch, rounds: int, frames: int = 400, hold: int = 1
     R = rb.R
     N = rounds * R
     ras = torch.empty((N, RASTER_G, RASTER_G, 2), dtype=torch.bfloat16, device=dev)
-    extra = torch.empty((N, 6 + 2 * n_knots + N_PHYS), dtype=torch.float32, device=dev)
+    extra = torch.empty((N, 6 + N_PHYS), dtype=torch.float32, device=dev)
     tgt = np.empty((N, 3), np.float32)
     starts_all = torch.empty((N, 15), dtype=torch.float32, device=dev)
     ctl_all = torch.empty((N, n_knots, 2), dtype=torch.float32, device=dev)
@@ -259,7 +283,6 @@ def make_dataset(rb: RolloutBatch, rounds: int, frames: int = 400, hold: int = 1
         for r in range(R):
             ras[k * R + r] = pr.of(r, starts[r:r + 1, :3])[0]
         extra[sl, :6] = ship_features(starts, start_xy)
-        extra[sl, 6:6 + 2 * n_knots] = torch.as_tensor(ctl.reshape(R, -1), device=dev)
         tgt[sl] = targets_of(out)
         starts_all[sl] = starts
         ctl_all[sl] = torch.as_tensor(ctl, device=dev)
@@ -272,9 +295,11 @@ def make_dataset(rb: RolloutBatch, rounds: int, frames: int = 400, hold: int = 1
         eng = rb.engines[r]
         cm = KECostmap(eng, rb.scenes[rb.inst[r]], rb.ship)
         ii = torch.arange(r, N, R, device=dev)
-        so = ship_rollouts(eng, starts_all[ii], ctl_all[ii], hold, frames, cm)
+        so, snap = _phys(eng, starts_all[ii], ctl_all[ii], hold, frames, cm)
         cmf[ii] = costmap_features(so)
-        extra[ii, 6 + 2 * n_knots:] = phys_features(so)
+        extra[ii, 6:] = phys_features(so, snap)
+    if raw_controls:
+        extra = torch.cat([extra, ctl_all.reshape(N, -1)], dim=1)
     return {"raster": ras, "extra": extra, "target": torch.as_tensor(tgt, device=dev), "costmap_feat": cmf}
 
 
@@ -296,11 +321,15 @@ class Norm:
 
 def mirror_signs(n_knots: int = 40, device=None) -> torch.Tensor:
     """Signs of the extra inputs under the mirror image about the ship's
-    centre line: sway, yaw rate, heading offset, lateral offset and the yaw
-    commands change sign; surge, along-track offset and thrust do not."""
-    s = torch.ones(6 + 2 * n_knots + N_PHYS, dtype=torch.float32, device=device)
+    centre line: sway, yaw rate, heading offset, lateral offset, the
+    ice-free path's across offsets and heading changes, and the yaw commands
+    change sign; the rest do not."""
+    s = torch.ones(6 + N_PHYS + 2 * n_knots, dtype=torch.float32, device=device)
     s[1:5] = -1.0
-    s[7:6 + 2 * n_knots:2] = -1.0
+    for k in range(N_SNAP):
+        s[6 + 3 + 4 * k + 1] = -1.0     # across
+        s[6 + 3 + 4 * k + 2] = -1.0     # heading change
+    s[6 + N_PHYS + 1::2] = -1.0
     return s
 
 
@@ -398,11 +427,13 @@ class Planner:
         """(S, n_knots, 2) -> (S,) predicted costs (one CNN pass for the
         shared raster, the ice-free ship rollouts, the MLP per candidate)."""
         S = ctl.shape[0]
-        extra = torch.empty((S, 6 + 2 * self.nk + N_PHYS), dtype=torch.float32, device=ctl.device)
+        raw = self.net.arch.extra > 6 + N_PHYS
+        extra = torch.empty((S, 6 + N_PHYS + (2 * self.nk if raw else 0)), dtype=torch.float32, device=ctl.device)
         extra[:, :6] = self.feat
-        extra[:, 6:6 + 2 * self.nk] = ctl.reshape(S, -1)
-        so = ship_rollouts(self.judge.engines[0], self.start, ctl, self.hold, self.frames, self.costmap)
-        extra[:, 6 + 2 * self.nk:] = phys_features(so)
+        so, snap = _phys(self.judge.engines[0], self.start, ctl, self.hold, self.frames, self.costmap)
+        extra[:, 6:6 + N_PHYS] = phys_features(so, snap)
+        if raw:
+            extra[:, 6 + N_PHYS:] = ctl.reshape(S, -1)
         y = self.norm.inv(self.net.predict_shared(self.ras, extra))
         return torch.expm1(y[:, 0]) + y[:, 1]
 

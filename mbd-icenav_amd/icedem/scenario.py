@@ -70,18 +70,19 @@ def clip_convex(subject: np.ndarray, clip: np.ndarray) -> Optional[np.ndarray]:
         a = clip[k]
         b = clip[(k + 1) % m]
         ex, ey = b[0] - a[0], b[1] - a[1]
-        side = ex * (out[:, 1] - a[1]) - ey * (out[:, 0] - a[0])   # >= 0 inside (left of edge)
+        side = (ex * (out[:, 1] - a[1]) - ey * (out[:, 0] - a[0])).tolist()   # >= 0 inside (left of edge)
+        pts = out.tolist()
         res = []
-        n = len(out)
+        n = len(pts)
         for i in range(n):
-            p, q = out[i], out[(i + 1) % n]
+            p, q = pts[i], pts[(i + 1) % n]
             sp, sq = side[i], side[(i + 1) % n]
             if sp >= 0:
                 res.append(p)
             if (sp >= 0) != (sq >= 0):
                 t = sp / (sp - sq)
-                res.append(p + t * (q - p))
-        out = np.array(res) if res else np.zeros((0, 2))
+                res.append([p[0] + t * (q[0] - p[0]), p[1] + t * (q[1] - p[1])])
+        out = np.array(res, dtype=np.float64) if res else np.zeros((0, 2))
     if len(out) < 3:
         return None
     # drop near-duplicate consecutive vertices
@@ -97,21 +98,23 @@ def tessellate_floe(rng, floe: np.ndarray, n_cells: int) -> List[np.ndarray]:
     area = polygon_area(floe)
     lo, hi = floe.min(axis=0), floe.max(axis=0)
     spacing = 0.5 * math.sqrt(area / max(n_cells, 1))
-    pts: List[np.ndarray] = []
     e = np.roll(floe, -1, axis=0) - floe
+    buf = np.empty((n_cells, 2))
+    npts = 0
     attempts = 0
-    while len(pts) < n_cells and attempts < 60 * n_cells:
+    while npts < n_cells and attempts < 60 * n_cells:
         attempts += 1
         p = rng.uniform(lo, hi)
         side = e[:, 0] * (p[1] - floe[:, 1]) - e[:, 1] * (p[0] - floe[:, 0])
         if np.any(side < 0):
             continue
-        if pts and np.min(np.sum((np.array(pts) - p) ** 2, axis=1)) < spacing * spacing:
+        if npts and np.min(np.sum((buf[:npts] - p) ** 2, axis=1)) < spacing * spacing:
             continue
-        pts.append(p)
-    if len(pts) < 3:
+        buf[npts] = p
+        npts += 1
+    if npts < 3:
         return [floe.copy()]
-    P = np.array(pts)
+    P = buf[:npts].copy()
     c = floe.mean(axis=0)
     R = float(np.max(np.linalg.norm(floe - c, axis=1)))
     far = c + 20.0 * R * np.array([[1, 1], [1, -1], [-1, 1], [-1, -1], [1, 0], [-1, 0], [0, 1], [0, -1]], float)
@@ -144,12 +147,10 @@ def proximity_bonds(centroids: np.ndarray, radii: np.ndarray, factor: float = 1.
         return []
     cand = np.sort(cand, axis=1)
     cand = cand[np.lexsort((cand[:, 1], cand[:, 0]))]
-    out = []
-    for i, j in cand:
-        dist = np.linalg.norm(centroids[i] - centroids[j])
-        if dist < (radii[i] + radii[j]) * factor:
-            out.append((int(i), int(j)))
-    return out
+    d = centroids[cand[:, 0]] - centroids[cand[:, 1]]
+    dist = np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1])
+    keep = dist < (radii[cand[:, 0]] + radii[cand[:, 1]]) * factor
+    return [(int(i), int(j)) for i, j in cand[keep]]
 
 
 @dataclass
@@ -304,8 +305,8 @@ def ship_start_below(scene: Scene, ship_length: float, gap: float = 1.0, target_
     """Ship centre so that its bow (+x body, heading +y) is ``gap`` m short of the
     lowest ice vertex of ``target_floe`` (default: the lowest floe)."""
     if target_floe is None:
-        lows = [min(p[:, 1].min() for p, f in zip(scene.polygons, scene.parent_ids) if f == k)
-                for k in range(scene.n_floes)]
+        lows = np.full(scene.n_floes, np.inf)
+        np.minimum.at(lows, np.asarray(scene.parent_ids), np.array([p[:, 1].min() for p in scene.polygons]))
         target_floe = int(np.argmin(lows))
     sel = [p for p, f in zip(scene.polygons, scene.parent_ids) if f == target_floe]
     allv = np.vstack(sel)

@@ -3,11 +3,12 @@
 Predicts a DEM rollout's (impulse, KE loss, peak |F|) from
   - a G x G x 2 raster of the ice ahead of the ship (area fraction, mean
     thickness; bf16), built on the GPU by ``icesur_raster``,
-  - the ship's initial state (6 values), the rollout's control knots
-    (n_knots x 2) and 3 values of the same controls' ice-free ship rollout
-    (KE loss, costmap integral, path length), fp32.
+  - the ship's initial state (6 values) and the rollout's control knots
+    (n_knots x 2) seen through the ice-free ship model: 3 values of the
+    controls' ship-only rollout (KE loss, KE-costmap integral, path length),
+    fp32; the raw knots can be appended as well.
 Network: 3x3 stride-2 convolutions (32/64/128/128 channels, ReLU) -> 4x4x128
-features, concatenated with the 89 extra inputs -> MLP 512 -> 512 -> 3.
+features, concatenated with the 9 extra inputs -> MLP 512 -> 512 -> 3.
 
 Every floating-point operation of the forward pass, the backward pass and
 the Adam update runs in this package's HIP kernels (icesur.h): GEMMs in
@@ -42,7 +43,7 @@ class Arch:
     G: int = 64
     cin: int = 2
     chans: Tuple[int, ...] = (32, 64, 128, 128)
-    extra: int = 89
+    extra: int = 9
     hidden: Tuple[int, ...] = (512, 512)
     out: int = 3
 

@@ -100,9 +100,18 @@ def test_ship_rollouts_match_dem_without_ice():
     straight[:, :, 0] = 0.5
     rb.reset()
     rb.run(straight, 10, 400)
-    s3 = pl.ship_rollouts(rb.engines[0], rb.ship_init, straight, 10, 400).cpu().numpy()
+    s3, snap = pl.ship_rollouts(rb.engines[0], rb.ship_init, straight, 10, 400, None, 4)
+    s3, snap = s3.cpu().numpy(), snap.cpu().numpy()
     d = (rb.ships[:, :2] - rb.ship_init[:, :2]).norm(dim=1).cpu().numpy()
     np.testing.assert_allclose(s3[:, 2], d, rtol=1e-3)
+    # the last snapshot is the DEM rollout's end pose seen from the start pose
+    psi0 = rb.ship_init[:, 2].cpu().numpy()
+    e = (rb.ships[:, :2] - rb.ship_init[:, :2]).cpu().numpy()
+    along = np.cos(psi0) * e[:, 0] + np.sin(psi0) * e[:, 1]
+    np.testing.assert_allclose(snap[:, 3, 0], along, rtol=1e-5, atol=1e-4)
+    np.testing.assert_allclose(snap[:, 3, 1], 0.0, atol=1e-3)
+    np.testing.assert_allclose(snap[:, :, 2], 0.0, atol=1e-6)
+    assert np.all(np.diff(snap[:, :, 0], axis=1) > 0)
     rb.close()
 
 

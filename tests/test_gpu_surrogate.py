@@ -290,7 +290,7 @@ def test_network_implicit_conv_equals_im2col_path():
     net = sur.Surrogate(sur.Arch(), seed=3)
     rng = np.random.default_rng(5)
     raster = _bf(rng.uniform(0, 1, (33, 64, 64, 2)))
-    extra = torch.as_tensor(rng.normal(0, 1, (33, 89)).astype(np.float32), device="cuda")
+    extra = torch.as_tensor(rng.normal(0, 1, (33, 9)).astype(np.float32), device="cuda")
     y1 = net.forward(raster, extra).clone()
     net.implicit_conv = False
     y2 = net.forward(raster, extra).clone()
