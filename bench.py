@@ -373,6 +373,29 @@ def surrogate_bench(plain_rps: float, rounds: int = 400, epochs: int = 20, test_
     return out
 
 
+def batched_rollouts_only(R: int = 256, frames: int = 400, hold: int = 10, n_knots: int = 40, reps: int = 3):
+    """The batched-rollout launch alone (for profiling its kernels)."""
+    import torch
+    from icedem import rollouts as ro
+    scenes = ro.rollout_patches(R, workers=min(16, os.cpu_count() or 1))
+    rb = ro.RolloutBatch(scenes, list(range(R)))
+    rb.set_mode("frames")
+    ctl = torch.as_tensor(ro.nominal_controls(R, n_knots, seed=0))
+    rb.reset()
+    rb.run(ctl, hold, frames)
+    best = float("inf")
+    for _ in range(reps):
+        rb.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rb.run(ctl, hold, frames)
+        best = min(best, time.perf_counter() - t0)
+    n_el = sum(len(scenes[i].polygons) for i in range(R))
+    rb.close()
+    return {"workload": f"{R} rollouts x {frames} frames, {n_el} elements", "best_s": best,
+            "rollouts_per_s": R / best, "element_steps_per_s": n_el * frames / best, "us_per_frame": 1e6 * best / frames}
+
+
 def surrogate_forward_bench(B: int, reps: int = 5, train_steps: int = 20):
     """The surrogate's kernels alone: forward over B distinct synthetic
     rasters (uniform occupancy and thickness) and training steps of 512."""
@@ -424,6 +447,8 @@ def main():
     ap.add_argument("--surrogate-forward", type=int, default=0, metavar="B",
                     help="only time the surrogate's forward pass on B synthetic rasters and 20 training steps of "
                          "512 (the GEMM kernels' workload for profiling)")
+    ap.add_argument("--rollouts-only", action="store_true",
+                    help="only the batched rollouts (R patches x 400 frames, frames mode), three timed runs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB five times (reference kernel for the PMC byte counters)")
     args = ap.parse_args()
@@ -435,6 +460,9 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_each_way": x.numel() * 4, "copies": 5}), flush=True)
+        return
+    if args.rollouts_only:
+        print(json.dumps(batched_rollouts_only(args.rollouts)), flush=True)
         return
     if args.surrogate_forward:
         print(json.dumps(surrogate_forward_bench(args.surrogate_forward)), flush=True)

@@ -76,7 +76,8 @@ typedef struct {
     int32_t n_edge_samples;
     float edge_t[ICEDEM_MAX_EDGE_SAMPLES];      /* linspace samples t_k */
     float edge_omt[ICEDEM_MAX_EDGE_SAMPLES];    /* 1 - t_k */
-    float ship_bound_radius;                    /* cull radius of the hull */
+    float ship_bound_radius;                    /* cull radius of the hull (leaf entry points) */
+    float ship_box_half_len, ship_box_half_beam; /* box holding the analytic hull and the hull points */
     /* contact pre-pass, jax_simulation.py:431-509 */
     float w_contact_factor, w_contact_cap, flexural_base;
     float pen_crit_ref, pen_crit_thk_ref, side_push_factor;

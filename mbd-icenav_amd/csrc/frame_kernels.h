@@ -346,16 +346,24 @@ __device__ __forceinline__ float wave_tree_global(GAS const float* __restrict__ 
 // for the frame that uses bucket set `set`; (x, y, vx, vy) is the element's
 // state at the start of that frame, (sx, sy) the ship's position.  Every
 // thread of the 256-thread block calls.
+// Near test: the element's centroid within its bound radius + 1 m of the box
+// (in the ship's frame) that holds both the analytic hull of pass 1 and the
+// hull points of pass 2; elements outside it have no hull contact.  The
+// ship's heading comes from a fast sine/cosine: the 1 m margin dwarfs its
+// error, and only which elements run the exact hull test depends on it.
 __device__ __forceinline__ void cull_chunk(const DevView* __restrict__ V, int chunk, bool valid, int e,
-                                           float x, float y, float vx, float vy, float sx, float sy, int set) {
+                                           float x, float y, float vx, float vy, GAS const float* ship, int set) {
     const IceDemParams& P = V->p;
     bool near = false;
     float r = 0.0f;
     if (valid) {
         r = G(V->radius)[e];
-        const float dx = x - sx, dy = y - sy;
-        const float R = G(V->bound)[e] + P.ship_bound_radius + 1.0f;
-        near = (dx * dx + dy * dy) <= R * R;
+        float sh, ch;
+        __sincosf(ship[2], &sh, &ch);
+        const float dx = x - ship[0], dy = y - ship[1];
+        const float u = ch * dx + sh * dy, v = ch * dy - sh * dx;
+        const float b = G(V->bound)[e] + 1.0f;
+        near = fabsf(u) <= P.ship_box_half_len + b && fabsf(v) <= P.ship_box_half_beam + b;
         G(V->nearf)[e] = near ? 1 : 0;
         if (near) G(V->near_list)[atomicAdd(near_count(V), 1)] = e;
         cell_insert(V, set, e, x, y, vx, vy, r);
@@ -390,7 +398,7 @@ __device__ __forceinline__ void cull_kernel_body(const DevView* __restrict__ V, 
     GAS const float* vel = G(D->s.d_vel);
     float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f;
     if (valid) { x = pos[e]; y = pos[n + e]; vx = vel[e]; vy = vel[n + e]; }
-    cull_chunk(V, lb, valid, e, x, y, vx, vy, G(D->s.d_ship)[0], G(D->s.d_ship)[1], 0);
+    cull_chunk(V, lb, valid, e, x, y, vx, vy, G(D->s.d_ship), 0);
     if (lb == 0 && threadIdx.x == 0) store_pose(V, D->s.d_ship);
 }
 
@@ -1197,7 +1205,7 @@ __device__ __forceinline__ void post_body(const DevView* __restrict__ V, const C
     if (t < V->nf) { G(V->floe_coll)[t] = 0u; G(V->floe_maxfm)[t] = 0u; }
     if (t == 0) *force_ticket(V) = 0;
     if (cull && chunk < V->nblk)
-        cull_chunk(V, chunk, t < n, t, px, py, pvx, pvy, G(D->s.d_ship)[0], G(D->s.d_ship)[1], set ^ 1);
+        cull_chunk(V, chunk, t < n, t, px, py, pvx, pvy, G(D->s.d_ship), set ^ 1);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_post(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
@@ -1305,7 +1313,7 @@ __global__ __launch_bounds__(BLOCK) void k_scene_rollouts(const DevView* __restr
         const bool valid = e < n;
         float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f;
         if (valid) { x = G(D->s.d_pos)[e]; y = G(D->s.d_pos)[n + e]; vx = G(D->s.d_vel)[e]; vy = G(D->s.d_vel)[n + e]; }
-        cull_chunk(V, c, valid, e, x, y, vx, vy, G(D->s.d_ship)[0], G(D->s.d_ship)[1], 0);
+        cull_chunk(V, c, valid, e, x, y, vx, vy, G(D->s.d_ship), 0);
     }
     __syncthreads();
     for (int k = 0; k < frames; ++k) {

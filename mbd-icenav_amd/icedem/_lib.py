@@ -37,7 +37,7 @@ class IceDemParams(C.Structure):
         ("half_len", _f), ("half_beam", _f),
         ("n_edge_samples", _i),
         ("edge_t", _f * MAX_EDGE_SAMPLES), ("edge_omt", _f * MAX_EDGE_SAMPLES),
-        ("ship_bound_radius", _f),
+        ("ship_bound_radius", _f), ("ship_box_half_len", _f), ("ship_box_half_beam", _f),
         ("w_contact_factor", _f), ("w_contact_cap", _f), ("flexural_base", _f),
         ("pen_crit_ref", _f), ("pen_crit_thk_ref", _f), ("side_push_factor", _f),
         ("has_thrust_cap", _i),

@@ -116,6 +116,8 @@ def make_params(static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_len
     hull = np.asarray(ship_dyn.ship_hull_local, dtype=np.float64)
     R = max(math.hypot(hl, hb), float(np.max(np.hypot(hull[:, 0], hull[:, 1]))))
     p.ship_bound_radius = F32(R * 1.0001 + 1e-3)
+    p.ship_box_half_len = F32(max(hl, float(np.max(np.abs(hull[:, 0])))) * 1.0001 + 1e-3)
+    p.ship_box_half_beam = F32(max(hb, float(np.max(np.abs(hull[:, 1])))) * 1.0001 + 1e-3)
     p.w_contact_factor = F32(cfg.w_contact_radius_factor)
     p.w_contact_cap = F32(cfg.w_contact_cap)
     p.flexural_base = F32(getattr(cfg, "flexural_strength_base", IceProperties.FLEXURAL_STRENGTH_BASE))
@@ -258,14 +260,17 @@ class DemEngine:
         self.max_v = st.max_v
 
     def near_stats(self, st: IceState, ship_buf: torch.Tensor):
-        """Elements passing the near-ship bounding-circle test for the current
-        state, and the 256-element blocks holding one (host-side count, for
-        traffic accounting)."""
-        pos = st.positions.detach().float().cpu().numpy()
-        sx, sy = (float(v) for v in ship_buf[:2].detach().cpu().numpy())
-        R = self._host["bound"].astype(np.float64) + float(self.params.ship_bound_radius) + 1.0
-        d2 = (pos[:, 0] - sx) ** 2 + (pos[:, 1] - sy) ** 2
-        near = d2 <= R * R
+        """Elements passing the near-ship box test for the current state, and
+        the 256-element blocks holding one (host-side count, for traffic
+        accounting)."""
+        pos = st.positions.detach().float().cpu().numpy().astype(np.float64)
+        sx, sy, sh = (float(v) for v in ship_buf[:3].detach().cpu().numpy())
+        b = self._host["bound"].astype(np.float64) + 1.0
+        dx, dy = pos[:, 0] - sx, pos[:, 1] - sy
+        u = math.cos(sh) * dx + math.sin(sh) * dy
+        v = math.cos(sh) * dy - math.sin(sh) * dx
+        near = (np.abs(u) <= float(self.params.ship_box_half_len) + b) & \
+               (np.abs(v) <= float(self.params.ship_box_half_beam) + b)
         blocks = np.unique(np.nonzero(near)[0] // 256)
         return int(near.sum()), int(len(blocks))
 
