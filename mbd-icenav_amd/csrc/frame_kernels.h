@@ -33,6 +33,24 @@ __device__ __forceinline__ int32_t* nearblk_count(const DevView* V) { return V->
 __device__ __forceinline__ int32_t* near_ticket(const DevView* V) { return V->counters + 2; }
 __device__ __forceinline__ int32_t* force_ticket(const DevView* V) { return V->counters + 3; }
 
+// Hand-offs between workgroups of one kernel (block partials -> the
+// workgroup that draws the last ticket): the producer writes with
+// device-coherent stores (written through past its XCD's L2), waits for them
+// to complete, then takes its ticket; the last-ticket consumer reads with
+// device-coherent loads.  Unlike a device-scope fence this needs no L2
+// write-back or invalidation, which would stall every wave on the XCD.
+__device__ __forceinline__ void st_dev(float* p, float v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float ld_dev(const float* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stores_done() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // ── broad phase: two uniform grids of fixed-capacity buckets ────────────────
 // Elements of radius <= grid_rsmall sit in the bucket of their fine cell
 // (cell >= 2 (grid_rsmall + delta)); larger ones sit in the bucket of their
@@ -270,12 +288,25 @@ __device__ __forceinline__ float block256_tree(const DevView* __restrict__ V, GA
 // (the order of block256_tree), every load issued before the first add.
 __device__ __forceinline__ void block256_tree3(const DevView* __restrict__ V, GAS const float* __restrict__ a0,
                                                GAS const float* __restrict__ a1, GAS const float* __restrict__ a2,
-                                               int blk, float& s0, float& s1, float& s2) {
+                                               int blk, float& s0, float& s1, float& s2, bool dev = false) {
     const int lane = threadIdx.x & 63;
     const int e0 = blk * BLOCK + lane * 4;
     uint32_t nf = 0;
     float4 x0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), x1 = x0, x2 = x0;
-    if (e0 + 3 < V->n) {
+    if (dev) {   // values written by other workgroups of the running kernel
+        const float *g0 = (const float*)a0, *g1 = (const float*)a1, *g2 = (const float*)a2;
+        float v0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, v1[4] = {0.0f, 0.0f, 0.0f, 0.0f}, v2[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        #pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (e0 + i < V->n) {
+                nf |= (uint32_t)G(V->nearf)[e0 + i] << (8 * i);
+                v0[i] = ld_dev(g0 + e0 + i); v1[i] = ld_dev(g1 + e0 + i); v2[i] = ld_dev(g2 + e0 + i);
+            }
+        }
+        x0 = make_float4(v0[0], v0[1], v0[2], v0[3]);
+        x1 = make_float4(v1[0], v1[1], v1[2], v1[3]);
+        x2 = make_float4(v2[0], v2[1], v2[2], v2[3]);
+    } else if (e0 + 3 < V->n) {
         nf = *(GAS const uint32_t*)(G(V->nearf) + e0);
         const f4u y0 = *(GAS const f4u*)(a0 + e0), y1 = *(GAS const f4u*)(a1 + e0), y2 = *(GAS const f4u*)(a2 + e0);
         x0 = make_float4(y0.x, y0.y, y0.z, y0.w);
@@ -314,16 +345,20 @@ __device__ __forceinline__ float wave_tree_small(float x, int npad) {
     return __shfl(x, 0, 64);
 }
 
-__device__ __forceinline__ float wave_tree_global(GAS const float* __restrict__ a, int npad) {
+// dev: the same array as a generic pointer, read with device-coherent loads
+// (values written by other workgroups of the running kernel)
+__device__ __forceinline__ float wave_tree_global(GAS const float* __restrict__ a, int npad,
+                                                  const float* dev = nullptr) {
     const int lane = threadIdx.x & 63;
-    if (npad <= 64) return wave_tree_small(lane < npad ? a[lane] : 0.0f, npad);
+    if (npad <= 64) return wave_tree_small(lane < npad ? (dev ? ld_dev(dev + lane) : a[lane]) : 0.0f, npad);
     const int c = npad >> 6;
     GAS const float* p = a + (size_t)lane * c;
+    const float* pd = dev ? dev + (size_t)lane * c : nullptr;
     float stk[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     #pragma unroll
     for (int i = 0; i < 64; ++i) {
         if (i < c) {
-            float x = p[i];
+            float x = pd ? ld_dev(pd + i) : p[i];
             int lvl = 0;
             #pragma unroll
             for (int t = 0; t < 6; ++t) {
@@ -457,7 +492,8 @@ __device__ __forceinline__ void near_elem(const DevView* __restrict__ V, const C
     if (lane == 0) {
         PrePass o = prepass(P, pen, nxo, nyo, rad, thk, px, py, SP.sx, SP.sy);
         G(V->pen)[e] = pen; G(V->nx)[e] = nxo; G(V->ny)[e] = nyo;
-        G(V->fm_raw)[e] = o.fm; G(V->soi_x)[e] = o.sx; G(V->soi_y)[e] = o.sy; G(V->mterm)[e] = o.m;
+        G(V->fm_raw)[e] = o.fm;
+        st_dev(V->soi_x + e, o.sx); st_dev(V->soi_y + e, o.sy); st_dev(V->mterm + e, o.m);   // read by near_block_done
         if (pen > 0.0f) {
             const int f = G(V->floe)[e];
             atomicOr(&V->floe_coll[f], 1u);
@@ -472,27 +508,25 @@ __device__ __forceinline__ void near_elem(const DevView* __restrict__ V, const C
 __device__ __forceinline__ void near_block_done(const DevView* __restrict__ V, int b) {
     const IceDemParams& P = V->p;
     const int lane = threadIdx.x & 63, nbpad = V->nbpad;
-    __threadfence();
     float a, c, d;
-    block256_tree3(V, G(V->soi_x), G(V->soi_y), G(V->mterm), b, a, c, d);
+    block256_tree3(V, G(V->soi_x), G(V->soi_y), G(V->mterm), b, a, c, d, true);
     int lastb = 0;
     if (lane == 0) {
-        G(V->part1)[b] = a; G(V->part1)[nbpad + b] = c; G(V->part1)[2 * nbpad + b] = d;
-        __threadfence();
+        st_dev(V->part1 + b, a); st_dev(V->part1 + nbpad + b, c); st_dev(V->part1 + 2 * nbpad + b, d);
+        stores_done();
         lastb = (atomicAdd(near_ticket(V), 1) == *nearblk_count(V) - 1) ? 1 : 0;
     }
     if (!__shfl(lastb, 0, 64)) return;
-    __threadfence();
     float tx, ty, tm;
     if (nbpad <= 64) {   // the three loads in one round trip
-        const float l0 = lane < nbpad ? G(V->part1)[lane] : 0.0f;
-        const float l1 = lane < nbpad ? G(V->part1)[nbpad + lane] : 0.0f;
-        const float l2 = lane < nbpad ? G(V->part1)[2 * nbpad + lane] : 0.0f;
+        const float l0 = lane < nbpad ? ld_dev(V->part1 + lane) : 0.0f;
+        const float l1 = lane < nbpad ? ld_dev(V->part1 + nbpad + lane) : 0.0f;
+        const float l2 = lane < nbpad ? ld_dev(V->part1 + 2 * nbpad + lane) : 0.0f;
         tx = wave_tree_small(l0, nbpad); ty = wave_tree_small(l1, nbpad); tm = wave_tree_small(l2, nbpad);
     } else {
-        tx = wave_tree_global(G(V->part1), nbpad);
-        ty = wave_tree_global(G(V->part1) + nbpad, nbpad);
-        tm = wave_tree_global(G(V->part1) + 2 * nbpad, nbpad);
+        tx = wave_tree_global(G(V->part1), nbpad, V->part1);
+        ty = wave_tree_global(G(V->part1) + nbpad, nbpad, V->part1 + nbpad);
+        tm = wave_tree_global(G(V->part1) + 2 * nbpad, nbpad, V->part1 + 2 * nbpad);
     }
     float scale = 1.0f;
     if (P.has_thrust_cap) {
@@ -526,7 +560,7 @@ __device__ __forceinline__ void near_kernel_body(const DevView* __restrict__ V, 
         int fin = 0;
         if (lane == 0) {
             const int b = e / BLOCK;
-            __threadfence();
+            stores_done();
             fin = (atomicAdd(&V->blkdone[b], 1) == G(V->nearcnt)[b] - 1) ? 1 : 0;
         }
         if (__shfl(fin, 0, 64)) near_block_done(V, e / BLOCK);
@@ -627,21 +661,20 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
             a = block256_tree(V, G(V->soi_x), b, s, true);
             c = block256_tree(V, G(V->soi_y), b, s, true);
         }
-        if (threadIdx.x == 0) { G(V->part2)[b] = a; G(V->part2)[nbpad + b] = c; }
+        if (threadIdx.x == 0) { st_dev(V->part2 + b, a); st_dev(V->part2 + nbpad + b, c); }
     }
     if (threadIdx.x == 0) {
-        __threadfence();
+        stores_done();
         *am_last = (atomicAdd(force_ticket(V), 1) == nextra - 1) ? 1 : 0;
     }
     __syncthreads();
     if (!*am_last) return;
-    __threadfence();
     const float scale = G(V->scal)[0];
     float fx = 0.0f, fy = 0.0f;
     if (threadIdx.x < 64) {
         if (capped) {
-            fx = wave_tree_global(G(V->part2), nbpad);
-            fy = wave_tree_global(G(V->part2) + nbpad, nbpad);
+            fx = wave_tree_global(G(V->part2), nbpad, V->part2);
+            fy = wave_tree_global(G(V->part2) + nbpad, nbpad, V->part2 + nbpad);
         } else {
             fx = G(V->scal)[4];
             fy = G(V->scal)[5];

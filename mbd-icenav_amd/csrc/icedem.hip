@@ -794,7 +794,8 @@ int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** ou
         const DevView& H = ctxs[i]->hv;
         const int fblk = (H.n + EPB - 1) / EPB;
         const int nextra = H.p.has_thrust_cap ? H.nblk : 1;
-        const int nnear = std::max(1, std::min(16, (H.n + 255) / 256));
+        // one wavefront per 16 elements (a quarter of a patch is near the ship at most)
+        const int nnear = std::max(1, std::min(64, (H.n + 63) / 64));
         const int post = std::max(std::max(H.nb, H.n), std::max(std::max(H.G, H.G2), H.nf));
         const int npost = (post + BLOCK - 1) / BLOCK;
         for (int l = 0; l < H.nblk; ++l) mc.push_back(make_int4(i, l, H.nblk, 0));
