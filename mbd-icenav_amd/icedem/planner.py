@@ -7,13 +7,18 @@ the DEM itself (``rollouts``) and, at planner scale, by a learned surrogate
 
   * ``make_dataset``: DEM rollouts over rollout patches with random start
     states and random control sequences; the surrogate's inputs are the
-    ship-frame ice raster, a 6-value ship state and the control knots.
+    ship-frame ice raster, a 6-value ship state and the control knots seen
+    through the ice-free ship model (``ship_rollouts``).
   * ``train``: Adam on the weighted MSE of standardised targets, on the GPU.
+  * ``KECostmap`` / ``ship_rollouts`` / ``costmap_baseline``: the reference
+    planners' scorer, the method without the learned part.
   * ``Planner.iterate``: one sampling iteration -- S candidates scored by the
-    surrogate (one CNN pass for the shared raster, the MLP per candidate), the
-    k best re-scored by the batched DEM as judge; when the surrogate's error
-    on the judged set exceeds a threshold, the next k by surrogate rank are
-    judged as well (fallback), up to a budget.
+    surrogate (one CNN pass for the shared raster, the ice-free rollouts and
+    the MLP per candidate), the k best re-scored by the batched DEM as judge;
+    when the surrogate's error on the judged set exceeds a threshold, the next
+    k by surrogate rank are judged as well (fallback), up to a budget; an
+    iteration that needed the fallback makes the next one judge the whole
+    budget in one launch.
 
 Rollout cost is ``rollouts.rollout_cost``: impulse / 1e5 N s + KE loss / 1e6 J.
 """
@@ -412,6 +417,7 @@ class Planner:
         self.thr, self.max_judged = err_threshold, max_judged
         self.cell_size_factor = cell_size_factor
         self._ex = None
+        self.width = k
         self.judge = RolloutBatch([scene], [0] * k, cell_size_factor=cell_size_factor)
         self.judge.set_mode("frames")
         self.nominal_start = self.judge.ship_init[0].clone()
@@ -444,14 +450,30 @@ class Planner:
         w = torch.as_tensor(self.cm_w, device=out.device)
         return costmap_features(out) @ w
 
-    def dem_costs(self, ctl: torch.Tensor) -> np.ndarray:
-        """DEM costs of up to k candidates (padded with the first)."""
+    def _batch(self, width: int) -> RolloutBatch:
+        """A judge batch of `width` instances of the patch (k, or the
+        exhaustive batch's 256)."""
+        if width == self.k:
+            return self.judge
+        if self._ex is None or self._ex.R != width:
+            if self._ex is not None:
+                self._ex.close()
+            self._ex = RolloutBatch([self.judge.scenes[0]], [0] * width, cell_size_factor=self.cell_size_factor)
+            self._ex.set_mode("frames")
+            self._ex.ship_init.copy_(self.start.expand(width, 15))
+        return self._ex
+
+    def dem_costs(self, ctl: torch.Tensor, width: Optional[int] = None) -> np.ndarray:
+        """DEM costs of up to `width` (default k) candidates in one launch
+        (padded with the first)."""
         m = ctl.shape[0]
-        full = torch.empty((self.k, self.nk, 2), dtype=torch.float32, device=ctl.device)
+        width = width or self.k
+        rb = self._batch(width)
+        full = torch.empty((width, self.nk, 2), dtype=torch.float32, device=ctl.device)
         full[:m] = ctl
         full[m:] = ctl[:1]
-        self.judge.reset()
-        out = self.judge.run(full, self.hold, self.frames).cpu().numpy()
+        rb.reset()
+        out = rb.run(full, self.hold, self.frames).cpu().numpy()
         return rollout_cost(out[:m])
 
     def iterate(self, nominal: np.ndarray, S: int = 16384, sigma: float = 0.2, seed: int = 0) -> Dict:
@@ -471,9 +493,11 @@ class Planner:
         err = float("inf")
         pos = 0
         while pos < len(order) and len(judged) < self.max_judged:
-            sel = order[pos:pos + self.k]
-            pos += self.k
-            c = self.dem_costs(ctl[torch.as_tensor(sel, device=dev)])
+            # the first judge launch is as wide as the last iteration needed
+            w = self.width if pos == 0 else self.k
+            sel = order[pos:pos + w]
+            pos += w
+            c = self.dem_costs(ctl[torch.as_tensor(sel, device=dev)], w)
             p = pred[torch.as_tensor(sel, device=dev)].cpu().numpy()
             judged.extend(sel.tolist()); dem.extend(c.tolist()); prd.extend(p.tolist())
             d, q = np.asarray(dem), np.asarray(prd)
@@ -482,6 +506,12 @@ class Planner:
                 break
         torch.cuda.synchronize()
         t2 = time.perf_counter()
+        # judge this many at once next time (the fallback rounds this one
+        # needed); the wide batch is built and its graph captured now
+        width = 256 if len(judged) > self.k and self.max_judged >= 256 else self.k
+        if width != self.width:
+            self.width = width
+            self.dem_costs(ctl[:1], width)
         dem = np.asarray(dem)
         b = int(np.argmin(dem))
         return {"best_index": int(judged[b]), "best_cost": float(dem[b]), "best_controls": cand[judged[b]],
@@ -503,10 +533,7 @@ class Planner:
     def exhaustive(self, cand: np.ndarray, batch: int = 256) -> np.ndarray:
         """DEM costs of every candidate (S, n_knots, 2), ``batch`` instances
         of the patch per launch: the plain batched-DEM planner's scores."""
-        if self._ex is None or self._ex.R != batch:
-            self._ex = RolloutBatch([self.judge.scenes[0]], [0] * batch, cell_size_factor=self.cell_size_factor)
-            self._ex.set_mode("frames")
-            self._ex.ship_init.copy_(self.start.expand(batch, 15))
+        ex = self._batch(batch)
         S = cand.shape[0]
         out = np.empty(S, np.float64)
         ctl = torch.as_tensor(cand, device=self.judge.device)
@@ -515,8 +542,8 @@ class Planner:
             full = torch.empty((batch, self.nk, 2), dtype=torch.float32, device=ctl.device)
             full[:m] = ctl[s0:s0 + m]
             full[m:] = ctl[s0:s0 + 1]
-            self._ex.reset()
-            out[s0:s0 + m] = rollout_cost(self._ex.run(full, self.hold, self.frames).cpu().numpy()[:m])
+            ex.reset()
+            out[s0:s0 + m] = rollout_cost(ex.run(full, self.hold, self.frames).cpu().numpy()[:m])
         return out
 
     def close(self):

@@ -142,3 +142,26 @@ def test_network_gains_over_costmap_cost():
           f"costmap cost {cm_err}, spearman {sp(pc, cost):.3f}; weights {w}", flush=True)
     assert sur_err["median"] < cm_err["median"]
     assert sp(ps, cost) > sp(pc, cost)
+
+
+def test_planner_reports_dem_costs_and_widens_its_judge():
+    """Every reported cost is a DEM cost of the reported controls; the
+    fallback judges more candidates, and the next iteration judges its whole
+    budget in one launch."""
+    sc = scenario.scene_patch(5000, 200, 0.4)
+    tmp = ro.RolloutBatch([sc], [0])
+    start = tmp.ship_init[0].clone()
+    tmp.close()
+    net = Surrogate(Arch(), seed=0)              # untrained: its error forces the fallback
+    norm = pl.Norm(np.zeros(3, np.float32), np.ones(3, np.float32))
+    P = pl.Planner(net, norm, sc, start, k=64, max_judged=256, err_threshold=0.05)
+    nominal = np.tile(np.array([0.7, 0.0], np.float32), (40, 1))
+    r1 = P.iterate(nominal, S=2048, seed=1)
+    assert r1["judged"] == 256 and P.width == 256
+    again = P.dem_costs(torch.as_tensor(r1["best_controls"][None], device="cuda"))
+    assert again[0] == pytest.approx(r1["best_cost"], rel=0, abs=0)
+    r2 = P.iterate(nominal, S=2048, seed=2)
+    assert r2["judged"] == 256
+    again = P.dem_costs(torch.as_tensor(r2["best_controls"][None], device="cuda"), 256)
+    assert again[0] == r2["best_cost"]
+    P.close()
