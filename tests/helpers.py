@@ -49,6 +49,7 @@ class Case:
     _engine: object = None
     _oracle: object = None
     _oracle64: object = None
+    pair_mode: str = "all"
 
     @staticmethod
     def make(name: str, n_frames: int = 200, seed: int = 0, cfg: Optional[SimulationConfig] = None,
@@ -82,6 +83,17 @@ class Case:
             traj = scenario.straight_trajectory(x0, y0, math.pi / 2, 2.0, dt, n_frames)
             hull = omodel.hull_polygon(L, B, n_pts=hull_points // 2 - 1)
             warm = 0
+        elif name in ("c1", "c2"):
+            # the benchmark fields at full size (bench.py build_case)
+            sc = scenario.scene_c1(seed) if name == "c1" else scenario.scene_c2(seed)
+            cfg = cfg or benchmark_config()
+            L, B = 60.0, 12.0
+            mass = ShipProperties(L, B, 5.0).mass
+            dt = cfg.dt
+            x0, y0 = scenario.ship_start_below(sc, L, gap=1.0)
+            traj = scenario.straight_trajectory(x0, y0, math.pi / 2, 2.0 if name == "c1" else 3.0, dt, n_frames)
+            hull = omodel.hull_polygon(L, B, n_pts=hull_points // 2 - 1)
+            warm = 0
         elif name == "ring":
             # one big square ringed by 60 small squares: >48 contacts on one element
             polys = [np.array([[-10.0, -10.0], [10.0, -10.0], [10.0, 10.0], [-10.0, 10.0]])]
@@ -103,6 +115,8 @@ class Case:
         else:
             raise ValueError(name)
         case = Case(name, sc, cfg, L, B, mass, hull, traj, dt, use_auto, warm)
+        if case.n > 20000:
+            case.pair_mode = "cells"      # the oracle's cell-list pairs (same pairs and order as all pairs)
         if current is not None:
             speed, direction = current
             v = np.array([speed * math.cos(direction), speed * math.sin(direction)], np.float32)
@@ -124,7 +138,7 @@ class Case:
             fos = omodel.Fossen(self.ship_mass, self.ship_length)
             dt_ice, nsteps = omodel.time_stepping(st, ocfg, self.dt)
             o = ostep.Oracle(st, ocfg, sd, self.ship_length, self.ship_beam, nsteps, self.dt, dt_ice,
-                             self.use_auto, fos.B, fos.T_n, dtype=dtype)
+                             self.use_auto, fos.B, fos.T_n, dtype=dtype, pair_mode=self.pair_mode)
             setattr(self, attr, o)
         return getattr(self, attr)
 
