@@ -207,7 +207,8 @@ template <int TN, int MODE>
 __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ X, int B, int H, int W, int C, int Ho,
                                               int Wo, const uint16_t* __restrict__ Wt, int N,
                                               const float* __restrict__ bias, uint16_t* __restrict__ out, int flags) {
-    constexpr int A_BYTES = 2 * CM * CKP * 2, B_BYTES = 2 * TN * CKP * 2;
+    constexpr int NBUF = MODE == CONV_PAIR ? 1 : 2;           // C = 2: K = 18 is one step
+    constexpr int A_BYTES = NBUF * CM * CKP * 2, B_BYTES = NBUF * TN * CKP * 2;
     constexpr int O_LD = TN + 8;
     constexpr int O_BYTES = CM * O_LD * 2;
     constexpr int L_BYTES = (A_BYTES + B_BYTES) > O_BYTES ? (A_BYTES + B_BYTES) : O_BYTES;
@@ -343,7 +344,7 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ X, in
         }
         if (more) stage(cur ^ 1);
         __syncthreads();
-        cur ^= 1;
+        if (NBUF == 2) cur ^= 1;
     }
     // epilogue into LDS (the loop's last barrier retired every read of As / Bs)
     #pragma unroll
