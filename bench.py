@@ -259,7 +259,7 @@ def dem_sensitivity(rb, shifts=(0.01, 0.1), seed: int = 5):
     return out
 
 
-def surrogate_bench(plain_rps: float, rounds: int = 400, epochs: int = 20, test_patches: int = 4,
+def surrogate_bench(plain_rps: float, rounds: int = 800, epochs: int = 20, test_patches: int = 4,
                     samples: int = 16384, k: int = 64, exhaustive: bool = True):
     """Surrogate + batched-DEM planner: dataset from the rollout patches, GPU
     training, then planner iterations on held-out patches, each checked
@@ -349,7 +349,8 @@ def surrogate_bench(plain_rps: float, rounds: int = 400, epochs: int = 20, test_
     out = {
         "workload": f"{N} training rollouts ({rounds} rounds x 256 patches, random starts and controls), "
                     f"{epochs} epochs; planner: {samples} candidates scored by the surrogate, top {k} judged by "
-                    f"the batched DEM (+{k} more per round while the judged median error > 5 %, up to 256)",
+                    f"the batched DEM (+{k} more per round while the judged median error > 5 %, up to 256; after an "
+                    f"iteration that needed that fallback, all 256 in one launch)",
         "rollouts_scored_per_s": rps, "plain_gpu_rollouts_per_s": plain_rps, "vs_plain": rps / plain_rps,
         "reported_cost_error": "0: the planner reports only DEM-judged costs",
         "surrogate_val_rel_cost_err": val,
