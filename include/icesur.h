@@ -129,6 +129,14 @@ int icesur_ke_costmap(const float* d_pos, const float* d_radius, const float* d_
                       double ship_mass, double ke_cost_scale, float buffer_factor, float base_cost,
                       int32_t dilate_half, int32_t* d_work, float* d_out, void* stream);
 
+/* One MPPI update of the reference's planners (run_mppi,
+ * jax_mppi_planner.py:34-115): with beta = min(costs) and
+ * w_i = exp(-(costs[i] - beta) / temperature), d_out (D) =
+ * clip(sum_i w_i samples[i] / sum_i w_i, -1, 1) over d_samples (S, D).
+ * Fixed-order sums; d_work holds 2 + ceil(S / 256) * D floats. */
+int icesur_mppi_update(const float* d_costs, const float* d_samples, int32_t S, int32_t D, float temperature,
+                       float* d_work, float* d_out, void* stream);
+
 /* The same window from the elements' convex polygons: d_verts (n, max_v, 2)
  * world vertices (the first d_nv[e] used, either orientation), each cell
  * sampled at 4 x 4 points; coverage and thickness in the same fixed point

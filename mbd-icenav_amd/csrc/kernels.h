@@ -100,7 +100,9 @@ __device__ __forceinline__ void point_to_ship(const ShipPose& P, float px, float
     float t_bow = vclip(((-x_rel) - 0.6f) / 0.4f, 0.0f, 1.0f);
     float lhb = P.hb;
     float nx_bow = 0.0f, ny_bow = 0.0f;
-    if (in_bow) {
+    // the taper only matters where the point can be inside (rel_x <= hl holds
+    // in the bow, and lhb <= hb): elsewhere pen, nx, ny are 0 whatever lhb is
+    if (in_bow && rel_x >= -P.hl && fabsf(rel_y) <= P.hb) {
         float lhb_bow = P.hb * (1.0f - 0.9f * dm_powf(t_bow, 1.5f));
         lhb = vmax(lhb_bow, P.hb * 0.05f);
         float bow_angle = 0.3f + 0.5f * t_bow;

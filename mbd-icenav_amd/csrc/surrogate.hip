@@ -694,6 +694,59 @@ __global__ void k_ke_dilate(const int32_t* __restrict__ bits, int rows, int cols
     out[i] = m;
 }
 
+// ── MPPI update (run_mppi, jax_mppi_planner.py:34-115) ──────────────────────
+// beta = min c, w_i = exp(-(c_i - beta) / T), v = clip(sum_i w_i u_i / sum_i w_i,
+// -1, 1).  Fixed-order sums: 256 interleaved running sums added as a pairwise
+// tree (weights), samples in chunks of 256 (each chunk in order, the chunks in
+// order) for the weighted sum.
+__device__ __forceinline__ float lds_tree256(float* sh, float x, bool take_min) {
+    sh[threadIdx.x] = x;
+    __syncthreads();
+    for (int w = 1; w < 256; w *= 2) {
+        if ((threadIdx.x & (2 * w - 1)) == 0)
+            sh[threadIdx.x] = take_min ? fminf(sh[threadIdx.x], sh[threadIdx.x + w]) : sh[threadIdx.x] + sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    const float r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_mppi_stats(const float* __restrict__ c, int S, float T, float* __restrict__ st) {
+    __shared__ float sh[256];
+    float m = INFINITY;
+    for (int i = threadIdx.x; i < S; i += 256) m = fminf(m, c[i]);
+    const float beta = lds_tree256(sh, m, true);
+    float z = 0.0f;
+    for (int i = threadIdx.x; i < S; i += 256) z = z + expf(-(c[i] - beta) / T);
+    const float eta = lds_tree256(sh, z, false);
+    if (threadIdx.x == 0) { st[0] = beta; st[1] = eta; }
+}
+
+// block b: part[b][d] = sum over samples 256 b .. 256 b + 255 of w_i u[i][d], in order
+__global__ __launch_bounds__(256) void k_mppi_part(const float* __restrict__ c, const float* __restrict__ u, int S, int D,
+                                                   float T, const float* __restrict__ st, float* __restrict__ part) {
+    __shared__ float w[256];
+    const int i0 = blockIdx.x * 256, n = min(256, S - i0);
+    const float beta = st[0];
+    if ((int)threadIdx.x < n) w[threadIdx.x] = expf(-(c[i0 + threadIdx.x] - beta) / T);
+    __syncthreads();
+    for (int d = threadIdx.x; d < D; d += 256) {
+        float a = 0.0f;
+        for (int i = 0; i < n; ++i) a = a + w[i] * u[(size_t)(i0 + i) * D + d];
+        part[(size_t)blockIdx.x * D + d] = a;
+    }
+}
+
+__global__ void k_mppi_final(const float* __restrict__ part, int nb, int D, const float* __restrict__ st,
+                             float* __restrict__ out) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= D) return;
+    float a = 0.0f;
+    for (int b = 0; b < nb; ++b) a = a + part[(size_t)b * D + d];
+    out[d] = fminf(fmaxf(a / st[1], -1.0f), 1.0f);
+}
+
 }  // namespace icesur
 
 using namespace icesur;
@@ -898,6 +951,21 @@ int icesur_ke_costmap(const float* d_pos, const float* d_radius, const float* d_
                            d_work);
     hipLaunchKernelGGL(k_ke_dilate, dim3(grid1(cells)), dim3(256), 0, s, d_work, rows, cols, base_cost, dilate_half,
                        d_out);
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+int icesur_mppi_update(const float* d_costs, const float* d_samples, int32_t S, int32_t D, float temperature,
+                       float* d_work, float* d_out, void* stream) {
+    SUR_CHECK(S > 0 && D > 0 && temperature > 0.0f && d_costs && d_samples && d_work && d_out,
+              "icesur_mppi_update: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const int nb = (S + 255) / 256;
+    float* st = d_work;
+    float* part = d_work + 2;
+    hipLaunchKernelGGL(k_mppi_stats, dim3(1), dim3(256), 0, s, d_costs, S, temperature, st);
+    hipLaunchKernelGGL(k_mppi_part, dim3(nb), dim3(256), 0, s, d_costs, d_samples, S, D, temperature, st, part);
+    hipLaunchKernelGGL(k_mppi_final, dim3((D + 255) / 256), dim3(256), 0, s, part, nb, D, st, d_out);
     SUR_LAUNCHED();
     return ICEDEM_OK;
 }

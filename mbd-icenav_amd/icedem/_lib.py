@@ -116,6 +116,7 @@ SIGNATURES = {
                                     C.c_double, C.c_double, C.c_double, C.c_float, C.c_float, _i, _p, _p, _p]),
     "icedem_ship_rollouts": (C.c_int, [_p, _p, _i, _p, _i, _i, _i, _i, _p, _i, _i, C.c_float, C.c_float,
                                        C.c_float, C.c_float, _p, _i, _i, _p, _p]),
+    "icesur_mppi_update": (C.c_int, [_p, _p, _i, _i, C.c_float, _p, _p, _p]),
     "icesur_conv": (C.c_int, [_p, _i, _i, _i, _i, _p, _i, _p, _i, _p, _p, _p]),
     "icesur_adamw": (C.c_int, [_p, _p, _p, _p, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                _i, _p, _p]),

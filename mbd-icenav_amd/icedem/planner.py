@@ -405,6 +405,23 @@ def relative_cost_error(net: Surrogate, norm: Norm, data, idx) -> Dict[str, floa
     return {"median": float(np.median(rel)), "mean": float(np.mean(rel)), "p90": float(np.percentile(rel, 90))}
 
 
+def mppi_update(costs: torch.Tensor, samples: torch.Tensor, temperature: float) -> torch.Tensor:
+    """run_mppi's update (jax_mppi_planner.py:95-101) on the GPU
+    (`icesur_mppi_update`): costs (S,), samples (S, ...) -> the clipped
+    exp(-(c - min c) / temperature) weighted mean, shaped like one sample."""
+    lib = _lib.load()
+    c = costs.to(torch.float32).contiguous()
+    u = samples.to(torch.float32).contiguous()
+    S = int(c.shape[0])
+    D = int(u[0].numel())
+    work = torch.empty(2 + ((S + 255) // 256) * D, dtype=torch.float32, device=c.device)
+    out = torch.empty(D, dtype=torch.float32, device=c.device)
+    s = C.c_void_p(torch.cuda.current_stream(c.device).cuda_stream)
+    _lib.check(lib.icesur_mppi_update(_ptr(c), _ptr(u), S, D, float(temperature), _ptr(work), _ptr(out), s), None,
+               "icesur_mppi_update")
+    return out.view(u.shape[1:])
+
+
 class Planner:
     """One patch, one start state: S candidates scored by the surrogate, the
     best k judged by the batched DEM (k instances of the patch)."""
@@ -476,8 +493,11 @@ class Planner:
         out = rb.run(full, self.hold, self.frames).cpu().numpy()
         return rollout_cost(out[:m])
 
-    def iterate(self, nominal: np.ndarray, S: int = 16384, sigma: float = 0.2, seed: int = 0) -> Dict:
-        """One sampling iteration around `nominal` (n_knots, 2)."""
+    def iterate(self, nominal: np.ndarray, S: int = 16384, sigma: float = 0.2, seed: int = 0,
+                keep: bool = False) -> Dict:
+        """One sampling iteration around `nominal` (n_knots, 2).  With `keep`
+        the result also holds the candidates (S, n_knots, 2) and their costs
+        (S,): the surrogate's, replaced by the DEM's where judged."""
         rng = np.random.default_rng(seed)
         dev = self.judge.device
         cand = np.clip(nominal[None] + rng.normal(0.0, sigma, (S, self.nk, 2)), -1.0, 1.0).astype(np.float32)
@@ -514,10 +534,34 @@ class Planner:
             self.dem_costs(ctl[:1], width)
         dem = np.asarray(dem)
         b = int(np.argmin(dem))
-        return {"best_index": int(judged[b]), "best_cost": float(dem[b]), "best_controls": cand[judged[b]],
-                "judged": len(judged), "median_rel_err_judged": err,
-                "surrogate_s": t1 - t0, "judge_s": t2 - t1, "scored": S + len(judged),
-                "rollouts_per_s": (S + len(judged)) / (t2 - t0)}
+        out = {"best_index": int(judged[b]), "best_cost": float(dem[b]), "best_controls": cand[judged[b]],
+               "judged": len(judged), "median_rel_err_judged": err,
+               "surrogate_s": t1 - t0, "judge_s": t2 - t1, "scored": S + len(judged),
+               "rollouts_per_s": (S + len(judged)) / (t2 - t0)}
+        if keep:
+            costs = pred.float().clone()
+            costs[torch.as_tensor(judged, device=dev)] = torch.as_tensor(dem, dtype=torch.float32, device=dev)
+            out["costs"], out["candidates"] = costs, ctl
+        return out
+
+    def mppi(self, nominal: np.ndarray, iterations: int = 3, temperature: float = 0.1, S: int = 16384,
+             sigma: float = 0.2, seed: int = 0) -> Dict:
+        """The reference's MPPI loop (run_mppi, jax_mppi_planner.py:34-115)
+        with this planner's scores: each iteration samples S candidates
+        around the nominal, scores them (surrogate, DEM on the judged ones)
+        and moves the nominal to their exp(-(c - min c) / temperature)
+        weighted mean (`mppi_update`).  Returns the final nominal, its DEM
+        cost, and per-iteration (mean, min, best DEM) costs."""
+        v = np.asarray(nominal, np.float32)
+        hist = []
+        for it in range(iterations):
+            r = self.iterate(v, S=S, sigma=sigma, seed=seed + it, keep=True)
+            c = r["costs"]
+            hist.append({"mean": float(c.mean()), "min": float(c.min()), "best_dem": r["best_cost"],
+                         "judged": r["judged"]})
+            v = mppi_update(c, r["candidates"], temperature).cpu().numpy()
+        final = float(self.dem_costs(torch.as_tensor(v[None], device=self.judge.device))[0])
+        return {"nominal": v, "dem_cost": final, "history": hist}
 
     def random_baseline(self, nominal: np.ndarray, S: int = 16384, sigma: float = 0.2, seed: int = 0) -> Dict:
         """The same DEM budget without the surrogate: k candidates of the same

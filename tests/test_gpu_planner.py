@@ -165,3 +165,37 @@ def test_planner_reports_dem_costs_and_widens_its_judge():
     again = P.dem_costs(torch.as_tensor(r2["best_controls"][None], device="cuda"), 256)
     assert again[0] == r2["best_cost"]
     P.close()
+
+
+def test_mppi_update_matches_the_weighted_mean():
+    """run_mppi's update (jax_mppi_planner.py:95-101) against float64 NumPy."""
+    rng = np.random.default_rng(4)
+    for S, D, T in [(1000, 80, 0.1), (16384, 80, 0.5), (3, 7, 2.0)]:
+        c = rng.normal(10.0, 1.0, S).astype(np.float32)
+        u = np.clip(rng.normal(0.5, 0.3, (S, D)), -1, 1).astype(np.float32)
+        got = pl.mppi_update(torch.as_tensor(c, device="cuda"), torch.as_tensor(u, device="cuda"), T).cpu().numpy()
+        w = np.exp(-(c.astype(np.float64) - c.min()) / T)
+        ref = np.clip((w[:, None] * u).sum(0) / w.sum(), -1, 1)
+        np.testing.assert_allclose(got, ref, rtol=2e-5, atol=2e-6)
+    # a single dominant sample wins outright; clipping to [-1, 1]
+    c = np.array([5.0, 0.0, 5.0], np.float32)
+    u = np.array([[0.0, 0.0], [3.0, -0.25], [0.0, 0.0]], np.float32)
+    got = pl.mppi_update(torch.as_tensor(c, device="cuda"), torch.as_tensor(u, device="cuda"), 0.01).cpu().numpy()
+    np.testing.assert_allclose(got, [1.0, -0.25], atol=1e-6)
+
+
+def test_mppi_loop_runs_on_planner_scores():
+    sc = scenario.scene_patch(5001, 200, 0.4)
+    tmp = ro.RolloutBatch([sc], [0])
+    start = tmp.ship_init[0].clone()
+    tmp.close()
+    net = Surrogate(Arch(), seed=0)
+    norm = pl.Norm(np.zeros(3, np.float32), np.ones(3, np.float32))
+    P = pl.Planner(net, norm, sc, start, k=64, max_judged=64)
+    nominal = np.tile(np.array([0.7, 0.0], np.float32), (40, 1))
+    r = P.mppi(nominal, iterations=2, temperature=0.1, S=1024, seed=3)
+    assert r["nominal"].shape == (40, 2) and np.all(np.abs(r["nominal"]) <= 1.0)
+    assert len(r["history"]) == 2 and np.isfinite(r["dem_cost"])
+    again = P.dem_costs(torch.as_tensor(r["nominal"][None], device="cuda"))[0]
+    assert again == r["dem_cost"]
+    P.close()
