@@ -579,7 +579,7 @@ __global__ void k_raster_poly(const float2* __restrict__ verts, const int32_t* _
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (size_t)B * n) return;
     const int b = (int)(idx / n), e = (int)(idx % n);
-    const int m = nv[e];
+    const int m = min(nv[e], maxV);
     if (m < 3) return;
     const float px = pose[3 * b], py = pose[3 * b + 1], ph = pose[3 * b + 2];
     const float c = cosf(ph), s = sinf(ph);
