@@ -35,6 +35,7 @@ extern "C" {
 
 #define ICEDEM_MAX_VERTS 64     /* max_V (padded vertices per element) */
 #define ICEDEM_MAX_EDGE_SAMPLES 16
+#define ICEDEM_LANES8_MIN_ELEMENTS 32768   /* force_lanes = 0: 8 lanes per element from here */
 #define ICEDEM_SHIP_STATE_LEN 15 /* eta[3], nu[3], n_actual[6], e_int[3] */
 
 typedef struct IceDemCtx IceDemCtx;
@@ -122,6 +123,11 @@ typedef struct {
      * overflow goes to a list that every element also scans. */
     float grid_x0, grid_y0, grid_cell, grid_rsmall;
     int32_t grid_nx, grid_ny, grid_bucket, grid_bucket2;
+    /* lanes per element in the force pass (build choice, results identical):
+     * 16 or 8; 0 = 16 below ICEDEM_LANES8_MIN_ELEMENTS elements, else 8.
+     * A batch uses the value of its first context, with 0 deciding by the
+     * batch's total element count. */
+    int32_t force_lanes;
 } IceDemParams;
 
 /* Mutable state (IceState + ShipState, jax_ice_state.py:31-52), device

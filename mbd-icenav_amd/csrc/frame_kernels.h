@@ -5,14 +5,13 @@
 //   k_cull    once per call: near-ship test of every element into a work
 //             queue, insertion into the cell buckets of set 0
 //   k_near    one wavefront per queued element: hull pass 1 + 2, pre-pass,
-//             floe or/max (integer atomics).  The wave that finishes the
-//             last near element of a 256-element block writes that block's
-//             pairwise-tree partials; the wave that finishes the last such
-//             block forms the totals and the thrust scale.
+//             floe or/max (integer atomics).  No cross-workgroup hand-off.
 //   k_force   16 lanes per element: activation, submergence, bond forces,
-//             contacts, drag, integration.  nblk extra workgroups form the
-//             scaled ship-force partials; the last of them computes the ship
-//             reaction, floe eligibility and the ship step.
+//             contacts, drag, integration.  tail_groups(nblk) extra
+//             workgroups form the pairwise-tree partials of the near
+//             elements' ship forces, one wave per 256-element block; the last
+//             of them forms the totals, the thrust scale, the scaled totals,
+//             the ship reaction, floe eligibility and the ship step.
 //   k_post    per bond: fracture; per element: commit, histories, and the
 //             next frame's near test and cell insertion (the bucket set of
 //             the next frame's parity); clears the set this frame used.
@@ -24,14 +23,24 @@
 
 namespace icedem {
 
-constexpr int GRP = 16;                 // lanes per element in k_force
-constexpr int EPB = BLOCK / GRP;        // elements per k_force block
+// k_force gives each element a group of GR lanes, GR = 16 or 8 (a build
+// choice per context: 16 shortens one element's chain of dependent loads,
+// 8 halves the wave instructions spent per element on its serial parts).
+constexpr int GRP = 16;                 // lanes per element of the default force pass
 constexpr int MAXC2 = 16;               // contact partners kept per element (more: exact slow path)
+template <int GR> constexpr int epb() { return BLOCK / GR; }   // elements per k_force block
 
 __device__ __forceinline__ int32_t* near_count(const DevView* V) { return V->counters; }
-__device__ __forceinline__ int32_t* nearblk_count(const DevView* V) { return V->counters + 1; }
-__device__ __forceinline__ int32_t* near_ticket(const DevView* V) { return V->counters + 2; }
 __device__ __forceinline__ int32_t* force_ticket(const DevView* V) { return V->counters + 3; }
+
+// extra (tail) workgroups of a frame's first k_force: one wave per 256-element block
+__host__ __device__ constexpr int tail_groups(int nblk) { return (nblk + BLOCK / 64 - 1) / (BLOCK / 64); }
+// element workgroups of k_force for n elements at `lanes` lanes each
+__host__ __device__ constexpr int force_chunks(int n, int lanes) { return (n + BLOCK / lanes - 1) / (BLOCK / lanes); }
+// lanes per element: the requested 8 or 16, else chosen by element count
+__host__ __device__ constexpr int resolve_lanes(int req, int64_t n) {
+    return req ? req : (n >= ICEDEM_LANES8_MIN_ELEMENTS ? 8 : 16);
+}
 
 // Hand-offs between workgroups of one kernel (block partials -> the
 // workgroup that draws the last ticket): the producer writes with
@@ -404,17 +413,7 @@ __device__ __forceinline__ void cull_chunk(const DevView* __restrict__ V, int ch
         cell_insert(V, set, e, x, y, vx, vy, r);
     }
     const int cnt = __syncthreads_count(near ? 1 : 0);
-    if (threadIdx.x == 0) {
-        G(V->nearcnt)[chunk] = cnt;
-        G(V->blkdone)[chunk] = 0;
-        if (cnt > 0) {
-            atomicAdd(nearblk_count(V), 1);
-        } else {
-            G(V->part1)[chunk] = 0.0f;
-            G(V->part1)[V->nbpad + chunk] = 0.0f;
-            G(V->part1)[2 * V->nbpad + chunk] = 0.0f;
-        }
-    }
+    if (threadIdx.x == 0) G(V->nearcnt)[chunk] = cnt;
 }
 
 // sin/cos of the ship heading for the next k_near (scal[8..11])
@@ -493,7 +492,7 @@ __device__ __forceinline__ void near_elem(const DevView* __restrict__ V, const C
         PrePass o = prepass(P, pen, nxo, nyo, rad, thk, px, py, SP.sx, SP.sy);
         G(V->pen)[e] = pen; G(V->nx)[e] = nxo; G(V->ny)[e] = nyo;
         G(V->fm_raw)[e] = o.fm;
-        st_dev(V->soi_x + e, o.sx); st_dev(V->soi_y + e, o.sy); st_dev(V->mterm + e, o.m);   // read by near_block_done
+        G(V->soi_x)[e] = o.sx; G(V->soi_y)[e] = o.sy; G(V->mterm)[e] = o.m;
         if (pen > 0.0f) {
             const int f = G(V->floe)[e];
             atomicOr(&V->floe_coll[f], 1u);
@@ -502,69 +501,22 @@ __device__ __forceinline__ void near_elem(const DevView* __restrict__ V, const C
     }
 }
 
-// The near elements of block b are all done: its partials, and for the last
-// such block the totals and the thrust scale (jax_simulation.py:456-470).
-// One wave.
-__device__ __forceinline__ void near_block_done(const DevView* __restrict__ V, int b) {
-    const IceDemParams& P = V->p;
-    const int lane = threadIdx.x & 63, nbpad = V->nbpad;
-    float a, c, d;
-    block256_tree3(V, G(V->soi_x), G(V->soi_y), G(V->mterm), b, a, c, d, true);
-    int lastb = 0;
-    if (lane == 0) {
-        st_dev(V->part1 + b, a); st_dev(V->part1 + nbpad + b, c); st_dev(V->part1 + 2 * nbpad + b, d);
-        stores_done();
-        lastb = (atomicAdd(near_ticket(V), 1) == *nearblk_count(V) - 1) ? 1 : 0;
-    }
-    if (!__shfl(lastb, 0, 64)) return;
-    float tx, ty, tm;
-    if (nbpad <= 64) {   // the three loads in one round trip
-        const float l0 = lane < nbpad ? ld_dev(V->part1 + lane) : 0.0f;
-        const float l1 = lane < nbpad ? ld_dev(V->part1 + nbpad + lane) : 0.0f;
-        const float l2 = lane < nbpad ? ld_dev(V->part1 + 2 * nbpad + lane) : 0.0f;
-        tx = wave_tree_small(l0, nbpad); ty = wave_tree_small(l1, nbpad); tm = wave_tree_small(l2, nbpad);
-    } else {
-        tx = wave_tree_global(G(V->part1), nbpad, V->part1);
-        ty = wave_tree_global(G(V->part1) + nbpad, nbpad, V->part1 + nbpad);
-        tm = wave_tree_global(G(V->part1) + 2 * nbpad, nbpad, V->part1 + 2 * nbpad);
-    }
-    float scale = 1.0f;
-    if (P.has_thrust_cap) {
-        const float F_tot = sqrtf(tx * tx + ty * ty);
-        scale = (F_tot > P.max_ship_ice_force) ? (P.max_ship_ice_force / vmax(F_tot, 1e-6f)) : 1.0f;
-    }
-    if (lane == 0) { G(V->scal)[0] = scale; G(V->scal)[1] = tm; G(V->scal)[4] = tx; G(V->scal)[5] = ty; }
-}
-
-// workgroup lb of lg working on one context's near queue
+// workgroup lb of lg working on one context's near queue: one wave per near
+// element.  The totals over the near elements (jax_simulation.py:456-470)
+// are formed by the tail of the following k_force.
 __device__ __forceinline__ void near_kernel_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                                  int lb, int lg) {
-    const int lane = threadIdx.x & 63;
     const int wave = lb * (BLOCK / 64) + (threadIdx.x >> 6);
     const int nwaves = lg * (BLOCK / 64);
     const int count = *near_count(V);
     const int e0 = (wave < V->n) ? G(V->near_list)[wave] : 0;   // read beside the count
-    if (count == 0) {
-        if (wave == 0 && lane == 0) { G(V->scal)[0] = 1.0f; G(V->scal)[1] = 0.0f; G(V->scal)[4] = 0.0f; G(V->scal)[5] = 0.0f; }
-        return;
-    }
     if (wave >= count) return;
     const float* ship = D->s.d_ship;
     ShipPose SP;   // heading sin/cos from the frame's ship step
     SP.sx = ship[0]; SP.sy = ship[1];
     SP.s_neg = G(V->scal)[8]; SP.c_neg = G(V->scal)[9]; SP.s_h = G(V->scal)[10]; SP.c_h = G(V->scal)[11];
     SP.hl = V->p.half_len; SP.hb = V->p.half_beam;
-    for (int q = wave; q < count; q += nwaves) {
-        const int e = (q == wave) ? e0 : G(V->near_list)[q];
-        near_elem(V, D, SP, e);
-        int fin = 0;
-        if (lane == 0) {
-            const int b = e / BLOCK;
-            stores_done();
-            fin = (atomicAdd(&V->blkdone[b], 1) == G(V->nearcnt)[b] - 1) ? 1 : 0;
-        }
-        if (__shfl(fin, 0, 64)) near_block_done(V, e / BLOCK);
-    }
+    for (int q = wave; q < count; q += nwaves) near_elem(V, D, SP, (q == wave) ? e0 : G(V->near_list)[q]);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_near(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
@@ -646,60 +598,87 @@ __device__ __forceinline__ void floe_eligibility(const DevView* __restrict__ V, 
     }
 }
 
-// Extra workgroup b of k_force: the scaled partials of element block b; the
-// last of the nextra workgroups forms the ship reaction, the floe
-// eligibility and the ship step, and resets the near counters.
+// Extra workgroup b of k_force (tail_groups(nblk) of them): its wave w forms
+// the near-masked pairwise trees of soi_x, soi_y and the moment term over
+// element block 4b + w (jax_simulation.py:456-470).  The last of the extra
+// workgroups forms the totals and the thrust scale, the scaled partials and
+// their totals (:471-479) over the blocks holding near elements, the floe
+// eligibility and the ship step, and resets the near counters.  The thrust
+// scale is left in scal[0] for k_post.
 __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int k, int last,
                            int b, int nextra, int* am_last) {
     const IceDemParams& P = V->p;
-    const int nbpad = V->nbpad;
-    const bool capped = P.has_thrust_cap != 0;
-    if (capped && threadIdx.x < 64) {
-        float a = 0.0f, c = 0.0f;
-        if (G(V->nearcnt)[b] > 0) {
-            const float s = G(V->scal)[0];
-            a = block256_tree(V, G(V->soi_x), b, s, true);
-            c = block256_tree(V, G(V->soi_y), b, s, true);
-        }
-        if (threadIdx.x == 0) { st_dev(V->part2 + b, a); st_dev(V->part2 + nbpad + b, c); }
-    }
-    if (threadIdx.x == 0) {
+    const int nbpad = V->nbpad, nb = V->nblk;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    constexpr int NW = BLOCK / 64;
+    const int blk = b * NW + w;
+    if (blk < nb) {
+        float a = 0.0f, c = 0.0f, d = 0.0f;
+        if (G(V->nearcnt)[blk] > 0) block256_tree3(V, G(V->soi_x), G(V->soi_y), G(V->mterm), blk, a, c, d);
+        if (lane == 0) { st_dev(V->part1 + blk, a); st_dev(V->part1 + nbpad + blk, c); st_dev(V->part1 + 2 * nbpad + blk, d); }
         stores_done();
-        *am_last = (atomicAdd(force_ticket(V), 1) == nextra - 1) ? 1 : 0;
     }
     __syncthreads();
+    if (threadIdx.x == 0) *am_last = (atomicAdd(force_ticket(V), 1) == nextra - 1) ? 1 : 0;
+    __syncthreads();
     if (!*am_last) return;
-    const float scale = G(V->scal)[0];
-    float fx = 0.0f, fy = 0.0f;
-    if (threadIdx.x < 64) {
-        if (capped) {
-            fx = wave_tree_global(G(V->part2), nbpad, V->part2);
-            fy = wave_tree_global(G(V->part2) + nbpad, nbpad, V->part2 + nbpad);
-        } else {
-            fx = G(V->scal)[4];
-            fy = G(V->scal)[5];
+    // totals, in every wave
+    float tx, ty, tm;
+    if (nbpad <= 64) {   // the three loads in one round trip
+        const float l0 = lane < nbpad ? ld_dev(V->part1 + lane) : 0.0f;
+        const float l1 = lane < nbpad ? ld_dev(V->part1 + nbpad + lane) : 0.0f;
+        const float l2 = lane < nbpad ? ld_dev(V->part1 + 2 * nbpad + lane) : 0.0f;
+        tx = wave_tree_small(l0, nbpad); ty = wave_tree_small(l1, nbpad); tm = wave_tree_small(l2, nbpad);
+    } else {
+        tx = wave_tree_global(G(V->part1), nbpad, V->part1);
+        ty = wave_tree_global(G(V->part1) + nbpad, nbpad, V->part1 + nbpad);
+        tm = wave_tree_global(G(V->part1) + 2 * nbpad, nbpad, V->part1 + 2 * nbpad);
+    }
+    float scale = 1.0f, fx = tx, fy = ty;
+    if (P.has_thrust_cap) {
+        const float F_tot = sqrtf(tx * tx + ty * ty);
+        scale = (F_tot > P.max_ship_ice_force) ? (P.max_ship_ice_force / vmax(F_tot, 1e-6f)) : 1.0f;
+        // scaled partials: 64 blocks at a time, the blocks holding near
+        // elements dealt to the waves in turn, zeros for the others
+        for (int base = 0; base < nb; base += 64) {
+            const int q = base + lane;
+            const bool has = q < nb && G(V->nearcnt)[q] > 0;
+            if (w == 0 && q < nb && !has) { st_dev(V->part2 + q, 0.0f); st_dev(V->part2 + nbpad + q, 0.0f); }
+            uint64_t m = __ballot(has);
+            for (int r = 0; m != 0ull; ++r, m &= m - 1ull) {
+                if ((r & (NW - 1)) != w) continue;
+                const int qb = base + __builtin_ctzll(m);
+                const float a = block256_tree(V, G(V->soi_x), qb, scale, true);
+                const float c = block256_tree(V, G(V->soi_y), qb, scale, true);
+                if (lane == 0) { st_dev(V->part2 + qb, a); st_dev(V->part2 + nbpad + qb, c); }
+            }
         }
+        stores_done();
+        __syncthreads();
+        fx = wave_tree_global(G(V->part2), nbpad, V->part2);
+        fy = wave_tree_global(G(V->part2) + nbpad, nbpad, V->part2 + nbpad);
     }
     floe_eligibility(V, D, scale);
     if (threadIdx.x == 0) {
-        ship_frame(V, D, k, last, fx, fy, G(V->scal)[1]);
+        G(V->scal)[0] = scale;
+        ship_frame(V, D, k, last, fx, fy, tm);
         store_pose(V, D->s.d_ship);
         *near_count(V) = 0;
-        *nearblk_count(V) = 0;
-        *near_ticket(V) = 0;
     }
 }
 
 // ── k_force ──────────────────────────────────────────────────────────────────
 
+template <int GR>
 struct ForceLds {
-    int cj[EPB][MAXC2];
-    float cx[EPB][MAXC2], cy[EPB][MAXC2], cvx[EPB][MAXC2], cvy[EPB][MAXC2], cr[EPB][MAXC2];
-    int ord[EPB][MAXC2];
-    int grec[EPB][GRP];
-    int goff[EPB][GRP];
-    int gcnt[EPB][GRP];
-    float bf[EPB][3][GRP];
+    static constexpr int E = BLOCK / GR;
+    int cj[E][MAXC2];
+    float cx[E][MAXC2], cy[E][MAXC2], cvx[E][MAXC2], cvy[E][MAXC2], cr[E][MAXC2];
+    int ord[E][MAXC2];
+    int grec[E][GR];
+    int goff[E][GR];
+    int gcnt[E][GR];
+    float bf[E][3][GR];
 };
 
 __device__ __forceinline__ void wave_sync_lds() {
@@ -708,18 +687,27 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// group-wide (16 lanes = one DPP row) inclusive prefix sum
+// group-wide inclusive prefix sum over GR lanes (16 = one DPP row; shifts
+// into a row bring 0, and for 8-lane groups lanes below the shift take 0)
+template <int GR>
 __device__ __forceinline__ int group_incl_scan(int v) {
-    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);   // row_shr:1
-    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);   // row_shr:2
-    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);   // row_shr:4
-    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);   // row_shr:8
+    const int l = threadIdx.x & (GR - 1);
+    int t = __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v += (GR == 16 || l >= 1) ? t : 0;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);       // row_shr:2
+    v += (GR == 16 || l >= 2) ? t : 0;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);       // row_shr:4
+    v += (GR == 16 || l >= 4) ? t : 0;
+    if (GR == 16) v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);   // row_shr:8
     return v;
 }
 
-// value of the group's last lane (lane 15 of each 16-lane row)
+// value of the group's last lane: ds_swizzle bit-mask mode within each
+// 32-lane half, source lane (l & and_mask) | or_mask
+template <int GR>
 __device__ __forceinline__ int group_last(int v) {
-    return __builtin_amdgcn_ds_swizzle(v, 0x1F0);   // lane (l & 0x10) | 0xF of each 32-lane half
+    constexpr int pat = (0x1F & ~(GR - 1)) | ((GR - 1) << 5);
+    return __builtin_amdgcn_ds_swizzle(v, pat);
 }
 
 // The element's scan list: the fine cells [x0..x1] x [y0..y1] first, then
@@ -737,36 +725,36 @@ __device__ __forceinline__ int list_rec(const DevView* __restrict__ V, int set, 
     return coarse_rec(V, set, (S.v0 + r / S.W2) * V->nx2 + (S.u0 + r % S.W2));
 }
 
-// The occupied slots of the scan list, 16 cells at a time: the lanes read
+// The occupied slots of the scan list, GR cells at a time: the lanes read
 // the cell counts, a group prefix sum lays the occupied slots out, and each
 // lane takes one slot per round.  visit(have, j, kin, r) is called by every
 // lane of the group, the same number of times.
-template <class F>
+template <int GR, class F>
 __device__ __forceinline__ void scan_list(const DevView* __restrict__ V, int set, const ScanList& S,
-                                          ForceLds& L, int g, F&& visit) {
-    const int l = threadIdx.x & (GRP - 1);
-    for (int c0 = 0; c0 < S.ntot; c0 += GRP) {
+                                          ForceLds<GR>& L, int g, F&& visit) {
+    const int l = threadIdx.x & (GR - 1);
+    for (int c0 = 0; c0 < S.ntot; c0 += GR) {
         const int q = c0 + l;
         int rix = 0, cc = 0;
         if (q < S.ntot) {
             rix = list_rec(V, set, S, q);
             cc = min(*(GAS const int*)rec_ptr(V, rix), rec_cap(V, rix));
         }
-        const int incl = group_incl_scan(cc);
-        const int total = group_last(incl);
+        const int incl = group_incl_scan<GR>(cc);
+        const int total = group_last<GR>(incl);
         L.grec[g][l] = rix;
         L.goff[g][l] = incl - cc;
         L.gcnt[g][l] = cc;
         wave_sync_lds();
-        const int nq = min(GRP, S.ntot - c0);
-        for (int base = 0; base < total; base += GRP) {
+        const int nq = min(GR, S.ntot - c0);
+        for (int base = 0; base < total; base += GR) {
             const int f = base + l;
             bool have = f < total;
             Slot sl = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, -1};
             if (have) {
                 int qq = 0;
                 #pragma unroll
-                for (int t = 1; t < GRP; ++t) qq += (t < nq && L.goff[g][t] <= f) ? 1 : 0;
+                for (int t = 1; t < GR; ++t) qq += (t < nq && L.goff[g][t] <= f) ? 1 : 0;
                 const int sq = f - L.goff[g][qq];
                 have = sq >= 0 && sq < L.gcnt[g][qq];
                 if (have) sl = slot_load(rec_ptr(V, L.grec[g][qq]), sq);
@@ -781,18 +769,19 @@ __device__ __forceinline__ void scan_list(const DevView* __restrict__ V, int set
 // reference's scatter order; result valid in the group's lane 0.
 // (px, py, pvx, pvy, pr) is e's own state; partners come from bucket set
 // `set` (and, when buckets overflowed, from pos/vel/radius).
+template <int GR>
 __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ V, int set, GAS const float* __restrict__ pos,
                                     GAS const float* __restrict__ vel, int e, float px, float py, float pvx,
-                                    float pvy, float pr, int novf, ForceLds& L, int g,
+                                    float pvy, float pr, int novf, ForceLds<GR>& L, int g,
                                     float& fx, float& fy, float& fm) {
     const IceDemParams& P = V->p;
     const int n = V->n;
-    const int l = threadIdx.x & (GRP - 1);
-    const int gshift = (threadIdx.x & 63) & ~(GRP - 1);
+    const int l = threadIdx.x & (GR - 1);
+    const int gshift = (threadIdx.x & 63) & ~(GR - 1);
     int count = 0;
     auto push = [&](bool ov, int j, float x, float y, float vx, float vy, float r) {
         const uint64_t bal = __ballot(ov);
-        const uint32_t m = (uint32_t)(bal >> gshift) & ((1u << GRP) - 1u);
+        const uint32_t m = (uint32_t)(bal >> gshift) & ((1u << GR) - 1u);
         const int before = __builtin_popcount(m & ((1u << l) - 1u));
         if (ov && count + before < MAXC2) {
             const int s = count + before;
@@ -800,10 +789,13 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
         }
         count += __builtin_popcount(m);
     };
+    // the pair's operands in (lo, hi) order by selects, so that one inlined
+    // copy of the pair arithmetic serves both orders
     auto visit = [&](bool have, int j, float4 k4, float r) {
-        bool ov = false;
-        if (have && j != e)
-            ov = (e < j) ? overlap_v(P, px, py, pr, k4.x, k4.y, r) : overlap_v(P, k4.x, k4.y, r, px, py, pr);
+        const bool lo = e < j;
+        const bool ov = have && j != e &&
+                        overlap_v(P, lo ? px : k4.x, lo ? py : k4.y, lo ? pr : r, lo ? k4.x : px, lo ? k4.y : py,
+                                  lo ? r : pr);
         push(ov, j, k4.x, k4.y, k4.z, k4.w, r);
     };
     int x0, x1, y0, y1, u0 = 0, u1 = -1, v0 = 0, v1 = -1;
@@ -812,8 +804,8 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
     ScanList S;
     S.x0 = x0; S.y0 = y0; S.W = x1 - x0 + 1; S.nfine = S.W * (y1 - y0 + 1);
     S.u0 = u0; S.v0 = v0; S.W2 = u1 - u0 + 1; S.ntot = S.nfine + S.W2 * (v1 - v0 + 1);
-    scan_list(V, set, S, L, g, visit);
-    for (int base = 0; base < novf; base += GRP) {
+    scan_list<GR>(V, set, S, L, g, visit);
+    for (int base = 0; base < novf; base += GR) {
         const int f = base + l;
         bool ov = false;
         int j = -1;
@@ -821,14 +813,15 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
         if (f < novf) {
             j = G(V->ovf_list)[(size_t)set * V->ovf_cap + f];
             x = pos[j]; y = pos[n + j]; vx = vel[j]; vy = vel[n + j]; r = G(V->radius)[j];
-            if (j != e) ov = (e < j) ? overlap_v(P, px, py, pr, x, y, r) : overlap_v(P, x, y, r, px, py, pr);
+            const bool lo = e < j;
+            ov = j != e && overlap_v(P, lo ? px : x, lo ? py : y, lo ? pr : r, lo ? x : px, lo ? y : py, lo ? r : pr);
         }
         push(ov, j, x, y, vx, vy, r);
     }
     fx = 0.0f; fy = 0.0f; fm = 0.0f;
     if (count <= MAXC2) {
         // every partner was seen once: rank by rotated partner index
-        for (int i = l; i < count; i += GRP) {
+        for (int i = l; i < count; i += GR) {
             const int ki = rot_key(L.cj[g][i], e, n);
             int rank = 0;
             for (int q = 0; q < count; ++q) rank += (rot_key(L.cj[g][q], e, n) < ki) ? 1 : 0;
@@ -837,24 +830,25 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
         wave_sync_lds();
         // contributions in rank order, written over the partner arrays once
         // every lane has read its inputs; lane 0 adds them in order
-        constexpr int PER = (MAXC2 + GRP - 1) / GRP;
+        constexpr int PER = (MAXC2 + GR - 1) / GR;
         float ca[PER], cb[PER], cc[PER];
         #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int i = l + u * GRP;
+            const int i = l + u * GR;
             ca[u] = 0.0f; cb[u] = 0.0f; cc[u] = 0.0f;
             if (i < count) {
                 const int s = L.ord[g][i];
                 const int j = L.cj[g][s];
                 const float x = L.cx[g][s], y = L.cy[g][s], vx = L.cvx[g][s], vy = L.cvy[g][s], r = L.cr[g][s];
-                if (e < j) contrib_v(P, true, px, py, pvx, pvy, pr, x, y, vx, vy, r, ca[u], cb[u], cc[u]);
-                else contrib_v(P, false, x, y, vx, vy, r, px, py, pvx, pvy, pr, ca[u], cb[u], cc[u]);
+                const bool lo = e < j;
+                contrib_v(P, lo, lo ? px : x, lo ? py : y, lo ? pvx : vx, lo ? pvy : vy, lo ? pr : r,
+                          lo ? x : px, lo ? y : py, lo ? vx : pvx, lo ? vy : pvy, lo ? r : pr, ca[u], cb[u], cc[u]);
             }
         }
         wave_sync_lds();
         #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int i = l + u * GRP;
+            const int i = l + u * GR;
             if (i < count) { L.cx[g][i] = ca[u]; L.cy[g][i] = cb[u]; L.cvx[g][i] = cc[u]; }
         }
         wave_sync_lds();
@@ -920,16 +914,17 @@ __device__ __forceinline__ BondIn bond_in(const DevView* __restrict__ V, const C
 // order of the reference's two scatter-add passes).  act_e is e's activity
 // as the bond pass sees it; (t0, p0, in0) is the lane's entry l with its
 // state, read ahead by the caller.  Result valid in lane 0.
+template <int GR>
 __device__ __forceinline__ void bond_force_group(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                                  GAS const float* __restrict__ pos, int e, float px, float py,
                                                  bool act_e, int deg, int4 t0, float2 p0, BondIn in0, int mode,
-                                                 ForceLds& L, int g, float& fbx, float& fby, float& mb,
+                                                 ForceLds<GR>& L, int g, float& fbx, float& fby, float& mb,
                                                  bool& crack) {
-    const int l = threadIdx.x & (GRP - 1);
-    const int gshift = (threadIdx.x & 63) & ~(GRP - 1);
+    const int l = threadIdx.x & (GR - 1);
+    const int gshift = (threadIdx.x & 63) & ~(GR - 1);
     fbx = 0.0f; fby = 0.0f; mb = 0.0f;
     crack = false;
-    for (int base = 0; base < deg; base += GRP) {
+    for (int base = 0; base < deg; base += GR) {
         const int q = base + l;
         bool broken = false;
         if (q < deg) {
@@ -963,10 +958,10 @@ __device__ __forceinline__ void bond_force_group(const DevView* __restrict__ V, 
             L.bf[g][2][l] = role_j ? -cross : cross;
         }
         const uint64_t bal = __ballot(broken);
-        crack = crack || (((uint32_t)(bal >> gshift) & ((1u << GRP) - 1u)) != 0u);
+        crack = crack || (((uint32_t)(bal >> gshift) & ((1u << GR) - 1u)) != 0u);
         wave_sync_lds();
         if (l == 0) {
-            const int m = min(GRP, deg - base);
+            const int m = min(GR, deg - base);
             for (int t = 0; t < m; ++t) {
                 fbx = fbx + L.bf[g][0][t]; fby = fby + L.bf[g][1][t]; mb = mb + L.bf[g][2][t];
             }
@@ -975,16 +970,18 @@ __device__ __forceinline__ void bond_force_group(const DevView* __restrict__ V, 
     }
 }
 
-// elements [chunk*16, chunk*16+16), 16 lanes each (no block barrier inside).
-// mode 0: a frame sub-step (integrates); mode 1: forces out (fout, mout).
+// elements [chunk*E, chunk*E+E), E = 256 / GR, GR lanes each (no block
+// barrier inside).  mode 0: a frame sub-step (integrates); mode 1: forces
+// out (fout, mout).
+template <int GR>
 __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                            int k, int sub, int src_is_state, int mode, int set,
                            const float* __restrict__ fship_in, float* __restrict__ fout, float* __restrict__ mout,
-                           ForceLds& L, int chunk) {
+                           ForceLds<GR>& L, int chunk) {
     const IceDemParams& P = V->p;
     const int n = V->n;
-    const int g = threadIdx.x / GRP, l = threadIdx.x & (GRP - 1);
-    const int e = chunk * EPB + g;
+    const int g = threadIdx.x / GR, l = threadIdx.x & (GR - 1);
+    const int e = chunk * epb<GR>() + g;
     if (e >= n) return;                       // group-uniform
     GAS const float* pos = G(src_is_state ? D->s.d_pos : V->nxt_pos);
     GAS const float* vel = G(src_is_state ? D->s.d_vel : V->nxt_vel);
@@ -1012,7 +1009,6 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
     const bool is_near = first && G(V->nearf)[e] != 0;
     float pen = 0.0f, fmr = 0.0f, sx_ = 0.0f, sy_ = 0.0f, nxe = 0.0f, nye = 0.0f;
     if (is_near) { pen = G(V->pen)[e]; fmr = G(V->fm_raw)[e]; sx_ = G(V->soi_x)[e]; sy_ = G(V->soi_y)[e]; nxe = G(V->nx)[e]; nye = G(V->ny)[e]; }
-    const float tscale = first ? G(V->scal)[0] : 1.0f;
     float vwx = 0.0f, vwy = 0.0f;
     GAS const float* vwp = G(D->in.d_v_water);
     if (vwp) {
@@ -1025,10 +1021,10 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
     BondIn bin0 = {0.0f, 0.0f, 0.0f, false, false};
     if (l < deg) bin0 = bond_in(V, D, pos, bt0, mode);   // read ahead of the contact pass
     float fix, fiy, mi;
-    contact_force_group(V, set, pos, vel, e, px, py, vx, vy, r, novf, L, g, fix, fiy, mi);
+    contact_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, L, g, fix, fiy, mi);
     float fbx, fby, mb;
     bool crack;
-    bond_force_group(V, D, pos, e, px, py, mode == 1 ? act_raw : act_bond, deg, bt0, bp0, bin0, mode, L, g,
+    bond_force_group<GR>(V, D, pos, e, px, py, mode == 1 ? act_raw : act_bond, deg, bt0, bp0, bin0, mode, L, g,
                      fbx, fby, mb, crack);
     if (l != 0) return;
     float shx, shy, subv;
@@ -1040,7 +1036,7 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
     } else if (sub == 0) {
         act = act_bond;
         G(D->s.d_active)[e] = act ? 1 : 0;
-        G(V->fm)[e] = P.has_thrust_cap ? fmr * tscale : fmr;
+        G(V->fm)[e] = fmr;   // k_post applies the frame's thrust scale
         const bool hc = pen > 0.0f;
         const float pen_c = vmin(pen, P.submergence_pen_cap);
         float ns = hc ? vmin(sub0 + P.submergence_rate * pen_c * P.dt, h)
@@ -1122,32 +1118,41 @@ __device__ __forceinline__ int chunk_of(int w, int total) {
 
 // workgroup lb of lg = nextra + element chunks of one context's force pass;
 // xcd: lb is the hardware workgroup id (chunks spread XCD by XCD)
+template <int GR>
 __device__ __forceinline__ void force_kernel_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                                   int k, int sub, int src_is_state, int mode, int set, int last,
                                                   int nextra, int lb, int lg, bool xcd,
                                                   const float* __restrict__ fship_in, float* __restrict__ fout,
-                                                  float* __restrict__ mout, ForceLds& L, int* am_last) {
+                                                  float* __restrict__ mout, ForceLds<GR>& L, int* am_last) {
     if (lb < nextra) {
         force_tail(V, D, k, last, lb, nextra, am_last);
         return;
     }
     const int nf = lg - nextra;
-    force_body(V, D, k, sub, src_is_state, mode, set, fship_in, fout, mout, L,
-               xcd ? chunk_of(lb - nextra, nf) : lb - nextra);
+    force_body<GR>(V, D, k, sub, src_is_state, mode, set, fship_in, fout, mout, L,
+                   xcd ? chunk_of(lb - nextra, nf) : lb - nextra);
 }
 
+template <int GR>
 __global__ __launch_bounds__(BLOCK) void k_force(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                                  int k, int sub, int src_is_state, int mode, int set, int last,
                                                  int nextra,
                                                  const float* __restrict__ fship_in,
                                                  float* __restrict__ fout, float* __restrict__ mout) {
-    __shared__ ForceLds L;
+    __shared__ ForceLds<GR> L;
     __shared__ int am_last;
-    force_kernel_body(V, D, k, sub, src_is_state, mode, set, last, nextra, (int)blockIdx.x, (int)gridDim.x, true,
+    force_kernel_body<GR>(V, D, k, sub, src_is_state, mode, set, last, nextra, (int)blockIdx.x, (int)gridDim.x, true,
                       fship_in, fout, mout, L, &am_last);
 }
 
 // ── k_post: fracture (jax_simulation.py:534-656), commit, histories, next cull ─
+
+// contact force magnitude of element i after the frame's thrust scale
+// (jax_simulation.py:456-470): fm holds it unscaled, scal[0] the scale
+__device__ __forceinline__ float fm_scaled(const DevView* __restrict__ V, int i) {
+    const float f = G(V->fm)[i];
+    return V->p.has_thrust_cap ? f * G(V->scal)[0] : f;
+}
 
 // Thread t of chunk t / 256 (every thread of the 256-thread block calls):
 // bond t, element t, cell t of the set just used, floe t.  With `cull`, the
@@ -1172,7 +1177,7 @@ __device__ __forceinline__ void post_body(const DevView* __restrict__ V, const C
         bool floe_ok = elig[G(V->floe)[i]] && elig[G(V->floe)[j]];
         const float hi_ = G(V->thick)[i], hj = G(V->thick)[j];
         bool breakable = (hi_ <= P.max_fracture_thickness) && (hj <= P.max_fracture_thickness);
-        const float fmi = G(V->fm)[i], fmj = G(V->fm)[j];
+        const float fmi = fm_scaled(V, i), fmj = fm_scaled(V, j);
         bool near = (fmi > 0.0f) || (fmj > 0.0f);
         bool at_crack = G(V->crack)[i] || G(V->crack)[j];
         bool dpdem_ok = both && floe_ok && (near || at_crack) && breakable;
@@ -1231,7 +1236,7 @@ __device__ __forceinline__ void post_body(const DevView* __restrict__ V, const C
         if (D->h.d_submergence) G(D->h.d_submergence)[(size_t)k * n + t] = G(D->s.d_sub)[t];
         if (last) {
             if (D->dg.d_pen_max) G(D->dg.d_pen_max)[t] = G(V->nearf)[t] ? G(V->pen)[t] : 0.0f;
-            if (D->dg.d_force_mag) G(D->dg.d_force_mag)[t] = G(V->fm)[t];
+            if (D->dg.d_force_mag) G(D->dg.d_force_mag)[t] = fm_scaled(V, t);
         }
     }
     clear_cells(V, set, t);
@@ -1263,12 +1268,13 @@ __global__ __launch_bounds__(BLOCK) void kb_near(const DevView* __restrict__ vie
     near_kernel_body(views + m.x, descs + m.x, m.y, m.z);
 }
 
+template <int GR>
 __global__ __launch_bounds__(BLOCK) void kb_force(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
                                                   const int4* __restrict__ map, int k, int last, int set) {
-    __shared__ ForceLds L;
+    __shared__ ForceLds<GR> L;
     __shared__ int am_last;
     const int4 m = map[blockIdx.x];
-    force_kernel_body(views + m.x, descs + m.x, k, 0, 1, 0, set, last, m.w, m.y, m.z, false, nullptr, nullptr,
+    force_kernel_body<GR>(views + m.x, descs + m.x, k, 0, 1, 0, set, last, m.w, m.y, m.z, false, nullptr, nullptr,
                       nullptr, L, &am_last);
 }
 
@@ -1320,25 +1326,24 @@ __device__ __forceinline__ void mid_body(const DevView* __restrict__ V, const Ca
         fx = lds_tree_sum(V->part2, nb, nbpad, la, lb);
         fy = lds_tree_sum(V->part2 + nbpad, nb, nbpad, la, lb);
     }
-    if (threadIdx.x == 0) { G(V->scal)[0] = scale; G(V->scal)[1] = tm; }
+    if (threadIdx.x == 0) G(V->scal)[0] = scale;
     floe_eligibility(V, D, scale);
     __syncthreads();
     if (threadIdx.x == 0) {
         ship_frame(V, D, k, last, fx, fy, tm);
         *near_count(V) = 0;
-        *nearblk_count(V) = 0;
     }
     __syncthreads();
 }
 
 __global__ __launch_bounds__(BLOCK) void k_scene_rollouts(const DevView* __restrict__ views,
                                                           const CallDesc* __restrict__ descs, int frames) {
-    __shared__ ForceLds L;
+    __shared__ ForceLds<GRP> L;
     __shared__ float la[256], lb[256];
     const DevView* V = views + blockIdx.x;
     const CallDesc* D = descs + blockIdx.x;
     const int n = V->n, nblk = V->nblk;
-    const int fch = (n + EPB - 1) / EPB;
+    const int fch = (n + epb<GRP>() - 1) / epb<GRP>();
     const int post = max(max(V->nb, n), max(V->G, V->nf));
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int c = 0; c < nblk; ++c) {
@@ -1362,7 +1367,7 @@ __global__ __launch_bounds__(BLOCK) void k_scene_rollouts(const DevView* __restr
         __syncthreads();
         mid_body(V, D, la, lb, k, last);
         for (int c = 0; c < fch; ++c)
-            force_body(V, D, k, 0, 1, 0, set, nullptr, nullptr, nullptr, L, c);
+            force_body<GRP>(V, D, k, 0, 1, 0, set, nullptr, nullptr, nullptr, L, c);
         __syncthreads();
         for (int base = 0; base < post; base += BLOCK)
             post_body(V, D, k, 1, last, base + threadIdx.x, base / BLOCK, set, last ? 0 : 1);

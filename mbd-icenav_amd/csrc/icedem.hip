@@ -428,6 +428,9 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
         set_err(nullptr, "bad grid"); return ICEDEM_E_INVALID;
     }
     if (prm->n_ice_steps < 1) { set_err(nullptr, "n_ice_steps < 1"); return ICEDEM_E_INVALID; }
+    if (prm->force_lanes != 0 && prm->force_lanes != 8 && prm->force_lanes != 16) {
+        set_err(nullptr, "force_lanes must be 0, 8 or 16"); return ICEDEM_E_INVALID;
+    }
     for (int b = 0; b < st->n_bonds; ++b) {
         if (st->h_bond_i[b] < 0 || st->h_bond_i[b] >= n || st->h_bond_j[b] < 0 || st->h_bond_j[b] >= n) {
             set_err(nullptr, "bond index out of range"); return ICEDEM_E_INVALID;
@@ -446,6 +449,7 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     H.n = n; H.nb = st->n_bonds; H.nf = st->n_floes; H.maxv = st->max_v; H.S = st->n_hull;
     H.G = prm->grid_nx * prm->grid_ny;
     H.nblk = nblk; H.nbpad = nbpad;
+    H.lanes = resolve_lanes(prm->force_lanes, n);
     if (prm->grid_bucket < 1 || (int64_t)H.G * prm->grid_bucket > (1LL << 28)) {
         set_err(nullptr, "bad grid_bucket"); delete ctx; return ICEDEM_E_INVALID;
     }
@@ -520,7 +524,7 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     AL(soi_x, float, n); AL(soi_y, float, n); AL(mterm, float, n); AL(fm, float, n);
     AL(shipf, float, 2 * (size_t)n);
     AL(nearf, uint8_t, n); AL(near_list, int32_t, n);
-    AL(nearcnt, int32_t, nblk); AL(blkdone, int32_t, nblk); AL(counters, int32_t, 4);
+    AL(nearcnt, int32_t, nblk); AL(counters, int32_t, 4);
     AL(crack, uint8_t, n);
     {   // bucket records of both grids in one allocation: fine sets, then coarse
         auto rec_bytes = [](int64_t cap) { return ((16 + 24 * cap + 127) / 128) * 128; };
@@ -552,7 +556,6 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     HIP_TRY(hipMemset(H.err, 0, sizeof(int32_t) * 4));
     HIP_TRY(hipMemset(H.nearf, 0, n));
     HIP_TRY(hipMemset(H.nearcnt, 0, sizeof(int32_t) * nblk));
-    HIP_TRY(hipMemset(H.blkdone, 0, sizeof(int32_t) * nblk));
     {
         DevView* d;
         rc = dalloc(ctx, &d, 1);
@@ -633,8 +636,8 @@ static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, LaunchTimer*
     const DevView* V = ctx->dv;
     const CallDesc* D = ctx->dd;
     const int nblk = H.nblk;
-    const int fblk = (H.n + EPB - 1) / EPB;
-    const int nextra = H.p.has_thrust_cap ? nblk : 1;
+    const int fblk = force_chunks(H.n, H.lanes);
+    const int nextra = tail_groups(nblk);
     const int near_blocks = std::max(1, std::min(256, (H.n + 3) / 4));
     const int steps = H.p.n_ice_steps;
     const int post = std::max(std::max(H.nb, H.n), std::max(std::max(H.G, H.G2), H.nf));
@@ -655,8 +658,8 @@ static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, LaunchTimer*
                 rc |= launch(lt, 1, k_cellkey, dim3(nblk), dim3(BLOCK), stream, V, D, sp, sv, odd ? 0 : 1, set);
             }
             const int extra = (s == 0) ? nextra : 0;
-            rc |= launch(lt, 1, k_force, dim3(fblk + extra), dim3(BLOCK), stream, V, D, k, s,
-                         (s % 2 == 0) ? 1 : 0, 0, set, last, extra,
+            rc |= launch(lt, 1, H.lanes == 8 ? k_force<8> : k_force<16>, dim3(fblk + extra), dim3(BLOCK), stream,
+                         V, D, k, s, (s % 2 == 0) ? 1 : 0, 0, set, last, extra,
                          (const float*)nullptr, (float*)nullptr, (float*)nullptr);
         }
         rc |= launch(lt, 2, k_post, dim3(nbpost), dim3(BLOCK), stream, V, D, k, (steps % 2 == 1) ? 1 : 0, last,
@@ -751,6 +754,7 @@ struct IceDemBatch {
     // the instance, the instance's workgroup count, the instance's nextra)
     int4* map_cull = nullptr; int4* map_near = nullptr; int4* map_force = nullptr; int4* map_post = nullptr;
     int n_cull = 0, n_near = 0, n_force = 0, n_post = 0;
+    int lanes = GRP;             // lanes per element of kb_force
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;   // (frames, mode)
     std::map<std::pair<int, int>, hipGraph_t> graph_defs;
     std::string err;
@@ -789,11 +793,16 @@ int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** ou
     IceDemBatch* b = new IceDemBatch();
     b->device = ctxs[0]->device;
     b->n = n_inst;
+    {
+        int64_t total = 0;
+        for (int i = 0; i < n_inst; ++i) total += ctxs[i]->hv.n;
+        b->lanes = resolve_lanes(ctxs[0]->hv.p.force_lanes, total);
+    }
     std::vector<int4> mc, mn, mf, mp;
     for (int i = 0; i < n_inst; ++i) {
         const DevView& H = ctxs[i]->hv;
-        const int fblk = (H.n + EPB - 1) / EPB;
-        const int nextra = H.p.has_thrust_cap ? H.nblk : 1;
+        const int fblk = force_chunks(H.n, b->lanes);
+        const int nextra = tail_groups(H.nblk);
         // one wavefront per 16 elements (a quarter of a patch is near the ship at most)
         const int nnear = std::max(1, std::min(64, (H.n + 63) / 64));
         const int post = std::max(std::max(H.nb, H.n), std::max(std::max(H.G, H.G2), H.nf));
@@ -855,7 +864,7 @@ static void batch_launch(IceDemBatch* b, int frames, hipStream_t stream) {
         const int set = k & 1;
         hipLaunchKernelGGL(kb_near, dim3(b->n_near), dim3(BLOCK), 0, stream, b->dviews, b->ddesc,
                            (const int4*)b->map_near);
-        hipLaunchKernelGGL(kb_force, dim3(b->n_force), dim3(BLOCK), 0, stream, b->dviews, b->ddesc,
+        hipLaunchKernelGGL(b->lanes == 8 ? kb_force<8> : kb_force<16>, dim3(b->n_force), dim3(BLOCK), 0, stream, b->dviews, b->ddesc,
                            (const int4*)b->map_force, k, last, set);
         hipLaunchKernelGGL(kb_post, dim3(b->n_post), dim3(BLOCK), 0, stream, b->dviews, b->ddesc,
                            (const int4*)b->map_post, k, last, set);
@@ -936,10 +945,10 @@ int icedem_substep_forces(IceDemCtx* ctx, const IceDemState* state, const float*
     rc = put_desc(ctx, cd, stream);
     if (rc) return rc;
     const DevView& H = ctx->hv;
-    const int fblk = (H.n + EPB - 1) / EPB;
+    const int fblk = force_chunks(H.n, H.lanes);
     hipLaunchKernelGGL(k_cellkey, dim3(H.nblk), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, (const float*)nullptr,
                        (const float*)nullptr, 1, 0);
-    hipLaunchKernelGGL(k_force, dim3(fblk), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, 0, 0, 1, 1, 0, 0, 0,
+    hipLaunchKernelGGL(H.lanes == 8 ? k_force<8> : k_force<16>, dim3(fblk), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, 0, 0, 1, 1, 0, 0, 0,
                        d_ship_forces, d_forces, d_moments);
     hipLaunchKernelGGL(k_clear_cells, dim3((std::max(H.G, H.G2) + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, ctx->dv, 0);
     HIP_TRY(hipGetLastError());

@@ -50,12 +50,13 @@ struct DevView {
     const int4* btab;         // (n W) bond, other element, floe of other, role (0 = i, 1 = j)
     const float2* bprm;       // (n W) rest length, stiffness E * h_avg * factor
     // per-frame scratch
+    // (fm: contact force magnitude before the frame's thrust scale)
     float* pen; float* nx; float* ny; float* fm_raw; float* soi_x; float* soi_y; float* mterm; float* fm;
     uint8_t* nearf;           // (n) element passed the hull cull this frame
     int32_t* near_list;       // (n) queue of near elements
     int32_t* nearcnt;         // (nblk) near elements of each 256-element block
-    int32_t* blkdone;         // (nblk) near elements of the block that k_near has finished
-    int32_t* counters;        // [0] near elements, [1] blocks holding one, [2] k_near ticket, [3] k_force ticket
+    int32_t* counters;        // [0] near elements, [3] k_force tail ticket ([1], [2] unused)
+    int32_t lanes;            // lanes per element in k_force (8 or 16)
     float* shipf;             // (2n) ship forces on ice, kept for sub-steps > 0
     uint8_t* crack;           // (n) element touches a broken bond
     // Two grids, each with two bucket sets (frame parity).  A bucket is one
@@ -76,8 +77,7 @@ struct DevView {
     uint32_t* floe_coll; uint32_t* floe_maxfm;                      // (nf)
     float* part1;             // (3 * nbpad) block partials: sum soi_x, soi_y, moment
     float* part2;             // (2 * nbpad) block partials: sum of scaled soi
-    float* scal;              // [0] thrust scale, [1] M_ice_on_ship, [4..5] unscaled sum of soi,
-                              // [8..11] sin/cos of -heading and heading
+    float* scal;              // [0] thrust scale of the frame, [8..11] sin/cos of -heading and heading
     float* nxt_pos; float* nxt_vel; float* nxt_rot; float* nxt_omega;
     int32_t* err;
 };

@@ -61,6 +61,7 @@ class IceDemParams(C.Structure):
         ("T_n", _f), ("vc_surge", _f), ("vc_sway", _f), ("vc_yaw", _f),
         ("grid_x0", _f), ("grid_y0", _f), ("grid_cell", _f), ("grid_rsmall", _f),
         ("grid_nx", _i), ("grid_ny", _i), ("grid_bucket", _i), ("grid_bucket2", _i),
+        ("force_lanes", _i),
     ]
 
 

@@ -96,8 +96,11 @@ def choose_grid(centroids: np.ndarray, radii: np.ndarray, dilation: float = 0.0,
 
 def make_params(static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_length, ship_beam, n_ice_steps,
                 dt, dt_ice, use_auto_icenav, B_mat, T_n, grid=None, ship_mode: int = 0,
-                ctrl_thrust_max: float = 2e6, ctrl_yaw_rate_max: float = 0.15) -> _lib.IceDemParams:
-    """Scalar parameters rounded to float32 in the reference's order of use."""
+                ctrl_thrust_max: float = 2e6, ctrl_yaw_rate_max: float = 0.15,
+                force_lanes: int = 0) -> _lib.IceDemParams:
+    """Scalar parameters rounded to float32 in the reference's order of use.
+    force_lanes: lanes per element in the force pass (8 or 16; 0 lets the
+    library choose by element count); the results do not depend on it."""
     p = _lib.IceDemParams()
     hl = ship_length / 2.0
     hb = ship_beam / 2.0
@@ -185,6 +188,9 @@ def make_params(static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_len
     p.grid_x0, p.grid_y0, p.grid_cell, p.grid_rsmall = F32(grid[0]), F32(grid[1]), F32(grid[2]), F32(grid[3])
     p.grid_nx, p.grid_ny, p.grid_bucket = int(grid[4]), int(grid[5]), int(grid[6])
     p.grid_bucket2 = int(grid[7]) if len(grid) > 7 else 0
+    if force_lanes not in (0, 8, 16):
+        raise ValueError("force_lanes must be 0, 8 or 16")
+    p.force_lanes = int(force_lanes)
     return p
 
 
@@ -203,7 +209,7 @@ class DemEngine:
     def __init__(self, static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_length, ship_beam,
                  n_ice_steps, dt, dt_ice, use_auto_icenav, B_mat, T_n, centroids=None, device=None, grid=None,
                  cell_size=None, ship_mode: int = 0, ctrl_thrust_max: float = 2e6, ctrl_yaw_rate_max: float = 0.15,
-                 grid_margin: float = 60.0, small_quantile: float = 0.98):
+                 grid_margin: float = 60.0, small_quantile: float = 0.98, force_lanes: int = 0):
         if not torch.cuda.is_available():
             raise _lib.IceDemError("icedem needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.lib = _lib.load()
@@ -222,7 +228,8 @@ class DemEngine:
         self.grid = grid
         self.params = make_params(static, cfg, ship_dyn, ship_length, ship_beam, n_ice_steps, dt, dt_ice,
                                   use_auto_icenav, B_mat, T_n, grid=grid, ship_mode=ship_mode,
-                                  ctrl_thrust_max=ctrl_thrust_max, ctrl_yaw_rate_max=ctrl_yaw_rate_max)
+                                  ctrl_thrust_max=ctrl_thrust_max, ctrl_yaw_rate_max=ctrl_yaw_rate_max,
+                                  force_lanes=force_lanes)
         lp = np.ascontiguousarray(static.local_polys_padded, dtype=F32)
         self._host = dict(
             lp=lp,

@@ -32,8 +32,9 @@ def test_frame_parity_hexagon():
     _run(case, 60)
 
 
-def test_frame_parity_c0_contact_window():
-    case = Case.make("c0", n_frames=1100)
+@pytest.mark.parametrize("lanes", [0, 8])
+def test_frame_parity_c0_contact_window(lanes):
+    case = Case.make("c0", n_frames=1100, force_lanes=lanes)
     _run(case, 1100, check_every=5, start_check=700)
 
 
@@ -44,8 +45,9 @@ def test_frame_parity_c0_current_dilation():
     _run(case, 900, check_every=10, start_check=700)
 
 
-def test_frame_parity_field():
-    case = Case.make("c1small", n_frames=300)
+@pytest.mark.parametrize("lanes", [0, 8])
+def test_frame_parity_field(lanes):
+    case = Case.make("c1small", n_frames=300, force_lanes=lanes)
     _run(case, 300, check_every=3)
 
 

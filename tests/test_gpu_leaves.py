@@ -24,9 +24,9 @@ def test_contact_pairs_exact(name):
     np.testing.assert_array_equal(gj.cpu().numpy(), oj)
 
 
-@pytest.mark.parametrize("name", ["c1small", "ring"])
-def test_substep_forces(name):
-    case = Case.make(name)
+@pytest.mark.parametrize("name,lanes", [("c1small", 0), ("ring", 0), ("c1small", 8), ("ring", 8)])
+def test_substep_forces(name, lanes):
+    case = Case.make(name, force_lanes=lanes)
     eng = case.engine()
     o = case.oracle()
     st, ship = case.gpu_initial()

@@ -16,11 +16,11 @@ from oracle import model as om, step as ostep
 pytestmark = pytest.mark.gpu
 
 
-def _batch(seeds_sizes, inst):
+def _batch(seeds_sizes, inst, force_lanes=0):
     from icedem import scenario
     from icedem.rollouts import RolloutBatch
     scenes = [scenario.scene_patch(s, n, c) for s, n, c in seeds_sizes]
-    return RolloutBatch(scenes, inst), scenes
+    return RolloutBatch(scenes, inst, force_lanes=force_lanes), scenes
 
 
 def _oracle_for(rb, scene):
@@ -35,10 +35,10 @@ def _oracle_for(rb, scene):
                             ctrl_yaw_rate_max=rb.ship.yaw_rate_max)
 
 
-@pytest.mark.parametrize("mode", ["frames", "scenes"])
-def test_batched_equals_sequential_and_oracle(mode):
+@pytest.mark.parametrize("mode,lanes", [("frames", 0), ("scenes", 0), ("frames", 8)])
+def test_batched_equals_sequential_and_oracle(mode, lanes):
     from icedem.rollouts import nominal_controls
-    rb, scenes = _batch([(1000, 60, 0.3), (1001, 150, 0.4), (1002, 90, 0.5)], [0, 1, 2, 0])
+    rb, scenes = _batch([(1000, 60, 0.3), (1001, 150, 0.4), (1002, 90, 0.5)], [0, 1, 2, 0], force_lanes=lanes)
     rb.set_mode(mode)
     frames, hold, nk = 120, 5, 24
     ctl = nominal_controls(rb.R, nk, seed=3)
