@@ -375,12 +375,12 @@ def surrogate_bench(plain_rps: float, rounds: int = 800, epochs: int = 20, test_
 
 
 def batched_rollouts_only(R: int = 256, frames: int = 400, hold: int = 10, n_knots: int = 40, reps: int = 3,
-                          force_lanes: int = 0):
+                          force_lanes: int = 0, cell_factor: float = 1.0):
     """The batched-rollout launch alone (for profiling its kernels)."""
     import torch
     from icedem import rollouts as ro
     scenes = ro.rollout_patches(R, workers=min(16, os.cpu_count() or 1))
-    rb = ro.RolloutBatch(scenes, list(range(R)), force_lanes=force_lanes)
+    rb = ro.RolloutBatch(scenes, list(range(R)), force_lanes=force_lanes, cell_size_factor=cell_factor)
     rb.set_mode("frames")
     ctl = torch.as_tensor(ro.nominal_controls(R, n_knots, seed=0))
     rb.reset()
@@ -453,6 +453,8 @@ def main():
                     help="only the batched rollouts (R patches x 400 frames, frames mode), three timed runs")
     ap.add_argument("--force-lanes", type=int, default=0, choices=[0, 8, 16],
                     help="lanes per element of the force pass (0: the library's choice by element count)")
+    ap.add_argument("--cell-factor", type=float, default=1.0,
+                    help="fine grid cell as a multiple of 2 (r_s + delta) (frames) or of the batch's default cell")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB five times (reference kernel for the PMC byte counters)")
     args = ap.parse_args()
@@ -466,7 +468,8 @@ def main():
         print(json.dumps({"copy_bytes_each_way": x.numel() * 4, "copies": 5}), flush=True)
         return
     if args.rollouts_only:
-        print(json.dumps(batched_rollouts_only(args.rollouts, force_lanes=args.force_lanes)), flush=True)
+        print(json.dumps(batched_rollouts_only(args.rollouts, force_lanes=args.force_lanes,
+                                               cell_factor=args.cell_factor)), flush=True)
         return
     if args.surrogate_forward:
         print(json.dumps(surrogate_forward_bench(args.surrogate_forward)), flush=True)
@@ -493,7 +496,7 @@ def main():
     dt_ice, nsteps = _time_stepping(static, cfg, cfg.dt)
     Bm, T_n = _thruster_config(sp_, L)
     eng = DemEngine(static, cfg, sd, L, B, nsteps, cfg.dt, dt_ice, True, Bm, T_n, centroids=sc.centroids,
-                    force_lanes=args.force_lanes)
+                    force_lanes=args.force_lanes, cell_factor=args.cell_factor)
     dev = eng.device
     st = build_initial_ice_state(n, sc.centroids, nb, sc.parent_ids, dev)
     ship = torch.zeros(15, dtype=torch.float32, device=dev)

@@ -115,10 +115,9 @@ typedef struct {
      * exact one for any cell >= 2 (grid_rsmall + delta)): elements with
      * radius <= grid_rsmall sit in the bucket of their fine cell; larger ones
      * sit in the bucket of their cell of a coarse grid that the library sizes
-     * (cell >= 2 (2 r_max + delta)).  A small element scans the 3x3 fine
-     * cells around it, a large one the fine cells within
-     * r + grid_rsmall + delta; every element scans the 2x2 coarse cells on
-     * its side of its coarse cell.  A fine bucket holds up to grid_bucket
+     * (cell >= 2 (2 r_max + delta)).  Every element scans the fine cells
+     * within r + grid_rsmall + delta of its centre and the 2x2 coarse cells
+     * on its side of its coarse cell.  A fine bucket holds up to grid_bucket
      * elements, a coarse one grid_bucket2 (0: min(large elements, 8));
      * overflow goes to a list that every element also scans. */
     float grid_x0, grid_y0, grid_cell, grid_rsmall;

@@ -64,10 +64,10 @@ __device__ __forceinline__ void stores_done() {
 // Elements of radius <= grid_rsmall sit in the bucket of their fine cell
 // (cell >= 2 (grid_rsmall + delta)); larger ones sit in the bucket of their
 // coarse cell (cell2 >= 2 (2 r_max + delta)).  Every element scans the 2x2
-// coarse cells on its side of its own coarse cell for large partners; small
-// partners are in the 3x3 fine cells around a small element, or in the fine
-// cells within r + grid_rsmall + delta of a large one.  Each element is in exactly one
-// bucket, so each partner is seen once.  A full bucket sends the element to
+// coarse cells on its side of its own coarse cell for large partners, and
+// the fine cells within r + grid_rsmall + delta of its centre for small ones
+// (a small partner in contact has its centre that close).  Each element is
+// in exactly one bucket, so each partner is seen once.  A full bucket sends the element to
 // an overflow list that every element also scans.  Any of these gives
 // exactly the reference's contact set.  A slot holds a copy of the element's
 // position, velocity and radius, so the contact pass reads its partners from
@@ -156,18 +156,12 @@ __device__ __forceinline__ void cell_insert(const DevView* __restrict__ V, int s
     }
 }
 
-// fine cells to scan for an element of radius r at (x, y): 3x3 around its
-// cell (small) or its reach box (big)
+// fine cells to scan for an element of radius r at (x, y): those its reach
+// box touches (for a small element with cell >= 2 (grid_rsmall + delta) at
+// most 3x3, and about 2.5 x 2.5 on average for a median element)
 __device__ __forceinline__ void scan_range(const IceDemParams& P, float r, float x, float y,
                                            int& x0, int& x1, int& y0, int& y1) {
-    if (!is_big(P, r)) {
-        int cx, cy;
-        cell_of(P, x, y, cx, cy);
-        x0 = max(cx - 1, 0); x1 = min(cx + 1, P.grid_nx - 1);
-        y0 = max(cy - 1, 0); y1 = min(cy + 1, P.grid_ny - 1);
-    } else {
-        cell_box(P, x, y, big_reach(P, r), x0, x1, y0, y1);
-    }
+    cell_box(P, x, y, big_reach(P, r), x0, x1, y0, y1);
 }
 
 // coarse cells to scan: contacts reach at most cell2 / 2, so the 2x2 cells

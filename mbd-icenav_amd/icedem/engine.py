@@ -41,13 +41,15 @@ def bound_radius(local_polys_padded: np.ndarray) -> np.ndarray:
 
 
 def choose_grid(centroids: np.ndarray, radii: np.ndarray, dilation: float = 0.0, cell: Optional[float] = None,
-                margin: float = 60.0, max_cells: int = 1 << 22, small_quantile: float = 0.98):
+                margin: float = 60.0, max_cells: int = 1 << 22, small_quantile: float = 0.98,
+                cell_factor: float = 1.0):
     """Fine uniform grid of the broad phase.
 
     Elements with radius <= r_s (the ``small_quantile`` radius) live in one
-    cell of side >= 2 (r_s + delta) and scan the 3x3 cells around it; the few
-    larger ones live in a coarse grid that the library sizes and scan the
-    fine cells within r + r_s + delta.  Either way every contacting pair is
+    cell of side ``cell`` (default cell_factor * 2 (r_s + delta)); the few
+    larger ones live in a coarse grid that the library sizes.  Every element
+    scans the fine cells within r + r_s + delta of its centre.  Either way
+    every contacting pair is
     found (exactly the reference's contact set); positions outside the grid
     are clamped to the border cells.  Returns (x0, y0, cell, r_s, nx, ny,
     bucket_capacity, coarse_bucket_capacity).
@@ -58,8 +60,12 @@ def choose_grid(centroids: np.ndarray, radii: np.ndarray, dilation: float = 0.0,
         r_s = float(F32(np.quantile(r, small_quantile)))
     else:
         r_s = 1.0
+    # any cell size gives the exact contact set (each element scans the cells
+    # its reach box touches); 2 (r_s + delta) keeps a small element's scan to
+    # at most 3x3 cells, smaller cells scan more cells holding fewer
+    # non-partners.  Floor: half of that (at most 5x5 cells).
     min_cell = 2.0 * (r_s + dilation) * 1.01 + 1e-3
-    cell = max(min_cell, float(cell) if cell is not None else min_cell)
+    cell = max(0.5 * min_cell, float(cell) if cell is not None else min_cell * cell_factor)
     c = np.asarray(centroids, dtype=np.float64)
     lo = c.min(axis=0) - margin if len(c) else np.zeros(2)
     hi = c.max(axis=0) + margin if len(c) else np.ones(2)
@@ -209,7 +215,8 @@ class DemEngine:
     def __init__(self, static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_length, ship_beam,
                  n_ice_steps, dt, dt_ice, use_auto_icenav, B_mat, T_n, centroids=None, device=None, grid=None,
                  cell_size=None, ship_mode: int = 0, ctrl_thrust_max: float = 2e6, ctrl_yaw_rate_max: float = 0.15,
-                 grid_margin: float = 60.0, small_quantile: float = 0.98, force_lanes: int = 0):
+                 grid_margin: float = 60.0, small_quantile: float = 0.98, force_lanes: int = 0,
+                 cell_factor: float = 1.0):
         if not torch.cuda.is_available():
             raise _lib.IceDemError("icedem needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.lib = _lib.load()
@@ -224,7 +231,7 @@ class DemEngine:
         if grid is None:
             cen = np.asarray(centroids, dtype=np.float64) if centroids is not None else np.zeros((self.n, 2))
             grid = choose_grid(cen, static.radii, float(getattr(cfg, "contact_dilation", 0.0)), cell=cell_size,
-                               margin=grid_margin, small_quantile=small_quantile)
+                               margin=grid_margin, small_quantile=small_quantile, cell_factor=cell_factor)
         self.grid = grid
         self.params = make_params(static, cfg, ship_dyn, ship_length, ship_beam, n_ice_steps, dt, dt_ice,
                                   use_auto_icenav, B_mat, T_n, grid=grid, ship_mode=ship_mode,

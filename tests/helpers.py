@@ -51,11 +51,12 @@ class Case:
     _oracle64: object = None
     pair_mode: str = "all"
     force_lanes: int = 0          # lanes per element of the GPU force pass (0: the library's choice)
+    cell_factor: float = 1.0      # GPU fine grid cell / 2 (r_s + delta)
 
     @staticmethod
     def make(name: str, n_frames: int = 200, seed: int = 0, cfg: Optional[SimulationConfig] = None,
              use_auto: bool = True, hull_points: int = 12, current: Optional[tuple] = None,
-             force_lanes: int = 0) -> "Case":
+             force_lanes: int = 0, cell_factor: float = 1.0) -> "Case":
         if name == "hexagon":
             sc = scenario.hexagon_scene()
             cfg = cfg or SimulationConfig(n_edge_samples=3)
@@ -118,6 +119,7 @@ class Case:
             raise ValueError(name)
         case = Case(name, sc, cfg, L, B, mass, hull, traj, dt, use_auto, warm)
         case.force_lanes = force_lanes
+        case.cell_factor = cell_factor
         if case.n > 20000:
             case.pair_mode = "cells"      # the oracle's cell-list pairs (same pairs and order as all pairs)
         if current is not None:
@@ -157,7 +159,8 @@ class Case:
             dt_ice, nsteps = _time_stepping(static, self.cfg, self.dt)
             B, T_n = _thruster_config(sp, self.ship_length)
             self._engine = DemEngine(static, self.cfg, sd, self.ship_length, self.ship_beam, nsteps, self.dt, dt_ice,
-                                     self.use_auto, B, T_n, centroids=sc.centroids, force_lanes=self.force_lanes)
+                                     self.use_auto, B, T_n, centroids=sc.centroids, force_lanes=self.force_lanes,
+                                     cell_factor=self.cell_factor)
         return self._engine
 
     # ── states ──────────────────────────────────────────────────────────────

@@ -45,9 +45,11 @@ def test_frame_parity_c0_current_dilation():
     _run(case, 900, check_every=10, start_check=700)
 
 
-@pytest.mark.parametrize("lanes", [0, 8])
-def test_frame_parity_field(lanes):
-    case = Case.make("c1small", n_frames=300, force_lanes=lanes)
+@pytest.mark.parametrize("lanes,cell", [(0, 1.0), (8, 1.0), (0, 0.55), (8, 0.55)])
+def test_frame_parity_field(lanes, cell):
+    """cell 0.55: reach boxes of up to 5x5 fine cells, scan lists longer
+    than one round of the force pass's lanes"""
+    case = Case.make("c1small", n_frames=300, force_lanes=lanes, cell_factor=cell)
     _run(case, 300, check_every=3)
 
 
