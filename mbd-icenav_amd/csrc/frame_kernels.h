@@ -96,10 +96,14 @@ __device__ __forceinline__ void coarse_of(const DevView* __restrict__ V, float x
     cy = (int)fminf(fmaxf(fy, 0.0f), (float)(V->ny2 - 1));
 }
 
-// bucket record r (fine: r < 2 G) and its capacity
+// byte offset of bucket record r (fine: r < 2 G) in the records buffer (under
+// 4 GiB by construction), the record, and its capacity
+__device__ __forceinline__ uint32_t rec_off(const DevView* __restrict__ V, int r) {
+    return r < 2 * V->G ? (uint32_t)r * (uint32_t)V->rec_f
+                        : (uint32_t)V->coarse_off + (uint32_t)(r - 2 * V->G) * (uint32_t)V->rec_c;
+}
 __device__ __forceinline__ GAS uint8_t* rec_ptr(const DevView* __restrict__ V, int r) {
-    return r < 2 * V->G ? G(V->cells) + (size_t)r * V->rec_f
-                        : G(V->cells) + V->coarse_off + (size_t)(r - 2 * V->G) * V->rec_c;
+    return G(V->cells) + rec_off(V, r);
 }
 __device__ __forceinline__ int rec_cap(const DevView* __restrict__ V, int r) {
     return r < 2 * V->G ? V->p.grid_bucket : V->cap2;
@@ -666,12 +670,11 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
 template <int GR>
 struct ForceLds {
     static constexpr int E = BLOCK / GR;
-    int cj[E][MAXC2];
+    int ck[E][MAXC2];                    // partner's rotated index rot_key(j, e, n)
     float cx[E][MAXC2], cy[E][MAXC2], cvx[E][MAXC2], cvy[E][MAXC2], cr[E][MAXC2];
     int ord[E][MAXC2];
-    int grec[E][GR];
-    int goff[E][GR];
-    int gcnt[E][GR];
+    uint32_t grec[E][GR];                // scan-list cell's record offset
+    int goff[E][GR];                     // its first slot in the group's slot order
     float bf[E][3][GR];
 };
 
@@ -729,29 +732,30 @@ __device__ __forceinline__ void scan_list(const DevView* __restrict__ V, int set
     const int l = threadIdx.x & (GR - 1);
     for (int c0 = 0; c0 < S.ntot; c0 += GR) {
         const int q = c0 + l;
-        int rix = 0, cc = 0;
+        uint32_t off = 0;
+        int cc = 0;
         if (q < S.ntot) {
-            rix = list_rec(V, set, S, q);
-            cc = min(*(GAS const int*)rec_ptr(V, rix), rec_cap(V, rix));
+            const int rix = list_rec(V, set, S, q);
+            off = rec_off(V, rix);
+            cc = min(*(GAS const int*)(G(V->cells) + off), rec_cap(V, rix));
         }
         const int incl = group_incl_scan<GR>(cc);
         const int total = group_last<GR>(incl);
-        L.grec[g][l] = rix;
-        L.goff[g][l] = incl - cc;
-        L.gcnt[g][l] = cc;
+        L.grec[g][l] = off;
+        L.goff[g][l] = incl - cc;   // lanes past the list: total
         wave_sync_lds();
-        const int nq = min(GR, S.ntot - c0);
         for (int base = 0; base < total; base += GR) {
+            // slot f lies in the last cell qq whose first slot is <= f (cells
+            // without slots share their successor's offset; offsets past the
+            // list are total > f), at position f - goff[qq] < its count
             const int f = base + l;
-            bool have = f < total;
+            const bool have = f < total;
             Slot sl = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, -1};
             if (have) {
                 int qq = 0;
                 #pragma unroll
-                for (int t = 1; t < GR; ++t) qq += (t < nq && L.goff[g][t] <= f) ? 1 : 0;
-                const int sq = f - L.goff[g][qq];
-                have = sq >= 0 && sq < L.gcnt[g][qq];
-                if (have) sl = slot_load(rec_ptr(V, L.grec[g][qq]), sq);
+                for (int t = 1; t < GR; ++t) qq += (L.goff[g][t] <= f) ? 1 : 0;
+                sl = slot_load(G(V->cells) + L.grec[g][qq], f - L.goff[g][qq]);
             }
             visit(have, sl.e, make_float4(sl.x, sl.y, sl.vx, sl.vy), sl.r);
         }
@@ -779,7 +783,8 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
         const int before = __builtin_popcount(m & ((1u << l) - 1u));
         if (ov && count + before < MAXC2) {
             const int s = count + before;
-            L.cj[g][s] = j; L.cx[g][s] = x; L.cy[g][s] = y; L.cvx[g][s] = vx; L.cvy[g][s] = vy; L.cr[g][s] = r;
+            L.ck[g][s] = rot_key(j, e, n);
+            L.cx[g][s] = x; L.cy[g][s] = y; L.cvx[g][s] = vx; L.cvy[g][s] = vy; L.cr[g][s] = r;
         }
         count += __builtin_popcount(m);
     };
@@ -816,9 +821,9 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
     if (count <= MAXC2) {
         // every partner was seen once: rank by rotated partner index
         for (int i = l; i < count; i += GR) {
-            const int ki = rot_key(L.cj[g][i], e, n);
+            const int ki = L.ck[g][i];
             int rank = 0;
-            for (int q = 0; q < count; ++q) rank += (rot_key(L.cj[g][q], e, n) < ki) ? 1 : 0;
+            for (int q = 0; q < count; ++q) rank += (L.ck[g][q] < ki) ? 1 : 0;
             L.ord[g][rank] = i;
         }
         wave_sync_lds();
@@ -832,9 +837,8 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
             ca[u] = 0.0f; cb[u] = 0.0f; cc[u] = 0.0f;
             if (i < count) {
                 const int s = L.ord[g][i];
-                const int j = L.cj[g][s];
                 const float x = L.cx[g][s], y = L.cy[g][s], vx = L.cvx[g][s], vy = L.cvy[g][s], r = L.cr[g][s];
-                const bool lo = e < j;
+                const bool lo = L.ck[g][s] < n - e;   // e < j
                 contrib_v(P, lo, lo ? px : x, lo ? py : y, lo ? pvx : vx, lo ? pvy : vy, lo ? pr : r,
                           lo ? x : px, lo ? y : py, lo ? vx : pvx, lo ? vy : pvy, lo ? r : pr, ca[u], cb[u], cc[u]);
             }

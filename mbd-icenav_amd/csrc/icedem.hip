@@ -532,7 +532,7 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
         H.rec_c = rec_bytes(H.cap2);
         H.coarse_off = 2 * (int64_t)H.G * H.rec_f;
         const int64_t total = H.coarse_off + 2 * (int64_t)H.G2 * H.rec_c;
-        if (2 * (int64_t)H.G + 2 * (int64_t)H.G2 > (int64_t)INT32_MAX || total > (1LL << 36)) {
+        if (2 * (int64_t)H.G + 2 * (int64_t)H.G2 > (int64_t)INT32_MAX || total >= (1LL << 32)) {
             set_err(nullptr, "bucket records beyond the addressable range"); icedem_destroy(ctx); return ICEDEM_E_LIMIT;
         }
         AL(cells, uint8_t, (size_t)total);
