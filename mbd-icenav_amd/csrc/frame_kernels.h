@@ -713,13 +713,24 @@ __device__ __forceinline__ int group_last(int v) {
 struct ScanList {
     int x0, y0, W, nfine;
     int u0, v0, W2, ntot;
+    float iW, iW2;   // approximate 1 / W, 1 / W2 (0 for an empty part)
 };
 
+// q = row * W + col for 0 <= q < 2^22, W >= 1, iW within a few ulp of 1 / W:
+// the float estimate of the row is off by at most one, fixed by one step
+__device__ __forceinline__ void split_rc(int q, int W, float iW, int& row, int& col) {
+    row = (int)((float)q * iW);
+    col = q - row * W;
+    if (col < 0) { row -= 1; col += W; }
+    else if (col >= W) { row += 1; col -= W; }
+}
 
 __device__ __forceinline__ int list_rec(const DevView* __restrict__ V, int set, const ScanList& S, int q) {
-    if (q < S.nfine) return fine_rec(V, set, (S.y0 + q / S.W) * V->p.grid_nx + (S.x0 + q % S.W));
-    const int r = q - S.nfine;
-    return coarse_rec(V, set, (S.v0 + r / S.W2) * V->nx2 + (S.u0 + r % S.W2));
+    const bool fine = q < S.nfine;
+    int row, col;
+    split_rc(fine ? q : q - S.nfine, fine ? S.W : S.W2, fine ? S.iW : S.iW2, row, col);
+    return fine ? fine_rec(V, set, (S.y0 + row) * V->p.grid_nx + (S.x0 + col))
+                : coarse_rec(V, set, (S.v0 + row) * V->nx2 + (S.u0 + col));
 }
 
 // The occupied slots of the scan list, GR cells at a time: the lanes read
@@ -803,6 +814,8 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
     ScanList S;
     S.x0 = x0; S.y0 = y0; S.W = x1 - x0 + 1; S.nfine = S.W * (y1 - y0 + 1);
     S.u0 = u0; S.v0 = v0; S.W2 = u1 - u0 + 1; S.ntot = S.nfine + S.W2 * (v1 - v0 + 1);
+    S.iW = __builtin_amdgcn_rcpf((float)S.W);
+    S.iW2 = S.W2 > 0 ? __builtin_amdgcn_rcpf((float)S.W2) : 0.0f;
     scan_list<GR>(V, set, S, L, g, visit);
     for (int base = 0; base < novf; base += GR) {
         const int f = base + l;
