@@ -132,23 +132,21 @@ __device__ __forceinline__ Slot slot_load(GAS const uint8_t* __restrict__ rec, i
     return o;
 }
 
-// Element e (state x, y, vx, vy, radius r) into its bucket of set `set`.
-__device__ __forceinline__ void cell_insert(const DevView* __restrict__ V, int set, int e,
-                                            float x, float y, float vx, float vy, float r) {
+// Bucket record of set `set` for an element of radius r at (x, y).
+__device__ __forceinline__ int insert_rec(const DevView* __restrict__ V, int set, float x, float y, float r) {
     const IceDemParams& P = V->p;
-    int rix;
-    if (!is_big(P, r)) {
-        int cx, cy;
-        rix = fine_rec(V, set, cell_of(P, x, y, cx, cy));
-    } else {
-        int cx, cy;
-        coarse_of(V, x, y, cx, cy);
-        rix = coarse_rec(V, set, cy * V->nx2 + cx);
-    }
-    GAS uint8_t* rec = rec_ptr(V, rix);
-    const int idx = atomicAdd((int*)rec, 1);
+    int cx, cy;
+    if (!is_big(P, r)) return fine_rec(V, set, cell_of(P, x, y, cx, cy));
+    coarse_of(V, x, y, cx, cy);
+    return coarse_rec(V, set, cy * V->nx2 + cx);
+}
+
+// Element e's slot idx (from the record's insertion count) of record rix:
+// the slot, or the overflow list when the bucket is full.
+__device__ __forceinline__ void cell_store(const DevView* __restrict__ V, int set, int rix, int idx, int e,
+                                           float x, float y, float vx, float vy, float r) {
     if (idx < rec_cap(V, rix)) {
-        GAS uint8_t* q = rec + 16 + 24 * idx;
+        GAS uint8_t* q = rec_ptr(V, rix) + 16 + 24 * idx;
         f4u a;
         a.x = x; a.y = y; a.z = vx; a.w = vy;
         f2u c;
@@ -158,6 +156,13 @@ __device__ __forceinline__ void cell_insert(const DevView* __restrict__ V, int s
     } else {
         G(V->ovf_list)[(size_t)set * V->ovf_cap + atomicAdd(&V->ovf_count[set], 1)] = e;
     }
+}
+
+// Element e (state x, y, vx, vy, radius r) into its bucket of set `set`.
+__device__ __forceinline__ void cell_insert(const DevView* __restrict__ V, int set, int e,
+                                            float x, float y, float vx, float vy, float r) {
+    const int rix = insert_rec(V, set, x, y, r);
+    cell_store(V, set, rix, atomicAdd((int*)rec_ptr(V, rix), 1), e, x, y, vx, vy, r);
 }
 
 // fine cells to scan for an element of radius r at (x, y): those its reach
@@ -406,9 +411,26 @@ __device__ __forceinline__ void cull_chunk(const DevView* __restrict__ V, int ch
         const float u = ch * dx + sh * dy, v = ch * dy - sh * dx;
         const float b = G(V->bound)[e] + 1.0f;
         near = fabsf(u) <= P.ship_box_half_len + b && fabsf(v) <= P.ship_box_half_beam + b;
+    }
+    // the wave's near elements take consecutive queue entries (one atomic per
+    // wave; queue order does not matter), and both atomics are issued before
+    // either result is used
+    const uint64_t nb = __ballot(near);
+    const int lane = threadIdx.x & 63;
+    const int first = __ffsll((unsigned long long)nb) - 1;
+    int qbase = 0;
+    if (lane == first) qbase = atomicAdd(near_count(V), __popcll(nb));
+    int rix = 0, idx = 0;
+    if (valid) {
+        rix = insert_rec(V, set, x, y, r);
+        idx = atomicAdd((int*)rec_ptr(V, rix), 1);
+    }
+    if (nb != 0ull) qbase = __shfl(qbase, first, 64);
+    if (valid) {
         G(V->nearf)[e] = near ? 1 : 0;
-        if (near) G(V->near_list)[atomicAdd(near_count(V), 1)] = e;
-        cell_insert(V, set, e, x, y, vx, vy, r);
+        if (near) G(V->near_list)[qbase + __builtin_amdgcn_mbcnt_hi((uint32_t)(nb >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)nb, 0u))] = e;
+        cell_store(V, set, rix, idx, e, x, y, vx, vy, r);
     }
     const int cnt = __syncthreads_count(near ? 1 : 0);
     if (threadIdx.x == 0) G(V->nearcnt)[chunk] = cnt;
@@ -1176,7 +1198,8 @@ __device__ __forceinline__ void post_body(const DevView* __restrict__ V, const C
     GAS const float* pos = G(final_in_nxt ? V->nxt_pos : D->s.d_pos);
     GAS const float* vel = G(final_in_nxt ? V->nxt_vel : D->s.d_vel);
     if (t < nb) {
-        const int i = G(V->bi)[t], j = G(V->bj)[t];
+        const int4 bp = ldg(G(V->bpair) + t);
+        const int i = bp.x, j = bp.y;
         float dx = pos[j] - pos[i];
         float dy = pos[n + j] - pos[n + i];
         float dist = sqrtf(dx * dx + dy * dy);
@@ -1185,9 +1208,8 @@ __device__ __forceinline__ void post_body(const DevView* __restrict__ V, const C
         GAS const uint8_t* act = G(D->s.d_active);
         GAS const uint8_t* elig = G(D->s.d_floe_elig);
         bool both = act[i] && act[j];
-        bool floe_ok = elig[G(V->floe)[i]] && elig[G(V->floe)[j]];
-        const float hi_ = G(V->thick)[i], hj = G(V->thick)[j];
-        bool breakable = (hi_ <= P.max_fracture_thickness) && (hj <= P.max_fracture_thickness);
+        bool floe_ok = elig[bp.z] && elig[bp.w & 0x3fffffff];
+        bool breakable = (bp.w >> 30) == 0;
         const float fmi = fm_scaled(V, i), fmj = fm_scaled(V, j);
         bool near = (fmi > 0.0f) || (fmj > 0.0f);
         bool at_crack = G(V->crack)[i] || G(V->crack)[j];

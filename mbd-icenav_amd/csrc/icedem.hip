@@ -490,6 +490,16 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     UP(floe, int32_t, st->h_floe_of, n);
     UP(bi, int32_t, st->h_bond_i, st->n_bonds);
     UP(bj, int32_t, st->h_bond_j, st->n_bonds);
+    {
+        std::vector<int4> bp(std::max(1, st->n_bonds));
+        for (int b = 0; b < st->n_bonds; ++b) {
+            const int i = st->h_bond_i[b], j = st->h_bond_j[b];
+            const bool brk = (st->h_thicknesses[i] <= prm->max_fracture_thickness) &&
+                             (st->h_thicknesses[j] <= prm->max_fracture_thickness);
+            bp[b] = make_int4(i, j, st->h_floe_of[i], st->h_floe_of[j] | (brk ? 0 : (1 << 30)));
+        }
+        UP(bpair, int4, bp.data(), bp.size());
+    }
     UP(rest, float, st->h_rest_lengths, st->n_bonds);
     UP(bstr, float, st->h_bond_strengths, st->n_bonds);
     UP(sbar, float, st->h_sigma_bar, st->n_bonds);

@@ -40,6 +40,9 @@ struct DevView {
     const float* thick; const float* bound;
     const int32_t* floe;
     const int32_t* bi; const int32_t* bj;
+    // (n_bonds) per bond: i, j, floe of i, floe of j | 1 << 30 when the bond is
+    // not breakable (a thickness above max_fracture_thickness)
+    const int4* bpair;
     const float* rest; const float* bstr; const float* sbar;
     const float2* hull;       // (S)
     // bonds of each element, element-major with fixed width W, in the
