@@ -69,24 +69,24 @@ def algorithmic_bytes(kernel: str, n: int, nb: int, nf: int, G: int, n_near: int
     count 28; the element's bucket slot as a contact partner 24; writes next
     pos/vel/rot/omega 24, active 1, sub 4, force_mag 4, crack flag 1 -> 115.
     Per bond (two table entries of 24 B, intact 1 + damage 4) -> 53; cell
-    counts 4 per cell; floe flags 4 per floe.
+    counts 4 per cell; floe flags 4 per floe; the tail's three masked block
+    trees per near block (256 x 13 B) and the partials 12 per block slot.
     k_near, per near element: pos 8, rot 4, polygon 9 per vertex slot,
     radius/thickness/floe 12, pre-pass writes 28, floe atomics 8; hull points
-    8 each; per near block the three masked block trees (256 x 13 B) and the
-    partials 12 per block slot.
-    k_post, per bond: indices 8, rest/strength/sigma 12, damage state 17 read
-    and written -> 54; per element: next state 24 read, state 24 written,
-    position/rotation/submergence history 16 (+ submergence read 4), force_mag
-    4, crack 1, thickness 4, floe 4, active 1, elig 1, cull reads
-    radius/bound 8, near flag 1, bucket slot 24 and count 4 -> 120; cell
-    clears 4 per cell, floe clears 8 per floe.
+    8 each.
+    k_post, per bond: pair table (i, j, floes) 16, rest/strength/sigma 12,
+    damage state 17 read and written -> 62; per element: next state 24 read,
+    state 24 written, position/rotation/submergence history 16 (+ submergence
+    read 4), force_mag 4, crack 1, active 1, elig 1, cull reads radius/bound
+    8, near flag 1, bucket slot 24 and count 4 -> 112; cell clears 4 per
+    cell, floe clears 8 per floe.
     """
     if kernel == "k_force":
-        return 115.0 * n + 53.0 * nb + 4.0 * G + 4.0 * nf
+        return 115.0 * n + 53.0 * nb + 4.0 * G + 4.0 * nf + near_blocks * 256 * 13.0 + 12.0 * nbpad
     if kernel == "k_near":
-        return n_near * (9.0 * max_v + 60.0) + 8.0 * n_hull + near_blocks * 256 * 13.0 + 12.0 * nbpad
+        return n_near * (9.0 * max_v + 60.0) + 8.0 * n_hull
     if kernel == "k_post":
-        return 54.0 * nb + 120.0 * n + 4.0 * G + 8.0 * nf
+        return 62.0 * nb + 112.0 * n + 4.0 * G + 8.0 * nf
     return 64.0
 
 
