@@ -375,13 +375,13 @@ def surrogate_bench(plain_rps: float, rounds: int = 800, epochs: int = 20, test_
 
 
 def batched_rollouts_only(R: int = 256, frames: int = 400, hold: int = 10, n_knots: int = 40, reps: int = 3,
-                          force_lanes: int = 0, cell_factor: float = 1.0):
+                          force_lanes: int = 0, cell_factor: float = 1.0, mode: str = "frames"):
     """The batched-rollout launch alone (for profiling its kernels)."""
     import torch
     from icedem import rollouts as ro
     scenes = ro.rollout_patches(R, workers=min(16, os.cpu_count() or 1))
     rb = ro.RolloutBatch(scenes, list(range(R)), force_lanes=force_lanes, cell_size_factor=cell_factor)
-    rb.set_mode("frames")
+    rb.set_mode(mode)
     ctl = torch.as_tensor(ro.nominal_controls(R, n_knots, seed=0))
     rb.reset()
     rb.run(ctl, hold, frames)
@@ -394,7 +394,7 @@ def batched_rollouts_only(R: int = 256, frames: int = 400, hold: int = 10, n_kno
         best = min(best, time.perf_counter() - t0)
     n_el = sum(len(scenes[i].polygons) for i in range(R))
     rb.close()
-    return {"workload": f"{R} rollouts x {frames} frames, {n_el} elements", "best_s": best,
+    return {"workload": f"{R} rollouts x {frames} frames, {n_el} elements", "mode": mode, "best_s": best,
             "rollouts_per_s": R / best, "element_steps_per_s": n_el * frames / best, "us_per_frame": 1e6 * best / frames}
 
 
@@ -455,6 +455,8 @@ def main():
                     help="lanes per element of the force pass (0: the library's choice by element count)")
     ap.add_argument("--cell-factor", type=float, default=1.0,
                     help="fine grid cell as a multiple of 2 (r_s + delta) (frames) or of the batch's default cell")
+    ap.add_argument("--batch-mode", default="frames", choices=["frames", "scenes"],
+                    help="batch mode of --rollouts-only")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB five times (reference kernel for the PMC byte counters)")
     args = ap.parse_args()
@@ -469,7 +471,7 @@ def main():
         return
     if args.rollouts_only:
         print(json.dumps(batched_rollouts_only(args.rollouts, force_lanes=args.force_lanes,
-                                               cell_factor=args.cell_factor)), flush=True)
+                                               cell_factor=args.cell_factor, mode=args.batch_mode)), flush=True)
         return
     if args.surrogate_forward:
         print(json.dumps(surrogate_forward_bench(args.surrogate_forward)), flush=True)
