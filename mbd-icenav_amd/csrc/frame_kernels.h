@@ -689,9 +689,17 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
 
 // ── k_force ──────────────────────────────────────────────────────────────────
 
+// per-element values staged for the block's epilogue (force_epilogue): the
+// element's own state and statics, its hull and drag inputs, the group's
+// contact and bond sums, and the flags (act_raw | act_bond << 1 | crack << 2)
+enum : int { S_PX, S_PY, S_VX, S_VY, S_RO, S_OM, S_R, S_H, S_AREA, S_MASS, S_INERTIA, S_SUB0, S_PEN, S_FMR, S_SX,
+             S_SY, S_NXE, S_NYE, S_VWX, S_VWY, S_FIX, S_FIY, S_MI, S_FBX, S_FBY, S_MB, S_FLAGS, S_N };
+constexpr int SPAD = S_N + 2;   // odd row stride: a column read by E threads hits E distinct banks
+
 template <int GR>
 struct ForceLds {
     static constexpr int E = BLOCK / GR;
+    float stg[E][SPAD];
     int ck[E][MAXC2];                    // partner's rotated index rot_key(j, e, n)
     float cx[E][MAXC2], cy[E][MAXC2], cvx[E][MAXC2], cvy[E][MAXC2], cr[E][MAXC2];
     int ord[E][MAXC2];
@@ -1003,63 +1011,27 @@ __device__ __forceinline__ void bond_force_group(const DevView* __restrict__ V, 
     }
 }
 
-// elements [chunk*E, chunk*E+E), E = 256 / GR, GR lanes each (no block
-// barrier inside).  mode 0: a frame sub-step (integrates); mode 1: forces
-// out (fout, mout).
-template <int GR>
-__device__ __forceinline__ void force_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                           int k, int sub, int src_is_state, int mode, int set,
-                           const float* __restrict__ fship_in, float* __restrict__ fout, float* __restrict__ mout,
-                           ForceLds<GR>& L, int chunk) {
+// The epilogue of element e by one thread (jax_simulation.py:140-268): ship
+// forces and submergence (first sub-step), drag, rotational drag, totals and
+// the integration into the next-state buffers, from the values its group
+// staged in sv.
+__device__ __forceinline__ void force_epilogue(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                               int sub, int src_is_state, int mode,
+                                               const float* __restrict__ fship_in, float* __restrict__ fout,
+                                               float* __restrict__ mout, const float* sv, int e) {
     const IceDemParams& P = V->p;
     const int n = V->n;
-    const int g = threadIdx.x / GR, l = threadIdx.x & (GR - 1);
-    const int e = chunk * epb<GR>() + g;
-    if (e >= n) return;                       // group-uniform
-    GAS const float* pos = G(src_is_state ? D->s.d_pos : V->nxt_pos);
-    GAS const float* vel = G(src_is_state ? D->s.d_vel : V->nxt_vel);
-    GAS const float* rotp = G(src_is_state ? D->s.d_rot : V->nxt_rot);
-    GAS const float* omp = G(src_is_state ? D->s.d_omega : V->nxt_omega);
     GAS float* dpos = G(src_is_state ? V->nxt_pos : D->s.d_pos);
     GAS float* dvel = G(src_is_state ? V->nxt_vel : D->s.d_vel);
     GAS float* drot = G(src_is_state ? V->nxt_rot : D->s.d_rot);
     GAS float* domg = G(src_is_state ? V->nxt_omega : D->s.d_omega);
-    GAS const uint8_t* active = G(D->s.d_active);
-    // everything that depends only on e, read in one round trip
-    const float px = pos[e], py = pos[n + e], vx = vel[e], vy = vel[n + e];
-    const float ro = rotp[e], om = omp[e];
-    const float r = G(V->radius)[e], h = G(V->thick)[e], area = G(V->area)[e];
-    const float mass = G(V->mass)[e], inertia = G(V->inertia)[e];
-    const int fl = G(V->floe)[e];
-    const bool act_raw = active[e] != 0;
-    const int deg = V->W > 0 ? G(V->bdeg)[e] : 0;
-    const int novf = G(V->ovf_count)[set];
-    int4 bt0 = make_int4(0, 0, 0, 0);
-    float2 bp0 = make_float2(0.0f, 0.0f);
-    if (l < V->W) { bt0 = ldg(G(V->btab) + (size_t)e * V->W + l); bp0 = ldg(G(V->bprm) + (size_t)e * V->W + l); }
-    const float sub0 = G(D->s.d_sub)[e];
-    const bool first = (mode == 0 && sub == 0);
-    const bool is_near = first && G(V->nearf)[e] != 0;
-    float pen = 0.0f, fmr = 0.0f, sx_ = 0.0f, sy_ = 0.0f, nxe = 0.0f, nye = 0.0f;
-    if (is_near) { pen = G(V->pen)[e]; fmr = G(V->fm_raw)[e]; sx_ = G(V->soi_x)[e]; sy_ = G(V->soi_y)[e]; nxe = G(V->nx)[e]; nye = G(V->ny)[e]; }
-    float vwx = 0.0f, vwy = 0.0f;
-    GAS const float* vwp = G(D->in.d_v_water);
-    if (vwp) {
-        const int64_t base = (int64_t)k * D->in.vw_stride_k + (int64_t)e * D->in.vw_stride_e;
-        vwx = vwp[base];
-        vwy = vwp[base + D->in.vw_stride_c];
-    }
-    const bool fcoll = (mode != 1) && (G(V->floe_coll)[fl] != 0u);
-    const bool act_bond = act_raw || fcoll;   // activity seen by the bond pass (mode 0)
-    BondIn bin0 = {0.0f, 0.0f, 0.0f, false, false};
-    if (l < deg) bin0 = bond_in(V, D, pos, bt0, mode);   // read ahead of the contact pass
-    float fix, fiy, mi;
-    contact_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, L, g, fix, fiy, mi);
-    float fbx, fby, mb;
-    bool crack;
-    bond_force_group<GR>(V, D, pos, e, px, py, mode == 1 ? act_raw : act_bond, deg, bt0, bp0, bin0, mode, L, g,
-                     fbx, fby, mb, crack);
-    if (l != 0) return;
+    const float px = sv[S_PX], py = sv[S_PY], vx = sv[S_VX], vy = sv[S_VY], ro = sv[S_RO], om = sv[S_OM];
+    const float r = sv[S_R], h = sv[S_H], area = sv[S_AREA], mass = sv[S_MASS], inertia = sv[S_INERTIA];
+    const float sub0 = sv[S_SUB0], pen = sv[S_PEN], fmr = sv[S_FMR], sx_ = sv[S_SX], sy_ = sv[S_SY];
+    const float nxe = sv[S_NXE], nye = sv[S_NYE], vwx = sv[S_VWX], vwy = sv[S_VWY];
+    const float fix = sv[S_FIX], fiy = sv[S_FIY], mi = sv[S_MI], fbx = sv[S_FBX], fby = sv[S_FBY], mb = sv[S_MB];
+    const int fl = __float_as_int(sv[S_FLAGS]);
+    const bool act_raw = (fl & 1) != 0, act_bond = (fl & 2) != 0, crack = (fl & 4) != 0;
     float shx, shy, subv;
     bool act;
     if (mode == 1) {
@@ -1133,6 +1105,86 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
     dpos[e] = npx; dpos[n + e] = npy;
     dvel[e] = nvx; dvel[n + e] = nvy;
     drot[e] = nr; domg[e] = nw;
+}
+
+
+// elements [chunk*E, chunk*E+E), E = 256 / GR, GR lanes each: the lane
+// groups form the contact and bond sums and stage them with the element's
+// values; the epilogues run on the group's lane 0 (GR = 16) or on threads
+// [0, E) after a block barrier (GR = 8; every thread of the block calls;
+// `again`: the block calls once more, so the staging area must be free on
+// return).  mode 0: a frame sub-step (integrates); mode 1: forces out
+// (fout, mout).
+template <int GR>
+__device__ __forceinline__ void force_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                           int k, int sub, int src_is_state, int mode, int set,
+                           const float* __restrict__ fship_in, float* __restrict__ fout, float* __restrict__ mout,
+                           ForceLds<GR>& L, int chunk, bool again = false) {
+    const int n = V->n;
+    const int g = threadIdx.x / GR, l = threadIdx.x & (GR - 1);
+    const int e = chunk * epb<GR>() + g;
+    if (e < n) {                              // group-uniform
+        GAS const float* pos = G(src_is_state ? D->s.d_pos : V->nxt_pos);
+        GAS const float* vel = G(src_is_state ? D->s.d_vel : V->nxt_vel);
+        GAS const float* rotp = G(src_is_state ? D->s.d_rot : V->nxt_rot);
+        GAS const float* omp = G(src_is_state ? D->s.d_omega : V->nxt_omega);
+        GAS const uint8_t* active = G(D->s.d_active);
+        // everything that depends only on e, read in one round trip
+        const float px = pos[e], py = pos[n + e], vx = vel[e], vy = vel[n + e];
+        const float ro = rotp[e], om = omp[e];
+        const float r = G(V->radius)[e], h = G(V->thick)[e], area = G(V->area)[e];
+        const float mass = G(V->mass)[e], inertia = G(V->inertia)[e];
+        const int fl = G(V->floe)[e];
+        const bool act_raw = active[e] != 0;
+        const int deg = V->W > 0 ? G(V->bdeg)[e] : 0;
+        const int novf = G(V->ovf_count)[set];
+        int4 bt0 = make_int4(0, 0, 0, 0);
+        float2 bp0 = make_float2(0.0f, 0.0f);
+        if (l < V->W) { bt0 = ldg(G(V->btab) + (size_t)e * V->W + l); bp0 = ldg(G(V->bprm) + (size_t)e * V->W + l); }
+        const float sub0 = G(D->s.d_sub)[e];
+        const bool first = (mode == 0 && sub == 0);
+        const bool is_near = first && G(V->nearf)[e] != 0;
+        float pen = 0.0f, fmr = 0.0f, sx_ = 0.0f, sy_ = 0.0f, nxe = 0.0f, nye = 0.0f;
+        if (is_near) { pen = G(V->pen)[e]; fmr = G(V->fm_raw)[e]; sx_ = G(V->soi_x)[e]; sy_ = G(V->soi_y)[e]; nxe = G(V->nx)[e]; nye = G(V->ny)[e]; }
+        float vwx = 0.0f, vwy = 0.0f;
+        GAS const float* vwp = G(D->in.d_v_water);
+        if (vwp) {
+            const int64_t base = (int64_t)k * D->in.vw_stride_k + (int64_t)e * D->in.vw_stride_e;
+            vwx = vwp[base];
+            vwy = vwp[base + D->in.vw_stride_c];
+        }
+        const bool fcoll = (mode != 1) && (G(V->floe_coll)[fl] != 0u);
+        const bool act_bond = act_raw || fcoll;   // activity seen by the bond pass (mode 0)
+        BondIn bin0 = {0.0f, 0.0f, 0.0f, false, false};
+        if (l < deg) bin0 = bond_in(V, D, pos, bt0, mode);   // read ahead of the contact pass
+        float fix, fiy, mi;
+        contact_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, L, g, fix, fiy, mi);
+        float fbx, fby, mb;
+        bool crack;
+        bond_force_group<GR>(V, D, pos, e, px, py, mode == 1 ? act_raw : act_bond, deg, bt0, bp0, bin0, mode, L, g,
+                             fbx, fby, mb, crack);
+        if (l == 0) {
+            float* sv = L.stg[g];
+            sv[S_PX] = px; sv[S_PY] = py; sv[S_VX] = vx; sv[S_VY] = vy; sv[S_RO] = ro; sv[S_OM] = om;
+            sv[S_R] = r; sv[S_H] = h; sv[S_AREA] = area; sv[S_MASS] = mass; sv[S_INERTIA] = inertia;
+            sv[S_SUB0] = sub0; sv[S_PEN] = pen; sv[S_FMR] = fmr; sv[S_SX] = sx_; sv[S_SY] = sy_;
+            sv[S_NXE] = nxe; sv[S_NYE] = nye; sv[S_VWX] = vwx; sv[S_VWY] = vwy;
+            sv[S_FIX] = fix; sv[S_FIY] = fiy; sv[S_MI] = mi; sv[S_FBX] = fbx; sv[S_FBY] = fby; sv[S_MB] = mb;
+            sv[S_FLAGS] = __int_as_float((act_raw ? 1 : 0) | (act_bond ? 2 : 0) | (crack ? 4 : 0));
+            // 16 lanes (small fields, latency-bound): the group's lane 0
+            // finishes its own element, no barrier
+            if (GR >= 16) force_epilogue(V, D, sub, src_is_state, mode, fship_in, fout, mout, sv, e);
+        }
+    }
+    if (GR < 16) {
+        // 8 lanes (large fields, instruction-bound): one thread per element
+        // of the block, so the epilogue's instructions serve 32 elements
+        __syncthreads();
+        if ((int)threadIdx.x < epb<GR>() && chunk * epb<GR>() + (int)threadIdx.x < n)
+            force_epilogue(V, D, sub, src_is_state, mode, fship_in, fout, mout, L.stg[threadIdx.x],
+                           chunk * epb<GR>() + threadIdx.x);
+        if (again) __syncthreads();
+    }
 }
 
 // With nextra > 0 (a frame's first sub-step) blocks [0, nextra) run
@@ -1400,7 +1452,7 @@ __global__ __launch_bounds__(BLOCK) void k_scene_rollouts(const DevView* __restr
         __syncthreads();
         mid_body(V, D, la, lb, k, last);
         for (int c = 0; c < fch; ++c)
-            force_body<GRP>(V, D, k, 0, 1, 0, set, nullptr, nullptr, nullptr, L, c);
+            force_body<GRP>(V, D, k, 0, 1, 0, set, nullptr, nullptr, nullptr, L, c, true);
         __syncthreads();
         for (int base = 0; base < post; base += BLOCK)
             post_body(V, D, k, 1, last, base + threadIdx.x, base / BLOCK, set, last ? 0 : 1);
