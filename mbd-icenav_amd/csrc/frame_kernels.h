@@ -6,8 +6,9 @@
 //             queue, insertion into the cell buckets of set 0
 //   k_near    one wavefront per queued element: hull pass 1 + 2, pre-pass,
 //             floe or/max (integer atomics).  No cross-workgroup hand-off.
-//   k_force   16 lanes per element: activation, submergence, bond forces,
-//             contacts, drag, integration.  tail_groups(nblk) extra
+//   k_force   16 or 8 lanes per element (k_force<GR>): activation,
+//             submergence, bond forces, contacts, drag, integration (the
+//             integration on one thread per element).  tail_groups(nblk) extra
 //             workgroups form the pairwise-tree partials of the near
 //             elements' ship forces, one wave per 256-element block; the last
 //             of them forms the totals, the thrust scale, the scaled totals,
