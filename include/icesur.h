@@ -26,6 +26,7 @@ extern "C" {
 #define ICESUR_RELU 1      /* epilogue: max(x, 0) */
 #define ICESUR_ACCUM 2     /* epilogue: C += result (fp32 C only) */
 #define ICESUR_BIAS 4      /* epilogue: + bias[col] */
+#define ICESUR_ROWINIT 8   /* accumulators start from d_work[col] (N floats, every row) instead of 0; split_k 1 */
 
 /* C (M x N) = op(A) (M x K) . op(B) (K x N) with bf16 operands and fp32
  * accumulation on MFMA; op(A) = A (M x K, lda) or, with trans_a, A^T where
@@ -36,7 +37,11 @@ extern "C" {
  * products go to d_work and are added in split order before the epilogue;
  * with more than 32 splits, in split order within groups of 32 and then the
  * group sums in order.  d_work holds split_k * M * N floats, plus
- * ceil(split_k / 32) * M * N when split_k > 32. */
+ * ceil(split_k / 32) * M * N when split_k > 32.  With ICESUR_ROWINIT
+ * (split_k 1) d_work instead holds one fp32 row of N accumulator start
+ * values: a product whose first K0 columns of A are one row shared by all
+ * rows equals, bit for bit, that row's product over those K0 (a 1-row GEMM,
+ * K0 a multiple of 32) used as the start of the remaining K - K0. */
 int icesur_gemm(int32_t trans_a, int32_t trans_b, int32_t M, int32_t N, int32_t K,
                 const uint16_t* d_a, int32_t lda, const uint16_t* d_b, int32_t ldb,
                 float* d_c, int32_t ldc, uint16_t* d_cb, int32_t ldcb, const float* d_bias, int32_t flags,

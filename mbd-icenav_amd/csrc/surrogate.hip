@@ -125,7 +125,7 @@ __device__ __forceinline__ void epilogue_store(const Epi& E, int row, int col, f
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_t* __restrict__ A, int lda,
                                               const uint16_t* __restrict__ B, int ldb, Epi E, int ks,
-                                              float* __restrict__ part) {
+                                              float* __restrict__ part, const float* __restrict__ init) {
     __shared__ __align__(16) uint16_t As[TM][KP];
     __shared__ __align__(16) uint16_t Bs[TN][KP];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -136,7 +136,12 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_
     #pragma unroll
     for (int i = 0; i < 2; ++i)
         #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int j = 0; j < 2; ++j) {
+            // accumulator column lane % 16 (rows 4 (lane / 16) + v)
+            const int col = n0 + wn + j * 16 + (lane & 15);
+            const float a0 = (init && col < N) ? init[col] : 0.0f;
+            acc[i][j] = f32x4{a0, a0, a0, a0};
+        }
     for (int k0 = kbeg; k0 < kend; k0 += TK) {
         load_a<TA>(A, lda, M, K, m0, k0, kend, As);
         load_b<TB>(B, ldb, N, K, n0, k0, kend, Bs);
@@ -765,6 +770,7 @@ int icesur_gemm(int32_t trans_a, int32_t trans_b, int32_t M, int32_t N, int32_t 
     SUR_CHECK(!(flags & ICESUR_ACCUM) || d_c, "icesur_gemm: accumulate needs fp32 C");
     SUR_CHECK(!(flags & ICESUR_BIAS) || d_bias, "icesur_gemm: bias missing");
     SUR_CHECK(split_k >= 1 && (split_k == 1 || d_work), "icesur_gemm: split_k needs a workspace");
+    SUR_CHECK(!(flags & ICESUR_ROWINIT) || (split_k == 1 && d_work), "icesur_gemm: row start needs split_k 1 and d_work");
     SUR_CHECK((trans_a ? lda >= M : lda >= K) && (trans_b ? ldb >= K : ldb >= N), "icesur_gemm: leading dimension");
     if (M == 0 || N == 0) return ICEDEM_OK;
     hipStream_t s = (hipStream_t)stream;
@@ -779,7 +785,8 @@ int icesur_gemm(int32_t trans_a, int32_t trans_b, int32_t M, int32_t N, int32_t 
     dim3 grid((M + TM - 1) / TM, (N + TN - 1) / TN, splits);
     float* part = splits > 1 ? d_work : nullptr;
     if (K == 0) part = nullptr;
-#define GEMM_LAUNCH(TA_, TB_) hipLaunchKernelGGL((k_gemm<TA_, TB_>), grid, dim3(256), 0, s, M, N, K, d_a, lda, d_b, ldb, E, ks > 0 ? ks : 1, part)
+    const float* init = (flags & ICESUR_ROWINIT) ? d_work : nullptr;
+#define GEMM_LAUNCH(TA_, TB_) hipLaunchKernelGGL((k_gemm<TA_, TB_>), grid, dim3(256), 0, s, M, N, K, d_a, lda, d_b, ldb, E, ks > 0 ? ks : 1, part, init)
     if (!trans_a && !trans_b) GEMM_LAUNCH(false, false);
     else if (trans_a && !trans_b) GEMM_LAUNCH(true, false);
     else if (!trans_a && trans_b) GEMM_LAUNCH(false, true);

@@ -27,7 +27,8 @@ import torch
 
 from . import _lib
 
-RELU, ACCUM, BIAS = 1, 2, 4
+RELU, ACCUM, BIAS, ROWINIT = 1, 2, 4, 8
+TK = 32   # K step of the GEMM kernels
 
 
 def _p(t: Optional[torch.Tensor]):
@@ -239,16 +240,64 @@ class Surrogate:
         feat = self._cnn(raster)
         return self._mlp(feat, self.arch.feat(), extra, B)
 
-    def predict_shared(self, raster1: torch.Tensor, extra: torch.Tensor, chunk: int = 16384) -> torch.Tensor:
+    def predict_shared(self, raster1: torch.Tensor, extra: torch.Tensor, chunk: int = 16384,
+                       shared_first_layer: bool = True) -> torch.Tensor:
         """Many candidates that share one raster (1, G, G, cin): the CNN runs
-        once, the MLP per candidate; -> (S, out) fp32."""
+        once, the MLP per candidate; -> (S, out) fp32.  With
+        shared_first_layer (and F a multiple of the GEMM's K step) the first
+        layer's feature part is one 1-row product that starts every
+        candidate's accumulators (icesur_gemm ICESUR_ROWINIT); the results are
+        bit-identical to the full product."""
         S = int(extra.shape[0])
+        a = self.arch
+        F = a.feat()
         feat = self._cnn(raster1).clone()
-        out = torch.empty((S, self.arch.out), dtype=torch.float32, device=self.device)
+        out = torch.empty((S, a.out), dtype=torch.float32, device=self.device)
+        fast = shared_first_layer and F % TK == 0 and len(a.hidden) > 0
+        p0 = None
+        if fast:
+            N0 = a.hidden[0]
+            p0 = torch.empty(N0, dtype=torch.float32, device=self.device)
+            self._gemm(0, 0, 1, N0, F, feat, F, self._view(self.wb, "fw0"), N0, Cf=p0, ldc=N0)
         for s0 in range(0, S, chunk):
             e = min(S, s0 + chunk)
-            out[s0:e] = self._mlp(feat, 0, extra[s0:e], e - s0)
+            out[s0:e] = self._mlp_rowinit(p0, extra[s0:e], e - s0) if fast else self._mlp(feat, 0, extra[s0:e], e - s0)
         return out
+
+    def _mlp_rowinit(self, p0: torch.Tensor, extra: torch.Tensor, B: int) -> torch.Tensor:
+        """The MLP of B candidates whose first-layer feature product is p0
+        (fp32, one row): the first layer runs over the extra inputs only,
+        its accumulators starting from p0."""
+        a = self.arch
+        F = a.feat()
+        assert extra.dtype == torch.float32 and tuple(extra.shape) == (B, a.extra)
+        extra = extra.contiguous()
+        lde = _rup(a.extra, 8)
+        ze = self._buf("ze", B * lde, torch.bfloat16)
+        self._chk(self.lib.icesur_concat(_p(extra), 0, 0, _p(extra), a.extra, a.extra, B, _p(ze), lde, self._s()),
+                  "concat")
+        dims = [F + a.extra] + list(a.hidden) + [a.out]
+        N0 = dims[1]
+        w0 = self._view(self.wb, "fw0")[F * N0:]
+        h = self._buf("z1", B * N0, torch.bfloat16)
+        self._chk(self.lib.icesur_gemm(0, 0, B, N0, a.extra, _p(ze), lde, _p(w0), N0, None, 0, _p(h), N0,
+                                       _p(self._view(self.w, "fb0")), BIAS | RELU | ROWINIT, 1, _p(p0), self._s()),
+                  "icesur_gemm")
+        zin, ldin = h, N0
+        nl = len(dims) - 1
+        y = None
+        for i in range(1, nl):
+            N = dims[i + 1]
+            if i < nl - 1:
+                h = self._buf(f"z{i + 1}", B * N, torch.bfloat16)
+                self._gemm(0, 0, B, N, dims[i], zin, ldin, self._view(self.wb, f"fw{i}"), N, Cb=h, ldcb=N,
+                           bias=self._view(self.w, f"fb{i}"), flags=BIAS | RELU)
+                zin, ldin = h, N
+            else:
+                y = self._buf("y", B * N, torch.float32)
+                self._gemm(0, 0, B, N, dims[i], zin, ldin, self._view(self.wb, f"fw{i}"), N, Cf=y, ldc=N,
+                           bias=self._view(self.w, f"fb{i}"), flags=BIAS)
+        return y.view(B, a.out)
 
     # ── backward ────────────────────────────────────────────────────────────
     def backward(self, dy: torch.Tensor, dyb: torch.Tensor):

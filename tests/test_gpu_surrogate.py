@@ -316,3 +316,31 @@ def test_training_reduces_loss():
     last = loss.item()
     print("loss", first, "->", last, flush=True)
     assert last < 0.2 * first
+
+
+def test_shared_first_layer_equals_full_product():
+    """predict_shared's first layer: the shared feature row's product as the
+    accumulators' start, then the extra inputs only -- bit-identical to the
+    full product over the broadcast features, and the row start used alone
+    (ICESUR_ROWINIT with K = 0) returns it unchanged."""
+    from icedem.surrogate import Arch, Surrogate
+    arch = Arch()
+    net = Surrogate(arch, seed=3)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    ras = torch.rand((1, arch.G, arch.G, arch.cin), device="cuda", generator=g).to(torch.bfloat16)
+    ext = torch.randn((3000, arch.extra), device="cuda", generator=g)
+    fast = net.predict_shared(ras, ext, chunk=1024)
+    full = net.predict_shared(ras, ext, chunk=1024, shared_first_layer=False)
+    torch.cuda.synchronize()
+    assert torch.equal(fast, full)
+    assert torch.isfinite(fast).all() and fast.abs().max() > 0
+    # row start alone
+    L, lib = _lib()
+    init = torch.randn(45, device="cuda", generator=g)
+    a = torch.zeros((7, 8), device="cuda", dtype=torch.bfloat16)
+    b = torch.zeros((8, 45), device="cuda", dtype=torch.bfloat16)
+    c = torch.empty((7, 45), device="cuda", dtype=torch.float32)
+    assert lib.icesur_gemm(0, 0, 7, 45, 0, _p(a), 8, _p(b), 45, _p(c), 45, None, 0, None, 8, 1, _p(init), _s()) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(c, init.expand(7, 45))
