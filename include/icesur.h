@@ -56,6 +56,16 @@ int icesur_gemm(int32_t trans_a, int32_t trans_b, int32_t M, int32_t N, int32_t 
 int icesur_conv(const uint16_t* d_x, int32_t B, int32_t H, int32_t W, int32_t C, const uint16_t* d_w, int32_t N,
                 const float* d_bias, int32_t flags, uint16_t* d_wt, uint16_t* d_out, void* stream);
 
+/* The network's first two convolutions in one launch, for inference: d_x
+ * (B, 64, 64, 2) bf16 -> conv (d_w1 (18, 32), d_b1) -> conv (d_w2 (288, 64),
+ * d_b2) -> d_out (B, 16, 16, 64) bf16, each [relu](conv + bias) per flags
+ * (ICESUR_BIAS, ICESUR_RELU).  Bit-identical to two icesur_conv calls; the
+ * intermediate (B, 32, 32, 32) activation is not written.  d_wt1 (18 x 32)
+ * and d_wt2 (288 x 64, 16-byte aligned) are caller workspace for the
+ * transposed weights; d_x and d_out 16-byte aligned; B < 65536. */
+int icesur_conv12(const uint16_t* d_x, int32_t B, const uint16_t* d_w1, const float* d_b1, const uint16_t* d_w2,
+                  const float* d_b2, int32_t flags, uint16_t* d_wt1, uint16_t* d_wt2, uint16_t* d_out, void* stream);
+
 /* 3x3, stride 2, padding 1 patches of an NHWC bf16 tensor (B, H, W, C):
  * d_col row b*Ho*Wo + ho*Wo + wo holds (kh*3 + kw)*C + c, Ho = (H+1)/2,
  * Wo = (W+1)/2, zeros outside the input. */

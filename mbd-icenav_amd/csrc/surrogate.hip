@@ -380,6 +380,126 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ X, in
     }
 }
 
+// ── the first two convolutions fused (inference) ────────────────────────────
+// conv 1 (64x64x2 -> 32x32x32) and conv 2 (-> 16x16x64), 3x3 stride 2
+// padding 1, for R2 conv-2 output rows [R2 h, R2 h + R2) of one sample per
+// workgroup: the 2 R2 + 1 conv-1 rows those read are computed into LDS -- one
+// MFMA per 16 pixels x 16 channels with K = 18 padded to 32, the operands of
+// k_conv's C = 2 path -- and conv 2 reads them from LDS tap by tap, k_conv's
+// K order, so the output is bit-identical to the two k_conv launches while
+// a sample's 64 KB conv-1 activation never leaves the CU.  Conv-1 pixels are
+// held by column parity (stride-2 reads of neighbouring output pixels hit
+// neighbouring 80-byte LDS rows).  Wt1 (32, 18) and Wt2 (64, 288) are the
+// transposed weights.
+constexpr int F1_CP = 40, F2_LD = 64 + 8;
+
+template <int R2>
+__global__ __launch_bounds__(256) void k_conv12(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt1,
+                                                const float* __restrict__ b1, const uint16_t* __restrict__ Wt2,
+                                                const float* __restrict__ b2, int flags, uint16_t* __restrict__ out) {
+    constexpr int F1_ROWS = 2 * R2 + 1, FX_ROWS = 2 * F1_ROWS + 1;
+    constexpr int XS_BYTES = FX_ROWS * 64 * 4;
+    constexpr int W1_BYTES = 32 * KP * 2;
+    constexpr int O1_BYTES = F1_ROWS * 2 * 16 * F1_CP * 2;
+    constexpr int OS_BYTES = 16 * R2 * F2_LD * 2;
+    constexpr int L1_BYTES = XS_BYTES + W1_BYTES + O1_BYTES;
+    constexpr int L_BYTES = L1_BYTES > OS_BYTES ? L1_BYTES : OS_BYTES;
+    __shared__ __align__(16) uint8_t lds[L_BYTES];
+    uint32_t (*Xs)[64] = reinterpret_cast<uint32_t (*)[64]>(lds);                 // 2 channels per dword
+    uint16_t (*W1s)[KP] = reinterpret_cast<uint16_t (*)[KP]>(lds + XS_BYTES);
+    uint16_t (*O1)[2][16][F1_CP] = reinterpret_cast<uint16_t (*)[2][16][F1_CP]>(lds + XS_BYTES + W1_BYTES);
+    uint16_t (*Os)[F2_LD] = reinterpret_cast<uint16_t (*)[F2_LD]>(lds);           // after conv 2's reads
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, kc = lane >> 4;
+    const int b = blockIdx.x, h = blockIdx.y;
+    // conv 2's weight fragments of this lane (output channel 16 w + lane % 16,
+    // k chunk kc of each tap), issued first so they arrive during conv 1
+    const int n2 = 16 * w + (lane & 15), w2 = lane & 15;
+    bf16x8 bw[9];
+    #pragma unroll
+    for (int tap = 0; tap < 9; ++tap)
+        bw[tap] = *reinterpret_cast<const bf16x8*>(Wt2 + (size_t)n2 * 288 + 32 * tap + 8 * kc);
+    // input rows 4 R2 h - 3 .. 4 R2 h + 4 R2 - 1 (zeros outside the image), 4 pixels a chunk
+    const uint32_t* Xb = reinterpret_cast<const uint32_t*>(X) + (size_t)b * 64 * 64;
+    for (int ch = t; ch < FX_ROWS * 16; ch += 256) {
+        const int i = ch >> 4, c4 = (ch & 15) * 4, x = 4 * R2 * h - 3 + i;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (x >= 0 && x < 64) v = *reinterpret_cast<const uint4*>(Xb + x * 64 + c4);
+        *reinterpret_cast<uint4*>(&Xs[i][c4]) = v;
+    }
+    for (int e = t; e < 32 * 32; e += 256) {
+        const int n = e >> 5, k = e & 31;
+        W1s[n][k] = k < 18 ? Wt1[n * 18 + k] : (uint16_t)0;
+    }
+    __syncthreads();
+    // conv 1: tile q = (local row j, half hh), pixels w1 = 16 hh + lane % 16;
+    // this lane's k chunk holds taps 4 kc .. 4 kc + 3 (k = 2 tap + channel)
+    for (int q = w; q < 2 * F1_ROWS; q += 4) {
+        const int j = q >> 1, hh = q & 1;
+        const int r1 = 2 * R2 * h - 1 + j;             // -1: conv 2's zero padding
+        const int w1 = 16 * hh + (lane & 15);
+        uint4 d = make_uint4(0u, 0u, 0u, 0u);
+        uint32_t* dv = reinterpret_cast<uint32_t*>(&d);
+        #pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int tap = 4 * kc + i;
+            if (tap < 9) {
+                const int kh = tap / 3, kw = tap - 3 * (tap / 3);
+                const int col = 2 * w1 - 1 + kw;
+                if (col >= 0 && col < 64) dv[i] = Xs[2 * j + kh][col];
+            }
+        }
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&d);
+        #pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const bf16x8 bb = *reinterpret_cast<const bf16x8*>(&W1s[nt * 16 + (lane & 15)][8 * kc]);
+            const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+            const int n = nt * 16 + (lane & 15);
+            const float bc = (flags & ICESUR_BIAS) ? b1[n] : 0.0f;
+            #pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int px = 16 * hh + 4 * kc + v;
+                float x = acc[v] + bc;
+                if (flags & ICESUR_RELU) x = fmaxf(x, 0.0f);
+                O1[j][px & 1][px >> 1][n] = r1 >= 0 ? f2bf(x) : (uint16_t)0;
+            }
+        }
+    }
+    __syncthreads();
+    // conv 2: wave w owns output channels 16 w .. 16 w + 15 of the R2 rows
+    f32x4 acc2[R2];
+    #pragma unroll
+    for (int rr = 0; rr < R2; ++rr) acc2[rr] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    #pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+        const int kh = tap / 3, kw = tap - 3 * (tap / 3);
+        const bf16x8 bb = bw[tap];
+        const int par = kw == 1 ? 0 : 1, u = kw == 0 ? w2 - 1 : w2;   // conv-1 column 2 w2 - 1 + kw
+        #pragma unroll
+        for (int rr = 0; rr < R2; ++rr) {
+            bf16x8 a = bf16x8{};
+            if (u >= 0) a = *reinterpret_cast<const bf16x8*>(&O1[2 * rr + kh][par][u][8 * kc]);
+            acc2[rr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc2[rr], 0, 0, 0);
+        }
+    }
+    __syncthreads();   // every wave's conv-1 reads are done before Os reuses the space
+    const float bc2 = (flags & ICESUR_BIAS) ? b2[n2] : 0.0f;
+    #pragma unroll
+    for (int rr = 0; rr < R2; ++rr)
+        #pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            float x = acc2[rr][v] + bc2;
+            if (flags & ICESUR_RELU) x = fmaxf(x, 0.0f);
+            Os[rr * 16 + 4 * kc + v][n2] = f2bf(x);
+        }
+    __syncthreads();
+    // the R2 x 16 x 64 output block is contiguous in NHWC
+    uint16_t* ob = out + ((size_t)b * 16 + R2 * h) * 16 * 64;
+    for (int ch = t; ch < 16 * R2 * 8; ch += 256) {
+        const int P = ch >> 3, c8 = (ch & 7) * 8;
+        *reinterpret_cast<uint4*>(ob + (size_t)P * 64 + c8) = *reinterpret_cast<const uint4*>(&Os[P][c8]);
+    }
+}
+
 // (K, N) -> (N, K) bf16
 __global__ void k_transpose16(const uint16_t* __restrict__ a, int K, int N, uint16_t* __restrict__ at) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -826,6 +946,25 @@ int icesur_conv(const uint16_t* d_x, int32_t B, int32_t H, int32_t W, int32_t C,
     else CONV_MODES(32);
 #undef CONV_MODES
 #undef CONV_LAUNCH
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+int icesur_conv12(const uint16_t* d_x, int32_t B, const uint16_t* d_w1, const float* d_b1, const uint16_t* d_w2,
+                  const float* d_b2, int32_t flags, uint16_t* d_wt1, uint16_t* d_wt2, uint16_t* d_out, void* stream) {
+    SUR_CHECK(B >= 0 && d_x && d_w1 && d_w2 && d_wt1 && d_wt2 && d_out, "icesur_conv12: bad arguments");
+    SUR_CHECK(!(flags & ICESUR_BIAS) || (d_b1 && d_b2), "icesur_conv12: bias missing");
+    SUR_CHECK(!(flags & ~(ICESUR_BIAS | ICESUR_RELU)), "icesur_conv12: only bias and relu");
+    SUR_CHECK(((uintptr_t)d_x % 16) == 0 && ((uintptr_t)d_wt2 % 16) == 0 && ((uintptr_t)d_out % 16) == 0,
+              "icesur_conv12: d_x, d_wt2 and d_out must be 16-byte aligned");
+    SUR_CHECK(B < 65536, "icesur_conv12: B >= 65536");
+    if (B == 0) return ICEDEM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_transpose16, dim3(grid1(18 * 32)), dim3(256), 0, s, d_w1, 18, 32, d_wt1);
+    hipLaunchKernelGGL(k_transpose16, dim3(grid1(288 * 64)), dim3(256), 0, s, d_w2, 288, 64, d_wt2);
+    constexpr int R2 = 4;   // conv-2 rows per workgroup: 30 KB of LDS, 5 workgroups per CU
+    hipLaunchKernelGGL((k_conv12<R2>), dim3((unsigned)B, 16 / R2), dim3(256), 0, s, d_x, d_wt1, d_b1, d_wt2, d_b2,
+                       flags, d_out);
     SUR_LAUNCHED();
     return ICEDEM_OK;
 }

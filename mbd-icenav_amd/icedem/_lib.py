@@ -107,6 +107,7 @@ SIGNATURES = {
     "icedem_batch_mode": (C.c_int, [_p, _i]),
     # surrogate kernels (icesur.h)
     "icesur_gemm": (C.c_int, [_i, _i, _i, _i, _i, _p, _i, _p, _i, _p, _i, _p, _i, _p, _i, _i, _p, _p]),
+    "icesur_conv12": (C.c_int, [_p, _i, _p, _p, _p, _p, _i, _p, _p, _p, _p]),
     "icesur_im2col": (C.c_int, [_p, _i, _i, _i, _i, _p, _p]),
     "icesur_col2im": (C.c_int, [_p, _i, _i, _i, _i, _p, _p]),
     "icesur_relu_back": (C.c_int, [_p, _i, _p, _i, _i, _i, _p, _i, _p]),

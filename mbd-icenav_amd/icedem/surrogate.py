@@ -110,6 +110,7 @@ class Surrogate:
         self._bufs: Dict[Tuple[str, int], torch.Tensor] = {}
         self.out_w = torch.ones(self.arch.out, dtype=torch.float32, device=dev)
         self.implicit_conv = True     # False: im2col + GEMM (the same sums, for tests)
+        self.fuse_convs = True        # inference: conv 1 + 2 in one launch (the same sums)
         self.loss = torch.zeros(1, dtype=torch.float32, device=dev)
 
     # ── parameters ──────────────────────────────────────────────────────────
@@ -176,8 +177,11 @@ class Surrogate:
         return int(max(1, min((1024 + tiles - 1) // tiles, K // 512, 1024)))
 
     # ── forward ─────────────────────────────────────────────────────────────
-    def _cnn(self, raster: torch.Tensor) -> torch.Tensor:
-        """(B, G, G, cin) bf16 -> (B, F) bf16 features (the last activation)."""
+    def _cnn(self, raster: torch.Tensor, keep: bool = True) -> torch.Tensor:
+        """(B, G, G, cin) bf16 -> (B, F) bf16 features (the last activation).
+        keep: hold every layer's input and activation for backward; without
+        it the first two convolutions of the default shapes run fused
+        (icesur_conv12, bit-identical)."""
         a = self.arch
         B = int(raster.shape[0])
         assert raster.dtype == torch.bfloat16 and tuple(raster.shape[1:]) == (a.G, a.G, a.cin)
@@ -185,7 +189,20 @@ class Surrogate:
         shapes, _, _ = a.conv_shapes()
         self._acts: List[torch.Tensor] = []
         self._ins: List[torch.Tensor] = []
+        i0 = 0
+        if (not keep and self.fuse_convs and self.implicit_conv and len(shapes) >= 2
+                and tuple(shapes[0]) == (64, 2, 32) and tuple(shapes[1]) == (32, 32, 64)):
+            act = self._buf("act1", B * 16 * 16 * 64, torch.bfloat16)
+            wt0 = self._buf("wt0", 18 * 32, torch.bfloat16)
+            wt1 = self._buf("wt1", 288 * 64, torch.bfloat16)
+            self._chk(self.lib.icesur_conv12(_p(x), B, _p(self._view(self.wb, "cw0")), _p(self._view(self.w, "cb0")),
+                                             _p(self._view(self.wb, "cw1")), _p(self._view(self.w, "cb1")),
+                                             BIAS | RELU, _p(wt0), _p(wt1), _p(act), self._s()), "conv12")
+            x = act
+            i0 = 2
         for i, (s, c, co) in enumerate(shapes):
+            if i < i0:
+                continue
             so = (s + 1) // 2
             rows = B * so * so
             act = self._buf(f"act{i}", rows * co, torch.bfloat16)
@@ -234,10 +251,11 @@ class Surrogate:
         self._B = B
         return y.view(B, a.out)
 
-    def forward(self, raster: torch.Tensor, extra: torch.Tensor) -> torch.Tensor:
-        """raster (B, G, G, cin) bf16, extra (B, arch.extra) fp32 -> (B, out) fp32."""
+    def forward(self, raster: torch.Tensor, extra: torch.Tensor, keep: bool = True) -> torch.Tensor:
+        """raster (B, G, G, cin) bf16, extra (B, arch.extra) fp32 -> (B, out) fp32.
+        keep=False: inference only (no backward after it)."""
         B = int(raster.shape[0])
-        feat = self._cnn(raster)
+        feat = self._cnn(raster, keep)
         return self._mlp(feat, self.arch.feat(), extra, B)
 
     def predict_shared(self, raster1: torch.Tensor, extra: torch.Tensor, chunk: int = 16384,
@@ -380,7 +398,7 @@ class Surrogate:
         out = torch.empty((B, self.arch.out), dtype=torch.float32, device=self.device)
         for s in range(0, B, chunk):
             e = min(B, s + chunk)
-            out[s:e] = self.forward(raster[s:e], extra[s:e])
+            out[s:e] = self.forward(raster[s:e], extra[s:e], keep=False)
         return out
 
 

@@ -344,3 +344,21 @@ def test_shared_first_layer_equals_full_product():
     assert lib.icesur_gemm(0, 0, 7, 45, 0, _p(a), 8, _p(b), 45, _p(c), 45, None, 0, None, 8, 1, _p(init), _s()) == 0
     torch.cuda.synchronize()
     assert torch.equal(c, init.expand(7, 45))
+
+
+@pytest.mark.parametrize("B", [1, 33, 300])
+def test_fused_first_convolutions_equal_separate(B):
+    """icesur_conv12 (inference forward) == the two icesur_conv launches, bit
+    for bit, and the network's predict equals its training-mode forward."""
+    from icedem import surrogate as sur
+    net = sur.Surrogate(sur.Arch(), seed=7)
+    rng = np.random.default_rng(B)
+    raster = _bf(rng.uniform(0, 1, (B, 64, 64, 2)))
+    extra = torch.as_tensor(rng.normal(0, 1, (B, 9)).astype(np.float32), device="cuda")
+    fused = net.predict(raster, extra, chunk=max(B, 1)).clone()
+    full = net.forward(raster, extra).clone()          # keep=True: separate launches
+    np.testing.assert_array_equal(fused.cpu().numpy(), full.cpu().numpy())
+    net._cnn(raster, keep=False)
+    a12 = net._buf("act1", B * 16 * 16 * 64, torch.bfloat16).clone()
+    net._cnn(raster, keep=True)
+    np.testing.assert_array_equal(a12.view(torch.int16).cpu().numpy(), net._acts[1].view(torch.int16).cpu().numpy())
