@@ -65,14 +65,36 @@ __device__ __forceinline__ float rl(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// First-index argmax across the wave of (val, idx); every lane gets the winner.
+// One step of the row-wise argmax: lane l takes lane l - k of its 16-lane row
+// (row_shr:k, DPP control ctrl; lanes below k keep their own pair).
+template <int CTRL>
+__device__ __forceinline__ void argmax_dpp_step(float& val, int& idx) {
+    const float ov = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(val), __float_as_int(val), CTRL,
+                                                                0xf, 0xf, false));
+    const int oi = __builtin_amdgcn_update_dpp(idx, idx, CTRL, 0xf, 0xf, false);
+    if (ov > val || (ov == val && oi < idx)) { val = ov; idx = oi; }
+}
+
+// First-index argmax across the wave of (val, idx); every lane gets the
+// winner.  (value, then smaller index) is a total order on the non-NaN
+// pairs, so the order of the reduction does not change the winner: each
+// 16-lane row is reduced by DPP row shifts (its last lane holds the row's
+// winner), then the four rows' winners are compared in row order.
 __device__ __forceinline__ void wave_argmax(float& val, int& idx) {
+    argmax_dpp_step<0x111>(val, idx);   // row_shr:1
+    argmax_dpp_step<0x112>(val, idx);   // row_shr:2
+    argmax_dpp_step<0x114>(val, idx);   // row_shr:4
+    argmax_dpp_step<0x118>(val, idx);   // row_shr:8
+    float bv = rl(val, 15);
+    int bi = __builtin_amdgcn_readlane(idx, 15);
     #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        float ov = __shfl_xor(val, off, 64);
-        int oi = __shfl_xor(idx, off, 64);
-        if (ov > val || (ov == val && oi < idx)) { val = ov; idx = oi; }
+    for (int r = 1; r < 4; ++r) {
+        const float v = rl(val, 16 * r + 15);
+        const int i = __builtin_amdgcn_readlane(idx, 16 * r + 15);
+        if (v > bv || (v == bv && i < bi)) { bv = v; bi = i; }
     }
+    val = bv;
+    idx = bi;
 }
 
 // ── hull narrow phase (jax_geometry.py) ──────────────────────────────────────
