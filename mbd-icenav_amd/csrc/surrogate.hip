@@ -3,9 +3,10 @@
 // GEMMs run on the matrix cores: v_mfma_f32_16x16x32_bf16, bf16 operands,
 // fp32 accumulation.  A workgroup of 4 wavefronts computes a 64 x 64 tile of
 // C; each wave owns a 32 x 32 quarter as 2 x 2 MFMA tiles.  K advances 32 at
-// a time through LDS: A as (m, k) rows and B as (n, k) rows, so every MFMA
-// operand fragment (8 consecutive k of one row or column) is one 16-byte LDS
-// read.  Split-K writes fp32 partials that a second kernel adds in split
+// a time through two LDS buffers (the next step's global loads in flight
+// during the current step's MFMAs): A as (m, k) rows and B as (n, k) rows,
+// so every MFMA operand fragment (8 consecutive k of one row or column) is
+// one 16-byte LDS read.  Split-K writes fp32 partials that a second kernel adds in split
 // order.  No floating-point atomics anywhere.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -30,79 +31,68 @@ __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint
 
 constexpr int TM = 64, TN = 64, TK = 32, KP = TK + 8;   // LDS rows padded to 80 bytes
 
-// op(A)[m][k] for the tile loader
+// 8 consecutive bf16 of a tile row or column from global memory: one 16-byte
+// load where the run is in range and aligned, else element by element with
+// zeros outside [0, lim) (lim counts from the run's first element)
+__device__ __forceinline__ uint4 fetch8(const uint16_t* __restrict__ p, bool row_ok, int lim) {
+    if (row_ok && lim >= 8 && ((uintptr_t)p & 15) == 0) return *reinterpret_cast<const uint4*>(p);
+    uint16_t v[8];
+    #pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = (row_ok && j < lim) ? p[j] : (uint16_t)0;
+    return make_uint4(v[0] | ((uint32_t)v[1] << 16), v[2] | ((uint32_t)v[3] << 16),
+                      v[4] | ((uint32_t)v[5] << 16), v[6] | ((uint32_t)v[7] << 16));
+}
+__device__ __forceinline__ uint16_t lane16(const uint4& q, int j) {
+    const uint32_t w = j < 2 ? q.x : j < 4 ? q.y : j < 6 ? q.z : q.w;
+    return (uint16_t)((j & 1) ? (w >> 16) : (w & 0xffffu));
+}
+
+// this thread's share of the op(A) tile [m0, m0 + 64) x [k0, k0 + 32) (stored
+// as As[m][k]): 8 consecutive k of one row (A is M x K) or 8 consecutive m
+// of one k (A is K x M)
 template <bool TA>
-__device__ __forceinline__ void load_a(const uint16_t* __restrict__ A, int lda, int M, int K, int m0, int k0,
-                                       int kend, uint16_t (*As)[KP]) {
+__device__ __forceinline__ uint4 fetch_a(const uint16_t* __restrict__ A, int lda, int M, int m0, int k0, int kend) {
     const int t = threadIdx.x;
-    if (!TA) {   // A is M x K: thread loads 8 consecutive k of one row
+    if (!TA) {
         const int m = t >> 2, kq = (t & 3) * 8;
-        const int gm = m0 + m, gk = k0 + kq;
-        uint16_t v[8];
-        const uint16_t* p = A + (size_t)gm * lda + gk;
-        if (gm < M && gk + 7 < kend && ((uintptr_t)p & 15) == 0) {
-            const uint4 q = *reinterpret_cast<const uint4*>(p);
-            v[0] = q.x & 0xffff; v[1] = q.x >> 16; v[2] = q.y & 0xffff; v[3] = q.y >> 16;
-            v[4] = q.z & 0xffff; v[5] = q.z >> 16; v[6] = q.w & 0xffff; v[7] = q.w >> 16;
-        } else {
-            #pragma unroll
-            for (int j = 0; j < 8; ++j) v[j] = (gm < M && gk + j < kend) ? p[j] : (uint16_t)0;
-        }
-        #pragma unroll
-        for (int j = 0; j < 8; ++j) As[m][kq + j] = v[j];
-    } else {     // A is K x M: thread loads 8 consecutive m of one k
+        return fetch8(A + (size_t)(m0 + m) * lda + k0 + kq, m0 + m < M, kend - (k0 + kq));
+    }
+    const int k = t >> 3, mq = (t & 7) * 8;
+    return fetch8(A + (size_t)(k0 + k) * lda + m0 + mq, k0 + k < kend, M - (m0 + mq));
+}
+template <bool TA>
+__device__ __forceinline__ void stage_a(const uint4& q, uint16_t (*As)[KP]) {
+    const int t = threadIdx.x;
+    if (!TA) {
+        *reinterpret_cast<uint4*>(&As[t >> 2][(t & 3) * 8]) = q;
+    } else {
         const int k = t >> 3, mq = (t & 7) * 8;
-        const int gk = k0 + k, gm = m0 + mq;
-        const uint16_t* p = A + (size_t)gk * lda + gm;
-        uint16_t v[8];
-        if (gk < kend && gm + 7 < M && ((uintptr_t)p & 15) == 0) {
-            const uint4 q = *reinterpret_cast<const uint4*>(p);
-            v[0] = q.x & 0xffff; v[1] = q.x >> 16; v[2] = q.y & 0xffff; v[3] = q.y >> 16;
-            v[4] = q.z & 0xffff; v[5] = q.z >> 16; v[6] = q.w & 0xffff; v[7] = q.w >> 16;
-        } else {
-            #pragma unroll
-            for (int j = 0; j < 8; ++j) v[j] = (gk < kend && gm + j < M) ? p[j] : (uint16_t)0;
-        }
         #pragma unroll
-        for (int j = 0; j < 8; ++j) As[mq + j][k] = v[j];
+        for (int j = 0; j < 8; ++j) As[mq + j][k] = lane16(q, j);
     }
 }
 
-// op(B)[k][n] stored as Bs[n][k]
+// op(B)[k][n] stored as Bs[n][k]: 8 consecutive n of one k (B is K x N) or
+// 8 consecutive k of one row (B is N x K)
 template <bool TB>
-__device__ __forceinline__ void load_b(const uint16_t* __restrict__ B, int ldb, int N, int K, int n0, int k0,
-                                       int kend, uint16_t (*Bs)[KP]) {
+__device__ __forceinline__ uint4 fetch_b(const uint16_t* __restrict__ B, int ldb, int N, int n0, int k0, int kend) {
     const int t = threadIdx.x;
-    if (!TB) {   // B is K x N: thread loads 8 consecutive n of one k
+    if (!TB) {
         const int k = t >> 3, nq = (t & 7) * 8;
-        const int gk = k0 + k, gn = n0 + nq;
-        const uint16_t* p = B + (size_t)gk * ldb + gn;
-        uint16_t v[8];
-        if (gk < kend && gn + 7 < N && ((uintptr_t)p & 15) == 0) {
-            const uint4 q = *reinterpret_cast<const uint4*>(p);
-            v[0] = q.x & 0xffff; v[1] = q.x >> 16; v[2] = q.y & 0xffff; v[3] = q.y >> 16;
-            v[4] = q.z & 0xffff; v[5] = q.z >> 16; v[6] = q.w & 0xffff; v[7] = q.w >> 16;
-        } else {
-            #pragma unroll
-            for (int j = 0; j < 8; ++j) v[j] = (gk < kend && gn + j < N) ? p[j] : (uint16_t)0;
-        }
+        return fetch8(B + (size_t)(k0 + k) * ldb + n0 + nq, k0 + k < kend, N - (n0 + nq));
+    }
+    const int n = t >> 2, kq = (t & 3) * 8;
+    return fetch8(B + (size_t)(n0 + n) * ldb + k0 + kq, n0 + n < N, kend - (k0 + kq));
+}
+template <bool TB>
+__device__ __forceinline__ void stage_b(const uint4& q, uint16_t (*Bs)[KP]) {
+    const int t = threadIdx.x;
+    if (!TB) {
+        const int k = t >> 3, nq = (t & 7) * 8;
         #pragma unroll
-        for (int j = 0; j < 8; ++j) Bs[nq + j][k] = v[j];
-    } else {     // B is N x K: thread loads 8 consecutive k of one row
-        const int n = t >> 2, kq = (t & 3) * 8;
-        const int gn = n0 + n, gk = k0 + kq;
-        uint16_t v[8];
-        const uint16_t* p = B + (size_t)gn * ldb + gk;
-        if (gn < N && gk + 7 < kend && ((uintptr_t)p & 15) == 0) {
-            const uint4 q = *reinterpret_cast<const uint4*>(p);
-            v[0] = q.x & 0xffff; v[1] = q.x >> 16; v[2] = q.y & 0xffff; v[3] = q.y >> 16;
-            v[4] = q.z & 0xffff; v[5] = q.z >> 16; v[6] = q.w & 0xffff; v[7] = q.w >> 16;
-        } else {
-            #pragma unroll
-            for (int j = 0; j < 8; ++j) v[j] = (gn < N && gk + j < kend) ? p[j] : (uint16_t)0;
-        }
-        #pragma unroll
-        for (int j = 0; j < 8; ++j) Bs[n][kq + j] = v[j];
+        for (int j = 0; j < 8; ++j) Bs[nq + j][k] = lane16(q, j);
+    } else {
+        *reinterpret_cast<uint4*>(&Bs[t >> 2][(t & 3) * 8]) = q;
     }
 }
 
@@ -126,8 +116,10 @@ template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_t* __restrict__ A, int lda,
                                               const uint16_t* __restrict__ B, int ldb, Epi E, int ks,
                                               float* __restrict__ part, const float* __restrict__ init) {
-    __shared__ __align__(16) uint16_t As[TM][KP];
-    __shared__ __align__(16) uint16_t Bs[TN][KP];
+    // two LDS buffers: the next K step's global loads are in flight while
+    // the current step's MFMAs run
+    __shared__ __align__(16) uint16_t As[2][TM][KP];
+    __shared__ __align__(16) uint16_t Bs[2][TN][KP];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
@@ -142,20 +134,28 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_
             const float a0 = (init && col < N) ? init[col] : 0.0f;
             acc[i][j] = f32x4{a0, a0, a0, a0};
         }
+    int cur = 0;
+    if (kbeg < kend) {
+        stage_a<TA>(fetch_a<TA>(A, lda, M, m0, kbeg, kend), As[0]);
+        stage_b<TB>(fetch_b<TB>(B, ldb, N, n0, kbeg, kend), Bs[0]);
+    }
+    __syncthreads();
     for (int k0 = kbeg; k0 < kend; k0 += TK) {
-        load_a<TA>(A, lda, M, K, m0, k0, kend, As);
-        load_b<TB>(B, ldb, N, K, n0, k0, kend, Bs);
-        __syncthreads();
+        const bool more = k0 + TK < kend;
+        uint4 qa = make_uint4(0u, 0u, 0u, 0u), qb = qa;
+        if (more) { qa = fetch_a<TA>(A, lda, M, m0, k0 + TK, kend); qb = fetch_b<TB>(B, ldb, N, n0, k0 + TK, kend); }
         bf16x8 a[2], b[2];
         #pragma unroll
-        for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&As[wm + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+        for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&As[cur][wm + i * 16 + (lane & 15)][8 * (lane >> 4)]);
         #pragma unroll
-        for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn + j * 16 + (lane & 15)][8 * (lane >> 4)]);
+        for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&Bs[cur][wn + j * 16 + (lane & 15)][8 * (lane >> 4)]);
         #pragma unroll
         for (int i = 0; i < 2; ++i)
             #pragma unroll
             for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        if (more) { stage_a<TA>(qa, As[cur ^ 1]); stage_b<TB>(qb, Bs[cur ^ 1]); }
         __syncthreads();
+        cur ^= 1;
     }
     // D fragment: lane holds column lane % 16, rows 4 (lane / 16) + v
     #pragma unroll
