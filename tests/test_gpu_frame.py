@@ -45,6 +45,27 @@ def test_frame_parity_c0_current_dilation():
     _run(case, 900, check_every=10, start_check=700)
 
 
+@pytest.mark.parametrize("cap", [None, 2e4])
+def test_frame_parity_c0_thrust_cap(cap):
+    """The ship-force totals without a thrust cap, and with a cap small
+    enough that the scale is below 1 in contact (the scaled partials, the
+    scaled force magnitudes of the fracture pass and floe eligibility)."""
+    from icedem.config import benchmark_config
+    cfg = benchmark_config(max_ship_ice_force=cap)
+    case = Case.make("c0", n_frames=1010, cfg=cfg)
+    st, ship = _run(case, 1000, check_every=10, start_check=700)
+    if cap is not None:
+        # the cap is active: the ship's reaction is the capped total
+        eng = case.engine()
+        peak = 0.0
+        for k in range(1000, 1010):
+            diag = eng.alloc_diag()
+            st, ship = case.gpu_frame(st, ship, k, diag=diag)
+            peak = max(peak, float(torch.linalg.vector_norm(diag["F_ice_on_ship"])))
+        print("peak ship reaction", peak, "cap", cap, flush=True)
+        assert 0.5 * cap < peak <= cap * (1 + 1e-4)
+
+
 @pytest.mark.parametrize("lanes,cell", [(0, 1.0), (8, 1.0), (0, 0.55), (8, 0.55)])
 def test_frame_parity_field(lanes, cell):
     """cell 0.55: reach boxes of up to 5x5 fine cells, scan lists longer
