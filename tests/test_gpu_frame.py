@@ -66,6 +66,18 @@ def test_frame_parity_c0_thrust_cap(cap):
         assert 0.5 * cap < peak <= cap * (1 + 1e-4)
 
 
+@pytest.mark.parametrize("factor,lanes", [(0.5, 0), (0.36, 0), (0.36, 8)])
+def test_frame_parity_c0_ice_substeps(factor, lanes):
+    """Ice sub-steps (the Rayleigh time-step cap: 3 and 4 sub-steps per
+    frame): cell rebuilds between sub-steps, the kept ship forces, the
+    next-state buffers' parity."""
+    from icedem.config import benchmark_config
+    cfg = benchmark_config(use_rayleigh_dt_cap=True, rayleigh_safety_factor=factor)
+    case = Case.make("c0", n_frames=900, cfg=cfg, force_lanes=lanes)
+    assert case.engine().params.n_ice_steps == (3 if factor == 0.5 else 4)
+    _run(case, 900, check_every=10, start_check=700)
+
+
 @pytest.mark.parametrize("lanes,cell", [(0, 1.0), (8, 1.0), (0, 0.55), (8, 0.55)])
 def test_frame_parity_field(lanes, cell):
     """cell 0.55: reach boxes of up to 5x5 fine cells, scan lists longer
