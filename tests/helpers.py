@@ -52,6 +52,7 @@ class Case:
     pair_mode: str = "all"
     force_lanes: int = 0          # lanes per element of the GPU force pass (0: the library's choice)
     cell_factor: float = 1.0      # GPU fine grid cell / 2 (r_s + delta)
+    bucket_caps: Optional[tuple] = None   # GPU (fine, coarse) bucket capacities in place of the sized ones
 
     @staticmethod
     def make(name: str, n_frames: int = 200, seed: int = 0, cfg: Optional[SimulationConfig] = None,
@@ -158,9 +159,16 @@ class Case:
             sd = build_ship_dynamics_data(sp, self.cfg, self.ship_length, self.ship_beam, ship_hull_local=self.hull)
             dt_ice, nsteps = _time_stepping(static, self.cfg, self.dt)
             B, T_n = _thruster_config(sp, self.ship_length)
+            grid = None
+            if self.bucket_caps is not None:
+                from icedem.engine import choose_grid
+                grid = list(choose_grid(np.asarray(sc.centroids, dtype=np.float64), static.radii,
+                                        float(getattr(self.cfg, "contact_dilation", 0.0)),
+                                        cell_factor=self.cell_factor))
+                grid[6], grid[7] = self.bucket_caps
             self._engine = DemEngine(static, self.cfg, sd, self.ship_length, self.ship_beam, nsteps, self.dt, dt_ice,
                                      self.use_auto, B, T_n, centroids=sc.centroids, force_lanes=self.force_lanes,
-                                     cell_factor=self.cell_factor)
+                                     cell_factor=self.cell_factor, grid=grid)
         return self._engine
 
     # ── states ──────────────────────────────────────────────────────────────

@@ -66,6 +66,15 @@ def test_frame_parity_c0_thrust_cap(cap):
         assert 0.5 * cap < peak <= cap * (1 + 1e-4)
 
 
+@pytest.mark.parametrize("lanes", [0, 8])
+def test_frame_parity_field_bucket_overflow(lanes):
+    """Buckets of capacity 1: most insertions spill into the overflow list,
+    which every element scans; the contact set and sums stay the oracle's."""
+    case = Case.make("c1small", n_frames=120, force_lanes=lanes)
+    case.bucket_caps = (1, 1)
+    _run(case, 120, check_every=4)
+
+
 @pytest.mark.parametrize("factor,lanes", [(0.5, 0), (0.36, 0), (0.36, 8)])
 def test_frame_parity_c0_ice_substeps(factor, lanes):
     """Ice sub-steps (the Rayleigh time-step cap: 3 and 4 sub-steps per

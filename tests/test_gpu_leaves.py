@@ -9,9 +9,10 @@ from oracle import bonds as obonds
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["c1small", "ring"])
-def test_contact_pairs_exact(name):
+@pytest.mark.parametrize("name,caps", [("c1small", None), ("ring", None), ("c1small", (1, 1))])
+def test_contact_pairs_exact(name, caps):
     case = Case.make(name)
+    case.bucket_caps = caps
     eng = case.engine()
     o = case.oracle()
     rng = np.random.default_rng(1)
