@@ -194,6 +194,13 @@ int icedem_version(void);
 int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, IceDemCtx** out);
 int icedem_destroy(IceDemCtx* ctx);
 
+/* No reference counterpart (many rollouts of one scene): a context for the
+ * same scene, parameters and device as base that shares base's static
+ * tables (read-only) and has its own state scratch.  The tables are freed
+ * with the last context that uses them, whichever order they are destroyed
+ * in. */
+int icedem_create_like(IceDemCtx* base, IceDemCtx** out);
+
 /* Replaces JaxSimContext.scan_segment / frame_step
  * (jax_simulation.py:1328-1335, 397-699): runs K frames in place on `state`;
  * K = 1 is frame_step.  use_graph != 0 replays a cached hipGraph of the

@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 #include <algorithm>
@@ -328,6 +329,18 @@ __global__ void k_ship_rollouts(const DevView* __restrict__ V, const float* __re
 // Host side
 // ═════════════════════════════════════════════════════════════════════════════
 
+// The scene's static tables (polygons, masses, bonds, bond table and
+// parameters, hull): read-only after creation, shared by every context made
+// with icedem_create_like and freed with the last of them.
+struct StaticTables {
+    int device = 0;
+    std::vector<void*> allocs;
+    ~StaticTables() {
+        (void)hipSetDevice(device);
+        for (void* p : allocs) (void)hipFree(p);
+    }
+};
+
 struct IceDemCtx {
     int device = 0;
     DevView hv{};              // host copy of the device view
@@ -335,7 +348,8 @@ struct IceDemCtx {
     CallDesc* dd = nullptr;    // device call descriptor
     CallDesc* hd = nullptr;    // pinned host staging for the descriptor
     hipEvent_t desc_done = nullptr;
-    std::vector<void*> allocs;
+    std::shared_ptr<StaticTables> statics;
+    std::vector<void*> allocs; // this context's own buffers
     std::map<int, hipGraphExec_t> graphs;
     std::map<int, hipGraph_t> graph_defs;
     std::string err;
@@ -360,23 +374,27 @@ static void set_err(IceDemCtx* ctx, const std::string& s) {
 // error text for the surrogate entry points (surrogate.hip)
 void icedem_set_error(const std::string& msg) { set_err(nullptr, msg); }
 
+// device buffer of count T (at least one) owned by the context, or with
+// stat by its static tables
 template <class T>
-static int dalloc(IceDemCtx* ctx, T** p, size_t count) {
+static int dalloc(IceDemCtx* ctx, T** p, size_t count, bool stat = false) {
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
     void* q = nullptr;
     HIP_TRY(hipMalloc(&q, bytes));
-    ctx->allocs.push_back(q);
+    (stat ? ctx->statics->allocs : ctx->allocs).push_back(q);
     *p = reinterpret_cast<T*>(q);
     return 0;
 }
 
 template <class T>
-static int dupload(IceDemCtx* ctx, T** p, const T* src, size_t count) {
-    int rc = dalloc(ctx, p, count);
+static int dupload(IceDemCtx* ctx, T** p, const T* src, size_t count, bool stat = false) {
+    int rc = dalloc(ctx, p, count, stat);
     if (rc) return rc;
     if (count) HIP_TRY(hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice));
     return 0;
 }
+
+static int alloc_state_buffers(IceDemCtx* ctx);
 
 extern "C" {
 
@@ -443,6 +461,8 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     }
     ctx = new IceDemCtx();
     ctx->device = device;
+    ctx->statics = std::make_shared<StaticTables>();
+    ctx->statics->device = device;
     HIP_TRY(hipSetDevice(device));
     DevView& H = ctx->hv;
     H.p = *prm;
@@ -477,8 +497,7 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     }
     const size_t nv = (size_t)n * st->max_v;
     int rc = 0;
-#define UP(field, type, src, cnt) do { type* _p; rc = dupload(ctx, &_p, (const type*)(src), (cnt)); if (rc) { icedem_destroy(ctx); return rc; } H.field = _p; } while (0)
-#define AL(field, type, cnt) do { type* _p; rc = dalloc(ctx, &_p, (cnt)); if (rc) { icedem_destroy(ctx); return rc; } H.field = _p; } while (0)
+#define UP(field, type, src, cnt) do { type* _p; rc = dupload(ctx, &_p, (const type*)(src), (cnt), true); if (rc) { icedem_destroy(ctx); return rc; } H.field = _p; } while (0)
     UP(polys, float2, st->h_local_polys, nv);
     UP(mask, uint8_t, st->h_poly_mask, nv);
     UP(mass, float, st->h_masses, n);
@@ -528,15 +547,13 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
         }
         UP(bdeg, int32_t, deg.data(), n);
         UP(btab, int4, tab.data(), tw);
-        AL(bprm, float2, tw);
+        float2* bp;
+        rc = dalloc(ctx, &bp, tw, true);
+        if (rc) { icedem_destroy(ctx); return rc; }
+        H.bprm = bp;
     }
-    AL(pen, float, n); AL(nx, float, n); AL(ny, float, n); AL(fm_raw, float, n);
-    AL(soi_x, float, n); AL(soi_y, float, n); AL(mterm, float, n); AL(fm, float, n);
-    AL(shipf, float, 2 * (size_t)n);
-    AL(nearf, uint8_t, n); AL(near_list, int32_t, n);
-    AL(nearcnt, int32_t, nblk); AL(counters, int32_t, 4);
-    AL(crack, uint8_t, n);
-    {   // bucket records of both grids in one allocation: fine sets, then coarse
+#undef UP
+    {   // bucket record sizes
         auto rec_bytes = [](int64_t cap) { return ((16 + 24 * cap + 127) / 128) * 128; };
         H.rec_f = rec_bytes(prm->grid_bucket);
         H.rec_c = rec_bytes(H.cap2);
@@ -545,15 +562,56 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
         if (2 * (int64_t)H.G + 2 * (int64_t)H.G2 > (int64_t)INT32_MAX || total >= (1LL << 32)) {
             set_err(nullptr, "bucket records beyond the addressable range"); icedem_destroy(ctx); return ICEDEM_E_LIMIT;
         }
-        AL(cells, uint8_t, (size_t)total);
     }
+    rc = alloc_state_buffers(ctx);
+    if (rc) { icedem_destroy(ctx); return rc; }
+    if ((size_t)n * H.W > 0) {
+        const size_t tw = (size_t)n * H.W;
+        hipLaunchKernelGGL(k_bond_prm, dim3((unsigned)((tw + 255) / 256)), dim3(256), 0, 0, (const DevView*)ctx->dv);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    *out = ctx;
+    return ICEDEM_OK;
+}
+
+int icedem_create_like(IceDemCtx* base, IceDemCtx** out) {
+    if (!base || !out) { set_err(nullptr, "null argument"); return ICEDEM_E_INVALID; }
+    *out = nullptr;
+    IceDemCtx* ctx = new IceDemCtx();
+    ctx->device = base->device;
+    ctx->statics = base->statics;
+    ctx->hv = base->hv;                 // sizes, parameters and the static tables' pointers
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int rc = alloc_state_buffers(ctx);
+    if (rc) { icedem_destroy(ctx); return rc; }
+    HIP_TRY(hipDeviceSynchronize());
+    *out = ctx;
+    return ICEDEM_OK;
+}
+
+}  // extern "C"
+
+// The context's own buffers (per-frame scratch, buckets, counters, next
+// state), zeroed where the frame kernels expect zeros, and the device view.
+static int alloc_state_buffers(IceDemCtx* ctx) {
+    DevView& H = ctx->hv;
+    const int n = H.n, nblk = H.nblk, nbpad = H.nbpad;
+    int rc = 0;
+#define AL(field, type, cnt) do { type* _p; rc = dalloc(ctx, &_p, (cnt)); if (rc) return rc; H.field = _p; } while (0)
+    AL(pen, float, n); AL(nx, float, n); AL(ny, float, n); AL(fm_raw, float, n);
+    AL(soi_x, float, n); AL(soi_y, float, n); AL(mterm, float, n); AL(fm, float, n);
+    AL(shipf, float, 2 * (size_t)n);
+    AL(nearf, uint8_t, n); AL(near_list, int32_t, n);
+    AL(nearcnt, int32_t, nblk); AL(counters, int32_t, 4);
+    AL(crack, uint8_t, n);
+    AL(cells, uint8_t, (size_t)(H.coarse_off + 2 * (int64_t)H.G2 * H.rec_c));   // fine sets, then coarse
     AL(ovf_list, int32_t, 2 * (size_t)H.ovf_cap); AL(ovf_count, int32_t, 2);
     AL(floe_coll, uint32_t, H.nf); AL(floe_maxfm, uint32_t, H.nf);
     AL(part1, float, 3 * (size_t)nbpad); AL(part2, float, 2 * (size_t)nbpad); AL(scal, float, 16);
     AL(nxt_pos, float, 2 * (size_t)n); AL(nxt_vel, float, 2 * (size_t)n);
     AL(nxt_rot, float, n); AL(nxt_omega, float, n);
     AL(err, int32_t, 4);
-#undef UP
 #undef AL
     HIP_TRY(hipMemset(H.cells, 0, (size_t)(H.coarse_off + 2 * (int64_t)H.G2 * H.rec_c)));
     HIP_TRY(hipMemset(H.ovf_count, 0, sizeof(int32_t) * 2));
@@ -569,26 +627,21 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     {
         DevView* d;
         rc = dalloc(ctx, &d, 1);
-        if (rc) { icedem_destroy(ctx); return rc; }
+        if (rc) return rc;
         ctx->dv = d;
         CallDesc* cd;
         rc = dalloc(ctx, &cd, 1);
-        if (rc) { icedem_destroy(ctx); return rc; }
+        if (rc) return rc;
         ctx->dd = cd;
     }
     HIP_TRY(hipMemcpy(ctx->dv, &H, sizeof(DevView), hipMemcpyHostToDevice));
-    if ((size_t)n * H.W > 0) {
-        const size_t tw = (size_t)n * H.W;
-        hipLaunchKernelGGL(k_bond_prm, dim3((unsigned)((tw + 255) / 256)), dim3(256), 0, 0, (const DevView*)ctx->dv);
-        HIP_TRY(hipGetLastError());
-    }
     HIP_TRY(hipHostMalloc((void**)&ctx->hd, sizeof(CallDesc), hipHostMallocDefault));
     memset(ctx->hd, 0, sizeof(CallDesc));
     HIP_TRY(hipEventCreateWithFlags(&ctx->desc_done, hipEventDisableTiming));
-    HIP_TRY(hipDeviceSynchronize());
-    *out = ctx;
     return ICEDEM_OK;
 }
+
+extern "C" {
 
 static int validate_state(IceDemCtx* ctx, const IceDemState* s) {
     if (!s || !s->d_pos || !s->d_vel || !s->d_rot || !s->d_omega || !s->d_sub || !s->d_active || !s->d_ship ||

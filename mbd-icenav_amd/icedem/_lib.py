@@ -96,6 +96,7 @@ SIGNATURES = {
     "icedem_version": (C.c_int, []),
     "icedem_create": (C.c_int, [C.POINTER(IceDemStatic), C.POINTER(IceDemParams), C.c_int, C.POINTER(_p)]),
     "icedem_destroy": (C.c_int, [_p]),
+    "icedem_create_like": (C.c_int, [_p, C.POINTER(_p)]),
     "icedem_frames": (C.c_int, [_p, C.POINTER(IceDemState), C.POINTER(IceDemInputs), _i,
                                 C.POINTER(IceDemHistory), C.POINTER(IceDemDiag), _i, _p]),
     "icedem_frames_profiled": (C.c_int, [_p, C.POINTER(IceDemState), C.POINTER(IceDemInputs), _i,

@@ -273,6 +273,21 @@ class DemEngine:
         self.ctx = ctx
         self.max_v = st.max_v
 
+    @classmethod
+    def like(cls, base: "DemEngine") -> "DemEngine":
+        """An engine for base's scene, parameters and device that shares its
+        static tables on the GPU (icedem_create_like), with its own state
+        scratch: many rollout instances of one scene."""
+        eng = cls.__new__(cls)
+        for k in ("lib", "device", "static", "cfg", "ship_dyn", "n", "n_bonds", "n_floes", "dt", "grid", "params",
+                  "_host", "max_v"):
+            setattr(eng, k, getattr(base, k))
+        ctx = C.c_void_p()
+        with torch.cuda.device(eng.device):
+            _lib.check(eng.lib.icedem_create_like(base.ctx, C.byref(ctx)), None, "icedem_create_like")
+        eng.ctx = ctx
+        return eng
+
     def near_stats(self, st: IceState, ship_buf: torch.Tensor):
         """Elements passing the near-ship box test for the current state, and
         the 256-element blocks holding one (host-side count, for traffic

@@ -86,8 +86,12 @@ class RolloutBatch:
         from .simulation import _thruster_config, _time_stepping
         B, T_n = _thruster_config(sp, self.ship.length)
         statics = {}
+        first = {}     # scene -> its first engine: later instances share its static tables
         self.engines: List[DemEngine] = []
         for s_idx in self.inst:
+            if s_idx in first:
+                self.engines.append(DemEngine.like(first[s_idx]))
+                continue
             if s_idx not in statics:
                 sc = self.scenes[s_idx]
                 statics[s_idx] = build_static_data(sc.polygons, sc.centroids, sc.bonds, sc.thicknesses,
@@ -103,6 +107,7 @@ class RolloutBatch:
                             cell_size=rmin * cell_size_factor, ship_mode=1,
                             ctrl_thrust_max=self.ship.thrust_max, ctrl_yaw_rate_max=self.ship.yaw_rate_max,
                             grid_margin=25.0, small_quantile=small_quantile, force_lanes=force_lanes)
+            first[s_idx] = eng
             self.engines.append(eng)
         self._alloc_states()
         ctxs = (C.c_void_p * self.R)(*[e.ctx.value for e in self.engines])

@@ -164,3 +164,32 @@ def test_history_lengths_and_goal_line():
     assert len(pf[0]) == len(ck[0]) == f + 2
     np.testing.assert_array_equal(np.asarray(pf[0][-1]), np.asarray(ck[0][-1]))
     np.testing.assert_array_equal(_eta(pf[3][-1]), _eta(ck[3][-1]))
+
+
+def test_engine_like_shares_tables_and_outlives_base():
+    """icedem_create_like: a second context of the same scene runs the same
+    frames bit for bit, and keeps working after the context whose tables it
+    shares is destroyed."""
+    import numpy as np
+    import torch
+    from helpers import Case
+    from icedem.engine import DemEngine
+    case = Case.make("c1small", n_frames=70)
+    base = case.engine()
+    twin = DemEngine.like(base)
+    sa, sha = case.gpu_initial()
+    sb, shb = case.gpu_initial()
+    sp1 = torch.as_tensor(np.stack([case.setpoint_row(k) for k in range(30)]), device=base.device)
+    sp2 = torch.as_tensor(np.stack([case.setpoint_row(k) for k in range(30, 60)]), device=base.device)
+    base.run_frames(sa, sha, sp1, use_graph=False)
+    twin.run_frames(sb, shb, sp1, use_graph=True)
+    assert torch.equal(sa.positions, sb.positions) and torch.equal(sha, shb)
+    assert torch.equal(sa.bond_intact, sb.bond_intact)
+    base.close()
+    case._engine = None
+    fresh = case.engine()
+    fresh.run_frames(sa, sha, sp2, use_graph=False)
+    twin.run_frames(sb, shb, sp2, use_graph=True)
+    assert torch.equal(sa.positions, sb.positions) and torch.equal(sha, shb)
+    assert torch.equal(sa.velocities, sb.velocities) and torch.equal(sa.bond_intact, sb.bond_intact)
+    twin.close()
