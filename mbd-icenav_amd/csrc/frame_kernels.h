@@ -695,18 +695,30 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
 // contact and bond sums, and the flags (act_raw | act_bond << 1 | crack << 2)
 enum : int { S_PX, S_PY, S_VX, S_VY, S_RO, S_OM, S_R, S_H, S_AREA, S_MASS, S_INERTIA, S_SUB0, S_PEN, S_FMR, S_SX,
              S_SY, S_NXE, S_NYE, S_VWX, S_VWY, S_FIX, S_FIY, S_MI, S_FBX, S_FBY, S_MB, S_FLAGS, S_N };
-constexpr int SPAD = S_N + 2;   // odd row stride: a column read by E threads hits E distinct banks
+// Per group, one row of CFS floats holds the partners' x, y, vx, vy and
+// radius (MAXC2 each) during the contact pass; once the group's lane 0 has
+// summed the contact terms the row is dead and its first S_N floats stage
+// the element's epilogue.  Sharing the row keeps the 8-lane block at
+// 19.5 KB of LDS, 8 blocks per CU.  CFS = 84 = 20 mod 32: the 8 groups of a
+// wave start on distinct banks.
+constexpr int CFS = 5 * MAXC2 + 4;
+static_assert(S_N <= CFS, "epilogue staging must fit a partner row");
 
 template <int GR>
 struct ForceLds {
     static constexpr int E = BLOCK / GR;
-    float stg[E][SPAD];
     int ck[E][MAXC2];                    // partner's rotated index rot_key(j, e, n)
-    float cx[E][MAXC2], cy[E][MAXC2], cvx[E][MAXC2], cvy[E][MAXC2], cr[E][MAXC2];
+    float cf[E][CFS];
     int ord[E][MAXC2];
     uint32_t grec[E][GR];                // scan-list cell's record offset
     int goff[E][GR];                     // its first slot in the group's slot order
     float bf[E][3][GR];
+    __device__ float* cx(int g) { return cf[g]; }
+    __device__ float* cy(int g) { return cf[g] + MAXC2; }
+    __device__ float* cvx(int g) { return cf[g] + 2 * MAXC2; }
+    __device__ float* cvy(int g) { return cf[g] + 3 * MAXC2; }
+    __device__ float* cr(int g) { return cf[g] + 4 * MAXC2; }
+    __device__ float* stg(int g) { return cf[g]; }
 };
 
 __device__ __forceinline__ void wave_sync_lds() {
@@ -826,7 +838,7 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
         if (ov && count + before < MAXC2) {
             const int s = count + before;
             L.ck[g][s] = rot_key(j, e, n);
-            L.cx[g][s] = x; L.cy[g][s] = y; L.cvx[g][s] = vx; L.cvy[g][s] = vy; L.cr[g][s] = r;
+            L.cx(g)[s] = x; L.cy(g)[s] = y; L.cvx(g)[s] = vx; L.cvy(g)[s] = vy; L.cr(g)[s] = r;
         }
         count += __builtin_popcount(m);
     };
@@ -881,7 +893,7 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
             ca[u] = 0.0f; cb[u] = 0.0f; cc[u] = 0.0f;
             if (i < count) {
                 const int s = L.ord[g][i];
-                const float x = L.cx[g][s], y = L.cy[g][s], vx = L.cvx[g][s], vy = L.cvy[g][s], r = L.cr[g][s];
+                const float x = L.cx(g)[s], y = L.cy(g)[s], vx = L.cvx(g)[s], vy = L.cvy(g)[s], r = L.cr(g)[s];
                 const bool lo = L.ck[g][s] < n - e;   // e < j
                 contrib_v(P, lo, lo ? px : x, lo ? py : y, lo ? pvx : vx, lo ? pvy : vy, lo ? pr : r,
                           lo ? x : px, lo ? y : py, lo ? vx : pvx, lo ? vy : pvy, lo ? r : pr, ca[u], cb[u], cc[u]);
@@ -891,11 +903,11 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
         #pragma unroll
         for (int u = 0; u < PER; ++u) {
             const int i = l + u * GR;
-            if (i < count) { L.cx[g][i] = ca[u]; L.cy[g][i] = cb[u]; L.cvx[g][i] = cc[u]; }
+            if (i < count) { L.cx(g)[i] = ca[u]; L.cy(g)[i] = cb[u]; L.cvx(g)[i] = cc[u]; }
         }
         wave_sync_lds();
         if (l == 0)
-            for (int i = 0; i < count; ++i) { fx = fx + L.cx[g][i]; fy = fy + L.cy[g][i]; fm = fm + L.cvx[g][i]; }
+            for (int i = 0; i < count; ++i) { fx = fx + L.cx(g)[i]; fy = fy + L.cy(g)[i]; fm = fm + L.cvx(g)[i]; }
         wave_sync_lds();
         return;
     }
@@ -1116,6 +1128,41 @@ __device__ __forceinline__ void force_epilogue(const DevView* __restrict__ V, co
 // `again`: the block calls once more, so the staging area must be free on
 // return).  mode 0: a frame sub-step (integrates); mode 1: forces out
 // (fout, mout).
+// The epilogue's inputs that neither the contact nor the bond pass reads:
+// rotation and spin, statics, the sub-step accumulator, the hull terms of a
+// near element (first sub-step of a frame only) and the water velocity.
+struct EpiIn {
+    float ro, om, h, area, mass, inertia, sub0, pen, fmr, sx, sy, nxe, nye, vwx, vwy;
+};
+
+__device__ __forceinline__ EpiIn epi_in(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int k, int sub,
+                                        int src_is_state, int mode, int e) {
+    EpiIn o;
+    o.ro = G(src_is_state ? D->s.d_rot : V->nxt_rot)[e];
+    o.om = G(src_is_state ? D->s.d_omega : V->nxt_omega)[e];
+    o.h = G(V->thick)[e]; o.area = G(V->area)[e]; o.mass = G(V->mass)[e]; o.inertia = G(V->inertia)[e];
+    o.sub0 = G(D->s.d_sub)[e];
+    o.pen = 0.0f; o.fmr = 0.0f; o.sx = 0.0f; o.sy = 0.0f; o.nxe = 0.0f; o.nye = 0.0f;
+    if (mode == 0 && sub == 0 && G(V->nearf)[e] != 0) {
+        o.pen = G(V->pen)[e]; o.fmr = G(V->fm_raw)[e]; o.sx = G(V->soi_x)[e]; o.sy = G(V->soi_y)[e];
+        o.nxe = G(V->nx)[e]; o.nye = G(V->ny)[e];
+    }
+    o.vwx = 0.0f; o.vwy = 0.0f;
+    GAS const float* vwp = G(D->in.d_v_water);
+    if (vwp) {
+        const int64_t base = (int64_t)k * D->in.vw_stride_k + (int64_t)e * D->in.vw_stride_e;
+        o.vwx = vwp[base];
+        o.vwy = vwp[base + D->in.vw_stride_c];
+    }
+    return o;
+}
+
+__device__ __forceinline__ void epi_put(float* sv, const EpiIn& o) {
+    sv[S_RO] = o.ro; sv[S_OM] = o.om; sv[S_H] = o.h; sv[S_AREA] = o.area; sv[S_MASS] = o.mass;
+    sv[S_INERTIA] = o.inertia; sv[S_SUB0] = o.sub0; sv[S_PEN] = o.pen; sv[S_FMR] = o.fmr; sv[S_SX] = o.sx;
+    sv[S_SY] = o.sy; sv[S_NXE] = o.nxe; sv[S_NYE] = o.nye; sv[S_VWX] = o.vwx; sv[S_VWY] = o.vwy;
+}
+
 template <int GR>
 __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                            int k, int sub, int src_is_state, int mode, int set,
@@ -1127,14 +1174,10 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
     if (e < n) {                              // group-uniform
         GAS const float* pos = G(src_is_state ? D->s.d_pos : V->nxt_pos);
         GAS const float* vel = G(src_is_state ? D->s.d_vel : V->nxt_vel);
-        GAS const float* rotp = G(src_is_state ? D->s.d_rot : V->nxt_rot);
-        GAS const float* omp = G(src_is_state ? D->s.d_omega : V->nxt_omega);
         GAS const uint8_t* active = G(D->s.d_active);
         // everything that depends only on e, read in one round trip
         const float px = pos[e], py = pos[n + e], vx = vel[e], vy = vel[n + e];
-        const float ro = rotp[e], om = omp[e];
-        const float r = G(V->radius)[e], h = G(V->thick)[e], area = G(V->area)[e];
-        const float mass = G(V->mass)[e], inertia = G(V->inertia)[e];
+        const float r = G(V->radius)[e];
         const int fl = G(V->floe)[e];
         const bool act_raw = active[e] != 0;
         const int deg = V->W > 0 ? G(V->bdeg)[e] : 0;
@@ -1142,18 +1185,9 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
         int4 bt0 = make_int4(0, 0, 0, 0);
         float2 bp0 = make_float2(0.0f, 0.0f);
         if (l < V->W) { bt0 = ldg(G(V->btab) + (size_t)e * V->W + l); bp0 = ldg(G(V->bprm) + (size_t)e * V->W + l); }
-        const float sub0 = G(D->s.d_sub)[e];
-        const bool first = (mode == 0 && sub == 0);
-        const bool is_near = first && G(V->nearf)[e] != 0;
-        float pen = 0.0f, fmr = 0.0f, sx_ = 0.0f, sy_ = 0.0f, nxe = 0.0f, nye = 0.0f;
-        if (is_near) { pen = G(V->pen)[e]; fmr = G(V->fm_raw)[e]; sx_ = G(V->soi_x)[e]; sy_ = G(V->soi_y)[e]; nxe = G(V->nx)[e]; nye = G(V->ny)[e]; }
-        float vwx = 0.0f, vwy = 0.0f;
-        GAS const float* vwp = G(D->in.d_v_water);
-        if (vwp) {
-            const int64_t base = (int64_t)k * D->in.vw_stride_k + (int64_t)e * D->in.vw_stride_e;
-            vwx = vwp[base];
-            vwy = vwp[base + D->in.vw_stride_c];
-        }
+        // 16 lanes: the epilogue-only inputs too, in the same round trip
+        EpiIn ei;
+        if (GR >= 16) ei = epi_in(V, D, k, sub, src_is_state, mode, e);
         const bool fcoll = (mode != 1) && (G(V->floe_coll)[fl] != 0u);
         const bool act_bond = act_raw || fcoll;   // activity seen by the bond pass (mode 0)
         BondIn bin0 = {0.0f, 0.0f, 0.0f, false, false};
@@ -1165,11 +1199,9 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
         bond_force_group<GR>(V, D, pos, e, px, py, mode == 1 ? act_raw : act_bond, deg, bt0, bp0, bin0, mode, L, g,
                              fbx, fby, mb, crack);
         if (l == 0) {
-            float* sv = L.stg[g];
-            sv[S_PX] = px; sv[S_PY] = py; sv[S_VX] = vx; sv[S_VY] = vy; sv[S_RO] = ro; sv[S_OM] = om;
-            sv[S_R] = r; sv[S_H] = h; sv[S_AREA] = area; sv[S_MASS] = mass; sv[S_INERTIA] = inertia;
-            sv[S_SUB0] = sub0; sv[S_PEN] = pen; sv[S_FMR] = fmr; sv[S_SX] = sx_; sv[S_SY] = sy_;
-            sv[S_NXE] = nxe; sv[S_NYE] = nye; sv[S_VWX] = vwx; sv[S_VWY] = vwy;
+            float* sv = L.stg(g);
+            sv[S_PX] = px; sv[S_PY] = py; sv[S_VX] = vx; sv[S_VY] = vy; sv[S_R] = r;
+            if (GR >= 16) epi_put(sv, ei);
             sv[S_FIX] = fix; sv[S_FIY] = fiy; sv[S_MI] = mi; sv[S_FBX] = fbx; sv[S_FBY] = fby; sv[S_MB] = mb;
             sv[S_FLAGS] = __int_as_float((act_raw ? 1 : 0) | (act_bond ? 2 : 0) | (crack ? 4 : 0));
             // 16 lanes (small fields, latency-bound): the group's lane 0
@@ -1181,9 +1213,14 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
         // 8 lanes (large fields, instruction-bound): one thread per element
         // of the block, so the epilogue's instructions serve 32 elements
         __syncthreads();
-        if ((int)threadIdx.x < epb<GR>() && chunk * epb<GR>() + (int)threadIdx.x < n)
-            force_epilogue(V, D, sub, src_is_state, mode, fship_in, fout, mout, L.stg[threadIdx.x],
-                           chunk * epb<GR>() + threadIdx.x);
+        const int et = chunk * epb<GR>() + (int)threadIdx.x;
+        if ((int)threadIdx.x < epb<GR>() && et < n) {
+            // the epilogue-only inputs, loaded here (coalesced, one thread per
+            // element) rather than held in registers through the passes
+            float* sv = L.stg(threadIdx.x);
+            epi_put(sv, epi_in(V, D, k, sub, src_is_state, mode, et));
+            force_epilogue(V, D, sub, src_is_state, mode, fship_in, fout, mout, sv, et);
+        }
         if (again) __syncthreads();
     }
 }
@@ -1354,9 +1391,15 @@ __global__ __launch_bounds__(BLOCK) void kb_near(const DevView* __restrict__ vie
     near_kernel_body(views + m.x, descs + m.x, m.y, m.z);
 }
 
+// 8 lanes: held to 8 waves per SIMD (<= 64 VGPRs and <= 100 SGPRs, the
+// excess SGPRs spilled to VGPR lanes; 19.5 KB of LDS is 8 blocks per CU).
+// Measured on the 256-patch batch: 68.8 us per frame against 72.5 at 7
+// waves.  The single-field k_force<8> measured slower with the same bound
+// and is left to the compiler.
 template <int GR>
-__global__ __launch_bounds__(BLOCK) void kb_force(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
-                                                  const int4* __restrict__ map, int k, int last, int set) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GR < 16 ? 8 : 1)))
+void kb_force(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
+              const int4* __restrict__ map, int k, int last, int set) {
     __shared__ ForceLds<GR> L;
     __shared__ int am_last;
     const int4 m = map[blockIdx.x];
