@@ -1277,12 +1277,13 @@ __device__ __forceinline__ float fm_scaled(const DevView* __restrict__ V, int i)
     return V->p.has_thrust_cap ? f * G(V->scal)[0] : f;
 }
 
-// Thread t of chunk t / 256 (every thread of the 256-thread block calls):
-// bond t, element t, cell t of the set just used, floe t.  With `cull`, the
-// element's committed state also goes through the next frame's near test
-// and into bucket set set ^ 1.
-__device__ __forceinline__ void post_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                          int k, int final_in_nxt, int last, int t, int chunk, int set, int cull) {
+// Bond t's fracture update (the DPDEM gates and collision breaks of
+// jax_simulation.py:534-656 around update_bond_damage_state,
+// jax_bond_fracture.py:134).  Reads only what k_post does not
+// write (final positions and velocities, activity, eligibility, force
+// magnitudes), so it runs beside the element work in any order.
+__device__ __forceinline__ void post_bond(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                          int k, int final_in_nxt, int t) {
     const IceDemParams& P = V->p;
     const int n = V->n, nb = V->nb;
     GAS const float* pos = G(final_in_nxt ? V->nxt_pos : D->s.d_pos);
@@ -1342,6 +1343,17 @@ __device__ __forceinline__ void post_body(const DevView* __restrict__ V, const C
         G(D->s.d_bond_intact)[t] = fin ? 1 : 0;
         if (D->h.d_bond_intact) G(D->h.d_bond_intact)[(size_t)k * nb + t] = fin ? 1 : 0;
     }
+}
+
+// Thread t of chunk t / 256 (every thread of the 256-thread block calls):
+// element t, cell t of the set just used, floe t.  With `cull`, the
+// element's committed state also goes through the next frame's near test
+// and into bucket set set ^ 1.
+__device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                          int k, int final_in_nxt, int last, int t, int chunk, int set, int cull) {
+    const int n = V->n;
+    GAS const float* pos = G(final_in_nxt ? V->nxt_pos : D->s.d_pos);
+    GAS const float* vel = G(final_in_nxt ? V->nxt_vel : D->s.d_vel);
     float px = 0.0f, py = 0.0f, pvx = 0.0f, pvy = 0.0f;
     if (t < n) {
         px = pos[t]; py = pos[n + t];
@@ -1369,14 +1381,32 @@ __device__ __forceinline__ void post_body(const DevView* __restrict__ V, const C
         cull_chunk(V, chunk, t < n, t, px, py, pvx, pvy, G(D->s.d_ship), set ^ 1);
 }
 
+// Both parts for thread t (the one-workgroup scene kernel).
+__device__ __forceinline__ void post_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                          int k, int final_in_nxt, int last, int t, int chunk, int set, int cull) {
+    post_bond(V, D, k, final_in_nxt, t);
+    post_elem(V, D, k, final_in_nxt, last, t, chunk, set, cull);
+}
+
+// Post workgroup b of a context's post grid: blocks [0, nbe) take elements,
+// cells and floes (chunk b), blocks [nbe, nbe + ceil(nb / 256)) take bonds,
+// so neither chain waits behind the other in one thread and a small field
+// spreads over more CUs.
+__device__ __forceinline__ void post_block(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                           int k, int final_in_nxt, int last, int b, int nbe, int set, int cull) {
+    if (b >= nbe) post_bond(V, D, k, final_in_nxt, (b - nbe) * BLOCK + (int)threadIdx.x);
+    else post_elem(V, D, k, final_in_nxt, last, b * BLOCK + (int)threadIdx.x, b, set, cull);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_post(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                                int k, int final_in_nxt, int last, int set, int cull) {
-    post_body(V, D, k, final_in_nxt, last, blockIdx.x * BLOCK + threadIdx.x, blockIdx.x, set, cull);
+                                                int k, int final_in_nxt, int last, int set, int cull, int nbe) {
+    post_block(V, D, k, final_in_nxt, last, blockIdx.x, nbe, set, cull);
 }
 
 // ── batched frames: the same kernels over many contexts in one launch ───────
 // map[blockIdx.x] = (instance, workgroup of that instance, its workgroup
-// count, instance's nextra); the instances' contexts and call descriptors
+// count, instance's nextra for kb_force / element-block count nbe for
+// kb_post); the instances' contexts and call descriptors
 // are views[i], descs[i].  Every instance runs the same frame k.
 
 __global__ __launch_bounds__(BLOCK) void kb_cull(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
@@ -1410,7 +1440,7 @@ void kb_force(const DevView* __restrict__ views, const CallDesc* __restrict__ de
 __global__ __launch_bounds__(BLOCK) void kb_post(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
                                                  const int4* __restrict__ map, int k, int last, int set) {
     const int4 m = map[blockIdx.x];
-    post_body(views + m.x, descs + m.x, k, 1, last, m.y * BLOCK + threadIdx.x, m.y, set, last ? 0 : 1);
+    post_block(views + m.x, descs + m.x, k, 1, last, m.y, m.w, set, last ? 0 : 1);
 }
 
 

@@ -703,8 +703,9 @@ static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, LaunchTimer*
     const int nextra = tail_groups(nblk);
     const int near_blocks = std::max(1, std::min(256, (H.n + 3) / 4));
     const int steps = H.p.n_ice_steps;
-    const int post = std::max(std::max(H.nb, H.n), std::max(std::max(H.G, H.G2), H.nf));
-    const int nbpost = (post + BLOCK - 1) / BLOCK;
+    // post grid: element/cell/floe blocks, then bond blocks
+    const int nbe = (std::max(H.n, std::max(std::max(H.G, H.G2), H.nf)) + BLOCK - 1) / BLOCK;
+    const int nbpost = nbe + (H.nb + BLOCK - 1) / BLOCK;
     const int gcl = (std::max(H.G, H.G2) + BLOCK - 1) / BLOCK;
     int rc = launch(lt, 3, k_cull, dim3(nblk), dim3(BLOCK), stream, V, D);
     for (int k = 0; k < K && rc == ICEDEM_OK; ++k) {
@@ -726,7 +727,7 @@ static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, LaunchTimer*
                          (const float*)nullptr, (float*)nullptr, (float*)nullptr);
         }
         rc |= launch(lt, 2, k_post, dim3(nbpost), dim3(BLOCK), stream, V, D, k, (steps % 2 == 1) ? 1 : 0, last,
-                     set, last ? 0 : 1);
+                     set, last ? 0 : 1, nbe);
     }
     if (rc) { set_err(ctx, "event creation failed"); return ICEDEM_E_HIP; }
     HIP_TRY(hipGetLastError());
@@ -868,12 +869,12 @@ int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** ou
         const int nextra = tail_groups(H.nblk);
         // one wavefront per 16 elements (a quarter of a patch is near the ship at most)
         const int nnear = std::max(1, std::min(64, (H.n + 63) / 64));
-        const int post = std::max(std::max(H.nb, H.n), std::max(std::max(H.G, H.G2), H.nf));
-        const int npost = (post + BLOCK - 1) / BLOCK;
+        const int nbe = (std::max(H.n, std::max(std::max(H.G, H.G2), H.nf)) + BLOCK - 1) / BLOCK;
+        const int npost = nbe + (H.nb + BLOCK - 1) / BLOCK;
         for (int l = 0; l < H.nblk; ++l) mc.push_back(make_int4(i, l, H.nblk, 0));
         for (int l = 0; l < nnear; ++l) mn.push_back(make_int4(i, l, nnear, 0));
         for (int l = 0; l < nextra + fblk; ++l) mf.push_back(make_int4(i, l, nextra + fblk, nextra));
-        for (int l = 0; l < npost; ++l) mp.push_back(make_int4(i, l, npost, 0));
+        for (int l = 0; l < npost; ++l) mp.push_back(make_int4(i, l, npost, nbe));
     }
     b->n_cull = (int)mc.size(); b->n_near = (int)mn.size(); b->n_force = (int)mf.size(); b->n_post = (int)mp.size();
     hipError_t e = hipSetDevice(b->device);
