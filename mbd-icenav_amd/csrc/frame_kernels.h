@@ -494,6 +494,7 @@ __device__ __forceinline__ void near_elem(const DevView* __restrict__ V, const C
     GAS const float* pos = G(D->s.d_pos);
     const float px = pos[e], py = pos[n + e];
     const float rad = G(V->radius)[e], thk = G(V->thick)[e];
+    const int fl = G(V->floe)[e];   // read with the element, used if it touches the hull
     float wx = 0.0f, wy = 0.0f;
     bool mv = false;
     float2 lv = make_float2(0.0f, 0.0f);
@@ -515,9 +516,8 @@ __device__ __forceinline__ void near_elem(const DevView* __restrict__ V, const C
         G(V->fm_raw)[e] = o.fm;
         G(V->soi_x)[e] = o.sx; G(V->soi_y)[e] = o.sy; G(V->mterm)[e] = o.m;
         if (pen > 0.0f) {
-            const int f = G(V->floe)[e];
-            atomicOr(&V->floe_coll[f], 1u);
-            atomicMax(&V->floe_maxfm[f], __float_as_uint(o.fm));
+            atomicOr(&V->floe_coll[fl], 1u);
+            atomicMax(&V->floe_maxfm[fl], __float_as_uint(o.fm));
         }
     }
 }

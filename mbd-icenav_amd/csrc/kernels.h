@@ -164,6 +164,11 @@ __device__ __forceinline__ void hull_element(const ShipPose& P, const IceDemPara
     const int K = prm.n_edge_samples;
     const int per = 1 + K;
     const int npts = maxv * per;
+    // GP lanes per hull point in pass 2; its first round's hull point is
+    // loaded here so the load overlaps pass 1
+    const int GP = S < 64 ? 64 / S : 1;
+    const int per_round = 64 / GP;
+    const float2 hl0 = ldg(hull + ((lane / GP) < per_round && lane / GP < S ? lane / GP : 0));
     // ---- pass 1: vertices + edge samples vs analytic hull (:139-284)
     float bv = -1.0f, bnx = 0.0f, bny = 0.0f, bcx = 0.0f, bcy = 0.0f;
     int bi_ = 0x7fffffff;
@@ -214,13 +219,12 @@ __device__ __forceinline__ void hull_element(const ShipPose& P, const IceDemPara
     // GP lanes per hull point share its polygon edges (kk = sub, sub + GP,
     // ...); crossings add up and the nearest edge is the first minimum, so
     // combining the lanes gives the sequential scan's result exactly
-    const int GP = S < 64 ? 64 / S : 1;
-    const int per_round = 64 / GP;
     for (int base = 0; base < S; base += per_round) {
         const int s = base + lane / GP, sub = lane % GP;
         const bool live = (lane / GP) < per_round && s < S;
         const int sc = live ? s : 0;
-        float2 hl = ldg(hull + sc);
+        float2 hl = hl0;
+        if (base > 0) hl = ldg(hull + sc);
         float hx = (hl.x * P.c_h + hl.y * (-P.s_h)) + P.sx;
         float hy = (hl.x * P.s_h + hl.y * P.c_h) + P.sy;
         int cross = 0;
