@@ -1389,18 +1389,26 @@ __device__ __forceinline__ void post_body(const DevView* __restrict__ V, const C
 }
 
 // Post workgroup b of a context's post grid: blocks [0, nbe) take elements,
-// cells and floes (chunk b), blocks [nbe, nbe + ceil(nb / 256)) take bonds,
-// so neither chain waits behind the other in one thread and a small field
-// spreads over more CUs.
+// cells and floes, blocks [nbe, nbe + ceil(nb / 256)) take bonds, so neither
+// chain waits behind the other in one thread and a small field spreads over
+// more CUs.  xcd: b is the hardware workgroup id, and each part's chunks are
+// spread XCD by XCD (chunk_of), so bonds and elements with neighbouring
+// indices share an L2.
 __device__ __forceinline__ void post_block(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                           int k, int final_in_nxt, int last, int b, int nbe, int set, int cull) {
-    if (b >= nbe) post_bond(V, D, k, final_in_nxt, (b - nbe) * BLOCK + (int)threadIdx.x);
-    else post_elem(V, D, k, final_in_nxt, last, b * BLOCK + (int)threadIdx.x, b, set, cull);
+                                           int k, int final_in_nxt, int last, int b, int nbe, int set, int cull,
+                                           int nbb, bool xcd) {
+    if (b >= nbe) {
+        const int c = xcd ? chunk_of(b - nbe, nbb) : b - nbe;
+        post_bond(V, D, k, final_in_nxt, c * BLOCK + (int)threadIdx.x);
+    } else {
+        const int c = xcd ? chunk_of(b, nbe) : b;
+        post_elem(V, D, k, final_in_nxt, last, c * BLOCK + (int)threadIdx.x, c, set, cull);
+    }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_post(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                                 int k, int final_in_nxt, int last, int set, int cull, int nbe) {
-    post_block(V, D, k, final_in_nxt, last, blockIdx.x, nbe, set, cull);
+    post_block(V, D, k, final_in_nxt, last, blockIdx.x, nbe, set, cull, (int)gridDim.x - nbe, true);
 }
 
 // ── batched frames: the same kernels over many contexts in one launch ───────
@@ -1440,7 +1448,7 @@ void kb_force(const DevView* __restrict__ views, const CallDesc* __restrict__ de
 __global__ __launch_bounds__(BLOCK) void kb_post(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
                                                  const int4* __restrict__ map, int k, int last, int set) {
     const int4 m = map[blockIdx.x];
-    post_block(views + m.x, descs + m.x, k, 1, last, m.y, m.w, set, last ? 0 : 1);
+    post_block(views + m.x, descs + m.x, k, 1, last, m.y, m.w, set, last ? 0 : 1, m.z - m.w, false);
 }
 
 
