@@ -1,19 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark: DPDEM frame step on one MI355X.
 
-Workload (default): the plain-DEM single-GPU configuration -- a seeded
-300 x 750 m field at 30 % concentration (up to 400 lognormal convex floes,
-median circumradius 8 m, sigma 0.5, 6-12 vertices) tessellated to ~20 cells
-per floe (~8k elements, ~25k bonds), h = 1 m, hull 60 x 12 m (12-point
-polygon) moving +y at 2 m/s, contact stiffness 1e8 N/m, friction 0.3,
-dt 5e-3 s.  One step = one frame (hull contact, pre-pass, bonds, ice-ice
-contact, drag, integration, fracture, ship) plus the scan histories.
+Workload (default, the largest single-GPU field): a seeded 1.5 x 3 km field
+at 50 % concentration, 5000 lognormal convex floes (6-12 vertices, sigma 0.5)
+tessellated to ~20 cells each (~100k elements, ~280k bonds), thickness 0.5 or
+2.0 m (50/50), hull 60 x 12 m (12-point polygon) moving +y at 3 m/s, contact
+stiffness 1e8 N/m, friction 0.3, dt 5e-3 s.  One step = one frame (hull
+contact, pre-pass, bonds, ice-ice contact, drag, integration, fracture, ship)
+plus the scan histories.  ``--config c1`` is the 300 x 750 m, 30 %, ~8k
+element field, also reported under ``other_frame_configs``.
 
 Prints one JSON line.  ``value`` = element-steps/s with the state resident in
 HBM.  ``roofline`` is for the dominant kernel, its duration measured with HIP
 events on the launch stream; ``cpu_baseline`` times the float32 NumPy oracle
-(the reference's all-pairs algorithm) on a bounded sample of frames on this
-host, single thread.
+on a bounded sample of frames on this host, single thread (all pairs, the
+reference's algorithm, up to 20k elements; the oracle's cell list above).
 """
 from __future__ import annotations
 
@@ -125,6 +126,112 @@ def cpu_baseline(case_args, frames_budget_s: float = 20.0, max_frames: int = 8, 
                          f"{n * (n - 1) // 2} pairs/frame would not fit), " if cells else
                          f"all-pairs contact ({n * (n - 1) // 2} pairs/frame), ")
                       + f"{dt:.1f} s, host os.cpu_count()={os.cpu_count()}"}
+
+
+def frame_bench(case_args, config: str, steps: int, warmup: int, chunk: int, profile_frames: int,
+                force_lanes: int = 0, cell_factor: float = 1.0, use_graph: bool = True):
+    """Time `steps` frames of one field after `warmup` frames.
+
+    Frames run as hipGraph replays of `chunk` frames (plus one shorter graph
+    for the remainder), histories written.  Every graph size the timed loop
+    replays is captured and run once on a copy of the state before the clock
+    starts, and every history buffer is allocated then, so the timed region
+    holds only replays.  Then per-kernel durations from HIP events on the
+    launch stream give the dominant kernel's roofline figure."""
+    import numpy as np
+    import torch
+    from icedem import _lib
+    from icedem.engine import DemEngine, _ptr
+    from icedem.ice_state import build_static_data, build_ship_dynamics_data, build_initial_ice_state
+    from icedem.properties import ShipProperties
+    from icedem.simulation import _time_stepping, _thruster_config
+    sc, cfg, L, B, mass, hull, x0, y0, speed = case_args
+    n, nb = len(sc.polygons), len(sc.bonds)
+    static = build_static_data(sc.polygons, sc.centroids, sc.bonds, sc.thicknesses, sc.parent_ids, cfg)
+    sp_ = ShipProperties(L, B, 1.0, displacement=mass)
+    sd = build_ship_dynamics_data(sp_, cfg, L, B, ship_hull_local=hull)
+    dt_ice, nsteps = _time_stepping(static, cfg, cfg.dt)
+    Bm, T_n = _thruster_config(sp_, L)
+    eng = DemEngine(static, cfg, sd, L, B, nsteps, cfg.dt, dt_ice, True, Bm, T_n, centroids=sc.centroids,
+                    force_lanes=force_lanes, cell_factor=cell_factor)
+    dev = eng.device
+    st = build_initial_ice_state(n, sc.centroids, nb, sc.parent_ids, dev)
+    ship = torch.zeros(15, dtype=torch.float32, device=dev)
+    ship[0], ship[1], ship[2] = x0, y0, math.pi / 2
+    total = warmup + steps
+    t_all = np.arange(total + 1) * cfg.dt
+    traj = np.stack([np.full_like(t_all, x0), y0 + speed * t_all, np.full_like(t_all, math.pi / 2)], axis=1)
+    setp = torch.as_tensor(np.concatenate([traj[:-1], traj[1:]], axis=1).astype(np.float32), device=dev)
+    chunk = max(1, min(chunk, steps))
+    hists = {chunk: eng.alloc_history(chunk)}
+    if steps % chunk:
+        hists[steps % chunk] = eng.alloc_history(steps % chunk)
+    if warmup % chunk and warmup % chunk not in hists:
+        hists[warmup % chunk] = eng.alloc_history(warmup % chunk)
+
+    def run(state, ship_buf, frame0, frames):
+        k = 0
+        while k < frames:
+            m = min(chunk, frames - k)
+            eng.run_frames(state, ship_buf, setp[frame0 + k: frame0 + k + m], hist=hists[m], use_graph=use_graph,
+                           sync=False)
+            k += m
+
+    run(st, ship, 0, warmup)
+    # capture (and run once) every graph size of the timed loop on a copy
+    primed = sorted({chunk} | ({steps % chunk} if steps % chunk else set()))
+    pst, pship = eng.clone_state(st), ship.clone()
+    for m in primed:
+        eng.run_frames(pst, pship, setp[warmup: warmup + m], hist=hists[m], use_graph=use_graph, sync=False)
+    torch.cuda.synchronize()
+    del pst, pship
+    strm = torch.cuda.current_stream().cuda_stream
+    _lib.check(eng.lib.icedem_sync(eng.ctx, strm), eng.ctx, "sync")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(st, ship, warmup, steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    _lib.check(eng.lib.icedem_sync(eng.ctx, strm), eng.ctx, "sync")
+    out = {"workload": f"plain DPDEM step, {config}: {n} elements, {nb} bonds, {sc.n_floes} floes, "
+                       f"concentration {sc.concentration:.3f}, dt {cfg.dt}",
+           "value": n * steps / elapsed, "ms_per_step": 1000.0 * elapsed / steps, "steps": steps,
+           "elapsed_s": elapsed, "chunk": chunk, "graphs_primed": primed, "roofline": None}
+    P = profile_frames
+    if P <= 0:
+        eng.close()
+        return out
+    ms = (C.c_float * len(KERNELS))()
+    s = eng.state_struct(st, ship)
+    inp = _lib.IceDemInputs()
+    sp_slice = setp[total - P: total].contiguous() if total >= P else setp[:P].contiguous()
+    inp.d_setpoints = _ptr(sp_slice)
+    _lib.check(eng.lib.icedem_frames_profiled(eng.ctx, C.byref(s), C.byref(inp), P, None, ms, strm),
+               eng.ctx, "profiled")
+    torch.cuda.synchronize()
+    avg_us = {k: ms[i] * 1000.0 / P for i, k in enumerate(KERNELS[:3])}
+    avg_us["k_cull (once per call)"] = ms[3] * 1000.0
+    dom = max(KERNELS[:3], key=avg_us.get)
+    G = eng.grid[4] * eng.grid[5]
+    n_near, near_blk = eng.near_stats(st, ship)
+    nbpad = 1
+    while nbpad < (n + 255) // 256:
+        nbpad <<= 1
+    abytes = algorithmic_bytes(dom, n, nb, sc.n_floes, G, n_near, eng.max_v, hull.shape[0], near_blk, nbpad)
+    achieved = abytes / (avg_us[dom] * 1e-6) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_{config}.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get(dom)
+        except Exception:
+            traffic = None
+    eng.close()
+    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                       "kernel_avg_us": avg_us[dom], "algorithmic_bytes": abytes, "kernel_avg_us_all": avg_us,
+                       "near_elements": n_near}
+    return out
 
 
 def _cpu_rollout(args):
@@ -435,14 +542,15 @@ def surrogate_forward_bench(B: int, reps: int = 5, train_steps: int = 20):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--warmup", type=int, default=500)
-    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2"])
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--config", default="c2", choices=["c0", "c1", "c2"])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=500)
     ap.add_argument("--profile-frames", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-extra-config", action="store_true", help="skip the c1 frame line next to the headline")
     ap.add_argument("--no-rollouts", action="store_true", help="skip the batched-rollout measurement")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate planner measurement")
     ap.add_argument("--rollouts", type=int, default=256)
@@ -483,88 +591,24 @@ def main():
 
     import numpy as np
     import torch
-    from icedem import _lib
-    from icedem.engine import DemEngine, _ptr
-    from icedem.ice_state import build_static_data, build_ship_dynamics_data, build_initial_ice_state
-    from icedem.properties import ShipProperties
-    from icedem.simulation import _time_stepping, _thruster_config
 
     case_args = build_case(args.config, args.seed)
-    sc, cfg, L, B, mass, hull, x0, y0, speed = case_args
+    fb = frame_bench(case_args, args.config, args.steps, args.warmup, args.chunk, args.profile_frames,
+                     force_lanes=args.force_lanes, cell_factor=args.cell_factor, use_graph=not args.no_graph)
+    sc, cfg = case_args[0], case_args[1]
     n, nb = len(sc.polygons), len(sc.bonds)
-    static = build_static_data(sc.polygons, sc.centroids, sc.bonds, sc.thicknesses, sc.parent_ids, cfg)
-    sp_ = ShipProperties(L, B, 1.0, displacement=mass)
-    sd = build_ship_dynamics_data(sp_, cfg, L, B, ship_hull_local=hull)
-    dt_ice, nsteps = _time_stepping(static, cfg, cfg.dt)
-    Bm, T_n = _thruster_config(sp_, L)
-    eng = DemEngine(static, cfg, sd, L, B, nsteps, cfg.dt, dt_ice, True, Bm, T_n, centroids=sc.centroids,
-                    force_lanes=args.force_lanes, cell_factor=args.cell_factor)
-    dev = eng.device
-    st = build_initial_ice_state(n, sc.centroids, nb, sc.parent_ids, dev)
-    ship = torch.zeros(15, dtype=torch.float32, device=dev)
-    ship[0], ship[1], ship[2] = x0, y0, math.pi / 2
-    total = args.warmup + args.steps
-    t_all = np.arange(total + 1) * cfg.dt
-    traj = np.stack([np.full_like(t_all, x0), y0 + speed * t_all, np.full_like(t_all, math.pi / 2)], axis=1)
-    setp = torch.as_tensor(np.concatenate([traj[:-1], traj[1:]], axis=1).astype(np.float32), device=dev)
-    chunk = max(1, min(args.chunk, args.steps))
-    hist = eng.alloc_history(chunk)
-    use_graph = not args.no_graph
-
-    def run(frame0, frames):
-        k = 0
-        while k < frames:
-            m = min(chunk, frames - k)
-            h = hist if m == chunk else eng.alloc_history(m)
-            eng.run_frames(st, ship, setp[frame0 + k: frame0 + k + m], hist=h, use_graph=use_graph, sync=False)
-            k += m
-
-    run(0, args.warmup)
-    torch.cuda.synchronize()
-    _lib.check(eng.lib.icedem_sync(eng.ctx, torch.cuda.current_stream().cuda_stream), eng.ctx, "sync")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    run(args.warmup, args.steps)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    _lib.check(eng.lib.icedem_sync(eng.ctx, torch.cuda.current_stream().cuda_stream), eng.ctx, "sync")
-
-    # per-kernel durations with HIP events on the launch stream (0 frames: skipped)
-    P = args.profile_frames
-    if P <= 0:
+    elapsed = fb["elapsed_s"]
+    if fb["roofline"] is None:
         print(json.dumps({"metric": METRIC, "value": n * args.steps / elapsed, "unit": "element-steps/s",
                           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                           "ms_per_step": 1000.0 * elapsed / args.steps, "roofline": None,
                           "cpu_baseline": None, "note": "per-kernel timing skipped"}), flush=True)
         return
-    ms = (C.c_float * len(KERNELS))()
-    s = eng.state_struct(st, ship)
-    inp = _lib.IceDemInputs()
-    sp_slice = setp[total - P: total].contiguous() if total >= P else setp[:P].contiguous()
-    inp.d_setpoints = _ptr(sp_slice)
-    _lib.check(eng.lib.icedem_frames_profiled(eng.ctx, C.byref(s), C.byref(inp), P, None, ms,
-                                              torch.cuda.current_stream().cuda_stream), eng.ctx, "profiled")
-    torch.cuda.synchronize()
-    avg_us = {k: ms[i] * 1000.0 / P for i, k in enumerate(KERNELS[:3])}
-    avg_us["k_cull (once per call)"] = ms[3] * 1000.0
-    dom = max(KERNELS[:3], key=avg_us.get)
-    G = eng.grid[4] * eng.grid[5]
-    n_near, near_blk = eng.near_stats(st, ship)
-    nbpad = 1
-    while nbpad < (n + 255) // 256:
-        nbpad <<= 1
-    abytes = algorithmic_bytes(dom, n, nb, sc.n_floes, G, n_near, eng.max_v, hull.shape[0], near_blk, nbpad)
-    achieved = abytes / (avg_us[dom] * 1e-6) / 1e9
-    traffic = None
-    tfile = ROOT / "profiles" / f"traffic_{args.config}.json"
-    if tfile.exists():
-        try:
-            traffic = json.loads(tfile.read_text()).get(dom)
-        except Exception:
-            traffic = None
-
+    extra_frames = None
+    if args.config != "c1" and not args.no_extra_config:
+        c1 = frame_bench(build_case("c1", args.seed), "c1", 2000, 100, 500, 200)
+        extra_frames = {"c1": {k: c1[k] for k in ("workload", "value", "ms_per_step", "steps", "roofline")}}
     cpu = None if args.no_cpu_baseline else cpu_baseline(case_args)
-    eng.close()
     rollouts = None if args.no_rollouts else rollout_bench(R=args.rollouts)
     surrogate = None
     if not args.no_surrogate and rollouts is not None:
@@ -583,15 +627,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded scene generator)",
-        "config": {"workload": f"plain DPDEM step, {args.config}: {n} elements, {nb} bonds, "
-                               f"{sc.n_floes} floes, conc {sc.concentration:.2f}, dt {cfg.dt}",
-                   "elements": n, "bonds": nb, "floes": sc.n_floes, "frames_per_graph": chunk,
-                   "graph": use_graph, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
-                     "kernel_avg_us": avg_us[dom], "algorithmic_bytes": abytes,
-                     "kernel_avg_us_all": avg_us},
+        "config": {"workload": fb["workload"], "elements": n, "bonds": nb, "floes": sc.n_floes,
+                   "concentration": sc.concentration, "frames_per_graph": fb["chunk"],
+                   "graph": not args.no_graph, "graphs_captured_before_timing": fb["graphs_primed"],
+                   "parallelism": "single GPU"},
+        "roofline": fb["roofline"],
         "cpu_baseline": cpu,
+        "other_frame_configs": extra_frames,
         "rollouts": rollouts,
         "surrogate": surrogate,
     }

@@ -753,7 +753,7 @@ int icedem_frames(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* 
     if (diag) cd.dg = *diag;
     rc = put_desc(ctx, cd, stream);
     if (rc) return rc;
-    if (!use_graph || ctx->hv.p.n_ice_steps > 1) return launch_frames(ctx, K, stream);
+    if (!use_graph) return launch_frames(ctx, K, stream);
     auto it = ctx->graphs.find(K);
     if (it == ctx->graphs.end()) {
         hipStream_t cap;

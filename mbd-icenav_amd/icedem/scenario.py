@@ -217,6 +217,73 @@ def place_floes(rng, x_lo, x_hi, y_lo, y_hi, concentration, max_floes, radius_fn
     return [placed[i] for i in order], area_sum / total
 
 
+def place_floes_sized(rng, x_lo, x_hi, y_lo, y_hi, concentration, n_floes, median_radius, sigma, nv,
+                      jitter=0.3, gap=0.05, tries=300, shrink=0.93):
+    """Exactly ``n_floes`` non-overlapping convex floes covering about
+    ``concentration`` of the field.
+
+    Shapes are drawn first (lognormal circumradius, 6-12 jittered vertices),
+    then all are scaled by one factor so that their areas sum to the target;
+    they are placed largest first by random sequential placement (large floes
+    find room while the field is empty, small ones fill the gaps), and a floe
+    that finds no free spot in ``tries`` draws is shrunk by ``shrink`` and
+    tried again.  Returns the floes and the achieved concentration.
+    """
+    total = (x_hi - x_lo) * (y_hi - y_lo)
+    radii = np.array([min(max(rng.lognormal(math.log(median_radius), sigma), 2.0), 6.0 * median_radius)
+                      for _ in range(n_floes)])
+    shapes = [convex_polygon(rng, (0.0, 0.0), R, int(rng.integers(nv[0], nv[1] + 1)), jitter) for R in radii]
+    areas = np.array([polygon_area(s) for s in shapes])
+    scale = math.sqrt(concentration * total / float(areas.sum()))
+    shapes = [s * scale for s in shapes]
+    radii = radii * scale
+    order = np.argsort(-radii, kind="stable")
+    cell = 2.0 * median_radius * scale
+    grid = {}
+    placed, centers, prad = [], [], []
+
+    def cells_of(c, R):
+        return [(ix, iy) for ix in range(int(math.floor((c[0] - R) / cell)), int(math.floor((c[0] + R) / cell)) + 1)
+                for iy in range(int(math.floor((c[1] - R) / cell)), int(math.floor((c[1] + R) / cell)) + 1)]
+
+    for k in order:
+        shape, R = shapes[k], float(radii[k])
+        while True:
+            ok_spot = None
+            for _ in range(tries):
+                c = np.array([rng.uniform(x_lo + R, x_hi - R), rng.uniform(y_lo + R, y_hi - R)])
+                poly = shape + c
+                seen = set()
+                ok = True
+                for key in cells_of(c, R + gap):
+                    for m in grid.get(key, ()):
+                        if m in seen:
+                            continue
+                        seen.add(m)
+                        if np.hypot(*(centers[m] - c)) > prad[m] + R + gap:
+                            continue
+                        if _sat_overlap(placed[m], poly, gap):
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if ok:
+                    ok_spot = (c, poly)
+                    break
+            if ok_spot is not None:
+                break
+            shape = shape * shrink
+            R *= shrink
+        c, poly = ok_spot
+        m = len(placed)
+        placed.append(poly); centers.append(c); prad.append(R)
+        for key in cells_of(c, R):
+            grid.setdefault(key, []).append(m)
+    area_sum = float(sum(polygon_area(p) for p in placed))
+    order = sorted(range(len(placed)), key=lambda i: (float(centers[i][1]), float(centers[i][0])))
+    return [placed[i] for i in order], area_sum / total
+
+
 def tessellate_all(rng, floes, cells_per_floe, floe_thickness):
     polys, cents, parents, thick = [], [], [], []
     for f, floe in enumerate(floes):
@@ -265,9 +332,15 @@ def scene_c1(seed: int = 0) -> Scene:
 
 
 def scene_c2(seed: int = 0) -> Scene:
-    """1.5 x 3 km at 50 % (or 5000 floes), bimodal thickness 0.5 / 2.0 m."""
-    return scene_field(seed, 1500.0, 3000.0, 0.50, 5000, median_radius=11.0,
-                       thickness_values=(0.5, 2.0), thickness_probs=(0.5, 0.5))
+    """1.5 x 3 km at 50 % with exactly 5000 floes (lognormal circumradius,
+    sigma 0.5, scaled to the concentration), ~20 cells each, bimodal
+    thickness 0.5 / 2.0 m."""
+    rng = np.random.default_rng(seed)
+    x_lo, x_hi, y_lo, y_hi = -750.0, 750.0, 0.0, 3000.0
+    floes, conc = place_floes_sized(rng, x_lo, x_hi, y_lo, y_hi, 0.50, 5000, 11.0, 0.5, (6, 12))
+    th = rng.choice(np.array([0.5, 2.0]), size=len(floes), p=(0.5, 0.5))
+    polys, cents, parents, thick, bonds = tessellate_all(rng, floes, 20, th)
+    return Scene(polys, cents, bonds, thick, parents, len(floes), conc, (x_lo, x_hi, y_lo, y_hi))
 
 
 def scene_patch(seed: int, n_target: int, concentration: float) -> Scene:

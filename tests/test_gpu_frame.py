@@ -133,3 +133,46 @@ def test_scan_bit_identical_to_oracle_trajectory():
     rep.add_float("hist_pos_last", hist["positions"][-1].cpu().numpy().T, ost.positions)
     print("after", K0 + K, "frames:", rep.summary(), flush=True)
     rep.assert_ok()
+
+
+def test_ice_substeps_graph_equals_launches():
+    """With 3 ice sub-steps per frame the K-frame graph (cell rebuilds
+    between sub-steps captured with the frame kernels) gives the same state
+    and histories, bit for bit, as per-kernel launches and the oracle."""
+    from icedem.config import benchmark_config
+    cfg = benchmark_config(use_rayleigh_dt_cap=True, rayleigh_safety_factor=0.5)
+    case = Case.make("c0", n_frames=900, cfg=cfg)
+    eng = case.engine()
+    assert eng.params.n_ice_steps == 3
+    st, ship = case.gpu_initial()
+    K0 = 760
+    sp = torch.as_tensor(np.stack([case.setpoint_row(k) for k in range(K0)]), device=eng.device)
+    eng.run_frames(st, ship, sp, use_graph=False)
+    K = 60
+    sp = torch.as_tensor(np.stack([case.setpoint_row(k) for k in range(K0, K0 + K)]), device=eng.device)
+    out = []
+    for use_graph in (True, False, True):
+        s2, sh2 = eng.clone_state(st), ship.clone()
+        hist = eng.alloc_history(K)
+        eng.run_frames(s2, sh2, sp, hist=hist, use_graph=use_graph)
+        out.append((case.to_oracle(s2, sh2), {k: v.cpu().numpy() for k, v in hist.items()}))
+    (g, gs), gh = out[0]
+    for (a, as_), ah in out[1:]:
+        np.testing.assert_array_equal(a.positions, g.positions)
+        np.testing.assert_array_equal(a.velocities, g.velocities)
+        np.testing.assert_array_equal(a.bond_intact, g.bond_intact)
+        np.testing.assert_array_equal(as_.eta, gs.eta)
+        for k in gh:
+            np.testing.assert_array_equal(ah[k], gh[k], err_msg=k)
+    o = case.oracle()
+    ost, oship = case.to_oracle(st, ship)
+    vw = np.zeros((case.n, 2), np.float32)
+    for k in range(K0, K0 + K):
+        r = case.setpoint_row(k)
+        ost, oship, _ = o.frame_step(ost, oship, r[:3], r[3:], vw, ost.dpdem_floe_eligible)
+    rep = Report()
+    rep.add_float("positions", g.positions, ost.positions)
+    rep.add_float("eta", gs.eta, oship.eta)
+    rep.add_exact("bond_intact", g.bond_intact, ost.bond_intact)
+    print("sub-step graph vs oracle after", K0 + K, "frames:", rep.summary(), flush=True)
+    rep.assert_ok()
