@@ -34,6 +34,7 @@ extern "C" {
 #define ICEDEM_E_DEVICE (-4)    /* error flagged by a kernel */
 
 #define ICEDEM_MAX_VERTS 64     /* max_V (padded vertices per element) */
+#define ICEDEM_CC_NOALLOC_MAX (1 << 20)  /* n limit of icedem_connected_components_noalloc */
 #define ICEDEM_MAX_EDGE_SAMPLES 16
 #define ICEDEM_LANES8_MIN_ELEMENTS 32768   /* force_lanes = 0: 8 lanes per element from here */
 #define ICEDEM_SHIP_STATE_LEN 15 /* eta[3], nu[3], n_actual[6], e_int[3] */
@@ -301,6 +302,18 @@ int icedem_bond_damage(int32_t n_bonds, const float* d_delta_n, const float* d_d
 int icedem_connected_components(int32_t n, int32_t n_bonds, const int32_t* d_bond_i,
                                 const int32_t* d_bond_j, const uint8_t* d_intact, int32_t n_iter,
                                 int32_t* d_labels, void* stream);
+
+/* No counterpart in the reference.  icedem_connected_components for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): same arguments, same labels, no device
+ * allocation, no host-device copy and no synchronisation.  Its label scratch
+ * is a file-scope device array of ICEDEM_CC_NOALLOC_MAX int32 (4 MiB) that
+ * every GPU the library runs on holds until the process ends; n above that
+ * returns ICEDEM_E_LIMIT.  All calls on one GPU share that array: calls that
+ * overlap on one GPU (other streams, other threads) race. */
+int icedem_connected_components_noalloc(int32_t n, int32_t n_bonds, const int32_t* d_bond_i,
+                                        const int32_t* d_bond_j, const uint8_t* d_intact, int32_t n_iter,
+                                        int32_t* d_labels, void* stream);
 
 /* Replaces component_com_velocities_jax (jax_bonds.py:96-141); d_vel and
  * d_out are (2, n). */

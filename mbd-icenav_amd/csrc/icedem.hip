@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 #include <algorithm>
@@ -76,6 +77,13 @@ __global__ void k_cc_sweep(int n, int nb, const int32_t* bi, const int32_t* bj, 
     if (!intact[t]) m = n;
     atomicMin(&new_l[bi[t]], m);
     atomicMin(&new_l[bj[t]], m);
+}
+
+// labels[i] = i: the starting labels of find_connected_components_jax
+// (jax_bonds.py:68)
+__global__ void k_cc_init(int n, int32_t* labels) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) labels[t] = t;
 }
 
 __global__ void k_iota_count(int n, const int32_t* labels, int32_t* count) {
@@ -1165,6 +1173,75 @@ int icedem_connected_components(int32_t n, int32_t n_bonds, const int32_t* d_bon
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(stream));
     HIP_TRY(hipFreeAsync(tmp, stream));
+    return ICEDEM_OK;
+}
+
+}  // extern "C"
+
+// ── connected components without allocation ──────────────────────────────────
+// The label scratch of icedem_connected_components_noalloc: ICEDEM_CC_NOALLOC_MAX
+// int32 (4 MiB), one copy on every GPU the library runs on, held until the
+// process ends.  Host code never passes `scratch` itself to a kernel (in host
+// code the name is a stand-in, not a device address): each device's address
+// comes from hipGetSymbolAddress, once per device, with that device current.
+static __device__ __align__(256) int32_t scratch[ICEDEM_CC_NOALLOC_MAX];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static int32_t* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// this device's address of `scratch` (looked up on the device's first call)
+static int cc_scratch(int32_t** out) {
+    IceDemCtx* ctx = nullptr;
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) { set_err(nullptr, "device index out of range"); return ICEDEM_E_LIMIT; }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_err(nullptr, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return ICEDEM_E_HIP;
+        }
+        g_scratch_ptr[dev] = (int32_t*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return ICEDEM_OK;
+}
+
+extern "C" {
+
+int icedem_connected_components_noalloc(int32_t n, int32_t n_bonds, const int32_t* d_bond_i,
+                                        const int32_t* d_bond_j, const uint8_t* d_intact, int32_t n_iter,
+                                        int32_t* d_labels, void* stream_) {
+    IceDemCtx* ctx = nullptr;
+    if (n < 0 || n_bonds < 0 || n_iter < 0) { set_err(nullptr, "bad argument"); return ICEDEM_E_INVALID; }
+    if (n > ICEDEM_CC_NOALLOC_MAX) {
+        set_err(nullptr, "icedem_connected_components_noalloc: n = " + std::to_string(n) + " exceeds the scratch "
+                         "limit of " + std::to_string(ICEDEM_CC_NOALLOC_MAX) + " elements");
+        return ICEDEM_E_LIMIT;
+    }
+    if (n == 0) return ICEDEM_OK;
+    if (!d_labels || (n_bonds > 0 && (!d_bond_i || !d_bond_j || !d_intact))) {
+        set_err(nullptr, "null argument"); return ICEDEM_E_INVALID;
+    }
+    int32_t* tmp = nullptr;
+    int rc = cc_scratch(&tmp);
+    if (rc) return rc;
+    hipStream_t stream = (hipStream_t)stream_;
+    // every sweep overwrites the scratch's first n entries before reading
+    // them, so nothing is cleared; what a previous call left there is unused
+    const int gb = (n + 255) / 256, gbb = (std::max(n_bonds, 1) + 255) / 256;
+    hipLaunchKernelGGL(k_cc_init, dim3(gb), dim3(256), 0, stream, n, d_labels);
+    for (int it = 0; it < n_iter; ++it) {
+        hipLaunchKernelGGL(k_cc_copy, dim3(gb), dim3(256), 0, stream, n, (const int32_t*)d_labels, tmp);
+        if (n_bonds > 0)
+            hipLaunchKernelGGL(k_cc_sweep, dim3(gbb), dim3(256), 0, stream, n, n_bonds, d_bond_i, d_bond_j,
+                               d_intact, (const int32_t*)d_labels, tmp);
+        hipLaunchKernelGGL(k_cc_copy, dim3(gb), dim3(256), 0, stream, n, (const int32_t*)tmp, d_labels);
+    }
+    HIP_TRY(hipGetLastError());
     return ICEDEM_OK;
 }
 

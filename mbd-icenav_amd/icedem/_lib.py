@@ -136,6 +136,7 @@ SIGNATURES = {
     "icedem_bond_damage": (C.c_int, [_i, _p, _p, _p, _p, _p, _f, _f, _f, _f, _f, _p, _p, _p, _p, _p, _p, _p, _p,
                                      _p]),
     "icedem_connected_components": (C.c_int, [_i, _i, _p, _p, _p, _i, _p, _p]),
+    "icedem_connected_components_noalloc": (C.c_int, [_i, _i, _p, _p, _p, _i, _p, _p]),
     "icedem_component_com_velocities": (C.c_int, [_i, _p, _p, _p, _p, _p]),
     "icedem_ship_step": (C.c_int, [_p, _p, _p, _p, _p, _p]),
 }
