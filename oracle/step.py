@@ -17,9 +17,12 @@ the reference's XLA lowering cannot be observed (JAX is absent):
 
 Pinning: ship path and connected components against the reference's own golden
 tests; fracture algebra against the known answers of
-``dpdem/validation/tests.py:100-146``; contact/bond/drag/integration/hull
-against closed forms and invariants (``tests/test_oracle_*.py``).  XLA's exact
-last-ulp results remain unpinned (no JAX here).
+``dpdem/validation/tests.py:100-146``; contact/drag/integration/hull against
+closed forms and invariants (``tests/test_oracle_*.py``); bond forces,
+rotational drag, the contact pre-pass, activation, eligibility and the
+fracture gates against hand-worked cases that also reject planted mutations of
+each method (``tests/test_oracle_pins.py``).  XLA's exact last-ulp results
+remain unpinned (no JAX here).
 """
 from __future__ import annotations
 
@@ -713,61 +716,13 @@ class Oracle:
         v_water = np.asarray(v_water, ft); v_water_ship = np.asarray(v_water_ship, ft)
         sx, sy, sh = ship.eta[0], ship.eta[1], ship.eta[2]
         pen, nx, ny, cx, cy = self.collisions_combined(st.positions, st.rotations, sx, sy, sh)
-        # contact pre-pass (:431-479)
-        hc = pen > ft(0.0)
-        w_c = vmin(ft(cfg.w_contact_radius_factor) * self.radii, ft(cfg.w_contact_cap))
-        sigma_f = ft(cfg.flexural_strength_base) * np.sqrt(self.thick)
-        F_break = sigma_f * w_c * (self.thick * self.thick)
-        pen_crit = ft(cfg.pen_crit_ref) * (self.thick / ft(cfg.pen_crit_thickness_ref))
-        fm = F_break * vmin(pen / vmax(pen_crit, ft(1e-10)), ft(1.0))
-        fm = np.where(hc, fm, ft(0.0))
-        soi_x = fm * nx
-        soi_y = fm * ny
-        rel_y = st.positions[:, 1] - sy
-        side_push = ft(cfg.side_push_factor) * fm * vsign(rel_y)
-        soi_y = soi_y + side_push
-        soi_sx, soi_sy = soi_x, soi_y
-        if cfg.max_ship_ice_force is not None:
-            tx = tree_sum(soi_x, ft); ty = tree_sum(soi_y, ft)
-            F_tot = np.sqrt(tx * tx + ty * ty)
-            cap = ft(cfg.max_ship_ice_force)
-            scale = (cap / vmax(F_tot, ft(1e-6))) if F_tot > cap else ft(1.0)
-            scale = ft(scale)
-            soi_sx = soi_x * scale; soi_sy = soi_y * scale
-            fm = fm * scale
-        F_ship_x = -tree_sum(soi_sx, ft)
-        F_ship_y = -tree_sum(soi_sy, ft)
-        r_x = st.positions[:, 0] - sx
-        r_y = st.positions[:, 1] - sy
-        M_ship = tree_sum(np.where(hc, r_x * (-soi_y) - r_y * (-soi_x), ft(0.0)), ft)
-        tau_ice = np.array([F_ship_x, F_ship_y, M_ship], dtype=ft)
-        # submergence + buoyancy (:481-498)
-        pen_c = vmin(pen, ft(cfg.submergence_pen_cap))
-        sub = st.submergence
-        new_sub = np.where(hc,
-                           vmin(sub + ft(cfg.submergence_rate) * pen_c * ft(self.dt), self.thick),
-                           vmax(ft(0.0), sub - ft(1.0 / cfg.rise_time_const) * sub * ft(self.dt)))
-        z_sub = vmin(new_sub, self.thick)
-        F_buoy = ft(WATER_DENSITY * 9.81) * self.areas * z_sub
-        soi_x = soi_x + np.where(hc, F_buoy * nx, ft(0.0))
-        soi_y = soi_y + np.where(hc, F_buoy * ny, ft(0.0))
-        ship_on_ice = np.stack([soi_x, soi_y], axis=1).astype(ft)
-        st = st._replace(submergence=new_sub.astype(ft))
-        # activation (:500-509)
-        n_floes = dpdem_eligible.shape[0]
-        collided = np.zeros(n_floes, np.int32)
-        np.maximum.at(collided, self.floe, hc.astype(np.int32))
-        collided = collided.astype(bool)
-        new_active = st.elem_active | collided[self.floe]
+        hc, fm, ship_on_ice, tau_ice, new_sub = self.contact_prepass(st.positions, st.submergence, pen, nx, ny,
+                                                                     sx, sy)
+        F_ship_x, F_ship_y = tau_ice[0], tau_ice[1]
+        st = st._replace(submergence=new_sub)
+        new_active, collided = self.activation(st.elem_active, hc, dpdem_eligible.shape[0])
         st = st._replace(elem_active=new_active)
-        # eligibility (:511-524)
-        if self.bond_i.shape[0] > 0:
-            thr = self.bstr.min() * ft(cfg.effective_strength_collision)
-            mx = np.zeros(n_floes, ft)
-            np.maximum.at(mx, self.floe, fm)
-            new_elig = dpdem_eligible | (collided & (mx > thr))
-        else:
-            new_elig = dpdem_eligible.copy()
+        new_elig = self.eligibility(dpdem_eligible, collided, fm)
         # sub-steps (:527-532, :363-395)
         for _ in range(self.n_ice_steps):
             f_bond, m_bond = self.bond_forces(st.positions, st.elem_active, st.bond_intact, st.bond_damage)
@@ -800,6 +755,75 @@ class Oracle:
             "nx": nx, "ny": ny, "ship_on_ice": ship_on_ice, "tau_ice": tau_ice,
         }
         return st, ship_new, diag
+
+    # ── a6: contact pre-pass, jax_simulation.py:431-498 ──────────────────────
+    def contact_prepass(self, pos, sub, pen, nx, ny, sx, sy):
+        """Force magnitudes, ship forces on the ice, the ship's reaction and
+        the submergence update from the hull results.  Returns (has_collision,
+        force_mag (thrust-scaled), ship_on_ice (n,2: unscaled partials plus
+        buoyancy), tau_ice [Fx, Fy, M] (scaled forces, unscaled moment), new
+        submergence)."""
+        ft = self.ft
+        cfg = self.cfg
+        hc = pen > ft(0.0)
+        w_c = vmin(ft(cfg.w_contact_radius_factor) * self.radii, ft(cfg.w_contact_cap))
+        sigma_f = ft(cfg.flexural_strength_base) * np.sqrt(self.thick)
+        F_break = sigma_f * w_c * (self.thick * self.thick)
+        pen_crit = ft(cfg.pen_crit_ref) * (self.thick / ft(cfg.pen_crit_thickness_ref))
+        fm = F_break * vmin(pen / vmax(pen_crit, ft(1e-10)), ft(1.0))
+        fm = np.where(hc, fm, ft(0.0))
+        soi_x = fm * nx
+        soi_y = fm * ny
+        rel_y = pos[:, 1] - sy
+        side_push = ft(cfg.side_push_factor) * fm * vsign(rel_y)
+        soi_y = soi_y + side_push
+        soi_sx, soi_sy = soi_x, soi_y
+        if cfg.max_ship_ice_force is not None:
+            tx = tree_sum(soi_x, ft); ty = tree_sum(soi_y, ft)
+            F_tot = np.sqrt(tx * tx + ty * ty)
+            cap = ft(cfg.max_ship_ice_force)
+            scale = (cap / vmax(F_tot, ft(1e-6))) if F_tot > cap else ft(1.0)
+            scale = ft(scale)
+            soi_sx = soi_x * scale; soi_sy = soi_y * scale
+            fm = fm * scale
+        F_ship_x = -tree_sum(soi_sx, ft)
+        F_ship_y = -tree_sum(soi_sy, ft)
+        r_x = pos[:, 0] - sx
+        r_y = pos[:, 1] - sy
+        M_ship = tree_sum(np.where(hc, r_x * (-soi_y) - r_y * (-soi_x), ft(0.0)), ft)
+        tau_ice = np.array([F_ship_x, F_ship_y, M_ship], dtype=ft)
+        # submergence + buoyancy (:481-498)
+        pen_c = vmin(pen, ft(cfg.submergence_pen_cap))
+        new_sub = np.where(hc,
+                           vmin(sub + ft(cfg.submergence_rate) * pen_c * ft(self.dt), self.thick),
+                           vmax(ft(0.0), sub - ft(1.0 / cfg.rise_time_const) * sub * ft(self.dt)))
+        z_sub = vmin(new_sub, self.thick)
+        F_buoy = ft(WATER_DENSITY * 9.81) * self.areas * z_sub
+        soi_x = soi_x + np.where(hc, F_buoy * nx, ft(0.0))
+        soi_y = soi_y + np.where(hc, F_buoy * ny, ft(0.0))
+        ship_on_ice = np.stack([soi_x, soi_y], axis=1).astype(ft)
+        return hc, fm.astype(ft), ship_on_ice, tau_ice, new_sub.astype(ft)
+
+    # ── a6: floe activation, jax_simulation.py:500-509 ───────────────────────
+    def activation(self, active, hc, n_floes):
+        """Every element of a floe with a hull contact becomes active.
+        Returns (new elem_active, collided per floe)."""
+        collided = np.zeros(n_floes, np.int32)
+        np.maximum.at(collided, self.floe, hc.astype(np.int32))
+        collided = collided.astype(bool)
+        return active | collided[self.floe], collided
+
+    # ── a7: DPDEM eligibility, jax_simulation.py:511-524 ─────────────────────
+    def eligibility(self, elig, collided, fm):
+        """A floe becomes eligible when collided and its largest (scaled)
+        force magnitude exceeds min(bond strength) x effective strength."""
+        ft = self.ft
+        if self.bond_i.shape[0] == 0:
+            return elig.copy()
+        thr = self.bstr.min() * ft(self.cfg.effective_strength_collision)
+        mx = np.zeros(elig.shape[0], ft)
+        np.maximum.at(mx, self.floe, fm)
+        return elig | (collided & (mx > thr))
 
     def _fracture(self, st: IceState, fm, new_active, new_elig):
         ft = self.ft
