@@ -36,24 +36,26 @@ HBM_PEAK_GBS = 8000.0
 KERNELS = ["k_near", "k_force", "k_post", "k_cull"]   # per frame, per frame, per frame, per call
 
 
-def build_case(config: str, seed: int):
-    import numpy as np
+def build_case(config: str, seed: int, cache_dir: str = ""):
+    """The scene of `config` (generated from `seed`, or read from
+    cache_dir/<config>_<seed>.npz when that was written by an earlier run)."""
     from icedem import scenario
     from icedem.config import benchmark_config
     from icedem.properties import ShipProperties
     from icedem.ice_state import ship_hull_polygon
     cfg = benchmark_config()
-    if config == "c0":
-        sc = scenario.scene_c0(seed)
-        speed = 2.0
-    elif config == "c1":
-        sc = scenario.scene_c1(seed)
-        speed = 2.0
-    elif config == "c2":
-        sc = scenario.scene_c2(seed)
-        speed = 3.0
-    else:
+    makers = {"c0": scenario.scene_c0, "c1": scenario.scene_c1, "c2": scenario.scene_c2}
+    if config not in makers:
         raise SystemExit(f"unknown config {config}")
+    speed = 3.0 if config == "c2" else 2.0
+    cache = Path(cache_dir) / f"{config}_{seed}.npz" if cache_dir else None
+    if cache is not None and cache.exists():
+        sc = scenario.load_scene(cache)
+    else:
+        sc = makers[config](seed)
+        if cache is not None:
+            cache.parent.mkdir(parents=True, exist_ok=True)
+            scenario.save_scene(cache, sc)
     L, B = 60.0, 12.0
     mass = ShipProperties(L, B, 5.0).mass
     x0, y0 = scenario.ship_start_below(sc, L, gap=1.0)
@@ -546,6 +548,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--config", default="c2", choices=["c0", "c1", "c2"])
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--scene-cache", default="", metavar="DIR",
+                    help="read the scene from DIR/<config>_<seed>.npz if present, else generate and write it there")
     ap.add_argument("--chunk", type=int, default=500)
     ap.add_argument("--profile-frames", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -592,7 +596,7 @@ def main():
     import numpy as np
     import torch
 
-    case_args = build_case(args.config, args.seed)
+    case_args = build_case(args.config, args.seed, args.scene_cache)
     fb = frame_bench(case_args, args.config, args.steps, args.warmup, args.chunk, args.profile_frames,
                      force_lanes=args.force_lanes, cell_factor=args.cell_factor, use_graph=not args.no_graph)
     sc, cfg = case_args[0], case_args[1]
@@ -606,7 +610,7 @@ def main():
         return
     extra_frames = None
     if args.config != "c1" and not args.no_extra_config:
-        c1 = frame_bench(build_case("c1", args.seed), "c1", 2000, 100, 500, 200)
+        c1 = frame_bench(build_case("c1", args.seed, args.scene_cache), "c1", 2000, 100, 500, 200)
         extra_frames = {"c1": {k: c1[k] for k in ("workload", "value", "ms_per_step", "steps", "roofline")}}
     cpu = None if args.no_cpu_baseline else cpu_baseline(case_args)
     rollouts = None if args.no_rollouts else rollout_bench(R=args.rollouts)

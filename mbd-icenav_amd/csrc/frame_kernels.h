@@ -33,6 +33,7 @@ template <int GR> constexpr int epb() { return BLOCK / GR; }   // elements per k
 
 __device__ __forceinline__ int32_t* near_count(const DevView* V) { return V->counters; }
 __device__ __forceinline__ int32_t* force_ticket(const DevView* V) { return V->counters + 3; }
+__device__ __forceinline__ int32_t* coll_count(const DevView* V) { return V->counters + 1; }
 
 // extra (tail) workgroups of a frame's first k_force: one wave per 256-element block
 __host__ __device__ constexpr int tail_groups(int nblk) { return (nblk + BLOCK / 64 - 1) / (BLOCK / 64); }
@@ -290,7 +291,11 @@ __device__ __forceinline__ float block256_tree(const DevView* __restrict__ V, GA
     for (int i = 0; i < 4; ++i) {
         const int e = blk * BLOCK + lane * 4 + i;
         float x = 0.0f;
-        if (e < V->n && G(V->nearf)[e]) x = scaled ? a[e] * scale : a[e];
+        if (e < V->n) {   // flag and value in one round trip
+            const bool nf = G(V->nearf)[e] != 0;
+            const float ae = a[e];
+            x = nf ? (scaled ? ae * scale : ae) : 0.0f;
+        }
         v[i] = x;
     }
     float x = (v[0] + v[1]) + (v[2] + v[3]);
@@ -356,6 +361,52 @@ __device__ __forceinline__ void block256_tree3(const DevView* __restrict__ V, GA
 __device__ __forceinline__ float wave_tree_small(float x, int npad) {
     for (int off = 1; off < npad; off <<= 1) x = x + __shfl_xor(x, off, 64);
     return __shfl(x, 0, 64);
+}
+
+// Pairwise trees of three arrays a0, a1, a2 (npad floats each, npad a power
+// of two) by one wave, written by other workgroups of the running kernel
+// (device-coherent loads); the order of wave_tree_global.  Up to
+// 64 * TREE3_CHUNK entries every load is issued before the first add; larger
+// arrays take wave_tree_global per array.  Results in every lane.
+constexpr int TREE3_CHUNK = 16;
+__device__ __forceinline__ float wave_tree_global(GAS const float* __restrict__ a, int npad, const float* dev);
+__device__ __forceinline__ void wave_tree_global3(const float* a0, const float* a1, const float* a2, int npad,
+                                                  float& s0, float& s1, float& s2) {
+    const int lane = threadIdx.x & 63;
+    if (npad <= 64) {
+        const float l0 = lane < npad ? ld_dev(a0 + lane) : 0.0f;
+        const float l1 = lane < npad ? ld_dev(a1 + lane) : 0.0f;
+        const float l2 = lane < npad ? ld_dev(a2 + lane) : 0.0f;
+        s0 = wave_tree_small(l0, npad); s1 = wave_tree_small(l1, npad); s2 = wave_tree_small(l2, npad);
+        return;
+    }
+    const int c = npad >> 6;
+    if (c > TREE3_CHUNK) {
+        s0 = wave_tree_global((GAS const float*)a0, npad, a0);
+        s1 = wave_tree_global((GAS const float*)a1, npad, a1);
+        s2 = wave_tree_global((GAS const float*)a2, npad, a2);
+        return;
+    }
+    float v0[TREE3_CHUNK], v1[TREE3_CHUNK], v2[TREE3_CHUNK];
+    const size_t b = (size_t)lane * c;
+    #pragma unroll
+    for (int i = 0; i < TREE3_CHUNK; ++i) {
+        const bool ok = i < c;
+        v0[i] = ok ? ld_dev(a0 + b + i) : 0.0f;
+        v1[i] = ok ? ld_dev(a1 + b + i) : 0.0f;
+        v2[i] = ok ? ld_dev(a2 + b + i) : 0.0f;
+    }
+    // the lane's chunk pairwise (c a power of two), then the butterfly
+    #pragma unroll
+    for (int st = 1; st < TREE3_CHUNK; st <<= 1) {
+        if (st < c) {
+            #pragma unroll
+            for (int i = 0; i + st < TREE3_CHUNK; i += 2 * st) {
+                v0[i] = v0[i] + v0[i + st]; v1[i] = v1[i] + v1[i + st]; v2[i] = v2[i] + v2[i + st];
+            }
+        }
+    }
+    s0 = wave_tree_sum(v0[0]); s1 = wave_tree_sum(v1[0]); s2 = wave_tree_sum(v2[0]);
 }
 
 // dev: the same array as a generic pointer, read with device-coherent loads
@@ -516,7 +567,11 @@ __device__ __forceinline__ void near_elem(const DevView* __restrict__ V, const C
         G(V->fm_raw)[e] = o.fm;
         G(V->soi_x)[e] = o.sx; G(V->soi_y)[e] = o.sy; G(V->mterm)[e] = o.m;
         if (pen > 0.0f) {
-            atomicOr(&V->floe_coll[fl], 1u);
+            // the first contact of the floe this frame lists it for the
+            // eligibility update (floes without a contact keep theirs)
+            if (atomicOr(&V->floe_coll[fl], 1u) == 0u)
+                __hip_atomic_store(V->coll_list + atomicAdd(coll_count(V), 1), fl, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
             atomicMax(&V->floe_maxfm[fl], __float_as_uint(o.fm));
         }
     }
@@ -553,7 +608,7 @@ __device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const 
     const IceDemParams& P = V->p;
     float Fx = -fsx, Fy = -fsy;
     float tau[3] = {Fx, Fy, tm};
-    float* ship = D->s.d_ship;
+    GAS float* ship = G(D->s.d_ship);
     float spc[3] = {0.0f, 0.0f, 0.0f}, spn[3] = {0.0f, 0.0f, 0.0f};
     float cu = 0.0f, cr = 0.0f;
     if (P.ship_mode == 1) {
@@ -563,7 +618,7 @@ __device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const 
             cr = G(D->in.d_controls)[2 * knot + 1];
         }
     } else {
-        const float* sp = D->in.d_setpoints + (size_t)k * 6;
+        GAS const float* sp = G(D->in.d_setpoints) + (size_t)k * 6;
         spc[0] = sp[0]; spc[1] = sp[1]; spc[2] = sp[2];
         spn[0] = sp[3]; spn[1] = sp[4]; spn[2] = sp[5];
         if (P.ice_brake_threshold > 0.0f) {
@@ -580,7 +635,7 @@ __device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const 
     for (int i = 0; i < ICEDEM_SHIP_STATE_LEN; ++i) st[i] = ship[i];
     const float Fm = sqrtf(Fx * Fx + Fy * Fy);
     const float J = Fm * P.dt;
-    float* ro = D->dg.d_rollout;
+    GAS float* ro = G(D->dg.d_rollout);
     if (ro) {
         if (k == 0) {
             ro[0] = 0.0f; ro[2] = 0.0f;
@@ -604,18 +659,20 @@ __device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const 
     }
 }
 
-// floe eligibility (jax_simulation.py:511-524); all threads of a workgroup
+// floe eligibility (jax_simulation.py:511-524); all threads of a workgroup.
+// Only floes with a hull contact this frame (the collided list k_near built)
+// can change: elig |= max force magnitude (thrust-scaled) > threshold.
 __device__ __forceinline__ void floe_eligibility(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                                  float scale) {
     const IceDemParams& P = V->p;
     if (V->nb <= 0) return;
-    uint8_t* elig = D->s.d_floe_elig;
-    for (int f = threadIdx.x; f < V->nf; f += blockDim.x) {
-        bool coll = G(V->floe_coll)[f] != 0u;
-        float mx = __uint_as_float(G(V->floe_maxfm)[f]);
+    GAS uint8_t* elig = G(D->s.d_floe_elig);
+    const int cnt = __hip_atomic_load(coll_count(V), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+        const int f = __hip_atomic_load(V->coll_list + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        float mx = __uint_as_float(__hip_atomic_load(V->floe_maxfm + f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         if (P.has_thrust_cap) mx = mx * scale;
-        bool el = (elig[f] != 0) || (coll && (mx > P.elig_threshold));
-        elig[f] = el ? 1 : 0;
+        if (mx > P.elig_threshold) elig[f] = 1;
     }
 }
 
@@ -645,20 +702,15 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
     if (!*am_last) return;
     // totals, in every wave
     float tx, ty, tm;
-    if (nbpad <= 64) {   // the three loads in one round trip
-        const float l0 = lane < nbpad ? ld_dev(V->part1 + lane) : 0.0f;
-        const float l1 = lane < nbpad ? ld_dev(V->part1 + nbpad + lane) : 0.0f;
-        const float l2 = lane < nbpad ? ld_dev(V->part1 + 2 * nbpad + lane) : 0.0f;
-        tx = wave_tree_small(l0, nbpad); ty = wave_tree_small(l1, nbpad); tm = wave_tree_small(l2, nbpad);
-    } else {
-        tx = wave_tree_global(G(V->part1), nbpad, V->part1);
-        ty = wave_tree_global(G(V->part1) + nbpad, nbpad, V->part1 + nbpad);
-        tm = wave_tree_global(G(V->part1) + 2 * nbpad, nbpad, V->part1 + 2 * nbpad);
-    }
+    wave_tree_global3(V->part1, V->part1 + nbpad, V->part1 + 2 * nbpad, nbpad, tx, ty, tm);
     float scale = 1.0f, fx = tx, fy = ty;
     if (P.has_thrust_cap) {
         const float F_tot = sqrtf(tx * tx + ty * ty);
         scale = (F_tot > P.max_ship_ice_force) ? (P.max_ship_ice_force / vmax(F_tot, 1e-6f)) : 1.0f;
+    }
+    // with the cap binding, the totals of the scaled forces (x * 1 = x: at
+    // scale 1 they are the totals above, bit for bit)
+    if (scale != 1.0f) {
         // scaled partials: 64 blocks at a time, the blocks holding near
         // elements dealt to the waves in turn, zeros for the others
         for (int base = 0; base < nb; base += 64) {
@@ -676,15 +728,17 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
         }
         stores_done();
         __syncthreads();
-        fx = wave_tree_global(G(V->part2), nbpad, V->part2);
-        fy = wave_tree_global(G(V->part2) + nbpad, nbpad, V->part2 + nbpad);
+        float unused;
+        wave_tree_global3(V->part2, V->part2 + nbpad, V->part2, nbpad, fx, fy, unused);
     }
     floe_eligibility(V, D, scale);
+    __syncthreads();
     if (threadIdx.x == 0) {
         G(V->scal)[0] = scale;
         ship_frame(V, D, k, last, fx, fy, tm);
         store_pose(V, D->s.d_ship);
         *near_count(V) = 0;
+        __hip_atomic_store(coll_count(V), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1499,6 +1553,7 @@ __device__ __forceinline__ void mid_body(const DevView* __restrict__ V, const Ca
     if (threadIdx.x == 0) {
         ship_frame(V, D, k, last, fx, fy, tm);
         *near_count(V) = 0;
+        __hip_atomic_store(coll_count(V), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
 }

@@ -872,6 +872,26 @@ __global__ void k_mppi_final(const float* __restrict__ part, int nb, int D, cons
     out[d] = fminf(fmaxf(a / st[1], -1.0f), 1.0f);
 }
 
+// ── MBD reverse-diffusion update (run_mbd, jax_mbd_planner.py:1363-1396) ────
+// The weighted mean Ybar_new = sum_i w_i Y0s_i / sum_i w_i has run_mppi's
+// weights (softmax of -(c - min c) / T); then the score step of the
+// diffusion, per dimension d:
+//   Yi = Ybar sqrt(ab_i); score = (-Yi + sqrt(ab_i) Ybar_new) / (1 - ab_i)
+//   Yim1 = (Yi + (1 - ab_i) score) / sqrt(a_i); out = clip(Yim1 / sqrt(ab_prev), -1, 1)
+__global__ void k_mbd_final(const float* __restrict__ part, int nb, int D, const float* __restrict__ st,
+                            const float* ybar, float ab, float a, float ab_prev, float* out) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= D) return;
+    float acc = 0.0f;
+    for (int b = 0; b < nb; ++b) acc = acc + part[(size_t)b * D + d];
+    const float ynew = acc / st[1];
+    const float sab = sqrtf(ab);
+    const float yi = ybar[d] * sab;
+    const float score = (1.0f / (1.0f - ab)) * ((-yi) + sab * ynew);
+    const float yim1 = (1.0f / sqrtf(a)) * (yi + (1.0f - ab) * score);
+    out[d] = fminf(fmaxf(yim1 / sqrtf(ab_prev), -1.0f), 1.0f);
+}
+
 }  // namespace icesur
 
 using namespace icesur;
@@ -1097,6 +1117,25 @@ int icesur_ke_costmap(const float* d_pos, const float* d_radius, const float* d_
                            d_work);
     hipLaunchKernelGGL(k_ke_dilate, dim3(grid1(cells)), dim3(256), 0, s, d_work, rows, cols, base_cost, dilate_half,
                        d_out);
+    SUR_LAUNCHED();
+    return ICEDEM_OK;
+}
+
+int icesur_mbd_update(const float* d_costs, const float* d_samples, int32_t S, int32_t D, float temperature,
+                      const float* d_ybar, float alpha_bar, float alpha, float alpha_bar_prev, float* d_work,
+                      float* d_out, void* stream) {
+    SUR_CHECK(S > 0 && D > 0 && temperature > 0.0f && d_costs && d_samples && d_ybar && d_work && d_out,
+              "icesur_mbd_update: bad arguments");
+    SUR_CHECK(alpha_bar > 0.0f && alpha_bar < 1.0f && alpha > 0.0f && alpha_bar_prev > 0.0f,
+              "icesur_mbd_update: schedule values out of range");
+    hipStream_t s = (hipStream_t)stream;
+    const int nb = (S + 255) / 256;
+    float* st = d_work;
+    float* part = d_work + 2;
+    hipLaunchKernelGGL(k_mppi_stats, dim3(1), dim3(256), 0, s, d_costs, S, temperature, st);
+    hipLaunchKernelGGL(k_mppi_part, dim3(nb), dim3(256), 0, s, d_costs, d_samples, S, D, temperature, st, part);
+    hipLaunchKernelGGL(k_mbd_final, dim3((D + 255) / 256), dim3(256), 0, s, part, nb, D, st, d_ybar, alpha_bar,
+                       alpha, alpha_bar_prev, d_out);
     SUR_LAUNCHED();
     return ICEDEM_OK;
 }

@@ -386,3 +386,25 @@ def ship_start_below(scene: Scene, ship_length: float, gap: float = 1.0, target_
     x0 = float(allv[:, 0].mean())
     y0 = float(allv[:, 1].min()) - gap - ship_length / 2.0
     return x0, y0
+
+
+def save_scene(path, scene: Scene) -> None:
+    """Write a scene to an .npz file (vertices concatenated with per-polygon
+    offsets; no pickled objects)."""
+    nv = np.array([len(p) for p in scene.polygons], np.int64)
+    verts = np.concatenate([np.asarray(p, np.float64) for p in scene.polygons]) if len(nv) else np.zeros((0, 2))
+    np.savez(path, verts=verts, nv=nv, centroids=np.asarray(scene.centroids, np.float64),
+             bonds=np.asarray(scene.bonds, np.int64).reshape(-1, 2), thicknesses=np.asarray(scene.thicknesses),
+             parent_ids=np.asarray(scene.parent_ids, np.int32), n_floes=np.int64(scene.n_floes),
+             concentration=np.float64(scene.concentration), field=np.asarray(scene.field, np.float64))
+
+
+def load_scene(path) -> Scene:
+    """Read a scene written by save_scene."""
+    z = np.load(path, allow_pickle=False)
+    offs = np.concatenate([[0], np.cumsum(z["nv"])])
+    verts = z["verts"]
+    polys = [verts[offs[i]:offs[i + 1]].copy() for i in range(len(z["nv"]))]
+    bonds = [(int(i), int(j)) for i, j in z["bonds"]]
+    return Scene(polys, z["centroids"], bonds, z["thicknesses"], z["parent_ids"], int(z["n_floes"]),
+                 float(z["concentration"]), tuple(float(v) for v in z["field"]))
