@@ -128,6 +128,12 @@ typedef struct {
      * A batch uses the value of its first context, with 0 deciding by the
      * batch's total element count. */
     int32_t force_lanes;
+    /* partner lists (build choice, results identical): with list_skin > 0
+     * and one ice sub-step per frame, each element keeps the partners within
+     * r_i + r_j + 2 delta + list_skin of it, rebuilt from the grids when an
+     * element may have moved list_skin / 2 since the last build; frames in
+     * between evaluate only the listed partners.  0 = the grids every frame. */
+    float list_skin;
 } IceDemParams;
 
 /* Mutable state (IceState + ShipState, jax_ice_state.py:31-52), device
@@ -214,6 +220,13 @@ int icedem_frames(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* 
  * durations of k_near, k_force (all sub-steps with their cell rebuilds) and
  * k_post over the K frames, and the call's one k_cull.  Instrumentation for
  * benchmarks; no reference counterpart. */
+/* No counterpart in the reference (build diagnostics): h_out[0] = frames
+ * that rebuilt the contact partner lists since the context was created,
+ * h_out[1] = 1 if an element had more partners than a list holds during the
+ * last call (it then scans the grids every frame), h_out[2] = 1 if the
+ * context uses partner lists.  Synchronises the device. */
+int icedem_list_stats(IceDemCtx* ctx, int32_t* h_out);
+
 int icedem_frames_profiled(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* in, int32_t K,
                            const IceDemHistory* hist, float* h_kernel_ms, void* stream);
 

@@ -35,6 +35,33 @@ __device__ __forceinline__ int32_t* near_count(const DevView* V) { return V->cou
 __device__ __forceinline__ int32_t* force_ticket(const DevView* V) { return V->counters + 3; }
 __device__ __forceinline__ int32_t* coll_count(const DevView* V) { return V->counters + 1; }
 
+// ── partner lists: control words ─────────────────────────────────────────────
+// Slot k % 3 holds the largest squared displacement, since the last list
+// build, of the positions frame k starts from (NLSUB sub-slots, one per XCD,
+// float bits under atomicMax).  Frame k's force pass reads its slot, its
+// epilogues fill slot k + 1, and frame k's k_post reads slots k and k + 1
+// and clears slot k + 2 for frame k + 1's epilogues.
+__device__ __forceinline__ uint32_t* nl_slot(const DevView* __restrict__ V, int k) {
+    return V->nlctl + (k % 3) * NLSUB;
+}
+__device__ __forceinline__ uint32_t* nl_ovf(const DevView* __restrict__ V) { return V->nlctl + 3 * NLSUB; }
+__device__ __forceinline__ uint32_t* nl_rebuilds(const DevView* __restrict__ V) { return V->nlctl + 3 * NLSUB + 1; }
+__device__ __forceinline__ uint32_t ld_dev_u(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// frame k rebuilds the lists: some element may have moved far enough that a
+// partner outside its list reaches contact (every slot starts a call at +inf)
+__device__ __forceinline__ bool nl_rebuild(const DevView* __restrict__ V, int k) {
+    const uint32_t* s = nl_slot(V, k);
+    uint32_t m = 0u;
+    #pragma unroll
+    for (int i = 0; i < NLSUB; ++i) m = max(m, ld_dev_u(s + i));
+    return __uint_as_float(m) > V->nl_thr2;
+}
+// some element had more than NLW candidates at a build of this call: it
+// scans the grids every frame, so the buckets are filled every frame
+__device__ __forceinline__ bool nl_overflowed(const DevView* __restrict__ V) { return ld_dev_u(nl_ovf(V)) != 0u; }
+
 // extra (tail) workgroups of a frame's first k_force: one wave per 256-element block
 __host__ __device__ constexpr int tail_groups(int nblk) { return (nblk + BLOCK / 64 - 1) / (BLOCK / 64); }
 // element workgroups of k_force for n elements at `lanes` lanes each
@@ -451,7 +478,8 @@ __device__ __forceinline__ float wave_tree_global(GAS const float* __restrict__ 
 // ship's heading comes from a fast sine/cosine: the 1 m margin dwarfs its
 // error, and only which elements run the exact hull test depends on it.
 __device__ __forceinline__ void cull_chunk(const DevView* __restrict__ V, int chunk, bool valid, int e,
-                                           float x, float y, float vx, float vy, GAS const float* ship, int set) {
+                                           float x, float y, float vx, float vy, GAS const float* ship, int set,
+                                           bool insert = true) {
     const IceDemParams& P = V->p;
     bool near = false;
     float r = 0.0f;
@@ -473,7 +501,7 @@ __device__ __forceinline__ void cull_chunk(const DevView* __restrict__ V, int ch
     int qbase = 0;
     if (lane == first) qbase = atomicAdd(near_count(V), __popcll(nb));
     int rix = 0, idx = 0;
-    if (valid) {
+    if (valid && insert) {
         rix = insert_rec(V, set, x, y, r);
         idx = atomicAdd((int*)rec_ptr(V, rix), 1);
     }
@@ -482,7 +510,14 @@ __device__ __forceinline__ void cull_chunk(const DevView* __restrict__ V, int ch
         G(V->nearf)[e] = near ? 1 : 0;
         if (near) G(V->near_list)[qbase + __builtin_amdgcn_mbcnt_hi((uint32_t)(nb >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)nb, 0u))] = e;
-        cell_store(V, set, rix, idx, e, x, y, vx, vy, r);
+        if (insert) cell_store(V, set, rix, idx, e, x, y, vx, vy, r);
+        if (V->nl_on) {   // the element's slot for the partner-list pass
+            GAS float* q = G(V->es) + 6 * (size_t)e;
+            f4u a;
+            a.x = x; a.y = y; a.z = vx; a.w = vy;
+            *(GAS f4u*)q = a;
+            q[4] = r;
+        }
     }
     const int cnt = __syncthreads_count(near ? 1 : 0);
     if (threadIdx.x == 0) G(V->nearcnt)[chunk] = cnt;
@@ -506,6 +541,10 @@ __device__ __forceinline__ void cull_kernel_body(const DevView* __restrict__ V, 
     if (valid) { x = pos[e]; y = pos[n + e]; vx = vel[e]; vy = vel[n + e]; }
     cull_chunk(V, lb, valid, e, x, y, vx, vy, G(D->s.d_ship), 0);
     if (lb == 0 && threadIdx.x == 0) store_pose(V, D->s.d_ship);
+    // a call starts with a list build (the caller may have changed the state)
+    if (V->nl_on && lb == 0 && threadIdx.x < 3 * NLSUB + 1)
+        __hip_atomic_store(V->nlctl + threadIdx.x, threadIdx.x < NLSUB ? 0x7f800000u : 0u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_cull(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
@@ -999,6 +1038,145 @@ __device__ __forceinline__ void contact_force_group(const DevView* __restrict__ 
     }
 }
 
+// ── partner lists ────────────────────────────────────────────────────────────
+
+// In-order running sum across a lane group: lane l adds its term c after the
+// terms of lanes 0 .. l - 1, starting from the group-uniform acc (a DPP chain
+// of single-lane row shifts).  Returns the new group-uniform total.  The
+// order is the sequential one of the reference's scatter-add; a +0 term
+// leaves a sum unchanged (sums that start at +0 never reach -0).
+template <int GR>
+__device__ __forceinline__ float group_chain(float acc, float c) {
+    const int l = threadIdx.x & (GR - 1);
+    float s = (l == 0) ? acc + c : c;
+    #pragma unroll
+    for (int t = 1; t < GR; ++t) {
+        const float prev = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(s), 0x111, 0xf, 0xf, false));
+        s = (l == t) ? prev + c : s;
+    }
+    return __int_as_float(group_last<GR>(__float_as_int(s)));
+}
+
+// element j's slot (state at the start of the frame) for the list pass
+__device__ __forceinline__ Slot es_load(const DevView* __restrict__ V, int j) {
+    GAS const float* q = G(V->es) + 6 * (size_t)j;
+    const f4u a = *(GAS const f4u*)q;
+    Slot o;
+    o.x = a.x; o.y = a.y; o.vx = a.z; o.vy = a.w; o.r = q[4]; o.e = j;
+    return o;
+}
+
+// list candidate: centre distance below r_i + r_j + 2 delta + skin (a
+// superset of the contacts of every frame until the next build)
+__device__ __forceinline__ bool within_skin(const IceDemParams& P, float skin, float xa, float ya, float ra,
+                                            float xb_, float yb_, float rb) {
+    const float dx = xb_ - xa, dy = yb_ - ya;
+    float md = ra + rb;
+    if (P.contact_dilation2 != 0.0f) md = md + P.contact_dilation2;
+    return sqrtf(dx * dx + dy * dy) < md + skin;
+}
+
+// Ice-ice contact force on element e from its partner list (group-uniform;
+// result valid in the group's lane 0).  On a build frame the grids (reach
+// extended by the skin) give the candidates, ranked by rotated index into the
+// list; an element with more than NLW candidates is marked and takes
+// contact_force_group, this frame and every later one of the call.  The sum
+// runs over the list in the reference's scatter order with +0 for listed
+// partners out of contact, so it equals the sum over the contacts.
+template <int GR>
+__device__ __forceinline__ void list_force_group(const DevView* __restrict__ V, int set, GAS const float* __restrict__ pos,
+                                                 GAS const float* __restrict__ vel, int e, float px, float py,
+                                                 float pvx, float pvy, float pr, int novf, bool rebuild,
+                                                 ForceLds<GR>& L, int g, float& fx, float& fy, float& fm) {
+    const IceDemParams& P = V->p;
+    const int n = V->n;
+    const int l = threadIdx.x & (GR - 1);
+    const int gshift = (threadIdx.x & 63) & ~(GR - 1);
+    int* lk = (int*)L.cf[g];          // candidate keys during a build (the row is free until the epilogue)
+    int* lj = lk + NLW;               // the built list
+    GAS int32_t* nlrow = G(V->nl) + (size_t)e * NLW;
+    int cnt;
+    if (rebuild) {
+        int count = 0;
+        const float skin = V->nl_skin;
+        auto push = [&](bool c, int j) {
+            const uint64_t bal = __ballot(c);
+            const uint32_t m = (uint32_t)(bal >> gshift) & ((1u << GR) - 1u);
+            const int before = __builtin_popcount(m & ((1u << l) - 1u));
+            if (c && count + before < NLW) lk[count + before] = rot_key(j, e, n);
+            count += __builtin_popcount(m);
+        };
+        auto visit = [&](bool have, int j, float4 k4, float r) {
+            push(have && j != e && within_skin(P, skin, px, py, pr, k4.x, k4.y, r), j);
+        };
+        int x0, x1, y0, y1, u0 = 0, u1 = -1, v0 = 0, v1 = -1;
+        cell_box(P, px, py, big_reach(P, pr) + skin, x0, x1, y0, y1);
+        if (V->nbig > 0) scan_range2(V, px, py, u0, u1, v0, v1);
+        ScanList S;
+        S.x0 = x0; S.y0 = y0; S.W = x1 - x0 + 1; S.nfine = S.W * (y1 - y0 + 1);
+        S.u0 = u0; S.v0 = v0; S.W2 = u1 - u0 + 1; S.ntot = S.nfine + S.W2 * (v1 - v0 + 1);
+        S.iW = __builtin_amdgcn_rcpf((float)S.W);
+        S.iW2 = S.W2 > 0 ? __builtin_amdgcn_rcpf((float)S.W2) : 0.0f;
+        scan_list<GR>(V, set, S, L, g, visit);
+        for (int base = 0; base < novf; base += GR) {
+            const int f = base + l;
+            bool c = false;
+            int j = -1;
+            if (f < novf) {
+                j = G(V->ovf_list)[(size_t)set * V->ovf_cap + f];
+                c = j != e && within_skin(P, skin, px, py, pr, pos[j], pos[n + j], G(V->radius)[j]);
+            }
+            push(c, j);
+        }
+        wave_sync_lds();
+        if (count > NLW) {
+            if (l == 0) { G(V->nl_cnt)[e] = NLW + 1; atomicOr(nl_ovf(V), 1u); }
+            wave_sync_lds();
+            contact_force_group<GR>(V, set, pos, vel, e, px, py, pvx, pvy, pr, novf, L, g, fx, fy, fm);
+            return;
+        }
+        // rank by rotated index: the reference's order (pairs (e, j > e) by j,
+        // then (i < e, e) by i)
+        for (int i = l; i < count; i += GR) {
+            const int ki = lk[i];
+            int rank = 0;
+            for (int q = 0; q < count; ++q) rank += (lk[q] < ki) ? 1 : 0;
+            const int j = (ki < n - e) ? e + ki : e + ki - n;
+            lj[rank] = j;
+            nlrow[rank] = j;
+        }
+        if (l == 0) G(V->nl_cnt)[e] = count;
+        wave_sync_lds();
+        cnt = count;
+    } else {
+        cnt = G(V->nl_cnt)[e];
+        if (cnt > NLW) {
+            contact_force_group<GR>(V, set, pos, vel, e, px, py, pvx, pvy, pr, novf, L, g, fx, fy, fm);
+            return;
+        }
+    }
+    float ax = 0.0f, ay = 0.0f, am = 0.0f;
+    for (int base = 0; base < cnt; base += GR) {
+        const int q = base + l;
+        float cx = 0.0f, cy = 0.0f, cm = 0.0f;
+        if (q < cnt) {
+            const int j = rebuild ? lj[q] : nlrow[q];
+            const Slot sl = es_load(V, j);
+            const bool lo = e < j;
+            const float xl = lo ? px : sl.x, yl = lo ? py : sl.y, rl = lo ? pr : sl.r;
+            const float xh = lo ? sl.x : px, yh = lo ? sl.y : py, rh = lo ? sl.r : pr;
+            if (overlap_v(P, xl, yl, rl, xh, yh, rh))
+                contrib_v(P, lo, xl, yl, lo ? pvx : sl.vx, lo ? pvy : sl.vy, rl, xh, yh, lo ? sl.vx : pvx,
+                          lo ? sl.vy : pvy, rh, cx, cy, cm);
+        }
+        ax = group_chain<GR>(ax, cx);
+        ay = group_chain<GR>(ay, cy);
+        am = group_chain<GR>(am, cm);
+    }
+    if (rebuild) wave_sync_lds();     // lj is read above; the row is the epilogue's staging next
+    fx = ax; fy = ay; fm = am;
+}
+
 // Partner and bond state of one bond-table entry (the dynamic reads of the
 // bond pass).
 struct BondIn {
@@ -1082,10 +1260,14 @@ __device__ __forceinline__ void bond_force_group(const DevView* __restrict__ V, 
 // forces and submergence (first sub-step), drag, rotational drag, totals and
 // the integration into the next-state buffers, from the values its group
 // staged in sv.
-__device__ __forceinline__ void force_epilogue(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                               int sub, int src_is_state, int mode,
-                                               const float* __restrict__ fship_in, float* __restrict__ fout,
-                                               float* __restrict__ mout, const float* sv, int e) {
+// With partner lists it returns the squared displacement of the element's
+// new position from its position at the last list build (written here on a
+// build frame), else 0.
+__device__ __forceinline__ float force_epilogue(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                                int sub, int src_is_state, int mode,
+                                                const float* __restrict__ fship_in, float* __restrict__ fout,
+                                                float* __restrict__ mout, const float* sv, int e,
+                                                bool lists = false, bool rebuild = false) {
     const IceDemParams& P = V->p;
     const int n = V->n;
     GAS float* dpos = G(src_is_state ? V->nxt_pos : D->s.d_pos);
@@ -1148,7 +1330,7 @@ __device__ __forceinline__ void force_epilogue(const DevView* __restrict__ V, co
     const float M_t = (mb + mi) + Md;
     if (mode == 1) {
         fout[e] = Fx_t; fout[n + e] = Fy_t; mout[e] = M_t;
-        return;
+        return 0.0f;
     }
     // integration (:215-268)
     const float dt = P.dt_ice;
@@ -1172,6 +1354,15 @@ __device__ __forceinline__ void force_epilogue(const DevView* __restrict__ V, co
     dpos[e] = npx; dpos[n + e] = npy;
     dvel[e] = nvx; dvel[n + e] = nvy;
     drot[e] = nr; domg[e] = nw;
+    float d2 = 0.0f;
+    if (lists) {
+        float bx, by;
+        if (rebuild) { bx = px; by = py; G(V->xb)[e] = px; G(V->xb)[n + e] = py; }
+        else { bx = G(V->xb)[e]; by = G(V->xb)[n + e]; }
+        const float ddx = npx - bx, ddy = npy - by;
+        d2 = ddx * ddx + ddy * ddy;
+    }
+    return d2;
 }
 
 
@@ -1221,10 +1412,12 @@ template <int GR>
 __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                            int k, int sub, int src_is_state, int mode, int set,
                            const float* __restrict__ fship_in, float* __restrict__ fout, float* __restrict__ mout,
-                           ForceLds<GR>& L, int chunk, bool again = false) {
+                           ForceLds<GR>& L, int chunk, bool again = false, bool lists = false) {
     const int n = V->n;
     const int g = threadIdx.x / GR, l = threadIdx.x & (GR - 1);
     const int e = chunk * epb<GR>() + g;
+    const bool rebuild = lists && nl_rebuild(V, k);
+    float d2 = 0.0f;
     if (e < n) {                              // group-uniform
         GAS const float* pos = G(src_is_state ? D->s.d_pos : V->nxt_pos);
         GAS const float* vel = G(src_is_state ? D->s.d_vel : V->nxt_vel);
@@ -1247,7 +1440,8 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
         BondIn bin0 = {0.0f, 0.0f, 0.0f, false, false};
         if (l < deg) bin0 = bond_in(V, D, pos, bt0, mode);   // read ahead of the contact pass
         float fix, fiy, mi;
-        contact_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, L, g, fix, fiy, mi);
+        if (lists) list_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, rebuild, L, g, fix, fiy, mi);
+        else contact_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, L, g, fix, fiy, mi);
         float fbx, fby, mb;
         bool crack;
         bond_force_group<GR>(V, D, pos, e, px, py, mode == 1 ? act_raw : act_bond, deg, bt0, bp0, bin0, mode, L, g,
@@ -1260,7 +1454,7 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
             sv[S_FLAGS] = __int_as_float((act_raw ? 1 : 0) | (act_bond ? 2 : 0) | (crack ? 4 : 0));
             // 16 lanes (small fields, latency-bound): the group's lane 0
             // finishes its own element, no barrier
-            if (GR >= 16) force_epilogue(V, D, sub, src_is_state, mode, fship_in, fout, mout, sv, e);
+            if (GR >= 16) d2 = force_epilogue(V, D, sub, src_is_state, mode, fship_in, fout, mout, sv, e, lists, rebuild);
         }
     }
     if (GR < 16) {
@@ -1273,10 +1467,17 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
             // element) rather than held in registers through the passes
             float* sv = L.stg(threadIdx.x);
             epi_put(sv, epi_in(V, D, k, sub, src_is_state, mode, et));
-            force_epilogue(V, D, sub, src_is_state, mode, fship_in, fout, mout, sv, et);
+            d2 = force_epilogue(V, D, sub, src_is_state, mode, fship_in, fout, mout, sv, et, lists, rebuild);
         }
-        if (again) __syncthreads();
     }
+    if (lists) {   // the wave's largest displacement into frame k + 1's slot
+        float m = d2;
+        #pragma unroll
+        for (int off = 1; off < 64; off <<= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+        if ((threadIdx.x & 63) == 0 && m > 0.0f)
+            atomicMax(nl_slot(V, k + 1) + (blockIdx.x & (NLSUB - 1)), __float_as_uint(m));
+    }
+    if (GR < 16 && again) __syncthreads();
 }
 
 // With nextra > 0 (a frame's first sub-step) blocks [0, nextra) run
@@ -1307,7 +1508,7 @@ __device__ __forceinline__ void force_kernel_body(const DevView* __restrict__ V,
     }
     const int nf = lg - nextra;
     force_body<GR>(V, D, k, sub, src_is_state, mode, set, fship_in, fout, mout, L,
-                   xcd ? chunk_of(lb - nextra, nf) : lb - nextra);
+                   xcd ? chunk_of(lb - nextra, nf) : lb - nextra, false, V->nl_on != 0 && mode == 0);
 }
 
 template <int GR>
@@ -1403,8 +1604,12 @@ __device__ __forceinline__ void post_bond(const DevView* __restrict__ V, const C
 // element t, cell t of the set just used, floe t.  With `cull`, the
 // element's committed state also goes through the next frame's near test
 // and into bucket set set ^ 1.
+// With partner lists the buckets are filled only for a frame that builds
+// lists (or when an element overflowed its list this call) and cleared only
+// after a frame that used them.
 __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                          int k, int final_in_nxt, int last, int t, int chunk, int set, int cull) {
+                                          int k, int final_in_nxt, int last, int t, int chunk, int set, int cull,
+                                          bool lists = false) {
     const int n = V->n;
     GAS const float* pos = G(final_in_nxt ? V->nxt_pos : D->s.d_pos);
     GAS const float* vel = G(final_in_nxt ? V->nxt_vel : D->s.d_vel);
@@ -1428,11 +1633,24 @@ __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const C
             if (D->dg.d_force_mag) G(D->dg.d_force_mag)[t] = fm_scaled(V, t);
         }
     }
-    clear_cells(V, set, t);
+    bool used = true, ins = true;
+    if (lists) {
+        const bool ovf = nl_overflowed(V);
+        const bool built = nl_rebuild(V, k);
+        used = ovf || built;
+        ins = ovf || nl_rebuild(V, k + 1);
+        if (t == 0) {
+            #pragma unroll
+            for (int i = 0; i < NLSUB; ++i)
+                __hip_atomic_store(nl_slot(V, k + 2) + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (built) atomicAdd(nl_rebuilds(V), 1u);
+        }
+    }
+    if (used) clear_cells(V, set, t);
     if (t < V->nf) { G(V->floe_coll)[t] = 0u; G(V->floe_maxfm)[t] = 0u; }
     if (t == 0) *force_ticket(V) = 0;
     if (cull && chunk < V->nblk)
-        cull_chunk(V, chunk, t < n, t, px, py, pvx, pvy, G(D->s.d_ship), set ^ 1);
+        cull_chunk(V, chunk, t < n, t, px, py, pvx, pvy, G(D->s.d_ship), set ^ 1, ins);
 }
 
 // Both parts for thread t (the one-workgroup scene kernel).
@@ -1456,7 +1674,7 @@ __device__ __forceinline__ void post_block(const DevView* __restrict__ V, const 
         post_bond(V, D, k, final_in_nxt, c * BLOCK + (int)threadIdx.x);
     } else {
         const int c = xcd ? chunk_of(b, nbe) : b;
-        post_elem(V, D, k, final_in_nxt, last, c * BLOCK + (int)threadIdx.x, c, set, cull);
+        post_elem(V, D, k, final_in_nxt, last, c * BLOCK + (int)threadIdx.x, c, set, cull, V->nl_on != 0);
     }
 }
 

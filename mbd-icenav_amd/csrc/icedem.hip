@@ -478,6 +478,16 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     H.G = prm->grid_nx * prm->grid_ny;
     H.nblk = nblk; H.nbpad = nbpad;
     H.lanes = resolve_lanes(prm->force_lanes, n);
+    if (!(prm->list_skin >= 0.0f) || !std::isfinite(prm->list_skin)) {
+        set_err(nullptr, "list_skin must be finite and >= 0"); delete ctx; return ICEDEM_E_INVALID;
+    }
+    {   // partner lists: rebuild once 2 x (largest displacement) may reach the
+        // skin less a margin far above the float32 rounding of distances
+        const float thr = 0.5f * (prm->list_skin - 0.01f);
+        H.nl_on = (prm->n_ice_steps == 1 && thr > 0.0f) ? 1 : 0;
+        H.nl_skin = H.nl_on ? prm->list_skin : 0.0f;
+        H.nl_thr2 = H.nl_on ? thr * thr : 0.0f;
+    }
     if (prm->grid_bucket < 1 || (int64_t)H.G * prm->grid_bucket > (1LL << 28)) {
         set_err(nullptr, "bad grid_bucket"); delete ctx; return ICEDEM_E_INVALID;
     }
@@ -492,7 +502,9 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
         }
         // contacts reach at most 2 r_max + delta; with cell2 >= 2 (2 r_max + delta)
         // the 2x2 coarse cells on an element's side hold all its large partners
-        const double c2 = std::max((double)prm->grid_cell, 2.0 * (2.0 * rmax + prm->contact_dilation2) * 1.002 + 1e-3);
+        // (list builds reach list_skin further)
+        const double c2 = std::max((double)prm->grid_cell,
+                                   2.0 * (2.0 * rmax + prm->contact_dilation2 + H.nl_skin) * 1.002 + 1e-3);
         H.cell2 = (float)c2;
         if ((double)H.cell2 < c2) H.cell2 = std::nextafter(H.cell2, INFINITY);
         H.nx2 = std::max(1, (int)std::ceil((double)prm->grid_nx * prm->grid_cell / H.cell2));
@@ -620,6 +632,10 @@ static int alloc_state_buffers(IceDemCtx* ctx) {
     AL(nxt_pos, float, 2 * (size_t)n); AL(nxt_vel, float, 2 * (size_t)n);
     AL(nxt_rot, float, n); AL(nxt_omega, float, n);
     AL(err, int32_t, 4);
+    if (H.nl_on) {
+        AL(nl, int32_t, (size_t)n * NLW); AL(nl_cnt, int32_t, n); AL(xb, float, 2 * (size_t)n);
+        AL(es, float, 6 * (size_t)n); AL(nlctl, uint32_t, NLCTL);
+    }
 #undef AL
     HIP_TRY(hipMemset(H.cells, 0, (size_t)(H.coarse_off + 2 * (int64_t)H.G2 * H.rec_c)));
     HIP_TRY(hipMemset(H.ovf_count, 0, sizeof(int32_t) * 2));
@@ -630,6 +646,10 @@ static int alloc_state_buffers(IceDemCtx* ctx) {
     HIP_TRY(hipMemset(H.part2, 0, sizeof(float) * 2 * nbpad));
     HIP_TRY(hipMemset(H.scal, 0, sizeof(float) * 16));
     HIP_TRY(hipMemset(H.err, 0, sizeof(int32_t) * 4));
+    if (H.nl_on) {
+        HIP_TRY(hipMemset(H.nlctl, 0, sizeof(uint32_t) * NLCTL));
+        HIP_TRY(hipMemset(H.es, 0, sizeof(float) * 6 * (size_t)n));
+    }
     HIP_TRY(hipMemset(H.nearf, 0, n));
     HIP_TRY(hipMemset(H.nearcnt, 0, sizeof(int32_t) * nblk));
     {
@@ -780,6 +800,18 @@ int icedem_frames(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* 
         it = ctx->graphs.find(K);
     }
     HIP_TRY(hipGraphLaunch(it->second, stream));
+    return ICEDEM_OK;
+}
+
+int icedem_list_stats(IceDemCtx* ctx, int32_t* h_out) {
+    if (!ctx || !h_out) { set_err(ctx, "null argument"); return ICEDEM_E_INVALID; }
+    h_out[0] = 0; h_out[1] = 0; h_out[2] = ctx->hv.nl_on;
+    if (!ctx->hv.nl_on) return ICEDEM_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    uint32_t w[2];
+    HIP_TRY(hipMemcpy(w, ctx->hv.nlctl + 3 * NLSUB, sizeof(w), hipMemcpyDeviceToHost));
+    h_out[0] = (int32_t)w[1];
+    h_out[1] = (int32_t)w[0];
     return ICEDEM_OK;
 }
 

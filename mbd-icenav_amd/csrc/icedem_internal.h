@@ -28,6 +28,9 @@ __device__ __forceinline__ int4 ldg(GAS const int4* p) {
 
 constexpr int BLOCK = 256;          // threads per block of the per-element kernels
 constexpr int MAX_PARTIALS = 4096;  // block partials reduced in one workgroup (n <= 1M)
+constexpr int NLW = 32;             // partner-list width (partners per element)
+constexpr int NLSUB = 8;            // sub-slots of each displacement maximum (one per XCD)
+constexpr int NLCTL = 3 * NLSUB + 2;  // partner-list control words
 
 // Static data, per-frame scratch and parameters of one context (device memory).
 struct DevView {
@@ -60,6 +63,17 @@ struct DevView {
     int32_t* nearcnt;         // (nblk) near elements of each 256-element block
     int32_t* counters;        // [0] near elements, [1] collided floes, [3] k_force tail ticket ([2] unused)
     int32_t* coll_list;       // (nf) floes with a hull contact this frame, counters[1] of them
+    // partner lists (params list_skin > 0 and one ice sub-step per frame)
+    int32_t nl_on;            // lists in use
+    float nl_thr2;            // rebuild once an element may have moved sqrt(nl_thr2) since the build
+    float nl_skin;            // list reach beyond r_i + r_j + 2 delta
+    int32_t* nl;              // (n, NLW) partners in the order of the reference's scatter (rotated index)
+    int32_t* nl_cnt;          // (n) listed partners; NLW + 1: more than NLW, the element scans the grids
+    float* xb;                // (2, n) positions at the last build
+    float* es;                // (n, 6) x, y, vx, vy, radius (and a pad) of every element for the frame
+    uint32_t* nlctl;          // [0, 24) squared-displacement maxima (float bits) of frame k's positions in
+                              // slots [8 (k % 3), 8 (k % 3) + 8); [24] a list overflowed this call;
+                              // [25] frames rebuilt since creation
     int32_t lanes;            // lanes per element in k_force (8 or 16)
     float* shipf;             // (2n) ship forces on ice, kept for sub-steps > 0
     uint8_t* crack;           // (n) element touches a broken bond

@@ -62,6 +62,7 @@ class IceDemParams(C.Structure):
         ("grid_x0", _f), ("grid_y0", _f), ("grid_cell", _f), ("grid_rsmall", _f),
         ("grid_nx", _i), ("grid_ny", _i), ("grid_bucket", _i), ("grid_bucket2", _i),
         ("force_lanes", _i),
+        ("list_skin", _f),
     ]
 
 
@@ -120,6 +121,7 @@ SIGNATURES = {
     "icedem_ship_rollouts": (C.c_int, [_p, _p, _i, _p, _i, _i, _i, _i, _p, _i, _i, C.c_float, C.c_float,
                                        C.c_float, C.c_float, _p, _i, _i, _p, _p]),
     "icesur_mppi_update": (C.c_int, [_p, _p, _i, _i, C.c_float, _p, _p, _p]),
+    "icesur_mbd_update": (C.c_int, [_p, _p, _i, _i, C.c_float, _p, C.c_float, C.c_float, C.c_float, _p, _p, _p]),
     "icesur_conv": (C.c_int, [_p, _i, _i, _i, _i, _p, _i, _p, _i, _p, _p, _p]),
     "icesur_adamw": (C.c_int, [_p, _p, _p, _p, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                _i, _p, _p]),
@@ -129,6 +131,7 @@ SIGNATURES = {
     "icesur_raster": (C.c_int, [_p, _p, _p, _i, _p, _i, _i, C.c_float, C.c_float, _p, _p, _p]),
     "icesur_raster_poly": (C.c_int, [_p, _p, _p, _i, _i, _p, _i, _i, C.c_float, C.c_float, _p, _p, _p]),
     "icedem_sync": (C.c_int, [_p, _p]),
+    "icedem_list_stats": (C.c_int, [_p, C.POINTER(C.c_int32)]),
     "icedem_substep_forces": (C.c_int, [_p, C.POINTER(IceDemState), _p, _p, C.c_int64, C.c_int64, _p, _p, _p]),
     "icedem_contact_pairs": (C.c_int, [_p, _p, _p, _p, C.c_int64, C.POINTER(C.c_int64), _p]),
     "icedem_collisions_combined": (C.c_int, [_p, _p, _p, _p, _i, _i, _f, _f, _f, _f, _f, _p, _i, _i, _p, _p,

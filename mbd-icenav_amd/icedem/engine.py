@@ -40,6 +40,11 @@ def bound_radius(local_polys_padded: np.ndarray) -> np.ndarray:
     return (r * 1.0001 + 1e-3).astype(F32)
 
 
+# Reach of the contact partner lists beyond contact (m): the lists are rebuilt
+# once an element may have moved about half of it.
+DEFAULT_LIST_SKIN = 0.5
+
+
 def choose_grid(centroids: np.ndarray, radii: np.ndarray, dilation: float = 0.0, cell: Optional[float] = None,
                 margin: float = 60.0, max_cells: int = 1 << 22, small_quantile: float = 0.98,
                 cell_factor: float = 1.0):
@@ -103,10 +108,12 @@ def choose_grid(centroids: np.ndarray, radii: np.ndarray, dilation: float = 0.0,
 def make_params(static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_length, ship_beam, n_ice_steps,
                 dt, dt_ice, use_auto_icenav, B_mat, T_n, grid=None, ship_mode: int = 0,
                 ctrl_thrust_max: float = 2e6, ctrl_yaw_rate_max: float = 0.15,
-                force_lanes: int = 0) -> _lib.IceDemParams:
+                force_lanes: int = 0, list_skin: float = DEFAULT_LIST_SKIN) -> _lib.IceDemParams:
     """Scalar parameters rounded to float32 in the reference's order of use.
     force_lanes: lanes per element in the force pass (8 or 16; 0 lets the
-    library choose by element count); the results do not depend on it."""
+    library choose by element count); list_skin: reach of the contact
+    partner lists beyond contact in metres (0: the grids every frame).
+    Neither changes a result."""
     p = _lib.IceDemParams()
     hl = ship_length / 2.0
     hb = ship_beam / 2.0
@@ -197,6 +204,9 @@ def make_params(static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_len
     if force_lanes not in (0, 8, 16):
         raise ValueError("force_lanes must be 0, 8 or 16")
     p.force_lanes = int(force_lanes)
+    if not (list_skin >= 0.0 and math.isfinite(list_skin)):
+        raise ValueError("list_skin must be finite and >= 0")
+    p.list_skin = F32(list_skin)
     return p
 
 
@@ -216,7 +226,7 @@ class DemEngine:
                  n_ice_steps, dt, dt_ice, use_auto_icenav, B_mat, T_n, centroids=None, device=None, grid=None,
                  cell_size=None, ship_mode: int = 0, ctrl_thrust_max: float = 2e6, ctrl_yaw_rate_max: float = 0.15,
                  grid_margin: float = 60.0, small_quantile: float = 0.98, force_lanes: int = 0,
-                 cell_factor: float = 1.0):
+                 cell_factor: float = 1.0, list_skin: float = None):
         if not torch.cuda.is_available():
             raise _lib.IceDemError("icedem needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.lib = _lib.load()
@@ -236,7 +246,8 @@ class DemEngine:
         self.params = make_params(static, cfg, ship_dyn, ship_length, ship_beam, n_ice_steps, dt, dt_ice,
                                   use_auto_icenav, B_mat, T_n, grid=grid, ship_mode=ship_mode,
                                   ctrl_thrust_max=ctrl_thrust_max, ctrl_yaw_rate_max=ctrl_yaw_rate_max,
-                                  force_lanes=force_lanes)
+                                  force_lanes=force_lanes,
+                                  list_skin=DEFAULT_LIST_SKIN if list_skin is None else list_skin)
         lp = np.ascontiguousarray(static.local_polys_padded, dtype=F32)
         self._host = dict(
             lp=lp,
@@ -431,6 +442,14 @@ class DemEngine:
         if sync:
             _lib.check(self.lib.icedem_sync(self.ctx, strm), self.ctx, "icedem_sync")
         return keep
+
+    def list_stats(self) -> dict:
+        """Partner-list diagnostics: frames that rebuilt the lists since
+        creation, whether an element overflowed its list in the last call,
+        whether the context uses lists."""
+        out = (C.c_int32 * 3)()
+        _lib.check(self.lib.icedem_list_stats(self.ctx, out), self.ctx, "icedem_list_stats")
+        return {"rebuilt_frames": int(out[0]), "overflowed": bool(out[1]), "lists": bool(out[2])}
 
     def alloc_history(self, K: int) -> dict:
         n, nb = self.n, self.n_bonds

@@ -422,6 +422,40 @@ def mppi_update(costs: torch.Tensor, samples: torch.Tensor, temperature: float) 
     return out.view(u.shape[1:])
 
 
+def mbd_schedule(n_diffuse: int, beta0: float, betaT: float):
+    """run_mbd's float32 noise schedule (jax_mbd_planner.py:1358-1361):
+    betas = linspace(beta0, betaT), alphas = 1 - betas, alphas_bar their
+    running product, sigmas = sqrt(1 - alphas_bar)."""
+    betas = np.linspace(beta0, betaT, n_diffuse).astype(np.float32)
+    alphas = (np.float32(1.0) - betas).astype(np.float32)
+    alphas_bar = np.cumprod(alphas, dtype=np.float32)
+    sigmas = np.sqrt(np.float32(1.0) - alphas_bar).astype(np.float32)
+    return betas, alphas, alphas_bar, sigmas
+
+
+def mbd_update(costs: torch.Tensor, samples: torch.Tensor, temperature: float, ybar: torch.Tensor,
+               alpha_bar: float, alpha: float, alpha_bar_prev: float) -> torch.Tensor:
+    """One reverse-diffusion step of run_mbd (jax_mbd_planner.py:1363-1396) on
+    the GPU (`icesur_mbd_update`): costs (S,), samples (S, ...) the step's
+    clipped samples, ybar the current mean -> the next mean, shaped like one
+    sample."""
+    lib = _lib.load()
+    c = costs.to(torch.float32).contiguous()
+    u = samples.to(torch.float32).contiguous()
+    S = int(c.shape[0])
+    D = int(u[0].numel())
+    yb = ybar.to(device=c.device, dtype=torch.float32).contiguous().reshape(-1)
+    if yb.numel() != D:
+        raise ValueError("ybar must have the shape of one sample")
+    work = torch.empty(2 + ((S + 255) // 256) * D, dtype=torch.float32, device=c.device)
+    out = torch.empty(D, dtype=torch.float32, device=c.device)
+    s = C.c_void_p(torch.cuda.current_stream(c.device).cuda_stream)
+    _lib.check(lib.icesur_mbd_update(_ptr(c), _ptr(u), S, D, float(temperature), _ptr(yb), float(alpha_bar),
+                                     float(alpha), float(alpha_bar_prev), _ptr(work), _ptr(out), s), None,
+               "icesur_mbd_update")
+    return out.view(u.shape[1:])
+
+
 class Planner:
     """One patch, one start state: S candidates scored by the surrogate, the
     best k judged by the batched DEM (k instances of the patch)."""
@@ -562,6 +596,40 @@ class Planner:
             v = mppi_update(c, r["candidates"], temperature).cpu().numpy()
         final = float(self.dem_costs(torch.as_tensor(v[None], device=self.judge.device))[0])
         return {"nominal": v, "dem_cost": final, "history": hist}
+
+    def mbd(self, nominal: Optional[np.ndarray] = None, n_diffuse: int = 8, temperature: float = 0.1,
+            beta0: float = 1e-5, betaT: float = 0.05, S: int = 16384, seed: int = 0,
+            init_surge_bias: float = 0.7) -> Dict:
+        """The reference's MBD loop (run_mbd, jax_mbd_planner.py:1314-1411)
+        over this planner's knots (n_knots, 2) with this planner's scores:
+        for i = n_diffuse - 1 .. 1 the step samples S candidates
+        clip(Ybar + sigma_i eps) (candidate 0 is Ybar itself), scores them
+        (surrogate, DEM on the judged ones) and moves Ybar by the diffusion
+        score step (`mbd_update`).  Starts from `nominal` (clipped) or the
+        constant surge bias.  Returns Ybar, the last step's mean and min
+        cost (the reference's three results), the final Ybar's DEM cost and
+        per-step history."""
+        if nominal is None:
+            ybar = np.zeros((self.nk, 2), np.float32)
+            ybar[:, 0] = init_surge_bias
+        else:
+            ybar = np.clip(np.asarray(nominal, np.float32), -1.0, 1.0)
+        if n_diffuse <= 1:
+            return {"ybar": ybar, "cost_mean": 0.0, "cost_min": 0.0, "history": []}
+        _, alphas, alphas_bar, sigmas = mbd_schedule(n_diffuse, beta0, betaT)
+        hist = []
+        cmean = cmin = 0.0
+        for step, i in enumerate(range(n_diffuse - 1, 0, -1)):
+            r = self.iterate(ybar, S=S, sigma=float(sigmas[i]), seed=seed + step, keep=True)
+            c = r["costs"]
+            cmean, cmin = float(c.mean()), float(c.min())
+            hist.append({"i": i, "sigma": float(sigmas[i]), "mean": cmean, "min": cmin, "best_dem": r["best_cost"],
+                         "judged": r["judged"]})
+            yb = torch.as_tensor(ybar, device=self.judge.device)
+            ybar = mbd_update(c, r["candidates"], temperature, yb, float(alphas_bar[i]), float(alphas[i]),
+                              float(alphas_bar[max(i - 1, 0)])).cpu().numpy()
+        final = float(self.dem_costs(torch.as_tensor(ybar[None], device=self.judge.device))[0])
+        return {"ybar": ybar, "cost_mean": cmean, "cost_min": cmin, "dem_cost": final, "history": hist}
 
     def random_baseline(self, nominal: np.ndarray, S: int = 16384, sigma: float = 0.2, seed: int = 0) -> Dict:
         """The same DEM budget without the surrogate: k candidates of the same

@@ -69,9 +69,11 @@ class RolloutBatch:
 
     def __init__(self, scenes: Sequence[Scene], instance_scene: Sequence[int], cfg: Optional[SimulationConfig] = None,
                  ship: Optional[ShipSpec] = None, device=None, cell_size_factor: float = 1.0,
-                 small_quantile: float = 0.98, force_lanes: int = 0):
+                 small_quantile: float = 0.98, force_lanes: int = 0, list_skin: Optional[float] = None):
         """force_lanes: lanes per element of the batched force pass (8 or 16;
-        0 chooses by the batch's total element count)."""
+        0 chooses by the batch's total element count); list_skin: reach of
+        the contact partner lists beyond contact (None: the engine default,
+        0: the grids every frame)."""
         if not torch.cuda.is_available():
             raise _lib.IceDemError("icedem needs a ROCm GPU")
         self.cfg = cfg or benchmark_config()
@@ -106,7 +108,8 @@ class RolloutBatch:
                             False, B, T_n, centroids=sc.centroids, device=self.device,
                             cell_size=rmin * cell_size_factor, ship_mode=1,
                             ctrl_thrust_max=self.ship.thrust_max, ctrl_yaw_rate_max=self.ship.yaw_rate_max,
-                            grid_margin=25.0, small_quantile=small_quantile, force_lanes=force_lanes)
+                            grid_margin=25.0, small_quantile=small_quantile, force_lanes=force_lanes,
+                            list_skin=list_skin)
             first[s_idx] = eng
             self.engines.append(eng)
         self._alloc_states()

@@ -51,6 +51,7 @@ class Case:
     _oracle64: object = None
     pair_mode: str = "all"
     force_lanes: int = 0          # lanes per element of the GPU force pass (0: the library's choice)
+    list_skin: Optional[float] = None   # GPU contact partner-list skin (None: the library's default, 0: off)
     cell_factor: float = 1.0      # GPU fine grid cell / 2 (r_s + delta)
     bucket_caps: Optional[tuple] = None   # GPU (fine, coarse) bucket capacities in place of the sized ones
 
@@ -168,6 +169,7 @@ class Case:
                 grid[6], grid[7] = self.bucket_caps
             self._engine = DemEngine(static, self.cfg, sd, self.ship_length, self.ship_beam, nsteps, self.dt, dt_ice,
                                      self.use_auto, B, T_n, centroids=sc.centroids, force_lanes=self.force_lanes,
+                                     list_skin=self.list_skin,
                                      cell_factor=self.cell_factor, grid=grid)
         return self._engine
 
