@@ -131,7 +131,7 @@ def cpu_baseline(case_args, frames_budget_s: float = 20.0, max_frames: int = 8, 
 
 
 def frame_bench(case_args, config: str, steps: int, warmup: int, chunk: int, profile_frames: int,
-                force_lanes: int = 0, cell_factor: float = 1.0, use_graph: bool = True):
+                force_lanes: int = 0, cell_factor: float = 1.0, use_graph: bool = True, list_skin=None):
     """Time `steps` frames of one field after `warmup` frames.
 
     Frames run as hipGraph replays of `chunk` frames (plus one shorter graph
@@ -155,7 +155,7 @@ def frame_bench(case_args, config: str, steps: int, warmup: int, chunk: int, pro
     dt_ice, nsteps = _time_stepping(static, cfg, cfg.dt)
     Bm, T_n = _thruster_config(sp_, L)
     eng = DemEngine(static, cfg, sd, L, B, nsteps, cfg.dt, dt_ice, True, Bm, T_n, centroids=sc.centroids,
-                    force_lanes=force_lanes, cell_factor=cell_factor)
+                    force_lanes=force_lanes, cell_factor=cell_factor, list_skin=list_skin)
     dev = eng.device
     st = build_initial_ice_state(n, sc.centroids, nb, sc.parent_ids, dev)
     ship = torch.zeros(15, dtype=torch.float32, device=dev)
@@ -190,15 +190,20 @@ def frame_bench(case_args, config: str, steps: int, warmup: int, chunk: int, pro
     strm = torch.cuda.current_stream().cuda_stream
     _lib.check(eng.lib.icedem_sync(eng.ctx, strm), eng.ctx, "sync")
     torch.cuda.synchronize()
+    ls0 = eng.list_stats()
     t0 = time.perf_counter()
     run(st, ship, warmup, steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     _lib.check(eng.lib.icedem_sync(eng.ctx, strm), eng.ctx, "sync")
+    ls1 = eng.list_stats()
     out = {"workload": f"plain DPDEM step, {config}: {n} elements, {nb} bonds, {sc.n_floes} floes, "
                        f"concentration {sc.concentration:.3f}, dt {cfg.dt}",
            "value": n * steps / elapsed, "ms_per_step": 1000.0 * elapsed / steps, "steps": steps,
-           "elapsed_s": elapsed, "chunk": chunk, "graphs_primed": primed, "roofline": None}
+           "elapsed_s": elapsed, "chunk": chunk, "graphs_primed": primed, "roofline": None,
+           "partner_lists": {"skin_m": float(eng.params.list_skin), "in_use": ls1["lists"],
+                             "timed_frames_rebuilding": ls1["rebuilt_frames"] - ls0["rebuilt_frames"],
+                             "overflowed": ls1["overflowed"]}}
     P = profile_frames
     if P <= 0:
         eng.close()
@@ -565,6 +570,8 @@ def main():
                     help="only the batched rollouts (R patches x 400 frames, frames mode), three timed runs")
     ap.add_argument("--force-lanes", type=int, default=0, choices=[0, 8, 16],
                     help="lanes per element of the force pass (0: the library's choice by element count)")
+    ap.add_argument("--list-skin", type=float, default=None,
+                    help="contact partner-list skin in metres (default: the library's; 0: grids every frame)")
     ap.add_argument("--cell-factor", type=float, default=1.0,
                     help="fine grid cell as a multiple of 2 (r_s + delta) (frames) or of the batch's default cell")
     ap.add_argument("--batch-mode", default="frames", choices=["frames", "scenes"],
@@ -598,7 +605,8 @@ def main():
 
     case_args = build_case(args.config, args.seed, args.scene_cache)
     fb = frame_bench(case_args, args.config, args.steps, args.warmup, args.chunk, args.profile_frames,
-                     force_lanes=args.force_lanes, cell_factor=args.cell_factor, use_graph=not args.no_graph)
+                     force_lanes=args.force_lanes, cell_factor=args.cell_factor, use_graph=not args.no_graph,
+                     list_skin=args.list_skin)
     sc, cfg = case_args[0], case_args[1]
     n, nb = len(sc.polygons), len(sc.bonds)
     elapsed = fb["elapsed_s"]
@@ -611,7 +619,8 @@ def main():
     extra_frames = None
     if args.config != "c1" and not args.no_extra_config:
         c1 = frame_bench(build_case("c1", args.seed, args.scene_cache), "c1", 2000, 100, 500, 200)
-        extra_frames = {"c1": {k: c1[k] for k in ("workload", "value", "ms_per_step", "steps", "roofline")}}
+        extra_frames = {"c1": {k: c1[k] for k in ("workload", "value", "ms_per_step", "steps", "roofline",
+                                                  "partner_lists")}}
     cpu = None if args.no_cpu_baseline else cpu_baseline(case_args)
     rollouts = None if args.no_rollouts else rollout_bench(R=args.rollouts)
     surrogate = None
@@ -634,6 +643,7 @@ def main():
         "config": {"workload": fb["workload"], "elements": n, "bonds": nb, "floes": sc.n_floes,
                    "concentration": sc.concentration, "frames_per_graph": fb["chunk"],
                    "graph": not args.no_graph, "graphs_captured_before_timing": fb["graphs_primed"],
+                   "partner_lists": fb["partner_lists"],
                    "parallelism": "single GPU"},
         "roofline": fb["roofline"],
         "cpu_baseline": cpu,

@@ -1256,6 +1256,65 @@ __device__ __forceinline__ void bond_force_group(const DevView* __restrict__ V, 
     }
 }
 
+// One bond-table entry with its bond force formed by the previous frame's
+// k_post, and the partner's activity as the bond pass sees it.
+struct BondPre {
+    float fx, fy, cr;
+    bool intact, act;
+};
+
+__device__ __forceinline__ BondPre bond_pre(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int4 t4,
+                                            int mode) {
+    const f4u f = *(GAS const f4u*)(G(V->bfv) + t4.x);
+    BondPre o;
+    o.fx = f.x; o.fy = f.y; o.cr = f.z; o.intact = f.w != 0.0f;
+    o.act = G(D->s.d_active)[t4.y] != 0;
+    if (mode != 1) o.act = o.act || (G(V->floe_coll)[t4.z] != 0u);
+    return o;
+}
+
+// Bond forces on element e from the forces k_post formed once per bond
+// (frames after a call's first, one ice sub-step): each table entry keeps
+// its bond's force when both ends are active or both frozen, negated for the
+// j role, and the sums run in table order (jax_simulation.py:93-106).  A
+// dropped entry would contribute a signed zero in the reference's
+// arithmetic; +0 terms leave the ordered sums unchanged.  Result valid in
+// every lane of the group.
+template <int GR>
+__device__ __forceinline__ void bond_pre_group(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int e,
+                                               bool act_e, int deg, int4 t0, BondPre b0, int mode,
+                                               float& fbx, float& fby, float& mb, bool& crack) {
+    const int l = threadIdx.x & (GR - 1);
+    const int gshift = (threadIdx.x & 63) & ~(GR - 1);
+    float ax = 0.0f, ay = 0.0f, am = 0.0f;
+    crack = false;
+    for (int base = 0; base < deg; base += GR) {
+        const int q = base + l;
+        float cx = 0.0f, cy = 0.0f, cm = 0.0f;
+        bool broken = false;
+        if (q < deg) {
+            int4 t4 = t0;
+            BondPre b = b0;
+            if (base > 0) {
+                t4 = ldg(G(V->btab) + (size_t)e * V->W + q);
+                b = bond_pre(V, D, t4, mode);
+            }
+            const bool role_j = t4.w != 0;
+            broken = !b.intact;
+            const bool ai = role_j ? b.act : act_e, aj = role_j ? act_e : b.act;
+            if ((ai && aj) || (!ai && !aj)) {
+                cx = role_j ? -b.fx : b.fx; cy = role_j ? -b.fy : b.fy; cm = role_j ? -b.cr : b.cr;
+            }
+        }
+        const uint64_t bal = __ballot(broken);
+        crack = crack || (((uint32_t)(bal >> gshift) & ((1u << GR) - 1u)) != 0u);
+        ax = group_chain<GR>(ax, cx);
+        ay = group_chain<GR>(ay, cy);
+        am = group_chain<GR>(am, cm);
+    }
+    fbx = ax; fby = ay; mb = am;
+}
+
 // The epilogue of element e by one thread (jax_simulation.py:140-268): ship
 // forces and submergence (first sub-step), drag, rotational drag, totals and
 // the integration into the next-state buffers, from the values its group
@@ -1412,10 +1471,14 @@ template <int GR>
 __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                            int k, int sub, int src_is_state, int mode, int set,
                            const float* __restrict__ fship_in, float* __restrict__ fout, float* __restrict__ mout,
-                           ForceLds<GR>& L, int chunk, bool again = false, bool lists = false) {
+                           ForceLds<GR>& L, int chunk, bool again = false, bool frames = false) {
     const int n = V->n;
     const int g = threadIdx.x / GR, l = threadIdx.x & (GR - 1);
     const int e = chunk * epb<GR>() + g;
+    // frames-mode kernels (a k_post between frames): partner lists, and bond
+    // forces formed by the previous frame's k_post
+    const bool lists = frames && V->nl_on != 0 && mode == 0;
+    const bool pre = frames && V->bf_on != 0 && mode == 0 && k > 0;
     const bool rebuild = lists && nl_rebuild(V, k);
     float d2 = 0.0f;
     if (e < n) {                              // group-uniform
@@ -1438,14 +1501,21 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
         const bool fcoll = (mode != 1) && (G(V->floe_coll)[fl] != 0u);
         const bool act_bond = act_raw || fcoll;   // activity seen by the bond pass (mode 0)
         BondIn bin0 = {0.0f, 0.0f, 0.0f, false, false};
-        if (l < deg) bin0 = bond_in(V, D, pos, bt0, mode);   // read ahead of the contact pass
+        BondPre bpre0 = {0.0f, 0.0f, 0.0f, false, false};
+        if (l < deg) {   // read ahead of the contact pass
+            if (pre) bpre0 = bond_pre(V, D, bt0, mode);
+            else bin0 = bond_in(V, D, pos, bt0, mode);
+        }
         float fix, fiy, mi;
         if (lists) list_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, rebuild, L, g, fix, fiy, mi);
         else contact_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, L, g, fix, fiy, mi);
         float fbx, fby, mb;
         bool crack;
-        bond_force_group<GR>(V, D, pos, e, px, py, mode == 1 ? act_raw : act_bond, deg, bt0, bp0, bin0, mode, L, g,
-                             fbx, fby, mb, crack);
+        if (pre)
+            bond_pre_group<GR>(V, D, e, act_bond, deg, bt0, bpre0, mode, fbx, fby, mb, crack);
+        else
+            bond_force_group<GR>(V, D, pos, e, px, py, mode == 1 ? act_raw : act_bond, deg, bt0, bp0, bin0, mode, L,
+                                 g, fbx, fby, mb, crack);
         if (l == 0) {
             float* sv = L.stg(g);
             sv[S_PX] = px; sv[S_PY] = py; sv[S_VX] = vx; sv[S_VY] = vy; sv[S_R] = r;
@@ -1508,7 +1578,7 @@ __device__ __forceinline__ void force_kernel_body(const DevView* __restrict__ V,
     }
     const int nf = lg - nextra;
     force_body<GR>(V, D, k, sub, src_is_state, mode, set, fship_in, fout, mout, L,
-                   xcd ? chunk_of(lb - nextra, nf) : lb - nextra, false, V->nl_on != 0 && mode == 0);
+                   xcd ? chunk_of(lb - nextra, nf) : lb - nextra, false, true);
 }
 
 template <int GR>
@@ -1538,7 +1608,7 @@ __device__ __forceinline__ float fm_scaled(const DevView* __restrict__ V, int i)
 // write (final positions and velocities, activity, eligibility, force
 // magnitudes), so it runs beside the element work in any order.
 __device__ __forceinline__ void post_bond(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                          int k, int final_in_nxt, int t) {
+                                          int k, int final_in_nxt, int t, bool prebond = false) {
     const IceDemParams& P = V->p;
     const int n = V->n, nb = V->nb;
     GAS const float* pos = G(final_in_nxt ? V->nxt_pos : D->s.d_pos);
@@ -1597,6 +1667,20 @@ __device__ __forceinline__ void post_bond(const DevView* __restrict__ V, const C
         G(D->s.d_bond_ds)[t] = new_ds;
         G(D->s.d_bond_intact)[t] = fin ? 1 : 0;
         if (D->h.d_bond_intact) G(D->h.d_bond_intact)[(size_t)k * nb + t] = fin ? 1 : 0;
+        if (prebond) {
+            // the next frame's bond force (jax_simulation.py:76-106) from the
+            // frame's final positions, damage and intact flag, with the float
+            // operations of bond_force_group; the activity factor stays with
+            // the force pass
+            const float stretch = dist - rest;
+            const float Fm = G(V->bstf)[t] * stretch * (1.0f - out_d);
+            float Fx = dirx * Fm, Fy = diry * Fm;
+            if (!fin) { Fx = 0.0f; Fy = 0.0f; }
+            const float cross = 0.5f * (dx * Fy - dy * Fx);
+            f4u o;
+            o.x = Fx; o.y = Fy; o.z = cross; o.w = fin ? 1.0f : 0.0f;
+            *(GAS f4u*)(G(V->bfv) + t) = o;
+        }
     }
 }
 
@@ -1671,7 +1755,7 @@ __device__ __forceinline__ void post_block(const DevView* __restrict__ V, const 
                                            int nbb, bool xcd) {
     if (b >= nbe) {
         const int c = xcd ? chunk_of(b - nbe, nbb) : b - nbe;
-        post_bond(V, D, k, final_in_nxt, c * BLOCK + (int)threadIdx.x);
+        post_bond(V, D, k, final_in_nxt, c * BLOCK + (int)threadIdx.x, V->bf_on != 0);
     } else {
         const int c = xcd ? chunk_of(b, nbe) : b;
         post_elem(V, D, k, final_in_nxt, last, c * BLOCK + (int)threadIdx.x, c, set, cull, V->nl_on != 0);

@@ -191,6 +191,15 @@ __global__ void k_bond_prm(const DevView* __restrict__ V) {
     const_cast<float2*>(V->bprm)[idx] = out;
 }
 
+// bond-major stiffness (the same float operations as k_bond_prm)
+__global__ void k_bond_stiff(const DevView* __restrict__ V) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= V->nb) return;
+    const int i = V->bi[b], j = V->bj[b];
+    const float avg_h = (V->thick[i] + V->thick[j]) * 0.5f;
+    const_cast<float*>(V->bstf)[b] = V->p.youngs_modulus * avg_h * V->p.bond_stiffness_factor;
+}
+
 // Contact pairs (e < j) of element e in ascending j, by repeated selection
 // over the scanned cells of both grids and the overflow list.
 __device__ int pairs_of(const DevView* __restrict__ V, const float* __restrict__ pos, int e,
@@ -488,6 +497,7 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
         H.nl_skin = H.nl_on ? prm->list_skin : 0.0f;
         H.nl_thr2 = H.nl_on ? thr * thr : 0.0f;
     }
+    H.bf_on = (prm->n_ice_steps == 1 && st->n_bonds > 0) ? 1 : 0;
     if (prm->grid_bucket < 1 || (int64_t)H.G * prm->grid_bucket > (1LL << 28)) {
         set_err(nullptr, "bad grid_bucket"); delete ctx; return ICEDEM_E_INVALID;
     }
@@ -540,6 +550,12 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
         UP(bpair, int4, bp.data(), bp.size());
     }
     UP(rest, float, st->h_rest_lengths, st->n_bonds);
+    if (st->n_bonds > 0) {   // bond-major stiffness, the formula of k_bond_prm
+        float* bs;
+        rc = dalloc(ctx, &bs, st->n_bonds, true);
+        if (rc) { icedem_destroy(ctx); return rc; }
+        H.bstf = bs;
+    }
     UP(bstr, float, st->h_bond_strengths, st->n_bonds);
     UP(sbar, float, st->h_sigma_bar, st->n_bonds);
     UP(hull, float2, st->h_hull_local, st->n_hull);
@@ -590,6 +606,11 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
         hipLaunchKernelGGL(k_bond_prm, dim3((unsigned)((tw + 255) / 256)), dim3(256), 0, 0, (const DevView*)ctx->dv);
         HIP_TRY(hipGetLastError());
     }
+    if (st->n_bonds > 0) {
+        hipLaunchKernelGGL(k_bond_stiff, dim3((unsigned)((st->n_bonds + 255) / 256)), dim3(256), 0, 0,
+                           (const DevView*)ctx->dv);
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipDeviceSynchronize());
     *out = ctx;
     return ICEDEM_OK;
@@ -632,6 +653,7 @@ static int alloc_state_buffers(IceDemCtx* ctx) {
     AL(nxt_pos, float, 2 * (size_t)n); AL(nxt_vel, float, 2 * (size_t)n);
     AL(nxt_rot, float, n); AL(nxt_omega, float, n);
     AL(err, int32_t, 4);
+    if (H.bf_on) AL(bfv, float4, (size_t)H.nb);
     if (H.nl_on) {
         AL(nl, int32_t, (size_t)n * NLW); AL(nl_cnt, int32_t, n); AL(xb, float, 2 * (size_t)n);
         AL(es, float, 6 * (size_t)n); AL(nlctl, uint32_t, NLCTL);

@@ -63,6 +63,11 @@ struct DevView {
     int32_t* nearcnt;         // (nblk) near elements of each 256-element block
     int32_t* counters;        // [0] near elements, [1] collided floes, [3] k_force tail ticket ([2] unused)
     int32_t* coll_list;       // (nf) floes with a hull contact this frame, counters[1] of them
+    // bond forces formed once per bond by k_post for the next frame (one ice
+    // sub-step per frame): (Fx, Fy, torque, intact) without the activity factor
+    int32_t bf_on;
+    float* bstf;              // (nb) bond stiffness E * mean thickness * bond_stiffness_factor
+    float4* bfv;              // (nb) the next frame's bond forces
     // partner lists (params list_skin > 0 and one ice sub-step per frame)
     int32_t nl_on;            // lists in use
     float nl_thr2;            // rebuild once an element may have moved sqrt(nl_thr2) since the build

@@ -42,7 +42,7 @@ def bound_radius(local_polys_padded: np.ndarray) -> np.ndarray:
 
 # Reach of the contact partner lists beyond contact (m): the lists are rebuilt
 # once an element may have moved about half of it.
-DEFAULT_LIST_SKIN = 0.5
+DEFAULT_LIST_SKIN = 1.0
 
 
 def choose_grid(centroids: np.ndarray, radii: np.ndarray, dilation: float = 0.0, cell: Optional[float] = None,
