@@ -569,7 +569,8 @@ class Planner:
         dem = np.asarray(dem)
         b = int(np.argmin(dem))
         out = {"best_index": int(judged[b]), "best_cost": float(dem[b]), "best_controls": cand[judged[b]],
-               "judged": len(judged), "median_rel_err_judged": err,
+               "judged": len(judged), "judged_indices": list(judged), "judged_costs": dem.copy(),
+               "median_rel_err_judged": err,
                "surrogate_s": t1 - t0, "judge_s": t2 - t1, "scored": S + len(judged),
                "rollouts_per_s": (S + len(judged)) / (t2 - t0)}
         if keep:

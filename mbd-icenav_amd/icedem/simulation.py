@@ -57,14 +57,42 @@ def _as_dev(x, device, dtype=torch.float32):
 
 
 class FrameStep:
-    """The compiled single-frame step returned by :func:`_make_frame_step`."""
+    """The compiled single-frame step returned by :func:`_make_frame_step`.
 
-    def __init__(self, engine: DemEngine):
-        self.engine = engine
+    The broad-phase grid covers the centroids given at construction, or, when
+    none were given, the positions of the first state the step receives; a
+    state with an element outside the grid's extent (positions outside still
+    give exact results, but every such element lands in a border cell) makes
+    the step rebuild its grid around that state (``regrids`` counts it)."""
+
+    def __init__(self, make_engine, centroids=None):
+        self._make = make_engine
+        self._engine = make_engine(centroids) if centroids is not None else None
+        self.regrids = 0
+
+    @property
+    def engine(self) -> DemEngine:
+        if self._engine is None:
+            self._engine = self._make(None)
+        return self._engine
+
+    def _grid_for(self, state: IceState) -> DemEngine:
+        pos = state.positions
+        if self._engine is None:
+            self._engine = self._make(torch.as_tensor(pos).detach().cpu().numpy().astype(np.float64))
+            return self._engine
+        g = self._engine.grid
+        lo = torch.as_tensor(pos).detach().amin(dim=0).cpu().numpy()
+        hi = torch.as_tensor(pos).detach().amax(dim=0).cpu().numpy()
+        if lo[0] < g[0] or lo[1] < g[1] or hi[0] > g[0] + g[4] * g[2] or hi[1] > g[1] + g[5] * g[2]:
+            self._engine.close()
+            self._engine = self._make(torch.as_tensor(pos).detach().cpu().numpy().astype(np.float64))
+            self.regrids += 1
+        return self._engine
 
     def __call__(self, state: IceState, ship_state: ShipState, setpoint, setpoint_next, v_water,
                  dpdem_eligible, v_water_ship=None):
-        eng = self.engine
+        eng = self._grid_for(state)
         st = eng.clone_state(state)
         st.dpdem_floe_eligible.copy_(_as_dev(dpdem_eligible, eng.device, torch.bool))
         ship_buf = eng.ship_buffer(ship_state)
@@ -80,13 +108,16 @@ class FrameStep:
 
 def _make_frame_step(static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_length, ship_beam, n_ice_steps,
                      dt, dt_ice, use_auto_icenav, B_mat, T_n, centroids=None, device=None, cell_size=None):
-    """jax_simulation.py:302-701.  ``centroids`` sets the broad-phase grid
-    extent (defaults to the origin +-; positions outside it are still exact)."""
-    if centroids is None:
-        centroids = np.zeros((len(static.radii), 2))
-    eng = DemEngine(static, cfg, ship_dyn, ship_length, ship_beam, n_ice_steps, dt, dt_ice, use_auto_icenav,
-                    B_mat, T_n, centroids=centroids, device=device, cell_size=cell_size)
-    return FrameStep(eng)
+    """jax_simulation.py:302-701.  ``centroids`` (not a reference argument)
+    sets the broad-phase grid's extent; without it the grid is sized from the
+    first state the returned step receives (see :class:`FrameStep`)."""
+
+    def make(cen):
+        if cen is None:
+            cen = np.zeros((len(static.radii), 2))
+        return DemEngine(static, cfg, ship_dyn, ship_length, ship_beam, n_ice_steps, dt, dt_ice, use_auto_icenav,
+                         B_mat, T_n, centroids=cen, device=device, cell_size=cell_size)
+    return FrameStep(make, centroids)
 
 
 class JaxSimContext(NamedTuple):
@@ -172,13 +203,30 @@ def _setpoints_for(traj, frame_start, K, eta=None):
     return np.concatenate([traj[idx], traj[idx_next]], axis=1)
 
 
-def _v_water(ocean_current, K, state, device):
-    if ocean_current is None or isinstance(ocean_current, NoCurrentModel):
-        return None
-    if hasattr(ocean_current, "sample_chunk"):
-        return ocean_current.sample_chunk(K, state.positions.shape[0], device=device)
+def _currents(ocean_current, K, state, ship_xy, device):
+    """Water velocity at the elements (K, n, 2) (None: still water) and at the
+    ship (K, 2) for a chunk of K frames, advancing the model K steps
+    (jax_simulation.py:1610-1638): a model with a chunk sampler is sampled
+    once and held over the chunk (static fields); any other is sampled at the
+    chunk's start positions and stepped frame by frame, for the ice and the
+    ship alike."""
+    ship_pt = np.asarray(ship_xy, dtype=np.float64)[None, :]
+    if ocean_current is None or isinstance(ocean_current, NoCurrentModel) or hasattr(ocean_current, "sample_chunk"):
+        vw = None
+        if ocean_current is not None and not isinstance(ocean_current, NoCurrentModel):
+            vw = ocean_current.sample_chunk(K, state.positions.shape[0], device=device)
+        vs = np.zeros(2, np.float32) if ocean_current is None else ocean_current.sample_batch(ship_pt)[0]
+        if ocean_current is not None:
+            for _ in range(K):
+                ocean_current.step()
+        return vw, np.tile(np.asarray(vs, dtype=np.float32), (K, 1))
     pos = state.positions.detach().cpu().numpy()
-    return np.stack([ocean_current.sample_batch(pos) for _ in range(K)]).astype(np.float32)
+    rows, ship_rows = [], []
+    for _ in range(K):
+        rows.append(ocean_current.sample_batch(pos))
+        ship_rows.append(ocean_current.sample_batch(ship_pt)[0])
+        ocean_current.step()
+    return np.stack(rows).astype(np.float32), np.stack(ship_rows).astype(np.float32)
 
 
 def simulate_jax_chunked_replan(polygons, centroids, bonds, ship_pos, ship_length, ship_beam, n_frames, dt,
@@ -238,11 +286,7 @@ def simulate_jax_chunked_replan(polygons, centroids, bonds, ship_pos, ship_lengt
                 if end_slice < len(ship_trajectory):
                     ship_trajectory[end_slice:] = new_traj[-1]
         sp = _setpoints_for(ship_trajectory, frame_start, K, ship_state.eta.cpu().numpy())
-        vw = _v_water(ocean_current, K, state, eng.device)
-        ship_xy = ship_state.eta.cpu().numpy()[:2]
-        vws = np.tile(np.asarray(ocean_current.sample_batch(ship_xy[None, :])[0], dtype=np.float32), (K, 1))
-        for _ in range(K):
-            ocean_current.step()
+        vw, vws = _currents(ocean_current, K, state, ship_state.eta.cpu().numpy()[:2], eng.device)
         pad = chunk - K
         if pad > 0:
             sp = np.concatenate([sp, np.repeat(sp[-1:], pad, axis=0)], axis=0)
