@@ -64,10 +64,19 @@ def build_case(config: str, seed: int, cache_dir: str = ""):
 
 
 def algorithmic_bytes(kernel: str, n: int, nb: int, nf: int, G: int, n_near: int, max_v: int, n_hull: int,
-                      near_blocks: int, nbpad: int) -> float:
+                      near_blocks: int, nbpad: int, listed: int = -1) -> float:
     """Bytes each kernel must move per launch, every array counted once.
 
-    k_force, per element: state reads pos 8, vel 8, rot 4, omega 4, active 1,
+    k_force on a frame that uses the partner lists (listed >= 0: partners
+    listed over all elements), per element: state reads pos 8, vel 8, rot 4,
+    omega 4, active 1, sub 4; statics radius, thickness, area, mass, inertia,
+    floe, bond count 28; its slot as a partner (x, y, vx, vy, r) 24; position
+    at the list build 8; list length 4; writes next pos/vel/rot/omega 24,
+    active 1, sub 4, force_mag 4, crack 1 -> 127, plus 4 per listed partner.
+    Per bond: two table entries of 16 B and the bond's force record
+    (Fx, Fy, torque, intact) 16 -> 48.  Floe flags 4 per floe; the tail's
+    trees as below.
+    k_force scanning the grids, per element: state reads pos 8, vel 8, rot 4, omega 4, active 1,
     sub 4; static reads radius, thickness, area, mass, inertia, floe, bond
     count 28; the element's bucket slot as a contact partner 24; writes next
     pos/vel/rot/omega 24, active 1, sub 4, force_mag 4, crack flag 1 -> 115.
@@ -84,6 +93,8 @@ def algorithmic_bytes(kernel: str, n: int, nb: int, nf: int, G: int, n_near: int
     8, near flag 1, bucket slot 24 and count 4 -> 112; cell clears 4 per
     cell, floe clears 8 per floe.
     """
+    if kernel == "k_force" and listed >= 0:
+        return 127.0 * n + 4.0 * listed + 48.0 * nb + 4.0 * nf + near_blocks * 256 * 13.0 + 12.0 * nbpad
     if kernel == "k_force":
         return 115.0 * n + 53.0 * nb + 4.0 * G + 4.0 * nf + near_blocks * 256 * 13.0 + 12.0 * nbpad
     if kernel == "k_near":
@@ -224,7 +235,9 @@ def frame_bench(case_args, config: str, steps: int, warmup: int, chunk: int, pro
     nbpad = 1
     while nbpad < (n + 255) // 256:
         nbpad <<= 1
-    abytes = algorithmic_bytes(dom, n, nb, sc.n_floes, G, n_near, eng.max_v, hull.shape[0], near_blk, nbpad)
+    lst = eng.list_stats()
+    abytes = algorithmic_bytes(dom, n, nb, sc.n_floes, G, n_near, eng.max_v, hull.shape[0], near_blk, nbpad,
+                               listed=lst["listed_partners"] if lst["lists"] else -1)
     achieved = abytes / (avg_us[dom] * 1e-6) / 1e9
     traffic = None
     tfile = ROOT / "profiles" / f"traffic_{config}.json"
@@ -237,7 +250,8 @@ def frame_bench(case_args, config: str, steps: int, warmup: int, chunk: int, pro
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                        "kernel_avg_us": avg_us[dom], "algorithmic_bytes": abytes, "kernel_avg_us_all": avg_us,
-                       "near_elements": n_near}
+                       "near_elements": n_near, "mean_listed_partners": (lst["listed_partners"] / n
+                                                                          if lst["lists"] else None)}
     return out
 
 

@@ -224,7 +224,9 @@ int icedem_frames(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* 
  * that rebuilt the contact partner lists since the context was created,
  * h_out[1] = 1 if an element had more partners than a list holds during the
  * last call (it then scans the grids every frame), h_out[2] = 1 if the
- * context uses partner lists.  Synchronises the device. */
+ * context uses partner lists, h_out[3] = partners listed over all elements at
+ * the last build (elements beyond a list's width not counted).  Synchronises
+ * the device. */
 int icedem_list_stats(IceDemCtx* ctx, int32_t* h_out);
 
 int icedem_frames_profiled(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* in, int32_t K,

@@ -827,13 +827,18 @@ int icedem_frames(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* 
 
 int icedem_list_stats(IceDemCtx* ctx, int32_t* h_out) {
     if (!ctx || !h_out) { set_err(ctx, "null argument"); return ICEDEM_E_INVALID; }
-    h_out[0] = 0; h_out[1] = 0; h_out[2] = ctx->hv.nl_on;
+    h_out[0] = 0; h_out[1] = 0; h_out[2] = ctx->hv.nl_on; h_out[3] = 0;
     if (!ctx->hv.nl_on) return ICEDEM_OK;
     HIP_TRY(hipSetDevice(ctx->device));
     uint32_t w[2];
     HIP_TRY(hipMemcpy(w, ctx->hv.nlctl + 3 * NLSUB, sizeof(w), hipMemcpyDeviceToHost));
     h_out[0] = (int32_t)w[1];
     h_out[1] = (int32_t)w[0];
+    std::vector<int32_t> cnt(ctx->hv.n);
+    HIP_TRY(hipMemcpy(cnt.data(), ctx->hv.nl_cnt, sizeof(int32_t) * cnt.size(), hipMemcpyDeviceToHost));
+    int64_t tot = 0;
+    for (int32_t c : cnt) tot += (c >= 0 && c <= NLW) ? c : 0;
+    h_out[3] = (int32_t)std::min<int64_t>(tot, INT32_MAX);
     return ICEDEM_OK;
 }
 

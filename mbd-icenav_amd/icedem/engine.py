@@ -446,10 +446,11 @@ class DemEngine:
     def list_stats(self) -> dict:
         """Partner-list diagnostics: frames that rebuilt the lists since
         creation, whether an element overflowed its list in the last call,
-        whether the context uses lists."""
-        out = (C.c_int32 * 3)()
+        whether the context uses lists, partners listed at the last build."""
+        out = (C.c_int32 * 4)()
         _lib.check(self.lib.icedem_list_stats(self.ctx, out), self.ctx, "icedem_list_stats")
-        return {"rebuilt_frames": int(out[0]), "overflowed": bool(out[1]), "lists": bool(out[2])}
+        return {"rebuilt_frames": int(out[0]), "overflowed": bool(out[1]), "lists": bool(out[2]),
+                "listed_partners": int(out[3])}
 
     def alloc_history(self, K: int) -> dict:
         n, nb = self.n, self.n_bonds
