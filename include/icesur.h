@@ -152,6 +152,20 @@ int icesur_ke_costmap(const float* d_pos, const float* d_radius, const float* d_
 int icesur_mppi_update(const float* d_costs, const float* d_samples, int32_t S, int32_t D, float temperature,
                        float* d_work, float* d_out, void* stream);
 
+/* One reverse-diffusion step of the reference's MBD planner (run_mbd,
+ * jax_mbd_planner.py:1363-1396): d_samples (S, D) are the step's clipped
+ * samples Y0s, d_costs (S) their costs, d_ybar (D) the current mean.  With
+ * run_mppi's weights w_i = exp(-(c_i - min c) / temperature),
+ * Ybar_new = sum_i w_i Y0s_i / sum_i w_i, Yi = Ybar sqrt(alpha_bar),
+ * score = (-Yi + sqrt(alpha_bar) Ybar_new) / (1 - alpha_bar),
+ * d_out = clip((Yi + (1 - alpha_bar) score) / sqrt(alpha) / sqrt(alpha_bar_prev), -1, 1)
+ * where alpha_bar = alphas_bar[i], alpha = alphas[i], alpha_bar_prev =
+ * alphas_bar[max(i - 1, 0)].  Fixed-order sums; d_work holds
+ * 2 + ceil(S / 256) * D floats; d_out may alias d_ybar. */
+int icesur_mbd_update(const float* d_costs, const float* d_samples, int32_t S, int32_t D, float temperature,
+                      const float* d_ybar, float alpha_bar, float alpha, float alpha_bar_prev, float* d_work,
+                      float* d_out, void* stream);
+
 /* The same window from the elements' convex polygons: d_verts (n, max_v, 2)
  * world vertices (the first d_nv[e] used, either orientation), each cell
  * sampled at 4 x 4 points; coverage and thickness in the same fixed point

@@ -199,3 +199,65 @@ def test_mppi_loop_runs_on_planner_scores():
     again = P.dem_costs(torch.as_tensor(r["nominal"][None], device="cuda"))[0]
     assert again == r["dem_cost"]
     P.close()
+
+
+def _mbd_ref(c, u, T, ybar, ab, a, abp):
+    """run_mbd's reverse step (jax_mbd_planner.py:1363-1396) in float64 NumPy."""
+    c = c.astype(np.float64); u = u.astype(np.float64); yb = ybar.astype(np.float64)
+    logp = (-c - (-c).max()) / T
+    w = np.exp(logp) / np.exp(logp).sum()
+    ynew = (w[:, None] * u).sum(0)
+    yi = yb * np.sqrt(ab)
+    score = (-yi + np.sqrt(ab) * ynew) / (1.0 - ab)
+    yim1 = (yi + (1.0 - ab) * score) / np.sqrt(a)
+    return np.clip(yim1 / np.sqrt(abp), -1.0, 1.0)
+
+
+def test_mbd_update_matches_float64():
+    """icesur_mbd_update against float64 NumPy of the same step, on the
+    reference's schedule (beta 1e-5 .. 0.05, 50 steps) and on arbitrary
+    schedule values (where the mean does not cancel)."""
+    rng = np.random.default_rng(6)
+    _, al, ab, _ = pl.mbd_schedule(50, 1e-5, 0.05)
+    cases = [(2048, 80, 0.01, ab[i], al[i], ab[i - 1]) for i in (1, 10, 49)]
+    cases += [(1000, 80, 0.1, 0.9, 0.95, 0.7), (5, 3, 1.0, 0.3, 0.6, 0.5)]
+    for S, D, T, ab_i, a_i, abp in cases:
+        c = rng.normal(10.0, 1.0, S).astype(np.float32)
+        u = np.clip(rng.normal(0.3, 0.4, (S, D)), -1, 1).astype(np.float32)
+        yb = np.clip(rng.normal(0.2, 0.5, D), -1, 1).astype(np.float32)
+        got = pl.mbd_update(torch.as_tensor(c, device="cuda"), torch.as_tensor(u, device="cuda"), T,
+                            torch.as_tensor(yb, device="cuda"), ab_i, a_i, abp).cpu().numpy()
+        ref = _mbd_ref(c, u, T, yb, float(ab_i), float(a_i), float(abp))
+        np.testing.assert_allclose(got, ref, rtol=2e-5, atol=5e-6)
+
+
+def test_mbd_step_on_its_schedule_is_the_weighted_mean():
+    """With alphas_bar[i] = alphas[i] alphas_bar[i-1] the score step cancels:
+    Ybar_{i-1} = clip(sum w Y0s / sum w), whatever Ybar_i was -- the MPPI
+    update of the same costs and samples."""
+    rng = np.random.default_rng(8)
+    _, al, ab, _ = pl.mbd_schedule(50, 1e-5, 0.05)
+    for i in (1, 25, 49):
+        c = torch.as_tensor(rng.normal(5.0, 2.0, 4096).astype(np.float32), device="cuda")
+        u = torch.as_tensor(np.clip(rng.normal(0.0, 0.6, (4096, 40, 2)), -1, 1).astype(np.float32), device="cuda")
+        yb = torch.as_tensor(rng.uniform(-1, 1, (40, 2)).astype(np.float32), device="cuda")
+        got = pl.mbd_update(c, u, 0.05, yb, ab[i], al[i], ab[i - 1]).cpu().numpy()
+        ref = pl.mppi_update(c, u, 0.05).cpu().numpy()
+        np.testing.assert_allclose(got, ref, atol=2e-5)
+
+
+def test_mbd_loop_runs_on_planner_scores():
+    sc = scenario.scene_patch(5001, 200, 0.4)
+    tmp = ro.RolloutBatch([sc], [0])
+    start = tmp.ship_init[0].clone()
+    tmp.close()
+    net = Surrogate(Arch(), seed=0)
+    norm = pl.Norm(np.zeros(3, np.float32), np.ones(3, np.float32))
+    P = pl.Planner(net, norm, sc, start, k=64, max_judged=64)
+    r = P.mbd(None, n_diffuse=3, temperature=0.1, S=1024, seed=3)
+    assert r["ybar"].shape == (40, 2) and np.all(np.abs(r["ybar"]) <= 1.0)
+    assert [h["i"] for h in r["history"]] == [2, 1] and np.isfinite(r["dem_cost"])
+    assert r["history"][0]["sigma"] > r["history"][1]["sigma"]
+    again = P.dem_costs(torch.as_tensor(r["ybar"][None], device="cuda"))[0]
+    assert again == r["dem_cost"]
+    P.close()
