@@ -1087,7 +1087,7 @@ template <int GR>
 __device__ __forceinline__ void list_force_group(const DevView* __restrict__ V, int set, GAS const float* __restrict__ pos,
                                                  GAS const float* __restrict__ vel, int e, float px, float py,
                                                  float pvx, float pvy, float pr, int novf, bool rebuild,
-                                                 ForceLds<GR>& L, int g, float& fx, float& fy, float& fm) {
+                                                 int cnt_in, ForceLds<GR>& L, int g, float& fx, float& fy, float& fm) {
     const IceDemParams& P = V->p;
     const int n = V->n;
     const int l = threadIdx.x & (GR - 1);
@@ -1149,29 +1149,49 @@ __device__ __forceinline__ void list_force_group(const DevView* __restrict__ V, 
         wave_sync_lds();
         cnt = count;
     } else {
-        cnt = G(V->nl_cnt)[e];
+        cnt = cnt_in;
         if (cnt > NLW) {
             contact_force_group<GR>(V, set, pos, vel, e, px, py, pvx, pvy, pr, novf, L, g, fx, fy, fm);
             return;
         }
     }
+    // 8 lanes: two rounds of partners per pass, both rounds' list entries,
+    // then both rounds' slots, in flight together (16 lanes: one round; the
+    // second set of registers cost occupancy there)
+    constexpr int NR = GR < 16 ? 2 : 1;
     float ax = 0.0f, ay = 0.0f, am = 0.0f;
-    for (int base = 0; base < cnt; base += GR) {
-        const int q = base + l;
-        float cx = 0.0f, cy = 0.0f, cm = 0.0f;
-        if (q < cnt) {
-            const int j = rebuild ? lj[q] : nlrow[q];
-            const Slot sl = es_load(V, j);
-            const bool lo = e < j;
-            const float xl = lo ? px : sl.x, yl = lo ? py : sl.y, rl = lo ? pr : sl.r;
-            const float xh = lo ? sl.x : px, yh = lo ? sl.y : py, rh = lo ? sl.r : pr;
-            if (overlap_v(P, xl, yl, rl, xh, yh, rh))
-                contrib_v(P, lo, xl, yl, lo ? pvx : sl.vx, lo ? pvy : sl.vy, rl, xh, yh, lo ? sl.vx : pvx,
-                          lo ? sl.vy : pvy, rh, cx, cy, cm);
+    auto term = [&](bool have, const Slot& sl, float& cx, float& cy, float& cm) {
+        cx = 0.0f; cy = 0.0f; cm = 0.0f;
+        if (!have) return;
+        const bool lo = e < sl.e;
+        const float xl = lo ? px : sl.x, yl = lo ? py : sl.y, rl = lo ? pr : sl.r;
+        const float xh = lo ? sl.x : px, yh = lo ? sl.y : py, rh = lo ? sl.r : pr;
+        if (overlap_v(P, xl, yl, rl, xh, yh, rh))
+            contrib_v(P, lo, xl, yl, lo ? pvx : sl.vx, lo ? pvy : sl.vy, rl, xh, yh, lo ? sl.vx : pvx,
+                      lo ? sl.vy : pvy, rh, cx, cy, cm);
+    };
+    for (int base = 0; base < cnt; base += NR * GR) {
+        Slot sl[NR];
+        bool h[NR];
+        #pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int q = base + r * GR + l;
+            h[r] = q < cnt;
+            sl[r].e = h[r] ? (rebuild ? lj[q] : nlrow[q]) : 0;
         }
-        ax = group_chain<GR>(ax, cx);
-        ay = group_chain<GR>(ay, cy);
-        am = group_chain<GR>(am, cm);
+        #pragma unroll
+        for (int r = 0; r < NR; ++r)
+            if (h[r]) sl[r] = es_load(V, sl[r].e);
+        #pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (base + r * GR < cnt) {
+                float cx, cy, cm;
+                term(h[r], sl[r], cx, cy, cm);
+                ax = group_chain<GR>(ax, cx);
+                ay = group_chain<GR>(ay, cy);
+                am = group_chain<GR>(am, cm);
+            }
+        }
     }
     if (rebuild) wave_sync_lds();     // lj is read above; the row is the epilogue's staging next
     fx = ax; fy = ay; fm = am;
@@ -1492,6 +1512,7 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
         const bool act_raw = active[e] != 0;
         const int deg = V->W > 0 ? G(V->bdeg)[e] : 0;
         const int novf = G(V->ovf_count)[set];
+        const int lcnt = (lists && !rebuild) ? G(V->nl_cnt)[e] : 0;   // read with the element
         int4 bt0 = make_int4(0, 0, 0, 0);
         float2 bp0 = make_float2(0.0f, 0.0f);
         if (l < V->W) { bt0 = ldg(G(V->btab) + (size_t)e * V->W + l); bp0 = ldg(G(V->bprm) + (size_t)e * V->W + l); }
@@ -1507,7 +1528,8 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
             else bin0 = bond_in(V, D, pos, bt0, mode);
         }
         float fix, fiy, mi;
-        if (lists) list_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, rebuild, L, g, fix, fiy, mi);
+        if (lists)
+            list_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, rebuild, lcnt, L, g, fix, fiy, mi);
         else contact_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, L, g, fix, fiy, mi);
         float fbx, fby, mb;
         bool crack;
@@ -1649,10 +1671,13 @@ __device__ __forceinline__ void post_bond(const DevView* __restrict__ V, const C
         float sigma_t = st_el * (1.0f - dmg);
         const float dmx = G(D->s.d_bond_dmmax)[t], dm0 = G(D->s.d_bond_dm0)[t];
         const bool intact = G(D->s.d_bond_intact)[t] != 0;
-        float nd, ndmx, ndm0;
-        bool nint;
-        bond_damage_update(delta_n, new_ds, sigma_n, sigma_t, G(V->sbar)[t], P.bond_cohesion, P.bond_friction,
-                           P.G_c_I, P.G_c_II, P.eta_bk, dmg, dmx, dm0, intact, nd, ndmx, ndm0, nint);
+        // the damage update only where its results are used (every one of
+        // them is selected by dpdem_ok below)
+        float nd = dmg, ndmx = dmx, ndm0 = dm0;
+        bool nint = intact;
+        if (dpdem_ok)
+            bond_damage_update(delta_n, new_ds, sigma_n, sigma_t, G(V->sbar)[t], P.bond_cohesion, P.bond_friction,
+                               P.G_c_I, P.G_c_II, P.eta_bk, dmg, dmx, dm0, intact, nd, ndmx, ndm0, nint);
         float out_d = dpdem_ok ? nd : dmg;
         float out_mx = dpdem_ok ? ndmx : dmx;
         float out_m0 = dpdem_ok ? ndm0 : dm0;
