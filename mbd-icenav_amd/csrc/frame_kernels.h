@@ -1077,7 +1077,8 @@ __device__ __forceinline__ bool within_skin(const IceDemParams& P, float skin, f
 }
 
 // Ice-ice contact force on element e from its partner list (group-uniform;
-// result valid in the group's lane 0).  On a build frame the grids (reach
+// result valid in the group's lane 0; with sums false only the list is
+// built, on a build frame, and the result is 0).  On a build frame the grids (reach
 // extended by the skin) give the candidates, ranked by rotated index into the
 // list; an element with more than NLW candidates is marked and takes
 // contact_force_group, this frame and every later one of the call.  The sum
@@ -1087,7 +1088,8 @@ template <int GR>
 __device__ __forceinline__ void list_force_group(const DevView* __restrict__ V, int set, GAS const float* __restrict__ pos,
                                                  GAS const float* __restrict__ vel, int e, float px, float py,
                                                  float pvx, float pvy, float pr, int novf, bool rebuild,
-                                                 int cnt_in, ForceLds<GR>& L, int g, float& fx, float& fy, float& fm) {
+                                                 int cnt_in, bool sums, ForceLds<GR>& L, int g, float& fx, float& fy,
+                                                 float& fm) {
     const IceDemParams& P = V->p;
     const int n = V->n;
     const int l = threadIdx.x & (GR - 1);
@@ -1132,7 +1134,8 @@ __device__ __forceinline__ void list_force_group(const DevView* __restrict__ V, 
         if (count > NLW) {
             if (l == 0) { G(V->nl_cnt)[e] = NLW + 1; atomicOr(nl_ovf(V), 1u); }
             wave_sync_lds();
-            contact_force_group<GR>(V, set, pos, vel, e, px, py, pvx, pvy, pr, novf, L, g, fx, fy, fm);
+            if (sums) contact_force_group<GR>(V, set, pos, vel, e, px, py, pvx, pvy, pr, novf, L, g, fx, fy, fm);
+            else { fx = 0.0f; fy = 0.0f; fm = 0.0f; }
             return;
         }
         // rank by rotated index: the reference's order (pairs (e, j > e) by j,
@@ -1147,6 +1150,7 @@ __device__ __forceinline__ void list_force_group(const DevView* __restrict__ V, 
         }
         if (l == 0) G(V->nl_cnt)[e] = count;
         wave_sync_lds();
+        if (!sums) { fx = 0.0f; fy = 0.0f; fm = 0.0f; return; }   // list built, forces not needed
         cnt = count;
     } else {
         cnt = cnt_in;
@@ -1333,6 +1337,33 @@ __device__ __forceinline__ void bond_pre_group(const DevView* __restrict__ V, co
         am = group_chain<GR>(am, cm);
     }
     fbx = ax; fby = ay; mb = am;
+}
+
+// Crack flag of a frozen element: whether any of its bond-table entries is
+// broken (the only output of the bond pass that a frozen element's frame
+// keeps).  intact0 is the lane's entry l as the caller read it; pre: the
+// flags come from the records k_post formed, else from the bond state.
+template <int GR>
+__device__ __forceinline__ bool crack_group(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int e,
+                                            int deg, bool intact0, bool pre) {
+    const int l = threadIdx.x & (GR - 1);
+    const int gshift = (threadIdx.x & 63) & ~(GR - 1);
+    bool crack = false;
+    for (int base = 0; base < deg; base += GR) {
+        const int q = base + l;
+        bool broken = false;
+        if (q < deg) {
+            bool it = intact0;
+            if (base > 0) {
+                const int4 t4 = ldg(G(V->btab) + (size_t)e * V->W + q);
+                it = pre ? (((GAS const f4u*)G(V->bfv))[t4.x].w != 0.0f) : (G(D->s.d_bond_intact)[t4.x] != 0);
+            }
+            broken = !it;
+        }
+        const uint64_t bal = __ballot(broken);
+        crack = crack || (((uint32_t)(bal >> gshift) & ((1u << GR) - 1u)) != 0u);
+    }
+    return crack;
 }
 
 // The epilogue of element e by one thread (jax_simulation.py:140-268): ship
@@ -1527,13 +1558,24 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
             if (pre) bpre0 = bond_pre(V, D, bt0, mode);
             else bin0 = bond_in(V, D, pos, bt0, mode);
         }
-        float fix, fiy, mi;
-        if (lists)
-            list_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, rebuild, lcnt, L, g, fix, fiy, mi);
-        else contact_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, L, g, fix, fiy, mi);
-        float fbx, fby, mb;
+        // A frame sub-step leaves an element that is frozen in it where it
+        // is whatever its forces (the integration is masked by elem_active,
+        // jax_simulation.py:256-261), so a frozen element forms no contact or
+        // bond sums: only its crack flag, and on a build frame its list.
+        const bool live = mode != 0 || (sub == 0 ? act_bond : act_raw);
+        float fix = 0.0f, fiy = 0.0f, mi = 0.0f;
+        if (lists) {
+            if (live || rebuild)
+                list_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, rebuild, lcnt, live, L, g, fix,
+                                     fiy, mi);
+        } else if (live) {
+            contact_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, L, g, fix, fiy, mi);
+        }
+        float fbx = 0.0f, fby = 0.0f, mb = 0.0f;
         bool crack;
-        if (pre)
+        if (!live)
+            crack = crack_group<GR>(V, D, e, deg, pre ? bpre0.intact : bin0.intact, pre);
+        else if (pre)
             bond_pre_group<GR>(V, D, e, act_bond, deg, bt0, bpre0, mode, fbx, fby, mb, crack);
         else
             bond_force_group<GR>(V, D, pos, e, px, py, mode == 1 ? act_raw : act_bond, deg, bt0, bp0, bin0, mode, L,
