@@ -8,7 +8,10 @@ tessellated to ~20 cells each (~100k elements, ~280k bonds), thickness 0.5 or
 stiffness 1e8 N/m, friction 0.3, dt 5e-3 s.  One step = one frame (hull
 contact, pre-pass, bonds, ice-ice contact, drag, integration, fracture, ship)
 plus the scan histories.  ``--config c1`` is the 300 x 750 m, 30 %, ~8k
-element field, also reported under ``other_frame_configs``.
+element field, also reported under ``other_frame_configs``, as is the c2
+field with every element active from the start (the reference's initial
+state has them all frozen; a frozen element's forces are discarded, so the
+force pass skips them, and the all-active line shows the full load).
 
 Prints one JSON line.  ``value`` = element-steps/s with the state resident in
 HBM.  ``roofline`` is for the dominant kernel, its duration measured with HIP
@@ -68,14 +71,16 @@ def algorithmic_bytes(kernel: str, n: int, nb: int, nf: int, G: int, n_near: int
     """Bytes each kernel must move per launch, every array counted once.
 
     k_force on a frame that uses the partner lists (listed >= 0: partners
-    listed over all elements), per element: state reads pos 8, vel 8, rot 4,
-    omega 4, active 1, sub 4; statics radius, thickness, area, mass, inertia,
-    floe, bond count 28; its slot as a partner (x, y, vx, vy, r) 24; position
-    at the list build 8; list length 4; writes next pos/vel/rot/omega 24,
-    active 1, sub 4, force_mag 4, crack 1 -> 127, plus 4 per listed partner.
-    Per bond: two table entries of 16 B and the bond's force record
-    (Fx, Fy, torque, intact) 16 -> 48.  Floe flags 4 per floe; the tail's
-    trees as below.
+    listed for the active elements; a frozen element forms no contact sum),
+    per element: state reads pos 8, vel 8, rot 4, omega 4, active 1, sub 4;
+    statics radius, thickness, area, mass, inertia, floe, bond count 28;
+    position at the list build 8; list length 4; writes next
+    pos/vel/rot/omega 24, active 1, sub 4, force_mag 4, crack 1 -> 103; per
+    listed partner its index 4 and, at most once per element, its slot
+    (x, y, vx, vy, r) 24.  Per bond: two table entries of 16 B and the bond's
+    force record (Fx, Fy, torque, intact) 16 -> 48 (a frozen element reads
+    its entries for the crack flag).  Floe flags 4 per floe; the tail's trees
+    as below.
     k_force scanning the grids, per element: state reads pos 8, vel 8, rot 4, omega 4, active 1,
     sub 4; static reads radius, thickness, area, mass, inertia, floe, bond
     count 28; the element's bucket slot as a contact partner 24; writes next
@@ -94,7 +99,8 @@ def algorithmic_bytes(kernel: str, n: int, nb: int, nf: int, G: int, n_near: int
     cell, floe clears 8 per floe.
     """
     if kernel == "k_force" and listed >= 0:
-        return 127.0 * n + 4.0 * listed + 48.0 * nb + 4.0 * nf + near_blocks * 256 * 13.0 + 12.0 * nbpad
+        return (103.0 * n + 24.0 * min(n, listed) + 4.0 * listed + 48.0 * nb + 4.0 * nf
+                + near_blocks * 256 * 13.0 + 12.0 * nbpad)
     if kernel == "k_force":
         return 115.0 * n + 53.0 * nb + 4.0 * G + 4.0 * nf + near_blocks * 256 * 13.0 + 12.0 * nbpad
     if kernel == "k_near":
@@ -142,8 +148,15 @@ def cpu_baseline(case_args, frames_budget_s: float = 20.0, max_frames: int = 8, 
 
 
 def frame_bench(case_args, config: str, steps: int, warmup: int, chunk: int, profile_frames: int,
-                force_lanes: int = 0, cell_factor: float = 1.0, use_graph: bool = True, list_skin=None):
+                force_lanes: int = 0, cell_factor: float = 1.0, use_graph: bool = True, list_skin=None,
+                all_active: bool = False):
     """Time `steps` frames of one field after `warmup` frames.
+
+    The field starts at rest with every element frozen (elem_active false,
+    as the reference's initial state) and elements activate as the hull
+    reaches their floes; ``all_active`` starts with every element active
+    instead (every element then integrates and forms its contact and bond
+    sums each frame: the force pass's full load).
 
     Frames run as hipGraph replays of `chunk` frames (plus one shorter graph
     for the remainder), histories written.  Every graph size the timed loop
@@ -169,6 +182,8 @@ def frame_bench(case_args, config: str, steps: int, warmup: int, chunk: int, pro
                     force_lanes=force_lanes, cell_factor=cell_factor, list_skin=list_skin)
     dev = eng.device
     st = build_initial_ice_state(n, sc.centroids, nb, sc.parent_ids, dev)
+    if all_active:
+        st.elem_active.fill_(True)
     ship = torch.zeros(15, dtype=torch.float32, device=dev)
     ship[0], ship[1], ship[2] = x0, y0, math.pi / 2
     total = warmup + steps
@@ -209,7 +224,8 @@ def frame_bench(case_args, config: str, steps: int, warmup: int, chunk: int, pro
     _lib.check(eng.lib.icedem_sync(eng.ctx, strm), eng.ctx, "sync")
     ls1 = eng.list_stats()
     out = {"workload": f"plain DPDEM step, {config}: {n} elements, {nb} bonds, {sc.n_floes} floes, "
-                       f"concentration {sc.concentration:.3f}, dt {cfg.dt}",
+                       f"concentration {sc.concentration:.3f}, dt {cfg.dt}"
+                       + (", every element active from the start" if all_active else ""),
            "value": n * steps / elapsed, "ms_per_step": 1000.0 * elapsed / steps, "steps": steps,
            "elapsed_s": elapsed, "chunk": chunk, "graphs_primed": primed, "roofline": None,
            "partner_lists": {"skin_m": float(eng.params.list_skin), "in_use": ls1["lists"],
@@ -235,9 +251,9 @@ def frame_bench(case_args, config: str, steps: int, warmup: int, chunk: int, pro
     nbpad = 1
     while nbpad < (n + 255) // 256:
         nbpad <<= 1
-    lst = eng.list_stats()
+    lst = eng.list_stats(st.elem_active)
     abytes = algorithmic_bytes(dom, n, nb, sc.n_floes, G, n_near, eng.max_v, hull.shape[0], near_blk, nbpad,
-                               listed=lst["listed_partners"] if lst["lists"] else -1)
+                               listed=lst["listed_partners_active"] if lst["lists"] else -1)
     achieved = abytes / (avg_us[dom] * 1e-6) / 1e9
     traffic = None
     tfile = ROOT / "profiles" / f"traffic_{config}.json"
@@ -251,7 +267,8 @@ def frame_bench(case_args, config: str, steps: int, warmup: int, chunk: int, pro
                        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                        "kernel_avg_us": avg_us[dom], "algorithmic_bytes": abytes, "kernel_avg_us_all": avg_us,
                        "near_elements": n_near, "mean_listed_partners": (lst["listed_partners"] / n
-                                                                          if lst["lists"] else None)}
+                                                                          if lst["lists"] else None),
+                       "active_elements_at_profile": lst["active_elements"]}
     return out
 
 
@@ -587,7 +604,8 @@ def main():
     ap.add_argument("--profile-frames", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--no-extra-config", action="store_true", help="skip the c1 frame line next to the headline")
+    ap.add_argument("--no-extra-config", action="store_true",
+                    help="skip the c1 and all-active c2 frame lines next to the headline")
     ap.add_argument("--no-rollouts", action="store_true", help="skip the batched-rollout measurement")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate planner measurement")
     ap.add_argument("--rollouts", type=int, default=256)
@@ -649,6 +667,11 @@ def main():
         c1 = frame_bench(build_case("c1", args.seed, args.scene_cache), "c1", 2000, 100, 500, 200)
         extra_frames = {"c1": {k: c1[k] for k in ("workload", "value", "ms_per_step", "steps", "roofline",
                                                   "partner_lists")}}
+        if args.config == "c2":
+            # the same field with every element active: the force pass's full load
+            full = frame_bench(case_args, "c2", 1000, 100, 500, 200, list_skin=args.list_skin, all_active=True)
+            extra_frames["c2_all_active"] = {k: full[k] for k in ("workload", "value", "ms_per_step", "steps",
+                                                                  "roofline", "partner_lists")}
     cpu = None if args.no_cpu_baseline else cpu_baseline(case_args)
     rollouts = None if args.no_rollouts else rollout_bench(R=args.rollouts)
     surrogate = None

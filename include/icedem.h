@@ -225,9 +225,11 @@ int icedem_frames(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* 
  * h_out[1] = 1 if an element had more partners than a list holds during the
  * last call (it then scans the grids every frame), h_out[2] = 1 if the
  * context uses partner lists, h_out[3] = partners listed over all elements at
- * the last build (elements beyond a list's width not counted).  Synchronises
- * the device. */
-int icedem_list_stats(IceDemCtx* ctx, int32_t* h_out);
+ * the last build (elements beyond a list's width not counted), h_out[4] = the
+ * same over the elements with d_active[e] != 0 (d_active: n flags, e.g. a
+ * state's elem_active; NULL: all elements), h_out[5] = the number of those
+ * elements (0 without d_active).  Synchronises the device. */
+int icedem_list_stats(IceDemCtx* ctx, const uint8_t* d_active, int32_t* h_out);
 
 int icedem_frames_profiled(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* in, int32_t K,
                            const IceDemHistory* hist, float* h_kernel_ms, void* stream);

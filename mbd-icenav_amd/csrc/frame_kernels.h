@@ -36,11 +36,12 @@ __device__ __forceinline__ int32_t* force_ticket(const DevView* V) { return V->c
 __device__ __forceinline__ int32_t* coll_count(const DevView* V) { return V->counters + 1; }
 
 // ── partner lists: control words ─────────────────────────────────────────────
-// Slot k % 3 holds the largest squared displacement, since the last list
-// build, of the positions frame k starts from (NLSUB sub-slots, one per XCD,
-// float bits under atomicMax).  Frame k's force pass reads its slot, its
-// epilogues fill slot k + 1, and frame k's k_post reads slots k and k + 1
-// and clears slot k + 2 for frame k + 1's epilogues.
+// Slot k % 3 tells whether a position frame k starts from lies farther than
+// sqrt(nl_thr2) from its position at the last list build (NLSUB sub-slots,
+// one per XCD, float bits of such a squared displacement, 0 if none).  Frame
+// k's force pass reads its slot, its epilogues fill slot k + 1, and frame k's
+// k_post reads slots k and k + 1 and clears slot k + 2 for frame k + 1's
+// epilogues.
 __device__ __forceinline__ uint32_t* nl_slot(const DevView* __restrict__ V, int k) {
     return V->nlctl + (k % 3) * NLSUB;
 }
@@ -1604,12 +1605,17 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
             d2 = force_epilogue(V, D, sub, src_is_state, mode, fship_in, fout, mout, sv, et, lists, rebuild);
         }
     }
-    if (lists) {   // the wave's largest displacement into frame k + 1's slot
+    if (lists) {
+        // frame k + 1 rebuilds when a displacement passes the threshold: a
+        // wave whose largest one does stores it into its slot (any value past
+        // the threshold will do, so racing stores need no read-modify-write);
+        // the other waves store nothing
         float m = d2;
         #pragma unroll
         for (int off = 1; off < 64; off <<= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-        if ((threadIdx.x & 63) == 0 && m > 0.0f)
-            atomicMax(nl_slot(V, k + 1) + (blockIdx.x & (NLSUB - 1)), __float_as_uint(m));
+        if ((threadIdx.x & 63) == 0 && m > V->nl_thr2)
+            __hip_atomic_store(nl_slot(V, k + 1) + (blockIdx.x & (NLSUB - 1)), __float_as_uint(m), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
     }
     if (GR < 16 && again) __syncthreads();
 }

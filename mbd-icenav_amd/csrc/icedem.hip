@@ -825,20 +825,33 @@ int icedem_frames(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* 
     return ICEDEM_OK;
 }
 
-int icedem_list_stats(IceDemCtx* ctx, int32_t* h_out) {
+int icedem_list_stats(IceDemCtx* ctx, const uint8_t* d_active, int32_t* h_out) {
     if (!ctx || !h_out) { set_err(ctx, "null argument"); return ICEDEM_E_INVALID; }
-    h_out[0] = 0; h_out[1] = 0; h_out[2] = ctx->hv.nl_on; h_out[3] = 0;
-    if (!ctx->hv.nl_on) return ICEDEM_OK;
+    h_out[0] = 0; h_out[1] = 0; h_out[2] = ctx->hv.nl_on; h_out[3] = 0; h_out[4] = 0; h_out[5] = 0;
     HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<uint8_t> act;
+    if (d_active) {
+        act.resize(ctx->hv.n);
+        HIP_TRY(hipMemcpy(act.data(), d_active, act.size(), hipMemcpyDeviceToHost));
+        int64_t na = 0;
+        for (uint8_t a : act) na += a != 0;
+        h_out[5] = (int32_t)na;
+    }
+    if (!ctx->hv.nl_on) return ICEDEM_OK;
     uint32_t w[2];
     HIP_TRY(hipMemcpy(w, ctx->hv.nlctl + 3 * NLSUB, sizeof(w), hipMemcpyDeviceToHost));
     h_out[0] = (int32_t)w[1];
     h_out[1] = (int32_t)w[0];
     std::vector<int32_t> cnt(ctx->hv.n);
     HIP_TRY(hipMemcpy(cnt.data(), ctx->hv.nl_cnt, sizeof(int32_t) * cnt.size(), hipMemcpyDeviceToHost));
-    int64_t tot = 0;
-    for (int32_t c : cnt) tot += (c >= 0 && c <= NLW) ? c : 0;
+    int64_t tot = 0, tot_act = 0;
+    for (size_t e = 0; e < cnt.size(); ++e) {
+        const int64_t c = (cnt[e] >= 0 && cnt[e] <= NLW) ? cnt[e] : 0;
+        tot += c;
+        if (d_active && act[e]) tot_act += c;
+    }
     h_out[3] = (int32_t)std::min<int64_t>(tot, INT32_MAX);
+    h_out[4] = (int32_t)std::min<int64_t>(d_active ? tot_act : tot, INT32_MAX);
     return ICEDEM_OK;
 }
 

@@ -131,7 +131,7 @@ SIGNATURES = {
     "icesur_raster": (C.c_int, [_p, _p, _p, _i, _p, _i, _i, C.c_float, C.c_float, _p, _p, _p]),
     "icesur_raster_poly": (C.c_int, [_p, _p, _p, _i, _i, _p, _i, _i, C.c_float, C.c_float, _p, _p, _p]),
     "icedem_sync": (C.c_int, [_p, _p]),
-    "icedem_list_stats": (C.c_int, [_p, C.POINTER(C.c_int32)]),
+    "icedem_list_stats": (C.c_int, [_p, _p, C.POINTER(C.c_int32)]),
     "icedem_substep_forces": (C.c_int, [_p, C.POINTER(IceDemState), _p, _p, C.c_int64, C.c_int64, _p, _p, _p]),
     "icedem_contact_pairs": (C.c_int, [_p, _p, _p, _p, C.c_int64, C.POINTER(C.c_int64), _p]),
     "icedem_collisions_combined": (C.c_int, [_p, _p, _p, _p, _i, _i, _f, _f, _f, _f, _f, _p, _i, _i, _p, _p,

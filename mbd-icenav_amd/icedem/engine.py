@@ -443,14 +443,26 @@ class DemEngine:
             _lib.check(self.lib.icedem_sync(self.ctx, strm), self.ctx, "icedem_sync")
         return keep
 
-    def list_stats(self) -> dict:
+    def list_stats(self, active: Optional[torch.Tensor] = None) -> dict:
         """Partner-list diagnostics: frames that rebuilt the lists since
         creation, whether an element overflowed its list in the last call,
-        whether the context uses lists, partners listed at the last build."""
-        out = (C.c_int32 * 4)()
-        _lib.check(self.lib.icedem_list_stats(self.ctx, out), self.ctx, "icedem_list_stats")
-        return {"rebuilt_frames": int(out[0]), "overflowed": bool(out[1]), "lists": bool(out[2]),
-                "listed_partners": int(out[3])}
+        whether the context uses lists, partners listed at the last build;
+        with ``active`` (a state's elem_active) also the partners listed for
+        the active elements and their number."""
+        out = (C.c_int32 * 6)()
+        a = None
+        if active is not None:
+            a = active.to(device=self.device, dtype=torch.uint8).contiguous()
+            if a.numel() != self.n:
+                raise ValueError(f"active has {a.numel()} flags for {self.n} elements")
+        _lib.check(self.lib.icedem_list_stats(self.ctx, _ptr(a) if a is not None else None, out), self.ctx,
+                   "icedem_list_stats")
+        d = {"rebuilt_frames": int(out[0]), "overflowed": bool(out[1]), "lists": bool(out[2]),
+             "listed_partners": int(out[3])}
+        if a is not None:
+            d["listed_partners_active"] = int(out[4])
+            d["active_elements"] = int(out[5])
+        return d
 
     def alloc_history(self, K: int) -> dict:
         n, nb = self.n, self.n_bonds

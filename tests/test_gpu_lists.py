@@ -27,6 +27,13 @@ def _gpu_one_call(case, frames):
     before = eng.list_stats()
     eng.run_frames(st, ship, sp, vw, hist=hist, use_graph=True)
     after = eng.list_stats()
+    # partners listed for the active elements: a subset, all of them for an all-true mask
+    act = eng.list_stats(st.elem_active)
+    assert act["active_elements"] == int(st.elem_active.sum())
+    assert act["listed_partners"] == after["listed_partners"]
+    assert 0 <= act["listed_partners_active"] <= after["listed_partners"]
+    every = eng.list_stats(torch.ones_like(st.elem_active))
+    assert every["listed_partners_active"] == after["listed_partners"] and every["active_elements"] == case.n
     return st, ship, hist, after["rebuilt_frames"] - before["rebuilt_frames"], after
 
 
