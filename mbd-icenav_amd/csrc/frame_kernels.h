@@ -47,21 +47,21 @@ __device__ __forceinline__ uint32_t* nl_slot(const DevView* __restrict__ V, int 
 }
 __device__ __forceinline__ uint32_t* nl_ovf(const DevView* __restrict__ V) { return V->nlctl + 3 * NLSUB; }
 __device__ __forceinline__ uint32_t* nl_rebuilds(const DevView* __restrict__ V) { return V->nlctl + 3 * NLSUB + 1; }
-__device__ __forceinline__ uint32_t ld_dev_u(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 // frame k rebuilds the lists: some element may have moved far enough that a
-// partner outside its list reaches contact (every slot starts a call at +inf)
+// partner outside its list reaches contact (every slot starts a call at +inf).
+// Plain loads (two 16-byte vectors, served by the XCD's L2 after its first
+// wave): a slot is read only by kernels launched after the ones that write it.
+static_assert(NLSUB == 8, "two uint4 per slot");
 __device__ __forceinline__ bool nl_rebuild(const DevView* __restrict__ V, int k) {
-    const uint32_t* s = nl_slot(V, k);
-    uint32_t m = 0u;
-    #pragma unroll
-    for (int i = 0; i < NLSUB; ++i) m = max(m, ld_dev_u(s + i));
+    const uint4* s = (const uint4*)nl_slot(V, k);
+    const uint4 a = s[0], b = s[1];
+    const uint32_t m = max(max(max(a.x, a.y), max(a.z, a.w)), max(max(b.x, b.y), max(b.z, b.w)));
     return __uint_as_float(m) > V->nl_thr2;
 }
 // some element had more than NLW candidates at a build of this call: it
-// scans the grids every frame, so the buckets are filled every frame
-__device__ __forceinline__ bool nl_overflowed(const DevView* __restrict__ V) { return ld_dev_u(nl_ovf(V)) != 0u; }
+// scans the grids every frame, so the buckets are filled every frame (read in
+// k_post, after the force pass that may set it)
+__device__ __forceinline__ bool nl_overflowed(const DevView* __restrict__ V) { return *nl_ovf(V) != 0u; }
 
 // extra (tail) workgroups of a frame's first k_force: one wave per 256-element block
 __host__ __device__ constexpr int tail_groups(int nblk) { return (nblk + BLOCK / 64 - 1) / (BLOCK / 64); }
