@@ -258,13 +258,40 @@ def cost_of_targets(t) -> np.ndarray:
 COST_FLOOR = 1.0   # cost units; relative errors use max(|cost|, COST_FLOOR)
 
 
+def jitter_offsets(jitters: int, radius: float, phase: float = 0.0) -> np.ndarray:
+    """(J, 2) start offsets (x, y) in metres: J points on a circle of
+    `radius` from angle `phase`, or one zero offset for J <= 1.  A rollout's
+    jitter-mean cost is the mean of its costs from these starts."""
+    if jitters <= 1:
+        return np.zeros((1, 2))
+    a = phase + 2.0 * math.pi * np.arange(jitters) / jitters
+    return radius * np.stack([np.cos(a), np.sin(a)], 1)
+
+
+def run_jittered(rb: RolloutBatch, starts: torch.Tensor, ctl, hold: int, frames: int,
+                 offsets: np.ndarray) -> np.ndarray:
+    """Mean rollout outputs (R, 4) of `ctl` over the start offsets."""
+    acc = None
+    for off in offsets:
+        s = starts.clone()
+        s[:, 0] += float(off[0])
+        s[:, 1] += float(off[1])
+        rb.ship_init.copy_(s)
+        rb.reset()
+        o = rb.run(ctl, hold, frames).cpu().numpy().astype(np.float64)
+        acc = o if acc is None else acc + o
+    return acc / len(offsets)
+
+
 def make_dataset(rb: RolloutBatch, rounds: int, frames: int = 400, hold: int = 10, n_knots: int = 40,
                  seed: int = 0, spread: Optional[StartSpread] = None, raw_controls: bool = False,
-                 progress=None) -> Dict[str, torch.Tensor]:
+                 progress=None, jitters: int = 1, jitter_m: float = 0.05) -> Dict[str, torch.Tensor]:
     """`rounds` x R DEM rollouts (one per patch instance per round).  The
     extra inputs are the ship state (6) and the controls seen through the
     ice-free ship model (N_PHYS); ``raw_controls`` appends the knots
-    themselves (2 n_knots)."""
+    themselves (2 n_knots).  With ``jitters`` > 1 each target is the mean
+    over that many starts on a circle of ``jitter_m`` metres around the
+    sampled one (jitter_offsets): the jitter-mean cost."""
     spread = spread or StartSpread()
     rng = np.random.default_rng(seed)
     dev = rb.device
@@ -281,9 +308,7 @@ def make_dataset(rb: RolloutBatch, rounds: int, frames: int = 400, hold: int = 1
     for k in range(rounds):
         starts = random_starts(rb, nominal, rng, spread)
         ctl = random_controls(R, n_knots, rng)
-        rb.ship_init.copy_(starts)
-        rb.reset()
-        out = rb.run(torch.as_tensor(ctl), hold, frames).cpu().numpy()
+        out = run_jittered(rb, starts, torch.as_tensor(ctl), hold, frames, jitter_offsets(jitters, jitter_m))
         sl = slice(k * R, (k + 1) * R)
         for r in range(R):
             ras[k * R + r] = pr.of(r, starts[r:r + 1, :3])[0]
