@@ -480,7 +480,7 @@ __device__ __forceinline__ float wave_tree_global(GAS const float* __restrict__ 
 // error, and only which elements run the exact hull test depends on it.
 __device__ __forceinline__ void cull_chunk(const DevView* __restrict__ V, int chunk, bool valid, int e,
                                            float x, float y, float vx, float vy, GAS const float* ship, int set,
-                                           bool insert = true) {
+                                           bool insert = true, bool slot = true) {
     const IceDemParams& P = V->p;
     bool near = false;
     float r = 0.0f;
@@ -512,7 +512,7 @@ __device__ __forceinline__ void cull_chunk(const DevView* __restrict__ V, int ch
         if (near) G(V->near_list)[qbase + __builtin_amdgcn_mbcnt_hi((uint32_t)(nb >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)nb, 0u))] = e;
         if (insert) cell_store(V, set, rix, idx, e, x, y, vx, vy, r);
-        if (V->nl_on) {   // the element's slot for the partner-list pass
+        if (V->nl_on && slot) {   // the element's slot for the partner-list pass
             GAS float* q = G(V->es) + 6 * (size_t)e;
             f4u a;
             a.x = x; a.y = y; a.z = vx; a.w = vy;
@@ -1367,6 +1367,24 @@ __device__ __forceinline__ bool crack_group(const DevView* __restrict__ V, const
     return crack;
 }
 
+// The epilogue of an element that stays frozen in frame k > 0 of a call with
+// one ice sub-step (mode 0).  Its next state, force magnitude (0: a hull
+// contact would have activated it), activity and list-build position are
+// what frame 0 wrote, so only its submergence (the no-contact branch of
+// force_epilogue's update, which needs no thickness) and its crack flag are
+// formed.  Returns the squared list displacement, 0.
+__device__ __forceinline__ bool frozen_fast(const IceDemParams& P, int k, int sub, int mode) {
+    return mode == 0 && sub == 0 && k > 0 && P.n_ice_steps == 1;
+}
+__device__ __forceinline__ float frozen_epilogue(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int e,
+                                                 bool crack) {
+    const IceDemParams& P = V->p;
+    const float sub0 = G(D->s.d_sub)[e];
+    G(D->s.d_sub)[e] = vmax(0.0f, sub0 - P.inv_rise_time * sub0 * P.dt);
+    G(V->crack)[e] = crack ? 1 : 0;
+    return 0.0f;
+}
+
 // The epilogue of element e by one thread (jax_simulation.py:140-268): ship
 // forces and submergence (first sub-step), drag, rotational drag, totals and
 // the integration into the next-state buffers, from the values its group
@@ -1589,7 +1607,10 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
             sv[S_FLAGS] = __int_as_float((act_raw ? 1 : 0) | (act_bond ? 2 : 0) | (crack ? 4 : 0));
             // 16 lanes (small fields, latency-bound): the group's lane 0
             // finishes its own element, no barrier
-            if (GR >= 16) d2 = force_epilogue(V, D, sub, src_is_state, mode, fship_in, fout, mout, sv, e, lists, rebuild);
+            if (GR >= 16)
+                d2 = (!live && frozen_fast(V->p, k, sub, mode))
+                         ? frozen_epilogue(V, D, e, crack)
+                         : force_epilogue(V, D, sub, src_is_state, mode, fship_in, fout, mout, sv, e, lists, rebuild);
         }
     }
     if (GR < 16) {
@@ -1601,8 +1622,13 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
             // the epilogue-only inputs, loaded here (coalesced, one thread per
             // element) rather than held in registers through the passes
             float* sv = L.stg(threadIdx.x);
-            epi_put(sv, epi_in(V, D, k, sub, src_is_state, mode, et));
-            d2 = force_epilogue(V, D, sub, src_is_state, mode, fship_in, fout, mout, sv, et, lists, rebuild);
+            const int fl = __float_as_int(sv[S_FLAGS]);
+            if ((fl & 2) == 0 && frozen_fast(V->p, k, sub, mode)) {
+                d2 = frozen_epilogue(V, D, et, (fl & 4) != 0);
+            } else {
+                epi_put(sv, epi_in(V, D, k, sub, src_is_state, mode, et));
+                d2 = force_epilogue(V, D, sub, src_is_state, mode, fship_in, fout, mout, sv, et, lists, rebuild);
+            }
         }
     }
     if (lists) {
@@ -1686,12 +1712,28 @@ __device__ __forceinline__ void post_bond(const DevView* __restrict__ V, const C
     if (t < nb) {
         const int4 bp = ldg(G(V->bpair) + t);
         const int i = bp.x, j = bp.y;
+        GAS const uint8_t* act = G(D->s.d_active);
+        if (act[i] == 0 && act[j] == 0) {
+            // Both ends frozen after the frame's activation: neither moved,
+            // neither has a hull contact (a positive force magnitude needs
+            // one, and a contact activates the element's floe), so every
+            // gate below is closed and the bond's state comes out as it went
+            // in (jax_simulation.py:563-640 select the old values; the shear
+            // sum adds +0, which turns only a -0 into +0).  Activity never
+            // reverts, so both ends have been frozen since the call's first
+            // frame, whose k_post formed the force record from the same
+            // positions and state: only that frame writes it.
+            const bool intact = G(D->s.d_bond_intact)[t] != 0;
+            if (D->h.d_bond_intact) G(D->h.d_bond_intact)[(size_t)k * nb + t] = intact ? 1 : 0;
+            GAS float* dsp = G(D->s.d_bond_ds) + t;
+            if (__float_as_uint(*dsp) == 0x80000000u) *dsp = 0.0f;
+            if (!(prebond && k == 0)) return;
+        }
         float dx = pos[j] - pos[i];
         float dy = pos[n + j] - pos[n + i];
         float dist = sqrtf(dx * dx + dy * dy);
         float ds_ = vmax(dist, 1e-10f);
         float dirx = dx / ds_, diry = dy / ds_;
-        GAS const uint8_t* act = G(D->s.d_active);
         GAS const uint8_t* elig = G(D->s.d_floe_elig);
         bool both = act[i] && act[j];
         bool floe_ok = elig[bp.z] && elig[bp.w & 0x3fffffff];
@@ -1771,10 +1813,15 @@ __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const C
     GAS const float* pos = G(final_in_nxt ? V->nxt_pos : D->s.d_pos);
     GAS const float* vel = G(final_in_nxt ? V->nxt_vel : D->s.d_vel);
     float px = 0.0f, py = 0.0f, pvx = 0.0f, pvy = 0.0f;
+    // a frozen element's next state is a copy of its state (the integration
+    // keeps it), and its partner-list slot is the one the call's first cull
+    // wrote: neither is written again
+    bool frozen = false;
     if (t < n) {
         px = pos[t]; py = pos[n + t];
         pvx = vel[t]; pvy = vel[n + t];
-        if (final_in_nxt) {
+        frozen = G(D->s.d_active)[t] == 0;
+        if (final_in_nxt && !frozen) {
             G(D->s.d_pos)[t] = px; G(D->s.d_pos)[n + t] = py;
             G(D->s.d_vel)[t] = pvx; G(D->s.d_vel)[n + t] = pvy;
             G(D->s.d_rot)[t] = G(V->nxt_rot)[t]; G(D->s.d_omega)[t] = G(V->nxt_omega)[t];
@@ -1807,7 +1854,7 @@ __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const C
     if (t < V->nf) { G(V->floe_coll)[t] = 0u; G(V->floe_maxfm)[t] = 0u; }
     if (t == 0) *force_ticket(V) = 0;
     if (cull && chunk < V->nblk)
-        cull_chunk(V, chunk, t < n, t, px, py, pvx, pvy, G(D->s.d_ship), set ^ 1, ins);
+        cull_chunk(V, chunk, t < n, t, px, py, pvx, pvy, G(D->s.d_ship), set ^ 1, ins, !frozen);
 }
 
 // Both parts for thread t (the one-workgroup scene kernel).
