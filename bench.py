@@ -404,24 +404,63 @@ def dem_sensitivity(rb, shifts=(0.01, 0.1), seed: int = 5):
     return out
 
 
-def surrogate_bench(plain_rps: float, rounds: int = 800, epochs: int = 20, test_patches: int = 4,
-                    samples: int = 16384, k: int = 64, exhaustive: bool = True):
-    """Surrogate + batched-DEM planner: dataset from the rollout patches, GPU
-    training, then planner iterations on held-out patches, each checked
-    against the batched DEM scoring of all its candidates."""
+def jitter_floor(rb, jitters: int, jitter_m: float, seed: int = 7):
+    """How far the jitter-mean rollout cost itself moves: random starts and
+    controls on the batch's patches, the single-start cost and two
+    jitter-mean costs over rings of start offsets rotated against each
+    other; relative changes as the surrogate's errors are measured."""
+    import numpy as np
+    import torch
+    from icedem import planner as pl, rollouts as ro
+    rng = np.random.default_rng(seed)
+    nominal = rb.ship_init.clone()
+    st = pl.random_starts(rb, nominal, rng, pl.StartSpread())
+    ctl = torch.as_tensor(pl.random_controls(rb.R, 40, rng))
+    one = ro.rollout_cost(pl.run_jittered(rb, st, ctl, 10, 400, np.zeros((1, 2))))
+    a = ro.rollout_cost(pl.run_jittered(rb, st, ctl, 10, 400, pl.jitter_offsets(jitters, jitter_m)))
+    b = ro.rollout_cost(pl.run_jittered(rb, st, ctl, 10, 400,
+                                        pl.jitter_offsets(jitters, jitter_m, math.pi / max(jitters, 1))))
+    rb.ship_init.copy_(nominal)
+
+    def rel(x, y):
+        r = np.abs(x - y) / np.maximum(np.abs(y), pl.COST_FLOOR)
+        return {"median_rel_change": float(np.median(r)), "frac_over_5pct": float(np.mean(r > 0.05))}
+    return {"single_vs_jitter_mean": rel(one, a), "jitter_mean_vs_rotated_ring": rel(b, a)}
+
+
+def surrogate_bench(plain_rps: float, rounds: int = 800, epochs: int = 20, test_patches: int = 8,
+                    samples: int = 16384, k: int = 64, exhaustive: bool = True, patch_sets: int = 8,
+                    jitters: int = 4, jitter_m: float = 0.05, eval_samples: int = 4096):
+    """Surrogate + batched-DEM planner.  Dataset: `patch_sets` sets of 256
+    rollout patches, rounds / patch_sets rounds each, every target the
+    jitter-mean cost over `jitters` starts on a `jitter_m` ring; GPU
+    training; then planner iterations on held-out patches (ranked by the
+    surrogate, and by the calibrated costmap for comparison), and on
+    `eval_samples` candidates of each held-out patch the single-start and
+    jitter-mean DEM costs of every candidate against both rankers."""
     import numpy as np
     import torch
     from icedem import planner as pl, rollouts as ro, scenario
     from icedem.surrogate import Arch, Surrogate
+    import multiprocessing as mp
     t0 = time.perf_counter()
-    scenes = ro.rollout_patches(256, workers=min(16, os.cpu_count() or 1))
-    rb = ro.RolloutBatch(scenes, list(range(256)))
-    rb.set_mode("frames")
-    data = pl.make_dataset(rb, rounds, seed=1)
+    offs = pl.jitter_offsets(jitters, jitter_m)
+    parts, floor, jfloor = [], None, None
+    per = max(1, rounds // patch_sets)
+    with mp.get_context("spawn").Pool(min(16, os.cpu_count() or 1)) as pool:
+        for ps in range(patch_sets):
+            scenes = pool.map(ro._patch, [ro.patch_params(r + 256 * ps) for r in range(256)], chunksize=4)
+            rb = ro.RolloutBatch(scenes, list(range(256)))
+            rb.set_mode("frames")
+            parts.append(pl.make_dataset(rb, per, seed=1 + ps, jitters=jitters, jitter_m=jitter_m))
+            if ps == 0:
+                floor = dem_sensitivity(rb)
+                jfloor = jitter_floor(rb, jitters, jitter_m)
+            rb.close()
+    data = {key: torch.cat([d[key] for d in parts]) for key in parts[0]}
+    del parts
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
-    floor = dem_sensitivity(rb)
-    rb.close()
     N = int(data["raster"].shape[0])
     cm_w, cm_err = pl.costmap_baseline(data)
     arch = Arch()
@@ -444,6 +483,10 @@ def surrogate_bench(plain_rps: float, rounds: int = 800, epochs: int = 20, test_
     t_fwd = (time.perf_counter() - t0) / reps
     fwd_tflops = B * arch.flops_per_sample() / t_fwd / 1e12
     del data, ras, ext
+    torch.cuda.empty_cache()
+
+    def rel(p, t):
+        return float(np.median(np.abs(p - t) / np.maximum(np.abs(t), pl.COST_FLOOR)))
     # planner on held-out patches
     rows = []
     for i in range(test_patches):
@@ -455,8 +498,10 @@ def surrogate_bench(plain_rps: float, rounds: int = 800, epochs: int = 20, test_
         nominal = np.tile(np.array([0.7, 0.0], np.float32), (40, 1))
         P.iterate(nominal, S=samples, seed=100)           # graph capture and warm-up
         res = P.iterate(nominal, S=samples, seed=i)
+        res_cm = P.iterate(nominal, S=samples, seed=i, ranker="costmap")
         base = P.random_baseline(nominal, S=samples, seed=i)
-        row = {"patch_seed": 5000 + i, "best_cost": res["best_cost"], "random64_best_cost": base["best_cost"],
+        row = {"patch_seed": 5000 + i, "best_cost": res["best_cost"], "costmap_ranked_best_cost": res_cm["best_cost"],
+               "random64_best_cost": base["best_cost"],
                "judged": res["judged"], "median_rel_err_judged": res["median_rel_err_judged"],
                "rollouts_per_s": res["rollouts_per_s"], "surrogate_ms": 1e3 * res["surrogate_s"],
                "judge_ms": 1e3 * res["judge_s"],
@@ -464,31 +509,32 @@ def surrogate_bench(plain_rps: float, rounds: int = 800, epochs: int = 20, test_
                "surrogate_candidates_per_s": samples / res["surrogate_s"]}
         if exhaustive:
             rng = np.random.default_rng(i)
-            cand = np.clip(nominal[None] + rng.normal(0.0, 0.2, (samples, 40, 2)), -1.0, 1.0).astype(np.float32)
+            cand = np.clip(nominal[None] + rng.normal(0.0, 0.2, (eval_samples, 40, 2)), -1.0, 1.0).astype(np.float32)
             cand[0] = nominal
             t0 = time.perf_counter()
             dem = P.exhaustive(cand)
             t_ex = time.perf_counter() - t0
+            demj = P.exhaustive(cand, offsets=offs)
             dc = torch.as_tensor(cand, device="cuda")
             pred = P.surrogate_costs(dc).cpu().numpy().astype(np.float64)
-            rel = np.abs(pred - dem) / np.maximum(np.abs(dem), pl.COST_FLOOR)
             cmc = P.costmap_costs(dc).cpu().numpy()
-            rel_cm = np.abs(cmc - dem) / np.maximum(np.abs(dem), pl.COST_FLOOR)
-            b = float(dem.min())
-            row.update({"dem_best_of_all": b,
-                        "regret_vs_dem_best": (res["best_cost"] - b) / max(abs(b), pl.COST_FLOOR),
-                        "pick_rank_fraction": float(np.mean(dem < res["best_cost"])),
-                        "surrogate_rel_err_median_all": float(np.median(rel)),
-                        "spearman_surrogate_vs_dem": _spearman(pred, dem),
-                        "costmap_rel_err_median_all": float(np.median(rel_cm)),
-                        "spearman_costmap_vs_dem": _spearman(cmc, dem),
-                        "dem_mean_of_top64": {"surrogate": float(dem[np.argsort(pred)[:k]].mean()),
-                                              "costmap": float(dem[np.argsort(cmc)[:k]].mean()),
-                                              "all": float(dem.mean())},
-                        "best_of_top64": {"surrogate": float(dem[np.argsort(pred)[:k]].min()),
-                                          "costmap": float(dem[np.argsort(cmc)[:k]].min())},
+            ts, tc = np.argsort(pred)[:k], np.argsort(cmc)[:k]
+            row.update({"eval_candidates": eval_samples,
+                        "surrogate_rel_err": {"vs_single": rel(pred, dem), "vs_jitter_mean": rel(pred, demj)},
+                        "costmap_rel_err": {"vs_single": rel(cmc, dem), "vs_jitter_mean": rel(cmc, demj)},
+                        "single_vs_jitter_mean_rel": rel(dem, demj),
+                        "spearman": {"surrogate_vs_single": _spearman(pred, dem),
+                                     "surrogate_vs_jitter_mean": _spearman(pred, demj),
+                                     "costmap_vs_single": _spearman(cmc, dem),
+                                     "costmap_vs_jitter_mean": _spearman(cmc, demj),
+                                     "single_vs_jitter_mean": _spearman(dem, demj)},
+                        "top64_mean_cost": {"single": {"surrogate": float(dem[ts].mean()),
+                                                       "costmap": float(dem[tc].mean()), "all": float(dem.mean())},
+                                            "jitter_mean": {"surrogate": float(demj[ts].mean()),
+                                                            "costmap": float(demj[tc].mean()),
+                                                            "all": float(demj.mean())}},
                         "dem_cost_sd": float(dem.std()),
-                        "exhaustive_batched_rollouts_per_s": samples / t_ex})
+                        "exhaustive_batched_rollouts_per_s": eval_samples / t_ex})
         rows.append(row)
         P.close()
     rps = float(np.mean([r["rollouts_per_s"] for r in rows]))
@@ -496,10 +542,12 @@ def surrogate_bench(plain_rps: float, rounds: int = 800, epochs: int = 20, test_
     cand_rps = float(np.mean([r["surrogate_candidates_per_s"] for r in rows]))
     val = hist[-1]["val_rel_cost_err"]
     out = {
-        "workload": f"{N} training rollouts ({rounds} rounds x 256 patches, random starts and controls), "
-                    f"{epochs} epochs; planner: {samples} candidates scored by the surrogate, top {k} judged by "
-                    f"the batched DEM (+{k} more per round while the judged median error > 5 %, up to 256; after an "
-                    f"iteration that needed that fallback, all 256 in one launch)",
+        "workload": f"{N} training rollouts ({patch_sets} sets of 256 patches x {per} rounds, random starts and "
+                    f"controls), targets the mean cost over {jitters} starts on a {100 * jitter_m:g} cm ring, "
+                    f"{epochs} epochs; planner: {samples} candidates ranked by the surrogate (or by the calibrated "
+                    f"costmap), top {k} judged by the batched DEM (+{k} more per round while the judged median error "
+                    f"> 5 %, up to 256; after an iteration that needed that fallback, all 256 in one launch); "
+                    f"{test_patches} held-out patches, {eval_samples} candidates each scored exhaustively",
         "throughput_by_error": {
             "dem_judged": {"rollouts_per_s": dem_rps, "vs_plain": dem_rps / plain_rps,
                            "rel_cost_error": 0.0,
@@ -517,8 +565,11 @@ def surrogate_bench(plain_rps: float, rounds: int = 800, epochs: int = 20, test_
         "costmap_weights": [float(x) for x in cm_w],
         "surrogate_err_target_5pct_met": bool(val["median"] <= 0.05),
         "dem_sensitivity_floor": floor,
+        "jitter_mean_floor": jfloor,
         "judged_median_rel_err": float(np.median([r["median_rel_err_judged"] for r in rows])),
         "planner_gain_vs_random64": float(np.mean([r["random64_best_cost"] - r["best_cost"] for r in rows])),
+        "planner_best_cost_surrogate_minus_costmap_ranked": float(
+            np.mean([r["best_cost"] - r["costmap_ranked_best_cost"] for r in rows])),
         "patches": rows,
         "dataset_s": t_data, "train_s": t_train, "train_steps": steps,
         "train_tflops": steps * 512 * 3 * arch.flops_per_sample() / t_train / 1e12,
@@ -527,9 +578,22 @@ def surrogate_bench(plain_rps: float, rounds: int = 800, epochs: int = 20, test_
                           "frac": fwd_tflops / 2500.0, "dtype": "bf16 x bf16 -> fp32"},
     }
     if exhaustive:
-        out["planner_regret_vs_dem_best_median"] = float(np.median([r["regret_vs_dem_best"] for r in rows]))
-        out["spearman_surrogate_vs_dem_median"] = float(np.median([r["spearman_surrogate_vs_dem"] for r in rows]))
-        out["spearman_costmap_vs_dem_median"] = float(np.median([r["spearman_costmap_vs_dem"] for r in rows]))
+        def med(f):
+            return float(np.median([f(r) for r in rows]))
+        out["held_out"] = {
+            "surrogate_rel_err_vs_single": med(lambda r: r["surrogate_rel_err"]["vs_single"]),
+            "surrogate_rel_err_vs_jitter_mean": med(lambda r: r["surrogate_rel_err"]["vs_jitter_mean"]),
+            "costmap_rel_err_vs_single": med(lambda r: r["costmap_rel_err"]["vs_single"]),
+            "costmap_rel_err_vs_jitter_mean": med(lambda r: r["costmap_rel_err"]["vs_jitter_mean"]),
+            "single_vs_jitter_mean_rel": med(lambda r: r["single_vs_jitter_mean_rel"]),
+            "spearman_median": {key: med(lambda r, key=key: r["spearman"][key]) for key in rows[0]["spearman"]},
+            "top64_surrogate_minus_costmap": {
+                c: {"mean": float(np.mean([r["top64_mean_cost"][c]["surrogate"] - r["top64_mean_cost"][c]["costmap"]
+                                           for r in rows])),
+                    "surrogate_better": int(sum(r["top64_mean_cost"][c]["surrogate"] < r["top64_mean_cost"][c]["costmap"]
+                                                for r in rows)), "patches": len(rows)}
+                for c in ("single", "jitter_mean")},
+        }
     return out
 
 

@@ -274,8 +274,10 @@ def run_jittered(rb: RolloutBatch, starts: torch.Tensor, ctl, hold: int, frames:
     acc = None
     for off in offsets:
         s = starts.clone()
-        s[:, 0] += float(off[0])
-        s[:, 1] += float(off[1])
+        if off[0] != 0.0:
+            s[:, 0] += float(off[0])
+        if off[1] != 0.0:
+            s[:, 1] += float(off[1])
         rb.ship_init.copy_(s)
         rb.reset()
         o = rb.run(ctl, hold, frames).cpu().numpy().astype(np.float64)
@@ -553,10 +555,14 @@ class Planner:
         return rollout_cost(out[:m])
 
     def iterate(self, nominal: np.ndarray, S: int = 16384, sigma: float = 0.2, seed: int = 0,
-                keep: bool = False) -> Dict:
+                keep: bool = False, ranker: str = "surrogate") -> Dict:
         """One sampling iteration around `nominal` (n_knots, 2).  With `keep`
         the result also holds the candidates (S, n_knots, 2) and their costs
-        (S,): the surrogate's, replaced by the DEM's where judged."""
+        (S,): the ranker's, replaced by the DEM's where judged.  ``ranker``
+        "costmap" ranks the candidates by the calibrated costmap cost instead
+        of the surrogate (the same DEM judge and fallback)."""
+        if ranker not in ("surrogate", "costmap"):
+            raise ValueError(f"ranker must be 'surrogate' or 'costmap', not {ranker!r}")
         rng = np.random.default_rng(seed)
         dev = self.judge.device
         cand = np.clip(nominal[None] + rng.normal(0.0, sigma, (S, self.nk, 2)), -1.0, 1.0).astype(np.float32)
@@ -564,7 +570,7 @@ class Planner:
         ctl = torch.as_tensor(cand, device=dev)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        pred = self.surrogate_costs(ctl)
+        pred = self.surrogate_costs(ctl) if ranker == "surrogate" else self.costmap_costs(ctl).float()
         order = torch.argsort(pred).cpu().numpy()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -668,20 +674,23 @@ class Planner:
         b = int(np.argmin(c))
         return {"best_cost": float(c[b]), "best_index": int(pick[b])}
 
-    def exhaustive(self, cand: np.ndarray, batch: int = 256) -> np.ndarray:
+    def exhaustive(self, cand: np.ndarray, batch: int = 256, offsets: Optional[np.ndarray] = None) -> np.ndarray:
         """DEM costs of every candidate (S, n_knots, 2), ``batch`` instances
-        of the patch per launch: the plain batched-DEM planner's scores."""
+        of the patch per launch: the plain batched-DEM planner's scores.
+        With ``offsets`` (jitter_offsets) the jitter-mean costs instead."""
         ex = self._batch(batch)
         S = cand.shape[0]
         out = np.empty(S, np.float64)
         ctl = torch.as_tensor(cand, device=self.judge.device)
+        starts = self.start.expand(batch, 15)
+        offs = np.zeros((1, 2)) if offsets is None else np.asarray(offsets, np.float64)
         for s0 in range(0, S, batch):
             m = min(batch, S - s0)
             full = torch.empty((batch, self.nk, 2), dtype=torch.float32, device=ctl.device)
             full[:m] = ctl[s0:s0 + m]
             full[m:] = ctl[s0:s0 + 1]
-            ex.reset()
-            out[s0:s0 + m] = rollout_cost(ex.run(full, self.hold, self.frames).cpu().numpy()[:m])
+            out[s0:s0 + m] = rollout_cost(run_jittered(ex, starts, full, self.hold, self.frames, offs)[:m])
+        ex.ship_init.copy_(starts)
         return out
 
     def close(self):

@@ -167,6 +167,53 @@ def test_planner_reports_dem_costs_and_widens_its_judge():
     P.close()
 
 
+def test_jitter_mean_is_the_mean_of_shifted_runs():
+    """run_jittered: zero offsets give the plain batched run, a ring gives the
+    float64 mean of the runs from the shifted starts, bit for bit."""
+    scenes = ro.rollout_patches(4, workers=1)
+    rb = ro.RolloutBatch(scenes, list(range(4)))
+    rb.set_mode("frames")
+    ctl = torch.as_tensor(ro.nominal_controls(4, 40, seed=3))
+    st = rb.ship_init.clone()
+    rb.reset()
+    plain = rb.run(ctl, 10, 400).cpu().numpy().astype(np.float64)
+    assert np.any(plain[:, 0] > 0), "the ship must reach the ice"
+    np.testing.assert_array_equal(pl.run_jittered(rb, st, ctl, 10, 400, np.zeros((1, 2))), plain)
+    offs = pl.jitter_offsets(3, 0.05)
+    acc = None
+    for o in offs:
+        s2 = st.clone()
+        s2[:, 0] += float(o[0])
+        s2[:, 1] += float(o[1])
+        rb.ship_init.copy_(s2)
+        rb.reset()
+        r = rb.run(ctl, 10, 400).cpu().numpy().astype(np.float64)
+        acc = r if acc is None else acc + r
+    np.testing.assert_array_equal(pl.run_jittered(rb, st, ctl, 10, 400, offs), acc / 3)
+    assert not np.array_equal(acc / 3, plain)
+    rb.close()
+
+
+def test_costmap_ranked_iteration_reports_dem_costs():
+    """The planner ranked by the calibrated costmap instead of the network:
+    the same DEM judge, so the reported best cost is the DEM cost of the
+    reported controls; an unknown ranker is refused."""
+    sc = scenario.scene_patch(5001, 200, 0.4)
+    tmp = ro.RolloutBatch([sc], [0])
+    start = tmp.ship_init[0].clone()
+    tmp.close()
+    net = Surrogate(Arch(), seed=0)
+    norm = pl.Norm(np.zeros(3, np.float32), np.ones(3, np.float32))
+    P = pl.Planner(net, norm, sc, start, k=64, costmap_weights=[1.0, 0.5, 2.0])
+    nominal = np.tile(np.array([0.7, 0.0], np.float32), (40, 1))
+    r = P.iterate(nominal, S=2048, seed=4, ranker="costmap")
+    again = P.dem_costs(torch.as_tensor(r["best_controls"][None], device="cuda"), P.width)
+    assert again[0] == r["best_cost"]
+    with pytest.raises(ValueError):
+        P.iterate(nominal, S=256, ranker="random")
+    P.close()
+
+
 def test_mppi_update_matches_the_weighted_mean():
     """run_mppi's update (jax_mppi_planner.py:95-101) against float64 NumPy."""
     rng = np.random.default_rng(4)

@@ -21,3 +21,18 @@ def test_mbd_schedule_closed_forms():
     assert np.all(np.diff(sig) > 0) and np.all(np.diff(ab) < 0)
     # the reverse step's identity alphas_bar[i] = alphas[i] alphas_bar[i - 1]
     np.testing.assert_allclose(ab[1:], alphas[1:] * ab[:-1], rtol=1e-6)
+
+
+def test_jitter_offsets_ring():
+    """J start offsets on a circle of the given radius, evenly spaced from the
+    phase; one zero offset for J <= 1."""
+    np.testing.assert_array_equal(pl.jitter_offsets(1, 0.05), np.zeros((1, 2)))
+    np.testing.assert_array_equal(pl.jitter_offsets(0, 0.05), np.zeros((1, 2)))
+    o = pl.jitter_offsets(4, 0.05)
+    assert o.shape == (4, 2)
+    np.testing.assert_allclose(np.hypot(o[:, 0], o[:, 1]), 0.05, rtol=1e-12)
+    np.testing.assert_allclose(o.sum(axis=0), 0.0, atol=1e-15)
+    np.testing.assert_allclose(o[0], [0.05, 0.0], atol=1e-15)
+    np.testing.assert_allclose(o[1], [0.0, 0.05], atol=1e-15)
+    r = pl.jitter_offsets(4, 0.05, np.pi / 4)
+    np.testing.assert_allclose(r[0], [0.05 / np.sqrt(2), 0.05 / np.sqrt(2)], rtol=1e-12)
