@@ -1646,6 +1646,118 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
     if (GR < 16 && again) __syncthreads();
 }
 
+// The force pass of a frame after a call's first, one ice sub-step
+// (frozen_fast), for elements [chunk*E, chunk*E+E).  An element frozen
+// through the frame (elem_active false and no hull contact on its floe this
+// frame) keeps its state whatever its forces (jax_simulation.py:256-261), so
+// it reads only its activity, floe and submergence: no contact or bond sums,
+// no next state (k_post keeps a frozen element's state), no force magnitude
+// (0 since the call's first frame: a contact would have activated it), and no
+// crack flag (k_post reads it only for bonds with both ends active).  A live
+// element reads what force_body reads, one round trip later; on a list-build
+// frame a frozen element also builds its list.
+template <int GR>
+__device__ __forceinline__ void force_body_lazy(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int k,
+                                                int set, ForceLds<GR>& L, int chunk) {
+    const int n = V->n;
+    const int g = threadIdx.x / GR, l = threadIdx.x & (GR - 1);
+    const int e = chunk * epb<GR>() + g;
+    const bool lists = V->nl_on != 0;
+    const bool pre = V->bf_on != 0;
+    const bool rebuild = lists && nl_rebuild(V, k);
+    float d2 = 0.0f;
+    if (e < n) {                              // group-uniform
+        GAS const float* pos = G(D->s.d_pos);
+        GAS const float* vel = G(D->s.d_vel);
+        const bool act_raw = G(D->s.d_active)[e] != 0;
+        const int fl = G(V->floe)[e];
+        const float sub0 = G(D->s.d_sub)[e];
+        float px = 0.0f, py = 0.0f, vx = 0.0f, vy = 0.0f, r = 0.0f;
+        int deg = 0, novf = 0, lcnt = 0;
+        int4 bt0 = make_int4(0, 0, 0, 0);
+        float2 bp0 = make_float2(0.0f, 0.0f);
+        EpiIn ei;
+        auto load_elem = [&]() {
+            px = pos[e]; py = pos[n + e]; vx = vel[e]; vy = vel[n + e];
+            // 16 lanes: the epilogue-only inputs too, in the same round trip
+            if (GR >= 16) ei = epi_in(V, D, k, 0, 1, 0, e);
+            r = G(V->radius)[e];
+            deg = V->W > 0 ? G(V->bdeg)[e] : 0;
+            novf = G(V->ovf_count)[set];
+            lcnt = (lists && !rebuild) ? G(V->nl_cnt)[e] : 0;
+            if (l < V->W) {
+                bt0 = ldg(G(V->btab) + (size_t)e * V->W + l);
+                if (!pre) bp0 = ldg(G(V->bprm) + (size_t)e * V->W + l);
+            }
+        };
+        // an active element's state is read beside nothing else; a frozen
+        // one first learns whether its floe was hit this frame
+        if (act_raw) load_elem();
+        const bool fcoll = !act_raw && G(V->floe_coll)[fl] != 0u;
+        const bool live = act_raw || fcoll;
+        if (!act_raw && (live || rebuild)) load_elem();
+        BondIn bin0 = {0.0f, 0.0f, 0.0f, false, false};
+        BondPre bpre0 = {0.0f, 0.0f, 0.0f, false, false};
+        if (live && l < deg) {
+            if (pre) bpre0 = bond_pre(V, D, bt0, 0);
+            else bin0 = bond_in(V, D, pos, bt0, 0);
+        }
+        float fix = 0.0f, fiy = 0.0f, mi = 0.0f;
+        if (lists) {
+            if (live || rebuild)
+                list_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, rebuild, lcnt, live, L, g, fix,
+                                     fiy, mi);
+        } else if (live) {
+            contact_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, L, g, fix, fiy, mi);
+        }
+        float fbx = 0.0f, fby = 0.0f, mb = 0.0f;
+        bool crack = false;
+        if (live) {
+            if (pre)
+                bond_pre_group<GR>(V, D, e, true, deg, bt0, bpre0, 0, fbx, fby, mb, crack);
+            else
+                bond_force_group<GR>(V, D, pos, e, px, py, true, deg, bt0, bp0, bin0, 0, L, g, fbx, fby, mb, crack);
+        }
+        if (l == 0) {
+            float* sv = L.stg(g);
+            if (live) {
+                sv[S_PX] = px; sv[S_PY] = py; sv[S_VX] = vx; sv[S_VY] = vy; sv[S_R] = r;
+                sv[S_FIX] = fix; sv[S_FIY] = fiy; sv[S_MI] = mi; sv[S_FBX] = fbx; sv[S_FBY] = fby; sv[S_MB] = mb;
+                sv[S_FLAGS] = __int_as_float((act_raw ? 1 : 0) | 2 | (crack ? 4 : 0));
+                if (GR >= 16) {
+                    epi_put(sv, ei);
+                    d2 = force_epilogue(V, D, 0, 1, 0, nullptr, nullptr, nullptr, sv, e, lists, rebuild);
+                }
+            } else {
+                // the no-contact branch of the submergence update; +0 stays
+                // +0 bit for bit, so only a positive depth is stored
+                if (__float_as_uint(sub0) != 0u)
+                    G(D->s.d_sub)[e] = vmax(0.0f, sub0 - V->p.inv_rise_time * sub0 * V->p.dt);
+                sv[S_FLAGS] = __int_as_float(8);   // done
+            }
+        }
+    }
+    if (GR < 16) {
+        __syncthreads();
+        const int et = chunk * epb<GR>() + (int)threadIdx.x;
+        if ((int)threadIdx.x < epb<GR>() && et < n) {
+            float* sv = L.stg(threadIdx.x);
+            if ((__float_as_int(sv[S_FLAGS]) & 8) == 0) {
+                epi_put(sv, epi_in(V, D, k, 0, 1, 0, et));
+                d2 = force_epilogue(V, D, 0, 1, 0, nullptr, nullptr, nullptr, sv, et, lists, rebuild);
+            }
+        }
+    }
+    if (lists) {
+        float m = d2;
+        #pragma unroll
+        for (int off = 1; off < 64; off <<= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+        if ((threadIdx.x & 63) == 0 && m > V->nl_thr2)
+            __hip_atomic_store(nl_slot(V, k + 1) + (blockIdx.x & (NLSUB - 1)), __float_as_uint(m), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // With nextra > 0 (a frame's first sub-step) blocks [0, nextra) run
 // force_tail (dispatched first, so the ship step overlaps the element
 // work); blocks [nextra, nextra + fblk) integrate their elements.
@@ -1673,8 +1785,11 @@ __device__ __forceinline__ void force_kernel_body(const DevView* __restrict__ V,
         return;
     }
     const int nf = lg - nextra;
-    force_body<GR>(V, D, k, sub, src_is_state, mode, set, fship_in, fout, mout, L,
-                   xcd ? chunk_of(lb - nextra, nf) : lb - nextra, false, true);
+    const int chunk = xcd ? chunk_of(lb - nextra, nf) : lb - nextra;
+    if (frozen_fast(V->p, k, sub, mode))
+        force_body_lazy<GR>(V, D, k, set, L, chunk);
+    else
+        force_body<GR>(V, D, k, sub, src_is_state, mode, set, fship_in, fout, mout, L, chunk, false, true);
 }
 
 template <int GR>
