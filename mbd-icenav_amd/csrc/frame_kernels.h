@@ -34,6 +34,11 @@ template <int GR> constexpr int epb() { return BLOCK / GR; }   // elements per k
 __device__ __forceinline__ int32_t* near_count(const DevView* V) { return V->counters; }
 __device__ __forceinline__ int32_t* force_ticket(const DevView* V) { return V->counters + 3; }
 __device__ __forceinline__ int32_t* coll_count(const DevView* V) { return V->counters + 1; }
+// per-floe hull-contact flag and largest contact force of the frames of
+// parity `set` (frame k uses k & 1: one frame's near test can set the next
+// frame's flags while that frame's k_post clears its own)
+__device__ __forceinline__ uint32_t* floe_coll_of(const DevView* V, int set) { return V->floe_coll + (set & 1) * V->nf; }
+__device__ __forceinline__ uint32_t* floe_maxfm_of(const DevView* V, int set) { return V->floe_maxfm + (set & 1) * V->nf; }
 
 // ── partner lists: control words ─────────────────────────────────────────────
 // Slot k % 3 tells whether a position frame k starts from lies farther than
@@ -478,11 +483,11 @@ __device__ __forceinline__ float wave_tree_global(GAS const float* __restrict__ 
 // hull points of pass 2; elements outside it have no hull contact.  The
 // ship's heading comes from a fast sine/cosine: the 1 m margin dwarfs its
 // error, and only which elements run the exact hull test depends on it.
-__device__ __forceinline__ void cull_chunk(const DevView* __restrict__ V, int chunk, bool valid, int e,
-                                           float x, float y, float vx, float vy, GAS const float* ship, int set,
-                                           bool insert = true, bool slot = true) {
+__device__ __forceinline__ int cull_chunk(const DevView* __restrict__ V, int chunk, bool valid, int e,
+                                          float x, float y, float vx, float vy, GAS const float* ship, int set,
+                                          bool& near, bool insert = true, bool slot = true) {
     const IceDemParams& P = V->p;
-    bool near = false;
+    near = false;
     float r = 0.0f;
     if (valid) {
         r = G(V->radius)[e];
@@ -522,34 +527,13 @@ __device__ __forceinline__ void cull_chunk(const DevView* __restrict__ V, int ch
     }
     const int cnt = __syncthreads_count(near ? 1 : 0);
     if (threadIdx.x == 0) G(V->nearcnt)[chunk] = cnt;
+    return cnt;
 }
 
 // sin/cos of the ship heading for the next k_near (scal[8..11])
 __device__ __forceinline__ void store_pose(const DevView* __restrict__ V, const float* __restrict__ ship) {
     const ShipPose SP = make_pose(ship[0], ship[1], ship[2], V->p.half_len, V->p.half_beam);
     G(V->scal)[8] = SP.s_neg; G(V->scal)[9] = SP.c_neg; G(V->scal)[10] = SP.s_h; G(V->scal)[11] = SP.c_h;
-}
-
-// workgroup lb of a call's first near test (grid: nblk workgroups)
-__device__ __forceinline__ void cull_kernel_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                                 int lb) {
-    const int n = V->n;
-    const int e = lb * BLOCK + threadIdx.x;
-    const bool valid = e < n;
-    GAS const float* pos = G(D->s.d_pos);
-    GAS const float* vel = G(D->s.d_vel);
-    float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f;
-    if (valid) { x = pos[e]; y = pos[n + e]; vx = vel[e]; vy = vel[n + e]; }
-    cull_chunk(V, lb, valid, e, x, y, vx, vy, G(D->s.d_ship), 0);
-    if (lb == 0 && threadIdx.x == 0) store_pose(V, D->s.d_ship);
-    // a call starts with a list build (the caller may have changed the state)
-    if (V->nl_on && lb == 0 && threadIdx.x < 3 * NLSUB + 1)
-        __hip_atomic_store(V->nlctl + threadIdx.x, threadIdx.x < NLSUB ? 0x7f800000u : 0u, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_cull(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
-    cull_kernel_body(V, D, blockIdx.x);
 }
 
 // cell insertion only (sub-steps > 0 and the leaf entry points)
@@ -575,15 +559,14 @@ __global__ __launch_bounds__(BLOCK) void k_clear_cells(const DevView* __restrict
 
 // ── k_near ───────────────────────────────────────────────────────────────────
 
-// Hull narrow phase and pre-pass of near element e by the calling wave;
-// lane 0 stores the per-element results and the floe or/max.
-__device__ __forceinline__ void near_elem(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                          const ShipPose& SP, int e) {
+// Hull narrow phase and pre-pass of near element e at position (px, py)
+// and rotation rot by the calling wave; lane 0 stores the per-element
+// results and the floe or/max of the frames of parity set.
+__device__ __forceinline__ void near_elem_at(const DevView* __restrict__ V, const ShipPose& SP, int e, float px,
+                                             float py, float rot, int set) {
     const IceDemParams& P = V->p;
-    const int n = V->n, maxv = V->maxv;
+    const int maxv = V->maxv;
     const int lane = threadIdx.x & 63;
-    GAS const float* pos = G(D->s.d_pos);
-    const float px = pos[e], py = pos[n + e];
     const float rad = G(V->radius)[e], thk = G(V->thick)[e];
     const int fl = G(V->floe)[e];   // read with the element, used if it touches the hull
     float wx = 0.0f, wy = 0.0f;
@@ -594,7 +577,7 @@ __device__ __forceinline__ void near_elem(const DevView* __restrict__ V, const C
         mv = G(V->mask)[(size_t)e * maxv + lane] != 0;
     }
     float se, ce;
-    dm_sincosf(G(D->s.d_rot)[e], &se, &ce);
+    dm_sincosf(rot, &se, &ce);
     if (lane < maxv) {
         wx = ce * lv.x - se * lv.y + px;
         wy = se * lv.x + ce * lv.y + py;
@@ -609,34 +592,117 @@ __device__ __forceinline__ void near_elem(const DevView* __restrict__ V, const C
         if (pen > 0.0f) {
             // the first contact of the floe this frame lists it for the
             // eligibility update (floes without a contact keep theirs)
-            if (atomicOr(&V->floe_coll[fl], 1u) == 0u)
+            if (atomicOr(floe_coll_of(V, set) + fl, 1u) == 0u)
                 __hip_atomic_store(V->coll_list + atomicAdd(coll_count(V), 1), fl, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
-            atomicMax(&V->floe_maxfm[fl], __float_as_uint(o.fm));
+            atomicMax(floe_maxfm_of(V, set) + fl, __float_as_uint(o.fm));
         }
     }
+}
+
+// near element e with its state read from the call's state buffers
+__device__ __forceinline__ void near_elem(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                          const ShipPose& SP, int e, int set) {
+    GAS const float* pos = G(D->s.d_pos);
+    near_elem_at(V, SP, e, pos[e], pos[V->n + e], G(D->s.d_rot)[e], set);
+}
+
+// The ship pose of the next near test: position from the ship state, the
+// heading's sin/cos as the frame's ship step left them (scal[8..11])
+__device__ __forceinline__ ShipPose stored_pose(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
+    const float* ship = D->s.d_ship;
+    ShipPose SP;
+    SP.sx = ship[0]; SP.sy = ship[1];
+    SP.s_neg = G(V->scal)[8]; SP.c_neg = G(V->scal)[9]; SP.s_h = G(V->scal)[10]; SP.c_h = G(V->scal)[11];
+    SP.hl = V->p.half_len; SP.hb = V->p.half_beam;
+    return SP;
+}
+
+// Near tests inside the culling workgroup (a single field's k_cull and
+// k_post, where a frame has a handful of near elements): the workgroup's
+// near elements are listed in LDS with the position and rotation their
+// threads hold, and its waves take them in turn.  Every thread calls, with
+// cnt its workgroup's near count (workgroup-uniform).  own_pose: the pose
+// from the ship state (a call's first cull, before any ship step stored it).
+struct NearLds {
+    int cnt;
+    int e[BLOCK];
+    float x[BLOCK], y[BLOCK], ro[BLOCK];
+};
+
+__device__ __forceinline__ void near_inline(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                            NearLds* __restrict__ L, int cnt, bool near, int e, float px, float py,
+                                            float ro, int set, bool own_pose) {
+    if (cnt == 0) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) L->cnt = 0;
+    __syncthreads();
+    const uint64_t nb = __ballot(near);
+    if (nb != 0ull) {
+        const int first = __ffsll((unsigned long long)nb) - 1;
+        int base = 0;
+        if (lane == first) base = atomicAdd(&L->cnt, __popcll(nb));
+        base = __shfl(base, first, 64);
+        if (near) {
+            const int q = base + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(nb >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)nb, 0u));
+            L->e[q] = e; L->x[q] = px; L->y[q] = py; L->ro[q] = ro;
+        }
+    }
+    __syncthreads();
+    ShipPose SP;
+    if (own_pose) {
+        const float* ship = D->s.d_ship;
+        SP = make_pose(ship[0], ship[1], ship[2], V->p.half_len, V->p.half_beam);
+    } else {
+        SP = stored_pose(V, D);
+    }
+    for (int q = w; q < cnt; q += BLOCK / 64) near_elem_at(V, SP, L->e[q], L->x[q], L->y[q], L->ro[q], set);
 }
 
 // workgroup lb of lg working on one context's near queue: one wave per near
 // element.  The totals over the near elements (jax_simulation.py:456-470)
 // are formed by the tail of the following k_force.
 __device__ __forceinline__ void near_kernel_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                                 int lb, int lg) {
+                                                 int lb, int lg, int set) {
     const int wave = lb * (BLOCK / 64) + (threadIdx.x >> 6);
     const int nwaves = lg * (BLOCK / 64);
     const int count = *near_count(V);
     const int e0 = (wave < V->n) ? G(V->near_list)[wave] : 0;   // read beside the count
     if (wave >= count) return;
-    const float* ship = D->s.d_ship;
-    ShipPose SP;   // heading sin/cos from the frame's ship step
-    SP.sx = ship[0]; SP.sy = ship[1];
-    SP.s_neg = G(V->scal)[8]; SP.c_neg = G(V->scal)[9]; SP.s_h = G(V->scal)[10]; SP.c_h = G(V->scal)[11];
-    SP.hl = V->p.half_len; SP.hb = V->p.half_beam;
-    for (int q = wave; q < count; q += nwaves) near_elem(V, D, SP, (q == wave) ? e0 : G(V->near_list)[q]);
+    const ShipPose SP = stored_pose(V, D);   // heading sin/cos from the frame's ship step
+    for (int q = wave; q < count; q += nwaves) near_elem(V, D, SP, (q == wave) ? e0 : G(V->near_list)[q], set);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_near(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
-    near_kernel_body(V, D, blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(BLOCK) void k_near(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                                int set) {
+    near_kernel_body(V, D, blockIdx.x, gridDim.x, set);
+}
+
+// workgroup lb of a call's first near test (grid: nblk workgroups)
+// (with nl: and the near tests of the call's first frame)
+__device__ __forceinline__ void cull_kernel_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                                 int lb, NearLds* __restrict__ nl) {
+    const int n = V->n;
+    const int e = lb * BLOCK + threadIdx.x;
+    const bool valid = e < n;
+    GAS const float* pos = G(D->s.d_pos);
+    GAS const float* vel = G(D->s.d_vel);
+    float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f;
+    if (valid) { x = pos[e]; y = pos[n + e]; vx = vel[e]; vy = vel[n + e]; }
+    bool near;
+    const int cnt = cull_chunk(V, lb, valid, e, x, y, vx, vy, G(D->s.d_ship), 0, near);
+    if (nl) near_inline(V, D, nl, cnt, near, e, x, y, near ? G(D->s.d_rot)[e] : 0.0f, 0, true);
+    if (lb == 0 && threadIdx.x == 0) store_pose(V, D->s.d_ship);
+    // a call starts with a list build (the caller may have changed the state)
+    if (V->nl_on && lb == 0 && threadIdx.x < 3 * NLSUB + 1)
+        __hip_atomic_store(V->nlctl + threadIdx.x, threadIdx.x < NLSUB ? 0x7f800000u : 0u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_cull(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
+    __shared__ NearLds nl;
+    cull_kernel_body(V, D, blockIdx.x, &nl);
 }
 
 // ── ship reaction, floe eligibility, ship step ───────────────────────────────
@@ -703,14 +769,15 @@ __device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const 
 // Only floes with a hull contact this frame (the collided list k_near built)
 // can change: elig |= max force magnitude (thrust-scaled) > threshold.
 __device__ __forceinline__ void floe_eligibility(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                                 float scale) {
+                                                 float scale, int set) {
     const IceDemParams& P = V->p;
     if (V->nb <= 0) return;
     GAS uint8_t* elig = G(D->s.d_floe_elig);
     const int cnt = __hip_atomic_load(coll_count(V), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
         const int f = __hip_atomic_load(V->coll_list + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        float mx = __uint_as_float(__hip_atomic_load(V->floe_maxfm + f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        float mx = __uint_as_float(__hip_atomic_load(floe_maxfm_of(V, set) + f, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT));
         if (P.has_thrust_cap) mx = mx * scale;
         if (mx > P.elig_threshold) elig[f] = 1;
     }
@@ -724,7 +791,7 @@ __device__ __forceinline__ void floe_eligibility(const DevView* __restrict__ V, 
 // eligibility and the ship step, and resets the near counters.  The thrust
 // scale is left in scal[0] for k_post.
 __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int k, int last,
-                           int b, int nextra, int* am_last) {
+                           int b, int nextra, int* am_last, int set) {
     const IceDemParams& P = V->p;
     const int nbpad = V->nbpad, nb = V->nblk;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -771,7 +838,7 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
         float unused;
         wave_tree_global3(V->part2, V->part2 + nbpad, V->part2, nbpad, fx, fy, unused);
     }
-    floe_eligibility(V, D, scale);
+    floe_eligibility(V, D, scale, set);
     __syncthreads();
     if (threadIdx.x == 0) {
         G(V->scal)[0] = scale;
@@ -1210,13 +1277,13 @@ struct BondIn {
 };
 
 __device__ __forceinline__ BondIn bond_in(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                          GAS const float* __restrict__ pos, int4 t4, int mode) {
+                                          GAS const float* __restrict__ pos, int4 t4, int mode, int set) {
     BondIn o;
     o.ox = pos[t4.y]; o.oy = pos[V->n + t4.y];
     o.intact = G(D->s.d_bond_intact)[t4.x] != 0;
     o.dmg = G(D->s.d_bond_damage)[t4.x];
     o.act = G(D->s.d_active)[t4.y] != 0;
-    if (mode != 1) o.act = o.act || (G(V->floe_coll)[t4.z] != 0u);
+    if (mode != 1) o.act = o.act || (G(floe_coll_of(V, set))[t4.z] != 0u);
     return o;
 }
 
@@ -1229,7 +1296,7 @@ template <int GR>
 __device__ __forceinline__ void bond_force_group(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                                  GAS const float* __restrict__ pos, int e, float px, float py,
                                                  bool act_e, int deg, int4 t0, float2 p0, BondIn in0, int mode,
-                                                 ForceLds<GR>& L, int g, float& fbx, float& fby, float& mb,
+                                                 int set, ForceLds<GR>& L, int g, float& fbx, float& fby, float& mb,
                                                  bool& crack) {
     const int l = threadIdx.x & (GR - 1);
     const int gshift = (threadIdx.x & 63) & ~(GR - 1);
@@ -1245,7 +1312,7 @@ __device__ __forceinline__ void bond_force_group(const DevView* __restrict__ V, 
             if (base > 0) {
                 t4 = ldg(G(V->btab) + (size_t)e * V->W + q);
                 pr = ldg(G(V->bprm) + (size_t)e * V->W + q);
-                bi = bond_in(V, D, pos, t4, mode);
+                bi = bond_in(V, D, pos, t4, mode, set);
             }
             const bool role_j = t4.w != 0;
             broken = !bi.intact;
@@ -1289,12 +1356,12 @@ struct BondPre {
 };
 
 __device__ __forceinline__ BondPre bond_pre(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int4 t4,
-                                            int mode) {
+                                            int mode, int set) {
     const f4u f = *(GAS const f4u*)(G(V->bfv) + t4.x);
     BondPre o;
     o.fx = f.x; o.fy = f.y; o.cr = f.z; o.intact = f.w != 0.0f;
     o.act = G(D->s.d_active)[t4.y] != 0;
-    if (mode != 1) o.act = o.act || (G(V->floe_coll)[t4.z] != 0u);
+    if (mode != 1) o.act = o.act || (G(floe_coll_of(V, set))[t4.z] != 0u);
     return o;
 }
 
@@ -1307,7 +1374,7 @@ __device__ __forceinline__ BondPre bond_pre(const DevView* __restrict__ V, const
 // every lane of the group.
 template <int GR>
 __device__ __forceinline__ void bond_pre_group(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int e,
-                                               bool act_e, int deg, int4 t0, BondPre b0, int mode,
+                                               bool act_e, int deg, int4 t0, BondPre b0, int mode, int set,
                                                float& fbx, float& fby, float& mb, bool& crack) {
     const int l = threadIdx.x & (GR - 1);
     const int gshift = (threadIdx.x & 63) & ~(GR - 1);
@@ -1322,7 +1389,7 @@ __device__ __forceinline__ void bond_pre_group(const DevView* __restrict__ V, co
             BondPre b = b0;
             if (base > 0) {
                 t4 = ldg(G(V->btab) + (size_t)e * V->W + q);
-                b = bond_pre(V, D, t4, mode);
+                b = bond_pre(V, D, t4, mode, set);
             }
             const bool role_j = t4.w != 0;
             broken = !b.intact;
@@ -1569,13 +1636,13 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
         // 16 lanes: the epilogue-only inputs too, in the same round trip
         EpiIn ei;
         if (GR >= 16) ei = epi_in(V, D, k, sub, src_is_state, mode, e);
-        const bool fcoll = (mode != 1) && (G(V->floe_coll)[fl] != 0u);
+        const bool fcoll = (mode != 1) && (G(floe_coll_of(V, set))[fl] != 0u);
         const bool act_bond = act_raw || fcoll;   // activity seen by the bond pass (mode 0)
         BondIn bin0 = {0.0f, 0.0f, 0.0f, false, false};
         BondPre bpre0 = {0.0f, 0.0f, 0.0f, false, false};
         if (l < deg) {   // read ahead of the contact pass
-            if (pre) bpre0 = bond_pre(V, D, bt0, mode);
-            else bin0 = bond_in(V, D, pos, bt0, mode);
+            if (pre) bpre0 = bond_pre(V, D, bt0, mode, set);
+            else bin0 = bond_in(V, D, pos, bt0, mode, set);
         }
         // A frame sub-step leaves an element that is frozen in it where it
         // is whatever its forces (the integration is masked by elem_active,
@@ -1595,9 +1662,9 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
         if (!live)
             crack = crack_group<GR>(V, D, e, deg, pre ? bpre0.intact : bin0.intact, pre);
         else if (pre)
-            bond_pre_group<GR>(V, D, e, act_bond, deg, bt0, bpre0, mode, fbx, fby, mb, crack);
+            bond_pre_group<GR>(V, D, e, act_bond, deg, bt0, bpre0, mode, set, fbx, fby, mb, crack);
         else
-            bond_force_group<GR>(V, D, pos, e, px, py, mode == 1 ? act_raw : act_bond, deg, bt0, bp0, bin0, mode, L,
+            bond_force_group<GR>(V, D, pos, e, px, py, mode == 1 ? act_raw : act_bond, deg, bt0, bp0, bin0, mode, set, L,
                                  g, fbx, fby, mb, crack);
         if (l == 0) {
             float* sv = L.stg(g);
@@ -1693,14 +1760,14 @@ __device__ __forceinline__ void force_body_lazy(const DevView* __restrict__ V, c
         // an active element's state is read beside nothing else; a frozen
         // one first learns whether its floe was hit this frame
         if (act_raw) load_elem();
-        const bool fcoll = !act_raw && G(V->floe_coll)[fl] != 0u;
+        const bool fcoll = !act_raw && G(floe_coll_of(V, set))[fl] != 0u;
         const bool live = act_raw || fcoll;
         if (!act_raw && (live || rebuild)) load_elem();
         BondIn bin0 = {0.0f, 0.0f, 0.0f, false, false};
         BondPre bpre0 = {0.0f, 0.0f, 0.0f, false, false};
         if (live && l < deg) {
-            if (pre) bpre0 = bond_pre(V, D, bt0, 0);
-            else bin0 = bond_in(V, D, pos, bt0, 0);
+            if (pre) bpre0 = bond_pre(V, D, bt0, 0, set);
+            else bin0 = bond_in(V, D, pos, bt0, 0, set);
         }
         float fix = 0.0f, fiy = 0.0f, mi = 0.0f;
         if (lists) {
@@ -1714,9 +1781,10 @@ __device__ __forceinline__ void force_body_lazy(const DevView* __restrict__ V, c
         bool crack = false;
         if (live) {
             if (pre)
-                bond_pre_group<GR>(V, D, e, true, deg, bt0, bpre0, 0, fbx, fby, mb, crack);
+                bond_pre_group<GR>(V, D, e, true, deg, bt0, bpre0, 0, set, fbx, fby, mb, crack);
             else
-                bond_force_group<GR>(V, D, pos, e, px, py, true, deg, bt0, bp0, bin0, 0, L, g, fbx, fby, mb, crack);
+                bond_force_group<GR>(V, D, pos, e, px, py, true, deg, bt0, bp0, bin0, 0, set, L, g, fbx, fby, mb,
+                                     crack);
         }
         if (l == 0) {
             float* sv = L.stg(g);
@@ -1781,7 +1849,7 @@ __device__ __forceinline__ void force_kernel_body(const DevView* __restrict__ V,
                                                   const float* __restrict__ fship_in, float* __restrict__ fout,
                                                   float* __restrict__ mout, ForceLds<GR>& L, int* am_last) {
     if (lb < nextra) {
-        force_tail(V, D, k, last, lb, nextra, am_last);
+        force_tail(V, D, k, last, lb, nextra, am_last, set);
         return;
     }
     const int nf = lg - nextra;
@@ -1923,11 +1991,11 @@ __device__ __forceinline__ void post_bond(const DevView* __restrict__ V, const C
 // after a frame that used them.
 __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                           int k, int final_in_nxt, int last, int t, int chunk, int set, int cull,
-                                          bool lists = false) {
+                                          bool lists = false, NearLds* __restrict__ nl = nullptr) {
     const int n = V->n;
     GAS const float* pos = G(final_in_nxt ? V->nxt_pos : D->s.d_pos);
     GAS const float* vel = G(final_in_nxt ? V->nxt_vel : D->s.d_vel);
-    float px = 0.0f, py = 0.0f, pvx = 0.0f, pvy = 0.0f;
+    float px = 0.0f, py = 0.0f, pvx = 0.0f, pvy = 0.0f, ro = 0.0f;
     // a frozen element's next state is a copy of its state (the integration
     // keeps it), and its partner-list slot is the one the call's first cull
     // wrote: neither is written again
@@ -1945,7 +2013,8 @@ __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const C
             G(D->h.d_positions)[(size_t)k * 2 * n + t] = px;
             G(D->h.d_positions)[(size_t)k * 2 * n + n + t] = py;
         }
-        if (D->h.d_rotations) G(D->h.d_rotations)[(size_t)k * n + t] = final_in_nxt ? G(V->nxt_rot)[t] : G(D->s.d_rot)[t];
+        if (D->h.d_rotations || nl) ro = final_in_nxt ? G(V->nxt_rot)[t] : G(D->s.d_rot)[t];
+        if (D->h.d_rotations) G(D->h.d_rotations)[(size_t)k * n + t] = ro;
         if (D->h.d_submergence) G(D->h.d_submergence)[(size_t)k * n + t] = G(D->s.d_sub)[t];
         if (last) {
             if (D->dg.d_pen_max) G(D->dg.d_pen_max)[t] = G(V->nearf)[t] ? G(V->pen)[t] : 0.0f;
@@ -1966,10 +2035,13 @@ __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const C
         }
     }
     if (used) clear_cells(V, set, t);
-    if (t < V->nf) { G(V->floe_coll)[t] = 0u; G(V->floe_maxfm)[t] = 0u; }
+    if (t < V->nf) { G(floe_coll_of(V, set))[t] = 0u; G(floe_maxfm_of(V, set))[t] = 0u; }
     if (t == 0) *force_ticket(V) = 0;
-    if (cull && chunk < V->nblk)
-        cull_chunk(V, chunk, t < n, t, px, py, pvx, pvy, G(D->s.d_ship), set ^ 1, ins, !frozen);
+    if (cull && chunk < V->nblk) {
+        bool near;
+        const int cnt = cull_chunk(V, chunk, t < n, t, px, py, pvx, pvy, G(D->s.d_ship), set ^ 1, near, ins, !frozen);
+        if (nl) near_inline(V, D, nl, cnt, near, t, px, py, ro, set ^ 1, false);
+    }
 }
 
 // Both parts for thread t (the one-workgroup scene kernel).
@@ -1987,19 +2059,20 @@ __device__ __forceinline__ void post_body(const DevView* __restrict__ V, const C
 // indices share an L2.
 __device__ __forceinline__ void post_block(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                            int k, int final_in_nxt, int last, int b, int nbe, int set, int cull,
-                                           int nbb, bool xcd) {
+                                           int nbb, bool xcd, NearLds* __restrict__ nl) {
     if (b >= nbe) {
         const int c = xcd ? chunk_of(b - nbe, nbb) : b - nbe;
         post_bond(V, D, k, final_in_nxt, c * BLOCK + (int)threadIdx.x, V->bf_on != 0);
     } else {
         const int c = xcd ? chunk_of(b, nbe) : b;
-        post_elem(V, D, k, final_in_nxt, last, c * BLOCK + (int)threadIdx.x, c, set, cull, V->nl_on != 0);
+        post_elem(V, D, k, final_in_nxt, last, c * BLOCK + (int)threadIdx.x, c, set, cull, V->nl_on != 0, nl);
     }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_post(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                                 int k, int final_in_nxt, int last, int set, int cull, int nbe) {
-    post_block(V, D, k, final_in_nxt, last, blockIdx.x, nbe, set, cull, (int)gridDim.x - nbe, true);
+    __shared__ NearLds nl;
+    post_block(V, D, k, final_in_nxt, last, blockIdx.x, nbe, set, cull, (int)gridDim.x - nbe, true, &nl);
 }
 
 // ── batched frames: the same kernels over many contexts in one launch ───────
@@ -2011,13 +2084,13 @@ __global__ __launch_bounds__(BLOCK) void k_post(const DevView* __restrict__ V, c
 __global__ __launch_bounds__(BLOCK) void kb_cull(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
                                                  const int4* __restrict__ map) {
     const int4 m = map[blockIdx.x];
-    cull_kernel_body(views + m.x, descs + m.x, m.y);
+    cull_kernel_body(views + m.x, descs + m.x, m.y, nullptr);
 }
 
 __global__ __launch_bounds__(BLOCK) void kb_near(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
-                                                 const int4* __restrict__ map) {
+                                                 const int4* __restrict__ map, int set) {
     const int4 m = map[blockIdx.x];
-    near_kernel_body(views + m.x, descs + m.x, m.y, m.z);
+    near_kernel_body(views + m.x, descs + m.x, m.y, m.z, set);
 }
 
 // 8 lanes: held to 8 waves per SIMD (<= 64 VGPRs and <= 100 SGPRs, the
@@ -2039,7 +2112,7 @@ void kb_force(const DevView* __restrict__ views, const CallDesc* __restrict__ de
 __global__ __launch_bounds__(BLOCK) void kb_post(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
                                                  const int4* __restrict__ map, int k, int last, int set) {
     const int4 m = map[blockIdx.x];
-    post_block(views + m.x, descs + m.x, k, 1, last, m.y, m.w, set, last ? 0 : 1, m.z - m.w, false);
+    post_block(views + m.x, descs + m.x, k, 1, last, m.y, m.w, set, last ? 0 : 1, m.z - m.w, false, nullptr);
 }
 
 
@@ -2048,7 +2121,7 @@ __global__ __launch_bounds__(BLOCK) void kb_post(const DevView* __restrict__ vie
 // Totals, thrust cap (jax_simulation.py:456-479), floe eligibility
 // (:511-524) and the ship step, by one workgroup; la, lb hold nbpad floats.
 __device__ __forceinline__ void mid_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D, float* la, float* lb,
-                         int k, int last) {
+                         int k, int last, int set) {
     const IceDemParams& P = V->p;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int nb = V->nblk, nbpad = V->nbpad;
@@ -2085,7 +2158,7 @@ __device__ __forceinline__ void mid_body(const DevView* __restrict__ V, const Ca
         fy = lds_tree_sum(V->part2 + nbpad, nb, nbpad, la, lb);
     }
     if (threadIdx.x == 0) G(V->scal)[0] = scale;
-    floe_eligibility(V, D, scale);
+    floe_eligibility(V, D, scale, set);
     __syncthreads();
     if (threadIdx.x == 0) {
         ship_frame(V, D, k, last, fx, fy, tm);
@@ -2110,7 +2183,8 @@ __global__ __launch_bounds__(BLOCK) void k_scene_rollouts(const DevView* __restr
         const bool valid = e < n;
         float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f;
         if (valid) { x = G(D->s.d_pos)[e]; y = G(D->s.d_pos)[n + e]; vx = G(D->s.d_vel)[e]; vy = G(D->s.d_vel)[n + e]; }
-        cull_chunk(V, c, valid, e, x, y, vx, vy, G(D->s.d_ship), 0);
+        bool near;
+        cull_chunk(V, c, valid, e, x, y, vx, vy, G(D->s.d_ship), 0, near);
     }
     __syncthreads();
     for (int k = 0; k < frames; ++k) {
@@ -2120,11 +2194,11 @@ __global__ __launch_bounds__(BLOCK) void k_scene_rollouts(const DevView* __restr
         if (count > 0) {
             const float* ship = D->s.d_ship;
             const ShipPose SP = make_pose(ship[0], ship[1], ship[2], V->p.half_len, V->p.half_beam);
-            for (int q = w; q < count; q += BLOCK / 64) near_elem(V, D, SP, G(V->near_list)[q]);
+            for (int q = w; q < count; q += BLOCK / 64) near_elem(V, D, SP, G(V->near_list)[q], set);
         }
         (void)lane;
         __syncthreads();
-        mid_body(V, D, la, lb, k, last);
+        mid_body(V, D, la, lb, k, last, set);
         for (int c = 0; c < fch; ++c)
             force_body<GRP>(V, D, k, 0, 1, 0, set, nullptr, nullptr, nullptr, L, c, true);
         __syncthreads();

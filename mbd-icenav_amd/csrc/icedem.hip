@@ -648,7 +648,7 @@ static int alloc_state_buffers(IceDemCtx* ctx) {
     AL(crack, uint8_t, n);
     AL(cells, uint8_t, (size_t)(H.coarse_off + 2 * (int64_t)H.G2 * H.rec_c));   // fine sets, then coarse
     AL(ovf_list, int32_t, 2 * (size_t)H.ovf_cap); AL(ovf_count, int32_t, 2);
-    AL(floe_coll, uint32_t, H.nf); AL(floe_maxfm, uint32_t, H.nf); AL(coll_list, int32_t, H.nf);
+    AL(floe_coll, uint32_t, 2 * (size_t)H.nf); AL(floe_maxfm, uint32_t, 2 * (size_t)H.nf); AL(coll_list, int32_t, H.nf);
     AL(part1, float, 3 * (size_t)nbpad); AL(part2, float, 2 * (size_t)nbpad); AL(scal, float, 16);
     AL(nxt_pos, float, 2 * (size_t)n); AL(nxt_vel, float, 2 * (size_t)n);
     AL(nxt_rot, float, n); AL(nxt_omega, float, n);
@@ -662,8 +662,8 @@ static int alloc_state_buffers(IceDemCtx* ctx) {
     HIP_TRY(hipMemset(H.cells, 0, (size_t)(H.coarse_off + 2 * (int64_t)H.G2 * H.rec_c)));
     HIP_TRY(hipMemset(H.ovf_count, 0, sizeof(int32_t) * 2));
     HIP_TRY(hipMemset(H.counters, 0, sizeof(int32_t) * 4));
-    HIP_TRY(hipMemset(H.floe_coll, 0, sizeof(uint32_t) * std::max(H.nf, 1)));
-    HIP_TRY(hipMemset(H.floe_maxfm, 0, sizeof(uint32_t) * std::max(H.nf, 1)));
+    HIP_TRY(hipMemset(H.floe_coll, 0, sizeof(uint32_t) * 2 * std::max(H.nf, 1)));
+    HIP_TRY(hipMemset(H.floe_maxfm, 0, sizeof(uint32_t) * 2 * std::max(H.nf, 1)));
     HIP_TRY(hipMemset(H.part1, 0, sizeof(float) * 3 * nbpad));
     HIP_TRY(hipMemset(H.part2, 0, sizeof(float) * 2 * nbpad));
     HIP_TRY(hipMemset(H.scal, 0, sizeof(float) * 16));
@@ -751,7 +751,6 @@ static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, LaunchTimer*
     const int nblk = H.nblk;
     const int fblk = force_chunks(H.n, H.lanes);
     const int nextra = tail_groups(nblk);
-    const int near_blocks = std::max(1, std::min(256, (H.n + 3) / 4));
     const int steps = H.p.n_ice_steps;
     // post grid: element/cell/floe blocks, then bond blocks
     const int nbe = (std::max(H.n, std::max(std::max(H.G, H.G2), H.nf)) + BLOCK - 1) / BLOCK;
@@ -761,7 +760,6 @@ static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, LaunchTimer*
     for (int k = 0; k < K && rc == ICEDEM_OK; ++k) {
         const int last = (k == K - 1) ? 1 : 0;
         const int set = k & 1;
-        rc |= launch(lt, 0, k_near, dim3(near_blocks), dim3(BLOCK), stream, V, D);
         for (int s = 0; s < steps; ++s) {
             if (s > 0) {
                 // rebuild the cell buckets for the sub-step's positions
@@ -1051,7 +1049,7 @@ static void batch_launch(IceDemBatch* b, int frames, hipStream_t stream) {
         const int last = (k == frames - 1) ? 1 : 0;
         const int set = k & 1;
         hipLaunchKernelGGL(kb_near, dim3(b->n_near), dim3(BLOCK), 0, stream, b->dviews, b->ddesc,
-                           (const int4*)b->map_near);
+                           (const int4*)b->map_near, set);
         hipLaunchKernelGGL(b->lanes == 8 ? kb_force<8> : kb_force<16>, dim3(b->n_force), dim3(BLOCK), 0, stream, b->dviews, b->ddesc,
                            (const int4*)b->map_force, k, last, set);
         hipLaunchKernelGGL(kb_post, dim3(b->n_post), dim3(BLOCK), 0, stream, b->dviews, b->ddesc,

@@ -97,7 +97,7 @@ struct DevView {
     float cell2;
     int32_t* ovf_list;        // (2 ovf_cap) insertions that found their bucket full
     int32_t* ovf_count;       // (2)
-    uint32_t* floe_coll; uint32_t* floe_maxfm;                      // (nf)
+    uint32_t* floe_coll; uint32_t* floe_maxfm;                      // (2, nf): frame parity
     float* part1;             // (3 * nbpad) block partials: sum soi_x, soi_y, moment
     float* part2;             // (2 * nbpad) block partials: sum of scaled soi
     float* scal;              // [0] thrust scale of the frame, [8..11] sin/cos of -heading and heading
