@@ -33,7 +33,6 @@ template <int GR> constexpr int epb() { return BLOCK / GR; }   // elements per k
 
 __device__ __forceinline__ int32_t* near_count(const DevView* V) { return V->counters; }
 __device__ __forceinline__ int32_t* force_ticket(const DevView* V) { return V->counters + 3; }
-__device__ __forceinline__ int32_t* coll_count(const DevView* V) { return V->counters + 1; }
 // per-floe hull-contact flag and largest contact force of the frames of
 // parity `set` (frame k uses k & 1: one frame's near test can set the next
 // frame's flags while that frame's k_post clears its own)
@@ -483,19 +482,20 @@ __device__ __forceinline__ float wave_tree_global(GAS const float* __restrict__ 
 // hull points of pass 2; elements outside it have no hull contact.  The
 // ship's heading comes from a fast sine/cosine: the 1 m margin dwarfs its
 // error, and only which elements run the exact hull test depends on it.
+// (r, bnd: the element's radius and bound radius, read by the caller with
+// its state; queue: list the near elements for a k_near launch)
 __device__ __forceinline__ int cull_chunk(const DevView* __restrict__ V, int chunk, bool valid, int e,
-                                          float x, float y, float vx, float vy, GAS const float* ship, int set,
-                                          bool& near, bool insert = true, bool slot = true) {
+                                          float x, float y, float vx, float vy, float r, float bnd,
+                                          GAS const float* ship, int set, bool& near, bool queue,
+                                          bool insert = true, bool slot = true) {
     const IceDemParams& P = V->p;
     near = false;
-    float r = 0.0f;
     if (valid) {
-        r = G(V->radius)[e];
         float sh, ch;
         __sincosf(ship[2], &sh, &ch);
         const float dx = x - ship[0], dy = y - ship[1];
         const float u = ch * dx + sh * dy, v = ch * dy - sh * dx;
-        const float b = G(V->bound)[e] + 1.0f;
+        const float b = bnd + 1.0f;
         near = fabsf(u) <= P.ship_box_half_len + b && fabsf(v) <= P.ship_box_half_beam + b;
     }
     // the wave's near elements take consecutive queue entries (one atomic per
@@ -505,16 +505,16 @@ __device__ __forceinline__ int cull_chunk(const DevView* __restrict__ V, int chu
     const int lane = threadIdx.x & 63;
     const int first = __ffsll((unsigned long long)nb) - 1;
     int qbase = 0;
-    if (lane == first) qbase = atomicAdd(near_count(V), __popcll(nb));
+    if (queue && lane == first) qbase = atomicAdd(near_count(V), __popcll(nb));
     int rix = 0, idx = 0;
     if (valid && insert) {
         rix = insert_rec(V, set, x, y, r);
         idx = atomicAdd((int*)rec_ptr(V, rix), 1);
     }
-    if (nb != 0ull) qbase = __shfl(qbase, first, 64);
+    if (queue && nb != 0ull) qbase = __shfl(qbase, first, 64);
     if (valid) {
         G(V->nearf)[e] = near ? 1 : 0;
-        if (near) G(V->near_list)[qbase + __builtin_amdgcn_mbcnt_hi((uint32_t)(nb >> 32),
+        if (queue && near) G(V->near_list)[qbase + __builtin_amdgcn_mbcnt_hi((uint32_t)(nb >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)nb, 0u))] = e;
         if (insert) cell_store(V, set, rix, idx, e, x, y, vx, vy, r);
         if (V->nl_on && slot) {   // the element's slot for the partner-list pass
@@ -568,6 +568,9 @@ __device__ __forceinline__ void near_elem_at(const DevView* __restrict__ V, cons
     const int maxv = V->maxv;
     const int lane = threadIdx.x & 63;
     const float rad = G(V->radius)[e], thk = G(V->thick)[e];
+    // every world vertex lies within the element's bound of its position (up
+    // to the vertices' rounding, which the 1 cm margin covers)
+    const float reach = G(V->bound)[e] + 0.01f;
     const int fl = G(V->floe)[e];   // read with the element, used if it touches the hull
     float wx = 0.0f, wy = 0.0f;
     bool mv = false;
@@ -583,19 +586,16 @@ __device__ __forceinline__ void near_elem_at(const DevView* __restrict__ V, cons
         wy = se * lv.x + ce * lv.y + py;
     }
     float pen, nxo, nyo, cxo, cyo;
-    hull_element(SP, P, G(V->hull), V->S, maxv, wx, wy, mv, pen, nxo, nyo, cxo, cyo);
+    hull_element(SP, P, G(V->hull), V->S, maxv, wx, wy, mv, pen, nxo, nyo, cxo, cyo, px, py, reach);
     if (lane == 0) {
         PrePass o = prepass(P, pen, nxo, nyo, rad, thk, px, py, SP.sx, SP.sy);
         G(V->pen)[e] = pen; G(V->nx)[e] = nxo; G(V->ny)[e] = nyo;
         G(V->fm_raw)[e] = o.fm;
         G(V->soi_x)[e] = o.sx; G(V->soi_y)[e] = o.sy; G(V->mterm)[e] = o.m;
         if (pen > 0.0f) {
-            // the first contact of the floe this frame lists it for the
-            // eligibility update (floes without a contact keep theirs)
-            if (atomicOr(floe_coll_of(V, set) + fl, 1u) == 0u)
-                __hip_atomic_store(V->coll_list + atomicAdd(coll_count(V), 1), fl, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            atomicMax(floe_maxfm_of(V, set) + fl, __float_as_uint(o.fm));
+            // results unused: neither atomic is waited for
+            (void)atomicOr(floe_coll_of(V, set) + fl, 1u);
+            (void)atomicMax(floe_maxfm_of(V, set) + fl, __float_as_uint(o.fm));
         }
     }
 }
@@ -688,10 +688,13 @@ __device__ __forceinline__ void cull_kernel_body(const DevView* __restrict__ V, 
     const bool valid = e < n;
     GAS const float* pos = G(D->s.d_pos);
     GAS const float* vel = G(D->s.d_vel);
-    float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f;
-    if (valid) { x = pos[e]; y = pos[n + e]; vx = vel[e]; vy = vel[n + e]; }
+    float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f, r = 0.0f, bnd = 0.0f;
+    if (valid) {
+        x = pos[e]; y = pos[n + e]; vx = vel[e]; vy = vel[n + e];
+        r = G(V->radius)[e]; bnd = G(V->bound)[e];
+    }
     bool near;
-    const int cnt = cull_chunk(V, lb, valid, e, x, y, vx, vy, G(D->s.d_ship), 0, near);
+    const int cnt = cull_chunk(V, lb, valid, e, x, y, vx, vy, r, bnd, G(D->s.d_ship), 0, near, nl == nullptr);
     if (nl) near_inline(V, D, nl, cnt, near, e, x, y, near ? G(D->s.d_rot)[e] : 0.0f, 0, true);
     if (lb == 0 && threadIdx.x == 0) store_pose(V, D->s.d_ship);
     // a call starts with a list build (the caller may have changed the state)
@@ -766,20 +769,46 @@ __device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const 
 }
 
 // floe eligibility (jax_simulation.py:511-524); all threads of a workgroup.
-// Only floes with a hull contact this frame (the collided list k_near built)
-// can change: elig |= max force magnitude (thrust-scaled) > threshold.
+// Only floes with a hull contact this frame can change, and the frame's near
+// elements with a positive penetration name them (found in the 256-element
+// blocks with a near count, a block per wave in turn; a floe named by
+// several elements is set as often, to the same value): elig |= max force
+// magnitude (thrust-scaled) > threshold.
+// Waves w0 .. of the workgroup take part.
 __device__ __forceinline__ void floe_eligibility(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                                 float scale, int set) {
+                                                 float scale, int set, int w0 = 0) {
     const IceDemParams& P = V->p;
     if (V->nb <= 0) return;
     GAS uint8_t* elig = G(D->s.d_floe_elig);
-    const int cnt = __hip_atomic_load(coll_count(V), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-        const int f = __hip_atomic_load(V->coll_list + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        float mx = __uint_as_float(__hip_atomic_load(floe_maxfm_of(V, set) + f, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT));
-        if (P.has_thrust_cap) mx = mx * scale;
-        if (mx > P.elig_threshold) elig[f] = 1;
+    const int n = V->n, nblk = V->nblk;
+    const int lane = threadIdx.x & 63, w = (threadIdx.x >> 6) - w0, nw = (blockDim.x >> 6) - w0;
+    for (int base = 64 * w; base < nblk; base += 64 * nw) {
+        const int q = base + lane;
+        uint64_t m = __ballot(q < nblk && G(V->nearcnt)[q] > 0);
+        while (m != 0ull) {
+            const int b = base + __builtin_ctzll(m);
+            m &= m - 1ull;
+            // the block's flags, penetrations and floe ids in one round trip
+            uint8_t nf_[BLOCK / 64];
+            float pe_[BLOCK / 64];
+            int fl_[BLOCK / 64];
+            #pragma unroll
+            for (int i = 0; i < BLOCK / 64; ++i) {
+                const int e = min(b * BLOCK + 64 * i + lane, n - 1);
+                nf_[i] = G(V->nearf)[e]; pe_[i] = G(V->pen)[e]; fl_[i] = G(V->floe)[e];
+            }
+            #pragma unroll
+            for (int i = 0; i < BLOCK / 64; ++i) {
+                const int e = b * BLOCK + 64 * i + lane;
+                if (e < n && nf_[i] != 0 && pe_[i] > 0.0f) {
+                    const int f = fl_[i];
+                    float mx = __uint_as_float(__hip_atomic_load(floe_maxfm_of(V, set) + f, __ATOMIC_RELAXED,
+                                                                 __HIP_MEMORY_SCOPE_AGENT));
+                    if (P.has_thrust_cap) mx = mx * scale;
+                    if (mx > P.elig_threshold) elig[f] = 1;
+                }
+            }
+        }
     }
 }
 
@@ -838,14 +867,17 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
         float unused;
         wave_tree_global3(V->part2, V->part2 + nbpad, V->part2, nbpad, fx, fy, unused);
     }
-    floe_eligibility(V, D, scale, set);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        G(V->scal)[0] = scale;
-        ship_frame(V, D, k, last, fx, fy, tm);
-        store_pose(V, D->s.d_ship);
-        *near_count(V) = 0;
-        __hip_atomic_store(coll_count(V), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the ship step on wave 0 while the other waves update floe eligibility
+    // (independent: neither reads what the other writes)
+    if (w == 0) {
+        if (threadIdx.x == 0) {
+            G(V->scal)[0] = scale;
+            ship_frame(V, D, k, last, fx, fy, tm);
+            store_pose(V, D->s.d_ship);
+            *near_count(V) = 0;
+        }
+    } else {
+        floe_eligibility(V, D, scale, set, 1);
     }
 }
 
@@ -1995,7 +2027,7 @@ __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const C
     const int n = V->n;
     GAS const float* pos = G(final_in_nxt ? V->nxt_pos : D->s.d_pos);
     GAS const float* vel = G(final_in_nxt ? V->nxt_vel : D->s.d_vel);
-    float px = 0.0f, py = 0.0f, pvx = 0.0f, pvy = 0.0f, ro = 0.0f;
+    float px = 0.0f, py = 0.0f, pvx = 0.0f, pvy = 0.0f, ro = 0.0f, r = 0.0f, bnd = 0.0f;
     // a frozen element's next state is a copy of its state (the integration
     // keeps it), and its partner-list slot is the one the call's first cull
     // wrote: neither is written again
@@ -2003,6 +2035,7 @@ __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const C
     if (t < n) {
         px = pos[t]; py = pos[n + t];
         pvx = vel[t]; pvy = vel[n + t];
+        if (cull) { r = G(V->radius)[t]; bnd = G(V->bound)[t]; }   // statics for the cull, read with the state
         frozen = G(D->s.d_active)[t] == 0;
         if (final_in_nxt && !frozen) {
             G(D->s.d_pos)[t] = px; G(D->s.d_pos)[n + t] = py;
@@ -2039,7 +2072,8 @@ __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const C
     if (t == 0) *force_ticket(V) = 0;
     if (cull && chunk < V->nblk) {
         bool near;
-        const int cnt = cull_chunk(V, chunk, t < n, t, px, py, pvx, pvy, G(D->s.d_ship), set ^ 1, near, ins, !frozen);
+        const int cnt = cull_chunk(V, chunk, t < n, t, px, py, pvx, pvy, r, bnd, G(D->s.d_ship), set ^ 1, near,
+                                   nl == nullptr, ins, !frozen);
         if (nl) near_inline(V, D, nl, cnt, near, t, px, py, ro, set ^ 1, false);
     }
 }
@@ -2163,7 +2197,6 @@ __device__ __forceinline__ void mid_body(const DevView* __restrict__ V, const Ca
     if (threadIdx.x == 0) {
         ship_frame(V, D, k, last, fx, fy, tm);
         *near_count(V) = 0;
-        __hip_atomic_store(coll_count(V), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
 }
@@ -2182,9 +2215,13 @@ __global__ __launch_bounds__(BLOCK) void k_scene_rollouts(const DevView* __restr
         const int e = c * BLOCK + threadIdx.x;
         const bool valid = e < n;
         float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f;
-        if (valid) { x = G(D->s.d_pos)[e]; y = G(D->s.d_pos)[n + e]; vx = G(D->s.d_vel)[e]; vy = G(D->s.d_vel)[n + e]; }
+        float r = 0.0f, bnd = 0.0f;
+        if (valid) {
+            x = G(D->s.d_pos)[e]; y = G(D->s.d_pos)[n + e]; vx = G(D->s.d_vel)[e]; vy = G(D->s.d_vel)[n + e];
+            r = G(V->radius)[e]; bnd = G(V->bound)[e];
+        }
         bool near;
-        cull_chunk(V, c, valid, e, x, y, vx, vy, G(D->s.d_ship), 0, near);
+        cull_chunk(V, c, valid, e, x, y, vx, vy, r, bnd, G(D->s.d_ship), 0, near, true);
     }
     __syncthreads();
     for (int k = 0; k < frames; ++k) {

@@ -648,7 +648,7 @@ static int alloc_state_buffers(IceDemCtx* ctx) {
     AL(crack, uint8_t, n);
     AL(cells, uint8_t, (size_t)(H.coarse_off + 2 * (int64_t)H.G2 * H.rec_c));   // fine sets, then coarse
     AL(ovf_list, int32_t, 2 * (size_t)H.ovf_cap); AL(ovf_count, int32_t, 2);
-    AL(floe_coll, uint32_t, 2 * (size_t)H.nf); AL(floe_maxfm, uint32_t, 2 * (size_t)H.nf); AL(coll_list, int32_t, H.nf);
+    AL(floe_coll, uint32_t, 2 * (size_t)H.nf); AL(floe_maxfm, uint32_t, 2 * (size_t)H.nf);
     AL(part1, float, 3 * (size_t)nbpad); AL(part2, float, 2 * (size_t)nbpad); AL(scal, float, 16);
     AL(nxt_pos, float, 2 * (size_t)n); AL(nxt_vel, float, 2 * (size_t)n);
     AL(nxt_rot, float, n); AL(nxt_omega, float, n);

@@ -61,8 +61,7 @@ struct DevView {
     uint8_t* nearf;           // (n) element passed the hull cull this frame
     int32_t* near_list;       // (n) queue of near elements
     int32_t* nearcnt;         // (nblk) near elements of each 256-element block
-    int32_t* counters;        // [0] near elements, [1] collided floes, [3] k_force tail ticket ([2] unused)
-    int32_t* coll_list;       // (nf) floes with a hull contact this frame, counters[1] of them
+    int32_t* counters;        // [0] near elements, [3] k_force tail ticket ([1], [2] unused)
     // bond forces formed once per bond by k_post for the next frame (one ice
     // sub-step per frame): (Fx, Fy, torque, intact) without the activity factor
     int32_t bf_on;

@@ -155,20 +155,23 @@ __device__ __forceinline__ void point_to_ship(const ShipPose& P, float px, float
 // One element, processed by the whole wave (wave-uniform control flow).
 // Lane v holds world vertex v in (wx, wy) and its mask bit in mv.
 // Returns pass-1/pass-2 combined (pen, nx, ny, cx, cy) in every lane.
+// (ex, ey): the element's position, every world vertex within `reach` of it
+// (reach < 0: unknown); pass 2 then tests only the hull points within reach:
+// a point farther out lies outside the polygon by more than the vertices'
+// rounding, so its ray crossings are even and its penetration 0, and a
+// point with penetration 0 cannot win the pass (it counts only above 0).
 __device__ __forceinline__ void hull_element(const ShipPose& P, const IceDemParams& prm,
                                              GAS const float2* __restrict__ hull, int S, int maxv,
                                              float wx, float wy, bool mv,
                                              float& pen_o, float& nx_o, float& ny_o,
-                                             float& cx_o, float& cy_o) {
+                                             float& cx_o, float& cy_o,
+                                             float ex = 0.0f, float ey = 0.0f, float reach = -1.0f) {
     const int lane = threadIdx.x & 63;
     const int K = prm.n_edge_samples;
     const int per = 1 + K;
     const int npts = maxv * per;
-    // GP lanes per hull point in pass 2; its first round's hull point is
-    // loaded here so the load overlaps pass 1
-    const int GP = S < 64 ? 64 / S : 1;
-    const int per_round = 64 / GP;
-    const float2 hl0 = ldg(hull + ((lane / GP) < per_round && lane / GP < S ? lane / GP : 0));
+    // pass 2's first 64 hull points, loaded here so the load overlaps pass 1
+    const float2 hl0 = ldg(hull + (lane < S ? lane : 0));
     // ---- pass 1: vertices + edge samples vs analytic hull (:139-284)
     float bv = -1.0f, bnx = 0.0f, bny = 0.0f, bcx = 0.0f, bcy = 0.0f;
     int bi_ = 0x7fffffff;
@@ -203,83 +206,106 @@ __device__ __forceinline__ void hull_element(const ShipPose& P, const IceDemPara
     float cx1 = __shfl(bcx, src, 64), cy1 = __shfl(bcy, src, 64);
 
     // ---- pass 2: hull points inside the ice polygon (:289-510)
-    // masked vertex mean (sequential over the padded vertices)
-    float nreal = 0.0f, sxv = 0.0f, syv = 0.0f;
-    for (int v = 0; v < maxv; ++v) {
-        int m = __builtin_amdgcn_readlane((int)mv, v);
-        float xv = rl(wx, v), yv = rl(wy, v);
-        nreal = nreal + (m ? 1.0f : 0.0f);
-        sxv = sxv + (m ? xv : 0.0f);
-        syv = syv + (m ? yv : 0.0f);
-    }
-    nreal = vmax(nreal, 1.0f);
-    float ccx = sxv / nreal, ccy = syv / nreal;
     float b2v = -1.0f, b2nx = 0.0f, b2ny = 0.0f, b2hx = 0.0f, b2hy = 0.0f;
     int b2i = 0x7fffffff;
-    // GP lanes per hull point share its polygon edges (kk = sub, sub + GP,
-    // ...); crossings add up and the nearest edge is the first minimum, so
-    // combining the lanes gives the sequential scan's result exactly
-    for (int base = 0; base < S; base += per_round) {
-        const int s = base + lane / GP, sub = lane % GP;
-        const bool live = (lane / GP) < per_round && s < S;
-        const int sc = live ? s : 0;
-        float2 hl = hl0;
-        if (base > 0) hl = ldg(hull + sc);
-        float hx = (hl.x * P.c_h + hl.y * (-P.s_h)) + P.sx;
-        float hy = (hl.x * P.s_h + hl.y * P.c_h) + P.sy;
-        int cross = 0;
-        float bd = 0.0f;
-        int bk = 0x7fffffff;
-        for (int k0 = 0; k0 < maxv; k0 += GP) {
-            const int kk = k0 + sub;
-            const bool ok = kk < maxv;
-            const int kkc = ok ? kk : 0;
-            int kj = (kkc + 1 == maxv) ? 0 : kkc + 1;
-            float xi = __shfl(wx, kkc, 64), yi = __shfl(wy, kkc, 64);
-            float xj = __shfl(wx, kj, 64), yj = __shfl(wy, kj, 64);
-            bool cy = (yi > hy) != (yj > hy);
-            float x_int = (xj - xi) * (hy - yi) / ((yj - yi) + 1e-10f) + xi;
-            if (ok && cy && (hx < x_int)) cross++;
-            float ex = xj - xi, ey = yj - yi;
-            float wxp = hx - xi, wyp = hy - yi;
-            float c1 = wxp * ex + wyp * ey;
-            float c2 = (ex * ex + ey * ey) + 1e-20f;
-            float t = vclip(c1 / c2, 0.0f, 1.0f);
-            float qx = xi + t * ex, qy = yi + t * ey;
-            float ddx = hx - qx, ddy = hy - qy;
-            float d = sqrtf((ddx * ddx + ddy * ddy) + 1e-20f);
-            if (ok && (bk == 0x7fffffff || d < bd)) { bd = d; bk = kk; }
+    float ccx = 0.0f, ccy = 0.0f;
+    bool have_cc = false;
+    const float reach2 = reach * reach;
+    for (int base = 0; base < S; base += 64) {
+        const int s = base + lane;
+        const bool inb = s < S;
+        const float2 hl = (base == 0) ? hl0 : ldg(hull + (inb ? s : 0));
+        const float hx = (hl.x * P.c_h + hl.y * (-P.s_h)) + P.sx;
+        const float hy = (hl.x * P.s_h + hl.y * P.c_h) + P.sy;
+        bool live = inb;
+        if (reach >= 0.0f) {
+            const float dx = hx - ex, dy = hy - ey;
+            live = live && (dx * dx + dy * dy) <= reach2;
         }
-        // combine the GP lanes of each hull point into its first lane (reading
-        // every lane's own partial result, not a combined one)
-        const int cross0 = cross, bk0 = bk;
-        const float bd0 = bd;
-        for (int t = 1; t < GP; ++t) {
-            const int src = min(lane + t, 63);
-            const int oc = __shfl(cross0, src, 64);
-            const float od = __shfl(bd0, src, 64);
-            const int ok_ = __shfl(bk0, src, 64);
-            if (sub + t < GP) {
+        const uint64_t m = __ballot(live);
+        const int nlive = __popcll(m);
+        if (nlive == 0) continue;
+        if (!have_cc) {
+            // masked vertex mean (sequential over the padded vertices)
+            float nreal = 0.0f, sxv = 0.0f, syv = 0.0f;
+            for (int v = 0; v < maxv; ++v) {
+                int mk = __builtin_amdgcn_readlane((int)mv, v);
+                float xv = rl(wx, v), yv = rl(wy, v);
+                nreal = nreal + (mk ? 1.0f : 0.0f);
+                sxv = sxv + (mk ? xv : 0.0f);
+                syv = syv + (mk ? yv : 0.0f);
+            }
+            nreal = vmax(nreal, 1.0f);
+            ccx = sxv / nreal; ccy = syv / nreal;
+            have_cc = true;
+        }
+        // GP lanes (a power of two) per live hull point share its polygon
+        // edges (kk = sub, sub + GP, ...); crossings add up and the nearest
+        // edge is the first minimum, a total order on (distance, edge), so
+        // combining the lanes gives the sequential scan's result exactly
+        const int GP = nlive > 32 ? 1 : nlive > 16 ? 2 : nlive > 8 ? 4 : nlive > 4 ? 8 : nlive > 2 ? 16 : nlive > 1 ? 32 : 64;
+        const int per_round = 64 / GP;
+        const int slot = lane / GP, sub = lane & (GP - 1);
+        uint64_t rest = m;
+        for (int j0 = 0; j0 < nlive; j0 += per_round) {
+            // the lane holding live point j0 + slot (points in ascending order)
+            int srcl = 0;
+            const int cnt = min(per_round, nlive - j0);
+            for (int q = 0; q < cnt; ++q) {
+                const int bq = __builtin_ctzll(rest);
+                rest &= rest - 1ull;
+                if (q == slot) srcl = bq;
+            }
+            const bool have = slot < cnt;
+            const float qx = __shfl(hx, srcl, 64), qy = __shfl(hy, srcl, 64);
+            int cross = 0;
+            float bd = 0.0f;
+            int bk = 0x7fffffff;
+            for (int k0 = 0; k0 < maxv; k0 += GP) {
+                const int kk = k0 + sub;
+                const bool ok = kk < maxv;
+                const int kkc = ok ? kk : 0;
+                int kj = (kkc + 1 == maxv) ? 0 : kkc + 1;
+                float xi = __shfl(wx, kkc, 64), yi = __shfl(wy, kkc, 64);
+                float xj = __shfl(wx, kj, 64), yj = __shfl(wy, kj, 64);
+                bool cy = (yi > qy) != (yj > qy);
+                float x_int = (xj - xi) * (qy - yi) / ((yj - yi) + 1e-10f) + xi;
+                if (ok && cy && (qx < x_int)) cross++;
+                float ex_ = xj - xi, ey_ = yj - yi;
+                float wxp = qx - xi, wyp = qy - yi;
+                float c1 = wxp * ex_ + wyp * ey_;
+                float c2 = (ex_ * ex_ + ey_ * ey_) + 1e-20f;
+                float t = vclip(c1 / c2, 0.0f, 1.0f);
+                float px_ = xi + t * ex_, py_ = yi + t * ey_;
+                float ddx = qx - px_, ddy = qy - py_;
+                float d = sqrtf((ddx * ddx + ddy * ddy) + 1e-20f);
+                if (ok && (bk == 0x7fffffff || d < bd)) { bd = d; bk = kk; }
+            }
+            // combine the GP lanes of each point (aligned groups, xor butterfly)
+            for (int off = 1; off < GP; off <<= 1) {
+                const int oc = __shfl_xor(cross, off, 64);
+                const float od = __shfl_xor(bd, off, 64);
+                const int ok_ = __shfl_xor(bk, off, 64);
                 cross += oc;
                 if (ok_ != 0x7fffffff && (bk == 0x7fffffff || od < bd || (od == bd && ok_ < bk))) { bd = od; bk = ok_; }
             }
-        }
-        if (bk == 0x7fffffff) bk = 0;
-        int bkj = (bk + 1 == maxv) ? 0 : bk + 1;
-        float xi = __shfl(wx, bk, 64), yi = __shfl(wy, bk, 64);
-        float xj = __shfl(wx, bkj, 64), yj = __shfl(wy, bkj, 64);
-        if (live && sub == 0) {
-            float ex = xj - xi, ey = yj - yi;
-            float nxe = ey, nye = -ex;
-            float nlen = sqrtf((nxe * nxe + nye * nye) + 1e-20f);
-            nxe = nxe / nlen; nye = nye / nlen;
-            float midx = (xi + xj) / 2.0f, midy = (yi + yj) / 2.0f;
-            float dot = (midx - ccx) * nxe + (midy - ccy) * nye;
-            if (dot < 0.0f) { nxe = -nxe; nye = -nye; }
-            bool inside = (cross & 1) != 0;
-            float ps = inside ? bd : 0.0f;
-            float nxs = inside ? nxe : 0.0f, nys = inside ? nye : 0.0f;
-            if (ps > b2v) { b2v = ps; b2i = s; b2nx = nxs; b2ny = nys; b2hx = hx; b2hy = hy; }
+            if (bk == 0x7fffffff) bk = 0;
+            int bkj = (bk + 1 == maxv) ? 0 : bk + 1;
+            float xi = __shfl(wx, bk, 64), yi = __shfl(wy, bk, 64);
+            float xj = __shfl(wx, bkj, 64), yj = __shfl(wy, bkj, 64);
+            if (have && sub == 0) {
+                float ex_ = xj - xi, ey_ = yj - yi;
+                float nxe = ey_, nye = -ex_;
+                float nlen = sqrtf((nxe * nxe + nye * nye) + 1e-20f);
+                nxe = nxe / nlen; nye = nye / nlen;
+                float midx = (xi + xj) / 2.0f, midy = (yi + yj) / 2.0f;
+                float dot = (midx - ccx) * nxe + (midy - ccy) * nye;
+                if (dot < 0.0f) { nxe = -nxe; nye = -nye; }
+                bool inside = (cross & 1) != 0;
+                float ps = inside ? bd : 0.0f;
+                float nxs = inside ? nxe : 0.0f, nys = inside ? nye : 0.0f;
+                if (ps > b2v) { b2v = ps; b2i = base + srcl; b2nx = nxs; b2ny = nys; b2hx = qx; b2hy = qy; }
+            }
         }
     }
     float v2 = b2v; int i2 = b2i;
@@ -347,7 +373,7 @@ __device__ __forceinline__ void hull_wave(const IceDemParams& prm, const float2*
             mv = mask[(size_t)qe * maxv + lane] != 0;
         }
         float a, b, c, d, f;
-        hull_element(P, prm, G(hull), S, maxv, wx, wy, mv, a, b, c, d, f);
+        hull_element(P, prm, G(hull), S, maxv, wx, wy, mv, a, b, c, d, f, qpx, qpy, bound[qe] + 0.01f);
         if (lane == q) { pen = a; nxo = b; nyo = c; cxo = d; cyo = f; }
     }
 }
