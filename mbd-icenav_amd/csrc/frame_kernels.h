@@ -395,35 +395,61 @@ __device__ __forceinline__ float wave_tree_small(float x, int npad) {
     return __shfl(x, 0, 64);
 }
 
+// Which entries of the lane's chunk of a block-partial array may be nonzero:
+// bit i for entry lane * c + i (c = npad / 64; npad <= 64: bit 0 for entry
+// lane), set where the 256-element block holds a near element.  A block
+// without one has partials of exactly +0.  nearcnt is zero-padded to npad.
+__device__ __forceinline__ uint64_t near_chunk_bits(const DevView* __restrict__ V, int npad) {
+    const int lane = threadIdx.x & 63;
+    GAS const int32_t* nc = G(V->nearcnt);
+    if (npad <= 64) return (lane < npad && nc[lane] > 0) ? 1ull : 0ull;
+    const int c = npad >> 6;
+    uint64_t bits = 0;
+    if (c == 2) {
+        typedef int gas_i2 __attribute__((ext_vector_type(2)));
+        const gas_i2 v = *(GAS const gas_i2*)(nc + 2 * lane);
+        return (uint64_t)(v.x > 0) | ((uint64_t)(v.y > 0) << 1);
+    }
+    for (int i = 0; i < c; i += 4) {
+        const gas_i4 v = *(GAS const gas_i4*)(nc + (size_t)lane * c + i);
+        bits |= ((uint64_t)(v.x > 0) | ((uint64_t)(v.y > 0) << 1) | ((uint64_t)(v.z > 0) << 2) |
+                 ((uint64_t)(v.w > 0) << 3)) << i;
+    }
+    return bits;
+}
+
 // Pairwise trees of three arrays a0, a1, a2 (npad floats each, npad a power
 // of two) by one wave, written by other workgroups of the running kernel
-// (device-coherent loads); the order of wave_tree_global.  Up to
-// 64 * TREE3_CHUNK entries every load is issued before the first add; larger
-// arrays take wave_tree_global per array.  Results in every lane.
+// (device-coherent loads of the entries `live` marks, +0 for the others,
+// which hold +0); the order of wave_tree_global.  Up to 64 * TREE3_CHUNK
+// entries every load is issued before the first add; larger arrays take
+// wave_tree_global per array.  Results in every lane.
 constexpr int TREE3_CHUNK = 16;
-__device__ __forceinline__ float wave_tree_global(GAS const float* __restrict__ a, int npad, const float* dev);
+__device__ __forceinline__ float wave_tree_global(GAS const float* __restrict__ a, int npad, const float* dev,
+                                                  uint64_t live);
 __device__ __forceinline__ void wave_tree_global3(const float* a0, const float* a1, const float* a2, int npad,
-                                                  float& s0, float& s1, float& s2) {
+                                                  float& s0, float& s1, float& s2, uint64_t live) {
     const int lane = threadIdx.x & 63;
     if (npad <= 64) {
-        const float l0 = lane < npad ? ld_dev(a0 + lane) : 0.0f;
-        const float l1 = lane < npad ? ld_dev(a1 + lane) : 0.0f;
-        const float l2 = lane < npad ? ld_dev(a2 + lane) : 0.0f;
+        const bool ok = lane < npad && (live & 1ull) != 0ull;
+        const float l0 = ok ? ld_dev(a0 + lane) : 0.0f;
+        const float l1 = ok ? ld_dev(a1 + lane) : 0.0f;
+        const float l2 = ok ? ld_dev(a2 + lane) : 0.0f;
         s0 = wave_tree_small(l0, npad); s1 = wave_tree_small(l1, npad); s2 = wave_tree_small(l2, npad);
         return;
     }
     const int c = npad >> 6;
     if (c > TREE3_CHUNK) {
-        s0 = wave_tree_global((GAS const float*)a0, npad, a0);
-        s1 = wave_tree_global((GAS const float*)a1, npad, a1);
-        s2 = wave_tree_global((GAS const float*)a2, npad, a2);
+        s0 = wave_tree_global((GAS const float*)a0, npad, a0, live);
+        s1 = wave_tree_global((GAS const float*)a1, npad, a1, live);
+        s2 = wave_tree_global((GAS const float*)a2, npad, a2, live);
         return;
     }
     float v0[TREE3_CHUNK], v1[TREE3_CHUNK], v2[TREE3_CHUNK];
     const size_t b = (size_t)lane * c;
     #pragma unroll
     for (int i = 0; i < TREE3_CHUNK; ++i) {
-        const bool ok = i < c;
+        const bool ok = i < c && ((live >> i) & 1ull) != 0ull;
         v0[i] = ok ? ld_dev(a0 + b + i) : 0.0f;
         v1[i] = ok ? ld_dev(a1 + b + i) : 0.0f;
         v2[i] = ok ? ld_dev(a2 + b + i) : 0.0f;
@@ -444,9 +470,10 @@ __device__ __forceinline__ void wave_tree_global3(const float* a0, const float* 
 // dev: the same array as a generic pointer, read with device-coherent loads
 // (values written by other workgroups of the running kernel)
 __device__ __forceinline__ float wave_tree_global(GAS const float* __restrict__ a, int npad,
-                                                  const float* dev = nullptr) {
+                                                  const float* dev, uint64_t live) {
     const int lane = threadIdx.x & 63;
-    if (npad <= 64) return wave_tree_small(lane < npad ? (dev ? ld_dev(dev + lane) : a[lane]) : 0.0f, npad);
+    if (npad <= 64)
+        return wave_tree_small((lane < npad && (live & 1ull)) ? (dev ? ld_dev(dev + lane) : a[lane]) : 0.0f, npad);
     const int c = npad >> 6;
     GAS const float* p = a + (size_t)lane * c;
     const float* pd = dev ? dev + (size_t)lane * c : nullptr;
@@ -454,7 +481,8 @@ __device__ __forceinline__ float wave_tree_global(GAS const float* __restrict__ 
     #pragma unroll
     for (int i = 0; i < 64; ++i) {
         if (i < c) {
-            float x = pd ? ld_dev(pd + i) : p[i];
+            const bool ok = ((live >> i) & 1ull) != 0ull;
+            float x = ok ? (pd ? ld_dev(pd + i) : p[i]) : 0.0f;
             int lvl = 0;
             #pragma unroll
             for (int t = 0; t < 6; ++t) {
@@ -703,24 +731,91 @@ __device__ __forceinline__ void cull_kernel_body(const DevView* __restrict__ V, 
                            __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_cull(const DevView* __restrict__ V, const CallDesc* __restrict__ D) {
-    __shared__ NearLds nl;
-    cull_kernel_body(V, D, blockIdx.x, &nl);
-}
 
 // ── ship reaction, floe eligibility, ship step ───────────────────────────────
 
 // ship reaction and ship step (jax_simulation.py:471-479, 658-697); one thread.
 // (fsx, fsy) is the sum of the (scaled) ship forces on ice, tm their moment.
+// The frame's ship inputs (setpoints, or the rollout's control knot, and the
+// current at the ship) for frame k.
+__device__ __forceinline__ void ship_inputs(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int k,
+                                            float* spc, float* spn, float& cu, float& cr, float& vwx, float& vwy) {
+    const IceDemParams& P = V->p;
+    for (int i = 0; i < 3; ++i) { spc[i] = 0.0f; spn[i] = 0.0f; }
+    cu = 0.0f; cr = 0.0f;
+    if (P.ship_mode == 1) {
+        if (D->in.d_controls && D->in.n_knots > 0) {
+            const int knot = min(k / max(D->in.hold, 1), D->in.n_knots - 1);
+            cu = G(D->in.d_controls)[2 * knot];
+            cr = G(D->in.d_controls)[2 * knot + 1];
+        }
+    } else {
+        GAS const float* sp = G(D->in.d_setpoints) + (size_t)k * 6;
+        spc[0] = sp[0]; spc[1] = sp[1]; spc[2] = sp[2];
+        spn[0] = sp[3]; spn[1] = sp[4]; spn[2] = sp[5];
+    }
+    vwx = 0.0f; vwy = 0.0f;
+    if (D->in.d_v_water_ship) { vwx = G(D->in.d_v_water_ship)[2 * k]; vwy = G(D->in.d_v_water_ship)[2 * k + 1]; }
+}
+
+// Whether the force-independent part of the ship step can be formed before
+// the frame's ice forces are known: not with the ice brake, whose setpoint
+// blend depends on them (jax_simulation.py:662-674).
+__device__ __forceinline__ bool ship_pre_early(const IceDemParams& P) {
+    return !(P.ship_mode != 1 && P.ice_brake_threshold > 0.0f);
+}
+
+// ShipPre through device memory, from one kernel to the next
+constexpr int SHIP_PRE_N = (int)(sizeof(ShipPre) / sizeof(float));
+static_assert(sizeof(ShipPre) == SHIP_PRE_N * sizeof(float), "ShipPre is floats only");
+__device__ __forceinline__ void ship_pre_get(const float* src, ShipPre& o) {
+    float* f = (float*)&o;
+    GAS const float* g = G(src);
+    #pragma unroll
+    for (int i = 0; i < SHIP_PRE_N; ++i) f[i] = g[i];
+}
+
+// One workgroup of the kernel before frame k's force pass (a call's first
+// cull, or frame k - 1's k_post, after that frame's ship step) forms the
+// force-independent part of frame k's ship step, off the force pass's path.
+__device__ __forceinline__ void ship_pre_block(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int k) {
+    if (threadIdx.x != 0) return;
+    float spc[3], spn[3], cu, cr, vwx, vwy;
+    ship_inputs(V, D, k, spc, spn, cu, cr, vwx, vwy);
+    float st[ICEDEM_SHIP_STATE_LEN];
+    GAS const float* ship = G(D->s.d_ship);
+    for (int i = 0; i < ICEDEM_SHIP_STATE_LEN; ++i) st[i] = ship[i];
+    ShipPre o;
+    ship_pre(V->p, st, spc, spn, vwx, vwy, cu, cr, o);
+    const float* f = (const float*)&o;
+    GAS float* dst = G(V->shpre);
+    for (int i = 0; i < SHIP_PRE_N; ++i) dst[i] = f[i];
+}
+
+// shipblk: the grid's last workgroup forms frame 0's ship_pre_block
+__global__ __launch_bounds__(BLOCK) void k_cull(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                                int shipblk) {
+    __shared__ NearLds nl;
+    if (shipblk && (int)blockIdx.x == (int)gridDim.x - 1) {
+        ship_pre_block(V, D, 0);
+        return;
+    }
+    cull_kernel_body(V, D, blockIdx.x, &nl);
+}
+
+// pre: the force-independent part of this frame's ship step formed earlier
+// (ship_pre_early), else nullptr
 __device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                           int k, int last, float fsx, float fsy, float tm) {
+                                           int k, int last, float fsx, float fsy, float tm,
+                                           const ShipPre* pre = nullptr) {
     const IceDemParams& P = V->p;
     float Fx = -fsx, Fy = -fsy;
     float tau[3] = {Fx, Fy, tm};
     GAS float* ship = G(D->s.d_ship);
     float spc[3] = {0.0f, 0.0f, 0.0f}, spn[3] = {0.0f, 0.0f, 0.0f};
     float cu = 0.0f, cr = 0.0f;
-    if (P.ship_mode == 1) {
+    if (pre) {
+    } else if (P.ship_mode == 1) {
         if (D->in.d_controls && D->in.n_knots > 0) {
             const int knot = min(k / max(D->in.hold, 1), D->in.n_knots - 1);
             cu = G(D->in.d_controls)[2 * knot];
@@ -739,7 +834,9 @@ __device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const 
         }
     }
     float vwx = 0.0f, vwy = 0.0f;
-    if (D->in.d_v_water_ship) { vwx = G(D->in.d_v_water_ship)[2 * k]; vwy = G(D->in.d_v_water_ship)[2 * k + 1]; }
+    if (!pre && D->in.d_v_water_ship) {
+        vwx = G(D->in.d_v_water_ship)[2 * k]; vwy = G(D->in.d_v_water_ship)[2 * k + 1];
+    }
     float st[ICEDEM_SHIP_STATE_LEN];
     for (int i = 0; i < ICEDEM_SHIP_STATE_LEN; ++i) st[i] = ship[i];
     const float Fm = sqrtf(Fx * Fx + Fy * Fy);
@@ -753,7 +850,8 @@ __device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const 
         ro[0] = ro[0] + J;
         ro[2] = vmax(ro[2], Fm);
     }
-    ship_step(P, st, spc, spn, tau, vwx, vwy, cu, cr);
+    if (pre) ship_post(P, *pre, tau, st);
+    else ship_step(P, st, spc, spn, tau, vwx, vwy, cu, cr);
     for (int i = 0; i < ICEDEM_SHIP_STATE_LEN; ++i) ship[i] = st[i];
     if (ro && last) {
         const float ke = 0.5f * P.ship_mass * (st[3] * st[3] + st[4] * st[4]) + 0.5f * P.ship_Izz * (st[5] * st[5]);
@@ -819,13 +917,18 @@ __device__ __forceinline__ void floe_eligibility(const DevView* __restrict__ V, 
 // their totals (:471-479) over the blocks holding near elements, the floe
 // eligibility and the ship step, and resets the near counters.  The thrust
 // scale is left in scal[0] for k_post.
+// preship: the previous kernel left the force-independent part of this
+// frame's ship step in shpre (ship_pre_block).
 __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int k, int last,
-                           int b, int nextra, int* am_last, int set) {
+                           int b, int nextra, int* am_last, int set, bool preship) {
     const IceDemParams& P = V->p;
     const int nbpad = V->nbpad, nb = V->nblk;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     constexpr int NW = BLOCK / 64;
     const int blk = b * NW + w;
+    // which partials can be nonzero, for the last workgroup's trees (read
+    // before the ticket, off the last workgroup's path)
+    const uint64_t live = near_chunk_bits(V, nbpad);
     if (blk < nb) {
         float a = 0.0f, c = 0.0f, d = 0.0f;
         if (G(V->nearcnt)[blk] > 0) block256_tree3(V, G(V->soi_x), G(V->soi_y), G(V->mterm), blk, a, c, d);
@@ -836,9 +939,11 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
     if (threadIdx.x == 0) *am_last = (atomicAdd(force_ticket(V), 1) == nextra - 1) ? 1 : 0;
     __syncthreads();
     if (!*am_last) return;
+    ShipPre pre;
+    if (preship && w == 0) ship_pre_get(V->shpre, pre);   // in flight with the totals' loads
     // totals, in every wave
     float tx, ty, tm;
-    wave_tree_global3(V->part1, V->part1 + nbpad, V->part1 + 2 * nbpad, nbpad, tx, ty, tm);
+    wave_tree_global3(V->part1, V->part1 + nbpad, V->part1 + 2 * nbpad, nbpad, tx, ty, tm, live);
     float scale = 1.0f, fx = tx, fy = ty;
     if (P.has_thrust_cap) {
         const float F_tot = sqrtf(tx * tx + ty * ty);
@@ -865,14 +970,14 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
         stores_done();
         __syncthreads();
         float unused;
-        wave_tree_global3(V->part2, V->part2 + nbpad, V->part2, nbpad, fx, fy, unused);
+        wave_tree_global3(V->part2, V->part2 + nbpad, V->part2, nbpad, fx, fy, unused, live);
     }
     // the ship step on wave 0 while the other waves update floe eligibility
     // (independent: neither reads what the other writes)
     if (w == 0) {
         if (threadIdx.x == 0) {
             G(V->scal)[0] = scale;
-            ship_frame(V, D, k, last, fx, fy, tm);
+            ship_frame(V, D, k, last, fx, fy, tm, preship ? &pre : nullptr);
             store_pose(V, D->s.d_ship);
             *near_count(V) = 0;
         }
@@ -1879,9 +1984,10 @@ __device__ __forceinline__ void force_kernel_body(const DevView* __restrict__ V,
                                                   int k, int sub, int src_is_state, int mode, int set, int last,
                                                   int nextra, int lb, int lg, bool xcd,
                                                   const float* __restrict__ fship_in, float* __restrict__ fout,
-                                                  float* __restrict__ mout, ForceLds<GR>& L, int* am_last) {
+                                                  float* __restrict__ mout, ForceLds<GR>& L, int* am_last,
+                                                  bool preship = false) {
     if (lb < nextra) {
-        force_tail(V, D, k, last, lb, nextra, am_last, set);
+        force_tail(V, D, k, last, lb, nextra, am_last, set, preship);
         return;
     }
     const int nf = lg - nextra;
@@ -1897,11 +2003,11 @@ __global__ __launch_bounds__(BLOCK) void k_force(const DevView* __restrict__ V, 
                                                  int k, int sub, int src_is_state, int mode, int set, int last,
                                                  int nextra,
                                                  const float* __restrict__ fship_in,
-                                                 float* __restrict__ fout, float* __restrict__ mout) {
+                                                 float* __restrict__ fout, float* __restrict__ mout, int preship) {
     __shared__ ForceLds<GR> L;
     __shared__ int am_last;
     force_kernel_body<GR>(V, D, k, sub, src_is_state, mode, set, last, nextra, (int)blockIdx.x, (int)gridDim.x, true,
-                      fship_in, fout, mout, L, &am_last);
+                      fship_in, fout, mout, L, &am_last, preship != 0);
 }
 
 // ── k_post: fracture (jax_simulation.py:534-656), commit, histories, next cull ─
@@ -2103,10 +2209,16 @@ __device__ __forceinline__ void post_block(const DevView* __restrict__ V, const 
     }
 }
 
+// shipblk: the grid's last workgroup forms frame k + 1's ship_pre_block
 __global__ __launch_bounds__(BLOCK) void k_post(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                                int k, int final_in_nxt, int last, int set, int cull, int nbe) {
+                                                int k, int final_in_nxt, int last, int set, int cull, int nbe,
+                                                int shipblk) {
     __shared__ NearLds nl;
-    post_block(V, D, k, final_in_nxt, last, blockIdx.x, nbe, set, cull, (int)gridDim.x - nbe, true, &nl);
+    if (shipblk && (int)blockIdx.x == (int)gridDim.x - 1) {
+        ship_pre_block(V, D, k + 1);
+        return;
+    }
+    post_block(V, D, k, final_in_nxt, last, blockIdx.x, nbe, set, cull, (int)gridDim.x - shipblk - nbe, true, &nl);
 }
 
 // ── batched frames: the same kernels over many contexts in one launch ───────

@@ -644,12 +644,12 @@ static int alloc_state_buffers(IceDemCtx* ctx) {
     AL(soi_x, float, n); AL(soi_y, float, n); AL(mterm, float, n); AL(fm, float, n);
     AL(shipf, float, 2 * (size_t)n);
     AL(nearf, uint8_t, n); AL(near_list, int32_t, n);
-    AL(nearcnt, int32_t, nblk); AL(counters, int32_t, 4);
+    AL(nearcnt, int32_t, nbpad); AL(counters, int32_t, 4);
     AL(crack, uint8_t, n);
     AL(cells, uint8_t, (size_t)(H.coarse_off + 2 * (int64_t)H.G2 * H.rec_c));   // fine sets, then coarse
     AL(ovf_list, int32_t, 2 * (size_t)H.ovf_cap); AL(ovf_count, int32_t, 2);
     AL(floe_coll, uint32_t, 2 * (size_t)H.nf); AL(floe_maxfm, uint32_t, 2 * (size_t)H.nf);
-    AL(part1, float, 3 * (size_t)nbpad); AL(part2, float, 2 * (size_t)nbpad); AL(scal, float, 16);
+    AL(part1, float, 3 * (size_t)nbpad); AL(part2, float, 2 * (size_t)nbpad); AL(scal, float, 16); AL(shpre, float, 32);
     AL(nxt_pos, float, 2 * (size_t)n); AL(nxt_vel, float, 2 * (size_t)n);
     AL(nxt_rot, float, n); AL(nxt_omega, float, n);
     AL(err, int32_t, 4);
@@ -662,6 +662,7 @@ static int alloc_state_buffers(IceDemCtx* ctx) {
     HIP_TRY(hipMemset(H.cells, 0, (size_t)(H.coarse_off + 2 * (int64_t)H.G2 * H.rec_c)));
     HIP_TRY(hipMemset(H.ovf_count, 0, sizeof(int32_t) * 2));
     HIP_TRY(hipMemset(H.counters, 0, sizeof(int32_t) * 4));
+    HIP_TRY(hipMemset(H.nearcnt, 0, sizeof(int32_t) * nbpad));   // [nblk, nbpad) stays 0
     HIP_TRY(hipMemset(H.floe_coll, 0, sizeof(uint32_t) * 2 * std::max(H.nf, 1)));
     HIP_TRY(hipMemset(H.floe_maxfm, 0, sizeof(uint32_t) * 2 * std::max(H.nf, 1)));
     HIP_TRY(hipMemset(H.part1, 0, sizeof(float) * 3 * nbpad));
@@ -756,7 +757,11 @@ static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, LaunchTimer*
     const int nbe = (std::max(H.n, std::max(std::max(H.G, H.G2), H.nf)) + BLOCK - 1) / BLOCK;
     const int nbpost = nbe + (H.nb + BLOCK - 1) / BLOCK;
     const int gcl = (std::max(H.G, H.G2) + BLOCK - 1) / BLOCK;
-    int rc = launch(lt, 3, k_cull, dim3(nblk), dim3(BLOCK), stream, V, D);
+    // the force-independent part of each frame's ship step is formed by one
+    // extra workgroup of the kernel before the frame's force pass (not with
+    // the ice brake, whose setpoint depends on the frame's ice force)
+    const int sb = (H.p.ship_mode == 1 || H.p.ice_brake_threshold <= 0.0f) ? 1 : 0;
+    int rc = launch(lt, 3, k_cull, dim3(nblk + sb), dim3(BLOCK), stream, V, D, sb);
     for (int k = 0; k < K && rc == ICEDEM_OK; ++k) {
         const int last = (k == K - 1) ? 1 : 0;
         const int set = k & 1;
@@ -772,10 +777,11 @@ static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, LaunchTimer*
             const int extra = (s == 0) ? nextra : 0;
             rc |= launch(lt, 1, H.lanes == 8 ? k_force<8> : k_force<16>, dim3(fblk + extra), dim3(BLOCK), stream,
                          V, D, k, s, (s % 2 == 0) ? 1 : 0, 0, set, last, extra,
-                         (const float*)nullptr, (float*)nullptr, (float*)nullptr);
+                         (const float*)nullptr, (float*)nullptr, (float*)nullptr, sb);
         }
-        rc |= launch(lt, 2, k_post, dim3(nbpost), dim3(BLOCK), stream, V, D, k, (steps % 2 == 1) ? 1 : 0, last,
-                     set, last ? 0 : 1, nbe);
+        const int psb = last ? 0 : sb;
+        rc |= launch(lt, 2, k_post, dim3(nbpost + psb), dim3(BLOCK), stream, V, D, k, (steps % 2 == 1) ? 1 : 0, last,
+                     set, last ? 0 : 1, nbe, psb);
     }
     if (rc) { set_err(ctx, "event creation failed"); return ICEDEM_E_HIP; }
     HIP_TRY(hipGetLastError());
@@ -1135,7 +1141,7 @@ int icedem_substep_forces(IceDemCtx* ctx, const IceDemState* state, const float*
     hipLaunchKernelGGL(k_cellkey, dim3(H.nblk), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, (const float*)nullptr,
                        (const float*)nullptr, 1, 0);
     hipLaunchKernelGGL(H.lanes == 8 ? k_force<8> : k_force<16>, dim3(fblk), dim3(BLOCK), 0, stream, ctx->dv, ctx->dd, 0, 0, 1, 1, 0, 0, 0,
-                       d_ship_forces, d_forces, d_moments);
+                       d_ship_forces, d_forces, d_moments, 0);
     hipLaunchKernelGGL(k_clear_cells, dim3((std::max(H.G, H.G2) + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, ctx->dv, 0);
     HIP_TRY(hipGetLastError());
     return ICEDEM_OK;

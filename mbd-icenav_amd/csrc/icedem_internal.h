@@ -60,7 +60,7 @@ struct DevView {
     float* pen; float* nx; float* ny; float* fm_raw; float* soi_x; float* soi_y; float* mterm; float* fm;
     uint8_t* nearf;           // (n) element passed the hull cull this frame
     int32_t* near_list;       // (n) queue of near elements
-    int32_t* nearcnt;         // (nblk) near elements of each 256-element block
+    int32_t* nearcnt;         // (nbpad) near elements of each 256-element block, 0 past nblk
     int32_t* counters;        // [0] near elements, [3] k_force tail ticket ([1], [2] unused)
     // bond forces formed once per bond by k_post for the next frame (one ice
     // sub-step per frame): (Fx, Fy, torque, intact) without the activity factor
@@ -100,6 +100,7 @@ struct DevView {
     float* part1;             // (3 * nbpad) block partials: sum soi_x, soi_y, moment
     float* part2;             // (2 * nbpad) block partials: sum of scaled soi
     float* scal;              // [0] thrust scale of the frame, [8..11] sin/cos of -heading and heading
+    float* shpre;             // (32) the force-independent part of the frame's ship step
     float* nxt_pos; float* nxt_vel; float* nxt_rot; float* nxt_omega;
     int32_t* err;
 };

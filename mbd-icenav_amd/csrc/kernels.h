@@ -457,32 +457,39 @@ __device__ __forceinline__ float jremf(float x, float y) {
     return r;
 }
 
-// ship_step_jax (jax_vessel_dynamics.py:246-348); ship = eta[3] nu[3] n[6] e_int[3].
-// ship_mode 1 (rollouts; no reference counterpart): the thrust is given
-// directly, tau = [u X_max, 0, Kd[2,2] (r_cmd r_max - r)] with u, r_cmd in
-// [-1, 1]; the Euler step and the velocity clip are the reference's.
-__device__ __forceinline__ void ship_step(const IceDemParams& P, float* ship, const float* sp, const float* spn,
-                                          const float* tau_ice, float vwx, float vwy,
-                                          float ctrl_u = 0.0f, float ctrl_r = 0.0f) {
+// The part of the ship step that does not depend on the ice forces: DP
+// control or direct control, allocation and propeller lag, the current in
+// body axes and its damping term (every value ship_post needs).
+struct ShipPre {
+    float eta[3], nu[3];
+    float s, c;                  // sin, cos of the heading
+    float tau_act[3];            // actuator forces (body)
+    float dn[3];                 // D3 (nu - nu_current)
+    float nnew[6], e_new[3];     // propeller states, DP integral
+};
+
+__device__ __forceinline__ void ship_pre(const IceDemParams& P, const float* ship, const float* sp, const float* spn,
+                                         float vwx, float vwy, float ctrl_u, float ctrl_r, ShipPre& o) {
     const float PI_F = (float)3.141592653589793;        // float32(pi)
     const float TWO_PI_F = (float)6.283185307179586;    // float32(2 pi)
     const float dt = P.dt;
-    float eta[3] = {ship[0], ship[1], ship[2]};
-    float nu[3] = {ship[3], ship[4], ship[5]};
+    float* eta = o.eta;
+    float* nu = o.nu;
+    for (int i = 0; i < 3; ++i) { eta[i] = ship[i]; nu[i] = ship[3 + i]; }
     float nact[6];
     for (int i = 0; i < 6; ++i) nact[i] = ship[6 + i];
     float e_int[3] = {ship[12], ship[13], ship[14]};
     float s, c;
     dm_sincosf(eta[2], &s, &c);
+    o.s = s; o.c = c;
     // R^T (row-major)
     float RT[9] = {c, s, 0.0f, -s, c, 0.0f, 0.0f, 0.0f, 1.0f};
     float vw3[3] = {vwx, vwy, 0.0f};
-    float nu_c[3], tib[3];
+    float nu_c[3];
     mv3(RT, vw3, nu_c);
-    mv3(RT, tau_ice, tib);
-    float tau_act[3];
-    float nnew[6];
-    float e_new[3];
+    float* tau_act = o.tau_act;
+    float* nnew = o.nnew;
+    float* e_new = o.e_new;
     if (P.ship_mode == 1) {
         const float u = vclip(ctrl_u, -1.0f, 1.0f), rc = vclip(ctrl_r, -1.0f, 1.0f);
         tau_act[0] = u * P.ctrl_thrust_max;
@@ -529,26 +536,51 @@ __device__ __forceinline__ void ship_step(const IceDemParams& P, float* ship, co
             for (int i = 0; i < 6; ++i) nnew[i] = nact[i];
         }
     }
-    float tot[3] = {tau_act[0] + tib[0], tau_act[1] + tib[1], tau_act[2] + tib[2]};
-    // ship_euler_step_jax (:54-84)
+    // ship_euler_step_jax (:54-84): the damping of the velocity relative to
+    // the current
     float nur[3] = {nu[0] - nu_c[0], nu[1] - nu_c[1], nu[2] - nu_c[2]};
-    float dn[3], rhs[3], nud2[3];
-    mv3(P.D3, nur, dn);
-    for (int i = 0; i < 3; ++i) rhs[i] = tot[i] - dn[i];
+    mv3(P.D3, nur, o.dn);
+}
+
+// The rest of the ship step once the ice force tau_ice (world frame) is
+// known: its body-frame value, the Euler step and the clips.
+__device__ __forceinline__ void ship_post(const IceDemParams& P, const ShipPre& o, const float* tau_ice, float* ship) {
+    const float PI_F = (float)3.141592653589793;        // float32(pi)
+    const float TWO_PI_F = (float)6.283185307179586;    // float32(2 pi)
+    const float dt = P.dt;
+    const float s = o.s, c = o.c;
+    float RT[9] = {c, s, 0.0f, -s, c, 0.0f, 0.0f, 0.0f, 1.0f};
+    float tib[3];
+    mv3(RT, tau_ice, tib);
+    float tot[3] = {o.tau_act[0] + tib[0], o.tau_act[1] + tib[1], o.tau_act[2] + tib[2]};
+    float rhs[3], nud2[3];
+    for (int i = 0; i < 3; ++i) rhs[i] = tot[i] - o.dn[i];
     mv3(P.M3inv, rhs, nud2);
     float nun[3];
-    for (int i = 0; i < 3; ++i) nun[i] = nu[i] + dt * nud2[i];
+    for (int i = 0; i < 3; ++i) nun[i] = o.nu[i] + dt * nud2[i];
     float vx = c * nun[0] - s * nun[1];
     float vy = s * nun[0] + c * nun[1];
-    float psi_new = jremf(eta[2] + dt * nun[2] + PI_F, TWO_PI_F) - PI_F;
-    ship[0] = eta[0] + dt * vx;
-    ship[1] = eta[1] + dt * vy;
+    float psi_new = jremf(o.eta[2] + dt * nun[2] + PI_F, TWO_PI_F) - PI_F;
+    ship[0] = o.eta[0] + dt * vx;
+    ship[1] = o.eta[1] + dt * vy;
     ship[2] = psi_new;
     ship[3] = vclip(nun[0], -P.vc_surge, P.vc_surge);
     ship[4] = vclip(nun[1], -P.vc_sway, P.vc_sway);
     ship[5] = vclip(nun[2], -P.vc_yaw, P.vc_yaw);
-    for (int i = 0; i < 6; ++i) ship[6 + i] = nnew[i];
-    for (int i = 0; i < 3; ++i) ship[12 + i] = e_new[i];
+    for (int i = 0; i < 6; ++i) ship[6 + i] = o.nnew[i];
+    for (int i = 0; i < 3; ++i) ship[12 + i] = o.e_new[i];
+}
+
+// ship_step_jax (jax_vessel_dynamics.py:246-348); ship = eta[3] nu[3] n[6] e_int[3].
+// ship_mode 1 (rollouts; no reference counterpart): the thrust is given
+// directly, tau = [u X_max, 0, Kd[2,2] (r_cmd r_max - r)] with u, r_cmd in
+// [-1, 1]; the Euler step and the velocity clip are the reference's.
+__device__ __forceinline__ void ship_step(const IceDemParams& P, float* ship, const float* sp, const float* spn,
+                                          const float* tau_ice, float vwx, float vwy,
+                                          float ctrl_u = 0.0f, float ctrl_r = 0.0f) {
+    ShipPre o;
+    ship_pre(P, ship, sp, spn, vwx, vwy, ctrl_u, ctrl_r, o);
+    ship_post(P, o, tau_ice, ship);
 }
 
 // ── grid cell of a position ──────────────────────────────────────────────────
