@@ -512,19 +512,52 @@ __device__ __forceinline__ float wave_tree_global(GAS const float* __restrict__ 
 // error, and only which elements run the exact hull test depends on it.
 // (r, bnd: the element's radius and bound radius, read by the caller with
 // its state; queue: list the near elements for a k_near launch)
+// The near test of an element at (x, y) with bound radius bnd against the
+// ship at ship[0..2]: its centroid within bnd + margin of the hull's box in
+// the ship's frame (margin 1 m: the exact test; larger: a candidate test).
+__device__ __forceinline__ bool near_box(const IceDemParams& P, float x, float y, float bnd, GAS const float* ship,
+                                         float margin) {
+    float sh, ch;
+    __sincosf(ship[2], &sh, &ch);
+    const float dx = x - ship[0], dy = y - ship[1];
+    const float u = ch * dx + sh * dy, v = ch * dy - sh * dx;
+    const float b = bnd + margin;
+    return fabsf(u) <= P.ship_box_half_len + b && fabsf(v) <= P.ship_box_half_beam + b;
+}
+
+// near candidates of the frames of parity p: count counters[1 + p], list
+// cand[p n ..], flag cflag[p n + e]
+__device__ __forceinline__ int32_t* cand_count(const DevView* V, int p) { return V->counters + 1 + (p & 1); }
+
+// cpar >= 0: also list the elements that pass the near test with margin
+// cand_margin as the candidates of the near test one frame later (parity
+// cpar); the margin covers one frame's largest displacement of an element
+// and of the hull's box, so the candidates hold that test's near elements
+// (an element that is near but not listed is caught where it is culled)
 __device__ __forceinline__ int cull_chunk(const DevView* __restrict__ V, int chunk, bool valid, int e,
                                           float x, float y, float vx, float vy, float r, float bnd,
                                           GAS const float* ship, int set, bool& near, bool queue,
-                                          bool insert = true, bool slot = true) {
+                                          bool insert = true, bool slot = true, int cpar = -1) {
     const IceDemParams& P = V->p;
     near = false;
+    bool cnear = false;
     if (valid) {
-        float sh, ch;
-        __sincosf(ship[2], &sh, &ch);
-        const float dx = x - ship[0], dy = y - ship[1];
-        const float u = ch * dx + sh * dy, v = ch * dy - sh * dx;
-        const float b = bnd + 1.0f;
-        near = fabsf(u) <= P.ship_box_half_len + b && fabsf(v) <= P.ship_box_half_beam + b;
+        near = near_box(P, x, y, bnd, ship, 1.0f);
+        if (cpar >= 0) cnear = near_box(P, x, y, bnd, ship, V->cand_margin);
+    }
+    if (cpar >= 0) {
+        const uint64_t cb = __ballot(cnear);
+        if (cb != 0ull) {
+            const int lane = threadIdx.x & 63;
+            const int cf = __ffsll((unsigned long long)cb) - 1;
+            int cbase = 0;
+            if (lane == cf) cbase = atomicAdd(cand_count(V, cpar), __popcll(cb));
+            cbase = __shfl(cbase, cf, 64);
+            if (cnear)
+                G(V->cand)[(size_t)(cpar & 1) * V->n + cbase +
+                           __builtin_amdgcn_mbcnt_hi((uint32_t)(cb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)cb, 0u))] = e;
+        }
+        if (valid) G(V->cflag)[(size_t)(cpar & 1) * V->n + e] = cnear ? 1 : 0;
     }
     // the wave's near elements take consecutive queue entries (one atomic per
     // wave; queue order does not matter), and both atomics are issued before
@@ -722,7 +755,8 @@ __device__ __forceinline__ void cull_kernel_body(const DevView* __restrict__ V, 
         r = G(V->radius)[e]; bnd = G(V->bound)[e];
     }
     bool near;
-    const int cnt = cull_chunk(V, lb, valid, e, x, y, vx, vy, r, bnd, G(D->s.d_ship), 0, near, nl == nullptr);
+    const int cnt = cull_chunk(V, lb, valid, e, x, y, vx, vy, r, bnd, G(D->s.d_ship), 0, near, nl == nullptr, true,
+                               true, nl ? 1 : -1);
     if (nl) near_inline(V, D, nl, cnt, near, e, x, y, near ? G(D->s.d_rot)[e] : 0.0f, 0, true);
     if (lb == 0 && threadIdx.x == 0) store_pose(V, D->s.d_ship);
     // a call starts with a list build (the caller may have changed the state)
@@ -980,6 +1014,7 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
             ship_frame(V, D, k, last, fx, fy, tm, preship ? &pre : nullptr);
             store_pose(V, D->s.d_ship);
             *near_count(V) = 0;
+            *cand_count(V, set) = 0;   // consumed by the previous k_post, refilled by this frame's
         }
     } else {
         floe_eligibility(V, D, scale, set, 1);
@@ -2134,14 +2169,24 @@ __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const C
     GAS const float* pos = G(final_in_nxt ? V->nxt_pos : D->s.d_pos);
     GAS const float* vel = G(final_in_nxt ? V->nxt_vel : D->s.d_vel);
     float px = 0.0f, py = 0.0f, pvx = 0.0f, pvy = 0.0f, ro = 0.0f, r = 0.0f, bnd = 0.0f;
+    // the list control words, read beside the element's state
+    bool used = true, ins = true, built = false;
+    if (lists) {
+        const bool ovf = nl_overflowed(V);
+        built = nl_rebuild(V, k);
+        used = ovf || built;
+        ins = ovf || nl_rebuild(V, k + 1);
+    }
     // a frozen element's next state is a copy of its state (the integration
     // keeps it), and its partner-list slot is the one the call's first cull
     // wrote: neither is written again
-    bool frozen = false;
+    bool frozen = false, listed = false;
     if (t < n) {
         px = pos[t]; py = pos[n + t];
         pvx = vel[t]; pvy = vel[n + t];
         if (cull) { r = G(V->radius)[t]; bnd = G(V->bound)[t]; }   // statics for the cull, read with the state
+        // with nl: whether the next frame's near candidates list the element
+        if (cull && nl) listed = G(V->cflag)[(size_t)(set ^ 1) * n + t] != 0;
         frozen = G(D->s.d_active)[t] == 0;
         if (final_in_nxt && !frozen) {
             G(D->s.d_pos)[t] = px; G(D->s.d_pos)[n + t] = py;
@@ -2160,12 +2205,7 @@ __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const C
             if (D->dg.d_force_mag) G(D->dg.d_force_mag)[t] = fm_scaled(V, t);
         }
     }
-    bool used = true, ins = true;
     if (lists) {
-        const bool ovf = nl_overflowed(V);
-        const bool built = nl_rebuild(V, k);
-        used = ovf || built;
-        ins = ovf || nl_rebuild(V, k + 1);
         if (t == 0) {
             #pragma unroll
             for (int i = 0; i < NLSUB; ++i)
@@ -2176,11 +2216,44 @@ __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const C
     if (used) clear_cells(V, set, t);
     if (t < V->nf) { G(floe_coll_of(V, set))[t] = 0u; G(floe_maxfm_of(V, set))[t] = 0u; }
     if (t == 0) *force_ticket(V) = 0;
+    // after a call's last frame no candidate list is pending
+    if (last && t == 0) { *cand_count(V, 0) = 0; *cand_count(V, 1) = 0; }
     if (cull && chunk < V->nblk) {
         bool near;
-        const int cnt = cull_chunk(V, chunk, t < n, t, px, py, pvx, pvy, r, bnd, G(D->s.d_ship), set ^ 1, near,
-                                   nl == nullptr, ins, !frozen);
-        if (nl) near_inline(V, D, nl, cnt, near, t, px, py, ro, set ^ 1, false);
+        (void)cull_chunk(V, chunk, t < n, t, px, py, pvx, pvy, r, bnd, G(D->s.d_ship), set ^ 1, near,
+                         nl == nullptr, ins, !frozen, nl ? set : -1);
+        if (nl) {
+            // the candidates' near tests run in the kernel's near workgroups;
+            // a near element they miss takes its test here
+            const bool miss = near && !listed;
+            const int cm = __syncthreads_count(miss ? 1 : 0);
+            near_inline(V, D, nl, cm, miss, t, px, py, ro, set ^ 1, false);
+        }
+    }
+}
+
+// Near workgroup lb of lg of a single field's k_post: one wave per listed
+// candidate of the next frame's near test (parity par), with the state this
+// frame committed; a candidate passing the exact test (the cull's, with the
+// same inputs) takes its hull test and pre-pass.  Runs from the kernel's
+// start, beside the cull.
+__device__ __forceinline__ void near_cand_block(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                                int lb, int lg, int final_in_nxt, int par) {
+    const int n = V->n;
+    const int wave = lb * (BLOCK / 64) + (threadIdx.x >> 6);
+    const int nwaves = lg * (BLOCK / 64);
+    GAS const int32_t* cl = G(V->cand) + (size_t)(par & 1) * n;
+    const int count = *G(cand_count(V, par));
+    const int e0 = (wave < n) ? cl[wave] : 0;   // read beside the count
+    if (wave >= count) return;
+    GAS const float* ship = G(D->s.d_ship);
+    const ShipPose SP = stored_pose(V, D);
+    GAS const float* pos = G(final_in_nxt ? V->nxt_pos : D->s.d_pos);
+    GAS const float* rotp = G(final_in_nxt ? V->nxt_rot : D->s.d_rot);
+    for (int q = wave; q < count; q += nwaves) {
+        const int e = (q == wave) ? e0 : cl[q];
+        const float x = pos[e], y = pos[n + e], ro = rotp[e], bnd = G(V->bound)[e];
+        if (near_box(V->p, x, y, bnd, ship, 1.0f)) near_elem_at(V, SP, e, x, y, ro, par);
     }
 }
 
@@ -2209,16 +2282,23 @@ __device__ __forceinline__ void post_block(const DevView* __restrict__ V, const 
     }
 }
 
-// shipblk: the grid's last workgroup forms frame k + 1's ship_pre_block
+// shipblk: the grid's last workgroup forms frame k + 1's ship_pre_block;
+// nnw: the first nnw workgroups (a multiple of 8, so the others keep their
+// XCD dealing) run the next frame's near tests over its candidates
 __global__ __launch_bounds__(BLOCK) void k_post(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                                 int k, int final_in_nxt, int last, int set, int cull, int nbe,
-                                                int shipblk) {
+                                                int shipblk, int nnw) {
     __shared__ NearLds nl;
     if (shipblk && (int)blockIdx.x == (int)gridDim.x - 1) {
         ship_pre_block(V, D, k + 1);
         return;
     }
-    post_block(V, D, k, final_in_nxt, last, blockIdx.x, nbe, set, cull, (int)gridDim.x - shipblk - nbe, true, &nl);
+    if ((int)blockIdx.x < nnw) {
+        near_cand_block(V, D, blockIdx.x, nnw, final_in_nxt, set ^ 1);
+        return;
+    }
+    post_block(V, D, k, final_in_nxt, last, (int)blockIdx.x - nnw, nbe, set, cull,
+               (int)gridDim.x - shipblk - nnw - nbe, true, &nl);
 }
 
 // ── batched frames: the same kernels over many contexts in one launch ───────

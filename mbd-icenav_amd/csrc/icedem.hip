@@ -498,6 +498,19 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
         H.nl_thr2 = H.nl_on ? thr * thr : 0.0f;
     }
     H.bf_on = (prm->n_ice_steps == 1 && st->n_bonds > 0) ? 1 : 0;
+    {   // near candidates: the near test's 1 m margin plus one frame's largest
+        // element displacement (velocity clipped per axis), the hull box's
+        // shift at the clipped ship speed and the turn of its far corner at
+        // the clipped yaw rate, and 0.15 m (the ship's unclipped velocity of a
+        // step); a near element outside the candidates is still tested
+        double maxb = 0.0;
+        for (int e = 0; e < n; ++e) maxb = std::max(maxb, (double)st->h_bound_radius[e]);
+        const double R = std::hypot((double)prm->ship_box_half_len + maxb + 1.0,
+                                    (double)prm->ship_box_half_beam + maxb + 1.0);
+        H.cand_margin = (float)(1.0 + (double)prm->dt * ((double)prm->velocity_clip * std::sqrt(2.0) +
+                                                         std::hypot((double)prm->vc_surge, (double)prm->vc_sway) +
+                                                         (double)prm->vc_yaw * R) + 0.15);
+    }
     if (prm->grid_bucket < 1 || (int64_t)H.G * prm->grid_bucket > (1LL << 28)) {
         set_err(nullptr, "bad grid_bucket"); delete ctx; return ICEDEM_E_INVALID;
     }
@@ -644,6 +657,7 @@ static int alloc_state_buffers(IceDemCtx* ctx) {
     AL(soi_x, float, n); AL(soi_y, float, n); AL(mterm, float, n); AL(fm, float, n);
     AL(shipf, float, 2 * (size_t)n);
     AL(nearf, uint8_t, n); AL(near_list, int32_t, n);
+    AL(cand, int32_t, 2 * (size_t)n); AL(cflag, uint8_t, 2 * (size_t)n);
     AL(nearcnt, int32_t, nbpad); AL(counters, int32_t, 4);
     AL(crack, uint8_t, n);
     AL(cells, uint8_t, (size_t)(H.coarse_off + 2 * (int64_t)H.G2 * H.rec_c));   // fine sets, then coarse
@@ -663,6 +677,7 @@ static int alloc_state_buffers(IceDemCtx* ctx) {
     HIP_TRY(hipMemset(H.ovf_count, 0, sizeof(int32_t) * 2));
     HIP_TRY(hipMemset(H.counters, 0, sizeof(int32_t) * 4));
     HIP_TRY(hipMemset(H.nearcnt, 0, sizeof(int32_t) * nbpad));   // [nblk, nbpad) stays 0
+    HIP_TRY(hipMemset(H.cflag, 0, 2 * (size_t)n));
     HIP_TRY(hipMemset(H.floe_coll, 0, sizeof(uint32_t) * 2 * std::max(H.nf, 1)));
     HIP_TRY(hipMemset(H.floe_maxfm, 0, sizeof(uint32_t) * 2 * std::max(H.nf, 1)));
     HIP_TRY(hipMemset(H.part1, 0, sizeof(float) * 3 * nbpad));
@@ -761,6 +776,9 @@ static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, LaunchTimer*
     // extra workgroup of the kernel before the frame's force pass (not with
     // the ice brake, whose setpoint depends on the frame's ice force)
     const int sb = (H.p.ship_mode == 1 || H.p.ice_brake_threshold <= 0.0f) ? 1 : 0;
+    // k_post's near workgroups (a multiple of 8): one wave per candidate,
+    // at least one candidate per 64 elements' worth of waves
+    const int nnw = std::max(8, std::min(64, ((nblk + 7) / 8) * 8));
     int rc = launch(lt, 3, k_cull, dim3(nblk + sb), dim3(BLOCK), stream, V, D, sb);
     for (int k = 0; k < K && rc == ICEDEM_OK; ++k) {
         const int last = (k == K - 1) ? 1 : 0;
@@ -780,8 +798,9 @@ static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, LaunchTimer*
                          (const float*)nullptr, (float*)nullptr, (float*)nullptr, sb);
         }
         const int psb = last ? 0 : sb;
-        rc |= launch(lt, 2, k_post, dim3(nbpost + psb), dim3(BLOCK), stream, V, D, k, (steps % 2 == 1) ? 1 : 0, last,
-                     set, last ? 0 : 1, nbe, psb);
+        const int pnw = last ? 0 : nnw;
+        rc |= launch(lt, 2, k_post, dim3(pnw + nbpost + psb), dim3(BLOCK), stream, V, D, k,
+                     (steps % 2 == 1) ? 1 : 0, last, set, last ? 0 : 1, nbe, psb, pnw);
     }
     if (rc) { set_err(ctx, "event creation failed"); return ICEDEM_E_HIP; }
     HIP_TRY(hipGetLastError());
