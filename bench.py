@@ -36,7 +36,10 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "DEM element-steps/s; planner rollouts scored/s, surrogate vs plain DEM, ≤5% cost err"
 HBM_PEAK_GBS = 8000.0
-KERNELS = ["k_near", "k_force", "k_post", "k_cull"]   # per frame, per frame, per frame, per call
+# categories of icedem_frames_profiled: a separate near-test kernel (batched
+# rollouts only; a single field tests its near elements in k_cull and
+# k_post), the force pass, the post pass (per frame), the call's first cull
+KERNELS = ["k_near", "k_force", "k_post", "k_cull"]
 
 
 def build_case(config: str, seed: int, cache_dir: str = ""):
@@ -68,7 +71,11 @@ def build_case(config: str, seed: int, cache_dir: str = ""):
 
 def algorithmic_bytes(kernel: str, n: int, nb: int, nf: int, G: int, n_near: int, max_v: int, n_hull: int,
                       near_blocks: int, nbpad: int, listed: int = -1) -> float:
-    """Bytes each kernel must move per launch, every array counted once.
+    """Bytes of the reference algorithm's share of each kernel per launch,
+    every array counted once: the reference's step reads and writes every
+    element's state every frame, frozen or not, so these count every element
+    (the kernels skip what a frozen element leaves unchanged; the measured
+    HBM-side traffic beside this figure shows what they move).
 
     k_force on a frame that uses the partner lists (listed >= 0: partners
     listed for the active elements; a frozen element forms no contact sum),
@@ -78,8 +85,7 @@ def algorithmic_bytes(kernel: str, n: int, nb: int, nf: int, G: int, n_near: int
     pos/vel/rot/omega 24, active 1, sub 4, force_mag 4, crack 1 -> 103; per
     listed partner its index 4 and, at most once per element, its slot
     (x, y, vx, vy, r) 24.  Per bond: two table entries of 16 B and the bond's
-    force record (Fx, Fy, torque, intact) 16 -> 48 (a frozen element reads
-    its entries for the crack flag).  Floe flags 4 per floe; the tail's trees
+    force record (Fx, Fy, torque, intact) 16 -> 48.  Floe flags 4 per floe; the tail's trees
     as below.
     k_force scanning the grids, per element: state reads pos 8, vel 8, rot 4, omega 4, active 1,
     sub 4; static reads radius, thickness, area, mass, inertia, floe, bond
@@ -96,7 +102,8 @@ def algorithmic_bytes(kernel: str, n: int, nb: int, nf: int, G: int, n_near: int
     state 24 written, position/rotation/submergence history 16 (+ submergence
     read 4), force_mag 4, crack 1, active 1, elig 1, cull reads radius/bound
     8, near flag 1, bucket slot 24 and count 4 -> 112; cell clears 4 per
-    cell, floe clears 8 per floe.
+    cell, floe clears 8 per floe; and the next frame's near tests (the k_near
+    terms).
     """
     if kernel == "k_force" and listed >= 0:
         return (103.0 * n + 24.0 * min(n, listed) + 4.0 * listed + 48.0 * nb + 4.0 * nf
@@ -106,7 +113,7 @@ def algorithmic_bytes(kernel: str, n: int, nb: int, nf: int, G: int, n_near: int
     if kernel == "k_near":
         return n_near * (9.0 * max_v + 60.0) + 8.0 * n_hull
     if kernel == "k_post":
-        return 62.0 * nb + 112.0 * n + 4.0 * G + 8.0 * nf
+        return 62.0 * nb + 112.0 * n + 4.0 * G + 8.0 * nf + n_near * (9.0 * max_v + 60.0) + 8.0 * n_hull
     return 64.0
 
 
@@ -243,9 +250,9 @@ def frame_bench(case_args, config: str, steps: int, warmup: int, chunk: int, pro
     _lib.check(eng.lib.icedem_frames_profiled(eng.ctx, C.byref(s), C.byref(inp), P, None, ms, strm),
                eng.ctx, "profiled")
     torch.cuda.synchronize()
-    avg_us = {k: ms[i] * 1000.0 / P for i, k in enumerate(KERNELS[:3])}
+    avg_us = {k: ms[i] * 1000.0 / P for i, k in enumerate(KERNELS[:3]) if ms[i] > 0.0}
     avg_us["k_cull (once per call)"] = ms[3] * 1000.0
-    dom = max(KERNELS[:3], key=avg_us.get)
+    dom = max((k for k in KERNELS[:3] if k in avg_us), key=avg_us.get)
     G = eng.grid[4] * eng.grid[5]
     n_near, near_blk = eng.near_stats(st, ship)
     nbpad = 1
@@ -256,7 +263,7 @@ def frame_bench(case_args, config: str, steps: int, warmup: int, chunk: int, pro
                                listed=lst["listed_partners_active"] if lst["lists"] else -1)
     achieved = abytes / (avg_us[dom] * 1e-6) / 1e9
     traffic = None
-    tfile = ROOT / "profiles" / f"traffic_{config}.json"
+    tfile = ROOT / "profiles" / f"traffic_{config}{'_all_active' if all_active else ''}.json"
     if tfile.exists():
         try:
             traffic = json.loads(tfile.read_text()).get(dom)
@@ -265,6 +272,10 @@ def frame_bench(case_args, config: str, steps: int, warmup: int, chunk: int, pro
     eng.close()
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                       "bytes_basis": "reference algorithm: every element's state and statics read and its next "
+                                      "state written each frame, frozen or not; traffic is the measured HBM-side "
+                                      "bytes per launch (PMC, copy-calibrated)",
+                       "traffic_rate_GBs": (traffic / (avg_us[dom] * 1e-6) / 1e9) if traffic else None,
                        "kernel_avg_us": avg_us[dom], "algorithmic_bytes": abytes, "kernel_avg_us_all": avg_us,
                        "near_elements": n_near, "mean_listed_partners": (lst["listed_partners"] / n
                                                                           if lst["lists"] else None),
@@ -598,11 +609,12 @@ def surrogate_bench(plain_rps: float, rounds: int = 800, epochs: int = 20, test_
 
 
 def batched_rollouts_only(R: int = 256, frames: int = 400, hold: int = 10, n_knots: int = 40, reps: int = 3,
-                          force_lanes: int = 0, cell_factor: float = 1.0, mode: str = "frames"):
-    """The batched-rollout launch alone (for profiling its kernels)."""
+                          force_lanes: int = 0, cell_factor: float = 1.0, mode: str = "frames", workers: int = 16):
+    """The batched-rollout launch alone (for profiling its kernels; workers 1
+    builds the patches in this process, with no worker pool)."""
     import torch
     from icedem import rollouts as ro
-    scenes = ro.rollout_patches(R, workers=min(16, os.cpu_count() or 1))
+    scenes = ro.rollout_patches(R, workers=max(1, min(workers, os.cpu_count() or 1)))
     rb = ro.RolloutBatch(scenes, list(range(R)), force_lanes=force_lanes, cell_size_factor=cell_factor)
     rb.set_mode(mode)
     ctl = torch.as_tensor(ro.nominal_controls(R, n_knots, seed=0))
@@ -668,6 +680,8 @@ def main():
     ap.add_argument("--profile-frames", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--all-active", action="store_true",
+                    help="start the field with every element active (the force pass's full load)")
     ap.add_argument("--no-extra-config", action="store_true",
                     help="skip the c1 and all-active c2 frame lines next to the headline")
     ap.add_argument("--no-rollouts", action="store_true", help="skip the batched-rollout measurement")
@@ -684,6 +698,8 @@ def main():
                     help="contact partner-list skin in metres (default: the library's; 0: grids every frame)")
     ap.add_argument("--cell-factor", type=float, default=1.0,
                     help="fine grid cell as a multiple of 2 (r_s + delta) (frames) or of the batch's default cell")
+    ap.add_argument("--patch-workers", type=int, default=16,
+                    help="processes building the rollout patches of --rollouts-only (1: no pool)")
     ap.add_argument("--batch-mode", default="frames", choices=["frames", "scenes"],
                     help="batch mode of --rollouts-only")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -700,7 +716,8 @@ def main():
         return
     if args.rollouts_only:
         print(json.dumps(batched_rollouts_only(args.rollouts, force_lanes=args.force_lanes,
-                                               cell_factor=args.cell_factor, mode=args.batch_mode)), flush=True)
+                                               cell_factor=args.cell_factor, mode=args.batch_mode,
+                                               workers=args.patch_workers)), flush=True)
         return
     if args.surrogate_forward:
         print(json.dumps(surrogate_forward_bench(args.surrogate_forward)), flush=True)
@@ -713,10 +730,14 @@ def main():
     import numpy as np
     import torch
 
+    def progress(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
     case_args = build_case(args.config, args.seed, args.scene_cache)
+    progress(f"scene {args.config} ready")
     fb = frame_bench(case_args, args.config, args.steps, args.warmup, args.chunk, args.profile_frames,
                      force_lanes=args.force_lanes, cell_factor=args.cell_factor, use_graph=not args.no_graph,
-                     list_skin=args.list_skin)
+                     list_skin=args.list_skin, all_active=args.all_active)
     sc, cfg = case_args[0], case_args[1]
     n, nb = len(sc.polygons), len(sc.bonds)
     elapsed = fb["elapsed_s"]
@@ -726,6 +747,7 @@ def main():
                           "ms_per_step": 1000.0 * elapsed / args.steps, "roofline": None,
                           "cpu_baseline": None, "note": "per-kernel timing skipped"}), flush=True)
         return
+    progress(f"{args.config}: {1000.0 * elapsed / args.steps:.4f} ms/frame")
     extra_frames = None
     if args.config != "c1" and not args.no_extra_config:
         c1 = frame_bench(build_case("c1", args.seed, args.scene_cache), "c1", 2000, 100, 500, 200)
@@ -736,11 +758,15 @@ def main():
             full = frame_bench(case_args, "c2", 1000, 100, 500, 200, list_skin=args.list_skin, all_active=True)
             extra_frames["c2_all_active"] = {k: full[k] for k in ("workload", "value", "ms_per_step", "steps",
                                                                   "roofline", "partner_lists")}
+        progress("extra frame configs done")
     cpu = None if args.no_cpu_baseline else cpu_baseline(case_args)
+    progress("cpu baseline done")
     rollouts = None if args.no_rollouts else rollout_bench(R=args.rollouts)
+    progress("batched rollouts done")
     surrogate = None
     if not args.no_surrogate and rollouts is not None:
         surrogate = surrogate_bench(rollouts["plain_gpu_rollouts_per_s"])
+        progress("surrogate done")
     value = n * args.steps / elapsed
     out = {
         "metric": METRIC,
