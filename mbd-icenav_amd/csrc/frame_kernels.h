@@ -8,7 +8,7 @@
 //             floe or/max (integer atomics).  No cross-workgroup hand-off.
 //   k_force   16 or 8 lanes per element (k_force<GR>): activation,
 //             submergence, bond forces, contacts, drag, integration (the
-//             integration on one thread per element).  tail_groups(nblk) extra
+//             integration on one thread per element).  TAIL_GROUPS extra
 //             workgroups form the pairwise-tree partials of the near
 //             elements' ship forces, one wave per 256-element block; the last
 //             of them forms the totals, the thrust scale, the scaled totals,
@@ -32,7 +32,6 @@ constexpr int MAXC2 = 16;               // contact partners kept per element (mo
 template <int GR> constexpr int epb() { return BLOCK / GR; }   // elements per k_force block
 
 __device__ __forceinline__ int32_t* near_count(const DevView* V) { return V->counters; }
-__device__ __forceinline__ int32_t* force_ticket(const DevView* V) { return V->counters + 3; }
 // per-floe hull-contact flag and largest contact force of the frames of
 // parity `set` (frame k uses k & 1: one frame's near test can set the next
 // frame's flags while that frame's k_post clears its own)
@@ -68,30 +67,11 @@ __device__ __forceinline__ bool nl_rebuild(const DevView* __restrict__ V, int k)
 __device__ __forceinline__ bool nl_overflowed(const DevView* __restrict__ V) { return *nl_ovf(V) != 0u; }
 
 // extra (tail) workgroups of a frame's first k_force: one wave per 256-element block
-__host__ __device__ constexpr int tail_groups(int nblk) { return (nblk + BLOCK / 64 - 1) / (BLOCK / 64); }
 // element workgroups of k_force for n elements at `lanes` lanes each
 __host__ __device__ constexpr int force_chunks(int n, int lanes) { return (n + BLOCK / lanes - 1) / (BLOCK / lanes); }
 // lanes per element: the requested 8 or 16, else chosen by element count
 __host__ __device__ constexpr int resolve_lanes(int req, int64_t n) {
     return req ? req : (n >= ICEDEM_LANES8_MIN_ELEMENTS ? 8 : 16);
-}
-
-// Hand-offs between workgroups of one kernel (block partials -> the
-// workgroup that draws the last ticket): the producer writes with
-// device-coherent stores (written through past its XCD's L2), waits for them
-// to complete, then takes its ticket; the last-ticket consumer reads with
-// device-coherent loads.  Unlike a device-scope fence this needs no L2
-// write-back or invalidation, which would stall every wave on the XCD.
-__device__ __forceinline__ void st_dev(float* p, float v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ float ld_dev(const float* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void stores_done() {
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
 // ── broad phase: two uniform grids of fixed-capacity buckets ────────────────
@@ -338,25 +318,12 @@ __device__ __forceinline__ float block256_tree(const DevView* __restrict__ V, GA
 // (the order of block256_tree), every load issued before the first add.
 __device__ __forceinline__ void block256_tree3(const DevView* __restrict__ V, GAS const float* __restrict__ a0,
                                                GAS const float* __restrict__ a1, GAS const float* __restrict__ a2,
-                                               int blk, float& s0, float& s1, float& s2, bool dev = false) {
+                                               int blk, float& s0, float& s1, float& s2) {
     const int lane = threadIdx.x & 63;
     const int e0 = blk * BLOCK + lane * 4;
     uint32_t nf = 0;
     float4 x0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), x1 = x0, x2 = x0;
-    if (dev) {   // values written by other workgroups of the running kernel
-        const float *g0 = (const float*)a0, *g1 = (const float*)a1, *g2 = (const float*)a2;
-        float v0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, v1[4] = {0.0f, 0.0f, 0.0f, 0.0f}, v2[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (e0 + i < V->n) {
-                nf |= (uint32_t)G(V->nearf)[e0 + i] << (8 * i);
-                v0[i] = ld_dev(g0 + e0 + i); v1[i] = ld_dev(g1 + e0 + i); v2[i] = ld_dev(g2 + e0 + i);
-            }
-        }
-        x0 = make_float4(v0[0], v0[1], v0[2], v0[3]);
-        x1 = make_float4(v1[0], v1[1], v1[2], v1[3]);
-        x2 = make_float4(v2[0], v2[1], v2[2], v2[3]);
-    } else if (e0 + 3 < V->n) {
+    if (e0 + 3 < V->n) {
         nf = *(GAS const uint32_t*)(G(V->nearf) + e0);
         const f4u y0 = *(GAS const f4u*)(a0 + e0), y1 = *(GAS const f4u*)(a1 + e0), y2 = *(GAS const f4u*)(a2 + e0);
         x0 = make_float4(y0.x, y0.y, y0.z, y0.w);
@@ -384,119 +351,11 @@ __device__ __forceinline__ void block256_tree3(const DevView* __restrict__ V, GA
     s2 = wave_tree_sum(tree4(x2));
 }
 
-// Pairwise tree over a[0..npad) (npad a power of two, at most 4096) by one
-// wave; result in every lane.  Up to 64 entries: one per lane and a butterfly
-// of log2(npad) steps; beyond: each lane reduces its contiguous chunk
-// pairwise (a stack of partial sums per level), then the full butterfly.
 // pairwise tree of one value per lane over lanes [0, npad), npad <= 64 a
 // power of two; result in every lane
 __device__ __forceinline__ float wave_tree_small(float x, int npad) {
     for (int off = 1; off < npad; off <<= 1) x = x + __shfl_xor(x, off, 64);
     return __shfl(x, 0, 64);
-}
-
-// Which entries of the lane's chunk of a block-partial array may be nonzero:
-// bit i for entry lane * c + i (c = npad / 64; npad <= 64: bit 0 for entry
-// lane), set where the 256-element block holds a near element.  A block
-// without one has partials of exactly +0.  nearcnt is zero-padded to npad.
-__device__ __forceinline__ uint64_t near_chunk_bits(const DevView* __restrict__ V, int npad) {
-    const int lane = threadIdx.x & 63;
-    GAS const int32_t* nc = G(V->nearcnt);
-    if (npad <= 64) return (lane < npad && nc[lane] > 0) ? 1ull : 0ull;
-    const int c = npad >> 6;
-    uint64_t bits = 0;
-    if (c == 2) {
-        typedef int gas_i2 __attribute__((ext_vector_type(2)));
-        const gas_i2 v = *(GAS const gas_i2*)(nc + 2 * lane);
-        return (uint64_t)(v.x > 0) | ((uint64_t)(v.y > 0) << 1);
-    }
-    for (int i = 0; i < c; i += 4) {
-        const gas_i4 v = *(GAS const gas_i4*)(nc + (size_t)lane * c + i);
-        bits |= ((uint64_t)(v.x > 0) | ((uint64_t)(v.y > 0) << 1) | ((uint64_t)(v.z > 0) << 2) |
-                 ((uint64_t)(v.w > 0) << 3)) << i;
-    }
-    return bits;
-}
-
-// Pairwise trees of three arrays a0, a1, a2 (npad floats each, npad a power
-// of two) by one wave, written by other workgroups of the running kernel
-// (device-coherent loads of the entries `live` marks, +0 for the others,
-// which hold +0); the order of wave_tree_global.  Up to 64 * TREE3_CHUNK
-// entries every load is issued before the first add; larger arrays take
-// wave_tree_global per array.  Results in every lane.
-constexpr int TREE3_CHUNK = 16;
-__device__ __forceinline__ float wave_tree_global(GAS const float* __restrict__ a, int npad, const float* dev,
-                                                  uint64_t live);
-__device__ __forceinline__ void wave_tree_global3(const float* a0, const float* a1, const float* a2, int npad,
-                                                  float& s0, float& s1, float& s2, uint64_t live) {
-    const int lane = threadIdx.x & 63;
-    if (npad <= 64) {
-        const bool ok = lane < npad && (live & 1ull) != 0ull;
-        const float l0 = ok ? ld_dev(a0 + lane) : 0.0f;
-        const float l1 = ok ? ld_dev(a1 + lane) : 0.0f;
-        const float l2 = ok ? ld_dev(a2 + lane) : 0.0f;
-        s0 = wave_tree_small(l0, npad); s1 = wave_tree_small(l1, npad); s2 = wave_tree_small(l2, npad);
-        return;
-    }
-    const int c = npad >> 6;
-    if (c > TREE3_CHUNK) {
-        s0 = wave_tree_global((GAS const float*)a0, npad, a0, live);
-        s1 = wave_tree_global((GAS const float*)a1, npad, a1, live);
-        s2 = wave_tree_global((GAS const float*)a2, npad, a2, live);
-        return;
-    }
-    float v0[TREE3_CHUNK], v1[TREE3_CHUNK], v2[TREE3_CHUNK];
-    const size_t b = (size_t)lane * c;
-    #pragma unroll
-    for (int i = 0; i < TREE3_CHUNK; ++i) {
-        const bool ok = i < c && ((live >> i) & 1ull) != 0ull;
-        v0[i] = ok ? ld_dev(a0 + b + i) : 0.0f;
-        v1[i] = ok ? ld_dev(a1 + b + i) : 0.0f;
-        v2[i] = ok ? ld_dev(a2 + b + i) : 0.0f;
-    }
-    // the lane's chunk pairwise (c a power of two), then the butterfly
-    #pragma unroll
-    for (int st = 1; st < TREE3_CHUNK; st <<= 1) {
-        if (st < c) {
-            #pragma unroll
-            for (int i = 0; i + st < TREE3_CHUNK; i += 2 * st) {
-                v0[i] = v0[i] + v0[i + st]; v1[i] = v1[i] + v1[i + st]; v2[i] = v2[i] + v2[i + st];
-            }
-        }
-    }
-    s0 = wave_tree_sum(v0[0]); s1 = wave_tree_sum(v1[0]); s2 = wave_tree_sum(v2[0]);
-}
-
-// dev: the same array as a generic pointer, read with device-coherent loads
-// (values written by other workgroups of the running kernel)
-__device__ __forceinline__ float wave_tree_global(GAS const float* __restrict__ a, int npad,
-                                                  const float* dev, uint64_t live) {
-    const int lane = threadIdx.x & 63;
-    if (npad <= 64)
-        return wave_tree_small((lane < npad && (live & 1ull)) ? (dev ? ld_dev(dev + lane) : a[lane]) : 0.0f, npad);
-    const int c = npad >> 6;
-    GAS const float* p = a + (size_t)lane * c;
-    const float* pd = dev ? dev + (size_t)lane * c : nullptr;
-    float stk[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    #pragma unroll
-    for (int i = 0; i < 64; ++i) {
-        if (i < c) {
-            const bool ok = ((live >> i) & 1ull) != 0ull;
-            float x = ok ? (pd ? ld_dev(pd + i) : p[i]) : 0.0f;
-            int lvl = 0;
-            #pragma unroll
-            for (int t = 0; t < 6; ++t) {
-                if (((i >> t) & 1) == 0) break;
-                x = stk[t] + x;
-                lvl = t + 1;
-            }
-            stk[lvl] = x;
-        }
-    }
-    float r = stk[0];
-    #pragma unroll
-    for (int t = 1; t < 7; ++t) r = (c == (1 << t)) ? stk[t] : r;
-    return wave_tree_sum(r);
 }
 
 // ── near test and cell insertion ─────────────────────────────────────────────
@@ -800,13 +659,15 @@ __device__ __forceinline__ bool ship_pre_early(const IceDemParams& P) {
 }
 
 // ShipPre through device memory, from one kernel to the next
-constexpr int SHIP_PRE_N = (int)(sizeof(ShipPre) / sizeof(float));
-static_assert(sizeof(ShipPre) == SHIP_PRE_N * sizeof(float), "ShipPre is floats only");
-__device__ __forceinline__ void ship_pre_get(const float* src, ShipPre& o) {
-    float* f = (float*)&o;
-    GAS const float* g = G(src);
-    #pragma unroll
-    for (int i = 0; i < SHIP_PRE_N; ++i) f[i] = g[i];
+// shpre layout: eta 0-2, nu 3-5, sin 6, cos 7, tau_act 8-10, dn 11-13,
+// propeller states 14-19, DP integral 20-22
+static_assert(sizeof(ShipPre) == 23 * sizeof(float), "shpre layout covers every field");
+template <class P>
+__device__ __forceinline__ void ship_pre_get(P g, ShipPre& o) {
+    for (int i = 0; i < 3; ++i) { o.eta[i] = g[i]; o.nu[i] = g[3 + i]; o.tau_act[i] = g[8 + i]; o.dn[i] = g[11 + i]; }
+    o.s = g[6]; o.c = g[7];
+    for (int i = 0; i < 6; ++i) o.nnew[i] = g[14 + i];
+    for (int i = 0; i < 3; ++i) o.e_new[i] = g[20 + i];
 }
 
 // One workgroup of the kernel before frame k's force pass (a call's first
@@ -821,9 +682,11 @@ __device__ __forceinline__ void ship_pre_block(const DevView* __restrict__ V, co
     for (int i = 0; i < ICEDEM_SHIP_STATE_LEN; ++i) st[i] = ship[i];
     ShipPre o;
     ship_pre(V->p, st, spc, spn, vwx, vwy, cu, cr, o);
-    const float* f = (const float*)&o;
-    GAS float* dst = G(V->shpre);
-    for (int i = 0; i < SHIP_PRE_N; ++i) dst[i] = f[i];
+    GAS float* g = G(V->shpre);
+    for (int i = 0; i < 3; ++i) { g[i] = o.eta[i]; g[3 + i] = o.nu[i]; g[8 + i] = o.tau_act[i]; g[11 + i] = o.dn[i]; }
+    g[6] = o.s; g[7] = o.c;
+    for (int i = 0; i < 6; ++i) g[14 + i] = o.nnew[i];
+    for (int i = 0; i < 3; ++i) g[20 + i] = o.e_new[i];
 }
 
 // shipblk: the grid's last workgroup forms frame 0's ship_pre_block
@@ -837,18 +700,18 @@ __global__ __launch_bounds__(BLOCK) void k_cull(const DevView* __restrict__ V, c
     cull_kernel_body(V, D, blockIdx.x, &nl);
 }
 
-// pre: the force-independent part of this frame's ship step formed earlier
-// (ship_pre_early), else nullptr
+// use_pre: pre holds the force-independent part of this frame's ship step,
+// formed earlier (ship_pre_early)
 __device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                            int k, int last, float fsx, float fsy, float tm,
-                                           const ShipPre* pre = nullptr) {
+                                           bool use_pre = false, const ShipPre& pre = ShipPre()) {
     const IceDemParams& P = V->p;
     float Fx = -fsx, Fy = -fsy;
     float tau[3] = {Fx, Fy, tm};
     GAS float* ship = G(D->s.d_ship);
     float spc[3] = {0.0f, 0.0f, 0.0f}, spn[3] = {0.0f, 0.0f, 0.0f};
     float cu = 0.0f, cr = 0.0f;
-    if (pre) {
+    if (use_pre) {
     } else if (P.ship_mode == 1) {
         if (D->in.d_controls && D->in.n_knots > 0) {
             const int knot = min(k / max(D->in.hold, 1), D->in.n_knots - 1);
@@ -868,23 +731,28 @@ __device__ __forceinline__ void ship_frame(const DevView* __restrict__ V, const 
         }
     }
     float vwx = 0.0f, vwy = 0.0f;
-    if (!pre && D->in.d_v_water_ship) {
+    if (!use_pre && D->in.d_v_water_ship) {
         vwx = G(D->in.d_v_water_ship)[2 * k]; vwy = G(D->in.d_v_water_ship)[2 * k + 1];
     }
+    // the ship state (with pre: already in it, as pre.eta and pre.nu, and
+    // every entry is rewritten by ship_post)
     float st[ICEDEM_SHIP_STATE_LEN];
-    for (int i = 0; i < ICEDEM_SHIP_STATE_LEN; ++i) st[i] = ship[i];
+    if (!use_pre)
+        for (int i = 0; i < ICEDEM_SHIP_STATE_LEN; ++i) st[i] = ship[i];
     const float Fm = sqrtf(Fx * Fx + Fy * Fy);
     const float J = Fm * P.dt;
     GAS float* ro = G(D->dg.d_rollout);
     if (ro) {
         if (k == 0) {
+            const float u0 = use_pre ? pre.nu[0] : st[3], v0 = use_pre ? pre.nu[1] : st[4];
+            const float r0 = use_pre ? pre.nu[2] : st[5];
             ro[0] = 0.0f; ro[2] = 0.0f;
-            ro[3] = 0.5f * P.ship_mass * (st[3] * st[3] + st[4] * st[4]) + 0.5f * P.ship_Izz * (st[5] * st[5]);
+            ro[3] = 0.5f * P.ship_mass * (u0 * u0 + v0 * v0) + 0.5f * P.ship_Izz * (r0 * r0);
         }
         ro[0] = ro[0] + J;
         ro[2] = vmax(ro[2], Fm);
     }
-    if (pre) ship_post(P, *pre, tau, st);
+    if (use_pre) ship_post(P, pre, tau, st);
     else ship_step(P, st, spc, spn, tau, vwx, vwy, cu, cr);
     for (int i = 0; i < ICEDEM_SHIP_STATE_LEN; ++i) ship[i] = st[i];
     if (ro && last) {
@@ -944,40 +812,105 @@ __device__ __forceinline__ void floe_eligibility(const DevView* __restrict__ V, 
     }
 }
 
-// Extra workgroup b of k_force (tail_groups(nblk) of them): its wave w forms
-// the near-masked pairwise trees of soi_x, soi_y and the moment term over
-// element block 4b + w (jax_simulation.py:456-470).  The last of the extra
-// workgroups forms the totals and the thrust scale, the scaled partials and
-// their totals (:471-479) over the blocks holding near elements, the floe
-// eligibility and the ship step, and resets the near counters.  The thrust
-// scale is left in scal[0] for k_post.
-// preship: the previous kernel left the force-independent part of this
-// frame's ship step in shpre (ship_pre_block).
-__device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int k, int last,
-                           int b, int nextra, int* am_last, int set, bool preship) {
+// Pairwise tree over lv[0..npad) (npad a power of two, at most 64 * 64;
+// LDS or memory this workgroup wrote; P the pointer type) by one wave, the
+// order of the oracle's tree_sum; result in every lane.  Up to 64 entries:
+// one per lane and a butterfly of log2(npad) steps; beyond: each lane reduces
+// its contiguous chunk pairwise, up to 16 entries loaded before the first
+// add (aligned subtrees of the full tree), then the butterfly.
+template <class P>
+__device__ __forceinline__ float wg_wave_tree(P lv, int npad) {
+    const int lane = threadIdx.x & 63;
+    if (npad <= 64) return wave_tree_small(lane < npad ? lv[lane] : 0.0f, npad);
+    const int c = npad >> 6;
+    const int g = c < 16 ? c : 16;
+    P p = lv + (size_t)lane * c;
+    float s0 = 0.0f, s1 = 0.0f, r = 0.0f;
+    for (int base = 0; base < c; base += 16) {
+        float v[16];
+        #pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = (i < g) ? p[base + i] : 0.0f;
+        #pragma unroll
+        for (int st = 1; st < 16; st <<= 1) {
+            if (st < g) {
+                #pragma unroll
+                for (int i = 0; i + st < 16; i += 2 * st) v[i] = v[i] + v[i + st];
+            }
+        }
+        // groups of 16 combined pairwise: (g0 + g1) + (g2 + g3)
+        const int q = base >> 4;
+        float x = v[0];
+        if (q & 1) {
+            x = s0 + x;
+            if (q & 2) r = s1 + x;
+            else s1 = x;
+        } else {
+            s0 = x;
+        }
+        r = (c <= 16) ? s0 : (c == 32 ? s1 : r);
+    }
+    return wave_tree_sum(r);
+}
+
+// The tail of a frame's first k_force, one workgroup (dispatched first, so it
+// overlaps the element work): the pairwise trees of the near-masked ship
+// forces and moment over each 256-element block holding a near element
+// (jax_simulation.py:456-470), the other blocks' partials +0, so the totals
+// need no hand-off between workgroups; then the thrust scale, the scaled
+// partials only when the cap binds (at scale 1 they are the unscaled
+// totals, bit for bit), the ship step's final part on wave 0 (preship: its
+// force-independent part left by the previous kernel) and the floe
+// eligibility on waves 1 .. 3.  The partials (3 nbpad floats) and the list
+// of near blocks (nblk ints) live in the workgroup's LDS when they fit
+// (`lds`, `cap` bytes of it, unused by the tail otherwise), else in the
+// context's partial buffers.
+__device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const CallDesc* __restrict__ D, int k,
+                                           int last, int set, bool preship, float* lds, int cap) {
     const IceDemParams& P = V->p;
     const int nbpad = V->nbpad, nb = V->nblk;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     constexpr int NW = BLOCK / 64;
-    const int blk = b * NW + w;
-    // which partials can be nonzero, for the last workgroup's trees (read
-    // before the ticket, off the last workgroup's path)
-    const uint64_t live = near_chunk_bits(V, nbpad);
-    if (blk < nb) {
-        float a = 0.0f, c = 0.0f, d = 0.0f;
-        if (G(V->nearcnt)[blk] > 0) block256_tree3(V, G(V->soi_x), G(V->soi_y), G(V->mterm), blk, a, c, d);
-        if (lane == 0) { st_dev(V->part1 + blk, a); st_dev(V->part1 + nbpad + blk, c); st_dev(V->part1 + 2 * nbpad + blk, d); }
-        stores_done();
+    typedef __attribute__((address_space(3))) float lds_f;
+    const bool in_lds = 16 * nbpad <= cap;
+    float* lv = in_lds ? lds : V->part1;                                 // [3][nbpad] block partials
+    int* nlist = in_lds ? (int*)(lds + 3 * nbpad) : (int*)V->part2;      // the near blocks
+    // the trees read LDS through LDS pointers (not flat loads)
+    auto tree = [&](int a) {
+        return in_lds ? wg_wave_tree((lds_f*)(lv + a * nbpad), nbpad) : wg_wave_tree(lv + a * nbpad, nbpad);
+    };
+    __shared__ int ncnt;
+    __shared__ float lpre[32];                    // the ship step's force-independent part
+    // staged through LDS while the near counts load (not held in registers)
+    if (preship && threadIdx.x < 23) lpre[threadIdx.x] = G(V->shpre)[threadIdx.x];
+    if (threadIdx.x == 0) ncnt = 0;
+    for (int i = threadIdx.x; i < 3 * nbpad; i += BLOCK) lv[i] = 0.0f;
+    __syncthreads();
+    // the blocks holding a near element, listed (order free: partials go by index)
+    for (int base = 64 * w; base < nb; base += 64 * NW) {
+        const bool has = base + lane < nb && G(V->nearcnt)[base + lane] > 0;
+        const uint64_t m = __ballot(has);
+        if (m != 0ull) {
+            const int first = __ffsll((unsigned long long)m) - 1;
+            int qb = 0;
+            if (lane == first) qb = atomicAdd(&ncnt, __popcll(m));
+            qb = __shfl(qb, first, 64);
+            if (has)
+                nlist[qb + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
+                    base + lane;
+        }
     }
     __syncthreads();
-    if (threadIdx.x == 0) *am_last = (atomicAdd(force_ticket(V), 1) == nextra - 1) ? 1 : 0;
+    const int cnt = ncnt;
+    for (int i = w; i < cnt; i += NW) {
+        const int b = nlist[i];
+        float a, c, d;
+        block256_tree3(V, G(V->soi_x), G(V->soi_y), G(V->mterm), b, a, c, d);
+        if (lane == 0) { lv[b] = a; lv[nbpad + b] = c; lv[2 * nbpad + b] = d; }
+    }
     __syncthreads();
-    if (!*am_last) return;
-    ShipPre pre;
-    if (preship && w == 0) ship_pre_get(V->shpre, pre);   // in flight with the totals' loads
     // totals, in every wave
-    float tx, ty, tm;
-    wave_tree_global3(V->part1, V->part1 + nbpad, V->part1 + 2 * nbpad, nbpad, tx, ty, tm, live);
+    const float tx = tree(0), ty = tree(1);
+    const float tm = tree(2);
     float scale = 1.0f, fx = tx, fy = ty;
     if (P.has_thrust_cap) {
         const float F_tot = sqrtf(tx * tx + ty * ty);
@@ -986,32 +919,23 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
     // with the cap binding, the totals of the scaled forces (x * 1 = x: at
     // scale 1 they are the totals above, bit for bit)
     if (scale != 1.0f) {
-        // scaled partials: 64 blocks at a time, the blocks holding near
-        // elements dealt to the waves in turn, zeros for the others
-        for (int base = 0; base < nb; base += 64) {
-            const int q = base + lane;
-            const bool has = q < nb && G(V->nearcnt)[q] > 0;
-            if (w == 0 && q < nb && !has) { st_dev(V->part2 + q, 0.0f); st_dev(V->part2 + nbpad + q, 0.0f); }
-            uint64_t m = __ballot(has);
-            for (int r = 0; m != 0ull; ++r, m &= m - 1ull) {
-                if ((r & (NW - 1)) != w) continue;
-                const int qb = base + __builtin_ctzll(m);
-                const float a = block256_tree(V, G(V->soi_x), qb, scale, true);
-                const float c = block256_tree(V, G(V->soi_y), qb, scale, true);
-                if (lane == 0) { st_dev(V->part2 + qb, a); st_dev(V->part2 + nbpad + qb, c); }
-            }
+        __syncthreads();   // every wave has read the unscaled partials
+        for (int i = w; i < cnt; i += NW) {
+            const int b = nlist[i];
+            const float a = block256_tree(V, G(V->soi_x), b, scale, true);
+            const float c = block256_tree(V, G(V->soi_y), b, scale, true);
+            if (lane == 0) { lv[b] = a; lv[nbpad + b] = c; }
         }
-        stores_done();
         __syncthreads();
-        float unused;
-        wave_tree_global3(V->part2, V->part2 + nbpad, V->part2, nbpad, fx, fy, unused, live);
+        fx = tree(0);
+        fy = tree(1);
     }
-    // the ship step on wave 0 while the other waves update floe eligibility
-    // (independent: neither reads what the other writes)
     if (w == 0) {
         if (threadIdx.x == 0) {
             G(V->scal)[0] = scale;
-            ship_frame(V, D, k, last, fx, fy, tm, preship ? &pre : nullptr);
+            ShipPre pre;
+            if (preship) ship_pre_get(lpre, pre);
+            ship_frame(V, D, k, last, fx, fy, tm, preship, pre);
             store_pose(V, D->s.d_ship);
             *near_count(V) = 0;
             *cand_count(V, set) = 0;   // consumed by the previous k_post, refilled by this frame's
@@ -1053,6 +977,9 @@ struct ForceLds {
     __device__ float* cr(int g) { return cf[g] + 4 * MAXC2; }
     __device__ float* stg(int g) { return cf[g]; }
 };
+
+// extra (tail) workgroups of a frame's first k_force
+constexpr int TAIL_GROUPS = 1;
 
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -2019,10 +1946,10 @@ __device__ __forceinline__ void force_kernel_body(const DevView* __restrict__ V,
                                                   int k, int sub, int src_is_state, int mode, int set, int last,
                                                   int nextra, int lb, int lg, bool xcd,
                                                   const float* __restrict__ fship_in, float* __restrict__ fout,
-                                                  float* __restrict__ mout, ForceLds<GR>& L, int* am_last,
+                                                  float* __restrict__ mout, ForceLds<GR>& L,
                                                   bool preship = false) {
     if (lb < nextra) {
-        force_tail(V, D, k, last, lb, nextra, am_last, set, preship);
+        force_tail(V, D, k, last, set, preship, (float*)&L, (int)sizeof(ForceLds<GR>));
         return;
     }
     const int nf = lg - nextra;
@@ -2040,9 +1967,8 @@ __global__ __launch_bounds__(BLOCK) void k_force(const DevView* __restrict__ V, 
                                                  const float* __restrict__ fship_in,
                                                  float* __restrict__ fout, float* __restrict__ mout, int preship) {
     __shared__ ForceLds<GR> L;
-    __shared__ int am_last;
     force_kernel_body<GR>(V, D, k, sub, src_is_state, mode, set, last, nextra, (int)blockIdx.x, (int)gridDim.x, true,
-                      fship_in, fout, mout, L, &am_last, preship != 0);
+                      fship_in, fout, mout, L, preship != 0);
 }
 
 // ── k_post: fracture (jax_simulation.py:534-656), commit, histories, next cull ─
@@ -2215,7 +2141,6 @@ __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const C
     }
     if (used) clear_cells(V, set, t);
     if (t < V->nf) { G(floe_coll_of(V, set))[t] = 0u; G(floe_maxfm_of(V, set))[t] = 0u; }
-    if (t == 0) *force_ticket(V) = 0;
     // after a call's last frame no candidate list is pending
     if (last && t == 0) { *cand_count(V, 0) = 0; *cand_count(V, 1) = 0; }
     if (cull && chunk < V->nblk) {
@@ -2329,10 +2254,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GR < 16 ?
 void kb_force(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
               const int4* __restrict__ map, int k, int last, int set) {
     __shared__ ForceLds<GR> L;
-    __shared__ int am_last;
     const int4 m = map[blockIdx.x];
     force_kernel_body<GR>(views + m.x, descs + m.x, k, 0, 1, 0, set, last, m.w, m.y, m.z, false, nullptr, nullptr,
-                      nullptr, L, &am_last);
+                      nullptr, L);
 }
 
 __global__ __launch_bounds__(BLOCK) void kb_post(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,

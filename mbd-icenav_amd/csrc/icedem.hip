@@ -766,7 +766,7 @@ static int launch_frames(IceDemCtx* ctx, int K, hipStream_t stream, LaunchTimer*
     const CallDesc* D = ctx->dd;
     const int nblk = H.nblk;
     const int fblk = force_chunks(H.n, H.lanes);
-    const int nextra = tail_groups(nblk);
+    const int nextra = TAIL_GROUPS;
     const int steps = H.p.n_ice_steps;
     // post grid: element/cell/floe blocks, then bond blocks
     const int nbe = (std::max(H.n, std::max(std::max(H.G, H.G2), H.nf)) + BLOCK - 1) / BLOCK;
@@ -1007,7 +1007,7 @@ int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** ou
     for (int i = 0; i < n_inst; ++i) {
         const DevView& H = ctxs[i]->hv;
         const int fblk = force_chunks(H.n, b->lanes);
-        const int nextra = tail_groups(H.nblk);
+        const int nextra = TAIL_GROUPS;
         // one wavefront per 16 elements (a quarter of a patch is near the ship
         // at most), and at least 8: a small patch with the bow inside it can
         // have most of its elements near (256 patches: 65.1 -> 64.1 us per

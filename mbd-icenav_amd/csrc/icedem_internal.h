@@ -61,7 +61,7 @@ struct DevView {
     uint8_t* nearf;           // (n) element passed the hull cull this frame
     int32_t* near_list;       // (n) queue of near elements
     int32_t* nearcnt;         // (nbpad) near elements of each 256-element block, 0 past nblk
-    int32_t* counters;        // [0] near elements, [1], [2] near candidates (frame parity), [3] k_force tail ticket
+    int32_t* counters;        // [0] near elements, [1], [2] near candidates (frame parity) ([3] unused)
     int32_t* cand;            // (2, n) near candidates of the next near test of each frame parity
     uint8_t* cflag;           // (2, n) element listed among them
     float cand_margin;        // candidate test margin (1 m + one frame's largest displacements)
