@@ -351,12 +351,6 @@ __device__ __forceinline__ void block256_tree3(const DevView* __restrict__ V, GA
     s2 = wave_tree_sum(tree4(x2));
 }
 
-// pairwise tree of one value per lane over lanes [0, npad), npad <= 64 a
-// power of two; result in every lane
-__device__ __forceinline__ float wave_tree_small(float x, int npad) {
-    for (int off = 1; off < npad; off <<= 1) x = x + __shfl_xor(x, off, 64);
-    return __shfl(x, 0, 64);
-}
 
 // ── near test and cell insertion ─────────────────────────────────────────────
 
@@ -1802,9 +1796,7 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
         // wave whose largest one does stores it into its slot (any value past
         // the threshold will do, so racing stores need no read-modify-write);
         // the other waves store nothing
-        float m = d2;
-        #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+        const float m = wave_max(d2);
         if ((threadIdx.x & 63) == 0 && m > V->nl_thr2)
             __hip_atomic_store(nl_slot(V, k + 1) + (blockIdx.x & (NLSUB - 1)), __float_as_uint(m), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
@@ -1916,9 +1908,7 @@ __device__ __forceinline__ void force_body_lazy(const DevView* __restrict__ V, c
         }
     }
     if (lists) {
-        float m = d2;
-        #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+        const float m = wave_max(d2);
         if ((threadIdx.x & 63) == 0 && m > V->nl_thr2)
             __hip_atomic_store(nl_slot(V, k + 1) + (blockIdx.x & (NLSUB - 1)), __float_as_uint(m), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);

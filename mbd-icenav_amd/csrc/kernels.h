@@ -11,12 +11,46 @@ namespace icedem {
 
 // ── wave / block reductions with a fixed pairwise order ──────────────────────
 
-__device__ __forceinline__ float wave_tree_sum(float x) {
-    // xor-butterfly: after offset 2^k lane i holds the sum of its aligned
-    // 2^(k+1)-block, combined pairwise (the tree of tree_sum over 64 lanes)
-    #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) x = x + __shfl_xor(x, off, 64);
-    return x;
+__device__ __forceinline__ float rl(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// lane l takes the value of the lane DPP control CTRL names
+template <int CTRL>
+__device__ __forceinline__ float dppf(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+
+// Pairwise tree of the npad (a power of two, at most 64) lane values from
+// lane 0 (the tree of tree_sum over npad entries), in every lane.  The
+// xor-butterfly pairing: lane i adds lane i ^ 2^k, so after step k every
+// lane of an aligned 2^(k+1)-block holds that block's pairwise sum.  Steps 1
+// and 2 are DPP quad permutations; for steps 4 and 8 the blocks of the
+// previous step hold equal values in every lane, so the half-row and row
+// mirrors pair the same sums as xor 4 and xor 8; the rows' sums are combined
+// from lanes 0, 16, 32, 48 as the xor-16 and xor-32 steps would ((a + b) and
+// (b + a) are the same float).
+__device__ __forceinline__ float wave_tree_small(float x, int npad) {
+    if (npad > 1) x = x + dppf<0xB1>(x);    // quad_perm [1,0,3,2]: xor 1
+    if (npad > 2) x = x + dppf<0x4E>(x);    // quad_perm [2,3,0,1]: xor 2
+    if (npad > 4) x = x + dppf<0x141>(x);   // row_half_mirror
+    if (npad > 8) x = x + dppf<0x140>(x);   // row_mirror
+    if (npad <= 16) return rl(x, 0);
+    const float r01 = rl(x, 0) + rl(x, 16);
+    if (npad <= 32) return r01;
+    return r01 + (rl(x, 32) + rl(x, 48));
+}
+
+// pairwise tree of the 64 lane values (tree_sum's order), in every lane
+__device__ __forceinline__ float wave_tree_sum(float x) { return wave_tree_small(x, 64); }
+
+// largest of the 64 lane values (order-free), in every lane
+__device__ __forceinline__ float wave_max(float x) {
+    x = fmaxf(x, dppf<0xB1>(x));
+    x = fmaxf(x, dppf<0x4E>(x));
+    x = fmaxf(x, dppf<0x141>(x));
+    x = fmaxf(x, dppf<0x140>(x));
+    return fmaxf(fmaxf(rl(x, 0), rl(x, 16)), fmaxf(rl(x, 32), rl(x, 48)));
 }
 
 // Tree sum over the block (blockDim = NW*64); result valid in thread 0.
@@ -59,10 +93,6 @@ __device__ __forceinline__ float lds_tree_sum(const float* src, int count, int n
     float r = a[0];
     __syncthreads();
     return r;
-}
-
-__device__ __forceinline__ float rl(float v, int lane) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
 // One step of the row-wise argmax: lane l takes lane l - k of its 16-lane row
