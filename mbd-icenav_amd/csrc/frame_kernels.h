@@ -473,43 +473,61 @@ __global__ __launch_bounds__(BLOCK) void k_clear_cells(const DevView* __restrict
 
 // ── k_near ───────────────────────────────────────────────────────────────────
 
+// The static inputs of a near element's hull test (read before the test
+// that decides whether it runs, in one round trip with the element's state):
+// radius, thickness, floe, reach (every world vertex lies within the
+// element's bound of its position, up to the vertices' rounding, which the
+// 1 cm margin covers), the lane's body-frame vertex and mask bit, and the
+// lane's first hull point.
+struct NearIn {
+    float rad, thk, reach;
+    int fl;
+    float2 lv, hl0;
+    bool mv;
+};
+
+__device__ __forceinline__ NearIn near_in(const DevView* __restrict__ V, int e) {
+    const int lane = threadIdx.x & 63, maxv = V->maxv;
+    NearIn o;
+    o.rad = G(V->radius)[e]; o.thk = G(V->thick)[e];
+    o.reach = G(V->bound)[e] + 0.01f;
+    o.fl = G(V->floe)[e];
+    o.lv = make_float2(0.0f, 0.0f);
+    o.mv = false;
+    if (lane < maxv) {
+        o.lv = ldg(G(V->polys) + (size_t)e * maxv + lane);
+        o.mv = G(V->mask)[(size_t)e * maxv + lane] != 0;
+    }
+    o.hl0 = ldg(G(V->hull) + (lane < V->S ? lane : 0));
+    return o;
+}
+
 // Hull narrow phase and pre-pass of near element e at position (px, py)
 // and rotation rot by the calling wave; lane 0 stores the per-element
 // results and the floe or/max of the frames of parity set.
 __device__ __forceinline__ void near_elem_at(const DevView* __restrict__ V, const ShipPose& SP, int e, float px,
-                                             float py, float rot, int set) {
+                                             float py, float rot, int set, const NearIn& ni) {
     const IceDemParams& P = V->p;
     const int maxv = V->maxv;
     const int lane = threadIdx.x & 63;
-    const float rad = G(V->radius)[e], thk = G(V->thick)[e];
-    // every world vertex lies within the element's bound of its position (up
-    // to the vertices' rounding, which the 1 cm margin covers)
-    const float reach = G(V->bound)[e] + 0.01f;
-    const int fl = G(V->floe)[e];   // read with the element, used if it touches the hull
     float wx = 0.0f, wy = 0.0f;
-    bool mv = false;
-    float2 lv = make_float2(0.0f, 0.0f);
-    if (lane < maxv) {
-        lv = ldg(G(V->polys) + (size_t)e * maxv + lane);
-        mv = G(V->mask)[(size_t)e * maxv + lane] != 0;
-    }
     float se, ce;
     dm_sincosf(rot, &se, &ce);
     if (lane < maxv) {
-        wx = ce * lv.x - se * lv.y + px;
-        wy = se * lv.x + ce * lv.y + py;
+        wx = ce * ni.lv.x - se * ni.lv.y + px;
+        wy = se * ni.lv.x + ce * ni.lv.y + py;
     }
     float pen, nxo, nyo, cxo, cyo;
-    hull_element(SP, P, G(V->hull), V->S, maxv, wx, wy, mv, pen, nxo, nyo, cxo, cyo, px, py, reach);
+    hull_element(SP, P, G(V->hull), V->S, maxv, wx, wy, ni.mv, pen, nxo, nyo, cxo, cyo, px, py, ni.reach, ni.hl0);
     if (lane == 0) {
-        PrePass o = prepass(P, pen, nxo, nyo, rad, thk, px, py, SP.sx, SP.sy);
+        PrePass o = prepass(P, pen, nxo, nyo, ni.rad, ni.thk, px, py, SP.sx, SP.sy);
         G(V->pen)[e] = pen; G(V->nx)[e] = nxo; G(V->ny)[e] = nyo;
         G(V->fm_raw)[e] = o.fm;
         G(V->soi_x)[e] = o.sx; G(V->soi_y)[e] = o.sy; G(V->mterm)[e] = o.m;
         if (pen > 0.0f) {
             // results unused: neither atomic is waited for
-            (void)atomicOr(floe_coll_of(V, set) + fl, 1u);
-            (void)atomicMax(floe_maxfm_of(V, set) + fl, __float_as_uint(o.fm));
+            (void)atomicOr(floe_coll_of(V, set) + ni.fl, 1u);
+            (void)atomicMax(floe_maxfm_of(V, set) + ni.fl, __float_as_uint(o.fm));
         }
     }
 }
@@ -518,7 +536,7 @@ __device__ __forceinline__ void near_elem_at(const DevView* __restrict__ V, cons
 __device__ __forceinline__ void near_elem(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
                                           const ShipPose& SP, int e, int set) {
     GAS const float* pos = G(D->s.d_pos);
-    near_elem_at(V, SP, e, pos[e], pos[V->n + e], G(D->s.d_rot)[e], set);
+    near_elem_at(V, SP, e, pos[e], pos[V->n + e], G(D->s.d_rot)[e], set, near_in(V, e));
 }
 
 // The ship pose of the next near test: position from the ship state, the
@@ -571,7 +589,10 @@ __device__ __forceinline__ void near_inline(const DevView* __restrict__ V, const
     } else {
         SP = stored_pose(V, D);
     }
-    for (int q = w; q < cnt; q += BLOCK / 64) near_elem_at(V, SP, L->e[q], L->x[q], L->y[q], L->ro[q], set);
+    for (int q = w; q < cnt; q += BLOCK / 64) {
+        const int e = L->e[q];
+        near_elem_at(V, SP, e, L->x[q], L->y[q], L->ro[q], set, near_in(V, e));
+    }
 }
 
 // workgroup lb of lg working on one context's near queue: one wave per near
@@ -2167,8 +2188,10 @@ __device__ __forceinline__ void near_cand_block(const DevView* __restrict__ V, c
     GAS const float* rotp = G(final_in_nxt ? V->nxt_rot : D->s.d_rot);
     for (int q = wave; q < count; q += nwaves) {
         const int e = (q == wave) ? e0 : cl[q];
+        // the hull test's statics read with the state, before the exact test
         const float x = pos[e], y = pos[n + e], ro = rotp[e], bnd = G(V->bound)[e];
-        if (near_box(V->p, x, y, bnd, ship, 1.0f)) near_elem_at(V, SP, e, x, y, ro, par);
+        const NearIn ni = near_in(V, e);
+        if (near_box(V->p, x, y, bnd, ship, 1.0f)) near_elem_at(V, SP, e, x, y, ro, par, ni);
     }
 }
 

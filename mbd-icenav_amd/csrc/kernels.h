@@ -17,8 +17,12 @@ __device__ __forceinline__ float rl(float v, int lane) {
 
 // lane l takes the value of the lane DPP control CTRL names
 template <int CTRL>
+__device__ __forceinline__ int dppi(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
 __device__ __forceinline__ float dppf(float x) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+    return __int_as_float(dppi<CTRL>(__float_as_int(x)));
 }
 
 // Pairwise tree of the npad (a power of two, at most 64) lane values from
@@ -195,13 +199,13 @@ __device__ __forceinline__ void hull_element(const ShipPose& P, const IceDemPara
                                              float wx, float wy, bool mv,
                                              float& pen_o, float& nx_o, float& ny_o,
                                              float& cx_o, float& cy_o,
-                                             float ex = 0.0f, float ey = 0.0f, float reach = -1.0f) {
+                                             float ex, float ey, float reach, float2 hl0) {
     const int lane = threadIdx.x & 63;
     const int K = prm.n_edge_samples;
     const int per = 1 + K;
     const int npts = maxv * per;
-    // pass 2's first 64 hull points, loaded here so the load overlaps pass 1
-    const float2 hl0 = ldg(hull + (lane < S ? lane : 0));
+    // hl0: pass 2's first 64 hull points (hull[min(lane, S - 1)]), loaded
+    // by the caller ahead of everything that precedes them here
     // ---- pass 1: vertices + edge samples vs analytic hull (:139-284)
     float bv = -1.0f, bnx = 0.0f, bny = 0.0f, bcx = 0.0f, bcy = 0.0f;
     int bi_ = 0x7fffffff;
@@ -312,13 +316,18 @@ __device__ __forceinline__ void hull_element(const ShipPose& P, const IceDemPara
                 if (ok && (bk == 0x7fffffff || d < bd)) { bd = d; bk = kk; }
             }
             // combine the GP lanes of each point (aligned groups, xor butterfly)
-            for (int off = 1; off < GP; off <<= 1) {
-                const int oc = __shfl_xor(cross, off, 64);
-                const float od = __shfl_xor(bd, off, 64);
-                const int ok_ = __shfl_xor(bk, off, 64);
+            // (xor 1 to 8 by DPP: after each step an aligned block's lanes
+            // hold the same combined value, so the mirrors pair as xor does)
+            auto comb = [&](int oc, float od, int ok_) {
                 cross += oc;
                 if (ok_ != 0x7fffffff && (bk == 0x7fffffff || od < bd || (od == bd && ok_ < bk))) { bd = od; bk = ok_; }
-            }
+            };
+            if (GP > 1) comb(dppi<0xB1>(cross), dppf<0xB1>(bd), dppi<0xB1>(bk));
+            if (GP > 2) comb(dppi<0x4E>(cross), dppf<0x4E>(bd), dppi<0x4E>(bk));
+            if (GP > 4) comb(dppi<0x141>(cross), dppf<0x141>(bd), dppi<0x141>(bk));
+            if (GP > 8) comb(dppi<0x140>(cross), dppf<0x140>(bd), dppi<0x140>(bk));
+            for (int off = 16; off < GP; off <<= 1)
+                comb(__shfl_xor(cross, off, 64), __shfl_xor(bd, off, 64), __shfl_xor(bk, off, 64));
             if (bk == 0x7fffffff) bk = 0;
             int bkj = (bk + 1 == maxv) ? 0 : bk + 1;
             float xi = __shfl(wx, bk, 64), yi = __shfl(wy, bk, 64);
@@ -386,6 +395,7 @@ __device__ __forceinline__ void hull_wave(const IceDemParams& prm, const float2*
     }
     uint64_t todo = __ballot(near);
     if (todo == 0) return;
+    const float2 hl0 = ldg(G(hull) + (lane < S ? lane : 0));
     ShipPose P = make_pose(sx, sy, sh, prm.half_len, prm.half_beam);
     float se = 0.0f, ce = 1.0f;
     if (near) dm_sincosf(rot, &se, &ce);
@@ -403,7 +413,7 @@ __device__ __forceinline__ void hull_wave(const IceDemParams& prm, const float2*
             mv = mask[(size_t)qe * maxv + lane] != 0;
         }
         float a, b, c, d, f;
-        hull_element(P, prm, G(hull), S, maxv, wx, wy, mv, a, b, c, d, f, qpx, qpy, bound[qe] + 0.01f);
+        hull_element(P, prm, G(hull), S, maxv, wx, wy, mv, a, b, c, d, f, qpx, qpy, bound[qe] + 0.01f, hl0);
         if (lane == q) { pen = a; nxo = b; nyo = c; cxo = d; cyo = f; }
     }
 }
