@@ -134,6 +134,12 @@ typedef struct {
      * element may have moved list_skin / 2 since the last build; frames in
      * between evaluate only the listed partners.  0 = the grids every frame. */
     float list_skin;
+    /* near-candidate margin (build choice, results identical): a single
+     * field's post pass tests the next frame's near elements among the
+     * elements the previous cull listed within bound + margin of the hull's
+     * box; a near element outside them is tested where it is culled.  < 0 =
+     * the library's (1 m plus one frame's largest displacements). */
+    float near_cand_margin;
 } IceDemParams;
 
 /* Mutable state (IceState + ShipState, jax_ice_state.py:31-52), device

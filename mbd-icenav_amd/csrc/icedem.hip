@@ -510,6 +510,7 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
         H.cand_margin = (float)(1.0 + (double)prm->dt * ((double)prm->velocity_clip * std::sqrt(2.0) +
                                                          std::hypot((double)prm->vc_surge, (double)prm->vc_sway) +
                                                          (double)prm->vc_yaw * R) + 0.15);
+        if (prm->near_cand_margin >= 0.0f) H.cand_margin = prm->near_cand_margin;
     }
     if (prm->grid_bucket < 1 || (int64_t)H.G * prm->grid_bucket > (1LL << 28)) {
         set_err(nullptr, "bad grid_bucket"); delete ctx; return ICEDEM_E_INVALID;

@@ -63,6 +63,7 @@ class IceDemParams(C.Structure):
         ("grid_nx", _i), ("grid_ny", _i), ("grid_bucket", _i), ("grid_bucket2", _i),
         ("force_lanes", _i),
         ("list_skin", _f),
+        ("near_cand_margin", _f),
     ]
 
 

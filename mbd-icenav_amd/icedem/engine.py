@@ -108,12 +108,14 @@ def choose_grid(centroids: np.ndarray, radii: np.ndarray, dilation: float = 0.0,
 def make_params(static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_length, ship_beam, n_ice_steps,
                 dt, dt_ice, use_auto_icenav, B_mat, T_n, grid=None, ship_mode: int = 0,
                 ctrl_thrust_max: float = 2e6, ctrl_yaw_rate_max: float = 0.15,
-                force_lanes: int = 0, list_skin: float = DEFAULT_LIST_SKIN) -> _lib.IceDemParams:
+                force_lanes: int = 0, list_skin: float = DEFAULT_LIST_SKIN,
+                near_cand_margin: float = -1.0) -> _lib.IceDemParams:
     """Scalar parameters rounded to float32 in the reference's order of use.
     force_lanes: lanes per element in the force pass (8 or 16; 0 lets the
     library choose by element count); list_skin: reach of the contact
-    partner lists beyond contact in metres (0: the grids every frame).
-    Neither changes a result."""
+    partner lists beyond contact in metres (0: the grids every frame);
+    near_cand_margin: margin of the next frame's near candidates (< 0: the
+    library's).  None of them changes a result."""
     p = _lib.IceDemParams()
     hl = ship_length / 2.0
     hb = ship_beam / 2.0
@@ -207,6 +209,7 @@ def make_params(static: IceStaticData, cfg, ship_dyn: ShipDynamicsData, ship_len
     if not (list_skin >= 0.0 and math.isfinite(list_skin)):
         raise ValueError("list_skin must be finite and >= 0")
     p.list_skin = F32(list_skin)
+    p.near_cand_margin = F32(near_cand_margin)
     return p
 
 
@@ -226,7 +229,7 @@ class DemEngine:
                  n_ice_steps, dt, dt_ice, use_auto_icenav, B_mat, T_n, centroids=None, device=None, grid=None,
                  cell_size=None, ship_mode: int = 0, ctrl_thrust_max: float = 2e6, ctrl_yaw_rate_max: float = 0.15,
                  grid_margin: float = 60.0, small_quantile: float = 0.98, force_lanes: int = 0,
-                 cell_factor: float = 1.0, list_skin: float = None):
+                 cell_factor: float = 1.0, list_skin: float = None, near_cand_margin: float = -1.0):
         if not torch.cuda.is_available():
             raise _lib.IceDemError("icedem needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.lib = _lib.load()
@@ -247,7 +250,8 @@ class DemEngine:
                                   use_auto_icenav, B_mat, T_n, grid=grid, ship_mode=ship_mode,
                                   ctrl_thrust_max=ctrl_thrust_max, ctrl_yaw_rate_max=ctrl_yaw_rate_max,
                                   force_lanes=force_lanes,
-                                  list_skin=DEFAULT_LIST_SKIN if list_skin is None else list_skin)
+                                  list_skin=DEFAULT_LIST_SKIN if list_skin is None else list_skin,
+                                  near_cand_margin=near_cand_margin)
         lp = np.ascontiguousarray(static.local_polys_padded, dtype=F32)
         self._host = dict(
             lp=lp,

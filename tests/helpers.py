@@ -54,6 +54,7 @@ class Case:
     list_skin: Optional[float] = None   # GPU contact partner-list skin (None: the library's default, 0: off)
     cell_factor: float = 1.0      # GPU fine grid cell / 2 (r_s + delta)
     bucket_caps: Optional[tuple] = None   # GPU (fine, coarse) bucket capacities in place of the sized ones
+    near_cand_margin: float = -1.0   # GPU near-candidate margin (< 0: the library's)
 
     @staticmethod
     def make(name: str, n_frames: int = 200, seed: int = 0, cfg: Optional[SimulationConfig] = None,
@@ -169,7 +170,7 @@ class Case:
                 grid[6], grid[7] = self.bucket_caps
             self._engine = DemEngine(static, self.cfg, sd, self.ship_length, self.ship_beam, nsteps, self.dt, dt_ice,
                                      self.use_auto, B, T_n, centroids=sc.centroids, force_lanes=self.force_lanes,
-                                     list_skin=self.list_skin,
+                                     list_skin=self.list_skin, near_cand_margin=self.near_cand_margin,
                                      cell_factor=self.cell_factor, grid=grid)
         return self._engine
 

@@ -115,3 +115,15 @@ def test_lists_off_and_substeps_use_the_grids():
     case2 = Case.make("c0", n_frames=820, cfg=cfg)
     assert case2.engine().params.n_ice_steps == 3
     _check(case2, 820, expect_lists=False)
+
+
+@pytest.mark.parametrize("margin", [0.0, 25.0])
+def test_near_candidates_missed_or_many(margin):
+    """The post pass tests the next frame's near elements among candidates the
+    previous cull listed.  Margin 0 leaves the elements within the near
+    test's 1 m band unlisted, so they take the test where they are culled;
+    25 m lists a wide band, most of which fails the exact test.  Either way
+    every frame equals the oracle bit for bit."""
+    case = Case.make("c0", n_frames=1100)
+    case.near_cand_margin = margin
+    _check(case, 1100)
