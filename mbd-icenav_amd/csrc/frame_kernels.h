@@ -44,19 +44,26 @@ __device__ __forceinline__ uint32_t* floe_maxfm_of(const DevView* V, int set) { 
 // one per XCD, float bits of such a squared displacement, 0 if none).  Frame
 // k's force pass reads its slot, its epilogues fill slot k + 1, and frame k's
 // k_post reads slots k and k + 1 and clears slot k + 2 for frame k + 1's
-// epilogues.
+// epilogues.  Frame 0 of a call reads the call-start slot instead: the
+// call's cull measures the positions the caller passed against the build
+// positions (the lists hold every pair that can touch while no element has
+// moved farther than that from where its list was built, whoever moved it),
+// and marks it when one is farther, when an earlier call's build overflowed,
+// or before the context's first build (the slot starts at +inf).
 __device__ __forceinline__ uint32_t* nl_slot(const DevView* __restrict__ V, int k) {
     return V->nlctl + (k % 3) * NLSUB;
 }
 __device__ __forceinline__ uint32_t* nl_ovf(const DevView* __restrict__ V) { return V->nlctl + 3 * NLSUB; }
 __device__ __forceinline__ uint32_t* nl_rebuilds(const DevView* __restrict__ V) { return V->nlctl + 3 * NLSUB + 1; }
+__device__ __forceinline__ uint32_t* nl_ovf_prev(const DevView* __restrict__ V) { return V->nlctl + 3 * NLSUB + 2; }
+__device__ __forceinline__ uint32_t* nl_start(const DevView* __restrict__ V) { return V->nlctl + 4 * NLSUB; }
 // frame k rebuilds the lists: some element may have moved far enough that a
-// partner outside its list reaches contact (every slot starts a call at +inf).
-// Plain loads (two 16-byte vectors, served by the XCD's L2 after its first
-// wave): a slot is read only by kernels launched after the ones that write it.
+// partner outside its list reaches contact.  Plain loads (two 16-byte
+// vectors, served by the XCD's L2 after its first wave): a slot is read only
+// by kernels launched after the ones that write it.
 static_assert(NLSUB == 8, "two uint4 per slot");
 __device__ __forceinline__ bool nl_rebuild(const DevView* __restrict__ V, int k) {
-    const uint4* s = (const uint4*)nl_slot(V, k);
+    const uint4* s = (const uint4*)(k == 0 ? nl_start(V) : nl_slot(V, k));
     const uint4 a = s[0], b = s[1];
     const uint32_t m = max(max(max(a.x, a.y), max(a.z, a.w)), max(max(b.x, b.y), max(b.z, b.w)));
     return __uint_as_float(m) > V->nl_thr2;
@@ -629,14 +636,27 @@ __device__ __forceinline__ void cull_kernel_body(const DevView* __restrict__ V, 
         r = G(V->radius)[e]; bnd = G(V->bound)[e];
     }
     bool near;
+    // frame 0 builds the lists when an element lies farther from its build
+    // position than the threshold (NaN positions included) or an earlier
+    // call's build overflowed a list; the build positions are read with the
+    // state
+    bool far = false;
+    if (V->nl_on && valid) {
+        const float dx = x - G(V->xb)[e], dy = y - G(V->xb)[n + e];
+        far = !(dx * dx + dy * dy <= V->nl_thr2) || *G(nl_ovf_prev(V)) != 0u;
+    }
     const int cnt = cull_chunk(V, lb, valid, e, x, y, vx, vy, r, bnd, G(D->s.d_ship), 0, near, nl == nullptr, true,
                                true, nl ? 1 : -1);
     if (nl) near_inline(V, D, nl, cnt, near, e, x, y, near ? G(D->s.d_rot)[e] : 0.0f, 0, true);
     if (lb == 0 && threadIdx.x == 0) store_pose(V, D->s.d_ship);
-    // a call starts with a list build (the caller may have changed the state)
-    if (V->nl_on && lb == 0 && threadIdx.x < 3 * NLSUB + 1)
-        __hip_atomic_store(V->nlctl + threadIdx.x, threadIdx.x < NLSUB ? 0x7f800000u : 0u, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+    if (V->nl_on) {
+        if (__ballot(far) != 0ull && (threadIdx.x & 63) == 0)
+            __hip_atomic_store(nl_start(V) + (lb & (NLSUB - 1)), 0x7f800000u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        // the frame slots and this call's overflow flag start at 0
+        if (lb == 0 && threadIdx.x < 3 * NLSUB + 1)
+            __hip_atomic_store(V->nlctl + threadIdx.x, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 
@@ -897,7 +917,10 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
     __shared__ float lpre[32];                    // the ship step's force-independent part
     // staged through LDS while the near counts load (not held in registers)
     if (preship && threadIdx.x < 23) lpre[threadIdx.x] = G(V->shpre)[threadIdx.x];
-    if (threadIdx.x == 0) ncnt = 0;
+    if (threadIdx.x == 0) {
+        ncnt = 0;
+        if (V->nl_on && nl_rebuild(V, k)) atomicAdd(nl_rebuilds(V), 1u);   // frames that built the lists
+    }
     for (int i = threadIdx.x; i < 3 * nbpad; i += BLOCK) lv[i] = 0.0f;
     __syncthreads();
     // the blocks holding a near element, listed (order free: partials go by index)
@@ -1913,6 +1936,8 @@ __device__ __forceinline__ void force_body_lazy(const DevView* __restrict__ V, c
                 // +0 bit for bit, so only a positive depth is stored
                 if (__float_as_uint(sub0) != 0u)
                     G(D->s.d_sub)[e] = vmax(0.0f, sub0 - V->p.inv_rise_time * sub0 * V->p.dt);
+                // its list was built where it stands
+                if (rebuild) { G(V->xb)[e] = px; G(V->xb)[n + e] = py; }
                 sv[S_FLAGS] = __int_as_float(8);   // done
             }
         }
@@ -2110,7 +2135,9 @@ __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const C
     bool used = true, ins = true, built = false;
     if (lists) {
         const bool ovf = nl_overflowed(V);
-        built = nl_rebuild(V, k);
+        // frame 0's buckets were filled by the call's cull whether or not the
+        // frame built (and its call-start slot is cleared below, unread)
+        built = k == 0 || nl_rebuild(V, k);
         used = ovf || built;
         ins = ovf || nl_rebuild(V, k + 1);
     }
@@ -2145,9 +2172,14 @@ __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const C
     if (lists) {
         if (t == 0) {
             #pragma unroll
-            for (int i = 0; i < NLSUB; ++i)
+            for (int i = 0; i < NLSUB; ++i) {
                 __hip_atomic_store(nl_slot(V, k + 2) + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (built) atomicAdd(nl_rebuilds(V), 1u);
+                // the call-start slot, read by frame 0's force pass, is
+                // clear for the next call's cull
+                if (k == 0) __hip_atomic_store(nl_start(V) + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            // the overflow flag is final once the force pass has run
+            if (last) __hip_atomic_store(nl_ovf_prev(V), *nl_ovf(V), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     if (used) clear_cells(V, set, t);

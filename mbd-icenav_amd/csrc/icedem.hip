@@ -688,6 +688,10 @@ static int alloc_state_buffers(IceDemCtx* ctx) {
     if (H.nl_on) {
         HIP_TRY(hipMemset(H.nlctl, 0, sizeof(uint32_t) * NLCTL));
         HIP_TRY(hipMemset(H.es, 0, sizeof(float) * 6 * (size_t)n));
+        // the first call builds: call-start slot at +inf
+        const uint32_t inf[NLSUB] = {0x7f800000u, 0x7f800000u, 0x7f800000u, 0x7f800000u,
+                                     0x7f800000u, 0x7f800000u, 0x7f800000u, 0x7f800000u};
+        HIP_TRY(hipMemcpy(H.nlctl + 4 * NLSUB, inf, sizeof(inf), hipMemcpyHostToDevice));
     }
     HIP_TRY(hipMemset(H.nearf, 0, n));
     HIP_TRY(hipMemset(H.nearcnt, 0, sizeof(int32_t) * nblk));

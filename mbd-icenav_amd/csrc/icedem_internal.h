@@ -30,7 +30,7 @@ constexpr int BLOCK = 256;          // threads per block of the per-element kern
 constexpr int MAX_PARTIALS = 4096;  // block partials reduced in one workgroup (n <= 1M)
 constexpr int NLW = 32;             // partner-list width (partners per element)
 constexpr int NLSUB = 8;            // sub-slots of each displacement maximum (one per XCD)
-constexpr int NLCTL = 3 * NLSUB + 2;  // partner-list control words
+constexpr int NLCTL = 5 * NLSUB;      // partner-list control words (layout at DevView::nlctl)
 
 // Static data, per-frame scratch and parameters of one context (device memory).
 struct DevView {
@@ -80,7 +80,9 @@ struct DevView {
     float* es;                // (n, 6) x, y, vx, vy, radius (and a pad) of every element for the frame
     uint32_t* nlctl;          // [0, 24) squared-displacement maxima (float bits) of frame k's positions in
                               // slots [8 (k % 3), 8 (k % 3) + 8); [24] a list overflowed this call;
-                              // [25] frames rebuilt since creation
+                              // [25] frames rebuilt since creation; [26] [24] at the end of the last
+                              // call; [32, 40) the call-start flags (+inf: frame 0 builds), set by the
+                              // call's cull, cleared by its first k_post
     int32_t lanes;            // lanes per element in k_force (8 or 16)
     float* shipf;             // (2n) ship forces on ice, kept for sub-steps > 0
     uint8_t* crack;           // (n) element touches a broken bond

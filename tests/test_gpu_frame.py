@@ -176,3 +176,41 @@ def test_ice_substeps_graph_equals_launches():
     rep.add_exact("bond_intact", g.bond_intact, ost.bond_intact)
     print("sub-step graph vs oracle after", K0 + K, "frames:", rep.summary(), flush=True)
     rep.assert_ok()
+
+
+def test_profiled_frames_equal_graph_replays():
+    """icedem_frames_profiled (a start/stop event pair per dispatch) leaves
+    the same state, bit for bit, as the captured graph of the same frames,
+    and reports positive kernel times."""
+    import ctypes as C
+    from icedem import _lib
+    from icedem.engine import _ptr
+    case = Case.make("c0", n_frames=900)
+    eng = case.engine()
+    st, ship = case.gpu_initial()
+    K0, K = 700, 60
+    sp = torch.as_tensor(np.stack([case.setpoint_row(k) for k in range(K0)]), device=eng.device)
+    eng.run_frames(st, ship, sp, use_graph=True)
+    sp = torch.as_tensor(np.stack([case.setpoint_row(k) for k in range(K0, K0 + K)]), device=eng.device)
+    a, ash = eng.clone_state(st), ship.clone()
+    eng.run_frames(a, ash, sp, use_graph=True)
+    b, bsh = eng.clone_state(st), ship.clone()
+    strm = torch.cuda.current_stream().cuda_stream
+    s = eng.state_struct(b, bsh)
+    inp = _lib.IceDemInputs()
+    inp.d_setpoints = _ptr(sp)
+    ms = (C.c_float * 4)()
+    torch.cuda.synchronize()
+    _lib.check(eng.lib.icedem_frames_profiled(eng.ctx, C.byref(s), C.byref(inp), K, None, ms, strm),
+               eng.ctx, "profiled")
+    torch.cuda.synchronize()
+    (ga, gas), (gb, gbs) = case.to_oracle(a, ash), case.to_oracle(b, bsh)
+    np.testing.assert_array_equal(gb.positions, ga.positions)
+    np.testing.assert_array_equal(gb.velocities, ga.velocities)
+    np.testing.assert_array_equal(gb.rotations, ga.rotations)
+    np.testing.assert_array_equal(gb.bond_intact, ga.bond_intact)
+    np.testing.assert_array_equal(gb.elem_active, ga.elem_active)
+    np.testing.assert_array_equal(gbs.eta, gas.eta)
+    np.testing.assert_array_equal(gbs.nu, gas.nu)
+    print("profiled kernel ms:", list(ms), flush=True)
+    assert ms[1] > 0.0 and ms[2] > 0.0

@@ -127,3 +127,96 @@ def test_near_candidates_missed_or_many(margin):
     case = Case.make("c0", n_frames=1100)
     case.near_cand_margin = margin
     _check(case, 1100)
+
+
+def _chained(case, sizes, edit=None):
+    """Frames in calls of the given sizes (graph-replayed); edit(call index,
+    st) may change the state between calls.  Every frame and the final state
+    are compared with the oracle run over the same frames from the same
+    edited states."""
+    eng = case.engine()
+    st, ship = case.gpu_initial()
+    o = case.oracle()
+    ost, oship = case.oracle_initial()
+    vw = case.v_water if case.v_water is not None else np.zeros((case.n, 2), np.float32)
+    before = eng.list_stats()
+    k0 = 0
+    for c, K in enumerate(sizes):
+        if edit is not None and edit(c, st):
+            ost, oship = case.to_oracle(st, ship)   # the oracle continues from the edited state
+        rows = np.stack([case.setpoint_row(k) for k in range(k0, k0 + K)])
+        hist = eng.alloc_history(K)
+        eng.run_frames(st, ship, torch.as_tensor(rows, device=eng.device), None if case.v_water is None
+                       else torch.as_tensor(vw, device=eng.device), hist=hist, use_graph=True)
+        hp = hist["positions"].cpu().numpy()
+        for k in range(K):
+            r = case.setpoint_row(k0 + k)
+            ost, oship, _ = o.frame_step(ost, oship, r[:3], r[3:], vw, ost.dpdem_floe_eligible)
+            np.testing.assert_array_equal(hp[k].T, ost.positions, err_msg=f"call {c} frame {k0 + k}")
+        k0 += K
+        g, gs = case.to_oracle(st, ship)
+        for name in ("positions", "velocities", "rotations", "angular_vels", "submergence", "bond_intact",
+                     "bond_damage", "elem_active", "dpdem_floe_eligible"):
+            np.testing.assert_array_equal(getattr(g, name), getattr(ost, name), err_msg=f"call {c} {name}")
+        np.testing.assert_array_equal(gs.eta, oship.eta)
+    after = eng.list_stats()
+    return after["rebuilt_frames"] - before["rebuilt_frames"], after
+
+
+def test_lists_chained_calls_build_only_when_needed():
+    """A call's first frame builds the lists only when an element lies
+    farther than the threshold from its build position: 40 calls of 1-17
+    frames on a field the hull enters equal the oracle frame by frame, with
+    fewer builds than calls."""
+    case = Case.make("c1small", n_frames=400)
+    sizes = [1, 7, 3, 1, 1, 17, 2, 5] * 5
+    rebuilt, stats = _chained(case, sizes)
+    print(f"{len(sizes)} calls, {sum(sizes)} frames, {rebuilt} list builds, stats {stats}", flush=True)
+    assert stats["lists"]
+    assert 1 <= rebuilt < len(sizes)
+
+
+def test_lists_state_edited_between_calls():
+    """The caller moves elements between calls, by more than the threshold
+    (the next call builds) and by less (the lists stay valid), and flips
+    elements active and frozen: every frame still equals the oracle."""
+    case = Case.make("c1small", n_frames=200)
+    n = case.n
+    rng = np.random.default_rng(5)
+    far = rng.choice(n, 6, replace=False)
+    near = rng.choice(n, 12, replace=False)
+
+    def edit(c, st):
+        if c == 3:      # a far move: two elements swap places
+            with torch.no_grad():
+                a, b = int(far[0]), int(far[1])
+                pa = st.positions[a].clone()
+                st.positions[a] = st.positions[b]
+                st.positions[b] = pa
+            return True
+        if c == 5:      # small moves, inside the threshold
+            with torch.no_grad():
+                idx = torch.as_tensor(near, device=st.positions.device)
+                st.positions[idx] += 0.004
+            return True
+        if c == 7:      # activity flipped both ways, and a 3 m jump
+            with torch.no_grad():
+                st.elem_active[torch.as_tensor(far[:3], device=st.elem_active.device)] = True
+                st.elem_active[torch.as_tensor(far[3:], device=st.elem_active.device)] = False
+                st.positions[int(far[2])] += 3.0
+            return True
+        return False
+
+    rebuilt, stats = _chained(case, [4, 9, 2, 5, 6, 3, 8, 6, 20, 1], edit)
+    print(f"edited calls: {rebuilt} list builds, stats {stats}", flush=True)
+    assert rebuilt >= 3     # the first call, and the calls after the far moves
+
+
+def test_lists_overflow_chained_calls():
+    """An overflowed list forces the next call's first frame to build (the
+    grids path keeps its buckets filled); chained calls equal the oracle."""
+    case = Case.make("ring", n_frames=200)
+    rebuilt, stats = _chained(case, [3, 1, 5, 2, 9])
+    print(f"ring chained: {rebuilt} list builds, stats {stats}", flush=True)
+    assert stats["overflowed"]
+    assert rebuilt >= 5
