@@ -2277,10 +2277,13 @@ __global__ __launch_bounds__(BLOCK) void k_post(const DevView* __restrict__ V, c
 // kb_post); the instances' contexts and call descriptors
 // are views[i], descs[i].  Every instance runs the same frame k.
 
+// a call's first cull: frame 0's near tests inside the culling workgroups,
+// frame 1's near candidates listed
 __global__ __launch_bounds__(BLOCK) void kb_cull(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
                                                  const int4* __restrict__ map) {
+    __shared__ NearLds nl;
     const int4 m = map[blockIdx.x];
-    cull_kernel_body(views + m.x, descs + m.x, m.y, nullptr);
+    cull_kernel_body(views + m.x, descs + m.x, m.y, &nl);
 }
 
 __global__ __launch_bounds__(BLOCK) void kb_near(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
@@ -2304,10 +2307,21 @@ void kb_force(const DevView* __restrict__ views, const CallDesc* __restrict__ de
                       nullptr, L);
 }
 
+// map entries with m.w < 0 (first in the grid, so they start with the
+// kernel): near workgroup m.y of m.z of instance m.x, the next frame's near
+// tests over the candidates the previous cull listed (none after a call's
+// last frame); the others: post workgroup m.y of m.z, m.w element blocks,
+// whose culls list the candidates of the frame after and test a near
+// element the candidates missed
 __global__ __launch_bounds__(BLOCK) void kb_post(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
                                                  const int4* __restrict__ map, int k, int last, int set) {
+    __shared__ NearLds nl;
     const int4 m = map[blockIdx.x];
-    post_block(views + m.x, descs + m.x, k, 1, last, m.y, m.w, set, last ? 0 : 1, m.z - m.w, false, nullptr);
+    if (m.w < 0) {
+        if (!last) near_cand_block(views + m.x, descs + m.x, m.y, m.z, 1, set ^ 1);
+        return;
+    }
+    post_block(views + m.x, descs + m.x, k, 1, last, m.y, m.w, set, last ? 0 : 1, m.z - m.w, false, &nl);
 }
 
 

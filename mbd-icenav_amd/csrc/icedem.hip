@@ -924,8 +924,8 @@ struct IceDemBatch {
     hipEvent_t desc_done = nullptr;
     // workgroup maps of the batched frame kernels: (instance, workgroup of
     // the instance, the instance's workgroup count, the instance's nextra)
-    int4* map_cull = nullptr; int4* map_near = nullptr; int4* map_force = nullptr; int4* map_post = nullptr;
-    int n_cull = 0, n_near = 0, n_force = 0, n_post = 0;
+    int4* map_cull = nullptr; int4* map_force = nullptr; int4* map_post = nullptr;
+    int n_cull = 0, n_force = 0, n_post = 0;
     int lanes = GRP;             // lanes per element of kb_force
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;   // (frames, mode)
     std::map<std::pair<int, int>, hipGraph_t> graph_defs;
@@ -977,7 +977,7 @@ static void batch_free(IceDemBatch* b) {
     if (b->ddesc) (void)hipFree(b->ddesc);
     if (b->hdesc) (void)hipHostFree(b->hdesc);
     if (b->desc_done) (void)hipEventDestroy(b->desc_done);
-    for (int4* m : {b->map_cull, b->map_near, b->map_force, b->map_post})
+    for (int4* m : {b->map_cull, b->map_force, b->map_post})
         if (m) (void)hipFree(m);
     delete b;
 }
@@ -1021,14 +1021,16 @@ int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** ou
         const int nbe = (std::max(H.n, std::max(std::max(H.G, H.G2), H.nf)) + BLOCK - 1) / BLOCK;
         const int npost = nbe + (H.nb + BLOCK - 1) / BLOCK;
         for (int l = 0; l < H.nblk; ++l) mc.push_back(make_int4(i, l, H.nblk, 0));
-        for (int l = 0; l < nnear; ++l) mn.push_back(make_int4(i, l, nnear, 0));
+        for (int l = 0; l < nnear; ++l) mn.push_back(make_int4(i, l, nnear, -1));
         for (int l = 0; l < nextra + fblk; ++l) mf.push_back(make_int4(i, l, nextra + fblk, nextra));
         for (int l = 0; l < npost; ++l) mp.push_back(make_int4(i, l, npost, nbe));
     }
+    // kb_post: every instance's near workgroups first, then the post blocks
+    mp.insert(mp.begin(), mn.begin(), mn.end());
     // kb_force only: 256 patches 64.3 -> 62.2 us per frame; dealing kb_post
     // the same way gained less (62.4)
     mf = xcd_deal(mf);
-    b->n_cull = (int)mc.size(); b->n_near = (int)mn.size(); b->n_force = (int)mf.size(); b->n_post = (int)mp.size();
+    b->n_cull = (int)mc.size(); b->n_force = (int)mf.size(); b->n_post = (int)mp.size();
     hipError_t e = hipSetDevice(b->device);
     std::vector<DevView> hv(n_inst);
     for (int i = 0; i < n_inst; ++i) hv[i] = ctxs[i]->hv;
@@ -1041,7 +1043,7 @@ int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** ou
     if (e == hipSuccess) e = hipMemcpy(b->dviews, hv.data(), sizeof(DevView) * n_inst, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipHostMalloc((void**)&b->hdesc, sizeof(CallDesc) * n_inst, hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->desc_done, hipEventDisableTiming);
-    upmap(&b->map_cull, mc); upmap(&b->map_near, mn); upmap(&b->map_force, mf); upmap(&b->map_post, mp);
+    upmap(&b->map_cull, mc); upmap(&b->map_force, mf); upmap(&b->map_post, mp);
     if (e != hipSuccess) {
         set_err(nullptr, std::string("icedem_batch_create: ") + hipGetErrorString(e));
         batch_free(b);
@@ -1078,8 +1080,6 @@ static void batch_launch(IceDemBatch* b, int frames, hipStream_t stream) {
     for (int k = 0; k < frames; ++k) {
         const int last = (k == frames - 1) ? 1 : 0;
         const int set = k & 1;
-        hipLaunchKernelGGL(kb_near, dim3(b->n_near), dim3(BLOCK), 0, stream, b->dviews, b->ddesc,
-                           (const int4*)b->map_near, set);
         hipLaunchKernelGGL(b->lanes == 8 ? kb_force<8> : kb_force<16>, dim3(b->n_force), dim3(BLOCK), 0, stream, b->dviews, b->ddesc,
                            (const int4*)b->map_force, k, last, set);
         hipLaunchKernelGGL(kb_post, dim3(b->n_post), dim3(BLOCK), 0, stream, b->dviews, b->ddesc,
