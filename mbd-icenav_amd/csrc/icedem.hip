@@ -970,6 +970,33 @@ static std::vector<int4> xcd_deal(const std::vector<int4>& m) {
     return out;
 }
 
+// Tail entries t placed before the dealt element entries m: position p of
+// the result runs on XCD p % 8, and a tail goes where its instance's element
+// workgroups run while that XCD has tails left (else on the XCD with the most
+// tails left).
+static std::vector<int4> tails_first(const std::vector<int4>& t, const std::vector<int4>& m, int n_inst) {
+    constexpr int NX = 8;
+    const size_t T = t.size();
+    std::vector<int> xcd(n_inst, -1);
+    for (size_t w = 0; w < m.size(); ++w)
+        if (xcd[m[w].x] < 0) xcd[m[w].x] = (int)((T + w) % NX);
+    std::vector<std::vector<int4>> q(NX);
+    for (const int4& e : t) q[xcd[e.x] >= 0 ? xcd[e.x] : 0].push_back(e);
+    std::vector<size_t> pos(NX, 0);
+    std::vector<int4> out;
+    out.reserve(T + m.size());
+    for (size_t p = 0; p < T; ++p) {
+        int x = (int)(p % NX);
+        if (pos[x] >= q[x].size()) {
+            for (int y = 0; y < NX; ++y)
+                if (q[y].size() - pos[y] > q[x].size() - pos[x]) x = y;
+        }
+        out.push_back(q[x][pos[x]++]);
+    }
+    out.insert(out.end(), m.begin(), m.end());
+    return out;
+}
+
 static void batch_free(IceDemBatch* b) {
     for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.second);
     for (auto& g : b->graph_defs) (void)hipGraphDestroy(g.second);
@@ -1008,7 +1035,7 @@ int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** ou
         for (int i = 0; i < n_inst; ++i) total += ctxs[i]->hv.n;
         b->lanes = resolve_lanes(ctxs[0]->hv.p.force_lanes, total);
     }
-    std::vector<int4> mc, mn, mf, mp;
+    std::vector<int4> mc, mn, mt, mf, mp;
     for (int i = 0; i < n_inst; ++i) {
         const DevView& H = ctxs[i]->hv;
         const int fblk = force_chunks(H.n, b->lanes);
@@ -1022,7 +1049,8 @@ int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** ou
         const int npost = nbe + (H.nb + BLOCK - 1) / BLOCK;
         for (int l = 0; l < H.nblk; ++l) mc.push_back(make_int4(i, l, H.nblk, 0));
         for (int l = 0; l < nnear; ++l) mn.push_back(make_int4(i, l, nnear, -1));
-        for (int l = 0; l < nextra + fblk; ++l) mf.push_back(make_int4(i, l, nextra + fblk, nextra));
+        for (int l = 0; l < nextra; ++l) mt.push_back(make_int4(i, l, nextra + fblk, nextra));
+        for (int l = nextra; l < nextra + fblk; ++l) mf.push_back(make_int4(i, l, nextra + fblk, nextra));
         for (int l = 0; l < npost; ++l) mp.push_back(make_int4(i, l, npost, nbe));
     }
     // kb_post: every instance's near workgroups first, then the post blocks
@@ -1030,6 +1058,11 @@ int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** ou
     // kb_force only: 256 patches 64.3 -> 62.2 us per frame; dealing kb_post
     // the same way gained less (62.4)
     mf = xcd_deal(mf);
+    // the instances' tail workgroups (ship forces, ship step, eligibility: a
+    // chain of dependent round trips) come first, each on its instance's XCD
+    // where the counts allow, so every chain starts with the kernel instead
+    // of behind the element blocks dealt before it
+    mf = tails_first(mt, mf, n_inst);
     b->n_cull = (int)mc.size(); b->n_force = (int)mf.size(); b->n_post = (int)mp.size();
     hipError_t e = hipSetDevice(b->device);
     std::vector<DevView> hv(n_inst);
