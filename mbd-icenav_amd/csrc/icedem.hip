@@ -935,10 +935,13 @@ struct IceDemBatch {
 // Hardware workgroup w of a launch runs on XCD w % 8.  Re-deal a batch map
 // (each instance's entries contiguous) so that every instance's workgroups
 // run on one XCD, instances spread over the XCDs by size: an instance's
-// bucket records and partner state then stay in one L2.  Entries are
+// bucket records and partner state then stay in one L2.  With interleave,
+// an XCD takes its instances' first entries, then their second ones, and so
+// on (no instance's workgroups all queue behind the others': a patch's
+// first chunk, where the hull enters, starts with the kernel).  Entries are
 // self-contained (instance, workgroup, count, extra), so the order changes
 // no result.
-static std::vector<int4> xcd_deal(const std::vector<int4>& m) {
+static std::vector<int4> xcd_deal(const std::vector<int4>& m, bool interleave = false) {
     constexpr int NX = 8;
     std::vector<std::pair<size_t, size_t>> rng;   // (first entry, entries) per instance
     for (size_t a = 0; a < m.size();) {
@@ -951,10 +954,23 @@ static std::vector<int4> xcd_deal(const std::vector<int4>& m) {
     for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
     std::stable_sort(ord.begin(), ord.end(), [&](size_t p, size_t q) { return rng[p].second > rng[q].second; });
     std::vector<std::vector<int4>> L(NX);
+    std::vector<std::vector<size_t>> own(NX);   // instances (ranges) of each XCD
     for (size_t r : ord) {
         int x = 0;
         for (int y = 1; y < NX; ++y) if (L[y].size() < L[x].size()) x = y;
         L[x].insert(L[x].end(), m.begin() + rng[r].first, m.begin() + rng[r].first + rng[r].second);
+        own[x].push_back(r);
+    }
+    if (interleave) {
+        for (int x = 0; x < NX; ++x) {
+            L[x].clear();
+            for (size_t j = 0;; ++j) {
+                bool any = false;
+                for (size_t r : own[x])
+                    if (j < rng[r].second) { L[x].push_back(m[rng[r].first + j]); any = true; }
+                if (!any) break;
+            }
+        }
     }
     std::vector<int4> out;
     out.reserve(m.size());
@@ -1057,7 +1073,7 @@ int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** ou
     mp.insert(mp.begin(), mn.begin(), mn.end());
     // kb_force only: 256 patches 64.3 -> 62.2 us per frame; dealing kb_post
     // the same way gained less (62.4)
-    mf = xcd_deal(mf);
+    mf = xcd_deal(mf, true);
     // the instances' tail workgroups (ship forces, ship step, eligibility: a
     // chain of dependent round trips) come first, each on its instance's XCD
     // where the counts allow, so every chain starts with the kernel instead
