@@ -1070,6 +1070,7 @@ int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** ou
         for (int l = 0; l < npost; ++l) mp.push_back(make_int4(i, l, npost, nbe));
     }
     // kb_post: every instance's near workgroups first, then the post blocks
+    mp = xcd_deal(mp, true);
     mp.insert(mp.begin(), mn.begin(), mn.end());
     // kb_force only: 256 patches 64.3 -> 62.2 us per frame; dealing kb_post
     // the same way gained less (62.4)
