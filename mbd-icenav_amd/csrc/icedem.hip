@@ -1056,11 +1056,12 @@ int icedem_batch_create(IceDemCtx* const* ctxs, int32_t n_inst, IceDemBatch** ou
         const DevView& H = ctxs[i]->hv;
         const int fblk = force_chunks(H.n, b->lanes);
         const int nextra = TAIL_GROUPS;
-        // one wavefront per 16 elements (a quarter of a patch is near the ship
-        // at most), and at least 8: a small patch with the bow inside it can
-        // have most of its elements near (256 patches: 65.1 -> 64.1 us per
-        // frame against a floor of 4 waves)
-        const int nnear = std::max(2, std::min(64, (H.n + 63) / 64));
+        // kb_post's near workgroups of the instance: one wavefront per 64
+        // elements and at least 8 (each wave loops over the candidates);
+        // on the 256-patch batch 29.6 us per frame against 30.2 with one
+        // wavefront per 16 elements, 31.5 with one per 8, 30.5 with a floor
+        // of 4 waves
+        const int nnear = std::max(2, std::min(64, (H.n + 255) / 256));
         const int nbe = (std::max(H.n, std::max(std::max(H.G, H.G2), H.nf)) + BLOCK - 1) / BLOCK;
         const int npost = nbe + (H.nb + BLOCK - 1) / BLOCK;
         for (int l = 0; l < H.nblk; ++l) mc.push_back(make_int4(i, l, H.nblk, 0));
