@@ -330,8 +330,10 @@ def rollout_bench(R: int = 256, frames: int = 400, hold: int = 10, n_knots: int 
     scenes = ro.rollout_patches(R, workers=workers)
     t_gen = time.perf_counter() - t0
     n_el = [len(sc.polygons) for sc in scenes]
-    mode, cf, tune = ro.autotune(scenes, sample=32, frames=40)
-    rb = ro.RolloutBatch(scenes, list(range(R)), cell_size_factor=cf)
+    conf, tune = ro.autotune(scenes, frames=100)
+    mode, cf = conf["mode"], conf["cell_size_factor"]
+    rb = ro.RolloutBatch(scenes, list(range(R)), cell_size_factor=cf, force_lanes=conf["force_lanes"],
+                         list_skin=conf["list_skin"])
     rb.set_mode(mode)
     ctl_np = ro.nominal_controls(R, n_knots, seed=0)
     ctl = torch.as_tensor(ctl_np)
@@ -367,7 +369,8 @@ def rollout_bench(R: int = 256, frames: int = 400, hold: int = 10, n_knots: int 
         "workload": f"{R} rollouts x {frames} frames ({n_knots} knots x hold {hold}); patches seed 1000+r, "
                     f"{min(n_el)}-{max(n_el)} elements (sum {sum(n_el)}), concentration 0.2-0.6",
         "batched_rollouts_per_s": R / best, "batched_s": best, "mode": mode, "cell_size_factor": cf,
-        "autotune_s": {f"{m}/{c}": t for (m, c), t in tune.items()},
+        "force_lanes": conf["force_lanes"] or "auto", "list_skin": conf["list_skin"] or "default",
+        "autotune_s": tune,
         "plain_gpu_rollouts_per_s": R / t_plain, "plain_gpu_s": t_plain,
         "batched_vs_plain": t_plain / best,
         "batched_equals_plain": bool(np.array_equal(out_b, out_s)),

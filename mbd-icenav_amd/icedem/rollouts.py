@@ -5,10 +5,11 @@ The reference's MPPI/MBD planners (``jax_mppi_planner.py:34-115``,
 model against a costmap; they never roll out the DEM.  This module scores
 candidates with the DEM itself: every candidate is an instance (a scene
 context + its own state) and all instances advance ``frames`` frames in one
-graph launch (``icedem_batch_rollouts``): either every frame as three
+graph launch (``icedem_batch_rollouts``): either every frame as two
 kernels spanning all instances (mode "frames") or one workgroup per instance
-running all frames (mode "scenes").  ``autotune`` picks the mode and the grid
-cell size by timing.
+running all frames (mode "scenes").  ``autotune`` picks the mode, the grid
+cell size, the force pass's lanes per element and the partner-list skin by
+timing the batch.
 
 Ship control for rollouts is direct (ship_mode 1): a control sequence of
 ``n_knots`` knots (u, r_cmd) in [-1, 1], each held ``hold`` frames, gives the
@@ -244,31 +245,54 @@ def rollout_patches(R: int, workers: int = 1) -> List[Scene]:
         return pool.map(_patch, params, chunksize=max(1, R // (4 * workers)))
 
 
-def autotune(scenes: Sequence[Scene], cfg=None, ship: Optional[ShipSpec] = None, sample: int = 32,
-             frames: int = 40, cell_factors=(1.0, 1.25, 1.6), modes=("frames", "scenes"), seed: int = 0):
-    """Pick the batch mode and grid cell size factor by timing `frames` frames
-    of a `sample`-instance batch for every combination; returns (mode,
-    cell_factor, {(mode, factor): seconds})."""
+def autotune(scenes: Sequence[Scene], cfg=None, ship: Optional[ShipSpec] = None, sample: Optional[int] = None,
+             frames: int = 100, cell_factors=(1.0, 1.25), lanes=(0, 16), skins=(None, 1.5),
+             modes=("frames", "scenes"), seed: int = 0):
+    """Pick the batch's launch configuration by timing `frames` frames of the
+    batch (the first `sample` instances; None: all of them, so the timed
+    launches have the real batch's workgroup counts) for every combination
+    of grid cell size factor, lanes per element of the force pass (0: the
+    library's choice for the batch's size) and partner-list skin (None: the
+    engine default) in frames mode, and of cell size factor in scenes mode
+    (one workgroup per instance: no lists, fixed lanes).  Returns (best,
+    times): best = dict(mode, cell_size_factor, force_lanes, list_skin), times
+    = {label: seconds}.  Every configuration gives the same results bit for
+    bit; only the time differs."""
     import time
-    sub = list(range(min(sample, len(scenes))))
-    ctl = torch.as_tensor(nominal_controls(len(sub), 4, seed=seed))
-    times = {}
+    n = len(scenes) if sample is None else min(sample, len(scenes))
+    sub = list(range(n))
+    ctl = torch.as_tensor(nominal_controls(n, 4, seed=seed))
+    times, confs = {}, {}
+
+    def timed(rb):
+        rb.reset(); rb.run(ctl, max(1, frames // 4), frames)          # capture + warm
+        best = float("inf")
+        for _ in range(2):
+            rb.reset()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rb.run(ctl, max(1, frames // 4), frames)
+            best = min(best, time.perf_counter() - t0)
+        return best
+
     for cf in cell_factors:
-        rb = RolloutBatch([scenes[i] for i in sub], list(range(len(sub))), cfg, ship, cell_size_factor=cf)
-        for m in modes:
-            rb.set_mode(m)
-            rb.reset(); rb.run(ctl, max(1, frames // 4), frames)          # capture + warm
-            best = float("inf")
-            for _ in range(3):
-                rb.reset()
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                rb.run(ctl, max(1, frames // 4), frames)
-                best = min(best, time.perf_counter() - t0)
-            times[(m, cf)] = best
-        rb.close()
-    (m, cf) = min(times, key=times.get)
-    return m, cf, times
+        for ln in (lanes if "frames" in modes else (0,)):
+            for sk in (skins if "frames" in modes else (None,)):
+                rb = RolloutBatch([scenes[i] for i in sub], list(range(n)), cfg, ship, cell_size_factor=cf,
+                                  force_lanes=ln, list_skin=sk)
+                try:
+                    for m in modes:
+                        if m == "scenes" and (ln, sk) != (lanes[0], skins[0]):
+                            continue   # scenes mode uses neither setting
+                        rb.set_mode(m)
+                        label = (f"{m}/cell{cf}" if m == "scenes" else
+                                 f"{m}/cell{cf}/lanes{ln or 'auto'}/skin{sk if sk is not None else 'default'}")
+                        times[label] = timed(rb)
+                        confs[label] = dict(mode=m, cell_size_factor=cf, force_lanes=ln, list_skin=sk)
+                finally:
+                    rb.close()
+    best = min(times, key=times.get)
+    return confs[best], times
 
 
 def nominal_controls(R: int, n_knots: int = 40, u: float = 0.7, r: float = 0.0, sigma: float = 0.2,
