@@ -1589,6 +1589,36 @@ __device__ __forceinline__ float frozen_epilogue(const DevView* __restrict__ V, 
     return 0.0f;
 }
 
+// The epilogue of an element frozen through a call's first frame (mode 0,
+// one ice sub-step): what force_epilogue's masked integration leaves, formed
+// without its other inputs.  The next state is the state (px, py, vx, vy,
+// ro, om); the force magnitude fmr is the near test's (not positive: a
+// positive one needs a hull contact, which would have activated the
+// element's floe); the submergence takes the no-contact branch (likewise);
+// the activity stays 0, so it is not written.  With lists it returns the
+// squared displacement from the build position (0 on a build frame, which
+// records the position).
+__device__ __forceinline__ bool frozen_first(const IceDemParams& P, int k, int sub, int mode) {
+    return mode == 0 && sub == 0 && k == 0 && P.n_ice_steps == 1;
+}
+__device__ __forceinline__ float frozen_first_epilogue(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
+                                                       int e, float px, float py, float vx, float vy, float ro,
+                                                       float om, float sub0, float fmr, bool lists, bool rebuild) {
+    const IceDemParams& P = V->p;
+    const int n = V->n;
+    float bx = px, by = py;
+    if (lists && !rebuild) { bx = G(V->xb)[e]; by = G(V->xb)[n + e]; }
+    G(V->nxt_pos)[e] = px; G(V->nxt_pos)[n + e] = py;
+    G(V->nxt_vel)[e] = vx; G(V->nxt_vel)[n + e] = vy;
+    G(V->nxt_rot)[e] = ro; G(V->nxt_omega)[e] = om;
+    G(V->fm)[e] = fmr;
+    G(D->s.d_sub)[e] = vmax(0.0f, sub0 - P.inv_rise_time * sub0 * P.dt);
+    if (!lists) return 0.0f;
+    if (rebuild) { G(V->xb)[e] = px; G(V->xb)[n + e] = py; }
+    const float ddx = px - bx, ddy = py - by;
+    return ddx * ddx + ddy * ddy;
+}
+
 // The epilogue of element e by one thread (jax_simulation.py:140-268): ship
 // forces and submergence (first sub-step), drag, rotational drag, totals and
 // the integration into the next-state buffers, from the values its group
@@ -1777,14 +1807,22 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
         const bool act_bond = act_raw || fcoll;   // activity seen by the bond pass (mode 0)
         BondIn bin0 = {0.0f, 0.0f, 0.0f, false, false};
         BondPre bpre0 = {0.0f, 0.0f, 0.0f, false, false};
-        if (l < deg) {   // read ahead of the contact pass
+        // read ahead of the contact pass for an element known to be live
+        // (active, or every element in mode 1); with 8 lanes (large fields)
+        // one that its floe's contact activates reads its bond state after
+        // the contact pass and a frozen one none; with 16 lanes (small
+        // fields, latency-bound) every element reads ahead
+        const bool ahead = mode != 0 || act_raw || GR >= 16;
+        if (ahead && l < deg) {
             if (pre) bpre0 = bond_pre(V, D, bt0, mode, set);
             else bin0 = bond_in(V, D, pos, bt0, mode, set);
         }
         // A frame sub-step leaves an element that is frozen in it where it
         // is whatever its forces (the integration is masked by elem_active,
         // jax_simulation.py:256-261), so a frozen element forms no contact or
-        // bond sums: only its crack flag, and on a build frame its list.
+        // bond sums and no crack flag (the fracture pass reads an element's
+        // crack flag only for bonds whose ends are both active): on a build
+        // frame only its list.
         const bool live = mode != 0 || (sub == 0 ? act_bond : act_raw);
         float fix = 0.0f, fiy = 0.0f, mi = 0.0f;
         if (lists) {
@@ -1795,14 +1833,18 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
             contact_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, L, g, fix, fiy, mi);
         }
         float fbx = 0.0f, fby = 0.0f, mb = 0.0f;
-        bool crack;
-        if (!live)
-            crack = crack_group<GR>(V, D, e, deg, pre ? bpre0.intact : bin0.intact, pre);
-        else if (pre)
+        bool crack = false;
+        if (live && !ahead && l < deg) {
+            if (pre) bpre0 = bond_pre(V, D, bt0, mode, set);
+            else bin0 = bond_in(V, D, pos, bt0, mode, set);
+        }
+        if (!live) {
+        } else if (pre) {
             bond_pre_group<GR>(V, D, e, act_bond, deg, bt0, bpre0, mode, set, fbx, fby, mb, crack);
-        else
+        } else {
             bond_force_group<GR>(V, D, pos, e, px, py, mode == 1 ? act_raw : act_bond, deg, bt0, bp0, bin0, mode, set, L,
                                  g, fbx, fby, mb, crack);
+        }
         if (l == 0) {
             float* sv = L.stg(g);
             sv[S_PX] = px; sv[S_PY] = py; sv[S_VX] = vx; sv[S_VY] = vy; sv[S_R] = r;
@@ -1811,10 +1853,15 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
             sv[S_FLAGS] = __int_as_float((act_raw ? 1 : 0) | (act_bond ? 2 : 0) | (crack ? 4 : 0));
             // 16 lanes (small fields, latency-bound): the group's lane 0
             // finishes its own element, no barrier
-            if (GR >= 16)
-                d2 = (!live && frozen_fast(V->p, k, sub, mode))
-                         ? frozen_epilogue(V, D, e, crack)
-                         : force_epilogue(V, D, sub, src_is_state, mode, fship_in, fout, mout, sv, e, lists, rebuild);
+            if (GR >= 16) {
+                if (!live && frozen_fast(V->p, k, sub, mode))
+                    d2 = frozen_epilogue(V, D, e, crack);
+                else if (!live && frozen_first(V->p, k, sub, mode))
+                    d2 = frozen_first_epilogue(V, D, e, px, py, vx, vy, ei.ro, ei.om, ei.sub0, ei.fmr, lists,
+                                               rebuild);
+                else
+                    d2 = force_epilogue(V, D, sub, src_is_state, mode, fship_in, fout, mout, sv, e, lists, rebuild);
+            }
         }
     }
     if (GR < 16) {
@@ -1829,6 +1876,10 @@ __device__ __forceinline__ void force_body(const DevView* __restrict__ V, const 
             const int fl = __float_as_int(sv[S_FLAGS]);
             if ((fl & 2) == 0 && frozen_fast(V->p, k, sub, mode)) {
                 d2 = frozen_epilogue(V, D, et, (fl & 4) != 0);
+            } else if ((fl & 2) == 0 && frozen_first(V->p, k, sub, mode)) {
+                const float fmr = G(V->nearf)[et] != 0 ? G(V->fm_raw)[et] : 0.0f;
+                d2 = frozen_first_epilogue(V, D, et, sv[S_PX], sv[S_PY], sv[S_VX], sv[S_VY], G(D->s.d_rot)[et],
+                                           G(D->s.d_omega)[et], G(D->s.d_sub)[et], fmr, lists, rebuild);
             } else {
                 epi_put(sv, epi_in(V, D, k, sub, src_is_state, mode, et));
                 d2 = force_epilogue(V, D, sub, src_is_state, mode, fship_in, fout, mout, sv, et, lists, rebuild);
