@@ -140,8 +140,12 @@ struct ShipPose {
 __device__ __forceinline__ ShipPose make_pose(float sx, float sy, float sh, float hl, float hb) {
     ShipPose P;
     P.sx = sx; P.sy = sy; P.hl = hl; P.hb = hb;
-    dm_sincosf(-sh, &P.s_neg, &P.c_neg);
+    // sin(-h) = -sin(h) and cos(-h) = cos(h) bit for bit in dm_sincos64:
+    // the reduction of -h is exactly the negation of h's (same quadrant
+    // count, negated), and the sine series is odd, the cosine series even
     dm_sincosf(sh, &P.s_h, &P.c_h);
+    P.s_neg = -P.s_h;
+    P.c_neg = P.c_h;
     return P;
 }
 
