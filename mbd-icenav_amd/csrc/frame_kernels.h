@@ -915,17 +915,24 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
     };
     __shared__ int ncnt;
     __shared__ float lpre[32];                    // the ship step's force-independent part
-    // staged through LDS while the near counts load (not held in registers)
-    if (preship && threadIdx.x < 23) lpre[threadIdx.x] = G(V->shpre)[threadIdx.x];
-    if (threadIdx.x == 0) {
-        ncnt = 0;
-        if (V->nl_on && nl_rebuild(V, k)) atomicAdd(nl_rebuilds(V), 1u);   // frames that built the lists
+    // The first round trip's loads issue together: the ship step's
+    // force-independent part (a value per thread, staged into LDS once the
+    // near counts are in) and the near counts of the first NPRE rounds of
+    // the listing loop below.
+    float pv = 0.0f;
+    if (preship && threadIdx.x < 23) pv = G(V->shpre)[threadIdx.x];
+    constexpr int NPRE = 4;
+    int nc[NPRE];
+    #pragma unroll
+    for (int i = 0; i < NPRE; ++i) {
+        const int q = 64 * (w + NW * i) + lane;
+        nc[i] = q < nb ? G(V->nearcnt)[q] : 0;
     }
+    if (threadIdx.x == 0) ncnt = 0;
     for (int i = threadIdx.x; i < 3 * nbpad; i += BLOCK) lv[i] = 0.0f;
     __syncthreads();
     // the blocks holding a near element, listed (order free: partials go by index)
-    for (int base = 64 * w; base < nb; base += 64 * NW) {
-        const bool has = base + lane < nb && G(V->nearcnt)[base + lane] > 0;
+    auto list = [&](int base, bool has) {
         const uint64_t m = __ballot(has);
         if (m != 0ull) {
             const int first = __ffsll((unsigned long long)m) - 1;
@@ -936,7 +943,12 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
                 nlist[qb + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] =
                     base + lane;
         }
-    }
+    };
+    #pragma unroll
+    for (int i = 0; i < NPRE; ++i) list(64 * (w + NW * i), nc[i] > 0);
+    for (int base = 64 * (w + NW * NPRE); base < nb; base += 64 * NW)
+        list(base, base + lane < nb && G(V->nearcnt)[base + lane] > 0);
+    if (preship && threadIdx.x < 23) lpre[threadIdx.x] = pv;
     __syncthreads();
     const int cnt = ncnt;
     for (int i = w; i < cnt; i += NW) {
@@ -980,6 +992,8 @@ __device__ __forceinline__ void force_tail(const DevView* __restrict__ V, const 
         }
     } else {
         floe_eligibility(V, D, scale, set, 1);
+        // frames that built the lists, counted beside the ship step
+        if (threadIdx.x == 64 && V->nl_on && nl_rebuild(V, k)) atomicAdd(nl_rebuilds(V), 1u);
     }
 }
 
