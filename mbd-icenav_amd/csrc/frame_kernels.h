@@ -630,10 +630,11 @@ __device__ __forceinline__ void cull_kernel_body(const DevView* __restrict__ V, 
     const bool valid = e < n;
     GAS const float* pos = G(D->s.d_pos);
     GAS const float* vel = G(D->s.d_vel);
-    float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f, r = 0.0f, bnd = 0.0f;
+    float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f, r = 0.0f, bnd = 0.0f, ro = 0.0f;
     if (valid) {
         x = pos[e]; y = pos[n + e]; vx = vel[e]; vy = vel[n + e];
         r = G(V->radius)[e]; bnd = G(V->bound)[e];
+        if (nl) ro = G(D->s.d_rot)[e];   // for the near test, read with the state
     }
     bool near;
     // frame 0 builds the lists when an element lies farther from its build
@@ -647,7 +648,7 @@ __device__ __forceinline__ void cull_kernel_body(const DevView* __restrict__ V, 
     }
     const int cnt = cull_chunk(V, lb, valid, e, x, y, vx, vy, r, bnd, G(D->s.d_ship), 0, near, nl == nullptr, true,
                                true, nl ? 1 : -1);
-    if (nl) near_inline(V, D, nl, cnt, near, e, x, y, near ? G(D->s.d_rot)[e] : 0.0f, 0, true);
+    if (nl) near_inline(V, D, nl, cnt, near, e, x, y, ro, 0, true);
     if (lb == 0 && threadIdx.x == 0) store_pose(V, D->s.d_ship);
     if (V->nl_on) {
         if (__ballot(far) != 0ull && (threadIdx.x & 63) == 0)
