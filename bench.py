@@ -532,20 +532,26 @@ def surrogate_bench(plain_rps: float, rounds: int = 800, epochs: int = 20, test_
             dc = torch.as_tensor(cand, device="cuda")
             pred = P.surrogate_costs(dc).cpu().numpy().astype(np.float64)
             cmc = P.costmap_costs(dc).cpu().numpy()
-            ts, tc = np.argsort(pred)[:k], np.argsort(cmc)[:k]
+            ens = 0.5 * (pred + cmc)
+            ts, tc, te = np.argsort(pred)[:k], np.argsort(cmc)[:k], np.argsort(ens)[:k]
             row.update({"eval_candidates": eval_samples,
                         "surrogate_rel_err": {"vs_single": rel(pred, dem), "vs_jitter_mean": rel(pred, demj)},
+                        "ensemble_rel_err": {"vs_single": rel(ens, dem), "vs_jitter_mean": rel(ens, demj)},
                         "costmap_rel_err": {"vs_single": rel(cmc, dem), "vs_jitter_mean": rel(cmc, demj)},
                         "single_vs_jitter_mean_rel": rel(dem, demj),
                         "spearman": {"surrogate_vs_single": _spearman(pred, dem),
                                      "surrogate_vs_jitter_mean": _spearman(pred, demj),
                                      "costmap_vs_single": _spearman(cmc, dem),
+                                     "ensemble_vs_single": _spearman(ens, dem),
+                                     "ensemble_vs_jitter_mean": _spearman(ens, demj),
                                      "costmap_vs_jitter_mean": _spearman(cmc, demj),
                                      "single_vs_jitter_mean": _spearman(dem, demj)},
                         "top64_mean_cost": {"single": {"surrogate": float(dem[ts].mean()),
-                                                       "costmap": float(dem[tc].mean()), "all": float(dem.mean())},
+                                                       "costmap": float(dem[tc].mean()),
+                                                       "ensemble": float(dem[te].mean()), "all": float(dem.mean())},
                                             "jitter_mean": {"surrogate": float(demj[ts].mean()),
                                                             "costmap": float(demj[tc].mean()),
+                                                            "ensemble": float(demj[te].mean()),
                                                             "all": float(demj.mean())}},
                         "dem_cost_sd": float(dem.std()),
                         "exhaustive_batched_rollouts_per_s": eval_samples / t_ex})
@@ -607,6 +613,14 @@ def surrogate_bench(plain_rps: float, rounds: int = 800, epochs: int = 20, test_
                     "surrogate_better": int(sum(r["top64_mean_cost"][c]["surrogate"] < r["top64_mean_cost"][c]["costmap"]
                                                 for r in rows)), "patches": len(rows)}
                 for c in ("single", "jitter_mean")},
+            "top64_ensemble_minus_costmap": {
+                c: {"mean": float(np.mean([r["top64_mean_cost"][c]["ensemble"] - r["top64_mean_cost"][c]["costmap"]
+                                           for r in rows])),
+                    "ensemble_better": int(sum(r["top64_mean_cost"][c]["ensemble"] < r["top64_mean_cost"][c]["costmap"]
+                                               for r in rows)), "patches": len(rows)}
+                for c in ("single", "jitter_mean")},
+            "ensemble_rel_err_vs_single": med(lambda r: r["ensemble_rel_err"]["vs_single"]),
+            "ensemble_rel_err_vs_jitter_mean": med(lambda r: r["ensemble_rel_err"]["vs_jitter_mean"]),
         }
     return out
 

@@ -560,9 +560,11 @@ class Planner:
         the result also holds the candidates (S, n_knots, 2) and their costs
         (S,): the ranker's, replaced by the DEM's where judged.  ``ranker``
         "costmap" ranks the candidates by the calibrated costmap cost instead
-        of the surrogate (the same DEM judge and fallback)."""
-        if ranker not in ("surrogate", "costmap"):
-            raise ValueError(f"ranker must be 'surrogate' or 'costmap', not {ranker!r}")
+        of the surrogate, "ensemble" by the mean of the two estimates (both
+        are fitted to the same rollout cost), with the same DEM judge and
+        fallback."""
+        if ranker not in ("surrogate", "costmap", "ensemble"):
+            raise ValueError(f"ranker must be 'surrogate', 'costmap' or 'ensemble', not {ranker!r}")
         rng = np.random.default_rng(seed)
         dev = self.judge.device
         cand = np.clip(nominal[None] + rng.normal(0.0, sigma, (S, self.nk, 2)), -1.0, 1.0).astype(np.float32)
@@ -570,7 +572,12 @@ class Planner:
         ctl = torch.as_tensor(cand, device=dev)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        pred = self.surrogate_costs(ctl) if ranker == "surrogate" else self.costmap_costs(ctl).float()
+        if ranker == "surrogate":
+            pred = self.surrogate_costs(ctl)
+        elif ranker == "costmap":
+            pred = self.costmap_costs(ctl).float()
+        else:
+            pred = 0.5 * (self.surrogate_costs(ctl).float() + self.costmap_costs(ctl).float())
         order = torch.argsort(pred).cpu().numpy()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
