@@ -222,10 +222,12 @@ int icedem_frames(IceDemCtx* ctx, const IceDemState* state, const IceDemInputs* 
                   const IceDemHistory* hist, const IceDemDiag* diag, int32_t use_graph, void* stream);
 
 /* icedem_frames launched on `stream` without graph replay, with a HIP event
- * recorded after every kernel; synchronizes `stream`.  h_kernel_ms[0..3] receive the summed
- * durations of k_near, k_force (all sub-steps with their cell rebuilds) and
- * k_post over the K frames, and the call's one k_cull.  Instrumentation for
- * benchmarks; no reference counterpart. */
+ * recorded after every kernel; synchronizes `stream`.  h_kernel_ms[0..3] receive 0
+ * (slot kept for a separate near-test kernel, which a single field no longer
+ * launches: its near tests run inside k_cull and k_post), the summed durations
+ * of k_force (all sub-steps with their cell rebuilds) and k_post over the K
+ * frames, and the call's one k_cull.  Instrumentation for benchmarks; no
+ * reference counterpart. */
 /* No counterpart in the reference (build diagnostics): h_out[0] = frames
  * that rebuilt the contact partner lists since the context was created,
  * h_out[1] = 1 if an element had more partners than a list holds during the

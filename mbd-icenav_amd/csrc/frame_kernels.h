@@ -1,24 +1,28 @@
 // frame_kernels.h -- the per-frame kernels of the DPDEM step.
 //
 // A call of K frames (one ice sub-step each) launches
-//     k_cull | k_near k_force k_post | k_near k_force k_post | ...
-//   k_cull    once per call: near-ship test of every element into a work
-//             queue, insertion into the cell buckets of set 0
-//   k_near    one wavefront per queued element: hull pass 1 + 2, pre-pass,
-//             floe or/max (integer atomics).  No cross-workgroup hand-off.
+//     k_cull | k_force k_post | k_force k_post | ...
+//   k_cull    once per call: near-ship test of every element, the near
+//             elements' hull tests inside the culling workgroup, the next
+//             frame's near candidates, insertion into the cell buckets of
+//             set 0, the partner lists' call-start check.
 //   k_force   16 or 8 lanes per element (k_force<GR>): activation,
 //             submergence, bond forces, contacts, drag, integration (the
-//             integration on one thread per element).  TAIL_GROUPS extra
-//             workgroups form the pairwise-tree partials of the near
-//             elements' ship forces, one wave per 256-element block; the last
-//             of them forms the totals, the thrust scale, the scaled totals,
-//             the ship reaction, floe eligibility and the ship step.
+//             integration on one thread per element); frames after a
+//             call's first skip the elements frozen through them.  A tail
+//             workgroup (dispatched first) forms the pairwise trees of the
+//             near elements' ship forces over the blocks holding a near
+//             element, the totals, the thrust scale, floe eligibility and the
+//             ship step.
 //   k_post    per bond: fracture; per element: commit, histories, and the
 //             next frame's near test and cell insertion (the bucket set of
-//             the next frame's parity); clears the set this frame used.
+//             the next frame's parity); clears the set this frame used; its
+//             first workgroups run the next frame's hull tests over the
+//             candidates the previous cull listed.
 // Frame k uses bucket set k & 1.  Between calls every counter, both bucket
 // sets and the floe flags are zero.
-// The same bodies run inside one workgroup per scene for batched rollouts.
+// The same bodies run over many contexts in one launch (kb_*) and inside
+// one workgroup per scene for batched rollouts.
 #pragma once
 #include "kernels.h"
 
@@ -371,7 +375,8 @@ __device__ __forceinline__ void block256_tree3(const DevView* __restrict__ V, GA
 // ship's heading comes from a fast sine/cosine: the 1 m margin dwarfs its
 // error, and only which elements run the exact hull test depends on it.
 // (r, bnd: the element's radius and bound radius, read by the caller with
-// its state; queue: list the near elements for a k_near launch)
+// its state; queue: list the near elements in the context's near queue, for
+// the one-workgroup scene kernel)
 // The near test of an element at (x, y) with bound radius bnd against the
 // ship at ship[0..2]: its centroid within bnd + margin of the hull's box in
 // the ship's frame (margin 1 m: the exact test; larger: a candidate test).
@@ -451,7 +456,7 @@ __device__ __forceinline__ int cull_chunk(const DevView* __restrict__ V, int chu
     return cnt;
 }
 
-// sin/cos of the ship heading for the next k_near (scal[8..11])
+// sin/cos of the ship heading for the next near tests (scal[8..11])
 __device__ __forceinline__ void store_pose(const DevView* __restrict__ V, const float* __restrict__ ship) {
     const ShipPose SP = make_pose(ship[0], ship[1], ship[2], V->p.half_len, V->p.half_beam);
     G(V->scal)[8] = SP.s_neg; G(V->scal)[9] = SP.c_neg; G(V->scal)[10] = SP.s_h; G(V->scal)[11] = SP.c_h;
@@ -478,7 +483,7 @@ __global__ __launch_bounds__(BLOCK) void k_clear_cells(const DevView* __restrict
     clear_cells(V, set, blockIdx.x * BLOCK + threadIdx.x);
 }
 
-// ── k_near ───────────────────────────────────────────────────────────────────
+// ── near-ship hull tests ─────────────────────────────────────────────────────
 
 // The static inputs of a near element's hull test (read before the test
 // that decides whether it runs, in one round trip with the element's state):
@@ -602,24 +607,6 @@ __device__ __forceinline__ void near_inline(const DevView* __restrict__ V, const
     }
 }
 
-// workgroup lb of lg working on one context's near queue: one wave per near
-// element.  The totals over the near elements (jax_simulation.py:456-470)
-// are formed by the tail of the following k_force.
-__device__ __forceinline__ void near_kernel_body(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                                 int lb, int lg, int set) {
-    const int wave = lb * (BLOCK / 64) + (threadIdx.x >> 6);
-    const int nwaves = lg * (BLOCK / 64);
-    const int count = *near_count(V);
-    const int e0 = (wave < V->n) ? G(V->near_list)[wave] : 0;   // read beside the count
-    if (wave >= count) return;
-    const ShipPose SP = stored_pose(V, D);   // heading sin/cos from the frame's ship step
-    for (int q = wave; q < count; q += nwaves) near_elem(V, D, SP, (q == wave) ? e0 : G(V->near_list)[q], set);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_near(const DevView* __restrict__ V, const CallDesc* __restrict__ D,
-                                                int set) {
-    near_kernel_body(V, D, blockIdx.x, gridDim.x, set);
-}
 
 // workgroup lb of a call's first near test (grid: nblk workgroups)
 // (with nl: and the near tests of the call's first frame)
@@ -2352,11 +2339,6 @@ __global__ __launch_bounds__(BLOCK) void kb_cull(const DevView* __restrict__ vie
     cull_kernel_body(views + m.x, descs + m.x, m.y, &nl);
 }
 
-__global__ __launch_bounds__(BLOCK) void kb_near(const DevView* __restrict__ views, const CallDesc* __restrict__ descs,
-                                                 const int4* __restrict__ map, int set) {
-    const int4 m = map[blockIdx.x];
-    near_kernel_body(views + m.x, descs + m.x, m.y, m.z, set);
-}
 
 // 8 lanes: held to 8 waves per SIMD (<= 64 VGPRs and <= 100 SGPRs, the
 // excess SGPRs spilled to VGPR lanes; 19.5 KB of LDS is 8 blocks per CU).

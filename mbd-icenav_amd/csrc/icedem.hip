@@ -738,8 +738,9 @@ static int put_desc(IceDemCtx* ctx, const CallDesc& cd, hipStream_t stream) {
 
 // Kernel timing for icedem_frames_profiled: every launch gets a start/stop
 // event pair recorded by its own dispatch (hipExtLaunchKernelGGL), so a pair
-// spans exactly that kernel's execution.  Category: 0 k_near, 1 k_force
-// (with sub-step cell rebuilds), 2 k_post, 3 k_cull.
+// spans exactly that kernel's execution.  Category: 0 unused (a separate
+// near-test kernel, no longer launched), 1 k_force (with sub-step cell
+// rebuilds), 2 k_post, 3 k_cull.
 constexpr int NPROF = 4;
 struct LaunchTimer {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pairs;
