@@ -41,8 +41,15 @@ def bound_radius(local_polys_padded: np.ndarray) -> np.ndarray:
 
 
 # Reach of the contact partner lists beyond contact (m): the lists are rebuilt
-# once an element may have moved about half of it.
-DEFAULT_LIST_SKIN = 1.0
+# once an element may have moved about half of it.  A wider skin means fewer
+# build frames (each scans the grids for every element) but longer lists for
+# the live elements, and more elements past a list's 32 partners (those scan
+# the grids every frame).  On the 1.5 x 3 km field at 50 %: 1.0 m 17.0 us per
+# frame (157 builds in 2000 frames), 2.0 m 15.6 (75), 3.0 m 15.0 (49), 4.0 m
+# 15.0, 5.0 m 20.0 (lists overflow); with every element active 2.0 and 3.0 m
+# 36.9 against 40.4 at 1.0 m, 4.0 m overflows.  2.0 m keeps a margin from
+# the overflow.
+DEFAULT_LIST_SKIN = 2.0
 
 
 def choose_grid(centroids: np.ndarray, radii: np.ndarray, dilation: float = 0.0, cell: Optional[float] = None,

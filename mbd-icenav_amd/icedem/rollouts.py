@@ -246,7 +246,7 @@ def rollout_patches(R: int, workers: int = 1) -> List[Scene]:
 
 
 def autotune(scenes: Sequence[Scene], cfg=None, ship: Optional[ShipSpec] = None, sample: Optional[int] = None,
-             frames: int = 100, cell_factors=(1.0, 1.25), lanes=(0, 16), skins=(None, 1.5),
+             frames: int = 100, cell_factors=(1.0, 1.25), lanes=(0, 16), skins=(None, 3.0),
              modes=("frames", "scenes"), seed: int = 0):
     """Pick the batch's launch configuration by timing `frames` frames of the
     batch (the first `sample` instances; None: all of them, so the timed
