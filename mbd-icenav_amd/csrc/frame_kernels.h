@@ -121,6 +121,48 @@ __device__ __forceinline__ void coarse_of(const DevView* __restrict__ V, float x
     cy = (int)fminf(fmaxf(fy, 0.0f), (float)(V->ny2 - 1));
 }
 
+// Partial list builds.  A frozen element's list stays exact while every
+// element within a list's reach (dreach) of it stays where it was at the
+// element's build.  Live elements (the only ones that move within a call)
+// stamp the coarse cells they stand in with 1 + the count of builds so far:
+// on a build frame (start and end positions), on the frame they go live, and
+// whenever they enter another cell than their build position's (staying in
+// that cell, it carries the stamp from the build or the activation already).
+// The frame before a build marks an element for it when the element is
+// active or a cell its reach box touches carries a stamp of the current
+// build interval or the one before (a stamp written while the count moves on
+// reads either value): any element that can have come within reach since
+// the last build stood in one of those cells.  Builds at a call's first
+// frame rebuild every list; the call's cull stamps for what the caller may
+// have changed between calls (moves, activity).  With every element within
+// the threshold of its build position, an element that never stood in a cell
+// of a frozen element's box since that element's build stays out of contact.
+//
+// the stamps' cell of a position: the coarse grid's, by a multiplication
+// (stamping and checking use the same monotone map, which is all the
+// argument needs)
+__device__ __forceinline__ void stamp_xy(const DevView* __restrict__ V, float x, float y, int& cx, int& cy) {
+    const IceDemParams& P = V->p;
+    const float fx = floorf((x - P.grid_x0) * V->icell2), fy = floorf((y - P.grid_y0) * V->icell2);
+    cx = (int)fminf(fmaxf(fx, 0.0f), (float)(V->nx2 - 1));
+    cy = (int)fminf(fmaxf(fy, 0.0f), (float)(V->ny2 - 1));
+}
+__device__ __forceinline__ int coarse_idx(const DevView* __restrict__ V, float x, float y) {
+    int cx, cy;
+    stamp_xy(V, x, y, cx, cy);
+    return cy * V->nx2 + cx;
+}
+__device__ __forceinline__ bool stamped_near(const DevView* __restrict__ V, float x, float y, uint32_t c) {
+    const float R = V->dreach;
+    int ax, ay, bx, by;
+    stamp_xy(V, x - R, y - R, ax, ay);
+    stamp_xy(V, x + R, y + R, bx, by);
+    bool hit = !(x == x && y == y);   // NaN position: rebuild
+    for (int cy = ay; cy <= by; ++cy)
+        for (int cx = ax; cx <= bx; ++cx) hit = hit || G(V->dstamp)[cy * V->nx2 + cx] >= c;
+    return hit;
+}
+
 // byte offset of bucket record r (fine: r < 2 G) in the records buffer (under
 // 4 GiB by construction), the record, and its capacity
 __device__ __forceinline__ uint32_t rec_off(const DevView* __restrict__ V, int r) {
@@ -630,8 +672,19 @@ __device__ __forceinline__ void cull_kernel_body(const DevView* __restrict__ V, 
     // state
     bool far = false;
     if (V->nl_on && valid) {
-        const float dx = x - G(V->xb)[e], dy = y - G(V->xb)[n + e];
+        const float bx = G(V->xb)[e], by = G(V->xb)[n + e];
+        const float dx = x - bx, dy = y - by;
         far = !(dx * dx + dy * dy <= V->nl_thr2) || *G(nl_ovf_prev(V)) != 0u;
+        // what the caller may have changed between calls leaves no stamp
+        // of its own: a frozen element away from its build position was
+        // moved (a frozen element's build writes where it stands), and an
+        // active one may have been activated or moved.  Those stamp the
+        // cell they stand in, as an element that moved there in a frame
+        // would, so that frozen neighbours whose lists predate the change
+        // are rebuilt at the next build (stamped_near).  An element active
+        // through the last build stamped its cells then already.
+        if (x != bx || y != by || G(D->s.d_active)[e] != 0)
+            G(V->dstamp)[coarse_idx(V, x, y)] = *G(nl_rebuilds(V)) + 1u;
     }
     const int cnt = cull_chunk(V, lb, valid, e, x, y, vx, vy, r, bnd, G(D->s.d_ship), 0, near, nl == nullptr, true,
                                true, nl ? 1 : -1);
@@ -1726,6 +1779,14 @@ __device__ __forceinline__ float force_epilogue(const DevView* __restrict__ V, c
         else { bx = G(V->xb)[e]; by = G(V->xb)[n + e]; }
         const float ddx = npx - bx, ddy = npy - by;
         d2 = ddx * ddx + ddy * ddy;
+        if (act) {   // stamps of a moving element (see stamped_near)
+            const int cn = coarse_idx(V, npx, npy), cb = coarse_idx(V, bx, by);
+            if (rebuild || !act_raw || cn != cb) {
+                const uint32_t st = *G(nl_rebuilds(V)) + 1u;
+                G(V->dstamp)[cn] = st;
+                if (rebuild) G(V->dstamp)[cb] = st;
+            }
+        }
     }
     return d2;
 }
@@ -1927,6 +1988,7 @@ __device__ __forceinline__ void force_body_lazy(const DevView* __restrict__ V, c
         const bool act_raw = G(D->s.d_active)[e] != 0;
         const int fl = G(V->floe)[e];
         const float sub0 = G(D->s.d_sub)[e];
+        const bool need = rebuild && G(V->rneed)[e] != 0;   // read with the element
         float px = 0.0f, py = 0.0f, vx = 0.0f, vy = 0.0f, r = 0.0f;
         int deg = 0, novf = 0, lcnt = 0;
         int4 bt0 = make_int4(0, 0, 0, 0);
@@ -1950,7 +2012,10 @@ __device__ __forceinline__ void force_body_lazy(const DevView* __restrict__ V, c
         if (act_raw) load_elem();
         const bool fcoll = !act_raw && G(floe_coll_of(V, set))[fl] != 0u;
         const bool live = act_raw || fcoll;
-        if (!act_raw && (live || rebuild)) load_elem();
+        // on a build frame a frozen element rebuilds its list only when the
+        // frame before marked it (partial builds, stamped_near)
+        const bool rb = rebuild && (live || need);
+        if (!act_raw && (live || rb)) load_elem();
         BondIn bin0 = {0.0f, 0.0f, 0.0f, false, false};
         BondPre bpre0 = {0.0f, 0.0f, 0.0f, false, false};
         if (live && l < deg) {
@@ -1959,8 +2024,8 @@ __device__ __forceinline__ void force_body_lazy(const DevView* __restrict__ V, c
         }
         float fix = 0.0f, fiy = 0.0f, mi = 0.0f;
         if (lists) {
-            if (live || rebuild)
-                list_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, rebuild, lcnt, live, L, g, fix,
+            if (live || rb)
+                list_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, rb, lcnt, live, L, g, fix,
                                      fiy, mi);
         } else if (live) {
             contact_force_group<GR>(V, set, pos, vel, e, px, py, vx, vy, r, novf, L, g, fix, fiy, mi);
@@ -1990,7 +2055,7 @@ __device__ __forceinline__ void force_body_lazy(const DevView* __restrict__ V, c
                 if (__float_as_uint(sub0) != 0u)
                     G(D->s.d_sub)[e] = vmax(0.0f, sub0 - V->p.inv_rise_time * sub0 * V->p.dt);
                 // its list was built where it stands
-                if (rebuild) { G(V->xb)[e] = px; G(V->xb)[n + e] = py; }
+                if (rb) { G(V->xb)[e] = px; G(V->xb)[n + e] = py; }
                 sv[S_FLAGS] = __int_as_float(8);   // done
             }
         }
@@ -2185,14 +2250,15 @@ __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const C
     GAS const float* vel = G(final_in_nxt ? V->nxt_vel : D->s.d_vel);
     float px = 0.0f, py = 0.0f, pvx = 0.0f, pvy = 0.0f, ro = 0.0f, r = 0.0f, bnd = 0.0f;
     // the list control words, read beside the element's state
-    bool used = true, ins = true, built = false;
+    bool used = true, ins = true, built = false, nb1 = false;
     if (lists) {
         const bool ovf = nl_overflowed(V);
         // frame 0's buckets were filled by the call's cull whether or not the
         // frame built (and its call-start slot is cleared below, unread)
         built = k == 0 || nl_rebuild(V, k);
         used = ovf || built;
-        ins = ovf || nl_rebuild(V, k + 1);
+        nb1 = nl_rebuild(V, k + 1);
+        ins = ovf || nb1;
     }
     // a frozen element's next state is a copy of its state (the integration
     // keeps it), and its partner-list slot is the one the call's first cull
@@ -2205,6 +2271,10 @@ __device__ __forceinline__ void post_elem(const DevView* __restrict__ V, const C
         // with nl: whether the next frame's near candidates list the element
         if (cull && nl) listed = G(V->cflag)[(size_t)(set ^ 1) * n + t] != 0;
         frozen = G(D->s.d_active)[t] == 0;
+        // the next frame builds the lists (after a call's first frame): this
+        // element's list is rebuilt when it is active or a live element may
+        // have come within reach of it
+        if (lists && nb1) G(V->rneed)[t] = (!frozen || stamped_near(V, px, py, *G(nl_rebuilds(V)))) ? 1 : 0;
         if (final_in_nxt && !frozen) {
             G(D->s.d_pos)[t] = px; G(D->s.d_pos)[n + t] = py;
             G(D->s.d_vel)[t] = pvx; G(D->s.d_vel)[n + t] = pvy;

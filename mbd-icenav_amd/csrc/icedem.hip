@@ -529,8 +529,10 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
         // (list builds reach list_skin further)
         const double c2 = std::max((double)prm->grid_cell,
                                    2.0 * (2.0 * rmax + prm->contact_dilation2 + H.nl_skin) * 1.002 + 1e-3);
+        H.dreach = (float)((2.0 * rmax + prm->contact_dilation2 + H.nl_skin) * 1.001 + 1e-3);
         H.cell2 = (float)c2;
         if ((double)H.cell2 < c2) H.cell2 = std::nextafter(H.cell2, INFINITY);
+        H.icell2 = 1.0f / H.cell2;
         H.nx2 = std::max(1, (int)std::ceil((double)prm->grid_nx * prm->grid_cell / H.cell2));
         H.ny2 = std::max(1, (int)std::ceil((double)prm->grid_ny * prm->grid_cell / H.cell2));
         H.G2 = H.nx2 * H.ny2;
@@ -672,6 +674,7 @@ static int alloc_state_buffers(IceDemCtx* ctx) {
     if (H.nl_on) {
         AL(nl, int32_t, (size_t)n * NLW); AL(nl_cnt, int32_t, n); AL(xb, float, 2 * (size_t)n);
         AL(es, float, 6 * (size_t)n); AL(nlctl, uint32_t, NLCTL);
+        AL(dstamp, uint32_t, (size_t)H.G2); AL(rneed, uint8_t, n);
     }
 #undef AL
     HIP_TRY(hipMemset(H.cells, 0, (size_t)(H.coarse_off + 2 * (int64_t)H.G2 * H.rec_c)));
@@ -688,6 +691,8 @@ static int alloc_state_buffers(IceDemCtx* ctx) {
     if (H.nl_on) {
         HIP_TRY(hipMemset(H.nlctl, 0, sizeof(uint32_t) * NLCTL));
         HIP_TRY(hipMemset(H.es, 0, sizeof(float) * 6 * (size_t)n));
+        HIP_TRY(hipMemset(H.dstamp, 0, sizeof(uint32_t) * (size_t)H.G2));
+        HIP_TRY(hipMemset(H.rneed, 1, n));
         // the first call builds: call-start slot at +inf
         const uint32_t inf[NLSUB] = {0x7f800000u, 0x7f800000u, 0x7f800000u, 0x7f800000u,
                                      0x7f800000u, 0x7f800000u, 0x7f800000u, 0x7f800000u};

@@ -77,6 +77,10 @@ struct DevView {
     int32_t* nl;              // (n, NLW) partners in the order of the reference's scatter (rotated index)
     int32_t* nl_cnt;          // (n) listed partners; NLW + 1: more than NLW, the element scans the grids
     float* xb;                // (2, n) positions at the last build
+    uint32_t* dstamp;         // (G2) per coarse cell: 1 + the build count when a live element last stood in it
+    uint8_t* rneed;           // (n) the next build frame rebuilds the element's list (written by the frame before)
+    float dreach;             // a list's reach: 2 r_max + contact dilation + skin (and a rounding margin)
+    float icell2;             // 1 / cell2
     float* es;                // (n, 6) x, y, vx, vy, radius (and a pad) of every element for the frame
     uint32_t* nlctl;          // [0, 24) squared-displacement maxima (float bits) of frame k's positions in
                               // slots [8 (k % 3), 8 (k % 3) + 8); [24] a list overflowed this call;

@@ -3,7 +3,9 @@
 With one ice sub-step per frame the force pass keeps, per element, the
 partners within r_i + r_j + 2 delta + skin, rebuilt from the grids when an
 element may have moved skin / 2; the frames in between evaluate only the
-listed partners.  A call's first frame always builds.  These tests run many
+listed partners.  A call's first frame builds when an element lies farther
+than that from its build position; a build keeps the lists of frozen
+elements no live element came near since the last one.  These tests run many
 frames in ONE call (graph-replayed), so list frames, build frames and the
 switches between them all occur, and compare every frame's positions
 (histories) and the final state with the oracle, bit for bit.
@@ -13,6 +15,7 @@ import pytest
 import torch
 
 from helpers import Case
+from icedem.engine import DEFAULT_LIST_SKIN
 
 pytestmark = pytest.mark.gpu
 
@@ -220,3 +223,44 @@ def test_lists_overflow_chained_calls():
     print(f"ring chained: {rebuilt} list builds, stats {stats}", flush=True)
     assert stats["overflowed"]
     assert rebuilt >= 5
+
+
+def test_lists_caller_moves_beside_live_elements():
+    """Partial builds against the caller's edits.  Call after call the caller
+    moves frozen elements next to the live ones by 0.9 of the threshold in
+    random directions (builds in between rebuild some lists and keep others),
+    and activates frozen elements with and without a move: every frame
+    equals the oracle."""
+    from scipy.spatial import cKDTree
+    case = Case.make("c1small", n_frames=200)
+    rng = np.random.default_rng(11)
+    step = 0.9 * (DEFAULT_LIST_SKIN / 2 - 0.005)
+    moved = []
+
+    def edit(c, st):
+        if c == 0:
+            return False
+        pos = st.positions.detach().cpu().numpy()
+        act = st.elem_active.cpu().numpy().astype(bool)
+        live, frozen = np.flatnonzero(act), np.flatnonzero(~act)
+        if len(live) == 0 or len(frozen) == 0:
+            return False
+        d, _ = cKDTree(pos[live]).query(pos[frozen])
+        cand = frozen[d < 20.0]
+        if len(cand) < 8:
+            return False
+        pick = rng.choice(cand, min(30, len(cand)), replace=False)
+        ang = rng.uniform(0.0, 2.0 * np.pi, len(pick))
+        delta = (step * np.stack([np.cos(ang), np.sin(ang)], axis=1)).astype(np.float32)
+        dev = st.positions.device
+        with torch.no_grad():
+            st.positions[torch.as_tensor(pick, device=dev)] += torch.as_tensor(delta, device=dev)
+            if c % 3 == 2:   # activity: three moved elements, three unmoved ones
+                rest = np.setdiff1d(cand, pick)[:3]
+                st.elem_active[torch.as_tensor(np.concatenate([pick[:3], rest]), device=dev)] = True
+        moved.append(len(pick))
+        return True
+
+    rebuilt, stats = _chained(case, [8, 10, 6, 12, 8, 10, 6, 12, 8, 10], edit)
+    print(f"caller moves: {moved} elements moved per call, {rebuilt} list builds, stats {stats}", flush=True)
+    assert len(moved) >= 5
