@@ -2146,8 +2146,7 @@ __device__ __forceinline__ void post_bond(const DevView* __restrict__ V, const C
     GAS const float* pos = G(final_in_nxt ? V->nxt_pos : D->s.d_pos);
     GAS const float* vel = G(final_in_nxt ? V->nxt_vel : D->s.d_vel);
     if (t < nb) {
-        const int4 bp = ldg(G(V->bpair) + t);
-        const int i = bp.x, j = bp.y;
+        const int i = G(V->bi)[t], j = G(V->bj)[t];
         GAS const uint8_t* act = G(D->s.d_active);
         if (act[i] == 0 && act[j] == 0) {
             // Both ends frozen after the frame's activation: neither moved,
@@ -2158,12 +2157,15 @@ __device__ __forceinline__ void post_bond(const DevView* __restrict__ V, const C
             // sum adds +0, which turns only a -0 into +0).  Activity never
             // reverts, so both ends have been frozen since the call's first
             // frame, whose k_post formed the force record from the same
-            // positions and state: only that frame writes it.
+            // positions and state: only that frame writes it, and only that
+            // frame can find the -0 the shear sum turns into +0 (a later
+            // frame of the call finds what the first one left).
             const bool intact = G(D->s.d_bond_intact)[t] != 0;
             if (D->h.d_bond_intact) G(D->h.d_bond_intact)[(size_t)k * nb + t] = intact ? 1 : 0;
+            if (k != 0) return;
             GAS float* dsp = G(D->s.d_bond_ds) + t;
             if (__float_as_uint(*dsp) == 0x80000000u) *dsp = 0.0f;
-            if (!(prebond && k == 0)) return;
+            if (!prebond) return;
         }
         float dx = pos[j] - pos[i];
         float dy = pos[n + j] - pos[n + i];
@@ -2172,8 +2174,9 @@ __device__ __forceinline__ void post_bond(const DevView* __restrict__ V, const C
         float dirx = dx / ds_, diry = dy / ds_;
         GAS const uint8_t* elig = G(D->s.d_floe_elig);
         bool both = act[i] && act[j];
-        bool floe_ok = elig[bp.z] && elig[bp.w & 0x3fffffff];
-        bool breakable = (bp.w >> 30) == 0;
+        const int2 bf = ldg(G(V->bfloe) + t);
+        bool floe_ok = elig[bf.x] && elig[bf.y & 0x3fffffff];
+        bool breakable = (bf.y >> 30) == 0;
         const float fmi = fm_scaled(V, i), fmj = fm_scaled(V, j);
         bool near = (fmi > 0.0f) || (fmj > 0.0f);
         bool at_crack = G(V->crack)[i] || G(V->crack)[j];

@@ -556,14 +556,14 @@ int icedem_create(const IceDemStatic* st, const IceDemParams* prm, int device, I
     UP(bi, int32_t, st->h_bond_i, st->n_bonds);
     UP(bj, int32_t, st->h_bond_j, st->n_bonds);
     {
-        std::vector<int4> bp(std::max(1, st->n_bonds));
+        std::vector<int2> bp(std::max(1, st->n_bonds));
         for (int b = 0; b < st->n_bonds; ++b) {
             const int i = st->h_bond_i[b], j = st->h_bond_j[b];
             const bool brk = (st->h_thicknesses[i] <= prm->max_fracture_thickness) &&
                              (st->h_thicknesses[j] <= prm->max_fracture_thickness);
-            bp[b] = make_int4(i, j, st->h_floe_of[i], st->h_floe_of[j] | (brk ? 0 : (1 << 30)));
+            bp[b] = make_int2(st->h_floe_of[i], st->h_floe_of[j] | (brk ? 0 : (1 << 30)));
         }
-        UP(bpair, int4, bp.data(), bp.size());
+        UP(bfloe, int2, bp.data(), bp.size());
     }
     UP(rest, float, st->h_rest_lengths, st->n_bonds);
     if (st->n_bonds > 0) {   // bond-major stiffness, the formula of k_bond_prm

@@ -21,6 +21,11 @@ __device__ __forceinline__ float2 ldg(GAS const float2* p) {
     const gas_f2 v = *(GAS const gas_f2*)p;
     return make_float2(v.x, v.y);
 }
+typedef int gas_i2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ int2 ldg(GAS const int2* p) {
+    const gas_i2 v = *(GAS const gas_i2*)p;
+    return make_int2(v.x, v.y);
+}
 __device__ __forceinline__ int4 ldg(GAS const int4* p) {
     const gas_i4 v = *(GAS const gas_i4*)p;
     return make_int4(v.x, v.y, v.z, v.w);
@@ -43,9 +48,9 @@ struct DevView {
     const float* thick; const float* bound;
     const int32_t* floe;
     const int32_t* bi; const int32_t* bj;
-    // (n_bonds) per bond: i, j, floe of i, floe of j | 1 << 30 when the bond is
-    // not breakable (a thickness above max_fracture_thickness)
-    const int4* bpair;
+    // (n_bonds) per bond: floe of i, floe of j | 1 << 30 when the bond is not
+    // breakable (a thickness above max_fracture_thickness)
+    const int2* bfloe;
     const float* rest; const float* bstr; const float* sbar;
     const float2* hull;       // (S)
     // bonds of each element, element-major with fixed width W, in the
