@@ -36,10 +36,10 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "DEM element-steps/s; planner rollouts scored/s, surrogate vs plain DEM, ≤5% cost err"
 HBM_PEAK_GBS = 8000.0
-# categories of icedem_frames_profiled: a separate near-test kernel (batched
-# rollouts only; a single field tests its near elements in k_cull and
-# k_post), the force pass, the post pass (per frame), the call's first cull
-KERNELS = ["k_near", "k_force", "k_post", "k_cull"]
+# categories of icedem_frames_profiled: slot 0 is unused (a single field
+# tests its near elements in k_cull and k_post), the force pass, the post
+# pass (per frame), the call's first cull
+KERNELS = ["unused", "k_force", "k_post", "k_cull"]
 
 
 def build_case(config: str, seed: int, cache_dir: str = ""):
@@ -94,24 +94,22 @@ def algorithmic_bytes(kernel: str, n: int, nb: int, nf: int, G: int, n_near: int
     Per bond (two table entries of 24 B, intact 1 + damage 4) -> 53; cell
     counts 4 per cell; floe flags 4 per floe; the tail's three masked block
     trees per near block (256 x 13 B) and the partials 12 per block slot.
-    k_near, per near element: pos 8, rot 4, polygon 9 per vertex slot,
-    radius/thickness/floe 12, pre-pass writes 28, floe atomics 8; hull points
-    8 each.
+    Near tests (k_post's near workgroups), per near element: pos 8, rot 4,
+    polygon 9 per vertex slot, radius/thickness/floe 12, pre-pass writes 28,
+    floe atomics 8; hull points 8 each.
     k_post, per bond: pair table (i, j, floes) 16, rest/strength/sigma 12,
     damage state 17 read and written -> 62; per element: next state 24 read,
     state 24 written, position/rotation/submergence history 16 (+ submergence
     read 4), force_mag 4, crack 1, active 1, elig 1, cull reads radius/bound
     8, near flag 1, bucket slot 24 and count 4 -> 112; cell clears 4 per
-    cell, floe clears 8 per floe; and the next frame's near tests (the k_near
-    terms).
+    cell, floe clears 8 per floe; and the next frame's near tests (the terms
+    above).
     """
     if kernel == "k_force" and listed >= 0:
         return (103.0 * n + 24.0 * min(n, listed) + 4.0 * listed + 48.0 * nb + 4.0 * nf
                 + near_blocks * 256 * 13.0 + 12.0 * nbpad)
     if kernel == "k_force":
         return 115.0 * n + 53.0 * nb + 4.0 * G + 4.0 * nf + near_blocks * 256 * 13.0 + 12.0 * nbpad
-    if kernel == "k_near":
-        return n_near * (9.0 * max_v + 60.0) + 8.0 * n_hull
     if kernel == "k_post":
         return 62.0 * nb + 112.0 * n + 4.0 * G + 8.0 * nf + n_near * (9.0 * max_v + 60.0) + 8.0 * n_hull
     return 64.0
