@@ -131,8 +131,10 @@ typedef struct {
     /* partner lists (build choice, results identical): with list_skin > 0
      * and one ice sub-step per frame, each element keeps the partners within
      * r_i + r_j + 2 delta + list_skin of it, rebuilt from the grids when an
-     * element may have moved list_skin / 2 since the last build; frames in
-     * between evaluate only the listed partners.  0 = the grids every frame. */
+     * element may have moved list_skin / 2 since the last build (a build
+     * keeps the lists of frozen elements that no live element came near);
+     * frames in between evaluate only the listed partners.  0 = the grids
+     * every frame. */
     float list_skin;
     /* near-candidate margin (build choice, results identical): a single
      * field's post pass tests the next frame's near elements among the
